@@ -1,0 +1,284 @@
+"""Python bindings of the AST-octree projection library (include/astg.h).
+
+The library is ``libastg.so`` next to this file, built by
+``ast-grid_amd/build.py``. Bindings are plain ctypes over the C ABI; numpy
+arrays are passed as host pointers. Loading fails loudly when the library is
+missing: there is no fallback implementation.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libastg.so")
+
+ASTG_OK = 0
+ASTG_ERR_ARG = -1
+ASTG_ERR_HIP = -2
+ASTG_ERR_NOT_CONVERGED = -3
+ASTG_ERR_TOPOLOGY = -4
+ASTG_ERR_STATE = -5
+
+WALL_XNEG, WALL_XPOS, WALL_ZNEG, WALL_ZPOS, WALL_YNEG, WALL_YPOS = 1, 2, 4, 8, 16, 32
+PREC_FP64, PREC_MIXED = 0, 1
+NORM_INF, NORM_L2 = 0, 1
+
+# Header symbols, in the order include/astg.h declares them.
+EXPORTS = [
+    "astg_pcg_params_default", "astg_create", "astg_destroy", "astg_last_error",
+    "astg_pcg_solve_csr", "astg_split_spheres", "astg_set_topology_split", "astg_set_tilt_open",
+    "astg_level_info", "astg_get_nodes", "astg_get_tilt", "astg_get_half_tilts", "astg_get_numbering",
+    "astg_get_weights", "astg_set_velocity", "astg_get_velocity", "astg_project", "astg_get_system",
+    "astg_get_mic_factor", "astg_fields_snapshot", "astg_kernel_time",
+]
+K_SPMV64, K_FWD64, K_BWD64, K_MIC, K_ASSEMBLE, K_SPMV32, K_FWD32, K_BWD32 = range(8)
+
+
+class PcgParams(ctypes.Structure):
+    _fields_ = [("tol", ctypes.c_double), ("max_iterations", ctypes.c_int), ("norm", ctypes.c_int),
+                ("mic_tau", ctypes.c_double), ("min_diag_ratio", ctypes.c_double),
+                ("precision", ctypes.c_int), ("max_outer", ctypes.c_int)]
+
+
+class PcgResult(ctypes.Structure):
+    _fields_ = [("residual", ctypes.c_double), ("rel_residual", ctypes.c_double),
+                ("iterations", ctypes.c_int), ("outer", ctypes.c_int), ("converged", ctypes.c_int),
+                ("n", ctypes.c_int), ("nnz", ctypes.c_longlong), ("depth", ctypes.c_int),
+                ("ms_assemble", ctypes.c_double), ("ms_factor", ctypes.c_double),
+                ("ms_solve", ctypes.c_double), ("ms_velocity", ctypes.c_double),
+                ("ms_total", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    """Loads libastg.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libastg.so not built: run `python ast-grid_amd/build.py` (or __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    I = ctypes.c_int
+    D = ctypes.c_double
+    sig = {
+        "astg_pcg_params_default": (None, [ctypes.POINTER(PcgParams)]),
+        "astg_create": (I, [ctypes.POINTER(P), I, ctypes.c_uint32]),
+        "astg_destroy": (I, [P]),
+        "astg_last_error": (ctypes.c_char_p, [P]),
+        "astg_pcg_solve_csr": (I, [P, I, P, P, P, P, P, ctypes.POINTER(PcgParams), ctypes.POINTER(PcgResult)]),
+        "astg_split_spheres": (I, [I, I, I, P, P, P]),
+        "astg_set_topology_split": (I, [P, I, D, I, P, I]),
+        "astg_set_tilt_open": (I, [P, I, P]),
+        "astg_level_info": (I, [P, I, P, P]),
+        "astg_get_nodes": (I, [P, I, P, P]),
+        "astg_get_tilt": (I, [P, I, P, P, P]),
+        "astg_get_half_tilts": (I, [P, P, P]),
+        "astg_get_numbering": (I, [P, I, P, P, P]),
+        "astg_get_weights": (I, [P, I, P, P]),
+        "astg_set_velocity": (I, [P, I, P, P]),
+        "astg_get_velocity": (I, [P, I, P, P]),
+        "astg_project": (I, [P, D, D, ctypes.POINTER(PcgParams), ctypes.POINTER(PcgResult)]),
+        "astg_get_system": (I, [P, P, P, P, P, P]),
+        "astg_get_mic_factor": (I, [P, P, P, P, P, P]),
+        "astg_fields_snapshot": (I, [P, I]),
+        "astg_kernel_time": (I, [P, I, I, P, P]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class AstgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("astg error %d: %s" % (code, msg))
+        self.code = code
+
+
+def default_params(**kw):
+    p = PcgParams()
+    lib().astg_pcg_params_default(ctypes.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def split_spheres(R, levels, spheres, band):
+    """Split map (R^3 int32, x fastest) refined toward a union of spheres."""
+    sph = np.ascontiguousarray(np.asarray(spheres, np.float64).reshape(-1, 4))
+    band = np.ascontiguousarray(np.asarray(band, np.float64))
+    out = np.empty(R * R * R, np.int32)
+    rc = lib().astg_split_spheres(R, levels, len(sph), _ptr(sph), _ptr(band), _ptr(out))
+    if rc:
+        raise AstgError(rc, "split_spheres")
+    return out
+
+
+class Context:
+    """One library context (device >= 0: HIP stream + device buffers; -1: host only)."""
+
+    def __init__(self, device=0):
+        self._L = lib()
+        h = ctypes.c_void_p()
+        rc = self._L.astg_create(ctypes.byref(h), device, 0)
+        if rc:
+            raise AstgError(rc, "astg_create(device=%d)" % device)
+        self.h = h
+        self.levels = 0
+
+    def close(self):
+        if self.h:
+            self._L.astg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what, allow=()):
+        if rc and rc not in allow:
+            msg = self._L.astg_last_error(self.h)
+            raise AstgError(rc, "%s: %s" % (what, msg.decode() if msg else ""))
+        return rc
+
+    # ---- topology ----
+    def set_topology(self, split, R, levels, closed_flags, h0=None):
+        split = np.ascontiguousarray(split, np.int32)
+        assert split.size == R ** 3
+        self._chk(self._L.astg_set_topology_split(self.h, R, 1.0 / R if h0 is None else h0, levels,
+                                                  _ptr(split), closed_flags), "set_topology")
+        self.levels = levels
+        self.R = R
+        self.closed = closed_flags
+
+    def set_tilt_open(self, level, mask):
+        m = np.ascontiguousarray(mask, np.uint8)
+        self._chk(self._L.astg_set_tilt_open(self.h, level, _ptr(m)), "set_tilt_open")
+
+    def level_info(self, level):
+        nn, nl = ctypes.c_int(), ctypes.c_int()
+        self._chk(self._L.astg_level_info(self.h, level, ctypes.byref(nn), ctypes.byref(nl)), "level_info")
+        return nn.value, nl.value
+
+    def nodes(self, level):
+        nn, _ = self.level_info(level)
+        xyz = np.empty((nn, 3), np.int32)
+        fl = np.empty(nn, np.int32)
+        self._chk(self._L.astg_get_nodes(self.h, level, _ptr(xyz), _ptr(fl)), "get_nodes")
+        return xyz, fl
+
+    def tilt(self, level):
+        nn, _ = self.level_info(level)
+        e = np.empty(nn, np.float64)
+        bits = np.empty(nn, np.int32)
+        hd = np.empty(nn, np.int32)
+        self._chk(self._L.astg_get_tilt(self.h, level, _ptr(e), _ptr(bits), _ptr(hd)), "get_tilt")
+        return e, bits, hd
+
+    def half_tilts(self):
+        cnt = ctypes.c_int()
+        self._chk(self._L.astg_get_half_tilts(self.h, None, ctypes.byref(cnt)), "get_half_tilts")
+        out = np.empty((cnt.value, 4), np.int32)
+        self._chk(self._L.astg_get_half_tilts(self.h, _ptr(out), ctypes.byref(cnt)), "get_half_tilts")
+        return out
+
+    def numbering(self, level):
+        nn, _ = self.level_info(level)
+        o = np.empty(nn, np.int32)
+        t = np.empty(nn, np.int32)
+        n = ctypes.c_int()
+        self._chk(self._L.astg_get_numbering(self.h, level, _ptr(o), _ptr(t), ctypes.byref(n)), "get_numbering")
+        return o, t, n.value
+
+    def weights(self, level):
+        nn, _ = self.level_info(level)
+        m = np.empty((nn, 3), np.float64)
+        t = np.empty((nn, 8), np.float64)
+        self._chk(self._L.astg_get_weights(self.h, level, _ptr(m), _ptr(t)), "get_weights")
+        return m, t
+
+    # ---- fields ----
+    def set_velocity(self, level, mac, tilt8):
+        mac = np.ascontiguousarray(mac, np.float64)
+        tilt8 = np.ascontiguousarray(tilt8, np.float64)
+        self._chk(self._L.astg_set_velocity(self.h, level, _ptr(mac), _ptr(tilt8)), "set_velocity")
+
+    def velocity(self, level):
+        nn, _ = self.level_info(level)
+        m = np.empty((nn, 3), np.float64)
+        t = np.empty((nn, 8), np.float64)
+        self._chk(self._L.astg_get_velocity(self.h, level, _ptr(m), _ptr(t)), "get_velocity")
+        return m, t
+
+    # ---- projection ----
+    def project(self, dt, rho, params=None, allow_unconverged=False):
+        prm = params if params is not None else default_params()
+        res = PcgResult()
+        self._chk(self._L.astg_project(self.h, dt, rho, ctypes.byref(prm), ctypes.byref(res)), "project",
+                  allow=(ASTG_ERR_NOT_CONVERGED,) if allow_unconverged else ())
+        return res
+
+    def system(self, n, nnz):
+        rs = np.empty(n + 1, np.int32)
+        col = np.empty(nnz, np.int32)
+        val = np.empty(nnz, np.float64)
+        b = np.empty(n, np.float64)
+        p = np.empty(n, np.float64)
+        self._chk(self._L.astg_get_system(self.h, _ptr(rs), _ptr(col), _ptr(val), _ptr(b), _ptr(p)), "get_system")
+        return rs, col, val, b, p
+
+    def pattern(self):
+        n = self.numbering(0)[2]
+        rs = np.empty(n + 1, np.int32)
+        self._chk(self._L.astg_get_system(self.h, _ptr(rs), None, None, None, None), "get_system")
+        col = np.empty(rs[-1], np.int32)
+        self._chk(self._L.astg_get_system(self.h, _ptr(rs), _ptr(col), None, None, None), "get_system")
+        return rs, col
+
+    def mic_factor(self, n):
+        nnz = ctypes.c_longlong()
+        self._chk(self._L.astg_get_mic_factor(self.h, None, None, None, None, ctypes.byref(nnz)), "get_mic_factor")
+        inv = np.empty(n, np.float64)
+        cs = np.empty(n + 1, np.uint32)
+        ri = np.empty(nnz.value, np.uint32)
+        v = np.empty(nnz.value, np.float64)
+        self._chk(self._L.astg_get_mic_factor(self.h, _ptr(inv), _ptr(cs), _ptr(ri), _ptr(v), ctypes.byref(nnz)),
+                  "get_mic_factor")
+        return inv, cs, ri, v
+
+    def snapshot(self, restore=False):
+        self._chk(self._L.astg_fields_snapshot(self.h, 1 if restore else 0), "fields_snapshot")
+
+    def kernel_time(self, which, reps=10):
+        ms, by = ctypes.c_double(), ctypes.c_double()
+        self._chk(self._L.astg_kernel_time(self.h, which, reps, ctypes.byref(ms), ctypes.byref(by)), "kernel_time")
+        return ms.value, by.value
+
+    def pcg_solve_csr(self, rowstart, col, val, b, params=None, allow_unconverged=False):
+        rowstart = np.ascontiguousarray(rowstart, np.int32)
+        col = np.ascontiguousarray(col, np.int32)
+        val = np.ascontiguousarray(val, np.float64)
+        b = np.ascontiguousarray(b, np.float64)
+        n = len(rowstart) - 1
+        x = np.empty(n, np.float64)
+        prm = params if params is not None else default_params()
+        res = PcgResult()
+        self._chk(self._L.astg_pcg_solve_csr(self.h, n, _ptr(rowstart), _ptr(col), _ptr(val), _ptr(b), _ptr(x),
+                                             ctypes.byref(prm), ctypes.byref(res)), "pcg_solve_csr",
+                  allow=(ASTG_ERR_NOT_CONVERGED,) if allow_unconverged else ())
+        return x, res

@@ -1,0 +1,539 @@
+// C ABI (include/astg.h). No exception crosses this boundary: every entry
+// point catches, records the message on the context and returns a status.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/astg.h"
+#include "octree.h"
+#include "pcg.h"
+#include "project.h"
+#include "project_host.h"
+#include "scene.h"
+
+struct astg_ctx {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::unique_ptr<astg::Octree> tree;
+    astg::Numbering num;
+    astg::Weights w;
+    astg::ProjectTables tables;
+    bool tablesReady = false;
+    bool deviceReady = false;
+    std::vector<std::vector<double>> hostMac, hostTilt;  // host-only contexts
+    std::unique_ptr<astg::PcgDevice> pcg;                 // projection system
+    std::unique_ptr<astg::ProjectDevice> proj;
+    std::unique_ptr<astg::PcgDevice> csr;                 // CSR seam
+    std::vector<int> csrRs, csrCol;
+    astg::PcgDevice* lastPcg = nullptr;
+    hipEvent_t ev[6] = {};
+};
+
+namespace {
+
+int fail(astg_ctx* c, int code, const std::string& m) {
+    if (c) c->err = m;
+    return code;
+}
+
+template <typename F>
+int guarded(astg_ctx* c, F&& f) {
+    try {
+        return f();
+    } catch (const std::bad_alloc&) {
+        return fail(c, ASTG_ERR_ARG, "host allocation failed");
+    } catch (const std::exception& e) {
+        return fail(c, ASTG_ERR_TOPOLOGY, e.what());
+    }
+}
+
+int needDevice(astg_ctx* c) {
+    if (c->device < 0) return fail(c, ASTG_ERR_STATE, "context has no device (created with device < 0)");
+    if (hipSetDevice(c->device) != hipSuccess) return fail(c, ASTG_ERR_HIP, "hipSetDevice failed");
+    return ASTG_OK;
+}
+
+int ensureTables(astg_ctx* c) {
+    if (!c->tree) return fail(c, ASTG_ERR_STATE, "no topology set");
+    if (!c->tablesReady) {
+        astg::numberUnknowns(*c->tree, c->num);
+        astg::computeWeights(*c->tree, c->w);
+        astg::buildProjectTables(*c->tree, c->num, c->w, c->tables);
+        c->tablesReady = true;
+        c->deviceReady = false;
+        if (c->device < 0) {
+            c->hostMac.assign(c->tree->L, {});
+            c->hostTilt.assign(c->tree->L, {});
+            for (int l = 0; l < c->tree->L; ++l) {
+                c->hostMac[l].assign((size_t)c->tree->lv[l].nnode * 3, 0.0);
+                c->hostTilt[l].assign((size_t)c->tree->lv[l].nnode * 8, 0.0);
+            }
+        }
+    }
+    if (c->device >= 0 && !c->deviceReady) {
+        int rc = needDevice(c);
+        if (rc) return rc;
+        if (!c->pcg) c->pcg.reset(new astg::PcgDevice(c->stream));
+        if (!c->proj) c->proj.reset(new astg::ProjectDevice(c->stream));
+        rc = c->proj->upload(c->tables, *c->tree, c->err);
+        if (rc) return rc;
+        rc = c->pcg->setPattern(c->tables.n, c->tables.rowstart.data(), c->tables.col.data(), c->err);
+        if (rc) return rc;
+        // fields start at zero
+        const size_t G = c->tables.G;
+        std::vector<double> z((size_t)8 * G, 0.0);
+        if (hipMemcpyAsync(c->proj->uMac(), z.data(), 3 * G * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+            hipMemcpyAsync(c->proj->uTilt(), z.data(), 8 * G * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+            return fail(c, ASTG_ERR_HIP, "field upload failed");
+        c->deviceReady = true;
+    }
+    return ASTG_OK;
+}
+
+int checkLevel(astg_ctx* c, int level) {
+    if (!c || !c->tree) return fail(c, ASTG_ERR_STATE, "no topology set");
+    if (level < 0 || level >= c->tree->L) return fail(c, ASTG_ERR_ARG, "level out of range");
+    return ASTG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void astg_pcg_params_default(astg_pcg_params* p) {
+    if (!p) return;
+    p->tol = 1e-5;
+    p->max_iterations = 3000;
+    p->norm = ASTG_NORM_INF;
+    p->mic_tau = 0.97;
+    p->min_diag_ratio = 0.25;
+    p->precision = ASTG_PREC_FP64;
+    p->max_outer = 0;
+}
+
+int astg_create(astg_ctx** out, int device, uint32_t flags) {
+    (void)flags;
+    if (!out) return ASTG_ERR_ARG;
+    *out = nullptr;
+    auto* c = new (std::nothrow) astg_ctx();
+    if (!c) return ASTG_ERR_ARG;
+    c->device = device;
+    if (device >= 0) {
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || device >= count) {
+            delete c;
+            return ASTG_ERR_HIP;
+        }
+        if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+            delete c;
+            return ASTG_ERR_HIP;
+        }
+        for (auto& e : c->ev)
+            if (hipEventCreate(&e) != hipSuccess) {
+                delete c;
+                return ASTG_ERR_HIP;
+            }
+    }
+    *out = c;
+    return ASTG_OK;
+}
+
+int astg_destroy(astg_ctx* c) {
+    if (!c) return ASTG_ERR_ARG;
+    if (c->device >= 0) {
+        (void)hipSetDevice(c->device);
+        if (c->stream) (void)hipStreamSynchronize(c->stream);
+        c->pcg.reset();
+        c->proj.reset();
+        c->csr.reset();
+        for (auto& e : c->ev)
+            if (e) (void)hipEventDestroy(e);
+        if (c->stream) (void)hipStreamDestroy(c->stream);
+    }
+    delete c;
+    return ASTG_OK;
+}
+
+const char* astg_last_error(const astg_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int astg_split_spheres(int R, int levels, int nspheres, const double* spheres, const double* band, int* split_out) {
+    if (R <= 0 || levels <= 0 || levels > 15 || (R >> (levels - 1)) < 2 || (R % (1 << (levels - 1))) != 0 ||
+        nspheres < 0 || (nspheres && !spheres) || !band || !split_out)
+        return ASTG_ERR_ARG;
+    try {
+        std::vector<int> split;
+        astg::splitSpheres(R, levels, spheres, nspheres, band, split);
+        astg::balanceSplit(R, levels, split);
+        std::memcpy(split_out, split.data(), split.size() * sizeof(int));
+    } catch (...) {
+        return ASTG_ERR_ARG;
+    }
+    return ASTG_OK;
+}
+
+int astg_set_topology_split(astg_ctx* c, int R, double h0, int levels, const int* split, int closed_flags) {
+    if (!c) return ASTG_ERR_ARG;
+    if (R <= 0 || levels <= 0 || levels > 15 || !split || h0 <= 0 || (R % (1 << (levels - 1))) != 0 ||
+        R > (1 << 16))
+        return fail(c, ASTG_ERR_ARG, "bad topology arguments");
+    for (size_t i = 0, n = (size_t)R * R * R; i < n; ++i)
+        if (split[i] < 0 || split[i] >= levels) return fail(c, ASTG_ERR_ARG, "split map value out of range");
+    return guarded(c, [&] {
+        std::unique_ptr<astg::Octree> t(new astg::Octree());
+        t->build(split, R, h0, levels, closed_flags);
+        if (!t->err.empty()) return fail(c, ASTG_ERR_TOPOLOGY, t->err);
+        c->tree = std::move(t);
+        c->tablesReady = false;
+        c->deviceReady = false;
+        return ASTG_OK;
+    });
+}
+
+int astg_set_tilt_open(astg_ctx* c, int level, const unsigned char* mask) {
+    int rc = checkLevel(c, level);
+    if (rc) return rc;
+    if (!mask) return fail(c, ASTG_ERR_ARG, "null mask");
+    c->tree->setTiltOpen(level, mask);
+    c->tablesReady = false;
+    c->deviceReady = false;
+    return ASTG_OK;
+}
+
+int astg_level_info(astg_ctx* c, int level, int* nnode, int* nleaf) {
+    int rc = checkLevel(c, level);
+    if (rc) return rc;
+    if (nnode) *nnode = c->tree->lv[level].nnode;
+    if (nleaf) *nleaf = c->tree->lv[level].nleaf;
+    return ASTG_OK;
+}
+
+int astg_get_nodes(astg_ctx* c, int level, int* xyz, int* flags) {
+    int rc = checkLevel(c, level);
+    if (rc) return rc;
+    const auto& L = c->tree->lv[level];
+    for (int i = 0; i < L.nnode; ++i) {
+        if (xyz) {
+            xyz[3 * i] = L.x[i];
+            xyz[3 * i + 1] = L.y[i];
+            xyz[3 * i + 2] = L.z[i];
+        }
+        if (flags) flags[i] = L.flag[i];
+    }
+    return ASTG_OK;
+}
+
+int astg_get_tilt(astg_ctx* c, int level, double* e, int* bits, int* half_dir) {
+    int rc = checkLevel(c, level);
+    if (rc) return rc;
+    const auto& T = c->tree->tilt[level];
+    for (size_t i = 0; i < T.size(); ++i) {
+        if (e) e[i] = T[i].e;
+        if (bits) bits[i] = T[i].bits;
+        if (half_dir) half_dir[i] = T[i].half_dir;
+    }
+    return ASTG_OK;
+}
+
+int astg_get_half_tilts(astg_ctx* c, int* lxyz, int* count) {
+    if (!c || !c->tree) return fail(c, ASTG_ERR_STATE, "no topology set");
+    const auto& H = c->tree->half;
+    if (count) *count = (int)H.size();
+    if (lxyz)
+        for (size_t i = 0; i < H.size(); ++i) {
+            lxyz[4 * i] = astg::lc_level(H[i]);
+            lxyz[4 * i + 1] = astg::lc_x(H[i]);
+            lxyz[4 * i + 2] = astg::lc_y(H[i]);
+            lxyz[4 * i + 3] = astg::lc_z(H[i]);
+        }
+    return ASTG_OK;
+}
+
+int astg_get_numbering(astg_ctx* c, int level, int* oct, int* tilt, int* n_total) {
+    int rc = checkLevel(c, level);
+    if (rc) return rc;
+    return guarded(c, [&] {
+        if (!c->tablesReady) {
+            astg::numberUnknowns(*c->tree, c->num);
+            astg::computeWeights(*c->tree, c->w);
+            astg::buildProjectTables(*c->tree, c->num, c->w, c->tables);
+            c->tablesReady = true;
+            c->deviceReady = false;
+        }
+        const size_t nn = c->tree->lv[level].nnode;
+        if (oct) std::memcpy(oct, c->num.oct[level].data(), nn * 4);
+        if (tilt) std::memcpy(tilt, c->num.tilt[level].data(), nn * 4);
+        if (n_total) *n_total = c->num.n;
+        return ASTG_OK;
+    });
+}
+
+int astg_get_weights(astg_ctx* c, int level, double* mac, double* tilt8) {
+    int rc = checkLevel(c, level);
+    if (rc) return rc;
+    astg::Weights w;
+    astg::computeWeights(*c->tree, w);
+    if (mac) std::memcpy(mac, w.mac[level].data(), w.mac[level].size() * 8);
+    if (tilt8) std::memcpy(tilt8, w.tilt[level].data(), w.tilt[level].size() * 8);
+    return ASTG_OK;
+}
+
+int astg_set_velocity(astg_ctx* c, int level, const double* mac, const double* tilt8) {
+    int rc = checkLevel(c, level);
+    if (rc) return rc;
+    return guarded(c, [&] {
+        int r = ensureTables(c);
+        if (r) return r;
+        const int nn = c->tree->lv[level].nnode;
+        if (c->device < 0) {
+            if (mac) std::memcpy(c->hostMac[level].data(), mac, (size_t)nn * 3 * 8);
+            if (tilt8) std::memcpy(c->hostTilt[level].data(), tilt8, (size_t)nn * 8 * 8);
+            return ASTG_OK;
+        }
+        const size_t G = c->tables.G, off = c->tables.off[level];
+        std::vector<double> buf((size_t)nn);
+        for (int a = 0; a < 3 && mac; ++a) {
+            for (int i = 0; i < nn; ++i) buf[i] = mac[(size_t)3 * i + a];
+            if (hipMemcpy(c->proj->uMac() + a * G + off, buf.data(), (size_t)nn * 8, hipMemcpyHostToDevice) !=
+                hipSuccess)
+                return fail(c, ASTG_ERR_HIP, "velocity upload failed");
+        }
+        for (int k = 0; k < 8 && tilt8; ++k) {
+            for (int i = 0; i < nn; ++i) buf[i] = tilt8[(size_t)8 * i + k];
+            if (hipMemcpy(c->proj->uTilt() + k * G + off, buf.data(), (size_t)nn * 8, hipMemcpyHostToDevice) !=
+                hipSuccess)
+                return fail(c, ASTG_ERR_HIP, "velocity upload failed");
+        }
+        return ASTG_OK;
+    });
+}
+
+int astg_get_velocity(astg_ctx* c, int level, double* mac, double* tilt8) {
+    int rc = checkLevel(c, level);
+    if (rc) return rc;
+    return guarded(c, [&] {
+        int r = ensureTables(c);
+        if (r) return r;
+        const int nn = c->tree->lv[level].nnode;
+        if (c->device < 0) {
+            if (mac) std::memcpy(mac, c->hostMac[level].data(), (size_t)nn * 3 * 8);
+            if (tilt8) std::memcpy(tilt8, c->hostTilt[level].data(), (size_t)nn * 8 * 8);
+            return ASTG_OK;
+        }
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(c, ASTG_ERR_HIP, "stream sync failed");
+        const size_t G = c->tables.G, off = c->tables.off[level];
+        std::vector<double> buf((size_t)nn);
+        for (int a = 0; a < 3 && mac; ++a) {
+            if (hipMemcpy(buf.data(), c->proj->uMac() + a * G + off, (size_t)nn * 8, hipMemcpyDeviceToHost) !=
+                hipSuccess)
+                return fail(c, ASTG_ERR_HIP, "velocity download failed");
+            for (int i = 0; i < nn; ++i) mac[(size_t)3 * i + a] = buf[i];
+        }
+        for (int k = 0; k < 8 && tilt8; ++k) {
+            if (hipMemcpy(buf.data(), c->proj->uTilt() + k * G + off, (size_t)nn * 8, hipMemcpyDeviceToHost) !=
+                hipSuccess)
+                return fail(c, ASTG_ERR_HIP, "velocity download failed");
+            for (int i = 0; i < nn; ++i) tilt8[(size_t)8 * i + k] = buf[i];
+        }
+        return ASTG_OK;
+    });
+}
+
+int astg_project(astg_ctx* c, double dt, double rho, const astg_pcg_params* params, astg_pcg_result* result) {
+    if (!c) return ASTG_ERR_ARG;
+    int rc = needDevice(c);
+    if (rc) return rc;
+    if (!(dt > 0) || !(rho > 0)) return fail(c, ASTG_ERR_ARG, "dt and rho must be positive");
+    astg_pcg_params prm;
+    astg_pcg_params_default(&prm);
+    if (params) prm = *params;
+    return guarded(c, [&] {
+        int r = ensureTables(c);
+        if (r) return r;
+        if (c->tables.half_duplicates)
+            return fail(c, ASTG_ERR_TOPOLOGY, "half-tilt list holds duplicates (the reference races on them)");
+        astg_pcg_result res;
+        std::memset(&res, 0, sizeof(res));
+        hipEvent_t* ev = c->ev;
+        (void)hipEventRecord(ev[0], c->stream);
+        r = c->proj->assemble(c->pcg->values(), c->pcg->rhs(), c->pcg->devRowstart(), rho, dt, c->err);
+        if (r) return r;
+        (void)hipEventRecord(ev[1], c->stream);
+        r = c->pcg->factor(prm.mic_tau, prm.min_diag_ratio, c->err);
+        if (r) return r;
+        (void)hipEventRecord(ev[2], c->stream);
+        r = c->pcg->solve(prm, res, c->err);
+        if (r) return r;
+        (void)hipEventRecord(ev[3], c->stream);
+        r = c->proj->correct(c->pcg->solution(), rho, dt, c->err);
+        if (r) return r;
+        (void)hipEventRecord(ev[4], c->stream);
+        if (hipEventSynchronize(ev[4]) != hipSuccess) return fail(c, ASTG_ERR_HIP, "projection failed on the device");
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
+        res.ms_assemble = ms;
+        (void)hipEventElapsedTime(&ms, ev[1], ev[2]);
+        res.ms_factor = ms;
+        (void)hipEventElapsedTime(&ms, ev[2], ev[3]);
+        res.ms_solve = ms;
+        (void)hipEventElapsedTime(&ms, ev[3], ev[4]);
+        res.ms_velocity = ms;
+        (void)hipEventElapsedTime(&ms, ev[0], ev[4]);
+        res.ms_total = ms;
+        c->lastPcg = c->pcg.get();
+        if (result) *result = res;
+        if (!res.converged) return fail(c, ASTG_ERR_NOT_CONVERGED, "PCG did not reach the tolerance");
+        return ASTG_OK;
+    });
+}
+
+int astg_get_system(astg_ctx* c, int* rowstart, int* col, double* val, double* b, double* p) {
+    if (!c) return ASTG_ERR_ARG;
+    if (c->device < 0) {
+        // host-only context: the pattern is available, values are not
+        if (val || b || p) return fail(c, ASTG_ERR_STATE, "values need a device context");
+        int rc = guarded(c, [&] { return ensureTables(c); });
+        if (rc) return rc;
+        if (rowstart) std::memcpy(rowstart, c->tables.rowstart.data(), (c->tables.n + 1) * 4);
+        if (col) std::memcpy(col, c->tables.col.data(), c->tables.col.size() * 4);
+        return ASTG_OK;
+    }
+    int rc = needDevice(c);
+    if (rc) return rc;
+    if (!c->tablesReady || !c->deviceReady) return fail(c, ASTG_ERR_STATE, "no assembled system");
+    const auto& t = c->tables;
+    if (rowstart) std::memcpy(rowstart, t.rowstart.data(), (t.n + 1) * 4);
+    if (col) std::memcpy(col, t.col.data(), t.col.size() * 4);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(c, ASTG_ERR_HIP, "stream sync failed");
+    if (val && hipMemcpy(val, c->pcg->values(), t.col.size() * 8, hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(c, ASTG_ERR_HIP, "download failed");
+    if (b && hipMemcpy(b, c->pcg->rhs(), (size_t)t.n * 8, hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(c, ASTG_ERR_HIP, "download failed");
+    if (p && hipMemcpy(p, c->pcg->solution(), (size_t)t.n * 8, hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(c, ASTG_ERR_HIP, "download failed");
+    return ASTG_OK;
+}
+
+int astg_get_mic_factor(astg_ctx* c, double* invdiag, unsigned* colstart, unsigned* rowindex, double* value,
+                        long long* nnz_l) {
+    if (!c) return ASTG_ERR_ARG;
+    int rc = needDevice(c);
+    if (rc) return rc;
+    if (!c->lastPcg) return fail(c, ASTG_ERR_STATE, "no factor computed yet");
+    std::vector<double> inv, val;
+    std::vector<unsigned> cs, ri;
+    if (c->lastPcg->downloadFactor(inv, cs, ri, val)) return fail(c, ASTG_ERR_HIP, "factor download failed");
+    if (nnz_l) *nnz_l = (long long)val.size();
+    if (invdiag) std::memcpy(invdiag, inv.data(), inv.size() * 8);
+    if (colstart) std::memcpy(colstart, cs.data(), cs.size() * 4);
+    if (rowindex) std::memcpy(rowindex, ri.data(), ri.size() * 4);
+    if (value) std::memcpy(value, val.data(), val.size() * 8);
+    return ASTG_OK;
+}
+
+int astg_pcg_solve_csr(astg_ctx* c, int n, const int* rowstart, const int* colindex, const double* value,
+                       const double* b, double* x, const astg_pcg_params* params, astg_pcg_result* result) {
+    if (!c) return ASTG_ERR_ARG;
+    if (n <= 0 || !rowstart || !colindex || !value || !b || !x) return fail(c, ASTG_ERR_ARG, "null or empty input");
+    int rc = needDevice(c);
+    if (rc) return rc;
+    astg_pcg_params prm;
+    astg_pcg_params_default(&prm);
+    if (params) prm = *params;
+    return guarded(c, [&] {
+        const long long nnz = rowstart[n];
+        if (rowstart[0] != 0 || nnz < 0) return fail(c, ASTG_ERR_ARG, "bad rowstart");
+        for (int i = 0; i < n; ++i)
+            if (rowstart[i + 1] < rowstart[i]) return fail(c, ASTG_ERR_ARG, "rowstart not monotone");
+        if (!c->csr) c->csr.reset(new astg::PcgDevice(c->stream));
+        bool same = (int)c->csrRs.size() == n + 1 && (long long)c->csrCol.size() == nnz &&
+                    std::memcmp(c->csrRs.data(), rowstart, (size_t)(n + 1) * 4) == 0 &&
+                    std::memcmp(c->csrCol.data(), colindex, (size_t)nnz * 4) == 0;
+        if (!same) {
+            c->csrRs.clear();
+            c->csrCol.clear();
+            int r = c->csr->setPattern(n, rowstart, colindex, c->err);
+            if (r) return r;
+            c->csrRs.assign(rowstart, rowstart + n + 1);
+            c->csrCol.assign(colindex, colindex + nnz);
+        }
+        astg_pcg_result res;
+        std::memset(&res, 0, sizeof(res));
+        hipEvent_t* ev = c->ev;
+        if (hipMemcpyAsync(c->csr->values(), value, (size_t)nnz * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+            hipMemcpyAsync(c->csr->rhs(), b, (size_t)n * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+            return fail(c, ASTG_ERR_HIP, "upload failed");
+        (void)hipEventRecord(ev[0], c->stream);
+        int r = c->csr->factor(prm.mic_tau, prm.min_diag_ratio, c->err);
+        if (r) return r;
+        (void)hipEventRecord(ev[1], c->stream);
+        r = c->csr->solve(prm, res, c->err);
+        if (r) return r;
+        (void)hipEventRecord(ev[2], c->stream);
+        if (hipMemcpyAsync(x, c->csr->solution(), (size_t)n * 8, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+            return fail(c, ASTG_ERR_HIP, "download failed");
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
+        res.ms_factor = ms;
+        (void)hipEventElapsedTime(&ms, ev[1], ev[2]);
+        res.ms_solve = ms;
+        (void)hipEventElapsedTime(&ms, ev[0], ev[2]);
+        res.ms_total = ms;
+        c->lastPcg = c->csr.get();
+        if (result) *result = res;
+        if (!res.converged) return fail(c, ASTG_ERR_NOT_CONVERGED, "PCG did not reach the tolerance");
+        return ASTG_OK;
+    });
+}
+
+int astg_fields_snapshot(astg_ctx* c, int restore) {
+    if (!c) return ASTG_ERR_ARG;
+    int rc = needDevice(c);
+    if (rc) return rc;
+    return guarded(c, [&] {
+        int r = ensureTables(c);
+        if (r) return r;
+        return c->proj->snapshot(restore != 0, c->err);
+    });
+}
+
+int astg_kernel_time(astg_ctx* c, int which, int reps, double* avg_ms, double* bytes) {
+    if (!c || reps <= 0 || !avg_ms || !bytes) return fail(c, ASTG_ERR_ARG, "bad arguments");
+    int rc = needDevice(c);
+    if (rc) return rc;
+    if (!c->deviceReady || !c->lastPcg) return fail(c, ASTG_ERR_STATE, "run astg_project first");
+    return guarded(c, [&] {
+        double ms = 0, b = 0;
+        if (which == ASTG_K_ASSEMBLE) {
+            hipEvent_t e0 = c->ev[0], e1 = c->ev[1];
+            double tot = 0;
+            for (int i = 0; i < reps; ++i) {
+                (void)hipEventRecord(e0, c->stream);
+                int r = c->proj->assemble(c->pcg->values(), c->pcg->rhs(), c->pcg->devRowstart(), 2.0, 1.0, c->err);
+                if (r) return r;
+                (void)hipEventRecord(e1, c->stream);
+                if (hipEventSynchronize(e1) != hipSuccess) return fail(c, ASTG_ERR_HIP, "assembly failed");
+                float f = 0;
+                (void)hipEventElapsedTime(&f, e0, e1);
+                tot += f;
+            }
+            ms = tot / reps;
+            b = c->proj->assembleBytes();
+        } else {
+            int r = c->pcg->benchKernel(which, reps, ms, b, c->err);
+            if (r) return r;
+        }
+        *avg_ms = ms;
+        *bytes = b;
+        return ASTG_OK;
+    });
+}
+
+}  // extern "C"

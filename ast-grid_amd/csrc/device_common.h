@@ -1,0 +1,118 @@
+// Device helpers shared by the projection kernels (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace astg {
+
+// Unpublished-value sentinels: signalling-NaN bit patterns. GPU arithmetic
+// only ever produces quiet NaNs, so a computed value can never equal them.
+constexpr unsigned long long kSent64 = 0x7FF4DEADBEEF0001ull;
+constexpr unsigned int kSent32 = 0x7FA0BEEFu;
+constexpr int kSentLevel = -1;
+
+// Polls allowed per lane before a sweep gives up (error flag + NaN output).
+// A healthy sweep needs at most a few thousand polls per lane.
+constexpr int kSpinLimit = 1 << 22;
+
+// scalars[] slots in the per-solver scalar block (doubles).
+enum ScalarSlot {
+    S_RHO = 0,      // rho = z.r
+    S_RHO_NEW,      // new rho
+    S_SQ,           // s.q
+    S_RES,          // max |r| (or sum r^2 for L2)
+    S_TOL,          // absolute tolerance
+    S_DONE,         // 1.0 once converged or failed
+    S_ITER,         // iterations performed
+    S_BNORM,        // ||b||
+    S_ERR,          // nonzero: sweep spin limit hit / breakdown
+    S_ALPHA,
+    S_BETA,
+    S_COUNT = 16
+};
+// int counters in the counter block
+enum CounterSlot { C_TICKET_F = 0, C_TICKET_B, C_TICKET_L, C_TICKET_M, C_DONE_SQ, C_DONE_RES, C_DONE_RHO, C_DONE_B, C_ERR, C_COUNT = 16 };
+// Sticky error bits in cnt[C_ERR] (atomicOr): a spin limit was hit.
+enum ErrBit { E_LEVELS = 1, E_MIC = 2, E_FWD = 4, E_BWD = 8 };
+
+// Level gate: advisory throttle for the sync-free sweeps. Rows are stored in
+// level order, padded so each 64-row wave segment holds one level. A wave of
+// level L first waits (one lane, with backoff) until every wave of level
+// L - kGate (forward) or L + kGate (backward) has finished, so only about
+// kGate levels of lanes ever poll values at once. Correctness never depends on
+// the gate: values are still published and polled one by one.
+constexpr int kGate = 2;
+constexpr int kGateLimit = 1 << 16;
+struct LevelGate {
+    const int* wlev;  // level of each wave segment (-1: trailing padding)
+    const int* wcnt;  // wave segments per level
+    int* done;        // finished waves per level (zeroed before each sweep)
+    int nseg;
+    int depth;
+};
+
+__device__ inline void gate_wait(const LevelGate& G, int g) {
+    if (g < 0 || g >= G.depth) return;
+    if ((threadIdx.x & 63) == 0) {
+        const int need = G.wcnt[g];
+        int backoff = 1;
+        for (int s = 0; s < kGateLimit; ++s) {
+            if (__hip_atomic_load(&G.done[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= need) break;
+            for (int k = 0; k < backoff; ++k) __builtin_amdgcn_s_sleep(2);
+            backoff = backoff < 16 ? backoff * 2 : 16;
+        }
+    }
+}
+__device__ inline void gate_arrive(const LevelGate& G, int lev) {
+    if ((threadIdx.x & 63) == 0 && lev >= 0) atomicAdd(&G.done[lev], 1);
+}
+
+__device__ inline unsigned long long ld_agent_u64(const unsigned long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_agent_u64(unsigned long long* p, unsigned long long v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline unsigned int ld_agent_u32(const unsigned int* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_agent_u32(unsigned int* p, unsigned int v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline int ld_agent_i32(const int* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ inline void st_agent_i32(int* p, int v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+
+template <typename T>
+struct Sentinel;
+template <>
+struct Sentinel<double> {
+    typedef unsigned long long bits_t;
+    static constexpr bits_t value = kSent64;
+    __device__ static bits_t load(const double* p) { return ld_agent_u64(reinterpret_cast<const bits_t*>(p)); }
+    __device__ static void store(double* p, double v) { st_agent_u64(reinterpret_cast<bits_t*>(p), (bits_t)__double_as_longlong(v)); }
+    __device__ static void storeSent(double* p) { *reinterpret_cast<bits_t*>(p) = value; }
+    __device__ static double val(bits_t b) { return __longlong_as_double((long long)b); }
+};
+template <>
+struct Sentinel<float> {
+    typedef unsigned int bits_t;
+    static constexpr bits_t value = kSent32;
+    __device__ static bits_t load(const float* p) { return ld_agent_u32(reinterpret_cast<const bits_t*>(p)); }
+    __device__ static void store(float* p, float v) { st_agent_u32(reinterpret_cast<bits_t*>(p), (bits_t)__float_as_uint(v)); }
+    __device__ static void storeSent(float* p) { *reinterpret_cast<bits_t*>(p) = value; }
+    __device__ static float val(bits_t b) { return __uint_as_float(b); }
+};
+
+// Wave64 reductions via DPP/shuffle.
+__device__ inline double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ inline double wave_max(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+}  // namespace astg

@@ -1,0 +1,97 @@
+// Device MIC(0)-preconditioned conjugate gradients for the projection system.
+//
+// Algorithm: the reference's PCGSolver (include/pcgsolver/pcg_solver.h:366-426)
+// with the reference's MIC(0) factor (pcg_solver.h:89-190) and triangular
+// solves (pcg_solver.h:211-253), restated for a GPU:
+//   * the dependency levels of tril(A) are computed on the device by a
+//     sync-free pass, and rows are processed in level order;
+//   * the factorisation is left-looking per row (same accumulation order as the
+//     reference's right-looking column loop) and the triangular sweeps are
+//     sync-free: each row publishes its value with an agent-scope atomic store
+//     over a signalling-NaN sentinel that dependants poll;
+//   * dots, axpys and the convergence test are fused into the SpMV / update /
+//     sweep kernels with wave64 reductions; scalars stay on the device.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/astg.h"
+#include "device_common.h"
+
+namespace astg {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf();
+    hipError_t alloc(size_t b);
+    template <typename T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+class PcgDevice {
+public:
+    explicit PcgDevice(hipStream_t s) : stream_(s) {}
+    ~PcgDevice();
+
+    // Uploads a CSR pattern (host arrays) and builds everything that depends on
+    // the pattern only: diagonal / upper split, transpose map, dependency
+    // levels and level order (both computed on the device).
+    int setPattern(int n, const int* rowstart, const int* col, std::string& err);
+    // Device pointer to the FP64 value array (nnz) for in-place assembly.
+    double* values() { return val_.as<double>(); }
+    double* rhs() { return b_.as<double>(); }
+    double* solution() { return x_.as<double>(); }
+    int n() const { return n_; }
+    long long nnz() const { return nnz_; }
+    int depth() const { return depth_; }
+    const int* devRowstart() const { return rs_.as<int>(); }
+    const int* devCol() const { return col_.as<int>(); }
+
+    // MIC(0) factorisation of the current values (FP64).
+    int factor(double tau, double ratio, std::string& err);
+    // Solves with b = rhs(), x = solution(); both device FP64.
+    int solve(const astg_pcg_params& prm, astg_pcg_result& res, std::string& err);
+
+    // Times `reps` launches of one kernel (ASTG_K_*) with events on the stream.
+    int benchKernel(int which, int reps, double& avgMs, double& bytes, std::string& err);
+    long long nnzL() const { return (nnz_ - n_) / 2; }
+
+    // Reference-form copy of the factor (host arrays), for parity checks.
+    int downloadFactor(std::vector<double>& invdiag, std::vector<unsigned>& colstart, std::vector<unsigned>& rowindex,
+                       std::vector<double>& value);
+    const std::vector<int>& hostRowstart() const { return hrs_; }
+    const std::vector<int>& hostCol() const { return hcol_; }
+
+private:
+    template <typename T>
+    int pcgInner(const T* A, const T* F, const T* invd, const T* b, T* x, double tolAbs, int maxIt, int norm,
+                 int& iters, double& resOut, bool& conv, std::string& err);
+    int syncScalars(std::string& err);
+    int checkErrors(const char* where, std::string& err);
+    struct LevelGate gate(DevBuf& counters) const;
+
+    hipStream_t stream_;
+    int n_ = 0;
+    long long nnz_ = 0;
+    int depth_ = 0;
+    std::vector<int> hrs_, hcol_;
+    DevBuf rs_, col_, lend_, ubeg_, tpos_, order_, lev_;
+    DevBuf val_, F_, invd_;                 // FP64 A, factor (symmetric copy), 1/pivot
+    DevBuf val32_, F32_, invd32_;           // FP32 copies for the mixed form
+    DevBuf b_, x_;                          // FP64 rhs / solution
+    DevBuf r_, z_, s_, q_, y_;              // work vectors (sized for FP64; reused as FP32)
+    DevBuf r64_, d64_;                      // mixed form outer vectors
+    DevBuf scal_, cnt_, part_;              // device scalars, counters, partials
+    DevBuf wlev_, wcnt_, gateF_, gateB_, gateM_;  // level gate tables and per-sweep counters
+    double* hscal_ = nullptr;               // pinned mirror of the scalars
+    int nblk_ = 0;   // blocks of row-parallel kernels
+    int P_ = 0;      // padded length of the level order
+    int nblkS_ = 0;  // blocks of the sweep / factor kernels
+};
+
+}  // namespace astg

@@ -1,0 +1,42 @@
+// Device-side projection driver: owns the stencil tables and velocity fields on
+// the device; assembly and velocity correction kernels live in project.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "pcg.h"
+#include "project_host.h"
+
+namespace astg {
+
+class ProjectDevice {
+public:
+    explicit ProjectDevice(hipStream_t s) : stream_(s) {}
+    int upload(const ProjectTables& t, const Octree& o, std::string& err);
+    // Writes A's values (CSR of the tables' pattern) and b (scaled by rho/dt).
+    int assemble(double* A, double* b, const int* rs, double rho, double dt, std::string& err);
+    // u -= dt/rho * grad p on MAC, tilt and half-tilt slots.
+    int correct(const double* p, double rho, double dt, std::string& err);
+    // Device-side copy of the velocity fields (restore = false saves).
+    int snapshot(bool restore, std::string& err);
+    double assembleBytes() const;
+    double* uMac() { return u_mac_.as<double>(); }    // 3*G, axis-major
+    double* uTilt() { return u_tilt_.as<double>(); }  // 8*G, slot-major
+
+private:
+    hipStream_t stream_;
+    const ProjectTables* T_ = nullptr;
+    DevBuf level_, tilt_e_, tilt_bits_, tilt_hd_, pid_tilt_, w_mac_, w_tilt_;
+    DevBuf oct_row_, oct_corner_, oct_face_, oct_pos_;
+    DevBuf tilt_row_, tilt_node_, tilt_nb_, tilt_nblev_, tilt_pos_;
+    DevBuf half_node_, half_c_, half_t_, half_closed_, half_nb_;
+    DevBuf hc_row_, hc_start_, hc_q_, hc_pos_diag_, hc_pos_off_;
+    DevBuf hr_row_, hr_start_, hr_q_, hr_pos_;
+    DevBuf vm_node_, vm_c_, vm_nb_, vt_node_, vt_c_, vt_nb_, vt_lev_;
+    DevBuf u_mac_, u_tilt_, snap_mac_, snap_tilt_;
+    bool haveSnap_ = false;
+};
+
+}  // namespace astg

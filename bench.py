@@ -1,0 +1,169 @@
+"""Benchmark: pressure projection on a 256^3-effective AST octree, FP64 vs mixed.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--seed 0] [--no-cpu]
+
+A step is one full projection (assembly of A and b, MIC(0) factorisation,
+PCG to 1e-6 relative residual in the reference's inf-norm test, velocity
+correction) of the same synthetic N(0,1) face-velocity frame, restored on the
+device before each step. Topology tables and the level schedule are built once
+(reported separately as setup). Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "ast-grid_amd"))
+
+MI355X_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def time_steps(ctx, info, prm, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        ctx.snapshot(restore=True)
+        ctx.project(info["dt"], info["rho"], prm)
+    torch.cuda.synchronize()
+    results = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.snapshot(restore=True)
+        results.append(ctx.project(info["dt"], info["rho"], prm))
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    return (t1 - t0) * 1e3 / steps, results
+
+
+def cpu_baseline(ctx, res64, budget_iters=12):
+    """Reference MIC(0)-PCG (oracle/_ref, the reference's own code) or the
+    oracle's port on the same C2 system: factor + a bounded number of PCG
+    iterations, timed, then scaled to the full iteration count."""
+    import numpy as np
+    n, nnz = res64.n, res64.nnz
+    rs, col, val, b, p = ctx.system(n, nnz)
+    from oracle import ref
+    kind = "reference" if ref.available() else "port"
+    if kind == "reference":
+        lib = ref.lib()
+        cores = lib.ref_num_workers()
+        t0 = time.perf_counter()
+        ref.pcg_csr(rs, col, val, b, tol=1e-6, maxit=0)
+        t_factor = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ref.pcg_csr(rs, col, val, b, tol=1e-6, maxit=budget_iters)
+        t_run = time.perf_counter() - t0
+    else:
+        from oracle import oracle as orc
+        cores = 1
+        t0 = time.perf_counter()
+        orc.pcg(rs, col, val, b, tol=1e-6, maxit=0)
+        t_factor = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        orc.pcg(rs, col, val, b, tol=1e-6, maxit=budget_iters)
+        t_run = time.perf_counter() - t0
+    t_iter = max(t_run - t_factor, 0.0) / budget_iters
+    est_ms = (t_factor + t_iter * res64.iterations) * 1e3
+    return {"value": round(est_ms, 1), "unit": "ms/step (solve only)", "cores": int(cores), "kind": kind,
+            "sample": "MIC(0) factor + %d PCG iterations on the same 256^3 system (n=%d, nnz=%d), %.1f s measured, "
+                      "scaled to the FP64 iteration count %d; assembly and velocity update not included"
+                      % (budget_iters, n, nnz, t_run + t_factor, res64.iterations)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernel-reps", type=int, default=5)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py: this build runs on one GPU only; multi-GPU sharding of the projection is design-only "
+              "(see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch  # device memory / stream plumbing only
+    import astg
+    from astg import scenes
+
+    assert torch.cuda.is_available(), "no GPU visible"
+    t0 = time.perf_counter()
+    ctx, info = scenes.projection_256(device=0, seed=a.seed)
+    prm64 = astg.default_params(tol=1e-6, max_iterations=3000, precision=astg.PREC_FP64)
+    prmmx = astg.default_params(tol=1e-6, max_iterations=3000, precision=astg.PREC_MIXED)
+    ctx.snapshot(restore=False)
+    ctx.project(info["dt"], info["rho"], prm64)  # builds device tables + level schedule
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    log("setup %.1f s, leaves %d" % (setup_s, info["leaves"]))
+
+    ms64, r64 = time_steps(ctx, info, prm64, a.steps, a.warmup)
+    log("fp64  %.2f ms/step iters %d depth %d" % (ms64, r64[-1].iterations, r64[-1].depth))
+    msmx, rmx = time_steps(ctx, info, prmmx, a.steps, a.warmup)
+    log("mixed %.2f ms/step iters %d outer %d" % (msmx, rmx[-1].iterations, rmx[-1].outer))
+    last64, lastmx = r64[-1], rmx[-1]
+
+    # per-kernel device times (HIP events on the context stream)
+    ctx.snapshot(restore=True)
+    ctx.project(info["dt"], info["rho"], prm64)
+    kern = {}
+    for name, k in (("spmv64", astg.K_SPMV64), ("fwd64", astg.K_FWD64), ("bwd64", astg.K_BWD64),
+                    ("mic", astg.K_MIC), ("assemble", astg.K_ASSEMBLE)):
+        ms, by = ctx.kernel_time(k, a.kernel_reps)
+        kern[name] = {"ms": round(ms, 4), "bytes": by, "GBs": round(by / ms / 1e6, 1)}
+    per_iter = {k: kern[k]["ms"] for k in ("spmv64", "fwd64", "bwd64")}
+    dom = max(per_iter, key=per_iter.get)
+    ach = kern[dom]["GBs"]
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": MI355X_HBM_GBS, "unit": "GB/s",
+                "frac": round(ach / MI355X_HBM_GBS, 4), "traffic": None,
+                "kernels": kern}
+
+    cpu = None
+    if not a.no_cpu:
+        try:
+            cpu = cpu_baseline(ctx, last64)
+        except Exception as e:  # the baseline is reported, never required
+            log("cpu baseline failed: %s" % e)
+
+    out = {
+        "metric": "projection ms/step to 1e-6 rel. residual @256^3 octree (mixed vs FP64); HBM GB/s",
+        "value": round(msmx, 3),
+        "unit": "ms/step",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(msmx, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 inner PCG + f64 refinement (value); f64 reference form reported beside it",
+        "data": "synthetic (seeded sphere octree, N(0,1) FP32 face velocities)",
+        "config": {"workload": "projection_256: 256^3-effective AST octree, 4 levels, %d leaves, n=%d unknowns, "
+                               "nnz=%d, 6 seeded spheres, level-0 tilts opened in spheres" % (
+                                   info["leaves"], last64.n, last64.nnz),
+                   "seed": a.seed, "tol": 1e-6, "norm": "inf (reference)"},
+        "fp64": {"ms_per_step": round(ms64, 3), "iterations": last64.iterations,
+                 "ms_assemble": round(last64.ms_assemble, 3), "ms_factor": round(last64.ms_factor, 3),
+                 "ms_solve": round(last64.ms_solve, 3), "ms_velocity": round(last64.ms_velocity, 3),
+                 "rel_residual": last64.rel_residual},
+        "mixed": {"ms_per_step": round(msmx, 3), "iterations": lastmx.iterations, "outer": lastmx.outer,
+                  "ms_solve": round(lastmx.ms_solve, 3), "rel_residual": lastmx.rel_residual},
+        "speedup_mixed_vs_fp64": round(ms64 / msmx, 3),
+        "sweep_depth": last64.depth,
+        "setup_s": round(setup_s, 2),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

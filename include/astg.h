@@ -1,0 +1,172 @@
+/*
+ * astg.h - C ABI of the MI355X-native pressure projection for the adaptive
+ * staggered-tilted (AST) octree.
+ *
+ * Plain C types only: pointers, sizes, status codes. No exception crosses the
+ * boundary; every entry point returns ASTG_OK (0) or a negative ASTG_ERR_*
+ * code, and astg_last_error() describes the last failure on a context.
+ *
+ * Reference interfaces replaced (citations are into the AST-Grid sources):
+ *   - the CUDA solver seam `CudaSolver::solve(n, rowstart, colindex, value, b, x)`
+ *     with `solver_error` / `solver_iters`
+ *     (src/octree_omnigrid_solver3.cpp:664-669, include/octree_omnigrid_solver3.h:101-105)
+ *     -> astg_pcg_solve_csr
+ *   - `OctreeOmniGridSolver3::project(dt)` (src/octree_omnigrid_solver3.cpp:350-679),
+ *     i.e. numbering + assembly + MIC(0)-PCG + `updateVelocityByPressure`
+ *     (681-847) -> astg_set_topology_* + astg_set_velocity + astg_project
+ *   - `PCGSolver<T>::solve` / `factor_modified_incomplete_cholesky0`
+ *     (include/pcgsolver/pcg_solver.h:89-190, 366-426) -> astg_pcg_solve_csr
+ *
+ * Threading: one context per host thread; a context owns one HIP stream and
+ * its device buffers. Distinct contexts may be used concurrently.
+ */
+#ifndef ASTG_H
+#define ASTG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ASTG_OK 0
+#define ASTG_ERR_ARG (-1)           /* invalid argument or size */
+#define ASTG_ERR_HIP (-2)           /* HIP runtime / device error */
+#define ASTG_ERR_NOT_CONVERGED (-3) /* PCG hit max_iterations, or rho == 0 / NaN */
+#define ASTG_ERR_TOPOLOGY (-4)      /* topology missing or inconsistent */
+#define ASTG_ERR_STATE (-5)         /* call order violated (e.g. project before topology) */
+
+/* Reference wall flags (include/constants.h:11-26). */
+#define ASTG_WALL_XNEG (1 << 0)
+#define ASTG_WALL_XPOS (1 << 1)
+#define ASTG_WALL_ZNEG (1 << 2)
+#define ASTG_WALL_ZPOS (1 << 3)
+#define ASTG_WALL_YNEG (1 << 4)
+#define ASTG_WALL_YPOS (1 << 5)
+
+/* Precision modes of the projection solve. */
+#define ASTG_PREC_FP64 0  /* FP64 MIC(0)-PCG, the reference's arithmetic */
+#define ASTG_PREC_MIXED 1 /* FP32 inner PCG, FP64 MIC pivots and FP64 outer residual b-Ax */
+
+/* Norm of the convergence test. */
+#define ASTG_NORM_INF 0 /* reference: ||r||_inf <= tol * ||b||_inf (pcg_solver.h:376-408) */
+#define ASTG_NORM_L2 1
+
+typedef struct astg_ctx astg_ctx;
+
+typedef struct astg_pcg_params {
+    double tol;            /* relative tolerance factor (reference SOLVER_ACCURACY = 1e-5) */
+    int max_iterations;    /* reference initData: 3000 */
+    int norm;              /* ASTG_NORM_* */
+    double mic_tau;        /* modification parameter, reference 0.97 */
+    double min_diag_ratio; /* small-pivot fallback ratio, reference 0.25 */
+    int precision;         /* ASTG_PREC_* */
+    int max_outer;         /* mixed form: max refinement sweeps (0 = default 20) */
+} astg_pcg_params;
+
+typedef struct astg_pcg_result {
+    double residual;     /* final ||r|| in the chosen norm (absolute, like solver_error) */
+    double rel_residual; /* residual / ||b|| */
+    int iterations;      /* PCG iterations (mixed form: total inner iterations) */
+    int outer;           /* refinement sweeps (mixed form), 0 otherwise */
+    int converged;       /* 1 if the tolerance was met */
+    int n;               /* unknowns */
+    long long nnz;       /* stored entries of A */
+    int depth;           /* dependency levels of the triangular sweeps */
+    double ms_assemble, ms_factor, ms_solve, ms_velocity, ms_total; /* device times */
+} astg_pcg_result;
+
+/* Defaults: tol 1e-5, 3000 iterations, inf-norm, tau 0.97, ratio 0.25, FP64. */
+void astg_pcg_params_default(astg_pcg_params* p);
+
+int astg_create(astg_ctx** out, int device, uint32_t flags);
+int astg_destroy(astg_ctx* ctx);
+const char* astg_last_error(const astg_ctx* ctx);
+
+/* ---- CUDA-solver seam -------------------------------------------------- */
+/* Solves A x = b with MIC(0)-PCG on the device. A is CSR (rowstart n+1,
+ * colindex/value nnz), symmetric, with rows sorted by column as the
+ * reference's FixedSparseMatrix (include/pcgsolver/sparse_matrix.h:276-325).
+ * Host pointers; x is fully overwritten. Returns ASTG_ERR_NOT_CONVERGED when
+ * the tolerance is not met (x still holds the last iterate). */
+int astg_pcg_solve_csr(astg_ctx* ctx, int n, const int* rowstart, const int* colindex, const double* value,
+                       const double* b, double* x, const astg_pcg_params* params, astg_pcg_result* result);
+
+/* ---- scene helpers ---------------------------------------------------- */
+/* Split map (R^3 ints, x fastest, value = leaf level) refined toward a union
+ * of spheres (n x {cx,cy,cz,r}); band[l] = refinement distance for level l+1 -> l;
+ * then balanced so every leaf's 26 same-level neighbour blocks exist. */
+int astg_split_spheres(int R, int levels, int nspheres, const double* spheres, const double* band, int* split_out);
+
+/* ---- topology (reference node order) ---------------------------------- */
+/* Builds layers, ghosts, T-joint / half-tilt state (constrainOctreeTiltE3,
+ * octree_util.cpp:38-169) and initData's fix-ups (octree_omnigrid_solver3.cpp:128-145)
+ * from a split map. closed_flags: ASTG_WALL_* bits of closed walls. No collider
+ * (boundary SDF > 0 everywhere, so all cutting-cell weights are 1 except
+ * half-tilt slots, initCuttingCellWeights 1169-1238). */
+int astg_set_topology_split(astg_ctx* ctx, int R, double h0, int levels, const int* split, int closed_flags);
+
+/* Opens non-fixed level-l leaf tilts where mask[i] != 0 (e = h_l/2), closes the
+ * others, as adjustE commits its decision (octree_omnigrid_solver3.cpp:1150-1165). */
+int astg_set_tilt_open(astg_ctx* ctx, int level, const unsigned char* mask);
+
+int astg_level_info(astg_ctx* ctx, int level, int* nnode, int* nleaf);
+/* xyz: 3*nnode ints; flags: nnode ints (1 valid, 2 ghost, 4 ghost-from-coarse). */
+int astg_get_nodes(astg_ctx* ctx, int level, int* xyz, int* flags);
+/* bits: 1 largest, 2 closed, 4 fixed, 8 T-joint, 16 closed-by-T-joint, 32 half-tilt,
+ * 64 layout open, 128 layout fixed. */
+int astg_get_tilt(astg_ctx* ctx, int level, double* e, int* bits, int* half_dir);
+/* count receives the number of half-tilts; lxyz (may be NULL) 4*count ints. */
+int astg_get_half_tilts(astg_ctx* ctx, int* lxyz, int* count);
+/* Unknown numbering per node (-1 = not an unknown); n_total may be NULL. */
+int astg_get_numbering(astg_ctx* ctx, int level, int* oct, int* tilt, int* n_total);
+/* Cutting-cell weights: mac 3*nnode, tilt 8*nnode. */
+int astg_get_weights(astg_ctx* ctx, int level, double* mac, double* tilt8);
+
+/* ---- fields (per level, node order) ------------------------------------ */
+/* mac: 3*nnode (x,y,z per node), tilt8: 8*nnode. Host pointers. */
+int astg_set_velocity(astg_ctx* ctx, int level, const double* mac, const double* tilt8);
+int astg_get_velocity(astg_ctx* ctx, int level, double* mac, double* tilt8);
+
+/* ---- projection ------------------------------------------------------- */
+/* One pressure projection on the device: numbering (cached with topology),
+ * assembly of A and b, MIC(0) factorisation, PCG, velocity correction.
+ * Velocities are read from and written back to the context's device copy. */
+int astg_project(astg_ctx* ctx, double dt, double rho, const astg_pcg_params* params, astg_pcg_result* result);
+
+/* Last assembled system: rowstart n+1, col/val nnz, b n (scaled by rho/dt), p n.
+ * Any pointer may be NULL. On a host-only context (device < 0) only the
+ * pattern (rowstart, col) is available. */
+int astg_get_system(astg_ctx* ctx, int* rowstart, int* col, double* val, double* b, double* p);
+
+/* MIC(0) factor of the last assembled system, in the reference's column form
+ * (SparseColumnLowerFactor, pcg_solver.h:22-79): invdiag n, colstart n+1,
+ * rowindex / value nnz_L. Pass NULL arrays to query nnz_L. */
+int astg_get_mic_factor(astg_ctx* ctx, double* invdiag, unsigned* colstart, unsigned* rowindex, double* value,
+                        long long* nnz_l);
+
+/* ---- workload replay and diagnostics ---------------------------------- */
+/* Saves (restore = 0) or restores (restore = 1) a device-side copy of the
+ * velocity fields, so the same input frame can be projected repeatedly
+ * without host traffic. No counterpart in the reference. */
+int astg_fields_snapshot(astg_ctx* ctx, int restore);
+
+/* Average device time of one kernel of the current projection system over
+ * reps launches, timed with HIP events on the context's stream, and the
+ * kernel's algorithmic HBM bytes per launch. which: ASTG_K_*. Needs one
+ * astg_project call first. No counterpart in the reference. */
+#define ASTG_K_SPMV64 0  /* q = A s fused with s.q (FP64) */
+#define ASTG_K_FWD64 1   /* forward MIC sweep (FP64) */
+#define ASTG_K_BWD64 2   /* backward MIC sweep fused with z.r (FP64) */
+#define ASTG_K_MIC 3     /* MIC(0) factorisation */
+#define ASTG_K_ASSEMBLE 4 /* assembly of A and b */
+#define ASTG_K_SPMV32 5  /* q = A s fused with s.q (FP32 values and vectors) */
+#define ASTG_K_FWD32 6   /* forward MIC sweep (FP32) */
+#define ASTG_K_BWD32 7   /* backward MIC sweep (FP32) */
+int astg_kernel_time(astg_ctx* ctx, int which, int reps, double* avg_ms, double* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ASTG_H */

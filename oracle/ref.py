@@ -1,0 +1,224 @@
+"""ctypes wrapper of oracle/_ref/libastgref.so (TEST INFRASTRUCTURE ONLY).
+
+The library is the reference solver compiled from its own sources plus
+oracle/ref_harness.cpp (see oracle/ref.mk). It exists only in a container that
+has the reference tree; everything here raises ``RefUnavailable`` otherwise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_ref", "libastgref.so")
+
+
+class RefUnavailable(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def available():
+    return os.path.exists(LIB_PATH)
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not available():
+        raise RefUnavailable("oracle/_ref/libastgref.so not built (needs the reference sources)")
+    L = ctypes.CDLL(LIB_PATH)
+    P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    sig = {
+        "ref_last_error": (ctypes.c_char_p, []),
+        "ref_quiet": (None, []),
+        "ref_num_workers": (I, []),
+        "ref_create": (P, [I, D, I, P, I]),
+        "ref_destroy": (None, [P]),
+        "ref_level_size": (I, [P, I, I]),
+        "ref_nodes": (None, [P, I, P, P]),
+        "ref_tilt_nodes": (None, [P, I, P, P, P]),
+        "ref_half_tilts": (I, [P, P]),
+        "ref_set_tilt_open": (I, [P, I, P]),
+        "ref_set_velocity": (None, [P, I, P, P]),
+        "ref_get_velocity": (None, [P, I, P, P]),
+        "ref_get_weights": (None, [P, I, P, P]),
+        "ref_number": (I, [P]),
+        "ref_get_pindex": (None, [P, I, P, P]),
+        "ref_project": (I, [P, D, D, I]),
+        "ref_resolve": (I, [P, P, P, P]),
+        "ref_matrix_n": (I, [P]),
+        "ref_matrix_nnz": (I, [P]),
+        "ref_get_system": (None, [P, P, P, P, P, P]),
+        "ref_update_velocity": (I, [P, P, I, D]),
+        "ref_mic_factor": (I, [I, P, P, P, D, D, P, P, P, P]),
+        "ref_mic_apply": (None, [I, P, P, P, D, D, P, P]),
+        "ref_pcg_csr": (I, [I, P, P, P, P, P, D, I, D, D, P, P]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    L.ref_quiet()
+    _lib = L
+    return L
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class RefSolver:
+    """The reference OctreeOmniGridSolver3 driven through the harness."""
+
+    def __init__(self, split, R, levels, closed_flags):
+        self.L = lib()
+        split = np.ascontiguousarray(split, np.int32)
+        self.h = self.L.ref_create(R, 1.0 / R, levels, _p(split), closed_flags)
+        if not self.h:
+            raise RuntimeError("ref_create failed: %s" % self.L.ref_last_error())
+        self.levels = levels
+
+    def close(self):
+        if self.h:
+            self.L.ref_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def level_size(self, l, ghost=True):
+        return self.L.ref_level_size(self.h, l, 1 if ghost else 0)
+
+    def nodes(self, l):
+        n = self.level_size(l)
+        xyz = np.empty((n, 3), np.int32)
+        fl = np.empty(n, np.int32)
+        self.L.ref_nodes(self.h, l, _p(xyz), _p(fl))
+        return xyz, fl
+
+    def tilt(self, l):
+        n = self.level_size(l)
+        e = np.empty(n, np.float64)
+        bits = np.empty(n, np.int32)
+        hd = np.empty(n, np.int32)
+        self.L.ref_tilt_nodes(self.h, l, _p(e), _p(bits), _p(hd))
+        return e, bits, hd
+
+    def half_tilts(self):
+        n = self.L.ref_half_tilts(self.h, None)
+        out = np.empty((n, 4), np.int32)
+        self.L.ref_half_tilts(self.h, _p(out))
+        return out
+
+    def set_tilt_open(self, l, mask):
+        m = np.ascontiguousarray(mask, np.uint8)
+        if self.L.ref_set_tilt_open(self.h, l, _p(m)):
+            raise RuntimeError(self.L.ref_last_error())
+
+    def number(self):
+        n = self.L.ref_number(self.h)
+        if n < 0:
+            raise RuntimeError(self.L.ref_last_error())
+        return n
+
+    def pindex(self, l):
+        n = self.level_size(l)
+        o = np.empty(n, np.int32)
+        t = np.empty(n, np.int32)
+        self.L.ref_get_pindex(self.h, l, _p(o), _p(t))
+        return o, t
+
+    def weights(self, l):
+        n = self.level_size(l)
+        m = np.empty((n, 3), np.float64)
+        t = np.empty((n, 8), np.float64)
+        self.L.ref_get_weights(self.h, l, _p(m), _p(t))
+        return m, t
+
+    def set_velocity(self, l, mac, tilt8):
+        self.L.ref_set_velocity(self.h, l, _p(np.ascontiguousarray(mac, np.float64)),
+                                _p(np.ascontiguousarray(tilt8, np.float64)))
+
+    def velocity(self, l):
+        n = self.level_size(l)
+        m = np.empty((n, 3), np.float64)
+        t = np.empty((n, 8), np.float64)
+        self.L.ref_get_velocity(self.h, l, _p(m), _p(t))
+        return m, t
+
+    def project(self, dt, tol=1e-6, maxit=3000):
+        if self.L.ref_project(self.h, dt, tol, maxit):
+            raise RuntimeError(self.L.ref_last_error())
+
+    def system(self):
+        n = self.L.ref_matrix_n(self.h)
+        nnz = self.L.ref_matrix_nnz(self.h)
+        rs = np.empty(n + 1, np.int32)
+        col = np.empty(nnz, np.int32)
+        val = np.empty(nnz, np.float64)
+        b = np.empty(n, np.float64)
+        p = np.empty(n, np.float64)
+        self.L.ref_get_system(self.h, _p(rs), _p(col), _p(val), _p(b), _p(p))
+        return rs, col, val, b, p
+
+    def resolve(self, n):
+        x = np.empty(n, np.float64)
+        r = ctypes.c_double()
+        it = ctypes.c_int()
+        if self.L.ref_resolve(self.h, _p(x), ctypes.byref(r), ctypes.byref(it)):
+            raise RuntimeError(self.L.ref_last_error())
+        return x, r.value, it.value
+
+    def update_velocity(self, p, dt):
+        p = np.ascontiguousarray(p, np.float64)
+        if self.L.ref_update_velocity(self.h, _p(p), len(p), dt):
+            raise RuntimeError(self.L.ref_last_error())
+
+
+def mic_factor(rs, col, val, tau=0.97, ratio=0.25):
+    L = lib()
+    rs = np.ascontiguousarray(rs, np.int32)
+    col = np.ascontiguousarray(col, np.int32)
+    val = np.ascontiguousarray(val, np.float64)
+    n = len(rs) - 1
+    nnz = L.ref_mic_factor(n, _p(rs), _p(col), _p(val), tau, ratio, None, None, None, None)
+    inv = np.empty(n, np.float64)
+    cs = np.empty(n + 1, np.uint32)
+    ri = np.empty(nnz, np.uint32)
+    v = np.empty(nnz, np.float64)
+    L.ref_mic_factor(n, _p(rs), _p(col), _p(val), tau, ratio, _p(inv), _p(cs), _p(ri), _p(v))
+    return inv, cs, ri, v
+
+
+def mic_apply(rs, col, val, r, tau=0.97, ratio=0.25):
+    L = lib()
+    rs = np.ascontiguousarray(rs, np.int32)
+    col = np.ascontiguousarray(col, np.int32)
+    val = np.ascontiguousarray(val, np.float64)
+    r = np.ascontiguousarray(r, np.float64)
+    z = np.empty_like(r)
+    L.ref_mic_apply(len(rs) - 1, _p(rs), _p(col), _p(val), tau, ratio, _p(r), _p(z))
+    return z
+
+
+def pcg_csr(rs, col, val, b, tol=1e-6, maxit=3000, tau=0.97, ratio=0.25):
+    L = lib()
+    rs = np.ascontiguousarray(rs, np.int32)
+    col = np.ascontiguousarray(col, np.int32)
+    val = np.ascontiguousarray(val, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    x = np.empty_like(b)
+    r = ctypes.c_double()
+    it = ctypes.c_int()
+    ok = L.ref_pcg_csr(len(rs) - 1, _p(rs), _p(col), _p(val), _p(b), _p(x), tol, maxit, tau, ratio,
+                       ctypes.byref(r), ctypes.byref(it))
+    if ok < 0:
+        raise RuntimeError(L.ref_last_error())
+    return x, r.value, it.value, bool(ok)

@@ -1,0 +1,406 @@
+// Checker-side harness around the reference solver (TEST INFRASTRUCTURE ONLY).
+//
+// This file is compiled together with the reference's own sources, where they
+// lie under the reference tree, by oracle/ref.mk into oracle/_ref/libastgref.so.
+// Nothing in the product links or loads it: tests/golden/make_golden.py uses it
+// to write fixtures, the CPU tests use it (when present) to pin the oracle, and
+// bench.py may time it as the reference CPU baseline.
+//
+// It drives OctreeOmniGridSolver3 (include/octree_omnigrid_solver3.h:23-129)
+// through a subclass that
+//   * fills the split map from a caller-provided array (splitMapSetup callback,
+//     octree_omnigrid_solver3.h:36, consumed at octree_omnigrid_solver3.cpp:67),
+//   * replaces initObstacle (octree_omnigrid_solver3.cpp:152-219) by closed-wall
+//     flags plus a constant +1e4 boundary SDF (no collider),
+//   * exposes layout / tilt / numbering / matrix / velocity state for dumping.
+// The C functions below take plain pointers and sizes only.
+
+#include <cstdint>
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include <spdlog/spdlog.h>
+
+#include "octree_omnigrid_solver3.h"
+#include "octree_util.h"
+#include "pcgsolver/pcg_solver.h"
+
+using namespace Omni;
+
+namespace {
+
+std::string g_err;
+
+class HarnessSolver : public OctreeOmniGridSolver3 {
+public:
+    HarnessSolver(int R, double h, int levels, const int* split, int closedFlags)
+        : OctreeOmniGridSolver3(Size3(R, R, R), h, "astg_ref_harness"),
+          split_(split, split + (size_t)R * R * R), R_(R), closed_(closedFlags) {
+        numOctreeLayer = levels;
+        isRestart = true;         // skips the layout dump in initData
+        useFixedTimeStep = true;
+        enableTiltAdaptivity = false;
+        splitMapSetup = [this](OctreeOmniGridSolver3* s) {
+            auto& sm = s->getSplitMap();
+            for (int z = 0; z < R_; ++z)
+                for (int y = 0; y < R_; ++y)
+                    for (int x = 0; x < R_; ++x)
+                        sm->set(Size3(x, y, z), split_[((size_t)z * R_ + y) * R_ + x]);
+        };
+    }
+
+    void initObstacle() override {
+        boundarySolver->setClosedDomainBoundaryFlag(closed_);
+        auto sdf = std::make_shared<CellCenteredScalarGrid3>(resolution, delta_h, 10000);
+        boundarySolver->updateSdf(sdf);
+        boundarySdf = boundarySolver->colliderSdf();
+    }
+
+    // --- accessors -------------------------------------------------------
+    OctreeGridLayout3Ptr& lay() { return layout; }
+    OctreeTiltENodeGrid3Ptr& tilt() { return tiltE; }
+    OctreeTiltENodeGrid3Ptr& ttilt() { return tTiltE; }
+    OctreeOmniGridLayoutPtr<3>& tiltLayout() { return tilt_layout; }
+    OctreeOmniFlagGrid3Ptr& pidx() { return pIndex; }
+    OctreeOmniFaceCenteredGrid3Ptr& weights() { return weightsCC; }
+    PCG::SparseMatrix<REAL>& mat() { return A; }
+    VectorXd& rhs() { return b; }
+    VectorXd& sol() { return vRes; }
+    PCGSolver<REAL>& pcg() { return pcgSolver; }
+    double h() const { return delta_h; }
+
+    void doProject(double dt) { project(dt); }
+    void doUpdateVelocity(const VectorXd& p, double dt) { updateVelocityByPressure(p, dt); }
+    int doNumber() { return updateIndexParallel(); }
+    void doAdvect(double dt) { advect(dt); }
+    void doStep(double dt) { onUpdate(dt); }
+
+private:
+    std::vector<int> split_;
+    int R_;
+    int closed_;
+};
+
+HarnessSolver* H(void* p) { return static_cast<HarnessSolver*>(p); }
+
+LCOOR_T nodeCoor(OctreeGridLayout3Ptr& layout, int level, int i) {
+    auto layer = layout->getLayerPtr(level);
+    int nleaf = layer->nodeSize();
+    return i < nleaf ? layer->getNodeIdx()[i] : layer->getGhostNodeIdx()[i - nleaf];
+}
+
+template <typename F>
+int guarded(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    } catch (...) {
+        g_err = "unknown exception";
+        return -1;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ref_last_error() { return g_err.c_str(); }
+
+void ref_quiet() { spdlog::set_level(spdlog::level::off); }
+
+int ref_num_workers() { return TaskManager::tf.num_workers(); }
+
+// split: R^3 ints, x fastest, value = level of the leaf covering that finest cell.
+void* ref_create(int R, double h, int levels, const int* split, int closedFlags) {
+    HarnessSolver* s = nullptr;
+    int rc = guarded([&] {
+        s = new HarnessSolver(R, h, levels, split, closedFlags);
+        s->initialize();
+    });
+    if (rc != 0) {
+        delete s;
+        return nullptr;
+    }
+    return s;
+}
+
+void ref_destroy(void* p) { delete H(p); }
+
+int ref_level_size(void* p, int level, int includeGhost) {
+    return H(p)->lay()->levelSize(level, includeGhost != 0);
+}
+
+// xyz: 3*n ints, flags: n ints (LayerNodeFlagMask bits), in layer-node index order.
+void ref_nodes(void* p, int level, int* xyz, int* flags) {
+    auto& layout = H(p)->lay();
+    auto layer = layout->getLayerPtr(level);
+    int n = layer->size();
+    for (int i = 0; i < n; ++i) {
+        LCOOR_T c = nodeCoor(layout, level, i);
+        const auto& nd = layer->getMapTable()[c];
+        xyz[3 * i + 0] = nd.xIdx;
+        xyz[3 * i + 1] = nd.yIdx;
+        xyz[3 * i + 2] = nd.zIdx;
+        flags[i] = (int)nd.flag;
+        if ((int)nd.index != i) flags[i] |= 1 << 30;  // index mismatch marker
+    }
+}
+
+// Tilt state per layer node. bits: 0 is_largest, 1 is_closed, 2 is_fixed,
+// 3 is_T_joint, 4 is_closed_by_T_joint, 5 is_half_tilt, 6 layout.is_open,
+// 7 layout.is_fixed.
+void ref_tilt_nodes(void* p, int level, double* e, int* bits, int* halfDir) {
+    auto* s = H(p);
+    auto& layout = s->lay();
+    auto& data = s->tilt()->getData()[level];
+    int n = (int)data.size();
+    for (int i = 0; i < n; ++i) {
+        const TiltENode& t = data[i];
+        LCOOR_T c = nodeCoor(layout, level, i) | ((LCOOR_T)level << 60);
+        const TiltENodeS& ts = s->tiltLayout()->getTiltNode(c);
+        e[i] = t.e;
+        bits[i] = (t.is_largest ? 1 : 0) | (t.is_closed ? 2 : 0) | (t.is_fixed ? 4 : 0) |
+                  (t.is_T_joint ? 8 : 0) | (t.is_closed_by_T_joint ? 16 : 0) |
+                  (t.is_half_tilt ? 32 : 0) | (ts.is_open ? 64 : 0) | (ts.is_fixed ? 128 : 0);
+        halfDir[i] = t.half_direction;
+    }
+}
+
+// Returns the number of half-tilts; out (may be null) receives (level,x,y,z) quads.
+int ref_half_tilts(void* p, int* out) {
+    auto& v = H(p)->lay()->getHalfGhostIdx();
+    if (out) {
+        for (size_t i = 0; i < v.size(); ++i) {
+            out[4 * i + 0] = UNPACK_LEVEL3(v[i]);
+            out[4 * i + 1] = UNPACK_X3(v[i]);
+            out[4 * i + 2] = UNPACK_Y3(v[i]);
+            out[4 * i + 3] = UNPACK_Z3(v[i]);
+        }
+    }
+    return (int)v.size();
+}
+
+// Opens/closes non-fixed leaf tilts of one level from a per-leaf mask, the way
+// adjustE (octree_omnigrid_solver3.cpp:1125-1166) commits its decision:
+// layout flag, then setE(h/2,h/2) or setE(0), then ghost propagation.
+int ref_set_tilt_open(void* p, int level, const unsigned char* openMask) {
+    auto* s = H(p);
+    return guarded([&] {
+        auto& layout = s->lay();
+        auto layer = layout->getLayerPtr(level);
+        int nleaf = layer->nodeSize();
+        double maxE = layout->levelGridSpacing(level) * 0.5;
+        for (int i = 0; i < nleaf; ++i) {
+            LCOOR_T c = layer->getNodeIdx()[i] | ((LCOOR_T)level << 60);
+            auto& ts = s->tiltLayout()->getTiltNode(c);
+            if (ts.is_fixed) continue;
+            ts.is_open = openMask[i] != 0;
+            for (auto* g : {&s->tilt(), &s->ttilt()}) {
+                auto& node = (*g)->get(c);
+                if (ts.is_open) node.setE(maxE, maxE);
+                else node.setE(0);
+            }
+        }
+        s->tiltLayout()->updateGhost();
+        s->tilt()->updateGhost();
+        s->ttilt()->updateGhost();
+    });
+}
+
+// mac: n*3 (x,y,z per node), tilt: n*8, in layer-node index order.
+void ref_set_velocity(void* p, int level, const double* mac, const double* tilt8) {
+    auto* s = H(p);
+    auto& v = s->getVelocity();
+    auto& mg = v->getMACGrid();
+    auto& td = v->getTiltGrid()->getGridPtr()->getData()[level];
+    int n = (int)td.size();
+    for (int i = 0; i < n; ++i) {
+        mg->getUx()[level][i] = mac[3 * i + 0];
+        mg->getUy()[level][i] = mac[3 * i + 1];
+        mg->getUz()[level][i] = mac[3 * i + 2];
+        for (int k = 0; k < 8; ++k) td[i](k) = tilt8[8 * i + k];
+    }
+}
+
+void ref_get_velocity(void* p, int level, double* mac, double* tilt8) {
+    auto* s = H(p);
+    auto& v = s->getVelocity();
+    auto& mg = v->getMACGrid();
+    auto& td = v->getTiltGrid()->getGridPtr()->getData()[level];
+    int n = (int)td.size();
+    for (int i = 0; i < n; ++i) {
+        mac[3 * i + 0] = mg->getUx()[level][i];
+        mac[3 * i + 1] = mg->getUy()[level][i];
+        mac[3 * i + 2] = mg->getUz()[level][i];
+        for (int k = 0; k < 8; ++k) tilt8[8 * i + k] = td[i](k);
+    }
+}
+
+void ref_get_weights(void* p, int level, double* mac, double* tilt8) {
+    auto* s = H(p);
+    auto& w = s->weights();
+    auto& mg = w->getMACGrid();
+    auto& td = w->getTiltGrid()->getGridPtr()->getData()[level];
+    int n = (int)td.size();
+    for (int i = 0; i < n; ++i) {
+        mac[3 * i + 0] = mg->getUx()[level][i];
+        mac[3 * i + 1] = mg->getUy()[level][i];
+        mac[3 * i + 2] = mg->getUz()[level][i];
+        for (int k = 0; k < 8; ++k) tilt8[8 * i + k] = td[i](k);
+    }
+}
+
+// Runs updateIndexParallel (octree_omnigrid_solver3.cpp:292-347); returns n.
+int ref_number(void* p) {
+    int n = -1;
+    int rc = guarded([&] { n = H(p)->doNumber(); });
+    return rc == 0 ? n : -1;
+}
+
+void ref_get_pindex(void* p, int level, int* oct, int* tiltIdx) {
+    auto* s = H(p);
+    auto& od = s->pidx()->getOctagonGrid()->getData()[level];
+    auto& td = s->pidx()->getTiltGrid()->getData()[level];
+    for (size_t i = 0; i < od.size(); ++i) {
+        oct[i] = od[i];
+        tiltIdx[i] = td[i];
+    }
+}
+
+// Full reference project(dt) (octree_omnigrid_solver3.cpp:350-679) with the
+// PCG tolerance factor and iteration cap given here.
+int ref_project(void* p, double dt, double tol, int maxit) {
+    auto* s = H(p);
+    return guarded([&] {
+        s->pcg().set_solver_parameters(tol, maxit);
+        s->doProject(dt);
+    });
+}
+
+// Re-solves the last assembled system with the same PCGSolver settings to
+// recover the residual and iteration count that project() only logs.
+int ref_resolve(void* p, double* x, double* residual, int* iters) {
+    auto* s = H(p);
+    return guarded([&] {
+        VectorXd xs(s->rhs().rows());
+        double r = 0;
+        int it = 0;
+        s->pcg().solve(s->mat(), s->rhs(), xs, r, it);
+        for (int i = 0; i < xs.rows(); ++i) x[i] = xs[i];
+        *residual = r;
+        *iters = it;
+    });
+}
+
+int ref_matrix_n(void* p) { return (int)H(p)->mat().n; }
+
+int ref_matrix_nnz(void* p) {
+    auto& A = H(p)->mat();
+    long nnz = 0;
+    for (unsigned i = 0; i < A.n; ++i) nnz += (long)A.index[i].size();
+    return (int)nnz;
+}
+
+// CSR of the last assembled matrix, the scaled right-hand side b and solution.
+void ref_get_system(void* p, int* rowstart, int* col, double* val, double* b, double* x) {
+    auto* s = H(p);
+    auto& A = s->mat();
+    int k = 0;
+    rowstart[0] = 0;
+    for (unsigned i = 0; i < A.n; ++i) {
+        for (size_t j = 0; j < A.index[i].size(); ++j) {
+            col[k] = (int)A.index[i][j];
+            val[k] = A.value[i][j];
+            ++k;
+        }
+        rowstart[i + 1] = k;
+    }
+    for (unsigned i = 0; i < A.n; ++i) {
+        b[i] = s->rhs()[i];
+        x[i] = s->sol()[i];
+    }
+}
+
+// Velocity correction alone (octree_omnigrid_solver3.cpp:681-847) with a given p.
+int ref_update_velocity(void* p, const double* pres, int n, double dt) {
+    auto* s = H(p);
+    return guarded([&] {
+        VectorXd pv(n);
+        for (int i = 0; i < n; ++i) pv[i] = pres[i];
+        s->doUpdateVelocity(pv, dt);
+    });
+}
+
+// ---- solver-level entry points on a caller CSR (PCGSolver, pcg_solver.h) ----
+
+static void csrToRows(int n, const int* rs, const int* ci, const double* v, PCG::SparseMatrix<double>& A) {
+    A.resize(n);
+    A.zero();
+    for (int i = 0; i < n; ++i)
+        for (int k = rs[i]; k < rs[i + 1]; ++k) {
+            A.index[i].push_back((unsigned)ci[k]);
+            A.value[i].push_back(v[k]);
+        }
+}
+
+// MIC(0) factor (pcg_solver.h:89-190). Returns nnz of the strict lower part;
+// arrays may be null on a size query.
+int ref_mic_factor(int n, const int* rs, const int* ci, const double* v, double tau, double ratio,
+                   double* invdiag, unsigned* colstart, unsigned* rowindex, double* value) {
+    PCG::SparseMatrix<double> A;
+    csrToRows(n, rs, ci, v, A);
+    SparseColumnLowerFactor<double> f;
+    factor_modified_incomplete_cholesky0(A, f, tau, ratio);
+    if (invdiag) {
+        for (int i = 0; i < n; ++i) invdiag[i] = f.invdiag[i];
+        for (int i = 0; i <= n; ++i) colstart[i] = f.colstart[i];
+        for (size_t k = 0; k < f.value.size(); ++k) {
+            rowindex[k] = f.rowindex[k];
+            value[k] = f.value[k];
+        }
+    }
+    return (int)f.value.size();
+}
+
+// L L^T z = r with the factor above (pcg_solver.h:211-253).
+void ref_mic_apply(int n, const int* rs, const int* ci, const double* v, double tau, double ratio,
+                   const double* r, double* z) {
+    PCG::SparseMatrix<double> A;
+    csrToRows(n, rs, ci, v, A);
+    SparseColumnLowerFactor<double> f;
+    factor_modified_incomplete_cholesky0(A, f, tau, ratio);
+    Eigen::VectorXd rr(n), zz(n);
+    for (int i = 0; i < n; ++i) rr[i] = r[i];
+    solve_lower(f, rr, zz);
+    solve_lower_transpose_in_place(f, zz);
+    for (int i = 0; i < n; ++i) z[i] = zz[i];
+}
+
+// MIC(0)-PCG (Eigen overload, pcg_solver.h:366-426) on a caller CSR.
+int ref_pcg_csr(int n, const int* rs, const int* ci, const double* v, const double* b, double* x,
+                double tol, int maxit, double tau, double ratio, double* residual, int* iters) {
+    int ok = 0;
+    int rc = guarded([&] {
+        PCG::SparseMatrix<double> A;
+        csrToRows(n, rs, ci, v, A);
+        PCGSolver<double> solver;
+        solver.set_solver_parameters(tol, maxit, tau, ratio);
+        VectorXd bb(n), xx(n);
+        for (int i = 0; i < n; ++i) bb[i] = b[i];
+        double r = 0;
+        int it = 0;
+        ok = solver.solve(A, bb, xx, r, it) ? 1 : 0;
+        for (int i = 0; i < n; ++i) x[i] = xx[i];
+        *residual = r;
+        *iters = it;
+    });
+    return rc == 0 ? ok : -1;
+}
+
+}  // extern "C"
