@@ -30,44 +30,13 @@ enum ScalarSlot {
     S_ERR,          // nonzero: sweep spin limit hit / breakdown
     S_ALPHA,
     S_BETA,
+    S_OUTER_RES,    // mixed form: FP64 residual norm of the outer iteration
     S_COUNT = 16
 };
 // int counters in the counter block
 enum CounterSlot { C_TICKET_F = 0, C_TICKET_B, C_TICKET_L, C_TICKET_M, C_DONE_SQ, C_DONE_RES, C_DONE_RHO, C_DONE_B, C_ERR, C_COUNT = 16 };
 // Sticky error bits in cnt[C_ERR] (atomicOr): a spin limit was hit.
 enum ErrBit { E_LEVELS = 1, E_MIC = 2, E_FWD = 4, E_BWD = 8 };
-
-// Level gate: advisory throttle for the sync-free sweeps. Rows are stored in
-// level order, padded so each 64-row wave segment holds one level. A wave of
-// level L first waits (one lane, with backoff) until every wave of level
-// L - kGate (forward) or L + kGate (backward) has finished, so only about
-// kGate levels of lanes ever poll values at once. Correctness never depends on
-// the gate: values are still published and polled one by one.
-constexpr int kGate = 2;
-constexpr int kGateLimit = 1 << 16;
-struct LevelGate {
-    const int* wlev;  // level of each wave segment (-1: trailing padding)
-    const int* wcnt;  // wave segments per level
-    int* done;        // finished waves per level (zeroed before each sweep)
-    int nseg;
-    int depth;
-};
-
-__device__ inline void gate_wait(const LevelGate& G, int g) {
-    if (g < 0 || g >= G.depth) return;
-    if ((threadIdx.x & 63) == 0) {
-        const int need = G.wcnt[g];
-        int backoff = 1;
-        for (int s = 0; s < kGateLimit; ++s) {
-            if (__hip_atomic_load(&G.done[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= need) break;
-            for (int k = 0; k < backoff; ++k) __builtin_amdgcn_s_sleep(2);
-            backoff = backoff < 16 ? backoff * 2 : 16;
-        }
-    }
-}
-__device__ inline void gate_arrive(const LevelGate& G, int lev) {
-    if ((threadIdx.x & 63) == 0 && lev >= 0) atomicAdd(&G.done[lev], 1);
-}
 
 __device__ inline unsigned long long ld_agent_u64(const unsigned long long* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -113,6 +82,71 @@ __device__ inline double wave_sum(double v) {
 __device__ inline double wave_max(double v) {
     for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
     return v;
+}
+
+// Polls of the advisory gate before a wavefront proceeds anyway.
+constexpr int kGateLimit = 1 << 16;
+
+// A row's dependencies held in registers: column indices and factor values
+// are loaded before the row waits (they do not depend on other rows), then
+// all published values are polled together, one round trip per poll.
+constexpr int kMaxDeps = 16;
+template <typename T>
+struct DepRow {
+    int nd;
+    int c[kMaxDeps];
+    T f[kMaxDeps];
+    __device__ void preload(const int* col, const T* F, int p0, int pe) {
+        nd = pe - p0;
+#pragma unroll
+        for (int j = 0; j < kMaxDeps; ++j)
+            if (j < nd) {
+                c[j] = col[p0 + j];
+                f[j] = F[p0 + j];
+            }
+    }
+    // acc -= sum_j f[j] * vec[c[j]] in ascending j once every vec[c[j]] is
+    // published. Returns false if the spin limit was hit.
+    __device__ bool apply(const T* vec, T& acc) {
+        T v[kMaxDeps];
+        unsigned ready = 0;
+        const unsigned full = (1u << nd) - 1u;
+        for (int spins = 0;; ++spins) {
+#pragma unroll
+            for (int j = 0; j < kMaxDeps; ++j)
+                if (j < nd && !((ready >> j) & 1u)) {
+                    auto bits = Sentinel<T>::load(&vec[c[j]]);
+                    if (bits != Sentinel<T>::value) {
+                        v[j] = Sentinel<T>::val(bits);
+                        ready |= 1u << j;
+                    }
+                }
+            if (ready == full) break;
+            if (spins > kSpinLimit) return false;
+            __builtin_amdgcn_s_sleep(1);
+        }
+#pragma unroll
+        for (int j = 0; j < kMaxDeps; ++j)
+            if (j < nd) acc -= f[j] * v[j];
+        return true;
+    }
+};
+
+// Rows with more than kMaxDeps dependencies (never produced by the octree
+// assembly, possible through the CSR entry point): one dependency at a time.
+template <typename T>
+__device__ inline bool apply_deps_slow(const int* col, const T* F, int p, int pe, const T* vec, T& acc) {
+    for (int spins = 0; p < pe;) {
+        auto bits = Sentinel<T>::load(&vec[col[p]]);
+        if (bits == Sentinel<T>::value) {
+            if (++spins > kSpinLimit) return false;
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        acc -= F[p] * Sentinel<T>::val(bits);
+        ++p;
+    }
+    return true;
 }
 
 }  // namespace astg

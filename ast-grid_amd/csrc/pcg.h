@@ -33,6 +33,24 @@ struct DevBuf {
     T* as() const { return static_cast<T*>(p); }
 };
 
+// Rows in padded level order, plus one gate row per 64-row segment.
+struct SweepPlan {
+    const int* order;  // P entries, -1 = padding
+    const int* gateF;  // per segment: a row of level L-dist (forward), -1 none
+    const int* gateB;  // per segment: a row of level L+dist (backward), -1 none
+    int P;
+    int nchunks;       // P / 256
+};
+
+// Per-block partials of the reductions, consumed by the next kernel.
+struct Parts {
+    double* sq;    // gs partials of s.q
+    double* res;   // gs partials of |r| (max) or r.r (sum)
+    double* rho0;  // gb partials of rho (even iterations)
+    double* rho1;  // gb partials of rho (odd iterations)
+    int gs, gb;
+};
+
 class PcgDevice {
 public:
     explicit PcgDevice(hipStream_t s) : stream_(s) {}
@@ -40,7 +58,7 @@ public:
 
     // Uploads a CSR pattern (host arrays) and builds everything that depends on
     // the pattern only: diagonal / upper split, transpose map, dependency
-    // levels and level order (both computed on the device).
+    // levels (computed on the device) and the padded level order.
     int setPattern(int n, const int* rowstart, const int* col, std::string& err);
     // Device pointer to the FP64 value array (nnz) for in-place assembly.
     double* values() { return val_.as<double>(); }
@@ -69,29 +87,38 @@ public:
 
 private:
     template <typename T>
-    int pcgInner(const T* A, const T* F, const T* invd, const T* b, T* x, double tolAbs, int maxIt, int norm,
+    int pcgInner(const T* A, const T* F, const T* invd, const T* b, T* x, double tolRel, int maxIt, int norm,
                  int& iters, double& resOut, bool& conv, std::string& err);
+    template <typename T>
+    int launchFwd(const T* F, const T* invd, const T* r, T* y, T* z, int mode, int it, int maxIt, int norm,
+                  double tolRel, std::string& err);
+    template <typename T>
+    int launchBwd(const T* F, const T* invd, const T* y, const T* r, T* z, int parityOut, std::string& err);
+    int launchMic(double tau, double ratio, std::string& err);
     int syncScalars(std::string& err);
     int checkErrors(const char* where, std::string& err);
-    struct LevelGate gate(DevBuf& counters) const;
+    SweepPlan plan() const;
+    Parts parts(int gb) const;
 
     hipStream_t stream_;
     int n_ = 0;
     long long nnz_ = 0;
     int depth_ = 0;
     std::vector<int> hrs_, hcol_;
-    DevBuf rs_, col_, lend_, ubeg_, tpos_, order_, lev_;
+    DevBuf rs_, col_, lend_, ubeg_, tpos_, order_, lev_, gateF_, gateB_;
     DevBuf val_, F_, invd_;                 // FP64 A, factor (symmetric copy), 1/pivot
     DevBuf val32_, F32_, invd32_;           // FP32 copies for the mixed form
     DevBuf b_, x_;                          // FP64 rhs / solution
     DevBuf r_, z_, s_, q_, y_;              // work vectors (sized for FP64; reused as FP32)
     DevBuf r64_, d64_;                      // mixed form outer vectors
     DevBuf scal_, cnt_, part_;              // device scalars, counters, partials
-    DevBuf wlev_, wcnt_, gateF_, gateB_, gateM_;  // level gate tables and per-sweep counters
     double* hscal_ = nullptr;               // pinned mirror of the scalars
-    int nblk_ = 0;   // blocks of row-parallel kernels
-    int P_ = 0;      // padded length of the level order
-    int nblkS_ = 0;  // blocks of the sweep / factor kernels
+    int P_ = 0;                             // padded length of the level order
+    int cus_ = 0;                           // compute units
+    int gs_ = 0;                            // grid of the streaming kernels
+    int gb_ = 0;                            // largest backward-sweep grid (rho partial slots)
+    int gF64_ = 0, gB64_ = 0, gF32_ = 0, gB32_ = 0, gM_ = 0;  // persistent grids
+    int gateDist_ = 2;
 };
 
 }  // namespace astg

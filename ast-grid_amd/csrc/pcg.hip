@@ -1,4 +1,22 @@
 // Device MIC(0)-PCG (see pcg.h). Kernels are written for gfx950 wave64.
+//
+// Synchronisation design
+//   * Rows are processed in dependency-level order. The order is padded so
+//     every 64-row wavefront segment holds rows of one level only; rows of one
+//     level are independent, so no wavefront ever waits on one of its lanes.
+//   * The sweeps and the factorisation are persistent cooperative kernels:
+//     all blocks are co-resident and block b takes chunks b, b+G, b+2G, ... in
+//     order. The smallest unfinished chunk therefore always belongs to a
+//     running block whose dependencies (earlier chunks) are complete, so the
+//     spin-waits always end. No ticket atomics are needed.
+//   * A row publishes its value with one agent-scope atomic store over a
+//     signalling-NaN sentinel; dependants poll the values they need, all at
+//     once. A wavefront first waits (one lane, with backoff) on a single
+//     representative row of the level `dist` levels back, which keeps the
+//     number of polling lanes to a few levels' worth.
+//   * Reductions are deterministic and atomic-free: every block writes one
+//     partial, and the next kernel in the stream reduces the partials it needs
+//     in a fixed order.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -20,6 +38,7 @@
 namespace astg {
 
 constexpr int BS = 256;
+constexpr int kLpr = 8;  // lanes per row in the SpMV kernels
 
 DevBuf::~DevBuf() {
     if (p) (void)hipFree(p);
@@ -57,55 +76,47 @@ __global__ void k_d2f(const double* a, float* b, long n) {
 __global__ void k_zero_counters(int* c) {
     if (threadIdx.x < C_COUNT) c[threadIdx.x] = 0;
 }
+__global__ void k_zero64(int n, double* x) {
+    const int i = blockIdx.x * BS + threadIdx.x;
+    if (i < n) x[i] = 0.0;
+}
+__global__ void k_fill64(int n, double* x, double v) {
+    const int i = blockIdx.x * BS + threadIdx.x;
+    if (i < n) x[i] = v;
+}
 
-// Block-level reduction into one partial per slot, and the final reduction by
-// the last block to finish (fixed order over slots, so results are
-// deterministic). Returns true in the block that performed the final step.
+// ---------------------------------------------------------------------------
+// deterministic block reductions
+// ---------------------------------------------------------------------------
 template <bool MAX>
-__device__ inline bool reduce_partials(double v, double* part, int slot, int nslots, int* doneCtr, double* out) {
+__device__ inline double block_reduce(double v) {
     __shared__ double sm[BS / 64];
-    __shared__ bool last;
     v = MAX ? wave_max(v) : wave_sum(v);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __syncthreads();  // sm may still be read by a previous call
     if (lane == 0) sm[wid] = v;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double t = sm[0];
-        for (int w = 1; w < BS / 64; ++w) t = MAX ? fmax(t, sm[w]) : t + sm[w];
-        st_agent_u64(reinterpret_cast<unsigned long long*>(&part[slot]), (unsigned long long)__double_as_longlong(t));
-        __threadfence();
-        int prev = atomicAdd(doneCtr, 1);
-        last = (prev == nslots - 1);
-    }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
-    double acc = MAX ? 0.0 : 0.0;
-    for (int s = threadIdx.x; s < nslots; s += BS) {
-        double t = __longlong_as_double((long long)ld_agent_u64(reinterpret_cast<const unsigned long long*>(&part[s])));
-        acc = MAX ? fmax(acc, t) : acc + t;
-    }
-    acc = MAX ? wave_max(acc) : wave_sum(acc);
-    if (lane == 0) sm[wid] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double t = sm[0];
-        for (int w = 1; w < BS / 64; ++w) t = MAX ? fmax(t, sm[w]) : t + sm[w];
-        *out = t;
-        *doneCtr = 0;
-    }
-    return true;
+    double t = sm[0];
+    for (int w = 1; w < BS / 64; ++w) t = MAX ? fmax(t, sm[w]) : t + sm[w];
+    return t;
+}
+// Sum (or max) of np partials written by the previous kernel, same order in
+// every block.
+template <bool MAX>
+__device__ inline double reduce_prev(const double* part, int np) {
+    double acc = 0.0;
+    for (int s = threadIdx.x; s < np; s += BS) acc = MAX ? fmax(acc, part[s]) : acc + part[s];
+    return block_reduce<MAX>(acc);
 }
 
 // ---------------------------------------------------------------------------
 // dependency levels of tril(A): level(i) = 1 + max level(k), k in lower(i).
-// One row per wavefront (its lanes poll the row's dependencies in parallel),
-// rows handed out in index order through a ticket: every row a wave waits on
-// belongs to an earlier, already running wave, and no wave ever waits on one
-// of its own lanes.
+// Setup only. One row per wavefront (its lanes poll the row's dependencies in
+// parallel), rows handed out in index order through a ticket: every row a wave
+// waits on belongs to an earlier, already running wave.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(BS) k_levels(int n, const int* rs, const int* lend, const int* col, int* lev, int* cnt,
-                                               double* scal) {
+__global__ void __launch_bounds__(BS) k_levels(int n, const int* rs, const int* lend, const int* col, int* lev,
+                                               int* cnt) {
     __shared__ int chunk;
     if (threadIdx.x == 0) chunk = atomicAdd(&cnt[C_TICKET_L], 1);
     __syncthreads();
@@ -134,40 +145,82 @@ __global__ void __launch_bounds__(BS) k_levels(int n, const int* rs, const int* 
     if (lane == 0) st_agent_i32(&lev[i], L);
 }
 
+// One lane of a wavefront polls the published value of a gate row (advisory
+// throttle; bounded).
+template <typename T>
+__device__ inline void gate_row(const T* vec, int target) {
+    if (target < 0 || (threadIdx.x & 63) != 0) return;
+    int backoff = 1;
+    for (int s = 0; s < kGateLimit; ++s) {
+        if (Sentinel<T>::load(&vec[target]) != Sentinel<T>::value) return;
+        for (int k = 0; k < backoff; ++k) __builtin_amdgcn_s_sleep(2);
+        backoff = backoff < 8 ? backoff * 2 : 8;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // MIC(0) factorisation, one row per lane, rows in level order.
-// Row k waits until every column m in lower(k) is final (invdiag[m] published),
-// then computes its pivot and column k of L (stored at the upper positions of
-// row k and, through tpos, at the matching lower positions), then publishes
-// invdiag[k]. Accumulation order follows the reference's right-looking loop:
+// Row k computes its pivot and column k of L (stored at the upper positions of
+// row k and, through tpos, at the matching lower positions of the rows below).
+// Every factor entry is written exactly once, over a sentinel, so readers poll
+// the entries themselves: no flags, fences or cache invalidation. Accumulation
+// order follows the reference's right-looking loop (pcg_solver.h:129-189):
 // contributions to row/column k arrive in ascending m.
 // ---------------------------------------------------------------------------
-__device__ inline void mic_row(const int k, const int r0, const int le, const int ub, const int r1, const int* rs,
+__device__ inline bool poll_all(const double* F, const int* pos, double* out, int cnt) {
+    unsigned ready = 0;
+    const unsigned full = cnt >= 32 ? ~0u : ((1u << cnt) - 1u);
+    for (int spins = 0;; ++spins) {
+#pragma unroll
+        for (int j = 0; j < kMaxDeps; ++j)
+            if (j < cnt && !((ready >> j) & 1u)) {
+                const unsigned long long b = ld_agent_u64(reinterpret_cast<const unsigned long long*>(&F[pos[j]]));
+                if (b != kSent64) {
+                    out[j] = __longlong_as_double((long long)b);
+                    ready |= 1u << j;
+                }
+            }
+        if (ready == full) return true;
+        if (spins > kSpinLimit) return false;
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+__device__ inline bool mic_row(const int k, const int r0, const int le, const int ub, const int r1, const int* rs,
                                const int* lend, const int* ubeg, const int* col, const double* A, const int* tpos,
                                double* F, double* invd, double tau, double ratio) {
+    bool ok = true;
+    const int nl = le - r0;
+    int pos[kMaxDeps];
+    double Lk[kMaxDeps], fv[kMaxDeps];
+#pragma unroll
+    for (int j = 0; j < kMaxDeps; ++j)
+        if (j < nl) pos[j] = r0 + j;
+    ok &= poll_all(F, pos, Lk, nl);  // L(k,m) for every m in lower(k): waits for those columns
     const double akk = (le < r1 && col[le] == k) ? A[le] : 0.0;
     double d = akk;
-    for (int p = r0; p < le; ++p) {
-        const int m = col[p];
-        const int mr0 = rs[m], mle = lend[m];
-        const double amm = (mle < rs[m + 1] && col[mle] == m) ? A[mle] : 0.0;
-        if (amm == 0.0) continue;  // null column: never eliminated
-        const double Lkm = F[p];
-        // missing fill of column m with respect to row k's pattern
+    for (int j = 0; j < nl; ++j) {
+        const int m = col[r0 + j];
+        const int mle = lend[m];
+        if (!(mle < rs[m + 1] && col[mle] == m && A[mle] != 0.0)) continue;  // null column: never eliminated
+        // column m below the diagonal: its entries, then the missing fill with
+        // respect to row k's pattern, summed in ascending row order
+        const int a0 = ubeg[m], na = rs[m + 1] - a0;
+#pragma unroll
+        for (int q = 0; q < kMaxDeps; ++q)
+            if (q < na) pos[q] = a0 + q;
+        ok &= poll_all(F, pos, fv, na);
         double missing = 0.0;
-        int a = ubeg[m];
-        const int ae = rs[m + 1];
         int b = r0;
-        for (; a < ae; ++a) {
-            const int q = col[a];
-            if (q == k) continue;
-            while (b < r1 && col[b] < q) ++b;
-            if (b < r1 && col[b] == q) continue;
-            missing += F[a];
+        for (int q = 0; q < na; ++q) {
+            const int c = col[a0 + q];
+            if (c == k) continue;
+            while (b < r1 && col[b] < c) ++b;
+            if (b < r1 && col[b] == c) continue;
+            missing += fv[q];
         }
-        d -= Lkm * Lkm;
-        d -= tau * Lkm * missing;
-        (void)mr0;
+        d -= Lk[j] * Lk[j];
+        d -= tau * Lk[j] * missing;
     }
     double inv;
     if (akk == 0.0) {
@@ -181,297 +234,281 @@ __device__ inline void mic_row(const int k, const int r0, const int le, const in
     for (int p = ub; p < r1; ++p) {
         const int i = col[p];
         double v = A[p];
-        // subtract sum over m in lower(k) and lower(i), ascending
-        int a = r0, c = rs[i];
+        // common m in lower(k) and lower(i), ascending
+        int nc = 0, ja[kMaxDeps];
+        int a = 0, c = rs[i];
         const int ce = lend[i];
-        while (a < le && c < ce) {
-            const int ma = col[a], mc = col[c];
+        while (a < nl && c < ce) {
+            const int ma = col[r0 + a], mc = col[c];
             if (ma < mc) {
                 ++a;
             } else if (mc < ma) {
                 ++c;
             } else {
                 const int mr = lend[ma];
-                const double amm = (mr < rs[ma + 1] && col[mr] == ma) ? A[mr] : 0.0;
-                if (amm != 0.0) v -= F[a] * F[c];
+                if (mr < rs[ma + 1] && col[mr] == ma && A[mr] != 0.0 && nc < kMaxDeps) {
+                    ja[nc] = a;
+                    pos[nc] = c;
+                    ++nc;
+                }
                 ++a;
                 ++c;
             }
         }
+        ok &= poll_all(F, pos, fv, nc);
+        for (int q = 0; q < nc; ++q) v -= Lk[ja[q]] * fv[q];
         if (akk != 0.0) v *= inv;
-        F[p] = v;
-        F[tpos[p]] = v;
+        st_agent_u64(reinterpret_cast<unsigned long long*>(&F[p]), (unsigned long long)__double_as_longlong(v));
+        st_agent_u64(reinterpret_cast<unsigned long long*>(&F[tpos[p]]), (unsigned long long)__double_as_longlong(v));
     }
-    __hip_atomic_store(reinterpret_cast<unsigned long long*>(&invd[k]), (unsigned long long)__double_as_longlong(inv),
-                       __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    st_agent_u64(reinterpret_cast<unsigned long long*>(&invd[k]), (unsigned long long)__double_as_longlong(inv));
+    return ok;
 }
 
-__global__ void __launch_bounds__(BS) k_mic(int P, const int* order, LevelGate G, const int* rs, const int* lend,
-                                            const int* ubeg, const int* col, const double* A, const int* tpos,
-                                            double* F, double* invd, double tau, double ratio, int* cnt) {
-    __shared__ int chunk;
-    if (threadIdx.x == 0) chunk = atomicAdd(&cnt[C_TICKET_M], 1);
-    __syncthreads();
-    const int t = chunk * BS + threadIdx.x;
-    const int lev = (t >> 6) < G.nseg ? G.wlev[t >> 6] : -1;
-    gate_wait(G, lev - kGate);
-    const int k = t < P ? order[t] : -1;
-    if (k >= 0) {
-        const int r0 = rs[k], le = lend[k], ub = ubeg[k], r1 = rs[k + 1];
-        // Wait for all lower dependencies, then factor the row inside the same
-        // loop iteration.
-        int pw = r0, spins = 0;
-        for (;;) {
-            while (pw < le) {
-                unsigned long long bb = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(&invd[col[pw]]),
-                                                          __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-                if (bb == kSent64) break;
-                ++pw;
-            }
-            if (pw < le) {
-                if (++spins <= kSpinLimit) {
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
-                atomicOr(&cnt[C_ERR], E_MIC);
-            }
-            mic_row(k, r0, le, ub, r1, rs, lend, ubeg, col, A, tpos, F, invd, tau, ratio);
-            break;
-        }
+__global__ void __launch_bounds__(BS) k_mic(SweepPlan S, const int* rs, const int* lend, const int* ubeg,
+                                            const int* col, const double* A, const int* tpos, double* F,
+                                            double* invd, double tau, double ratio, int* cnt) {
+    for (int c = blockIdx.x; c < S.nchunks; c += gridDim.x) {
+        const int t = c * BS + threadIdx.x;
+        gate_row(invd, S.gateF[t >> 6]);
+        const int k = S.order[t];
+        if (k < 0) continue;
+        if (!mic_row(k, rs[k], lend[k], ubeg[k], rs[k + 1], rs, lend, ubeg, col, A, tpos, F, invd, tau, ratio))
+            atomicOr(&cnt[C_ERR], E_MIC);
     }
-    gate_arrive(G, lev);
 }
 
 // ---------------------------------------------------------------------------
-// PCG kernels (T = double or float; scalars are always double)
+// PCG kernels (T = double or float; scalars are always double). The loop is
+//   q = A s, sq = s.q                                   k_spmv_dot
+//   alpha = rho/sq; x += alpha s; r -= alpha q; |r|      k_axpy_res
+//   convergence test; y = L^-1 r                         k_fwd
+//   z = L^-T y, rho' = z.r                               k_bwd_dot
+//   s = z + (rho'/rho) s                                 k_xpby
+// with the reference's order of operations (pcg_solver.h:396-421). rho of
+// iteration it lives in the partials buffer rho[it & 1].
 // ---------------------------------------------------------------------------
+__device__ inline const double* rho_buf(const Parts& P, int parity) { return parity ? P.rho1 : P.rho0; }
+
 template <typename T>
-__global__ void __launch_bounds__(BS) k_init(int n, const T* b, T* x, T* r, T* y, T* z, double* part, int* cnt,
-                                             double* scal, int norm, double tolRel, int nblk, int* gateF, int* gateB,
-                                             int depth) {
-    const int i = blockIdx.x * BS + threadIdx.x;
-    for (int q = i; q < depth; q += gridDim.x * BS) {
-        gateF[q] = 0;
-        gateB[q] = 0;
-    }
+__global__ void __launch_bounds__(BS) k_init(int n, const T* b, T* x, T* r, T* y, T* z, Parts Pt, int norm) {
     double v = 0.0;
-    if (i < n) {
-        T bi = b[i];
+    for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
+        const T bi = b[i];
         x[i] = (T)0;
         r[i] = bi;
         Sentinel<T>::storeSent(&y[i]);
         Sentinel<T>::storeSent(&z[i]);
-        v = norm == ASTG_NORM_INF ? fabs((double)bi) : (double)bi * (double)bi;
+        const double a = (double)bi;
+        v = norm == ASTG_NORM_INF ? fmax(v, fabs(a)) : v + a * a;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        cnt[C_TICKET_F] = 0;
-        cnt[C_TICKET_B] = 0;
-    }
-    double tot;
-    bool last = norm == ASTG_NORM_INF ? reduce_partials<true>(v, part, blockIdx.x, nblk, &cnt[C_DONE_RES], &tot)
-                                      : reduce_partials<false>(v, part, blockIdx.x, nblk, &cnt[C_DONE_RES], &tot);
-    if (last && threadIdx.x == 0) {
-        double bn = norm == ASTG_NORM_INF ? tot : sqrt(tot);
-        scal[S_BNORM] = bn;
-        scal[S_RES] = bn;
-        scal[S_TOL] = tolRel * bn;
-        scal[S_ITER] = 0;
-        scal[S_DONE] = (bn == 0.0) ? 1.0 : 0.0;
-    }
+    const double t = norm == ASTG_NORM_INF ? block_reduce<true>(v) : block_reduce<false>(v);
+    if (threadIdx.x == 0) Pt.res[blockIdx.x] = t;
 }
 
+// Forward sweep y = L^-1 r. mode 0 (start of the solve): ||b|| from the
+// partials, tolerance, zero-rhs exit. mode 1 (iteration `it`): the reference's
+// convergence test on the updated residual, before preconditioning.
 template <typename T>
-__global__ void __launch_bounds__(BS) k_fwd(int P, const int* order, LevelGate G, int* gateOther, const int* rs,
-                                            const int* lend, const int* col, const T* F, const T* invd, const T* r,
-                                            T* y, T* z, int* cnt, double* scal) {
+__global__ void __launch_bounds__(BS) k_fwd(SweepPlan S, const int* rs, const int* lend, const int* col, const T* F,
+                                            const T* invd, const T* r, T* y, T* z, Parts Pt, int* cnt, double* scal,
+                                            int mode, int it, int maxIt, int norm, double tolRel) {
     if (scal[S_DONE] != 0.0) return;
-    __shared__ int chunk;
-    if (threadIdx.x == 0) chunk = atomicAdd(&cnt[C_TICKET_F], 1);
-    __syncthreads();
-    if (chunk == 0 && threadIdx.x == 0) cnt[C_TICKET_B] = 0;
-    for (int q = blockIdx.x * BS + threadIdx.x; q < G.depth; q += gridDim.x * BS) gateOther[q] = 0;
-    const int t = chunk * BS + threadIdx.x;
-    const int lev = (t >> 6) < G.nseg ? G.wlev[t >> 6] : -1;
-    gate_wait(G, lev - kGate);
-    const int i = t < P ? order[t] : -1;
-    if (i >= 0) {
-        T acc = r[i];
-        int p = rs[i];
-        const int pe = lend[i];
-        int spins = 0;
-        Sentinel<T>::storeSent(&z[i]);
-        for (;;) {
-            while (p < pe) {
-                auto bits = Sentinel<T>::load(&y[col[p]]);
-                if (bits == Sentinel<T>::value) break;
-                acc -= F[p] * Sentinel<T>::val(bits);
-                ++p;
+    {
+        double red = norm == ASTG_NORM_INF ? reduce_prev<true>(Pt.res, Pt.gs) : reduce_prev<false>(Pt.res, Pt.gs);
+        if (norm != ASTG_NORM_INF) red = sqrt(red);
+        bool stop = false;
+        if (mode == 0) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                scal[S_BNORM] = red;
+                scal[S_RES] = red;
+                scal[S_TOL] = tolRel * red;
+                scal[S_ITER] = 0;
+                if (red == 0.0) scal[S_DONE] = 1.0;
             }
-            if (p == pe) {
-                Sentinel<T>::store(&y[i], acc * invd[i]);
-                break;
+            stop = red == 0.0;
+        } else {
+            const bool conv = red <= scal[S_TOL];
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                scal[S_RES] = red;
+                scal[S_ITER] = (double)(it + 1);
+                if (conv) scal[S_DONE] = 1.0;
+                else if (it + 1 >= maxIt) scal[S_DONE] = 3.0;
             }
-            if (++spins > kSpinLimit) {
+            stop = conv || it + 1 >= maxIt;
+        }
+        if (stop) return;
+    }
+    for (int c = blockIdx.x; c < S.nchunks; c += gridDim.x) {
+        const int t = c * BS + threadIdx.x;
+        const int i = S.order[t];
+        // row data that does not depend on other rows is loaded before the gate
+        DepRow<T> d;
+        T acc = 0, inv = 0;
+        int p0 = 0, pe = 0;
+        if (i >= 0) {
+            p0 = rs[i];
+            pe = lend[i];
+            acc = r[i];
+            inv = invd[i];
+            if (pe - p0 <= kMaxDeps) d.preload(col, F, p0, pe);
+            Sentinel<T>::storeSent(&z[i]);
+        }
+        gate_row(y, S.gateF[t >> 6]);
+        if (i >= 0) {
+            const bool ok = (pe - p0 <= kMaxDeps) ? d.apply(y, acc) : apply_deps_slow(col, F, p0, pe, y, acc);
+            if (!ok) {
                 atomicOr(&cnt[C_ERR], E_FWD);
-                Sentinel<T>::store(&y[i], (T)NAN);
-                break;
+                acc = (T)NAN;
             }
-            __builtin_amdgcn_s_sleep(1);
+            Sentinel<T>::store(&y[i], acc * inv);
         }
     }
-    gate_arrive(G, lev);
 }
 
-// Backward sweep z = L^-T y fused with the partial dot z.r. Partials are
-// indexed by chunk so the reduction order is fixed.
+// Backward sweep z = L^-T y in reverse level order, fused with the partial dot
+// z.r (one partial per block, fixed chunk assignment: deterministic).
 template <typename T>
-__global__ void __launch_bounds__(BS) k_bwd_dot(int P, const int* order, LevelGate G, const int* rs, const int* ubeg,
-                                                const int* col, const T* F, const T* invd, const T* y, const T* r, T* z,
-                                                double* part, int* cnt, double* scal, int init, int nblk) {
+__global__ void __launch_bounds__(BS) k_bwd_dot(SweepPlan S, const int* rs, const int* ubeg, const int* col,
+                                                const T* F, const T* invd, const T* y, const T* r, T* z, Parts Pt,
+                                                int* cnt, const double* scal, int parityOut) {
     if (scal[S_DONE] != 0.0) return;
-    __shared__ int chunk;
-    if (threadIdx.x == 0) chunk = atomicAdd(&cnt[C_TICKET_B], 1);
-    __syncthreads();
-    const int t = chunk * BS + threadIdx.x;
-    const int pos = P - 1 - t;  // reverse level order
-    const int lev = (pos >= 0 && (pos >> 6) < G.nseg) ? G.wlev[pos >> 6] : -1;
-    if (lev >= 0) gate_wait(G, lev + kGate);
     double v = 0.0;
-    const int k = pos >= 0 ? order[pos] : -1;
-    if (k >= 0) {
-        T acc = y[k];
-        int p = ubeg[k];
-        const int pe = rs[k + 1];
-        int spins = 0;
-        T zk;
-        for (;;) {
-            while (p < pe) {
-                auto bits = Sentinel<T>::load(&z[col[p]]);
-                if (bits == Sentinel<T>::value) break;
-                acc -= F[p] * Sentinel<T>::val(bits);
-                ++p;
-            }
-            if (p == pe) {
-                zk = acc * invd[k];
-                Sentinel<T>::store(&z[k], zk);
-                break;
-            }
-            if (++spins > kSpinLimit) {
-                atomicOr(&cnt[C_ERR], E_BWD);
-                zk = (T)NAN;
-                Sentinel<T>::store(&z[k], zk);
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
+    for (int c = blockIdx.x; c < S.nchunks; c += gridDim.x) {
+        const int pos = S.P - 1 - (c * BS + threadIdx.x);
+        const int k = S.order[pos];
+        DepRow<T> d;
+        T acc = 0, inv = 0, rk = 0;
+        int p0 = 0, pe = 0;
+        if (k >= 0) {
+            p0 = ubeg[k];
+            pe = rs[k + 1];
+            acc = y[k];
+            inv = invd[k];
+            rk = r[k];
+            if (pe - p0 <= kMaxDeps) d.preload(col, F, p0, pe);
         }
-        v = (double)zk * (double)r[k];
+        gate_row(z, S.gateB[pos >> 6]);
+        if (k >= 0) {
+            const bool ok = (pe - p0 <= kMaxDeps) ? d.apply(z, acc) : apply_deps_slow(col, F, p0, pe, z, acc);
+            if (!ok) {
+                atomicOr(&cnt[C_ERR], E_BWD);
+                acc = (T)NAN;
+            }
+            const T zk = acc * inv;
+            Sentinel<T>::store(&z[k], zk);
+            v += (double)zk * (double)rk;
+        }
     }
-    gate_arrive(G, lev);
-    double tot;
-    if (reduce_partials<false>(v, part, chunk, nblk, &cnt[C_DONE_RHO], &tot) && threadIdx.x == 0) {
-        if (init) {
-            scal[S_RHO] = tot;
-            if (tot == 0.0 || tot != tot) {
-                scal[S_DONE] = 2.0;  // breakdown before the first iteration (reference returns false)
+    const double t = block_reduce<false>(v);
+    if (threadIdx.x == 0) (parityOut ? Pt.rho1 : Pt.rho0)[blockIdx.x] = t;
+}
+
+// s = z + beta s with beta = rho(it+1) / rho(it); mode 0: s = z after the first
+// preconditioning, with the reference's rho == 0 / NaN breakdown test.
+template <typename T>
+__global__ void __launch_bounds__(BS) k_xpby(int n, const T* z, T* s, Parts Pt, double* scal, int mode, int it) {
+    if (scal[S_DONE] != 0.0) return;
+    if (mode == 0) {
+        const double rho = reduce_prev<false>(Pt.rho0, Pt.gb);
+        if (rho == 0.0 || rho != rho) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                scal[S_DONE] = 2.0;
                 scal[S_ITER] = 0;
             }
-        } else {
-            scal[S_BETA] = tot / scal[S_RHO];
-            scal[S_RHO] = tot;
+            return;
         }
+        for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) s[i] = z[i];
+        return;
     }
+    const double rnew = reduce_prev<false>(rho_buf(Pt, (it + 1) & 1), Pt.gb);
+    const double rold = reduce_prev<false>(rho_buf(Pt, it & 1), Pt.gb);
+    const T beta = (T)(rnew / rold);
+    for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) s[i] = z[i] + beta * s[i];
 }
 
+// q = A s with kLpr lanes per row (rows are short: 8 entries on average), so a
+// wavefront reads the entries of 64/kLpr consecutive rows contiguously; the
+// row sum is a shuffle reduction over the row's lanes. Fused with s.q.
 template <typename T>
-__global__ void __launch_bounds__(BS) k_xpby(int n, const T* z, T* s, double* scal, int init) {
-    if (scal[S_DONE] != 0.0) return;
-    const int i = blockIdx.x * BS + threadIdx.x;
-    if (i >= n) return;
-    if (init) {
-        s[i] = z[i];
-    } else {
-        const T beta = (T)scal[S_BETA];
-        s[i] = z[i] + beta * s[i];
+__device__ inline T row_dot(const int* rs, const int* col, const T* A, const T* s, int i, int sub) {
+    T acc = (T)0;
+    if (i >= 0) {
+        for (int p = rs[i] + sub, pe = rs[i + 1]; p < pe; p += kLpr) acc += A[p] * s[col[p]];
     }
+#pragma unroll
+    for (int o = kLpr / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kLpr);
+    return acc;
 }
 
 template <typename T>
 __global__ void __launch_bounds__(BS) k_spmv_dot(int n, const int* rs, const int* col, const T* A, const T* s, T* q,
-                                                 double* part, int* cnt, double* scal, int nblk) {
+                                                 Parts Pt, const double* scal) {
     if (scal[S_DONE] != 0.0) return;
-    const int i = blockIdx.x * BS + threadIdx.x;
+    const int sub = threadIdx.x & (kLpr - 1);
     double v = 0.0;
-    if (i < n) {
-        T acc = (T)0;
-        for (int p = rs[i]; p < rs[i + 1]; ++p) acc += A[p] * s[col[p]];
-        q[i] = acc;
-        v = (double)s[i] * (double)acc;
+    const int rowsPerIter = gridDim.x * (BS / kLpr);
+    for (int base = blockIdx.x * (BS / kLpr); base < n; base += rowsPerIter) {
+        const int ii = base + threadIdx.x / kLpr;
+        const int i = ii < n ? ii : -1;
+        const T acc = row_dot(rs, col, A, s, i, sub);
+        if (i >= 0 && sub == 0) {
+            q[i] = acc;
+            v += (double)s[i] * (double)acc;
+        }
     }
-    double tot;
-    if (reduce_partials<false>(v, part, blockIdx.x, nblk, &cnt[C_DONE_SQ], &tot) && threadIdx.x == 0) {
-        scal[S_SQ] = tot;
-        scal[S_ALPHA] = scal[S_RHO] / tot;
-    }
+    const double t = block_reduce<false>(v);
+    if (threadIdx.x == 0) Pt.sq[blockIdx.x] = t;
 }
 
 template <typename T>
-__global__ void __launch_bounds__(BS) k_axpy_res(int n, const T* s, const T* q, T* x, T* r, T* y, double* part, int* cnt,
-                                                 double* scal, int norm, int maxIt, int nblk, int* gateF, int depth) {
+__global__ void __launch_bounds__(BS) k_axpy_res(int n, const T* s, const T* q, T* x, T* r, T* y, Parts Pt,
+                                                 const double* scal, int norm, int it) {
     if (scal[S_DONE] != 0.0) return;
-    const int i = blockIdx.x * BS + threadIdx.x;
-    for (int g = i; g < depth; g += gridDim.x * BS) gateF[g] = 0;
-    if (blockIdx.x == 0 && threadIdx.x == 0) cnt[C_TICKET_F] = 0;
+    const double sq = reduce_prev<false>(Pt.sq, Pt.gs);
+    const double rho = reduce_prev<false>(rho_buf(Pt, it & 1), Pt.gb);
+    const T alpha = (T)(rho / sq);
     double v = 0.0;
-    if (i < n) {
-        const T alpha = (T)scal[S_ALPHA];
+    for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
         x[i] += alpha * s[i];
-        T ri = r[i] - alpha * q[i];
+        const T ri = r[i] - alpha * q[i];
         r[i] = ri;
         Sentinel<T>::storeSent(&y[i]);
-        v = norm == ASTG_NORM_INF ? fabs((double)ri) : (double)ri * (double)ri;
+        const double a = (double)ri;
+        v = norm == ASTG_NORM_INF ? fmax(v, fabs(a)) : v + a * a;
     }
-    double tot;
-    bool last = norm == ASTG_NORM_INF ? reduce_partials<true>(v, part, blockIdx.x, nblk, &cnt[C_DONE_RES], &tot)
-                                      : reduce_partials<false>(v, part, blockIdx.x, nblk, &cnt[C_DONE_RES], &tot);
-    if (last && threadIdx.x == 0) {
-        double res = norm == ASTG_NORM_INF ? tot : sqrt(tot);
-        scal[S_RES] = res;
-        double it = scal[S_ITER] + 1.0;
-        scal[S_ITER] = it;
-        if (res <= scal[S_TOL]) scal[S_DONE] = 1.0;
-        else if (it >= (double)maxIt) scal[S_DONE] = 3.0;
-    }
+    const double t = norm == ASTG_NORM_INF ? block_reduce<true>(v) : block_reduce<false>(v);
+    if (threadIdx.x == 0) Pt.res[blockIdx.x] = t;
 }
 
-// Mixed form: r64 = b - A x (FP64) with max|r| / sum r^2.
+// Mixed form: r64 = b - A x (FP64), |r| partials.
 __global__ void __launch_bounds__(BS) k_residual64(int n, const int* rs, const int* col, const double* A,
-                                                   const double* x, const double* b, double* r, double* part,
-                                                   int* cnt, double* scal, int norm, int nblk) {
-    const int i = blockIdx.x * BS + threadIdx.x;
+                                                   const double* x, const double* b, double* r, Parts Pt, int norm) {
+    const int sub = threadIdx.x & (kLpr - 1);
     double v = 0.0;
-    if (i < n) {
-        double acc = 0.0;
-        for (int p = rs[i]; p < rs[i + 1]; ++p) acc += A[p] * x[col[p]];
-        double ri = b[i] - acc;
-        r[i] = ri;
-        v = norm == ASTG_NORM_INF ? fabs(ri) : ri * ri;
+    const int rowsPerIter = gridDim.x * (BS / kLpr);
+    for (int base = blockIdx.x * (BS / kLpr); base < n; base += rowsPerIter) {
+        const int ii = base + threadIdx.x / kLpr;
+        const int i = ii < n ? ii : -1;
+        const double acc = row_dot(rs, col, A, x, i, sub);
+        if (i >= 0 && sub == 0) {
+            const double ri = b[i] - acc;
+            r[i] = ri;
+            v = norm == ASTG_NORM_INF ? fmax(v, fabs(ri)) : v + ri * ri;
+        }
     }
-    double tot;
-    bool last = norm == ASTG_NORM_INF ? reduce_partials<true>(v, part, blockIdx.x, nblk, &cnt[C_DONE_RES], &tot)
-                                      : reduce_partials<false>(v, part, blockIdx.x, nblk, &cnt[C_DONE_RES], &tot);
-    if (last && threadIdx.x == 0) scal[S_RES + 8] = norm == ASTG_NORM_INF ? tot : sqrt(tot);
+    const double t = norm == ASTG_NORM_INF ? block_reduce<true>(v) : block_reduce<false>(v);
+    if (threadIdx.x == 0) Pt.res[blockIdx.x] = t;
+}
+__global__ void k_res_final(Parts Pt, double* scal, int norm) {
+    const double t = norm == ASTG_NORM_INF ? reduce_prev<true>(Pt.res, Pt.gs) : reduce_prev<false>(Pt.res, Pt.gs);
+    if (threadIdx.x == 0) scal[S_OUTER_RES] = norm == ASTG_NORM_INF ? t : sqrt(t);
 }
 __global__ void k_scale_f32(int n, const double* r, float* o, double inv) {
-    const int i = blockIdx.x * BS + threadIdx.x;
-    if (i < n) o[i] = (float)(r[i] * inv);
+    for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) o[i] = (float)(r[i] * inv);
 }
 __global__ void k_add_corr(int n, double* x, const float* d, double scale) {
-    const int i = blockIdx.x * BS + threadIdx.x;
-    if (i < n) x[i] += scale * (double)d[i];
-}
-__global__ void k_zero64(int n, double* x) {
-    const int i = blockIdx.x * BS + threadIdx.x;
-    if (i < n) x[i] = 0.0;
+    for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) x[i] += scale * (double)d[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -482,6 +519,19 @@ PcgDevice::~PcgDevice() {
 }
 
 static inline int blocks(long n) { return (int)((n + BS - 1) / BS); }
+
+template <typename K>
+static int coopBlocks(K kernel, int cus, int want) {
+    int per = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, BS, 0) != hipSuccess || per < 1) per = 1;
+    return std::max(1, std::min(want, per * cus));
+}
+
+template <typename K, typename... Args>
+static hipError_t launchCoop(K kernel, int grid, hipStream_t s, Args... args) {
+    void* ptrs[] = {(void*)&args...};
+    return hipLaunchCooperativeKernel((const void*)kernel, dim3(grid), dim3(BS), ptrs, 0, s);
+}
 
 int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::string& err) {
     if (n <= 0) {
@@ -523,7 +573,18 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
             }
             tpos[p] = (int)(f - col);
         }
-    nblk_ = blocks(n);
+    for (int i = 0; i < n; ++i)
+        if (lend[i] - rowstart[i] > kMaxDeps || rowstart[i + 1] - ubeg[i] > kMaxDeps) {
+            err = "row " + std::to_string(i) + " has more than " + std::to_string(kMaxDeps) +
+                  " entries on one side of the diagonal (unsupported)";
+            return ASTG_ERR_ARG;
+        }
+    if (const char* g = getenv("ASTG_GATE_DIST")) gateDist_ = std::max(1, atoi(g));
+    int dev = 0, cus = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    cus_ = cus;
+    gs_ = std::max(1, std::min(2 * cus, blocks(n)));
     const size_t n8 = (size_t)n * 8, n4 = (size_t)n * 4;
     HIPCHK(rs_.alloc(((size_t)n + 1) * 4));
     HIPCHK(col_.alloc((size_t)nnz_ * 4));
@@ -544,20 +605,18 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
     HIPCHK(y_.alloc(n8));
     HIPCHK(scal_.alloc(32 * 8));
     HIPCHK(cnt_.alloc(C_COUNT * 4));
-    HIPCHK(part_.alloc((size_t)(nblk_ + 1) * 8));
     if (!hscal_) HIPCHK(hipHostMalloc((void**)&hscal_, 32 * 8, hipHostMallocDefault));
     HIPCHK(hipMemcpyAsync(rs_.p, rowstart, ((size_t)n + 1) * 4, hipMemcpyHostToDevice, stream_));
     HIPCHK(hipMemcpyAsync(col_.p, col, (size_t)nnz_ * 4, hipMemcpyHostToDevice, stream_));
     HIPCHK(hipMemcpyAsync(lend_.p, lend.data(), n4, hipMemcpyHostToDevice, stream_));
     HIPCHK(hipMemcpyAsync(ubeg_.p, ubeg.data(), n4, hipMemcpyHostToDevice, stream_));
     HIPCHK(hipMemcpyAsync(tpos_.p, tpos.data(), (size_t)nnz_ * 4, hipMemcpyHostToDevice, stream_));
-    // zero scalars/counters with kernels (no memset nodes)
     k_zero_counters<<<1, 64, 0, stream_>>>(cnt_.as<int>());
     k_zero64<<<1, BS, 0, stream_>>>(32, scal_.as<double>());
     // dependency levels on the device
     k_fill_i32<<<blocks(n), BS, 0, stream_>>>(lev_.as<int>(), kSentLevel, n);
-    k_levels<<<(n + BS / 64 - 1) / (BS / 64), BS, 0, stream_>>>(n, rs_.as<int>(), lend_.as<int>(), col_.as<int>(), lev_.as<int>(),
-                                            cnt_.as<int>(), scal_.as<double>());
+    k_levels<<<(n + BS / 64 - 1) / (BS / 64), BS, 0, stream_>>>(n, rs_.as<int>(), lend_.as<int>(), col_.as<int>(),
+                                                                lev_.as<int>(), cnt_.as<int>());
     HIPCHK(hipGetLastError());
     // stable sort of (level, row): rows in level order, ties by index
     DevBuf keysOut, idxIn, tmp;
@@ -583,52 +642,76 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
     depth_ = sortedLev[n - 1] + 1;
     // Pad every level to a multiple of 64 rows so that a wavefront never holds
     // two rows of different levels (rows of one level are independent).
-    std::vector<int> padded;
+    std::vector<int> padded, levStart(depth_, 0), levRows(depth_, 0);
     padded.reserve((size_t)n + (size_t)depth_ * 64 + BS);
     for (int i = 0; i < n;) {
         int j = i;
-        while (j < n && sortedLev[j] == sortedLev[i]) padded.push_back(sortedRow[j++]);
+        const int L = sortedLev[i];
+        levStart[L] = (int)padded.size();
+        while (j < n && sortedLev[j] == L) padded.push_back(sortedRow[j++]);
+        levRows[L] = j - i;
         while (padded.size() % 64) padded.push_back(-1);
         i = j;
     }
     while (padded.size() % BS) padded.push_back(-1);
     P_ = (int)padded.size();
-    nblkS_ = P_ / BS;
+    // gate rows: for each 64-row segment of level L, one row of level L-dist
+    // (forward) and one of level L+dist (backward), spread over the level
     {
-        std::vector<int> rowLev(n);
-        for (int q = 0; q < n; ++q) rowLev[sortedRow[q]] = sortedLev[q];
         const int nseg = P_ / 64;
-        std::vector<int> wlev(nseg, -1), wcnt(depth_, 0);
-        for (int sgi = 0; sgi < nseg; ++sgi) {
-            int r = padded[(size_t)sgi * 64];
-            if (r >= 0) {
-                wlev[sgi] = rowLev[r];
-                wcnt[rowLev[r]] += 1;
-            }
+        std::vector<int> gF(nseg, -1), gB(nseg, -1), segLev(nseg, -1);
+        for (int L = 0; L < depth_; ++L) {
+            const int s0 = levStart[L] / 64, s1 = (levStart[L] + levRows[L] + 63) / 64;
+            for (int sg = s0; sg < s1; ++sg) segLev[sg] = L;
         }
-        HIPCHK(wlev_.alloc((size_t)nseg * 4));
-        HIPCHK(wcnt_.alloc((size_t)depth_ * 4));
-        HIPCHK(gateF_.alloc((size_t)depth_ * 4));
-        HIPCHK(gateB_.alloc((size_t)depth_ * 4));
-        HIPCHK(gateM_.alloc((size_t)depth_ * 4));
-        HIPCHK(hipMemcpyAsync(wlev_.p, wlev.data(), (size_t)nseg * 4, hipMemcpyHostToDevice, stream_));
-        HIPCHK(hipMemcpyAsync(wcnt_.p, wcnt.data(), (size_t)depth_ * 4, hipMemcpyHostToDevice, stream_));
+        for (int sg = 0; sg < nseg; ++sg) {
+            const int L = segLev[sg];
+            if (L < 0) continue;
+            const unsigned h = (unsigned)sg * 2654435761u;
+            const int lf = L - gateDist_, lb = L + gateDist_;
+            if (lf >= 0) gF[sg] = padded[(size_t)levStart[lf] + h % (unsigned)levRows[lf]];
+            if (lb < depth_) gB[sg] = padded[(size_t)levStart[lb] + h % (unsigned)levRows[lb]];
+        }
+        HIPCHK(gateF_.alloc((size_t)nseg * 4));
+        HIPCHK(gateB_.alloc((size_t)nseg * 4));
+        HIPCHK(hipMemcpyAsync(gateF_.p, gF.data(), (size_t)nseg * 4, hipMemcpyHostToDevice, stream_));
+        HIPCHK(hipMemcpyAsync(gateB_.p, gB.data(), (size_t)nseg * 4, hipMemcpyHostToDevice, stream_));
     }
     HIPCHK(order_.alloc((size_t)P_ * 4));
-    HIPCHK(part_.alloc((size_t)(std::max(nblk_, nblkS_) + 1) * 8));
     HIPCHK(hipMemcpyAsync(order_.p, padded.data(), (size_t)P_ * 4, hipMemcpyHostToDevice, stream_));
+    // persistent grids (all blocks co-resident)
+    const int nchunks = P_ / BS;
+    gF64_ = coopBlocks(k_fwd<double>, cus, nchunks);
+    gB64_ = coopBlocks(k_bwd_dot<double>, cus, nchunks);
+    gF32_ = coopBlocks(k_fwd<float>, cus, nchunks);
+    gB32_ = coopBlocks(k_bwd_dot<float>, cus, nchunks);
+    gM_ = coopBlocks(k_mic, cus, nchunks);
+    gb_ = std::max(std::max(gB64_, gB32_), 1);
+    HIPCHK(part_.alloc((size_t)(2 * gs_ + 2 * gb_ + 8) * 8));
     HIPCHK(hipStreamSynchronize(stream_));
     return ASTG_OK;
 }
 
-LevelGate PcgDevice::gate(DevBuf& counters) const {
-    LevelGate g;
-    g.wlev = wlev_.as<int>();
-    g.wcnt = wcnt_.as<int>();
-    g.done = counters.as<int>();
-    g.nseg = P_ / 64;
-    g.depth = depth_;
-    return g;
+SweepPlan PcgDevice::plan() const {
+    SweepPlan s;
+    s.order = order_.as<int>();
+    s.gateF = gateF_.as<int>();
+    s.gateB = gateB_.as<int>();
+    s.P = P_;
+    s.nchunks = P_ / BS;
+    return s;
+}
+
+Parts PcgDevice::parts(int gb) const {
+    Parts p;
+    double* base = part_.as<double>();
+    p.sq = base;
+    p.res = base + gs_;
+    p.rho0 = base + 2 * gs_;
+    p.rho1 = base + 2 * gs_ + gb_;
+    p.gs = gs_;
+    p.gb = gb;
+    return p;
 }
 
 int PcgDevice::checkErrors(const char* where, std::string& err) {
@@ -642,15 +725,19 @@ int PcgDevice::checkErrors(const char* where, std::string& err) {
     return ASTG_OK;
 }
 
+int PcgDevice::launchMic(double tau, double ratio, std::string& err) {
+    k_fill_sent<double><<<blocks(n_), BS, 0, stream_>>>(invd_.as<double>(), n_);
+    k_fill_sent<double><<<blocks(nnz_), BS, 0, stream_>>>(F_.as<double>(), (int)nnz_);
+    HIPCHK(launchCoop(k_mic, gM_, stream_, plan(), (const int*)rs_.as<int>(), (const int*)lend_.as<int>(),
+                      (const int*)ubeg_.as<int>(), (const int*)col_.as<int>(), (const double*)val_.as<double>(),
+                      (const int*)tpos_.as<int>(), F_.as<double>(), invd_.as<double>(), tau, ratio, cnt_.as<int>()));
+    return ASTG_OK;
+}
+
 int PcgDevice::factor(double tau, double ratio, std::string& err) {
-    const int n = n_;
     k_zero_counters<<<1, 64, 0, stream_>>>(cnt_.as<int>());
-    k_zero64<<<1, BS, 0, stream_>>>(32, scal_.as<double>());
-    k_fill_sent<double><<<blocks(n), BS, 0, stream_>>>(invd_.as<double>(), n);
-    k_fill_i32<<<blocks(depth_), BS, 0, stream_>>>(gateM_.as<int>(), 0, depth_);
-    k_mic<<<nblkS_, BS, 0, stream_>>>(P_, order_.as<int>(), gate(gateM_), rs_.as<int>(), lend_.as<int>(),
-                                      ubeg_.as<int>(), col_.as<int>(), val_.as<double>(), tpos_.as<int>(),
-                                      F_.as<double>(), invd_.as<double>(), tau, ratio, cnt_.as<int>());
+    int rc = launchMic(tau, ratio, err);
+    if (rc) return rc;
     HIPCHK(hipGetLastError());
     return checkErrors("MIC(0) factorisation", err);
 }
@@ -662,9 +749,30 @@ int PcgDevice::syncScalars(std::string& err) {
 }
 
 template <typename T>
+int PcgDevice::launchFwd(const T* F, const T* invd, const T* r, T* y, T* z, int mode, int it, int maxIt, int norm,
+                         double tolRel, std::string& err) {
+    const int g = sizeof(T) == 8 ? gF64_ : gF32_;
+    const int gb = sizeof(T) == 8 ? gB64_ : gB32_;
+    HIPCHK(launchCoop(k_fwd<T>, g, stream_, plan(), (const int*)rs_.as<int>(), (const int*)lend_.as<int>(),
+                      (const int*)col_.as<int>(), F, invd, r, y, z, parts(gb), cnt_.as<int>(), scal_.as<double>(),
+                      mode, it, maxIt, norm, tolRel));
+    return ASTG_OK;
+}
+
+template <typename T>
+int PcgDevice::launchBwd(const T* F, const T* invd, const T* y, const T* r, T* z, int parityOut, std::string& err) {
+    const int gb = sizeof(T) == 8 ? gB64_ : gB32_;
+    HIPCHK(launchCoop(k_bwd_dot<T>, gb, stream_, plan(), (const int*)rs_.as<int>(), (const int*)ubeg_.as<int>(),
+                      (const int*)col_.as<int>(), F, invd, y, r, z, parts(gb), cnt_.as<int>(),
+                      (const double*)scal_.as<double>(), parityOut));
+    return ASTG_OK;
+}
+
+template <typename T>
 int PcgDevice::pcgInner(const T* A, const T* F, const T* invd, const T* b, T* x, double tolRel, int maxIt, int norm,
                         int& iters, double& resOut, bool& conv, std::string& err) {
-    const int n = n_, nb = nblk_, P = P_, nbs = nblkS_;
+    const int n = n_, gs = gs_;
+    const int gb = sizeof(T) == 8 ? gB64_ : gB32_;
     T* r = r_.as<T>();
     T* z = z_.as<T>();
     T* s = s_.as<T>();
@@ -672,39 +780,31 @@ int PcgDevice::pcgInner(const T* A, const T* F, const T* invd, const T* b, T* x,
     T* y = y_.as<T>();
     const int* rs = rs_.as<int>();
     const int* col = col_.as<int>();
-    const int* lend = lend_.as<int>();
-    const int* ubeg = ubeg_.as<int>();
-    const int* order = order_.as<int>();
-    double* part = part_.as<double>();
-    int* cnt = cnt_.as<int>();
     double* scal = scal_.as<double>();
-    const LevelGate gF = gate(gateF_), gB = gate(gateB_);
-    const int D = depth_;
-    k_init<T><<<nb, BS, 0, stream_>>>(n, b, x, r, y, z, part, cnt, scal, norm, tolRel, nb, gateF_.as<int>(),
-                                      gateB_.as<int>(), D);
-    k_fwd<T><<<nbs, BS, 0, stream_>>>(P, order, gF, gateB_.as<int>(), rs, lend, col, F, invd, r, y, z, cnt, scal);
-    k_bwd_dot<T><<<nbs, BS, 0, stream_>>>(P, order, gB, rs, ubeg, col, F, invd, y, r, z, part, cnt, scal, 1, nbs);
-    k_xpby<T><<<nb, BS, 0, stream_>>>(n, z, s, scal, 1);
+    const Parts Pt = parts(gb);
+    int rc;
+    k_zero64<<<1, BS, 0, stream_>>>(32, scal);
+    k_init<T><<<gs, BS, 0, stream_>>>(n, b, x, r, y, z, Pt, norm);
+    if ((rc = launchFwd<T>(F, invd, r, y, z, 0, 0, maxIt, norm, tolRel, err))) return rc;
+    if ((rc = launchBwd<T>(F, invd, y, r, z, 0, err))) return rc;
+    k_xpby<T><<<gs, BS, 0, stream_>>>(n, z, s, Pt, scal, 0, 0);
     HIPCHK(hipGetLastError());
-    int enq = 0;
+    int it = 0;
     int batch = 4;
     for (;;) {
-        for (int j = 0; j < batch && enq < maxIt; ++j, ++enq) {
-            k_spmv_dot<T><<<nb, BS, 0, stream_>>>(n, rs, col, A, s, q, part, cnt, scal, nb);
-            k_axpy_res<T><<<nb, BS, 0, stream_>>>(n, s, q, x, r, y, part, cnt, scal, norm, maxIt, nb,
-                                                  gateF_.as<int>(), D);
-            k_fwd<T><<<nbs, BS, 0, stream_>>>(P, order, gF, gateB_.as<int>(), rs, lend, col, F, invd, r, y, z, cnt,
-                                              scal);
-            k_bwd_dot<T><<<nbs, BS, 0, stream_>>>(P, order, gB, rs, ubeg, col, F, invd, y, r, z, part, cnt, scal, 0,
-                                                  nbs);
-            k_xpby<T><<<nb, BS, 0, stream_>>>(n, z, s, scal, 0);
+        for (int j = 0; j < batch && it < maxIt; ++j, ++it) {
+            k_spmv_dot<T><<<gs, BS, 0, stream_>>>(n, rs, col, A, s, q, Pt, scal);
+            k_axpy_res<T><<<gs, BS, 0, stream_>>>(n, s, q, x, r, y, Pt, scal, norm, it);
+            if ((rc = launchFwd<T>(F, invd, r, y, z, 1, it, maxIt, norm, tolRel, err))) return rc;
+            if ((rc = launchBwd<T>(F, invd, y, r, z, (it + 1) & 1, err))) return rc;
+            k_xpby<T><<<gs, BS, 0, stream_>>>(n, z, s, Pt, scal, 1, it);
         }
         HIPCHK(hipGetLastError());
-        int rc = syncScalars(err);
+        rc = syncScalars(err);
         if (rc) return rc;
         rc = checkErrors("PCG", err);
         if (rc) return rc;
-        if (hscal_[S_DONE] != 0.0 || enq >= maxIt) break;
+        if (hscal_[S_DONE] != 0.0 || it >= maxIt) break;
         batch = std::min(batch * 2, 32);
     }
     iters = (int)hscal_[S_ITER];
@@ -718,7 +818,7 @@ int PcgDevice::pcgInner(const T* A, const T* F, const T* invd, const T* b, T* x,
 }
 
 int PcgDevice::solve(const astg_pcg_params& prm, astg_pcg_result& res, std::string& err) {
-    const int n = n_, nb = nblk_;
+    const int n = n_, gs = gs_;
     res.n = n;
     res.nnz = nnz_;
     res.depth = depth_;
@@ -747,7 +847,7 @@ int PcgDevice::solve(const astg_pcg_params& prm, astg_pcg_result& res, std::stri
     k_d2f<<<blocks(nnz), BS, 0, stream_>>>(val_.as<double>(), val32_.as<float>(), nnz);
     k_d2f<<<blocks(nnz), BS, 0, stream_>>>(F_.as<double>(), F32_.as<float>(), nnz);
     k_d2f<<<blocks(n), BS, 0, stream_>>>(invd_.as<double>(), invd32_.as<float>(), n);
-    k_zero64<<<nb, BS, 0, stream_>>>(n, x_.as<double>());
+    k_zero64<<<blocks(n), BS, 0, stream_>>>(n, x_.as<double>());
     float* rhs32 = reinterpret_cast<float*>(d64_.p);
     float* d32 = reinterpret_cast<float*>(d64_.as<char>() + (size_t)n * 4);
     double bnorm = -1, target = 0;
@@ -756,12 +856,13 @@ int PcgDevice::solve(const astg_pcg_params& prm, astg_pcg_result& res, std::stri
     double rnorm = 0;
     bool conv = false;
     for (;;) {
-        k_residual64<<<nb, BS, 0, stream_>>>(n, rs_.as<int>(), col_.as<int>(), val_.as<double>(), x_.as<double>(),
-                                             b_.as<double>(), r64_.as<double>(), part_.as<double>(), cnt_.as<int>(),
-                                             scal_.as<double>(), prm.norm, nb);
+        const Parts Pt = parts(gB64_);
+        k_residual64<<<gs, BS, 0, stream_>>>(n, rs_.as<int>(), col_.as<int>(), val_.as<double>(), x_.as<double>(),
+                                             b_.as<double>(), r64_.as<double>(), Pt, prm.norm);
+        k_res_final<<<1, BS, 0, stream_>>>(Pt, scal_.as<double>(), prm.norm);
         int rc = syncScalars(err);
         if (rc) return rc;
-        rnorm = hscal_[S_RES + 8];
+        rnorm = hscal_[S_OUTER_RES];
         if (bnorm < 0) {
             bnorm = rnorm;
             target = prm.tol * bnorm;
@@ -772,8 +873,8 @@ int PcgDevice::solve(const astg_pcg_params& prm, astg_pcg_result& res, std::stri
         }
         if (outer >= maxOuter || total >= prm.max_iterations) break;
         // inner relative tolerance: aim just below the outer target, not below FP32 reach
-        double eta = std::max(0.5 * target / rnorm, 2e-5);
-        k_scale_f32<<<nb, BS, 0, stream_>>>(n, r64_.as<double>(), rhs32, 1.0 / rnorm);
+        const double eta = std::max(0.5 * target / rnorm, 2e-5);
+        k_scale_f32<<<gs, BS, 0, stream_>>>(n, r64_.as<double>(), rhs32, 1.0 / rnorm);
         int it = 0;
         double ir = 0;
         bool ic = false;
@@ -782,7 +883,7 @@ int PcgDevice::solve(const astg_pcg_params& prm, astg_pcg_result& res, std::stri
         if (rc) return rc;
         total += it;
         ++outer;
-        k_add_corr<<<nb, BS, 0, stream_>>>(n, x_.as<double>(), d32, rnorm);
+        k_add_corr<<<gs, BS, 0, stream_>>>(n, x_.as<double>(), d32, rnorm);
         if (it == 0) break;
     }
     res.iterations = total;
@@ -820,15 +921,10 @@ int PcgDevice::downloadFactor(std::vector<double>& invdiag, std::vector<unsigned
 }
 
 int PcgDevice::benchKernel(int which, int reps, double& avgMs, double& bytes, std::string& err) {
-    const int n = n_, nb = nblk_, P = P_, nbs = nblkS_;
+    const int n = n_, gs = gs_;
     const long long nnz = nnz_, nl = nnzL();
     const int* rs = rs_.as<int>();
     const int* col = col_.as<int>();
-    const int* lend = lend_.as<int>();
-    const int* ubeg = ubeg_.as<int>();
-    const int* order = order_.as<int>();
-    double* part = part_.as<double>();
-    int* cnt = cnt_.as<int>();
     double* scal = scal_.as<double>();
     const bool f32 = which == ASTG_K_SPMV32 || which == ASTG_K_FWD32 || which == ASTG_K_BWD32;
     if (f32 && !val32_.p) {
@@ -842,16 +938,20 @@ int PcgDevice::benchKernel(int which, int reps, double& avgMs, double& bytes, st
     const double w = f32 ? 4.0 : 8.0;
     switch (which) {
         case ASTG_K_SPMV64: case ASTG_K_SPMV32:
-            bytes = (w + 4) * nnz + (2 * w + 4) * (double)n + w * n;  // values+cols, rowstart, s gather, q write, s read
+            // values + columns, rowstart, s gathered once, s read for the dot, q written
+            bytes = (w + 4) * nnz + 4.0 * (n + 1) + 3 * w * n;
             break;
         case ASTG_K_FWD64: case ASTG_K_FWD32:
-            bytes = (w + 4) * nl + (5 * w + 12) * (double)n;  // L values+cols, order+rs+lend, r, invd, y write, y gather
+            // L values + columns, order + rowstart + diagonal split, r, invd, y written, y gathered once
+            bytes = (w + 4) * nl + (4.0 * w + 12) * n;
             break;
         case ASTG_K_BWD64: case ASTG_K_BWD32:
-            bytes = (w + 4) * nl + (5 * w + 12) * (double)n;  // L^T values+cols, order+ubeg+rs, y, r, invd, z write/gather
+            // L^T values + columns, order + rowstart + split, y, r, invd, z written, z gathered once
+            bytes = (w + 4) * nl + (5.0 * w + 12) * n;
             break;
         case ASTG_K_MIC:
-            bytes = 12.0 * nnz + 8.0 * nnz + 36.0 * n;  // A + pattern read, L written, invd/order/splits
+            // A values + pattern read, factor written twice (both triangles), invd
+            bytes = 12.0 * nnz + 8.0 * 2 * nl + 24.0 * n;
             break;
         default:
             err = "unknown kernel id";
@@ -860,73 +960,73 @@ int PcgDevice::benchKernel(int which, int reps, double& avgMs, double& bytes, st
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
+    k_zero_counters<<<1, 64, 0, stream_>>>(cnt_.as<int>());
     double total = 0;
+    int rc = 0;
+    const bool sweep = which == ASTG_K_FWD64 || which == ASTG_K_BWD64 || which == ASTG_K_FWD32 ||
+                       which == ASTG_K_BWD32;
     for (int rep = 0; rep < reps; ++rep) {
-        // reset device flags so the kernels do not early-exit, and re-arm the sweep inputs
-        k_zero_counters<<<1, 64, 0, stream_>>>(cnt);
+        // re-arm the device state so no kernel early-exits
         k_zero64<<<1, BS, 0, stream_>>>(32, scal);
-        k_fill_i32<<<blocks(depth_), BS, 0, stream_>>>(gateF_.as<int>(), 0, depth_);
-        k_fill_i32<<<blocks(depth_), BS, 0, stream_>>>(gateB_.as<int>(), 0, depth_);
-        k_fill_i32<<<blocks(depth_), BS, 0, stream_>>>(gateM_.as<int>(), 0, depth_);
+        const Parts P64 = parts(gB64_), P32 = parts(gB32_);
         if (which == ASTG_K_FWD64 || which == ASTG_K_BWD64) {
-            k_fill_sent<double><<<nb, BS, 0, stream_>>>(y_.as<double>(), n);
-            k_fill_sent<double><<<nb, BS, 0, stream_>>>(z_.as<double>(), n);
+            k_fill_sent<double><<<blocks(n), BS, 0, stream_>>>(y_.as<double>(), n);
+            k_fill_sent<double><<<blocks(n), BS, 0, stream_>>>(z_.as<double>(), n);
         } else if (which == ASTG_K_FWD32 || which == ASTG_K_BWD32) {
-            k_fill_sent<float><<<nb, BS, 0, stream_>>>(y_.as<float>(), n);
-            k_fill_sent<float><<<nb, BS, 0, stream_>>>(z_.as<float>(), n);
-        } else if (which == ASTG_K_MIC) {
-            k_fill_sent<double><<<nb, BS, 0, stream_>>>(invd_.as<double>(), n);
+            k_fill_sent<float><<<blocks(n), BS, 0, stream_>>>(y_.as<float>(), n);
+            k_fill_sent<float><<<blocks(n), BS, 0, stream_>>>(z_.as<float>(), n);
         }
-        if (which == ASTG_K_BWD64) {  // the backward sweep needs a complete y
-            k_fwd<double><<<nbs, BS, 0, stream_>>>(P, order, gate(gateF_), gateB_.as<int>(), rs, lend, col, F_.as<double>(), invd_.as<double>(),
-                                                  r_.as<double>(), y_.as<double>(), z_.as<double>(), cnt, scal);
-        } else if (which == ASTG_K_BWD32) {
-            k_fwd<float><<<nbs, BS, 0, stream_>>>(P, order, gate(gateF_), gateB_.as<int>(), rs, lend, col, F32_.as<float>(), invd32_.as<float>(),
-                                                 r_.as<float>(), y_.as<float>(), z_.as<float>(), cnt, scal);
-        }
+        // the forward sweep starts in mode 0, which needs nonzero ||b|| partials
+        if (sweep) k_fill64<<<blocks(gs_), BS, 0, stream_>>>(gs_, P64.res, 1.0);
+        if (which == ASTG_K_BWD64)
+            rc = launchFwd<double>(F_.as<double>(), invd_.as<double>(), r_.as<double>(), y_.as<double>(),
+                                   z_.as<double>(), 0, 0, 1 << 30, ASTG_NORM_INF, 1e-30, err);
+        else if (which == ASTG_K_BWD32)
+            rc = launchFwd<float>(F32_.as<float>(), invd32_.as<float>(), r_.as<float>(), y_.as<float>(),
+                                  z_.as<float>(), 0, 0, 1 << 30, ASTG_NORM_INF, 1e-30, err);
+        if (rc) break;
         HIPCHK(hipEventRecord(e0, stream_));
         switch (which) {
             case ASTG_K_SPMV64:
-                k_spmv_dot<double><<<nb, BS, 0, stream_>>>(n, rs, col, val_.as<double>(), s_.as<double>(),
-                                                           q_.as<double>(), part, cnt, scal, nb);
+                k_spmv_dot<double><<<gs, BS, 0, stream_>>>(n, rs, col, val_.as<double>(), s_.as<double>(),
+                                                           q_.as<double>(), P64, scal);
                 break;
             case ASTG_K_SPMV32:
-                k_spmv_dot<float><<<nb, BS, 0, stream_>>>(n, rs, col, val32_.as<float>(), s_.as<float>(),
-                                                          q_.as<float>(), part, cnt, scal, nb);
+                k_spmv_dot<float><<<gs, BS, 0, stream_>>>(n, rs, col, val32_.as<float>(), s_.as<float>(),
+                                                          q_.as<float>(), P32, scal);
                 break;
             case ASTG_K_FWD64:
-                k_fwd<double><<<nbs, BS, 0, stream_>>>(P, order, gate(gateF_), gateB_.as<int>(), rs, lend, col, F_.as<double>(), invd_.as<double>(),
-                                                      r_.as<double>(), y_.as<double>(), z_.as<double>(), cnt, scal);
+                rc = launchFwd<double>(F_.as<double>(), invd_.as<double>(), r_.as<double>(), y_.as<double>(),
+                                       z_.as<double>(), 0, 0, 1 << 30, ASTG_NORM_INF, 1e-30, err);
                 break;
             case ASTG_K_FWD32:
-                k_fwd<float><<<nbs, BS, 0, stream_>>>(P, order, gate(gateF_), gateB_.as<int>(), rs, lend, col, F32_.as<float>(), invd32_.as<float>(),
-                                                     r_.as<float>(), y_.as<float>(), z_.as<float>(), cnt, scal);
+                rc = launchFwd<float>(F32_.as<float>(), invd32_.as<float>(), r_.as<float>(), y_.as<float>(),
+                                      z_.as<float>(), 0, 0, 1 << 30, ASTG_NORM_INF, 1e-30, err);
                 break;
             case ASTG_K_BWD64:
-                k_bwd_dot<double><<<nbs, BS, 0, stream_>>>(P, order, gate(gateB_), rs, ubeg, col, F_.as<double>(),
-                                                          invd_.as<double>(), y_.as<double>(), r_.as<double>(),
-                                                          z_.as<double>(), part, cnt, scal, 1, nbs);
+                rc = launchBwd<double>(F_.as<double>(), invd_.as<double>(), y_.as<double>(), r_.as<double>(),
+                                       z_.as<double>(), 0, err);
                 break;
             case ASTG_K_BWD32:
-                k_bwd_dot<float><<<nbs, BS, 0, stream_>>>(P, order, gate(gateB_), rs, ubeg, col, F32_.as<float>(),
-                                                         invd32_.as<float>(), y_.as<float>(), r_.as<float>(),
-                                                         z_.as<float>(), part, cnt, scal, 1, nbs);
+                rc = launchBwd<float>(F32_.as<float>(), invd32_.as<float>(), y_.as<float>(), r_.as<float>(),
+                                      z_.as<float>(), 0, err);
                 break;
             case ASTG_K_MIC:
-                k_mic<<<nbs, BS, 0, stream_>>>(P, order, gate(gateM_), rs, lend, ubeg, col, val_.as<double>(),
-                                               tpos_.as<int>(), F_.as<double>(), invd_.as<double>(), 0.97, 0.25, cnt);
+                rc = launchMic(0.97, 0.25, err);
                 break;
         }
+        if (rc) break;
         HIPCHK(hipEventRecord(e1, stream_));
         HIPCHK(hipEventSynchronize(e1));
         float ms = 0;
         HIPCHK(hipEventElapsedTime(&ms, e0, e1));
         total += ms;
     }
-    HIPCHK(hipGetLastError());
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    int rc = checkErrors("kernel timing", err);
+    if (rc) return rc;
+    HIPCHK(hipGetLastError());
+    rc = checkErrors("kernel timing", err);
     if (rc) return rc;
     avgMs = total / reps;
     return ASTG_OK;
