@@ -30,9 +30,9 @@ EXPORTS = [
     "astg_pcg_solve_csr", "astg_split_spheres", "astg_set_topology_split", "astg_set_tilt_open",
     "astg_level_info", "astg_get_nodes", "astg_get_tilt", "astg_get_half_tilts", "astg_get_numbering",
     "astg_get_weights", "astg_set_velocity", "astg_get_velocity", "astg_project", "astg_get_system",
-    "astg_get_mic_factor", "astg_fields_snapshot", "astg_kernel_time",
+    "astg_get_mic_factor", "astg_fields_snapshot", "astg_kernel_time", "astg_sweep_timeline",
 ]
-K_SPMV64, K_FWD64, K_BWD64, K_MIC, K_ASSEMBLE, K_SPMV32, K_FWD32, K_BWD32 = range(8)
+K_SPMV64, K_FWD64, K_BWD64, K_MIC, K_ASSEMBLE, K_SPMV32, K_FWD32, K_BWD32, K_COPY = range(9)
 
 
 class PcgParams(ctypes.Structure):
@@ -89,6 +89,7 @@ def lib():
         "astg_get_mic_factor": (I, [P, P, P, P, P, P]),
         "astg_fields_snapshot": (I, [P, I]),
         "astg_kernel_time": (I, [P, I, I, P, P]),
+        "astg_sweep_timeline": (I, [P, I, P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -268,6 +269,15 @@ class Context:
         ms, by = ctypes.c_double(), ctypes.c_double()
         self._chk(self._L.astg_kernel_time(self.h, which, reps, ctypes.byref(ms), ctypes.byref(by)), "kernel_time")
         return ms.value, by.value
+
+    def sweep_timeline(self, which, n):
+        """Per-row (gate, deps seen, published) device wall-clock ticks
+        (100 MHz) of one FP64 sweep launch, and the rows' dependency levels."""
+        t = np.empty((n, 3), np.int64)
+        lev = np.empty(n, np.int32)
+        self._chk(self._L.astg_sweep_timeline(self.h, which, t.ctypes.data_as(ctypes.c_void_p),
+                                              lev.ctypes.data_as(ctypes.c_void_p)), "sweep_timeline")
+        return t, lev
 
     def pcg_solve_csr(self, rowstart, col, val, b, params=None, allow_unconverged=False):
         rowstart = np.ascontiguousarray(rowstart, np.int32)

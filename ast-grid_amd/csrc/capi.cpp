@@ -536,4 +536,20 @@ int astg_kernel_time(astg_ctx* c, int which, int reps, double* avg_ms, double* b
     });
 }
 
+int astg_sweep_timeline(astg_ctx* c, int which, long long* t_out, int* level_out) {
+    if (!c || !t_out || !level_out) return fail(c, ASTG_ERR_ARG, "bad arguments");
+    int rc = needDevice(c);
+    if (rc) return rc;
+    if (!c->deviceReady || !c->lastPcg) return fail(c, ASTG_ERR_STATE, "run astg_project first");
+    return guarded(c, [&] {
+        std::vector<long long> t;
+        std::vector<int> lev;
+        int r = c->pcg->sweepTimeline(which, t, lev, c->err);
+        if (r) return r;
+        std::copy(t.begin(), t.end(), t_out);
+        std::copy(lev.begin(), lev.end(), level_out);
+        return (int)ASTG_OK;
+    });
+}
+
 }  // extern "C"

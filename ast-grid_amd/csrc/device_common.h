@@ -106,47 +106,59 @@ struct DepRow {
             }
     }
     // acc -= sum_j f[j] * vec[c[j]] in ascending j once every vec[c[j]] is
-    // published. Returns false if the spin limit was hit.
-    __device__ bool apply(const T* vec, T& acc) {
-        T v[kMaxDeps];
-        unsigned ready = 0;
-        const unsigned full = (1u << nd) - 1u;
-        for (int spins = 0;; ++spins) {
+    // published, then publish(acc) -- inside the polling loop, in the round
+    // the lane's own dependencies became ready, so a lane never waits for the
+    // other lanes of its wavefront before publishing. Between polling rounds
+    // the lane sleeps 1, 2, 4, ... up to `cap` units of 64 cycles. Returns
+    // false (after publishing NaN) if the spin limit was hit.
+    // One polling round over the dependencies not seen yet. All loads are
+    // issued before any result is examined, so a round costs one memory
+    // round trip, not one per dependency.
+    __device__ void poll_round(const T* vec, T* v, unsigned& ready) {
+        typename Sentinel<T>::bits_t b[kMaxDeps];
 #pragma unroll
-            for (int j = 0; j < kMaxDeps; ++j)
-                if (j < nd && !((ready >> j) & 1u)) {
-                    auto bits = Sentinel<T>::load(&vec[c[j]]);
-                    if (bits != Sentinel<T>::value) {
-                        v[j] = Sentinel<T>::val(bits);
-                        ready |= 1u << j;
-                    }
-                }
-            if (ready == full) break;
-            if (spins > kSpinLimit) return false;
-            __builtin_amdgcn_s_sleep(1);
+        for (int j = 0; j < kMaxDeps; ++j) {
+            b[j] = Sentinel<T>::value;
+            if (j < nd && !((ready >> j) & 1u)) b[j] = Sentinel<T>::load(&vec[c[j]]);
         }
 #pragma unroll
         for (int j = 0; j < kMaxDeps; ++j)
-            if (j < nd) acc -= f[j] * v[j];
-        return true;
+            if (b[j] != Sentinel<T>::value) {
+                v[j] = Sentinel<T>::val(b[j]);
+                ready |= 1u << j;
+            }
+    }
+    template <class Pub>
+    __device__ bool run(const T* vec, T acc, int cap, Pub publish) {
+        T v[kMaxDeps];
+        unsigned ready = 0;
+        const unsigned full = (1u << nd) - 1u;
+        int backoff = 1;
+        bool pending = true, ok = true;
+        // The loop exit is wave-uniform (vote), so the publishing branch is
+        // part of the loop body: a lane publishes in the round it became ready.
+        for (int spins = 0;; ++spins) {
+            if (pending) {
+                poll_round(vec, v, ready);
+                if (ready == full) {
+                    T a = acc;
+#pragma unroll
+                    for (int j = 0; j < kMaxDeps; ++j)
+                        if (j < nd) a -= f[j] * v[j];
+                    publish(a);
+                    pending = false;
+                } else if (spins > kSpinLimit) {
+                    publish((T)NAN);
+                    pending = false;
+                    ok = false;
+                }
+            }
+            if (!__any(pending)) break;
+            for (int k = 0; k < backoff; ++k) __builtin_amdgcn_s_sleep(1);
+            backoff = backoff < cap ? 2 * backoff : cap;
+        }
+        return ok;
     }
 };
-
-// Rows with more than kMaxDeps dependencies (never produced by the octree
-// assembly, possible through the CSR entry point): one dependency at a time.
-template <typename T>
-__device__ inline bool apply_deps_slow(const int* col, const T* F, int p, int pe, const T* vec, T& acc) {
-    for (int spins = 0; p < pe;) {
-        auto bits = Sentinel<T>::load(&vec[col[p]]);
-        if (bits == Sentinel<T>::value) {
-            if (++spins > kSpinLimit) return false;
-            __builtin_amdgcn_s_sleep(1);
-            continue;
-        }
-        acc -= F[p] * Sentinel<T>::val(bits);
-        ++p;
-    }
-    return true;
-}
 
 }  // namespace astg

@@ -40,6 +40,8 @@ struct SweepPlan {
     const int* gateB;  // per segment: a row of level L+dist (backward), -1 none
     int P;
     int nchunks;       // P / 256
+    int pollCap;       // largest sleep between polling rounds (units of 64 cycles)
+    long long* trace;  // optional per-row timeline (3 per row), null in normal runs
 };
 
 // Per-block partials of the reductions, consumed by the next kernel.
@@ -79,6 +81,11 @@ public:
     int benchKernel(int which, int reps, double& avgMs, double& bytes, std::string& err);
     long long nnzL() const { return (nnz_ - n_) / 2; }
 
+    // Timeline of one sweep launch (ASTG_K_FWD64 / ASTG_K_BWD64): per row the
+    // device wall clock (100 MHz) when its wavefront passed the gate, when all
+    // its dependencies were seen, and after it published its value; plus the
+    // dependency level of every row.
+    int sweepTimeline(int which, std::vector<long long>& t, std::vector<int>& lev, std::string& err);
     // Reference-form copy of the factor (host arrays), for parity checks.
     int downloadFactor(std::vector<double>& invdiag, std::vector<unsigned>& colstart, std::vector<unsigned>& rowindex,
                        std::vector<double>& value);
@@ -97,6 +104,10 @@ private:
     int launchMic(double tau, double ratio, std::string& err);
     int syncScalars(std::string& err);
     int checkErrors(const char* where, std::string& err);
+    template <typename T>
+    int fwdGrid() const { return sizeof(T) == 8 ? gF64_ : gF32_; }
+    template <typename T>
+    int bwdGrid() const { return sizeof(T) == 8 ? gB64_ : gB32_; }
     SweepPlan plan() const;
     Parts parts(int gb) const;
 
@@ -119,6 +130,8 @@ private:
     int gb_ = 0;                            // largest backward-sweep grid (rho partial slots)
     int gF64_ = 0, gB64_ = 0, gF32_ = 0, gB32_ = 0, gM_ = 0;  // persistent grids
     int gateDist_ = 2;
+    int pollCap_ = 1;
+    long long* trace_ = nullptr;            // set only while a timeline is recorded
 };
 
 }  // namespace astg

@@ -84,6 +84,11 @@ __global__ void k_fill64(int n, double* x, double v) {
     const int i = blockIdx.x * BS + threadIdx.x;
     if (i < n) x[i] = v;
 }
+// Plain streaming copy (calibration of the HBM counters: 16 bytes per element).
+__global__ void __launch_bounds__(BS) k_copy64(int n, const double* __restrict__ a, double* __restrict__ b) {
+    const int i = blockIdx.x * BS + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
 
 // ---------------------------------------------------------------------------
 // deterministic block reductions
@@ -167,18 +172,24 @@ __device__ inline void gate_row(const T* vec, int target) {
 // order follows the reference's right-looking loop (pcg_solver.h:129-189):
 // contributions to row/column k arrive in ascending m.
 // ---------------------------------------------------------------------------
+// Polls F[pos[j]], j < cnt, until all are published; every round issues all
+// its loads before examining any.
 __device__ inline bool poll_all(const double* F, const int* pos, double* out, int cnt) {
     unsigned ready = 0;
     const unsigned full = cnt >= 32 ? ~0u : ((1u << cnt) - 1u);
     for (int spins = 0;; ++spins) {
+        unsigned long long b[kMaxDeps];
+#pragma unroll
+        for (int j = 0; j < kMaxDeps; ++j) {
+            b[j] = kSent64;
+            if (j < cnt && !((ready >> j) & 1u))
+                b[j] = ld_agent_u64(reinterpret_cast<const unsigned long long*>(&F[pos[j]]));
+        }
 #pragma unroll
         for (int j = 0; j < kMaxDeps; ++j)
-            if (j < cnt && !((ready >> j) & 1u)) {
-                const unsigned long long b = ld_agent_u64(reinterpret_cast<const unsigned long long*>(&F[pos[j]]));
-                if (b != kSent64) {
-                    out[j] = __longlong_as_double((long long)b);
-                    ready |= 1u << j;
-                }
+            if (b[j] != kSent64) {
+                out[j] = __longlong_as_double((long long)b[j]);
+                ready |= 1u << j;
             }
         if (ready == full) return true;
         if (spins > kSpinLimit) return false;
@@ -309,36 +320,37 @@ __global__ void __launch_bounds__(BS) k_init(int n, const T* b, T* x, T* r, T* y
 // Forward sweep y = L^-1 r. mode 0 (start of the solve): ||b|| from the
 // partials, tolerance, zero-rhs exit. mode 1 (iteration `it`): the reference's
 // convergence test on the updated residual, before preconditioning.
+// Returns true when the solve stops here (every block takes the same decision).
+__device__ inline bool fwd_prologue(const Parts& Pt, double* scal, int mode, int it, int maxIt, int norm,
+                                    double tolRel) {
+    double red = norm == ASTG_NORM_INF ? reduce_prev<true>(Pt.res, Pt.gs) : reduce_prev<false>(Pt.res, Pt.gs);
+    if (norm != ASTG_NORM_INF) red = sqrt(red);
+    if (mode == 0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            scal[S_BNORM] = red;
+            scal[S_RES] = red;
+            scal[S_TOL] = tolRel * red;
+            scal[S_ITER] = 0;
+            if (red == 0.0) scal[S_DONE] = 1.0;
+        }
+        return red == 0.0;
+    }
+    const bool conv = red <= scal[S_TOL];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        scal[S_RES] = red;
+        scal[S_ITER] = (double)(it + 1);
+        if (conv) scal[S_DONE] = 1.0;
+        else if (it + 1 >= maxIt) scal[S_DONE] = 3.0;
+    }
+    return conv || it + 1 >= maxIt;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(BS) k_fwd(SweepPlan S, const int* rs, const int* lend, const int* col, const T* F,
                                             const T* invd, const T* r, T* y, T* z, Parts Pt, int* cnt, double* scal,
                                             int mode, int it, int maxIt, int norm, double tolRel) {
     if (scal[S_DONE] != 0.0) return;
-    {
-        double red = norm == ASTG_NORM_INF ? reduce_prev<true>(Pt.res, Pt.gs) : reduce_prev<false>(Pt.res, Pt.gs);
-        if (norm != ASTG_NORM_INF) red = sqrt(red);
-        bool stop = false;
-        if (mode == 0) {
-            if (blockIdx.x == 0 && threadIdx.x == 0) {
-                scal[S_BNORM] = red;
-                scal[S_RES] = red;
-                scal[S_TOL] = tolRel * red;
-                scal[S_ITER] = 0;
-                if (red == 0.0) scal[S_DONE] = 1.0;
-            }
-            stop = red == 0.0;
-        } else {
-            const bool conv = red <= scal[S_TOL];
-            if (blockIdx.x == 0 && threadIdx.x == 0) {
-                scal[S_RES] = red;
-                scal[S_ITER] = (double)(it + 1);
-                if (conv) scal[S_DONE] = 1.0;
-                else if (it + 1 >= maxIt) scal[S_DONE] = 3.0;
-            }
-            stop = conv || it + 1 >= maxIt;
-        }
-        if (stop) return;
-    }
+    if (fwd_prologue(Pt, scal, mode, it, maxIt, norm, tolRel)) return;
     for (int c = blockIdx.x; c < S.nchunks; c += gridDim.x) {
         const int t = c * BS + threadIdx.x;
         const int i = S.order[t];
@@ -351,17 +363,22 @@ __global__ void __launch_bounds__(BS) k_fwd(SweepPlan S, const int* rs, const in
             pe = lend[i];
             acc = r[i];
             inv = invd[i];
-            if (pe - p0 <= kMaxDeps) d.preload(col, F, p0, pe);
+            d.preload(col, F, p0, pe);
             Sentinel<T>::storeSent(&z[i]);
         }
         gate_row(y, S.gateF[t >> 6]);
+        const long long tg = S.trace ? wall_clock64() : 0;
         if (i >= 0) {
-            const bool ok = (pe - p0 <= kMaxDeps) ? d.apply(y, acc) : apply_deps_slow(col, F, p0, pe, y, acc);
-            if (!ok) {
-                atomicOr(&cnt[C_ERR], E_FWD);
-                acc = (T)NAN;
-            }
-            Sentinel<T>::store(&y[i], acc * inv);
+            const bool ok = d.run(y, acc, S.pollCap, [&](T a) {
+                const long long td = S.trace ? wall_clock64() : 0;
+                Sentinel<T>::store(&y[i], a * inv);
+                if (S.trace) {
+                    S.trace[3 * (long)i] = tg;
+                    S.trace[3 * (long)i + 1] = td;
+                    S.trace[3 * (long)i + 2] = wall_clock64();
+                }
+            });
+            if (!ok) atomicOr(&cnt[C_ERR], E_FWD);
         }
     }
 }
@@ -386,17 +403,23 @@ __global__ void __launch_bounds__(BS) k_bwd_dot(SweepPlan S, const int* rs, cons
             acc = y[k];
             inv = invd[k];
             rk = r[k];
-            if (pe - p0 <= kMaxDeps) d.preload(col, F, p0, pe);
+            d.preload(col, F, p0, pe);
         }
         gate_row(z, S.gateB[pos >> 6]);
+        const long long tg = S.trace ? wall_clock64() : 0;
         if (k >= 0) {
-            const bool ok = (pe - p0 <= kMaxDeps) ? d.apply(z, acc) : apply_deps_slow(col, F, p0, pe, z, acc);
-            if (!ok) {
-                atomicOr(&cnt[C_ERR], E_BWD);
-                acc = (T)NAN;
-            }
-            const T zk = acc * inv;
-            Sentinel<T>::store(&z[k], zk);
+            T zk = 0;
+            const bool ok = d.run(z, acc, S.pollCap, [&](T a) {
+                const long long td = S.trace ? wall_clock64() : 0;
+                zk = a * inv;
+                Sentinel<T>::store(&z[k], zk);
+                if (S.trace) {
+                    S.trace[3 * (long)k] = tg;
+                    S.trace[3 * (long)k + 1] = td;
+                    S.trace[3 * (long)k + 2] = wall_clock64();
+                }
+            });
+            if (!ok) atomicOr(&cnt[C_ERR], E_BWD);
             v += (double)zk * (double)rk;
         }
     }
@@ -580,6 +603,7 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
             return ASTG_ERR_ARG;
         }
     if (const char* g = getenv("ASTG_GATE_DIST")) gateDist_ = std::max(1, atoi(g));
+    if (const char* g = getenv("ASTG_POLL_CAP")) pollCap_ = std::max(1, atoi(g));
     int dev = 0, cus = 0;
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -640,6 +664,7 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
         return ASTG_ERR_HIP;
     }
     depth_ = sortedLev[n - 1] + 1;
+
     // Pad every level to a multiple of 64 rows so that a wavefront never holds
     // two rows of different levels (rows of one level are independent).
     std::vector<int> padded, levStart(depth_, 0), levRows(depth_, 0);
@@ -699,6 +724,8 @@ SweepPlan PcgDevice::plan() const {
     s.gateB = gateB_.as<int>();
     s.P = P_;
     s.nchunks = P_ / BS;
+    s.pollCap = pollCap_;
+    s.trace = trace_;
     return s;
 }
 
@@ -751,8 +778,7 @@ int PcgDevice::syncScalars(std::string& err) {
 template <typename T>
 int PcgDevice::launchFwd(const T* F, const T* invd, const T* r, T* y, T* z, int mode, int it, int maxIt, int norm,
                          double tolRel, std::string& err) {
-    const int g = sizeof(T) == 8 ? gF64_ : gF32_;
-    const int gb = sizeof(T) == 8 ? gB64_ : gB32_;
+    const int g = fwdGrid<T>(), gb = bwdGrid<T>();
     HIPCHK(launchCoop(k_fwd<T>, g, stream_, plan(), (const int*)rs_.as<int>(), (const int*)lend_.as<int>(),
                       (const int*)col_.as<int>(), F, invd, r, y, z, parts(gb), cnt_.as<int>(), scal_.as<double>(),
                       mode, it, maxIt, norm, tolRel));
@@ -761,7 +787,7 @@ int PcgDevice::launchFwd(const T* F, const T* invd, const T* r, T* y, T* z, int 
 
 template <typename T>
 int PcgDevice::launchBwd(const T* F, const T* invd, const T* y, const T* r, T* z, int parityOut, std::string& err) {
-    const int gb = sizeof(T) == 8 ? gB64_ : gB32_;
+    const int gb = bwdGrid<T>();
     HIPCHK(launchCoop(k_bwd_dot<T>, gb, stream_, plan(), (const int*)rs_.as<int>(), (const int*)ubeg_.as<int>(),
                       (const int*)col_.as<int>(), F, invd, y, r, z, parts(gb), cnt_.as<int>(),
                       (const double*)scal_.as<double>(), parityOut));
@@ -772,7 +798,7 @@ template <typename T>
 int PcgDevice::pcgInner(const T* A, const T* F, const T* invd, const T* b, T* x, double tolRel, int maxIt, int norm,
                         int& iters, double& resOut, bool& conv, std::string& err) {
     const int n = n_, gs = gs_;
-    const int gb = sizeof(T) == 8 ? gB64_ : gB32_;
+    const int gb = bwdGrid<T>();
     T* r = r_.as<T>();
     T* z = z_.as<T>();
     T* s = s_.as<T>();
@@ -953,6 +979,9 @@ int PcgDevice::benchKernel(int which, int reps, double& avgMs, double& bytes, st
             // A values + pattern read, factor written twice (both triangles), invd
             bytes = 12.0 * nnz + 8.0 * 2 * nl + 24.0 * n;
             break;
+        case ASTG_K_COPY:
+            bytes = 16.0 * n;
+            break;
         default:
             err = "unknown kernel id";
             return ASTG_ERR_ARG;
@@ -968,7 +997,7 @@ int PcgDevice::benchKernel(int which, int reps, double& avgMs, double& bytes, st
     for (int rep = 0; rep < reps; ++rep) {
         // re-arm the device state so no kernel early-exits
         k_zero64<<<1, BS, 0, stream_>>>(32, scal);
-        const Parts P64 = parts(gB64_), P32 = parts(gB32_);
+        const Parts P64 = parts(bwdGrid<double>()), P32 = parts(bwdGrid<float>());
         if (which == ASTG_K_FWD64 || which == ASTG_K_BWD64) {
             k_fill_sent<double><<<blocks(n), BS, 0, stream_>>>(y_.as<double>(), n);
             k_fill_sent<double><<<blocks(n), BS, 0, stream_>>>(z_.as<double>(), n);
@@ -1014,6 +1043,9 @@ int PcgDevice::benchKernel(int which, int reps, double& avgMs, double& bytes, st
             case ASTG_K_MIC:
                 rc = launchMic(0.97, 0.25, err);
                 break;
+            case ASTG_K_COPY:
+                k_copy64<<<blocks(n), BS, 0, stream_>>>(n, b_.as<double>(), q_.as<double>());
+                break;
         }
         if (rc) break;
         HIPCHK(hipEventRecord(e1, stream_));
@@ -1029,6 +1061,27 @@ int PcgDevice::benchKernel(int which, int reps, double& avgMs, double& bytes, st
     rc = checkErrors("kernel timing", err);
     if (rc) return rc;
     avgMs = total / reps;
+    return ASTG_OK;
+}
+
+int PcgDevice::sweepTimeline(int which, std::vector<long long>& t, std::vector<int>& lev, std::string& err) {
+    if (which != ASTG_K_FWD64 && which != ASTG_K_BWD64) {
+        err = "timeline: only the FP64 sweeps are instrumented";
+        return ASTG_ERR_ARG;
+    }
+    DevBuf tr;
+    HIPCHK(tr.alloc((size_t)n_ * 3 * sizeof(long long)));
+    HIPCHK(hipMemsetAsync(tr.p, 0, (size_t)n_ * 3 * sizeof(long long), stream_));
+    trace_ = tr.as<long long>();
+    double ms = 0, bytes = 0;
+    int rc = benchKernel(which, 1, ms, bytes, err);
+    trace_ = nullptr;
+    if (rc) return rc;
+    t.resize((size_t)n_ * 3);
+    lev.resize(n_);
+    HIPCHK(hipMemcpyAsync(t.data(), tr.p, t.size() * sizeof(long long), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipMemcpyAsync(lev.data(), lev_.p, (size_t)n_ * 4, hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
     return ASTG_OK;
 }
 

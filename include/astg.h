@@ -163,7 +163,16 @@ int astg_fields_snapshot(astg_ctx* ctx, int restore);
 #define ASTG_K_SPMV32 5  /* q = A s fused with s.q (FP32 values and vectors) */
 #define ASTG_K_FWD32 6   /* forward MIC sweep (FP32) */
 #define ASTG_K_BWD32 7   /* backward MIC sweep (FP32) */
+#define ASTG_K_COPY 8    /* plain copy of n doubles (HBM counter calibration) */
 int astg_kernel_time(astg_ctx* ctx, int which, int reps, double* avg_ms, double* bytes);
+
+/* Per-row timeline of one launch of an FP64 sweep (ASTG_K_FWD64 or
+ * ASTG_K_BWD64) of the current system: t_out[3*i + {0,1,2}] = device wall
+ * clock (100 MHz ticks) when row i's wavefront passed its gate, when all of
+ * row i's dependencies were seen, and after row i published its value;
+ * level_out[i] = dependency level of row i. Arrays of 3n and n entries. For
+ * latency analysis; no counterpart in the reference. */
+int astg_sweep_timeline(astg_ctx* ctx, int which, long long* t_out, int* level_out);
 
 #ifdef __cplusplus
 }
