@@ -608,7 +608,9 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     cus_ = cus;
-    gs_ = std::max(1, std::min(2 * cus, blocks(n)));
+    // streaming kernels: 8 blocks (32 wavefronts) per CU in a grid-stride loop;
+    // this many partials is still cheap to reduce in the consumer kernel
+    gs_ = std::max(1, std::min(8 * cus, blocks(n)));
     const size_t n8 = (size_t)n * 8, n4 = (size_t)n * 4;
     HIPCHK(rs_.alloc(((size_t)n + 1) * 4));
     HIPCHK(col_.alloc((size_t)nnz_ * 4));
