@@ -55,6 +55,23 @@ def build(jobs=8, verbose=False):
     if p.returncode != 0:
         raise RuntimeError("link failed: %s\n%s\n%s" % (" ".join(cmd), p.stdout, p.stderr))
     os.replace(OUT + ".tmp", OUT)
+    # C++ example of the CUDA-solver drop-in, linked against the library
+    ex = os.path.join(HERE, "astg", "cuda_solver_shim")
+    cmd = ["g++", "-std=c++17", "-O2", "-I" + os.path.join(HERE, "..", "include"),
+           os.path.join(HERE, "examples", "cuda_solver_shim.cpp"), "-o", ex + ".tmp",
+           "-L" + os.path.dirname(OUT), "-lastg", "-Wl,-rpath,$ORIGIN"]
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    if p.returncode != 0:
+        raise RuntimeError("shim example build failed: %s\n%s\n%s" % (" ".join(cmd), p.stdout, p.stderr))
+    os.replace(ex + ".tmp", ex)
+    so = os.path.join(HERE, "astg", "libastg_shim_example.so")
+    cmd = ["g++", "-std=c++17", "-O2", "-shared", "-fPIC", "-DASTG_SHIM_NO_MAIN",
+           "-I" + os.path.join(HERE, "..", "include"), os.path.join(HERE, "examples", "cuda_solver_shim.cpp"),
+           "-o", so + ".tmp", "-L" + os.path.dirname(OUT), "-lastg", "-Wl,-rpath,$ORIGIN"]
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    if p.returncode != 0:
+        raise RuntimeError("shim example build failed: %s\n%s\n%s" % (" ".join(cmd), p.stdout, p.stderr))
+    os.replace(so + ".tmp", so)
     return OUT
 
 

@@ -36,11 +36,12 @@ struct DevBuf {
 // Rows in padded level order, plus one gate row per 64-row segment.
 struct SweepPlan {
     const int* order;  // P entries, -1 = padding
-    const int* gateF;  // per segment: a row of level L-dist (forward), -1 none
-    const int* gateB;  // per segment: a row of level L+dist (backward), -1 none
+    const int* gateF;  // per segment: position of a row of level L-dist (forward), -1 none
+    const int* gateB;  // per segment: position of a row of level L+dist (backward), -1 none
     int P;
     int nchunks;       // P / 256
     int pollCap;       // largest sleep between polling rounds (units of 64 cycles)
+    int gateCap;       // largest sleep between gate polls (units of 128 cycles)
     long long* trace;  // optional per-row timeline (3 per row), null in normal runs
 };
 
@@ -97,10 +98,10 @@ private:
     int pcgInner(const T* A, const T* F, const T* invd, const T* b, T* x, double tolRel, int maxIt, int norm,
                  int& iters, double& resOut, bool& conv, std::string& err);
     template <typename T>
-    int launchFwd(const T* F, const T* invd, const T* r, T* y, T* z, int mode, int it, int maxIt, int norm,
+    int launchFwd(const T* F, const T* invd, const T* r, T* y, int mode, int it, int maxIt, int norm,
                   double tolRel, std::string& err);
     template <typename T>
-    int launchBwd(const T* F, const T* invd, const T* y, const T* r, T* z, int parityOut, std::string& err);
+    int launchBwd(const T* F, const T* invd, T* y, const T* r, T* z, int parityOut, std::string& err);
     int launchMic(double tau, double ratio, std::string& err);
     int syncScalars(std::string& err);
     int checkErrors(const char* where, std::string& err);
@@ -120,7 +121,8 @@ private:
     DevBuf val_, F_, invd_;                 // FP64 A, factor (symmetric copy), 1/pivot
     DevBuf val32_, F32_, invd32_;           // FP32 copies for the mixed form
     DevBuf b_, x_;                          // FP64 rhs / solution
-    DevBuf r_, z_, s_, q_, y_;              // work vectors (sized for FP64; reused as FP32)
+    DevBuf r_, z_, s_, q_;                  // work vectors (sized for FP64; reused as FP32)
+    DevBuf y_, zp_, colp_;                  // sweep vectors in padded level order (P), column -> position
     DevBuf r64_, d64_;                      // mixed form outer vectors
     DevBuf scal_, cnt_, part_;              // device scalars, counters, partials
     double* hscal_ = nullptr;               // pinned mirror of the scalars
@@ -131,6 +133,7 @@ private:
     int gF64_ = 0, gB64_ = 0, gF32_ = 0, gB32_ = 0, gM_ = 0;  // persistent grids
     int gateDist_ = 2;
     int pollCap_ = 1;
+    int gateCap_ = 8;
     long long* trace_ = nullptr;            // set only while a timeline is recorded
 };
 

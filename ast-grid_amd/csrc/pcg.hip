@@ -153,13 +153,13 @@ __global__ void __launch_bounds__(BS) k_levels(int n, const int* rs, const int* 
 // One lane of a wavefront polls the published value of a gate row (advisory
 // throttle; bounded).
 template <typename T>
-__device__ inline void gate_row(const T* vec, int target) {
+__device__ inline void gate_row(const T* vec, int target, int cap) {
     if (target < 0 || (threadIdx.x & 63) != 0) return;
     int backoff = 1;
     for (int s = 0; s < kGateLimit; ++s) {
         if (Sentinel<T>::load(&vec[target]) != Sentinel<T>::value) return;
         for (int k = 0; k < backoff; ++k) __builtin_amdgcn_s_sleep(2);
-        backoff = backoff < 8 ? backoff * 2 : 8;
+        backoff = backoff < cap ? backoff * 2 : cap;
     }
 }
 
@@ -281,7 +281,8 @@ __global__ void __launch_bounds__(BS) k_mic(SweepPlan S, const int* rs, const in
                                             double* invd, double tau, double ratio, int* cnt) {
     for (int c = blockIdx.x; c < S.nchunks; c += gridDim.x) {
         const int t = c * BS + threadIdx.x;
-        gate_row(invd, S.gateF[t >> 6]);
+        const int gp = S.gateF[t >> 6];
+        gate_row(invd, gp >= 0 ? S.order[gp] : -1, S.gateCap);
         const int k = S.order[t];
         if (k < 0) continue;
         if (!mic_row(k, rs[k], lend[k], ubeg[k], rs[k + 1], rs, lend, ubeg, col, A, tpos, F, invd, tau, ratio))
@@ -302,14 +303,12 @@ __global__ void __launch_bounds__(BS) k_mic(SweepPlan S, const int* rs, const in
 __device__ inline const double* rho_buf(const Parts& P, int parity) { return parity ? P.rho1 : P.rho0; }
 
 template <typename T>
-__global__ void __launch_bounds__(BS) k_init(int n, const T* b, T* x, T* r, T* y, T* z, Parts Pt, int norm) {
+__global__ void __launch_bounds__(BS) k_init(int n, const T* b, T* x, T* r, Parts Pt, int norm) {
     double v = 0.0;
     for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
         const T bi = b[i];
         x[i] = (T)0;
         r[i] = bi;
-        Sentinel<T>::storeSent(&y[i]);
-        Sentinel<T>::storeSent(&z[i]);
         const double a = (double)bi;
         v = norm == ASTG_NORM_INF ? fmax(v, fabs(a)) : v + a * a;
     }
@@ -364,14 +363,14 @@ __global__ void __launch_bounds__(BS) k_fwd(SweepPlan S, const int* rs, const in
             acc = r[i];
             inv = invd[i];
             d.preload(col, F, p0, pe);
-            Sentinel<T>::storeSent(&z[i]);
+            Sentinel<T>::storeSent(&z[t]);
         }
-        gate_row(y, S.gateF[t >> 6]);
+        gate_row(y, S.gateF[t >> 6], S.gateCap);
         const long long tg = S.trace ? wall_clock64() : 0;
         if (i >= 0) {
             const bool ok = d.run(y, acc, S.pollCap, [&](T a) {
                 const long long td = S.trace ? wall_clock64() : 0;
-                Sentinel<T>::store(&y[i], a * inv);
+                Sentinel<T>::store(&y[t], a * inv);
                 if (S.trace) {
                     S.trace[3 * (long)i] = tg;
                     S.trace[3 * (long)i + 1] = td;
@@ -387,7 +386,7 @@ __global__ void __launch_bounds__(BS) k_fwd(SweepPlan S, const int* rs, const in
 // z.r (one partial per block, fixed chunk assignment: deterministic).
 template <typename T>
 __global__ void __launch_bounds__(BS) k_bwd_dot(SweepPlan S, const int* rs, const int* ubeg, const int* col,
-                                                const T* F, const T* invd, const T* y, const T* r, T* z, Parts Pt,
+                                                const T* F, const T* invd, T* y, const T* r, T* z, T* zp, Parts Pt,
                                                 int* cnt, const double* scal, int parityOut) {
     if (scal[S_DONE] != 0.0) return;
     double v = 0.0;
@@ -400,19 +399,21 @@ __global__ void __launch_bounds__(BS) k_bwd_dot(SweepPlan S, const int* rs, cons
         if (k >= 0) {
             p0 = ubeg[k];
             pe = rs[k + 1];
-            acc = y[k];
+            acc = y[pos];
+            Sentinel<T>::storeSent(&y[pos]);  // re-armed for the next forward sweep
             inv = invd[k];
             rk = r[k];
             d.preload(col, F, p0, pe);
         }
-        gate_row(z, S.gateB[pos >> 6]);
+        gate_row(zp, S.gateB[pos >> 6], S.gateCap);
         const long long tg = S.trace ? wall_clock64() : 0;
         if (k >= 0) {
             T zk = 0;
-            const bool ok = d.run(z, acc, S.pollCap, [&](T a) {
+            const bool ok = d.run(zp, acc, S.pollCap, [&](T a) {
                 const long long td = S.trace ? wall_clock64() : 0;
                 zk = a * inv;
-                Sentinel<T>::store(&z[k], zk);
+                Sentinel<T>::store(&zp[pos], zk);
+                z[k] = zk;
                 if (S.trace) {
                     S.trace[3 * (long)k] = tg;
                     S.trace[3 * (long)k + 1] = td;
@@ -485,7 +486,7 @@ __global__ void __launch_bounds__(BS) k_spmv_dot(int n, const int* rs, const int
 }
 
 template <typename T>
-__global__ void __launch_bounds__(BS) k_axpy_res(int n, const T* s, const T* q, T* x, T* r, T* y, Parts Pt,
+__global__ void __launch_bounds__(BS) k_axpy_res(int n, const T* s, const T* q, T* x, T* r, Parts Pt,
                                                  const double* scal, int norm, int it) {
     if (scal[S_DONE] != 0.0) return;
     const double sq = reduce_prev<false>(Pt.sq, Pt.gs);
@@ -496,7 +497,6 @@ __global__ void __launch_bounds__(BS) k_axpy_res(int n, const T* s, const T* q, 
         x[i] += alpha * s[i];
         const T ri = r[i] - alpha * q[i];
         r[i] = ri;
-        Sentinel<T>::storeSent(&y[i]);
         const double a = (double)ri;
         v = norm == ASTG_NORM_INF ? fmax(v, fabs(a)) : v + a * a;
     }
@@ -604,6 +604,7 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
         }
     if (const char* g = getenv("ASTG_GATE_DIST")) gateDist_ = std::max(1, atoi(g));
     if (const char* g = getenv("ASTG_POLL_CAP")) pollCap_ = std::max(1, atoi(g));
+    if (const char* g = getenv("ASTG_GATE_CAP")) gateCap_ = std::max(1, atoi(g));
     int dev = 0, cus = 0;
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -628,7 +629,6 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
     HIPCHK(z_.alloc(n8));
     HIPCHK(s_.alloc(n8));
     HIPCHK(q_.alloc(n8));
-    HIPCHK(y_.alloc(n8));
     HIPCHK(scal_.alloc(32 * 8));
     HIPCHK(cnt_.alloc(C_COUNT * 4));
     if (!hscal_) HIPCHK(hipHostMalloc((void**)&hscal_, 32 * 8, hipHostMallocDefault));
@@ -696,8 +696,8 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
             if (L < 0) continue;
             const unsigned h = (unsigned)sg * 2654435761u;
             const int lf = L - gateDist_, lb = L + gateDist_;
-            if (lf >= 0) gF[sg] = padded[(size_t)levStart[lf] + h % (unsigned)levRows[lf]];
-            if (lb < depth_) gB[sg] = padded[(size_t)levStart[lb] + h % (unsigned)levRows[lb]];
+            if (lf >= 0) gF[sg] = levStart[lf] + (int)(h % (unsigned)levRows[lf]);
+            if (lb < depth_) gB[sg] = levStart[lb] + (int)(h % (unsigned)levRows[lb]);
         }
         HIPCHK(gateF_.alloc((size_t)nseg * 4));
         HIPCHK(gateB_.alloc((size_t)nseg * 4));
@@ -706,6 +706,20 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
     }
     HIPCHK(order_.alloc((size_t)P_ * 4));
     HIPCHK(hipMemcpyAsync(order_.p, padded.data(), (size_t)P_ * 4, hipMemcpyHostToDevice, stream_));
+    // The sweep vectors live in padded level order: a wavefront's dependencies
+    // sit in the segments of the previous levels in the same relative order,
+    // so its polls touch few, shared cache lines. colp = column -> position.
+    {
+        std::vector<int> pos(n), colp(nnz_);
+        for (int q = 0; q < P_; ++q)
+            if (padded[q] >= 0) pos[padded[q]] = q;
+        for (long long p = 0; p < nnz_; ++p) colp[p] = pos[col[p]];
+        HIPCHK(colp_.alloc((size_t)nnz_ * 4));
+        HIPCHK(y_.alloc((size_t)P_ * 8));
+        HIPCHK(zp_.alloc((size_t)P_ * 8));
+        HIPCHK(hipMemcpyAsync(colp_.p, colp.data(), (size_t)nnz_ * 4, hipMemcpyHostToDevice, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+    }
     // persistent grids (all blocks co-resident)
     const int nchunks = P_ / BS;
     gF64_ = coopBlocks(k_fwd<double>, cus, nchunks);
@@ -727,6 +741,7 @@ SweepPlan PcgDevice::plan() const {
     s.P = P_;
     s.nchunks = P_ / BS;
     s.pollCap = pollCap_;
+    s.gateCap = gateCap_;
     s.trace = trace_;
     return s;
 }
@@ -778,20 +793,20 @@ int PcgDevice::syncScalars(std::string& err) {
 }
 
 template <typename T>
-int PcgDevice::launchFwd(const T* F, const T* invd, const T* r, T* y, T* z, int mode, int it, int maxIt, int norm,
+int PcgDevice::launchFwd(const T* F, const T* invd, const T* r, T* y, int mode, int it, int maxIt, int norm,
                          double tolRel, std::string& err) {
     const int g = fwdGrid<T>(), gb = bwdGrid<T>();
     HIPCHK(launchCoop(k_fwd<T>, g, stream_, plan(), (const int*)rs_.as<int>(), (const int*)lend_.as<int>(),
-                      (const int*)col_.as<int>(), F, invd, r, y, z, parts(gb), cnt_.as<int>(), scal_.as<double>(),
-                      mode, it, maxIt, norm, tolRel));
+                      (const int*)colp_.as<int>(), F, invd, r, y, zp_.as<T>(), parts(gb), cnt_.as<int>(),
+                      scal_.as<double>(), mode, it, maxIt, norm, tolRel));
     return ASTG_OK;
 }
 
 template <typename T>
-int PcgDevice::launchBwd(const T* F, const T* invd, const T* y, const T* r, T* z, int parityOut, std::string& err) {
+int PcgDevice::launchBwd(const T* F, const T* invd, T* y, const T* r, T* z, int parityOut, std::string& err) {
     const int gb = bwdGrid<T>();
     HIPCHK(launchCoop(k_bwd_dot<T>, gb, stream_, plan(), (const int*)rs_.as<int>(), (const int*)ubeg_.as<int>(),
-                      (const int*)col_.as<int>(), F, invd, y, r, z, parts(gb), cnt_.as<int>(),
+                      (const int*)colp_.as<int>(), F, invd, y, r, z, zp_.as<T>(), parts(gb), cnt_.as<int>(),
                       (const double*)scal_.as<double>(), parityOut));
     return ASTG_OK;
 }
@@ -812,8 +827,9 @@ int PcgDevice::pcgInner(const T* A, const T* F, const T* invd, const T* b, T* x,
     const Parts Pt = parts(gb);
     int rc;
     k_zero64<<<1, BS, 0, stream_>>>(32, scal);
-    k_init<T><<<gs, BS, 0, stream_>>>(n, b, x, r, y, z, Pt, norm);
-    if ((rc = launchFwd<T>(F, invd, r, y, z, 0, 0, maxIt, norm, tolRel, err))) return rc;
+    k_init<T><<<gs, BS, 0, stream_>>>(n, b, x, r, Pt, norm);
+    k_fill_sent<T><<<blocks(P_), BS, 0, stream_>>>(y, P_);
+    if ((rc = launchFwd<T>(F, invd, r, y, 0, 0, maxIt, norm, tolRel, err))) return rc;
     if ((rc = launchBwd<T>(F, invd, y, r, z, 0, err))) return rc;
     k_xpby<T><<<gs, BS, 0, stream_>>>(n, z, s, Pt, scal, 0, 0);
     HIPCHK(hipGetLastError());
@@ -822,8 +838,8 @@ int PcgDevice::pcgInner(const T* A, const T* F, const T* invd, const T* b, T* x,
     for (;;) {
         for (int j = 0; j < batch && it < maxIt; ++j, ++it) {
             k_spmv_dot<T><<<gs, BS, 0, stream_>>>(n, rs, col, A, s, q, Pt, scal);
-            k_axpy_res<T><<<gs, BS, 0, stream_>>>(n, s, q, x, r, y, Pt, scal, norm, it);
-            if ((rc = launchFwd<T>(F, invd, r, y, z, 1, it, maxIt, norm, tolRel, err))) return rc;
+            k_axpy_res<T><<<gs, BS, 0, stream_>>>(n, s, q, x, r, Pt, scal, norm, it);
+            if ((rc = launchFwd<T>(F, invd, r, y, 1, it, maxIt, norm, tolRel, err))) return rc;
             if ((rc = launchBwd<T>(F, invd, y, r, z, (it + 1) & 1, err))) return rc;
             k_xpby<T><<<gs, BS, 0, stream_>>>(n, z, s, Pt, scal, 1, it);
         }
@@ -1001,20 +1017,20 @@ int PcgDevice::benchKernel(int which, int reps, double& avgMs, double& bytes, st
         k_zero64<<<1, BS, 0, stream_>>>(32, scal);
         const Parts P64 = parts(bwdGrid<double>()), P32 = parts(bwdGrid<float>());
         if (which == ASTG_K_FWD64 || which == ASTG_K_BWD64) {
-            k_fill_sent<double><<<blocks(n), BS, 0, stream_>>>(y_.as<double>(), n);
-            k_fill_sent<double><<<blocks(n), BS, 0, stream_>>>(z_.as<double>(), n);
+            k_fill_sent<double><<<blocks(P_), BS, 0, stream_>>>(y_.as<double>(), P_);
+            k_fill_sent<double><<<blocks(P_), BS, 0, stream_>>>(zp_.as<double>(), P_);
         } else if (which == ASTG_K_FWD32 || which == ASTG_K_BWD32) {
-            k_fill_sent<float><<<blocks(n), BS, 0, stream_>>>(y_.as<float>(), n);
-            k_fill_sent<float><<<blocks(n), BS, 0, stream_>>>(z_.as<float>(), n);
+            k_fill_sent<float><<<blocks(P_), BS, 0, stream_>>>(y_.as<float>(), P_);
+            k_fill_sent<float><<<blocks(P_), BS, 0, stream_>>>(zp_.as<float>(), P_);
         }
         // the forward sweep starts in mode 0, which needs nonzero ||b|| partials
         if (sweep) k_fill64<<<blocks(gs_), BS, 0, stream_>>>(gs_, P64.res, 1.0);
         if (which == ASTG_K_BWD64)
             rc = launchFwd<double>(F_.as<double>(), invd_.as<double>(), r_.as<double>(), y_.as<double>(),
-                                   z_.as<double>(), 0, 0, 1 << 30, ASTG_NORM_INF, 1e-30, err);
+                                   0, 0, 1 << 30, ASTG_NORM_INF, 1e-30, err);
         else if (which == ASTG_K_BWD32)
             rc = launchFwd<float>(F32_.as<float>(), invd32_.as<float>(), r_.as<float>(), y_.as<float>(),
-                                  z_.as<float>(), 0, 0, 1 << 30, ASTG_NORM_INF, 1e-30, err);
+                                   0, 0, 1 << 30, ASTG_NORM_INF, 1e-30, err);
         if (rc) break;
         HIPCHK(hipEventRecord(e0, stream_));
         switch (which) {
@@ -1028,11 +1044,11 @@ int PcgDevice::benchKernel(int which, int reps, double& avgMs, double& bytes, st
                 break;
             case ASTG_K_FWD64:
                 rc = launchFwd<double>(F_.as<double>(), invd_.as<double>(), r_.as<double>(), y_.as<double>(),
-                                       z_.as<double>(), 0, 0, 1 << 30, ASTG_NORM_INF, 1e-30, err);
+                                       0, 0, 1 << 30, ASTG_NORM_INF, 1e-30, err);
                 break;
             case ASTG_K_FWD32:
                 rc = launchFwd<float>(F32_.as<float>(), invd32_.as<float>(), r_.as<float>(), y_.as<float>(),
-                                      z_.as<float>(), 0, 0, 1 << 30, ASTG_NORM_INF, 1e-30, err);
+                                       0, 0, 1 << 30, ASTG_NORM_INF, 1e-30, err);
                 break;
             case ASTG_K_BWD64:
                 rc = launchBwd<double>(F_.as<double>(), invd_.as<double>(), y_.as<double>(), r_.as<double>(),

@@ -37,3 +37,27 @@ def test_bad_arguments_are_reported():
     with __import__("pytest").raises(astg.AstgError) as e:
         c.set_topology(np.full(8, 5, np.int32), 2, 2, 0)
     assert e.value.code == astg.ASTG_ERR_ARG
+
+
+SHIM = os.path.join(ROOT, "ast-grid_amd", "astg", "cuda_solver_shim")
+
+
+def write_csr(path, rs, col, val, b):
+    import numpy as np
+    with open(path, "wb") as f:
+        np.array([len(rs) - 1, len(col)], np.int32).tofile(f)
+        np.asarray(rs, np.int32).tofile(f)
+        np.asarray(col, np.int32).tofile(f)
+        np.asarray(val, np.float64).tofile(f)
+        np.asarray(b, np.float64).tofile(f)
+
+
+def test_cuda_solver_shim_is_built_and_rejects_truncated_input(tmp_path):
+    """The C++ CudaSolver-shaped shim (include/astg_cuda_solver.hpp) compiles,
+    links against libastg.so and validates its input before any device call."""
+    import subprocess
+    assert os.access(SHIM, os.X_OK)
+    bad = tmp_path / "bad.bin"
+    bad.write_bytes(b"\x03\x00\x00\x00")
+    p = subprocess.run([SHIM, str(bad), str(tmp_path / "out.bin")], capture_output=True, text=True)
+    assert p.returncode == 2 and "truncated" in p.stderr

@@ -114,3 +114,32 @@ def test_mixed_precision_reaches_tolerance(golden):
     if _closed_box(golden):
         p, pr = p - p.mean(), pr - pr.mean()
     assert rel(p, pr) < 1e-4
+
+
+def test_cuda_solver_shim_drop_in(golden):
+    """The reference's call sequence through the C++ shim (astg::HipSolver as
+    CudaSolver, ast-grid_amd/examples/cuda_solver_shim.cpp, loaded in this
+    process) on the reference's own assembled system reproduces its pressure
+    and iteration count."""
+    import ctypes
+    import os
+    import astg
+    lib = ctypes.CDLL(os.path.join(os.path.dirname(astg.LIB_PATH), "libastg_shim_example.so"))
+    n = len(golden["b"])
+    rs = np.ascontiguousarray(golden["A_rowstart"], np.int32)
+    col = np.ascontiguousarray(golden["A_col"], np.int32)
+    val = np.ascontiguousarray(golden["A_val"], np.float64)
+    b = np.ascontiguousarray(golden["b"], np.float64)
+    x = np.empty(n, np.float64)
+    err, iters = ctypes.c_double(), ctypes.c_int()
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    lib.astg_example_shim_solve.argtypes = [ctypes.c_int] + [ctypes.c_void_p] * 4 + [ctypes.c_double,
+                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    rc = lib.astg_example_shim_solve(n, P(rs), P(col), P(val), P(b), float(golden["tol"]), P(x),
+                                     ctypes.byref(err), ctypes.byref(iters))
+    assert rc == 0
+    assert abs(iters.value - int(golden["pcg_iters"])) <= 1
+    xr = golden["p"]
+    if _closed_box(golden):
+        x, xr = x - x.mean(), xr - xr.mean()
+    assert rel(x, xr) < 1e-6
