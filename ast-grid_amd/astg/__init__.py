@@ -31,7 +31,11 @@ EXPORTS = [
     "astg_level_info", "astg_get_nodes", "astg_get_tilt", "astg_get_half_tilts", "astg_get_numbering",
     "astg_get_weights", "astg_set_velocity", "astg_get_velocity", "astg_project", "astg_get_system",
     "astg_get_mic_factor", "astg_fields_snapshot", "astg_kernel_time", "astg_sweep_timeline",
+    "astg_set_scalar", "astg_get_scalar", "astg_post_project", "astg_scalar_update_ghost", "astg_get_dual",
+    "astg_sample_points", "astg_advect", "astg_step",
 ]
+FIELD_DENSITY, FIELD_HEAT = 0, 1
+SAMPLE_VELOCITY, SAMPLE_MAC, SAMPLE_BACKTRACE, SAMPLE_DENSITY, SAMPLE_HEAT = range(5)
 K_SPMV64, K_FWD64, K_BWD64, K_MIC, K_ASSEMBLE, K_SPMV32, K_FWD32, K_BWD32, K_COPY = range(9)
 
 
@@ -90,6 +94,14 @@ def lib():
         "astg_fields_snapshot": (I, [P, I]),
         "astg_kernel_time": (I, [P, I, I, P, P]),
         "astg_sweep_timeline": (I, [P, I, P, P]),
+        "astg_set_scalar": (I, [P, I, I, P, P]),
+        "astg_get_scalar": (I, [P, I, I, P, P]),
+        "astg_post_project": (I, [P]),
+        "astg_scalar_update_ghost": (I, [P, I]),
+        "astg_get_dual": (I, [P, I, P, P]),
+        "astg_sample_points": (I, [P, I, I, P, D, P]),
+        "astg_advect": (I, [P, D]),
+        "astg_step": (I, [P, D, D, ctypes.POINTER(PcgParams), ctypes.POINTER(PcgResult)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -269,6 +281,46 @@ class Context:
         ms, by = ctypes.c_double(), ctypes.c_double()
         self._chk(self._L.astg_kernel_time(self.h, which, reps, ctypes.byref(ms), ctypes.byref(by)), "kernel_time")
         return ms.value, by.value
+
+    # ---- the step around the projection and advection ----
+    def set_scalar(self, field, level, oct_, tilt):
+        o = np.ascontiguousarray(oct_, np.float64)
+        t = np.ascontiguousarray(tilt, np.float64)
+        self._chk(self._L.astg_set_scalar(self.h, field, level, _ptr(o), _ptr(t)), "set_scalar")
+
+    def scalar(self, field, level):
+        nn, _ = self.level_info(level)
+        o, t = np.empty(nn), np.empty(nn)
+        self._chk(self._L.astg_get_scalar(self.h, field, level, _ptr(o), _ptr(t)), "get_scalar")
+        return o, t
+
+    def post_project(self):
+        self._chk(self._L.astg_post_project(self.h), "post_project")
+
+    def scalar_update_ghost(self, field):
+        self._chk(self._L.astg_scalar_update_ghost(self.h, field), "scalar_update_ghost")
+
+    def dual(self, level):
+        nn, _ = self.level_info(level)
+        o, t = np.empty((nn, 3)), np.empty((nn, 3))
+        self._chk(self._L.astg_get_dual(self.h, level, _ptr(o), _ptr(t)), "get_dual")
+        return o, t
+
+    def sample_points(self, what, xyz, dt=0.0):
+        xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+        out = np.empty((len(xyz), 3)) if what <= SAMPLE_BACKTRACE else np.empty(len(xyz))
+        self._chk(self._L.astg_sample_points(self.h, what, len(xyz), _ptr(xyz), dt, _ptr(out)), "sample_points")
+        return out
+
+    def advect(self, dt):
+        self._chk(self._L.astg_advect(self.h, dt), "advect")
+
+    def step(self, dt, rho, params=None, allow_unconverged=False):
+        prm = params if params is not None else default_params()
+        res = PcgResult()
+        allow = (ASTG_ERR_NOT_CONVERGED,) if allow_unconverged else ()
+        self._chk(self._L.astg_step(self.h, dt, rho, ctypes.byref(prm), ctypes.byref(res)), "step", allow)
+        return res
 
     def sweep_timeline(self, which, n):
         """Per-row (gate, deps seen, published) device wall-clock ticks

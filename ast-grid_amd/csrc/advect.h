@@ -1,0 +1,98 @@
+// Device-side step around the projection on the AST octree: wall constraint
+// of the MAC velocity, ghost updates of the omni fields, the dual (cell-centre
+// and vertex) velocity grid, point sampling and semi-Lagrangian advection of
+// the scalar fields (density, heat) and of the velocity itself.
+//
+// Reference (file:line into the AST-Grid sources):
+//   onUpdate chain after project       src/octree_omnigrid_solver3.cpp:234-246
+//   constrainVelocity (no collider)    include/omni_boundary_solver3.h:26-162
+//   omni face ghost update             include/octree_omni_face_centered_grid3.h:97-120
+//   updateDualGrid                     src/octree_omni_face_centered_grid3.cpp:32-159
+//   omni grid ghost update             include/octree_omni_grid3.h:136-143, 287-337
+//   point location                     src/octree_grid_layout3.cpp:16-41
+//   MAC / omni velocity sampling       src/octree_face_centered_grid3.cpp:36-113,
+//                                      include/octree_omni_face_centered_grid3.h:37-80
+//   omni scalar sampling / sampleSub   include/octree_omni_grid3.h:146-286
+//   getNeighborCubeValue               include/octree_util.h:124-185
+//   BackTrace                          include/advection.h:15-64 (midpoint branch)
+//   advect                             src/octree_omnigrid_solver3.cpp:850-914,
+//                                      include/advection.h:195-239
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "octree.h"
+#include "pcg.h"
+#include "project.h"
+#include "project_host.h"
+
+namespace astg {
+
+// Device views passed to the kernels.
+struct AdvTopo {
+    const int* map;                 // concatenated dense index maps -> global node, -1
+    const uint16_t *nx, *ny, *nz;   // node coordinates at their level
+    const uint8_t* nf;              // NF_* node flags
+    const double* te;               // tilt size e
+    const uint8_t* tb;              // TB_* tilt bits
+    const int8_t* thd;              // half-tilt direction
+};
+struct AdvVel {
+    const double* mac;    // 3G axis-major
+    const double* tilt;   // 8G slot-major
+    const double* doct;   // dual cell vectors, 3 per node
+    const double* dtilt;  // dual vertex vectors, 3 per node
+};
+
+class AdvectDevice {
+public:
+    explicit AdvectDevice(hipStream_t s) : stream_(s) {}
+    int upload(const Octree& o, const ProjectTables& t, std::string& err);
+
+    // Scalar fields: 0 density, 1 heat; per level, octagon and tilt value per node.
+    int setScalar(int field, int level, const double* oct, const double* tilt, std::string& err);
+    int getScalar(int field, int level, double* oct, double* tilt, std::string& err);
+    int getDual(int level, double* oct3, double* tilt3, std::string& err);
+
+    // MAC wall constraint and velocity ghosts (onUpdate before project).
+    int constrainAndGhost(ProjectDevice& pd, std::string& err);
+    // onUpdate between project and advect: the above, then the dual grid and
+    // its ghosts.
+    int postProject(ProjectDevice& pd, std::string& err);
+    // Ghost update of one scalar field (omni grid).
+    int scalarGhost(int field, std::string& err);
+    // advect(dt) with tilt adaptivity off: scalars, their ghosts, velocity,
+    // then the 0.999 decay of both scalars.
+    int advect(ProjectDevice& pd, double dt, std::string& err);
+    // Point queries on host arrays: what = 0 omni velocity, 1 MAC velocity,
+    // 2 back-trace over dt, 3 density, 4 heat (out: 3 per point, or 1).
+    int samplePoints(ProjectDevice& pd, int what, int n, const double* xyz, double dt, double* out,
+                     std::string& err);
+    double advectBytes() const;
+
+private:
+    int omniGhost(double* cell, double* vert, int nc, std::string& err);
+    int checkBad(std::string& err);
+    AdvTopo topo() const;
+    AdvVel vel(ProjectDevice& pd) const;
+    hipStream_t stream_;
+    int L_ = 0, R_ = 0, G_ = 0;
+    double h0_ = 0;
+    int closed_ = 0;
+    std::vector<int> off_, nleaf_, nnode_;
+    int nLeaf_ = 0, nGhost_ = 0, nHalf_ = 0, nTJ_ = 0, nCBH_ = 0, nOpenTilt_ = 0, nVelTilt_ = 0, nHalfMac_ = 0;
+    DevBuf map_, nx_, ny_, nz_, nflag_, te_, tb_, thd_, level_, inHalf_;
+    DevBuf leaf_, ghost_, half_, tj_, cbh_, openTilt_, velTilt_;
+    DevBuf hmG_, hmT_, hmA_, hmS_;   // coarse MAC faces written from half-tilts
+    DevBuf bad_;
+    DevBuf scal_[2][2][2];           // [field][buffer][0 octagon, 1 tilt]
+    int cur_[2] = {0, 0};            // current buffer of each scalar field
+    DevBuf dualOct_, dualTilt_;      // 3 per node
+    DevBuf tMac_, tTilt_;            // advected velocity (swapped with the live one)
+    DevBuf pts_, out_;               // point queries
+};
+
+}  // namespace astg

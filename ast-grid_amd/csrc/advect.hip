@@ -1,0 +1,1357 @@
+// Post-projection chain, point sampling and semi-Lagrangian advection on the
+// AST octree (see advect.h for the reference map). One thread per node, DOF
+// or query point; every lookup goes through the dense per-level index maps.
+//
+// Arithmetic follows the reference expression by expression (interpolation
+// orders, the 1/sqrt(3) macro written as a division, clamping rules of the
+// two barycentric helpers) so results agree to rounding.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "advect.h"
+#include "device_common.h"
+
+#define HIPCHK(expr)                                                                 \
+    do {                                                                             \
+        hipError_t e_ = (expr);                                                      \
+        if (e_ != hipSuccess) {                                                      \
+            err = std::string(#expr) + ": " + hipGetErrorString(e_);                 \
+            return ASTG_ERR_HIP;                                                     \
+        }                                                                            \
+    } while (0)
+
+namespace astg {
+
+namespace {
+
+constexpr int BS = 256;
+constexpr int kMaxLevels = 8;
+constexpr double kSqrt3 = 1.7320508075688772;
+constexpr double kEps = 2.220446049250313e-16;  // numeric_limits<double>::epsilon
+
+inline int nblocks(long n) { return (int)((n + BS - 1) / BS); }
+
+__constant__ int c_L, c_R, c_G;
+__constant__ double c_h0;
+__constant__ int c_res[kMaxLevels];
+__constant__ int c_moff[kMaxLevels];
+__constant__ double c_hl[kMaxLevels];
+// least-squares dual-graph matrices [x][y][z][n0][n1][n2][n3] (3 x 7 each)
+__constant__ double c_dual[128 * 21];
+__constant__ double c_dualTilt[12];  // 3 x 4
+
+// OCT_NB_TILT_DXDYDZ (tilt corner k of a cell), restated from the corner order
+__constant__ int c_nbt[8][3] = {{1, 1, 1}, {0, 1, 1}, {0, 0, 1}, {1, 0, 1},
+                                {1, 1, 0}, {0, 1, 0}, {0, 0, 0}, {1, 0, 0}};
+// OCT_NB_DXDYDZ (6 axis neighbours)
+__constant__ int c_nb6[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+// HALF_TILT_SLOT_INDEX
+__constant__ int c_halfSlot[6][5] = {{0, 3, 4, 7, 1}, {0, 1, 4, 5, 2}, {0, 1, 2, 3, 5},
+                                     {1, 2, 5, 6, 0}, {2, 3, 6, 7, 4}, {4, 5, 6, 7, 3}};
+// OCTREE_SAMPLE_MARKER: 0x1000 = void axis; nibbles x,y,z: 1 = +, 2 = -
+__constant__ int c_marker[8][8] = {
+    {0x1000, 0x011, 0x001, 0x101, 0x110, 0x010, 0x1000, 0x100},
+    {0x011, 0x1000, 0x201, 0x001, 0x010, 0x210, 0x200, 0x1000},
+    {0x001, 0x201, 0x1000, 0x021, 0x1000, 0x200, 0x220, 0x020},
+    {0x101, 0x001, 0x021, 0x1000, 0x100, 0x1000, 0x020, 0x120},
+    {0x110, 0x010, 0x1000, 0x100, 0x1000, 0x012, 0x002, 0x102},
+    {0x010, 0x210, 0x200, 0x1000, 0x012, 0x1000, 0x202, 0x002},
+    {0x1000, 0x200, 0x220, 0x020, 0x002, 0x202, 0x1000, 0x022},
+    {0x100, 0x1000, 0x020, 0x120, 0x102, 0x002, 0x022, 0x1000}};
+
+typedef AdvTopo Topo;
+typedef AdvVel Vel;
+
+__device__ inline int nd(const Topo& T, int l, int x, int y, int z) {
+    if (l < 0 || l >= c_L) return -1;
+    const int r = c_res[l];
+    if (x < 0 || y < 0 || z < 0 || x >= r || y >= r || z >= r) return -1;
+    return T.map[c_moff[l] + ((size_t)z * r + y) * r + x];
+}
+__device__ inline bool ghost(const Topo& T, int g) { return (T.nf[g] & NF_GHOST) != 0; }
+__device__ inline uint8_t tbits(const Topo& T, int g) { return g >= 0 ? T.tb[g] : (uint8_t)TB_CLOSED; }
+
+// getUntil (octree_grid_layout3.cpp:51-69): first non-ghost node walking to
+// finer (toFinest) or coarser levels; -1 if none.
+__device__ int getUntil(const Topo& T, int l, int x, int y, int z, bool toFinest) {
+    const int r = c_res[l];
+    if (x < 0 || y < 0 || z < 0 || x >= r || y >= r || z >= r) return -1;
+    for (;;) {
+        const int g = nd(T, l, x, y, z);
+        if (g >= 0 && !ghost(T, g)) return g;
+        if (toFinest) {
+            x <<= 1;
+            y <<= 1;
+            z <<= 1;
+            l -= 1;
+        } else {
+            x >>= 1;
+            y >>= 1;
+            z >>= 1;
+            l += 1;
+        }
+        if (l < 0 || l >= c_L) return -1;
+    }
+}
+
+// getBarycentric (util.h:259-282) with iLow = 0
+__device__ inline void bary(double x, int iHigh, int& i, double& f) {
+    const double s = floor(x);
+    i = (int)s;
+    if (iHigh == 0) {
+        i = 0;
+        f = 0;
+    } else if (i < 0) {
+        i = 0;
+        f = 0;
+    } else if (i > iHigh - 1) {
+        i = iHigh - 1;
+        f = 1;
+    } else {
+        f = x - s;
+    }
+}
+// getBaryCentric, vector form (util.h:244-257): truncation, clamps on i and f
+__device__ inline void baryVec(double x, int high, int& i, double& f) {
+    i = (int)x;
+    i = max(i, 0);
+    i = min(i, high - 1);
+    f = x - (double)i;
+    f = fmax(f, 0.0);
+    f = fmin(f, 1.0);
+}
+
+__device__ inline double lerp(double v0, double v1, double f) { return v0 * (1 - f) + v1 * f; }
+__device__ inline double bilerp(double v00, double v01, double v11, double v10, double fx, double fy) {
+    return lerp(lerp(v00, v10, fx), lerp(v01, v11, fx), fy);
+}
+__device__ inline double trilerp(double f000, double f100, double f010, double f110, double f001, double f101,
+                                 double f011, double f111, double tx, double ty, double fz) {
+    return lerp(bilerp(f000, f010, f110, f100, tx, ty), bilerp(f001, f011, f111, f101, tx, ty), fz);
+}
+
+// levelCoordinate (octree_grid_layout3.cpp:16-41): level of the leaf holding p
+__device__ int locate(const Topo& T, double px, double py, double pz) {
+    int xi, yi, zi;
+    double f;
+    bary(px / c_h0, c_R, xi, f);
+    bary(py / c_h0, c_R, yi, f);
+    bary(pz / c_h0, c_R, zi, f);
+    for (int l = 0; l < c_L; ++l) {
+        const int g = nd(T, l, xi, yi, zi);
+        if (g >= 0 && !ghost(T, g)) return l;
+        xi >>= 1;
+        yi >>= 1;
+        zi >>= 1;
+    }
+    return -1;
+}
+
+// OctreeFaceCenteredGrid3::sampleX/Y/Z (octree_face_centered_grid3.cpp:47-113)
+__device__ double macAxis(const Topo& T, const double* mac, int a, int l, double rx, double ry, double rz) {
+    const int r = c_res[l];
+    const double rr[3] = {rx, ry, rz};
+    int i0[3], i1[3];
+    double f[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (d == a) {
+            bary(rr[d], r - 1, i0[d], f[d]);
+            i1[d] = min(i0[d] + 1, r - 1);
+        } else {
+            bary(rr[d] - 0.5, r - 2, i0[d], f[d]);
+            i1[d] = min(i0[d] + 1, r - 2);
+        }
+    }
+    const double* u = mac + (size_t)a * c_G;
+    auto U = [&](int x, int y, int z) {
+        const int g = nd(T, l, x, y, z);
+        return g >= 0 ? u[g] : 0.0;
+    };
+    return trilerp(U(i0[0], i0[1], i0[2]), U(i1[0], i0[1], i0[2]), U(i0[0], i1[1], i0[2]), U(i1[0], i1[1], i0[2]),
+                   U(i0[0], i0[1], i1[2]), U(i1[0], i0[1], i1[2]), U(i0[0], i1[1], i1[2]), U(i1[0], i1[1], i1[2]),
+                   f[0], f[1], f[2]);
+}
+
+// OctreeFaceCenteredGrid3::sample (octree_face_centered_grid3.cpp:36-45)
+__device__ bool macSample(const Topo& T, const double* mac, double px, double py, double pz, double* out) {
+    const int l = locate(T, px, py, pz);
+    if (l < 0) {
+        out[0] = out[1] = out[2] = NAN;
+        return false;
+    }
+    const double dh = c_hl[l];
+    const double rx = px / dh, ry = py / dh, rz = pz / dh;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) out[a] = macAxis(T, mac, a, l, rx, ry, rz);
+    return true;
+}
+
+// cell-centred trilinear at a given level (sampleGivenRelativePosition,
+// octree_cell_centered_grid3.h:156-177); NC values per node
+template <int NC>
+__device__ void cellRel(const Topo& T, const double* cell, int l, double rx, double ry, double rz, double* out) {
+    const int r = c_res[l];
+    int x0, y0, z0;
+    double fx, fy, fz;
+    bary(rx - 0.5, r - 2, x0, fx);
+    bary(ry - 0.5, r - 2, y0, fy);
+    bary(rz - 0.5, r - 2, z0, fz);
+    const int x1 = min(x0 + 1, r - 2), y1 = min(y0 + 1, r - 2), z1 = min(z0 + 1, r - 2);
+    const int g[8] = {nd(T, l, x0, y0, z0), nd(T, l, x1, y0, z0), nd(T, l, x0, y1, z0), nd(T, l, x1, y1, z0),
+                      nd(T, l, x0, y0, z1), nd(T, l, x1, y0, z1), nd(T, l, x0, y1, z1), nd(T, l, x1, y1, z1)};
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        double v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = g[k] >= 0 ? cell[(size_t)NC * g[k] + c] : 0.0;
+        out[c] = trilerp(v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], fx, fy, fz);
+    }
+}
+
+// OctreeCellCenteredGrid3::sample (octree_cell_centered_grid3.h:97-123)
+template <int NC>
+__device__ void cellSample(const Topo& T, const double* cell, double px, double py, double pz, double* out) {
+    const int l = locate(T, px, py, pz);
+    if (l < 0) {
+        for (int c = 0; c < NC; ++c) out[c] = NAN;
+        return;
+    }
+    const double dh = c_hl[l];
+    cellRel<NC>(T, cell, l, px / dh, py / dh, pz / dh, out);
+}
+
+__device__ inline double cellv(const Topo& T, const double* cell, int NC, int l, int x, int y, int z, int c) {
+    const int g = nd(T, l, x, y, z);
+    return g >= 0 ? cell[(size_t)NC * g + c] : 0.0;
+}
+
+// getNeighborCubeValue (octree_util.h:124-185)
+template <int NC>
+__device__ void neighborCube(const Topo& T, const double* cell, const double* vert, int l, int xi, int yi, int zi,
+                             int dir, double (*v)[NC], bool tjoint) {
+    const int reDir = ((dir & 1) ? 8 : 6) - dir;
+    double centerV[NC];
+    const int gc = nd(T, l, xi, yi, zi);
+    for (int c = 0; c < NC; ++c) centerV[c] = gc >= 0 ? cell[(size_t)NC * gc + c] : 0.0;
+    for (int c = 0; c < NC; ++c) v[reDir][c] = centerV[c];
+    const int r = c_res[l];
+    for (int i = 0; i < 8; ++i) {
+        const int marker = c_marker[dir][i];
+        if (marker == 0x1000) continue;
+        const int m[3] = {(marker >> 8) & 0xf, (marker >> 4) & 0xf, marker & 0xf};
+        if (tjoint && l > 0) {
+            bool flag = false;
+            for (int k = 0; k < 3; ++k) {
+                const int k1 = (k + 1) % 3, k2 = (k + 2) % 3;
+                if (m[k] != 0 && m[k1] == 0 && m[k2] == 0) {  // face node
+                    int id[3] = {xi, yi, zi};
+                    id[k] += (m[k] == 0x1);
+                    id[0] *= 2;
+                    id[1] *= 2;
+                    id[2] *= 2;
+                    id[k1] += 1;
+                    id[k2] += 1;
+                    const int g = nd(T, l - 1, id[0], id[1], id[2]);
+                    if (g >= 0 && (T.tb[g] & TB_HALF)) {
+                        for (int c = 0; c < NC; ++c) v[i][c] = vert[(size_t)NC * g + c];
+                        flag = true;
+                    }
+                }
+                if (m[k] == 0 && m[k1] != 0 && m[k2] != 0) {  // edge node
+                    const int id0 = (xi << 1) + (dir == 0 || dir == 3 || dir == 4 || dir == 7);
+                    const int id1 = (yi << 1) + ((dir % 4) < 2);
+                    const int id2 = (zi << 1) + (dir < 4);
+                    const int g = nd(T, l - 1, id0 + c_nbt[i][0], id1 + c_nbt[i][1], id2 + c_nbt[i][2]);
+                    if (g >= 0 && !ghost(T, g) && (T.tb[g] & TB_CLOSED_BY_TJ)) {
+                        for (int c = 0; c < NC; ++c) v[i][c] = vert[(size_t)NC * g + c];
+                        flag = true;
+                    }
+                }
+            }
+            if (flag) continue;
+        }
+        double tv[NC];
+        for (int c = 0; c < NC; ++c) tv[c] = centerV[c];
+        double count = 1;
+        for (int x = -(m[0] == 0x2); x <= +(m[0] == 0x1); ++x)
+            for (int y = -(m[1] == 0x2); y <= +(m[1] == 0x1); ++y)
+                for (int z = -(m[2] == 0x2); z <= +(m[2] == 0x1); ++z) {
+                    if (x == 0 && y == 0 && z == 0) continue;
+                    // OctreeCellCenteredGrid3::clamp: indices into [0, res-2]
+                    const int cx = min(max(0, xi + x), r - 2), cy = min(max(0, yi + y), r - 2),
+                              cz = min(max(0, zi + z), r - 2);
+                    for (int c = 0; c < NC; ++c) tv[c] += cellv(T, cell, NC, l, cx, cy, cz, c);
+                    count += 1;
+                }
+        for (int c = 0; c < NC; ++c) v[i][c] = tv[c] / count;
+    }
+}
+
+__device__ inline int region3D(double x, double y, double z) {
+    int dir = z >= 0.5 ? 0 : 4;
+    dir += (x >= 0.5) ? (y >= 0.5 ? 0 : 3) : (y >= 0.5 ? 1 : 2);
+    return dir;
+}
+
+// OctreeOmniGrid3<T>::sampleSub (octree_omni_grid3.h:242-286)
+template <int NC>
+__device__ void sampleSub(const Topo& T, const double* cell, const double* vert, double px, double py, double pz,
+                          double* out) {
+    const int l = locate(T, px, py, pz);
+    if (l < 0) {
+        for (int c = 0; c < NC; ++c) out[c] = NAN;
+        return;
+    }
+    const double dh = c_hl[l];
+    const int r = c_res[l];
+    const double rx = px / dh, ry = py / dh, rz = pz / dh;
+    int xi, yi, zi;
+    double fx, fy, fz;
+    bary(rx, r - 1, xi, fx);
+    bary(ry, r - 1, yi, fy);
+    bary(rz, r - 1, zi, fz);
+    const int reg = region3D(fx, fy, fz);
+    const int gt = nd(T, l, xi + c_nbt[reg][0], yi + c_nbt[reg][1], zi + c_nbt[reg][2]);
+    const uint8_t b = tbits(T, gt);
+    if (!(b & TB_HALF) && !(b & TB_CLOSED_BY_TJ)) {
+        if (b & TB_CLOSED) {
+            cellRel<NC>(T, cell, l, rx, ry, rz, out);
+            return;
+        }
+    }
+    double v[8][NC];
+    for (int c = 0; c < NC; ++c) v[reg][c] = gt >= 0 ? vert[(size_t)NC * gt + c] : 0.0;
+    neighborCube<NC>(T, cell, vert, l, xi, yi, zi, reg, v, (b & TB_TJOINT) != 0);
+    switch (reg) {
+        case 0: fx -= 0.5, fy -= 0.5, fz -= 0.5; break;
+        case 1: fy -= 0.5, fz -= 0.5; break;
+        case 2: fz -= 0.5; break;
+        case 3: fx -= 0.5, fz -= 0.5; break;
+        case 4: fx -= 0.5, fy -= 0.5; break;
+        case 5: fy -= 0.5; break;
+        case 6: break;
+        case 7: fx -= 0.5; break;
+    }
+    fx *= 2;
+    fy *= 2;
+    fz *= 2;
+    for (int c = 0; c < NC; ++c)
+        out[c] = trilerp(v[6][c], v[7][c], v[5][c], v[4][c], v[2][c], v[3][c], v[1][c], v[0][c], fx, fy, fz);
+}
+
+// level-transition test shared by the two correction-based samplers
+__device__ inline bool nearTransition(const Topo& T, int l, int xi, int yi, int zi) {
+    const int g0 = nd(T, l, xi, yi, zi);
+    return (g0 >= 0 && ghost(T, g0)) || (l == 0 && nd(T, l + 1, xi >> 1, yi >> 1, zi >> 1) >= 0) ||
+           (l == 1 && nd(T, l - 1, (xi << 1) + 1, (yi << 1) + 1, (zi << 1) + 1) >= 0) ||
+           (l == 1 && nd(T, l - 1, xi << 1, yi << 1, zi << 1) >= 0);
+}
+
+// OctreeOmniFaceCenteredGrid3::sample, correction-based branch
+// (octree_omni_face_centered_grid3.h:37-80)
+__device__ void velSample(const Topo& T, const Vel& V, double px, double py, double pz, double* out) {
+    const int l = locate(T, px, py, pz);
+    if (l < 0) {
+        out[0] = out[1] = out[2] = NAN;
+        return;
+    }
+    const double dh = c_hl[l];
+    const int r = c_res[l];
+    int xi, yi, zi;
+    double fx, fy, fz;
+    baryVec(px / dh - 0.5, r - 1, xi, fx);
+    baryVec(py / dh - 0.5, r - 1, yi, fy);
+    baryVec(pz / dh - 0.5, r - 1, zi, fz);
+    const int gt = nd(T, l, xi + 1, yi + 1, zi + 1);
+    const uint8_t b = tbits(T, gt);
+    if (nearTransition(T, l, xi, yi, zi)) {
+        sampleSub<3>(T, V.doct, V.dtilt, px, py, pz, out);
+        return;
+    }
+    double v[3];
+    macSample(T, V.mac, px, py, pz, v);
+    if (!(b & TB_HALF) && (b & TB_CLOSED)) {
+        out[0] = v[0], out[1] = v[1], out[2] = v[2];
+        return;
+    }
+    double mt[3];
+    macSample(T, V.mac, (double)(xi + 1) * dh, (double)(yi + 1) * dh, (double)(zi + 1) * dh, mt);
+    const double w = fmin(fmin(fmin(fx, fy), fz), fmin(fmin(1 - fx, 1 - fy), 1 - fz));
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const double diff = (gt >= 0 ? V.dtilt[3 * (size_t)gt + c] : 0.0) - mt[c];
+        out[c] = v[c] + diff * 2 * w;
+    }
+}
+
+// OctreeOmniGrid3<REAL>::sample, correction-based branch (octree_omni_grid3.h:146-191)
+__device__ double scalarSample(const Topo& T, const double* cell, const double* vert, double px, double py,
+                               double pz) {
+    const int l = locate(T, px, py, pz);
+    if (l < 0) return NAN;
+    const double dh = c_hl[l];
+    const int r = c_res[l];
+    int xi, yi, zi;
+    double fx, fy, fz;
+    bary(px / dh - 0.5, r - 2, xi, fx);
+    bary(py / dh - 0.5, r - 2, yi, fy);
+    bary(pz / dh - 0.5, r - 2, zi, fz);
+    const int x1 = min(xi + 1, r - 2), y1 = min(yi + 1, r - 2), z1 = min(zi + 1, r - 2);
+    const double o000 = cellv(T, cell, 1, l, xi, yi, zi, 0), o100 = cellv(T, cell, 1, l, x1, yi, zi, 0),
+                 o010 = cellv(T, cell, 1, l, xi, y1, zi, 0), o110 = cellv(T, cell, 1, l, x1, y1, zi, 0),
+                 o001 = cellv(T, cell, 1, l, xi, yi, z1, 0), o101 = cellv(T, cell, 1, l, x1, yi, z1, 0),
+                 o011 = cellv(T, cell, 1, l, xi, y1, z1, 0), o111 = cellv(T, cell, 1, l, x1, y1, z1, 0);
+    const double ret = trilerp(o000, o100, o010, o110, o001, o101, o011, o111, fx, fy, fz);
+    const int gt = nd(T, l, xi + 1, yi + 1, zi + 1);
+    const uint8_t b = tbits(T, gt);
+    if (nearTransition(T, l, xi, yi, zi)) {
+        double s;
+        sampleSub<1>(T, cell, vert, px, py, pz, &s);
+        return s;
+    }
+    if ((b & TB_CLOSED) && !(b & TB_HALF)) return ret;
+    const double diff = (gt >= 0 ? vert[gt] : 0.0) -
+                        (1 / 8.0) * (o000 + o100 + o010 + o110 + o001 + o101 + o011 + o111);
+    const double w = fmin(fmin(fmin(fx, fy), fz), fmin(fmin(1 - fx, 1 - fy), 1 - fz));
+    return ret + diff * 2.0 * w;
+}
+
+// BackTrace, midpoint branch (advection.h:45-62); no collider, so the SDF
+// sign test never fires
+__device__ void backTrace(const Topo& T, const Vel& V, const double* p, double dt, double* out) {
+    double v[3], mid[3];
+    velSample(T, V, p[0], p[1], p[2], v);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) mid[c] = p[c] - v[c] * dt * 0.5;
+    velSample(T, V, mid[0], mid[1], mid[2], v);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) out[c] = p[c] - v[c] * dt;
+}
+
+__device__ inline void nodeXYZ(const Topo& T, int g, int& x, int& y, int& z) {
+    x = T.nx[g];
+    y = T.ny[g];
+    z = T.nz[g];
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+// constrainVelocity(MAC) without collider: normal faces on closed walls -> 0
+__global__ void k_wall(Topo T, const uint8_t* level, double* mac, int closed) {
+    const int g = blockIdx.x * BS + threadIdx.x;
+    if (g >= c_G) return;
+    const int l = level[g], r = c_res[l];
+    int x, y, z;
+    nodeXYZ(T, g, x, y, z);
+    if (((closed & ASTG_WALL_XPOS) && x == r - 1) || ((closed & ASTG_WALL_XNEG) && x == 0)) mac[g] = 0.0;
+    if (((closed & ASTG_WALL_YPOS) && y == r - 1) || ((closed & ASTG_WALL_YNEG) && y == 0)) mac[c_G + g] = 0.0;
+    if (((closed & ASTG_WALL_ZPOS) && z == r - 1) || ((closed & ASTG_WALL_ZNEG) && z == 0)) mac[2 * c_G + g] = 0.0;
+}
+
+__global__ void k_mac_ghost_clear(const int* ghostList, int n, double* mac) {
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= n) return;
+    const int g = ghostList[t];
+    mac[g] = 0.0;
+    mac[c_G + g] = 0.0;
+    mac[2 * c_G + g] = 0.0;
+}
+
+// ghost MAC faces resampled at their positions; fromCoarse selects the pass
+__global__ void k_mac_ghost_sample(Topo T, const uint8_t* level, const int* ghostList, int n, double* mac,
+                                   int fromCoarse) {
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= n) return;
+    const int g = ghostList[t];
+    if (((T.nf[g] & NF_FROM_COARSE) != 0) != (fromCoarse != 0)) return;
+    const int l = level[g];
+    const double h = c_hl[l];
+    int x, y, z;
+    nodeXYZ(T, g, x, y, z);
+    const double bx = x * h, by = y * h, bz = z * h;
+    double v[3];
+    for (int a = 0; a < 3; ++a) {
+        const double px = bx + (a == 0 ? 0.0 : 0.5) * h, py = by + (a == 1 ? 0.0 : 0.5) * h,
+                     pz = bz + (a == 2 ? 0.0 : 0.5) * h;
+        macSample(T, mac, px, py, pz, v);
+        mac[(size_t)a * c_G + g] = v[a];
+    }
+}
+
+// updateGhostFromHalfTilt: coarse face <- half-tilt face slot (one writer per face)
+__global__ void k_mac_from_half(const int* halfG, const int* target, const int* axis, const int* slot, int n,
+                                const double* tilt, double* mac) {
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= n) return;
+    mac[(size_t)axis[t] * c_G + target[t]] = tilt[(size_t)slot[t] * c_G + halfG[t]];
+}
+
+// OctreeOmniTiltGrid3::updateGhost, half-tilts kept
+__global__ void k_tilt_ghost(Topo T, const uint8_t* level, const int* ghostList, int n, const uint8_t* inHalf,
+                             double* tilt) {
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= n) return;
+    const int g = ghostList[t];
+    if ((T.tb[g] & TB_HALF) || inHalf[g]) return;
+    const int l = level[g];
+    int x, y, z;
+    nodeXYZ(T, g, x, y, z);
+    int src;
+    if (T.nf[g] & NF_FROM_COARSE) {
+        if ((x & 1) || (y & 1) || (z & 1)) {
+            for (int k = 0; k < 8; ++k) tilt[(size_t)k * c_G + g] = 0.0;
+            return;
+        }
+        src = nd(T, l + 1, x >> 1, y >> 1, z >> 1);
+    } else {
+        src = nd(T, l - 1, x << 1, y << 1, z << 1);
+    }
+    for (int k = 0; k < 8; ++k) tilt[(size_t)k * c_G + g] = src >= 0 ? tilt[(size_t)k * c_G + src] : 0.0;
+}
+
+// updateDualGrid, tilt part
+__global__ void k_dual_tilt(Topo T, const int* leafList, int n, const double* tilt, double* dtilt) {
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= n) return;
+    const int g = leafList[t];
+    double* o = dtilt + 3 * (size_t)g;
+    if (T.tb[g] & TB_CLOSED) {
+        o[0] = o[1] = o[2] = 0.0;
+        return;
+    }
+    double gv[8];
+    for (int k = 0; k < 8; ++k) gv[k] = tilt[(size_t)k * c_G + g];
+    const double v[4] = {(gv[0] + gv[6]) * 0.5, (gv[1] + gv[7]) * 0.5, (gv[2] + gv[4]) * 0.5, (gv[3] + gv[5]) * 0.5};
+    for (int r = 0; r < 3; ++r) {
+        double s = 0;
+        for (int j = 0; j < 4; ++j) s += c_dualTilt[r * 4 + j] * v[j];
+        o[r] = s;
+    }
+}
+
+__device__ inline void dualApply(int idx, const double* v, double* o) {
+    const double* M = c_dual + idx * 21;
+    for (int r = 0; r < 3; ++r) {
+        double s = 0;
+        for (int j = 0; j < 7; ++j) s += M[r * 7 + j] * v[j];
+        o[r] = s;
+    }
+}
+__device__ inline int dualIndex(int x, int y, int z, int n0, int n1, int n2, int n3) {
+    return ((((((x * 2 + y) * 2 + z) * 2 + n0) * 2 + n1) * 2 + n2) * 2 + n3);
+}
+
+// updateDualGrid, octagon part (octree_omni_face_centered_grid3.cpp:50-128)
+__global__ void k_dual_oct(Topo T, const uint8_t* level, const int* leafList, int n, const double* mac,
+                           const double* tilt, double* doct) {
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= n) return;
+    const int g = leafList[t];
+    const int l = level[g], r = c_res[l];
+    int x, y, z;
+    nodeXYZ(T, g, x, y, z);
+    double* o = doct + 3 * (size_t)g;
+    if (!(x < r - 1 && y < r - 1 && z < r - 1)) {
+        o[0] = o[1] = o[2] = 0.0;
+        return;
+    }
+    const double dh = c_hl[l];
+    const double s3h = kSqrt3 * dh, hs3h = s3h * 0.5;
+    int nj[8], tg[8];
+    for (int i = 0; i < 8; ++i) {
+        tg[i] = nd(T, l, x + c_nbt[i][0], y + c_nbt[i][1], z + c_nbt[i][2]);
+        const uint8_t b = tbits(T, tg[i]);
+        const double e = tg[i] >= 0 ? T.te[tg[i]] : 0.0;
+        nj[i] = ((b & TB_TJOINT) && fabs(e - dh) < kEps) || (b & TB_HALF);
+    }
+    double v[7] = {0, 0, 0, 0, 0, 0, 0};
+    int cnt[7] = {0, 0, 0, 0, 0, 0, 0};
+    auto MX = [&](int a, int xx, int yy, int zz) {
+        const int q = nd(T, l, xx, yy, zz);
+        return q >= 0 ? mac[(size_t)a * c_G + q] : 0.0;
+    };
+    if (!((nj[1] && nj[6]) || (nj[2] && nj[5]))) v[0] += MX(0, x, y, z), cnt[0]++;
+    if (!((nj[0] && nj[7]) || (nj[3] && nj[4]))) v[0] += MX(0, x + 1, y, z), cnt[0]++;
+    if (!((nj[2] && nj[7]) || (nj[3] && nj[6]))) v[1] += MX(1, x, y, z), cnt[1]++;
+    if (!((nj[0] && nj[5]) || (nj[1] && nj[4]))) v[1] += MX(1, x, y + 1, z), cnt[1]++;
+    if (!((nj[4] && nj[6]) || (nj[5] && nj[7]))) v[2] += MX(2, x, y, z), cnt[2]++;
+    if (!((nj[0] && nj[2]) || (nj[1] && nj[3]))) v[2] += MX(2, x, y, z + 1), cnt[2]++;
+    for (int i = 0; i < 4; ++i) {
+        const int slot = i, reSlot = ((i & 1) ? 8 : 6) - i, vs = i + 3;
+        const int gn = tg[slot], gr = tg[reSlot];
+        const uint8_t bn = tbits(T, gn), br = tbits(T, gr);
+        const double en = gn >= 0 ? T.te[gn] : 0.0, er = gr >= 0 ? T.te[gr] : 0.0;
+        if (!(bn & TB_CLOSED) || (bn & TB_HALF)) {
+            v[vs] += gn >= 0 ? tilt[(size_t)reSlot * c_G + gn] : 0.0;
+            cnt[vs]++;
+        }
+        if (!(br & TB_CLOSED) || (br & TB_HALF)) {
+            const double tv = gr >= 0 ? tilt[(size_t)slot * c_G + gr] : 0.0;
+            // INV_SQRT3 is the unparenthesised macro 1.0 / SQRT3: e * INV_SQRT3 == e * 1.0 / SQRT3
+            v[vs] = cnt[vs] == 0 ? tv
+                                 : ((v[vs] * (hs3h - er * 1.0 / kSqrt3) + tv * (hs3h - en * 1.0 / kSqrt3)) /
+                                    (s3h - en * 1.0 / kSqrt3 - er * 1.0 / kSqrt3));
+            cnt[vs]++;
+        }
+    }
+    for (int i = 0; i < 3; ++i) v[i] = v[i] / (cnt[i] + kEps);
+    const int idx = dualIndex(cnt[0] > 0, cnt[1] > 0, cnt[2] > 0, cnt[3] > 1 || nj[0] || nj[6],
+                              cnt[4] > 1 || nj[1] || nj[7], cnt[5] > 1 || nj[2] || nj[4], cnt[6] > 1 || nj[3] || nj[5]);
+    dualApply(idx, v, o);
+}
+
+// updateDualGrid, half-tilt part
+__global__ void k_dual_half(Topo T, const int* halfList, int n, const double* tilt, double* dtilt) {
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= n) return;
+    const int g = halfList[t];
+    const int dir = T.thd[g];
+    const int axis = dir % 3;
+    double v[7] = {0, 0, 0, 0, 0, 0, 0};
+    v[axis] = tilt[(size_t)c_halfSlot[dir][4] * c_G + g];
+    for (int i = 0; i < 4; ++i) {
+        const int slot = c_halfSlot[dir][i];
+        const int reSlot = ((slot & 1) ? 8 : 6) - slot;
+        const int id = slot < reSlot ? slot : reSlot;
+        v[id + 3] = tilt[(size_t)slot * c_G + g];
+    }
+    dualApply(dualIndex(axis == 0, axis == 1, axis == 2, 1, 1, 1, 1), v, dtilt + 3 * (size_t)g);
+}
+
+// ---- omni grid ghost chain (octree_omni_grid3.h:136-143), NC values per node
+template <int NC>
+__global__ void k_cell_ghost_fine(Topo T, const uint8_t* level, const int* ghostList, int n, double* cell) {
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= n) return;
+    const int g = ghostList[t];
+    if (T.nf[g] & NF_FROM_COARSE) return;
+    const int l = level[g];
+    int x, y, z;
+    nodeXYZ(T, g, x, y, z);
+    x <<= 1;
+    y <<= 1;
+    z <<= 1;
+    for (int c = 0; c < NC; ++c) {
+        double v = 0;
+        for (int i = 0; i <= 1; ++i)
+            for (int j = 0; j <= 1; ++j)
+                for (int k = 0; k <= 1; ++k) v += cellv(T, cell, NC, l - 1, x + i, y + j, z + k, c);
+        cell[(size_t)NC * g + c] = v * (1 / 8.0);
+    }
+}
+
+template <int NC>
+__global__ void k_cell_ghost_coarse(Topo T, const uint8_t* level, const int* ghostList, int n, double* cell) {
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= n) return;
+    const int g = ghostList[t];
+    if (!(T.nf[g] & NF_FROM_COARSE)) return;
+    const int l = level[g];
+    const double h = c_hl[l];
+    int x, y, z;
+    nodeXYZ(T, g, x, y, z);
+    double out[NC];
+    cellSample<NC>(T, cell, x * h + 0.5 * h, y * h + 0.5 * h, z * h + 0.5 * h, out);
+    for (int c = 0; c < NC; ++c) cell[(size_t)NC * g + c] = out[c];
+}
+
+template <int NC>
+__global__ void k_vert_ghost(Topo T, const uint8_t* level, const int* ghostList, int n, const uint8_t* inHalf,
+                             double* vert) {
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= n) return;
+    const int g = ghostList[t];
+    if (inHalf[g]) return;
+    const int l = level[g];
+    int x, y, z;
+    nodeXYZ(T, g, x, y, z);
+    if (T.nf[g] & NF_FROM_COARSE) {
+        double v[NC];
+        for (int c = 0; c < NC; ++c) v[c] = 0;
+        int count = 0;
+        for (int dx = 0; dx <= (x & 1); ++dx)
+            for (int dy = 0; dy <= (y & 1); ++dy)
+                for (int dz = 0; dz <= (z & 1); ++dz) {
+                    const int q = getUntil(T, l + 1, (x >> 1) + dx, (y >> 1) + dy, (z >> 1) + dz, true);
+                    if (q >= 0) {
+                        for (int c = 0; c < NC; ++c) v[c] += vert[(size_t)NC * q + c];
+                        count += 1;
+                    }
+                }
+        for (int c = 0; c < NC; ++c) vert[(size_t)NC * g + c] = v[c] * (1.0 / (count + kEps));
+    } else {
+        const int q = getUntil(T, l - 1, x << 1, y << 1, z << 1, true);
+        for (int c = 0; c < NC; ++c) vert[(size_t)NC * g + c] = q >= 0 ? vert[(size_t)NC * q + c] : 0.0;
+    }
+}
+
+// updateCellGhost (octree_omni_grid3.h:287-302)
+template <int NC>
+__global__ void k_cell_ghost_tj(Topo T, const uint8_t* level, const int* ghostList, int n, const double* vert,
+                                double* cell) {
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= n) return;
+    const int g = ghostList[t];
+    if (!(T.nf[g] & NF_FROM_COARSE)) return;
+    const int l = level[g], r = c_res[l];
+    int x, y, z;
+    nodeXYZ(T, g, x, y, z);
+    if (!(x < r - 1 && y < r - 1 && z < r - 1)) return;
+    const int gd = nd(T, l, x + (x & 1), y + (y & 1), z + (z & 1));
+    const int gp = nd(T, l + 1, x >> 1, y >> 1, z >> 1);
+    for (int c = 0; c < NC; ++c) {
+        const double a = gd >= 0 ? vert[(size_t)NC * gd + c] : 0.0;
+        const double b = gp >= 0 ? cell[(size_t)NC * gp + c] : 0.0;
+        cell[(size_t)NC * g + c] = 0.5 * (a + b);
+    }
+}
+
+// getTiltFromOctagonNeighbor3 (octree_util.h:94-112)
+template <int NC>
+__device__ void tiltFromOctagons(const Topo& T, const double* cell, int l, int x, int y, int z, double* out) {
+    const int r = c_res[l];
+    for (int c = 0; c < NC; ++c) out[c] = cellv(T, cell, NC, l, x, y, z, c);
+    double count = 1;
+    for (int m = -1; m <= 0; ++m)
+        for (int nn = -1; nn <= 0; ++nn)
+            for (int k = -1; k <= 0; ++k) {
+                if (m == 0 && nn == 0 && k == 0) continue;
+                const int xx = x + m, yy = y + nn, zz = z + k;
+                if (xx >= 0 && yy >= 0 && zz >= 0 && xx < r - 1 && yy < r - 1 && zz < r - 1 &&
+                    nd(T, l, xx, yy, zz) >= 0) {
+                    for (int c = 0; c < NC; ++c) out[c] += cellv(T, cell, NC, l, xx, yy, zz, c);
+                    count += 1;
+                }
+            }
+    for (int c = 0; c < NC; ++c) out[c] = out[c] / count;
+}
+
+// interpolateClosedJoint (octree_omni_grid3.h:305-337), T-joint part
+template <int NC>
+__global__ void k_closed_joint(Topo T, const uint8_t* level, const int* tjList, int n, const double* cell,
+                               double* vert) {
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= n) return;
+    const int g = tjList[t];
+    int l = level[g];
+    int x, y, z;
+    nodeXYZ(T, g, x, y, z);
+    if (T.te[g] < c_hl[l]) {
+        l -= 1;
+        x <<= 1;
+        y <<= 1;
+        z <<= 1;
+    }
+    for (int i = 0; i < 6; ++i) {
+        const int xx = x + c_nb6[i][0], yy = y + c_nb6[i][1], zz = z + c_nb6[i][2];
+        const int q = nd(T, l, xx, yy, zz);
+        if (q >= 0) {
+            double v[NC];
+            tiltFromOctagons<NC>(T, cell, l, xx, yy, zz, v);
+            for (int c = 0; c < NC; ++c) vert[(size_t)NC * q + c] = v[c];
+        }
+    }
+}
+
+template <int NC>
+__global__ void k_closed_by_half(Topo T, const uint8_t* level, const int* list, int n, const double* cell,
+                                 double* vert, int* bad) {
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= n) return;
+    const int g = list[t];
+    if (!((T.tb[g] & TB_CLOSED) && T.thd[g] == -1)) {
+        atomicOr(bad, 1);
+        return;
+    }
+    int x, y, z;
+    nodeXYZ(T, g, x, y, z);
+    double v[NC];
+    tiltFromOctagons<NC>(T, cell, level[g], x, y, z, v);
+    for (int c = 0; c < NC; ++c) vert[(size_t)NC * g + c] = v[c];
+}
+
+// ---- advection (octree_omnigrid_solver3.cpp:850-914, advection.h:195-239)
+// kind 0: octagon DOF (cell centre), 1: tilt / half-tilt DOF (vertex)
+__global__ void k_adv_scalar(Topo T, const uint8_t* level, Vel V, const int* list, int n, int kind, double dt,
+                             const double* d0c, const double* d0v, const double* h0c, const double* h0v, double* d1,
+                             double* h1) {
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= n) return;
+    const int g = list[t];
+    const int l = level[g];
+    const double h = c_hl[l];
+    int x, y, z;
+    nodeXYZ(T, g, x, y, z);
+    double p[3] = {x * h, y * h, z * h};
+    if (kind == 0) {
+        p[0] += 0.5 * h;
+        p[1] += 0.5 * h;
+        p[2] += 0.5 * h;
+    }
+    double q[3];
+    backTrace(T, V, p, dt, q);
+    d1[g] = scalarSample(T, d0c, d0v, q[0], q[1], q[2]);
+    h1[g] = scalarSample(T, h0c, h0v, q[0], q[1], q[2]);
+}
+
+__global__ void k_adv_mac(Topo T, const uint8_t* level, Vel V, const int* leafList, int n, double dt, double* mac1) {
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= 3 * n) return;
+    const int a = t / n, g = leafList[t - a * n];
+    const int l = level[g];
+    const double h = c_hl[l];
+    int x, y, z;
+    nodeXYZ(T, g, x, y, z);
+    const double p[3] = {x * h + (a == 0 ? 0.0 : 0.5) * h, y * h + (a == 1 ? 0.0 : 0.5) * h,
+                         z * h + (a == 2 ? 0.0 : 0.5) * h};
+    double q[3], v[3];
+    backTrace(T, V, p, dt, q);
+    velSample(T, V, q[0], q[1], q[2], v);
+    mac1[(size_t)a * c_G + g] = v[a];
+}
+
+// positionTilt (octree_omni_tilt_grid3.h:33-40)
+__device__ inline void tiltPos(const double* c, int k, double e, double* p) {
+    const int k4 = k & 0x3;
+    p[0] = c[0] + ((k4 == 0 || k4 == 3) ? 1 : -1) * e / 3.0;
+    p[1] = c[1] + (k4 < 2 ? 1 : -1) * e / 3.0;
+    p[2] = c[2] + (k < 4 ? 1 : -1) * e / 3.0;
+}
+// TILT3_UNIT_DIR_UP[k] components as signs of 1/sqrt(3)
+__device__ inline void tiltDir(int k, double* d) {
+    const double s = 1.0 / kSqrt3;
+    const int sx[8] = {1, -1, -1, 1, -1, 1, 1, -1}, sy[8] = {1, 1, -1, -1, -1, -1, 1, 1};
+    d[0] = sx[k] * s;
+    d[1] = sy[k] * s;
+    d[2] = s;
+}
+
+__global__ void k_adv_tilt(Topo T, const uint8_t* level, Vel V, const int* list, int n, double dt, double* tilt1) {
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= 8 * n) return;
+    const int g = list[t >> 3], k = t & 7;
+    const int l = level[g];
+    const double h = c_hl[l];
+    int x, y, z;
+    nodeXYZ(T, g, x, y, z);
+    const double c[3] = {x * h, y * h, z * h};
+    double p[3], q[3], v[3], d[3];
+    tiltPos(c, k, T.te[g], p);
+    backTrace(T, V, p, dt, q);
+    velSample(T, V, q[0], q[1], q[2], v);
+    tiltDir(k, d);
+    tilt1[(size_t)k * c_G + g] = v[0] * d[0] + v[1] * d[1] + v[2] * d[2];
+}
+
+__global__ void k_adv_half(Topo T, const uint8_t* level, Vel V, const int* halfList, int n, double dt,
+                           double* tilt1) {
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= n) return;
+    const int g = halfList[t];
+    const int dir = T.thd[g];
+    const int l = level[g];
+    const double h = c_hl[l];
+    int x, y, z;
+    nodeXYZ(T, g, x, y, z);
+    const double c[3] = {x * h, y * h, z * h};
+    double nv[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < 5; ++i) {
+        const int k = c_halfSlot[dir][i];
+        double p[3], q[3], v[3], d[3];
+        if (i == 4) {
+            p[0] = c[0], p[1] = c[1], p[2] = c[2];
+            // HALF_TILT_XYZ_DIR_POSITIVE[k]
+            const int ax = (k == 0 || k == 1) ? 0 : (k == 2 || k == 4) ? 1 : 2;
+            d[0] = ax == 0, d[1] = ax == 1, d[2] = ax == 2;
+        } else {
+            tiltPos(c, k, T.te[g], p);
+            tiltDir(k, d);
+        }
+        backTrace(T, V, p, dt, q);
+        velSample(T, V, q[0], q[1], q[2], v);
+        nv[k] = v[0] * d[0] + v[1] * d[1] + v[2] * d[2];
+    }
+    for (int k = 0; k < 8; ++k) tilt1[(size_t)k * c_G + g] = nv[k];
+}
+
+__global__ void k_scale2(long n, double s, double* a, double* b) {
+    const long i = (long)blockIdx.x * BS + threadIdx.x;
+    if (i < n) {
+        a[i] *= s;
+        b[i] *= s;
+    }
+}
+
+__global__ void k_points(Topo T, Vel V, const double* cell, const double* vert, int what, int n, const double* xyz,
+                         double dt, double* out) {
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= n) return;
+    const double* p = xyz + 3 * (size_t)t;
+    if (what == 0) velSample(T, V, p[0], p[1], p[2], out + 3 * (size_t)t);
+    else if (what == 1) macSample(T, V.mac, p[0], p[1], p[2], out + 3 * (size_t)t);
+    else if (what == 2) backTrace(T, V, p, dt, out + 3 * (size_t)t);
+    else out[t] = scalarSample(T, cell, vert, p[0], p[1], p[2]);
+}
+
+// (M^T M)^-1 M^T of the rows selected by the 7 flags, in long double with
+// partial pivoting (Constants::init, constants.cpp:170-256)
+void dualMatrices(double* out) {
+    const long double s = 1.0L / 1.7320508075688772L;
+    const long double dirs[7][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {s, s, s}, {-s, s, s}, {-s, -s, s}, {s, -s, s}};
+    for (int idx = 0; idx < 128; ++idx) {
+        double* o = out + idx * 21;
+        for (int q = 0; q < 21; ++q) o[q] = 0.0;
+        int sel[7], cnt = 0;
+        for (int j = 0; j < 7; ++j)
+            if ((idx >> (6 - j)) & 1) sel[cnt++] = j;
+        if (cnt <= 1) continue;
+        long double A[3][3] = {}, I[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                for (int q = 0; q < cnt; ++q) A[a][b] += dirs[sel[q]][a] * dirs[sel[q]][b];
+        for (int c = 0; c < 3; ++c) {
+            int pr = c;
+            for (int r = c + 1; r < 3; ++r)
+                if (fabsl(A[r][c]) > fabsl(A[pr][c])) pr = r;
+            for (int k = 0; k < 3; ++k) {
+                std::swap(A[c][k], A[pr][k]);
+                std::swap(I[c][k], I[pr][k]);
+            }
+            const long double d = A[c][c];
+            for (int k = 0; k < 3; ++k) {
+                A[c][k] /= d;
+                I[c][k] /= d;
+            }
+            for (int r = 0; r < 3; ++r)
+                if (r != c) {
+                    const long double f = A[r][c];
+                    for (int k = 0; k < 3; ++k) {
+                        A[r][k] -= f * A[c][k];
+                        I[r][k] -= f * I[c][k];
+                    }
+                }
+        }
+        for (int r = 0; r < 3; ++r)
+            for (int q = 0; q < cnt; ++q) {
+                long double v = 0;
+                for (int k = 0; k < 3; ++k) v += I[r][k] * dirs[sel[q]][k];
+                o[r * 7 + sel[q]] = (double)v;
+            }
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+int AdvectDevice::upload(const Octree& o, const ProjectTables& t, std::string& err) {
+    if (o.L > kMaxLevels) {
+        err = "advection supports at most 8 levels";
+        return ASTG_ERR_ARG;
+    }
+    L_ = o.L;
+    R_ = o.R;
+    h0_ = o.h0;
+    closed_ = o.closed_flags;
+    G_ = t.G;
+    off_ = t.off;
+    nleaf_.assign(L_, 0);
+    nnode_.assign(L_, 0);
+    std::vector<int> res(L_), moff(L_);
+    std::vector<double> hl(L_);
+    long mtot = 0;
+    for (int l = 0; l < L_; ++l) {
+        res[l] = o.lv[l].res;
+        hl[l] = o.lv[l].h;
+        nleaf_[l] = o.lv[l].nleaf;
+        nnode_[l] = o.lv[l].nnode;
+        moff[l] = (int)mtot;
+        mtot += (long)res[l] * res[l] * res[l];
+    }
+    if (mtot >= (1L << 31)) {
+        err = "index maps too large";
+        return ASTG_ERR_ARG;
+    }
+    std::vector<int> map((size_t)mtot);
+    std::vector<uint16_t> nx(G_), ny(G_), nz(G_);
+    std::vector<uint8_t> nf(G_), inHalf(G_, 0);
+    std::vector<int> leaf, ghostL, openTilt, half, tj, cbh;
+    for (int l = 0; l < L_; ++l) {
+        const Level& Lv = o.lv[l];
+        for (size_t q = 0; q < Lv.map.size(); ++q) map[moff[l] + q] = Lv.map[q] >= 0 ? Lv.map[q] + off_[l] : -1;
+        for (int i = 0; i < Lv.nnode; ++i) {
+            const int g = off_[l] + i;
+            nx[g] = Lv.x[i];
+            ny[g] = Lv.y[i];
+            nz[g] = Lv.z[i];
+            nf[g] = Lv.flag[i];
+            if (i < Lv.nleaf) {
+                leaf.push_back(g);
+                const uint8_t b = o.tilt[l][i].bits;
+                if (b & TB_LAYOUT_OPEN) openTilt.push_back(g);
+                if (b & TB_TJOINT) tj.push_back(g);
+            } else {
+                ghostL.push_back(g);
+            }
+        }
+    }
+    for (lcoor_t c : o.half) {
+        const int g = o.node(lc_level(c), lc_x(c), lc_y(c), lc_z(c));
+        if (g < 0) continue;
+        half.push_back(g + off_[lc_level(c)]);
+        inHalf[g + off_[lc_level(c)]] = 1;
+    }
+    for (lcoor_t c : o.closed_by_half) {
+        const int g = o.node(lc_level(c), lc_x(c), lc_y(c), lc_z(c));
+        if (g >= 0) cbh.push_back(g + off_[lc_level(c)]);
+    }
+    // velocity tilt DOFs: non-closed leaf tilts (tiltE state)
+    std::vector<int> velTilt;
+    for (int l = 0; l < L_; ++l)
+        for (int i = 0; i < o.lv[l].nleaf; ++i)
+            if (!(o.tilt[l][i].bits & TB_CLOSED)) velTilt.push_back(off_[l] + i);
+    // coarse MAC faces overwritten from half-tilts: the last half-tilt in list order wins
+    std::vector<int> hmG, hmT, hmA, hmS;
+    {
+        std::vector<int> winner;  // index into half, per (target, axis)
+        std::vector<long> key;
+        for (size_t q = 0; q < half.size(); ++q) {
+            const int g = half[q];
+            const int l = t.level[g];
+            const int dir = t.tilt_hd[g];
+            const int axis = dir % 3;
+            const int tgt = o.node(l + 1, nx[g] >> 1, ny[g] >> 1, nz[g] >> 1);
+            if (tgt < 0) continue;
+            key.push_back(((long)(tgt + off_[l + 1]) * 3 + axis) * (long)half.size() + (long)q);
+        }
+        std::sort(key.begin(), key.end());
+        for (size_t q = 0; q < key.size(); ++q) {
+            const long fa = key[q] / (long)half.size();
+            if (q + 1 < key.size() && key[q + 1] / (long)half.size() == fa) continue;  // a later writer exists
+            const int hq = (int)(key[q] % (long)half.size());
+            const int g = half[hq];
+            const int dir = t.tilt_hd[g];
+            hmG.push_back(g);
+            hmT.push_back((int)(fa / 3));
+            hmA.push_back((int)(fa % 3));
+            hmS.push_back(kHalfSlot[dir][4]);
+        }
+    }
+    nLeaf_ = (int)leaf.size();
+    nGhost_ = (int)ghostL.size();
+    nHalf_ = (int)half.size();
+    nTJ_ = (int)tj.size();
+    nCBH_ = (int)cbh.size();
+    nOpenTilt_ = (int)openTilt.size();
+    auto up = [&](DevBuf& d, const void* src, size_t bytes) -> hipError_t {
+        hipError_t e = d.alloc(bytes ? bytes : 4);
+        if (e != hipSuccess || !bytes) return e;
+        return hipMemcpyAsync(d.p, src, bytes, hipMemcpyHostToDevice, stream_);
+    };
+    HIPCHK(up(map_, map.data(), map.size() * 4));
+    HIPCHK(up(nx_, nx.data(), (size_t)G_ * 2));
+    HIPCHK(up(ny_, ny.data(), (size_t)G_ * 2));
+    HIPCHK(up(nz_, nz.data(), (size_t)G_ * 2));
+    HIPCHK(up(nflag_, nf.data(), (size_t)G_));
+    HIPCHK(up(te_, t.tilt_e.data(), (size_t)G_ * 8));
+    HIPCHK(up(tb_, t.tilt_bits.data(), (size_t)G_));
+    HIPCHK(up(thd_, t.tilt_hd.data(), (size_t)G_));
+    HIPCHK(up(inHalf_, inHalf.data(), (size_t)G_));
+    HIPCHK(up(leaf_, leaf.data(), leaf.size() * 4));
+    HIPCHK(up(ghost_, ghostL.data(), ghostL.size() * 4));
+    HIPCHK(up(half_, half.data(), half.size() * 4));
+    HIPCHK(up(tj_, tj.data(), tj.size() * 4));
+    HIPCHK(up(cbh_, cbh.data(), cbh.size() * 4));
+    HIPCHK(up(openTilt_, openTilt.data(), openTilt.size() * 4));
+    HIPCHK(up(velTilt_, velTilt.data(), velTilt.size() * 4));
+    nVelTilt_ = (int)velTilt.size();
+    nHalfMac_ = (int)hmG.size();
+    HIPCHK(up(hmG_, hmG.data(), hmG.size() * 4));
+    HIPCHK(up(hmT_, hmT.data(), hmT.size() * 4));
+    HIPCHK(up(hmA_, hmA.data(), hmA.size() * 4));
+    HIPCHK(up(hmS_, hmS.data(), hmS.size() * 4));
+    HIPCHK(level_.alloc((size_t)G_));
+    HIPCHK(hipMemcpyAsync(level_.p, t.level.data(), (size_t)G_, hipMemcpyHostToDevice, stream_));
+    // fields: both buffers of each scalar start at zero, as the reference's
+    // temporaries do
+    for (int f = 0; f < 2; ++f)
+        for (int b = 0; b < 2; ++b)
+            for (int k = 0; k < 2; ++k) {
+                HIPCHK(scal_[f][b][k].alloc((size_t)G_ * 8));
+                HIPCHK(hipMemsetAsync(scal_[f][b][k].p, 0, (size_t)G_ * 8, stream_));
+            }
+    cur_[0] = cur_[1] = 0;
+    HIPCHK(dualOct_.alloc((size_t)G_ * 24));
+    HIPCHK(dualTilt_.alloc((size_t)G_ * 24));
+    HIPCHK(hipMemsetAsync(dualOct_.p, 0, (size_t)G_ * 24, stream_));
+    HIPCHK(hipMemsetAsync(dualTilt_.p, 0, (size_t)G_ * 24, stream_));
+    HIPCHK(tMac_.alloc((size_t)G_ * 24));
+    HIPCHK(tTilt_.alloc((size_t)G_ * 64));
+    HIPCHK(hipMemsetAsync(tMac_.p, 0, (size_t)G_ * 24, stream_));
+    HIPCHK(hipMemsetAsync(tTilt_.p, 0, (size_t)G_ * 64, stream_));
+    HIPCHK(bad_.alloc(4));
+    // constants
+    std::vector<double> dm(128 * 21);
+    dualMatrices(dm.data());
+    const double s = 1.0 / 1.7320508075688772;
+    double dt4[12] = {s, -s, -s, s, s, s, -s, -s, s, s, s, s};
+    for (double& v : dt4) v *= 3.0 / 4.0;
+    int LL = L_, RR = R_, GG = G_;
+    HIPCHK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_L), &LL, sizeof(int), 0, hipMemcpyHostToDevice, stream_));
+    HIPCHK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_R), &RR, sizeof(int), 0, hipMemcpyHostToDevice, stream_));
+    HIPCHK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_G), &GG, sizeof(int), 0, hipMemcpyHostToDevice, stream_));
+    HIPCHK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_h0), &h0_, sizeof(double), 0, hipMemcpyHostToDevice, stream_));
+    HIPCHK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_res), res.data(), sizeof(int) * L_, 0, hipMemcpyHostToDevice, stream_));
+    HIPCHK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_moff), moff.data(), sizeof(int) * L_, 0, hipMemcpyHostToDevice,
+                                  stream_));
+    HIPCHK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_hl), hl.data(), sizeof(double) * L_, 0, hipMemcpyHostToDevice,
+                                  stream_));
+    HIPCHK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_dual), dm.data(), sizeof(double) * dm.size(), 0,
+                                  hipMemcpyHostToDevice, stream_));
+    HIPCHK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_dualTilt), dt4, sizeof(dt4), 0, hipMemcpyHostToDevice, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    return ASTG_OK;
+}
+
+AdvTopo AdvectDevice::topo() const {
+    AdvTopo T;
+    T.map = map_.as<int>();
+    T.nx = static_cast<const uint16_t*>(nx_.p);
+    T.ny = static_cast<const uint16_t*>(ny_.p);
+    T.nz = static_cast<const uint16_t*>(nz_.p);
+    T.nf = static_cast<const uint8_t*>(nflag_.p);
+    T.te = te_.as<double>();
+    T.tb = static_cast<const uint8_t*>(tb_.p);
+    T.thd = static_cast<const int8_t*>(thd_.p);
+    return T;
+}
+
+int AdvectDevice::setScalar(int field, int level, const double* oct, const double* tilt, std::string& err) {
+    if (field < 0 || field > 1 || level < 0 || level >= L_) {
+        err = "bad field or level";
+        return ASTG_ERR_ARG;
+    }
+    const size_t o = (size_t)off_[level], n = (size_t)nnode_[level];
+    HIPCHK(hipMemcpyAsync(scal_[field][cur_[field]][0].as<double>() + o, oct, n * 8, hipMemcpyHostToDevice, stream_));
+    HIPCHK(hipMemcpyAsync(scal_[field][cur_[field]][1].as<double>() + o, tilt, n * 8, hipMemcpyHostToDevice, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    return ASTG_OK;
+}
+
+int AdvectDevice::getScalar(int field, int level, double* oct, double* tilt, std::string& err) {
+    if (field < 0 || field > 1 || level < 0 || level >= L_) {
+        err = "bad field or level";
+        return ASTG_ERR_ARG;
+    }
+    const size_t o = (size_t)off_[level], n = (size_t)nnode_[level];
+    HIPCHK(hipMemcpyAsync(oct, scal_[field][cur_[field]][0].as<double>() + o, n * 8, hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipMemcpyAsync(tilt, scal_[field][cur_[field]][1].as<double>() + o, n * 8, hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    return ASTG_OK;
+}
+
+int AdvectDevice::getDual(int level, double* oct3, double* tilt3, std::string& err) {
+    if (level < 0 || level >= L_) {
+        err = "bad level";
+        return ASTG_ERR_ARG;
+    }
+    const size_t o = (size_t)off_[level], n = (size_t)nnode_[level];
+    HIPCHK(hipMemcpyAsync(oct3, dualOct_.as<double>() + 3 * o, n * 24, hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipMemcpyAsync(tilt3, dualTilt_.as<double>() + 3 * o, n * 24, hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    return ASTG_OK;
+}
+
+int AdvectDevice::omniGhost(double* cell, double* vert, int nc, std::string& err) {
+    const Topo T = topo();
+    const uint8_t* lev = static_cast<const uint8_t*>(level_.p);
+    const int* gl = ghost_.as<int>();
+    const uint8_t* ih = static_cast<const uint8_t*>(inHalf_.p);
+    const int gb = nblocks(nGhost_);
+    if (nc == 1) {
+        if (nGhost_) {
+            k_cell_ghost_fine<1><<<gb, BS, 0, stream_>>>(T, lev, gl, nGhost_, cell);
+            k_cell_ghost_coarse<1><<<gb, BS, 0, stream_>>>(T, lev, gl, nGhost_, cell);
+            k_vert_ghost<1><<<gb, BS, 0, stream_>>>(T, lev, gl, nGhost_, ih, vert);
+            k_cell_ghost_tj<1><<<gb, BS, 0, stream_>>>(T, lev, gl, nGhost_, vert, cell);
+        }
+        if (nTJ_) k_closed_joint<1><<<nblocks(nTJ_), BS, 0, stream_>>>(T, lev, tj_.as<int>(), nTJ_, cell, vert);
+        if (nCBH_)
+            k_closed_by_half<1><<<nblocks(nCBH_), BS, 0, stream_>>>(T, lev, cbh_.as<int>(), nCBH_, cell, vert,
+                                                                     bad_.as<int>());
+    } else {
+        if (nGhost_) {
+            k_cell_ghost_fine<3><<<gb, BS, 0, stream_>>>(T, lev, gl, nGhost_, cell);
+            k_cell_ghost_coarse<3><<<gb, BS, 0, stream_>>>(T, lev, gl, nGhost_, cell);
+            k_vert_ghost<3><<<gb, BS, 0, stream_>>>(T, lev, gl, nGhost_, ih, vert);
+            k_cell_ghost_tj<3><<<gb, BS, 0, stream_>>>(T, lev, gl, nGhost_, vert, cell);
+        }
+        if (nTJ_) k_closed_joint<3><<<nblocks(nTJ_), BS, 0, stream_>>>(T, lev, tj_.as<int>(), nTJ_, cell, vert);
+        if (nCBH_)
+            k_closed_by_half<3><<<nblocks(nCBH_), BS, 0, stream_>>>(T, lev, cbh_.as<int>(), nCBH_, cell, vert,
+                                                                     bad_.as<int>());
+    }
+    HIPCHK(hipGetLastError());
+    return ASTG_OK;
+}
+
+int AdvectDevice::constrainAndGhost(ProjectDevice& pd, std::string& err) {
+    const Topo T = topo();
+    const uint8_t* lev = static_cast<const uint8_t*>(level_.p);
+    double* mac = pd.uMac();
+    double* tilt = pd.uTilt();
+    // constrain (MAC walls; tilt and dual constraints need a collider)
+    k_wall<<<nblocks(G_), BS, 0, stream_>>>(T, lev, mac, closed_);
+    // velocity ghosts (octree_omni_face_centered_grid3.h:97-120)
+    if (nGhost_) {
+        k_mac_ghost_clear<<<nblocks(nGhost_), BS, 0, stream_>>>(ghost_.as<int>(), nGhost_, mac);
+        k_mac_ghost_sample<<<nblocks(nGhost_), BS, 0, stream_>>>(T, lev, ghost_.as<int>(), nGhost_, mac, 0);
+    }
+    if (nHalfMac_)
+        k_mac_from_half<<<nblocks(nHalfMac_), BS, 0, stream_>>>(hmG_.as<int>(), hmT_.as<int>(), hmA_.as<int>(),
+                                                                 hmS_.as<int>(), nHalfMac_, tilt, mac);
+    if (nGhost_) {
+        k_mac_ghost_sample<<<nblocks(nGhost_), BS, 0, stream_>>>(T, lev, ghost_.as<int>(), nGhost_, mac, 1);
+        k_tilt_ghost<<<nblocks(nGhost_), BS, 0, stream_>>>(T, lev, ghost_.as<int>(), nGhost_,
+                                                            static_cast<const uint8_t*>(inHalf_.p), tilt);
+    }
+    HIPCHK(hipGetLastError());
+    return ASTG_OK;
+}
+
+int AdvectDevice::postProject(ProjectDevice& pd, std::string& err) {
+    const Topo T = topo();
+    const uint8_t* lev = static_cast<const uint8_t*>(level_.p);
+    double* mac = pd.uMac();
+    double* tilt = pd.uTilt();
+    HIPCHK(hipMemsetAsync(bad_.p, 0, 4, stream_));
+    int rc = constrainAndGhost(pd, err);
+    if (rc) return rc;
+    // dual grid
+    if (nLeaf_) {
+        k_dual_tilt<<<nblocks(nLeaf_), BS, 0, stream_>>>(T, leaf_.as<int>(), nLeaf_, tilt, dualTilt_.as<double>());
+        k_dual_oct<<<nblocks(nLeaf_), BS, 0, stream_>>>(T, lev, leaf_.as<int>(), nLeaf_, mac, tilt,
+                                                         dualOct_.as<double>());
+    }
+    if (nHalf_)
+        k_dual_half<<<nblocks(nHalf_), BS, 0, stream_>>>(T, half_.as<int>(), nHalf_, tilt, dualTilt_.as<double>());
+    HIPCHK(hipGetLastError());
+    rc = omniGhost(dualOct_.as<double>(), dualTilt_.as<double>(), 3, err);
+    if (rc) return rc;
+    return checkBad(err);
+}
+
+int AdvectDevice::checkBad(std::string& err) {
+    int bad = 0;
+    HIPCHK(hipMemcpyAsync(&bad, bad_.p, 4, hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    if (bad) {
+        err = "a tilt closed by a half-tilt is open or has a half direction";
+        return ASTG_ERR_TOPOLOGY;
+    }
+    return ASTG_OK;
+}
+
+int AdvectDevice::scalarGhost(int field, std::string& err) {
+    if (field < 0 || field > 1) {
+        err = "bad field";
+        return ASTG_ERR_ARG;
+    }
+    HIPCHK(hipMemsetAsync(bad_.p, 0, 4, stream_));
+    int rc = omniGhost(scal_[field][cur_[field]][0].as<double>(), scal_[field][cur_[field]][1].as<double>(), 1, err);
+    if (rc) return rc;
+    return checkBad(err);
+}
+
+AdvVel AdvectDevice::vel(ProjectDevice& pd) const {
+    AdvVel V;
+    V.mac = pd.uMac();
+    V.tilt = pd.uTilt();
+    V.doct = dualOct_.as<double>();
+    V.dtilt = dualTilt_.as<double>();
+    return V;
+}
+
+int AdvectDevice::advect(ProjectDevice& pd, double dt, std::string& err) {
+    const Topo T = topo();
+    const uint8_t* lev = static_cast<const uint8_t*>(level_.p);
+    const Vel V = vel(pd);
+    const int d = cur_[0], h = cur_[1];
+    double* d1c = scal_[0][1 - d][0].as<double>();
+    double* d1v = scal_[0][1 - d][1].as<double>();
+    double* h1c = scal_[1][1 - h][0].as<double>();
+    double* h1v = scal_[1][1 - h][1].as<double>();
+    const double* d0c = scal_[0][d][0].as<double>();
+    const double* d0v = scal_[0][d][1].as<double>();
+    const double* h0c = scal_[1][h][0].as<double>();
+    const double* h0v = scal_[1][h][1].as<double>();
+    if (nLeaf_)
+        k_adv_scalar<<<nblocks(nLeaf_), BS, 0, stream_>>>(T, lev, V, leaf_.as<int>(), nLeaf_, 0, dt, d0c, d0v, h0c,
+                                                          h0v, d1c, h1c);
+    if (nOpenTilt_)
+        k_adv_scalar<<<nblocks(nOpenTilt_), BS, 0, stream_>>>(T, lev, V, openTilt_.as<int>(), nOpenTilt_, 1, dt, d0c,
+                                                              d0v, h0c, h0v, d1v, h1v);
+    if (nHalf_)
+        k_adv_scalar<<<nblocks(nHalf_), BS, 0, stream_>>>(T, lev, V, half_.as<int>(), nHalf_, 1, dt, d0c, d0v, h0c,
+                                                          h0v, d1v, h1v);
+    HIPCHK(hipGetLastError());
+    cur_[0] = 1 - d;
+    cur_[1] = 1 - h;
+    int rc = scalarGhost(0, err);
+    if (rc) return rc;
+    rc = scalarGhost(1, err);
+    if (rc) return rc;
+    // velocity (advection.h:195-239) into the temporaries, then swap
+    if (nLeaf_)
+        k_adv_mac<<<nblocks(3L * nLeaf_), BS, 0, stream_>>>(T, lev, V, leaf_.as<int>(), nLeaf_, dt, tMac_.as<double>());
+    if (nVelTilt_)
+        k_adv_tilt<<<nblocks(8L * nVelTilt_), BS, 0, stream_>>>(T, lev, V, velTilt_.as<int>(), nVelTilt_, dt,
+                                                                 tTilt_.as<double>());
+    if (nHalf_)
+        k_adv_half<<<nblocks(nHalf_), BS, 0, stream_>>>(T, lev, V, half_.as<int>(), nHalf_, dt, tTilt_.as<double>());
+    HIPCHK(hipGetLastError());
+    pd.swapVelocity(tMac_, tTilt_);
+    // decay (density->scale, heat->scale)
+    const long G = G_;
+    k_scale2<<<nblocks(G), BS, 0, stream_>>>(G, 0.999, scal_[0][cur_[0]][0].as<double>(),
+                                             scal_[0][cur_[0]][1].as<double>());
+    k_scale2<<<nblocks(G), BS, 0, stream_>>>(G, 0.999, scal_[1][cur_[1]][0].as<double>(),
+                                             scal_[1][cur_[1]][1].as<double>());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(stream_));
+    return ASTG_OK;
+}
+
+int AdvectDevice::samplePoints(ProjectDevice& pd, int what, int n, const double* xyz, double dt, double* out,
+                               std::string& err) {
+    if (what < 0 || what > 4 || n < 0) {
+        err = "bad query";
+        return ASTG_ERR_ARG;
+    }
+    if (n == 0) return ASTG_OK;
+    const int w = what <= 2 ? 3 : 1;
+    HIPCHK(pts_.alloc((size_t)n * 24));
+    HIPCHK(out_.alloc((size_t)n * 8 * w));
+    HIPCHK(hipMemcpyAsync(pts_.p, xyz, (size_t)n * 24, hipMemcpyHostToDevice, stream_));
+    const int f = what == 4 ? 1 : 0;
+    k_points<<<nblocks(n), BS, 0, stream_>>>(topo(), vel(pd), scal_[f][cur_[f]][0].as<double>(),
+                                             scal_[f][cur_[f]][1].as<double>(), what, n, pts_.as<double>(), dt,
+                                             out_.as<double>());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, out_.p, (size_t)n * 8 * w, hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    return ASTG_OK;
+}
+
+double AdvectDevice::advectBytes() const {
+    // compulsory traffic: each source field and the velocity read once, each
+    // destination written once, plus the topology arrays
+    const double g = G_;
+    return g * (2 * 2 * 8 + 2 * 2 * 8) + g * (3 + 8) * 8 * 2 + g * 6 * 8 + g * (2 * 3 + 1 + 8 + 1 + 1);
+}
+
+}  // namespace astg

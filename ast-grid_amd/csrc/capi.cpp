@@ -12,6 +12,7 @@
 #include "../../include/astg.h"
 #include "octree.h"
 #include "pcg.h"
+#include "advect.h"
 #include "project.h"
 #include "project_host.h"
 #include "scene.h"
@@ -29,6 +30,7 @@ struct astg_ctx {
     std::vector<std::vector<double>> hostMac, hostTilt;  // host-only contexts
     std::unique_ptr<astg::PcgDevice> pcg;                 // projection system
     std::unique_ptr<astg::ProjectDevice> proj;
+    std::unique_ptr<astg::AdvectDevice> adv;              // post-projection step and advection
     std::unique_ptr<astg::PcgDevice> csr;                 // CSR seam
     std::vector<int> csrRs, csrCol;
     astg::PcgDevice* lastPcg = nullptr;
@@ -52,6 +54,8 @@ int guarded(astg_ctx* c, F&& f) {
         return fail(c, ASTG_ERR_TOPOLOGY, e.what());
     }
 }
+
+int runProject(astg_ctx* c, double dt, double rho, const astg_pcg_params& prm, astg_pcg_result* result);
 
 int needDevice(astg_ctx* c) {
     if (c->device < 0) return fail(c, ASTG_ERR_STATE, "context has no device (created with device < 0)");
@@ -82,6 +86,9 @@ int ensureTables(astg_ctx* c) {
         if (!c->pcg) c->pcg.reset(new astg::PcgDevice(c->stream));
         if (!c->proj) c->proj.reset(new astg::ProjectDevice(c->stream));
         rc = c->proj->upload(c->tables, *c->tree, c->err);
+        if (rc) return rc;
+        if (!c->adv) c->adv.reset(new astg::AdvectDevice(c->stream));
+        rc = c->adv->upload(*c->tree, c->tables, c->err);
         if (rc) return rc;
         rc = c->pcg->setPattern(c->tables.n, c->tables.rowstart.data(), c->tables.col.data(), c->err);
         if (rc) return rc;
@@ -353,7 +360,14 @@ int astg_project(astg_ctx* c, double dt, double rho, const astg_pcg_params* para
     astg_pcg_params prm;
     astg_pcg_params_default(&prm);
     if (params) prm = *params;
-    return guarded(c, [&] {
+    return guarded(c, [&] { return runProject(c, dt, rho, prm, result); });
+}
+
+}  // extern "C"
+
+namespace {
+int runProject(astg_ctx* c, double dt, double rho, const astg_pcg_params& prm, astg_pcg_result* result) {
+    {
         int r = ensureTables(c);
         if (r) return r;
         if (c->tables.half_duplicates)
@@ -390,8 +404,11 @@ int astg_project(astg_ctx* c, double dt, double rho, const astg_pcg_params* para
         if (result) *result = res;
         if (!res.converged) return fail(c, ASTG_ERR_NOT_CONVERGED, "PCG did not reach the tolerance");
         return ASTG_OK;
-    });
+    }
 }
+}  // namespace
+
+extern "C" {
 
 int astg_get_system(astg_ctx* c, int* rowstart, int* col, double* val, double* b, double* p) {
     if (!c) return ASTG_ERR_ARG;
@@ -550,6 +567,87 @@ int astg_sweep_timeline(astg_ctx* c, int which, long long* t_out, int* level_out
         std::copy(lev.begin(), lev.end(), level_out);
         return (int)ASTG_OK;
     });
+}
+
+// ---- post-projection step and advection ---------------------------------
+namespace {
+int needAdvect(astg_ctx* c) {
+    if (!c) return ASTG_ERR_ARG;
+    int rc = needDevice(c);
+    if (rc) return rc;
+    return guarded(c, [&] { return ensureTables(c); });
+}
+}  // namespace
+
+int astg_set_scalar(astg_ctx* c, int field, int level, const double* oct, const double* tilt) {
+    int rc = checkLevel(c, level);
+    if (rc) return rc;
+    if (!oct || !tilt) return fail(c, ASTG_ERR_ARG, "null field array");
+    rc = needAdvect(c);
+    if (rc) return rc;
+    return c->adv->setScalar(field, level, oct, tilt, c->err);
+}
+
+int astg_get_scalar(astg_ctx* c, int field, int level, double* oct, double* tilt) {
+    int rc = checkLevel(c, level);
+    if (rc) return rc;
+    if (!oct || !tilt) return fail(c, ASTG_ERR_ARG, "null field array");
+    rc = needAdvect(c);
+    if (rc) return rc;
+    return c->adv->getScalar(field, level, oct, tilt, c->err);
+}
+
+int astg_post_project(astg_ctx* c) {
+    int rc = needAdvect(c);
+    if (rc) return rc;
+    return c->adv->postProject(*c->proj, c->err);
+}
+
+int astg_scalar_update_ghost(astg_ctx* c, int field) {
+    int rc = needAdvect(c);
+    if (rc) return rc;
+    return c->adv->scalarGhost(field, c->err);
+}
+
+int astg_get_dual(astg_ctx* c, int level, double* oct3, double* tilt3) {
+    int rc = checkLevel(c, level);
+    if (rc) return rc;
+    if (!oct3 || !tilt3) return fail(c, ASTG_ERR_ARG, "null array");
+    rc = needAdvect(c);
+    if (rc) return rc;
+    return c->adv->getDual(level, oct3, tilt3, c->err);
+}
+
+int astg_sample_points(astg_ctx* c, int what, int n, const double* xyz, double dt, double* out) {
+    if (n < 0 || (n > 0 && (!xyz || !out))) return fail(c, ASTG_ERR_ARG, "bad point arrays");
+    int rc = needAdvect(c);
+    if (rc) return rc;
+    return c->adv->samplePoints(*c->proj, what, n, xyz, dt, out, c->err);
+}
+
+int astg_advect(astg_ctx* c, double dt) {
+    if (!(dt > 0)) return fail(c, ASTG_ERR_ARG, "dt must be positive");
+    int rc = needAdvect(c);
+    if (rc) return rc;
+    return c->adv->advect(*c->proj, dt, c->err);
+}
+
+int astg_step(astg_ctx* c, double dt, double rho, const astg_pcg_params* params, astg_pcg_result* result) {
+    if (!(dt > 0) || !(rho > 0)) return fail(c, ASTG_ERR_ARG, "dt and rho must be positive");
+    int rc = needAdvect(c);
+    if (rc) return rc;
+    astg_pcg_params prm;
+    astg_pcg_params_default(&prm);
+    if (params) prm = *params;
+    // onUpdate (octree_omnigrid_solver3.cpp:222-249) without addForce: constrain
+    // and ghost, project, constrain, ghost, dual grid, advect
+    rc = c->adv->constrainAndGhost(*c->proj, c->err);
+    if (rc) return rc;
+    rc = guarded(c, [&] { return runProject(c, dt, rho, prm, result); });
+    if (rc) return rc;
+    rc = c->adv->postProject(*c->proj, c->err);
+    if (rc) return rc;
+    return c->adv->advect(*c->proj, dt, c->err);
 }
 
 }  // extern "C"

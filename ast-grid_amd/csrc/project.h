@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <utility>
 
 #include "pcg.h"
 #include "project_host.h"
@@ -22,6 +23,14 @@ public:
     // Device-side copy of the velocity fields (restore = false saves).
     int snapshot(bool restore, std::string& err);
     double assembleBytes() const;
+    // Exchanges the live velocity with another pair of buffers of the same
+    // sizes (the advected velocity becomes live, the old one the temporary).
+    void swapVelocity(DevBuf& mac, DevBuf& tilt) {
+        std::swap(u_mac_.p, mac.p);
+        std::swap(u_mac_.bytes, mac.bytes);
+        std::swap(u_tilt_.p, tilt.p);
+        std::swap(u_tilt_.bytes, tilt.bytes);
+    }
     double* uMac() { return u_mac_.as<double>(); }    // 3*G, axis-major
     double* uTilt() { return u_tilt_.as<double>(); }  // 8*G, slot-major
 

@@ -174,6 +174,51 @@ int astg_kernel_time(astg_ctx* ctx, int which, int reps, double* avg_ms, double*
  * latency analysis; no counterpart in the reference. */
 int astg_sweep_timeline(astg_ctx* ctx, int which, long long* t_out, int* level_out);
 
+/* ---- the step around the projection and advection --------------------- */
+/* Scalar fields carried by the context: ASTG_FIELD_DENSITY, ASTG_FIELD_HEAT;
+ * per level, one octagon (cell) and one tilt (vertex) value per node, in node
+ * order, as OctreeOmniGrid3<REAL> stores them (include/octree_omni_grid3.h).
+ * Scalars and velocity start at zero and are reset when the topology changes. */
+#define ASTG_FIELD_DENSITY 0
+#define ASTG_FIELD_HEAT 1
+int astg_set_scalar(astg_ctx* ctx, int field, int level, const double* oct, const double* tilt);
+int astg_get_scalar(astg_ctx* ctx, int field, int level, double* oct, double* tilt);
+
+/* The chain between project and advect in OctreeOmniGridSolver3::onUpdate
+ * (src/octree_omnigrid_solver3.cpp:234-241): constrainVelocity on the MAC and
+ * tilt grids (closed walls; no collider), velocity->updateGhost, updateDualGrid,
+ * dual constraint, dual->updateGhost. */
+int astg_post_project(astg_ctx* ctx);
+/* OctreeOmniGrid3<REAL>::updateGhost of one scalar field (octree_omni_grid3.h:136-143). */
+int astg_scalar_update_ghost(astg_ctx* ctx, int field);
+/* Dual velocity grid of the last astg_post_project: 3 values per node for the
+ * cell centres and for the vertices. */
+int astg_get_dual(astg_ctx* ctx, int level, double* oct3, double* tilt3);
+
+/* Point queries (host arrays, n points of 3 coordinates):
+ *   ASTG_SAMPLE_VELOCITY  OctreeOmniFaceCenteredGrid3::sample (omni_face_centered_grid3.h:37-80)
+ *   ASTG_SAMPLE_MAC       OctreeFaceCenteredGrid3::sample (face_centered_grid3.cpp:36-45)
+ *   ASTG_SAMPLE_BACKTRACE BackTrace over dt (advection.h:15-64)
+ *   ASTG_SAMPLE_DENSITY / ASTG_SAMPLE_HEAT  OctreeOmniGrid3<REAL>::sample (omni_grid3.h:146-191)
+ * out holds 3 values per point for the first three, 1 for the scalars. */
+#define ASTG_SAMPLE_VELOCITY 0
+#define ASTG_SAMPLE_MAC 1
+#define ASTG_SAMPLE_BACKTRACE 2
+#define ASTG_SAMPLE_DENSITY 3
+#define ASTG_SAMPLE_HEAT 4
+int astg_sample_points(astg_ctx* ctx, int what, int n, const double* xyz, double dt, double* out);
+
+/* OctreeOmniGridSolver3::advect with tilt adaptivity off
+ * (src/octree_omnigrid_solver3.cpp:850-914): density and heat on octagon, open
+ * tilt and half-tilt DOFs, their ghosts, the velocity on MAC, tilt and
+ * half-tilt slots (SemiLagrangianOctreeOmniFace3, advection.h:195-239), then
+ * the 0.999 decay of both scalars. Needs astg_post_project first. */
+int astg_advect(astg_ctx* ctx, double dt);
+
+/* One step of onUpdate without addForce: constrain + ghosts, astg_project,
+ * astg_post_project, astg_advect. */
+int astg_step(astg_ctx* ctx, double dt, double rho, const astg_pcg_params* params, astg_pcg_result* result);
+
 #ifdef __cplusplus
 }
 #endif

@@ -57,6 +57,14 @@ def lib():
         "ref_mic_factor": (I, [I, P, P, P, D, D, P, P, P, P]),
         "ref_mic_apply": (None, [I, P, P, P, D, D, P, P]),
         "ref_pcg_csr": (I, [I, P, P, P, P, P, D, I, D, D, P, P]),
+        "ref_set_scalar": (None, [P, I, I, P, P]),
+        "ref_get_scalar": (None, [P, I, I, P, P]),
+        "ref_post_project": (I, [P]),
+        "ref_scalar_update_ghost": (I, [P, I]),
+        "ref_get_dual": (None, [P, I, P, P]),
+        "ref_sample_points": (I, [P, I, I, P, D, P]),
+        "ref_sample_scalar": (I, [P, I, I, P, P]),
+        "ref_advect": (I, [P, D]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -152,6 +160,48 @@ class RefSolver:
         t = np.empty((n, 8), np.float64)
         self.L.ref_get_velocity(self.h, l, _p(m), _p(t))
         return m, t
+
+    # ---- post-projection chain, sampling, advection ----
+    def _chk(self, rc):
+        if rc != 0:
+            raise RuntimeError(self.L.ref_last_error().decode())
+
+    def set_scalar(self, field, l, oct_, tilt):
+        self.L.ref_set_scalar(self.h, field, l, _p(np.ascontiguousarray(oct_, np.float64)),
+                              _p(np.ascontiguousarray(tilt, np.float64)))
+
+    def scalar(self, field, l):
+        n = self.level_size(l)
+        o, t = np.empty(n), np.empty(n)
+        self.L.ref_get_scalar(self.h, field, l, _p(o), _p(t))
+        return o, t
+
+    def post_project(self):
+        self._chk(self.L.ref_post_project(self.h))
+
+    def scalar_update_ghost(self, field):
+        self._chk(self.L.ref_scalar_update_ghost(self.h, field))
+
+    def dual(self, l):
+        n = self.level_size(l)
+        o, t = np.empty((n, 3)), np.empty((n, 3))
+        self.L.ref_get_dual(self.h, l, _p(o), _p(t))
+        return o, t
+
+    def sample_points(self, what, xyz, dt=0.0):
+        xyz = np.ascontiguousarray(xyz, np.float64)
+        out = np.empty_like(xyz)
+        self._chk(self.L.ref_sample_points(self.h, what, len(xyz), _p(xyz), dt, _p(out)))
+        return out
+
+    def sample_scalar(self, field, xyz):
+        xyz = np.ascontiguousarray(xyz, np.float64)
+        out = np.empty(len(xyz))
+        self._chk(self.L.ref_sample_scalar(self.h, field, len(xyz), _p(xyz), _p(out)))
+        return out
+
+    def advect(self, dt):
+        self._chk(self.L.ref_advect(self.h, dt))
 
     def project(self, dt, tol=1e-6, maxit=3000):
         if self.L.ref_project(self.h, dt, tol, maxit):

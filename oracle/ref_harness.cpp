@@ -15,6 +15,7 @@
 //   * exposes layout / tilt / numbering / matrix / velocity state for dumping.
 // The C functions below take plain pointers and sizes only.
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <exception>
@@ -24,6 +25,7 @@
 
 #include <spdlog/spdlog.h>
 
+#include "advection.h"
 #include "octree_omnigrid_solver3.h"
 #include "octree_util.h"
 #include "pcgsolver/pcg_solver.h"
@@ -71,6 +73,18 @@ public:
     VectorXd& sol() { return vRes; }
     PCGSolver<REAL>& pcg() { return pcgSolver; }
     double h() const { return delta_h; }
+
+    // onUpdate's chain between project and advect (octree_omnigrid_solver3.cpp:234-241)
+    void postProject() {
+        boundarySolver->constrainVelocity(velocity->getMACGrid());
+        boundarySolver->constrainVelocity(velocity->getTiltGrid(), velocity->getMACGrid());
+        velocity->updateGhost();
+        velocity->updateDualGrid();
+        boundarySolver->constrainVelocity(velocity->getDualGrid());
+        velocity->getDualGrid()->updateGhost();
+    }
+    Vector3f backTrace(const Vector3f& p, double dt) { return BackTrace(boundarySdf, velocity, p, dt); }
+    OctreeOmniScalarGrid3Ptr& scalar(int f) { return f == 0 ? density : heat; }
 
     void doProject(double dt) { project(dt); }
     void doUpdateVelocity(const VectorXd& p, double dt) { updateVelocityByPressure(p, dt); }
@@ -219,7 +233,7 @@ void ref_set_velocity(void* p, int level, const double* mac, const double* tilt8
     auto& v = s->getVelocity();
     auto& mg = v->getMACGrid();
     auto& td = v->getTiltGrid()->getGridPtr()->getData()[level];
-    int n = (int)td.size();
+    int n = std::min((int)td.size(), s->lay()->levelSize(level));
     for (int i = 0; i < n; ++i) {
         mg->getUx()[level][i] = mac[3 * i + 0];
         mg->getUy()[level][i] = mac[3 * i + 1];
@@ -233,13 +247,80 @@ void ref_get_velocity(void* p, int level, double* mac, double* tilt8) {
     auto& v = s->getVelocity();
     auto& mg = v->getMACGrid();
     auto& td = v->getTiltGrid()->getGridPtr()->getData()[level];
-    int n = (int)td.size();
+    int n = std::min((int)td.size(), s->lay()->levelSize(level));
     for (int i = 0; i < n; ++i) {
         mac[3 * i + 0] = mg->getUx()[level][i];
         mac[3 * i + 1] = mg->getUy()[level][i];
         mac[3 * i + 2] = mg->getUz()[level][i];
         for (int k = 0; k < 8; ++k) tilt8[8 * i + k] = td[i](k);
     }
+}
+
+// ---- post-projection chain, sampling and advection (a19-a26) -------------
+// Scalar fields: 0 density, 1 heat; per level, per layer node: octagon and tilt.
+// The advection temporaries can hold more entries than the layer has nodes;
+// only the first levelSize entries are node values.
+void ref_set_scalar(void* p, int field, int level, const double* oct, const double* tilt) {
+    auto& f = H(p)->scalar(field);
+    auto& od = f->getOctagonGrid()->getData()[level];
+    auto& td = f->getTiltGrid()->getData()[level];
+    const size_t n = (size_t)H(p)->lay()->levelSize(level);
+    for (size_t i = 0; i < n && i < od.size(); ++i) od[i] = oct[i];
+    for (size_t i = 0; i < n && i < td.size(); ++i) td[i] = tilt[i];
+}
+
+void ref_get_scalar(void* p, int field, int level, double* oct, double* tilt) {
+    auto& f = H(p)->scalar(field);
+    auto& od = f->getOctagonGrid()->getData()[level];
+    auto& td = f->getTiltGrid()->getData()[level];
+    const size_t n = (size_t)H(p)->lay()->levelSize(level);
+    for (size_t i = 0; i < n; ++i) oct[i] = i < od.size() ? od[i] : 0.0;
+    for (size_t i = 0; i < n; ++i) tilt[i] = i < td.size() ? td[i] : 0.0;
+}
+
+int ref_post_project(void* p) {
+    return guarded([&] { H(p)->postProject(); });
+}
+
+int ref_scalar_update_ghost(void* p, int field) {
+    return guarded([&] { H(p)->scalar(field)->updateGhost(); });
+}
+
+// Dual grid (cell-centre and vertex velocity vectors): 3 per node.
+void ref_get_dual(void* p, int level, double* oct3, double* tilt3) {
+    auto& d = H(p)->getVelocity()->getDualGrid();
+    auto& od = d->getOctagonGrid()->getData()[level];
+    auto& td = d->getTiltGrid()->getData()[level];
+    const size_t n = (size_t)H(p)->lay()->levelSize(level);
+    for (size_t i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) oct3[3 * i + k] = i < od.size() ? od[i](k) : 0.0;
+    for (size_t i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) tilt3[3 * i + k] = i < td.size() ? td[i](k) : 0.0;
+}
+
+// what: 0 omni velocity sample, 1 MAC-only sample, 2 back-trace (dt)
+int ref_sample_points(void* p, int what, int n, const double* xyz, double dt, double* out3) {
+    auto* s = H(p);
+    return guarded([&] {
+        for (int i = 0; i < n; ++i) {
+            Vector3f q(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]), r;
+            if (what == 0) r = s->getVelocity()->sample(q);
+            else if (what == 1) r = s->getVelocity()->getMACGrid()->sample(q);
+            else r = s->backTrace(q, dt);
+            for (int k = 0; k < 3; ++k) out3[3 * i + k] = r(k);
+        }
+    });
+}
+
+int ref_sample_scalar(void* p, int field, int n, const double* xyz, double* out) {
+    auto* s = H(p);
+    return guarded([&] {
+        for (int i = 0; i < n; ++i) out[i] = s->scalar(field)->sample(Vector3f(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]));
+    });
+}
+
+int ref_advect(void* p, double dt) {
+    return guarded([&] { H(p)->doAdvect(dt); });
 }
 
 void ref_get_weights(void* p, int level, double* mac, double* tilt8) {
