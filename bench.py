@@ -49,6 +49,44 @@ def time_steps(ctx, info, prm, steps, warmup):
     return (t1 - t0) * 1e3 / steps, results
 
 
+def advect_timing(ctx, info, prm, reps):
+    """One projected frame, then the post-projection chain (constrain, ghosts,
+    dual grid) and one advect step of density, heat and velocity, each timed
+    over `reps` repetitions (device work bracketed by synchronisation)."""
+    import numpy as np
+    import torch
+    import astg
+    L = info["levels"]
+    rng = np.random.default_rng(1)
+    dofs = 0
+    for l in range(L):
+        nn, nleaf = ctx.level_info(l)
+        for f in (astg.FIELD_DENSITY, astg.FIELD_HEAT):
+            ctx.set_scalar(f, l, rng.random(nn), rng.random(nn))
+        _, bits, _ = ctx.tilt(l)
+        leaf_bits = bits[:nleaf]
+        dofs += nleaf * (1 + 3) + int(((leaf_bits & 64) != 0).sum()) + 8 * int(((leaf_bits & 2) == 0).sum())
+    nh = len(ctx.half_tilts())
+    dofs += nh * (1 + 5)
+    ctx.snapshot(restore=True)
+    ctx.project(info["dt"], info["rho"], prm)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.post_project()
+    torch.cuda.synchronize()
+    t_post = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.advect(info["dt"])
+    torch.cuda.synchronize()
+    t_adv = (time.perf_counter() - t0) / reps
+    return {"ms_post_project": round(t_post * 1e3, 3), "ms_advect": round(t_adv * 1e3, 3),
+            "backtraces": dofs, "backtraces_per_s": round(dofs / t_adv, 1),
+            "note": "density+heat on octagon/open-tilt/half-tilt DOFs, velocity on MAC/tilt/half-tilt slots; "
+                    "2 velocity samples per back-trace"}
+
+
 def cpu_baseline(ctx, res64, budget_iters=12):
     """Reference MIC(0)-PCG (oracle/_ref, the reference's own code) or the
     oracle's port on the same C2 system: factor + a bounded number of PCG
@@ -145,6 +183,10 @@ def main():
                 "frac": round(ach / MI355X_HBM_GBS, 4), "traffic": TRAFFIC.get(dom),
                 "kernels": kern}
 
+    adv = advect_timing(ctx, info, prm64, a.kernel_reps)
+    log("advect %.2f ms (post-project chain %.2f ms), %.1f M back-traces/s" % (
+        adv["ms_advect"], adv["ms_post_project"], adv["backtraces_per_s"] / 1e6))
+
     cpu = None
     if not a.no_cpu:
         try:
@@ -179,6 +221,7 @@ def main():
         "sweep_depth": last64.depth,
         "setup_s": round(setup_s, 2),
         "roofline": roofline,
+        "advect": adv,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
