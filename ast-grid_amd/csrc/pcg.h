@@ -4,11 +4,13 @@
 // with the reference's MIC(0) factor (pcg_solver.h:89-190) and triangular
 // solves (pcg_solver.h:211-253), restated for a GPU:
 //   * the dependency levels of tril(A) are computed on the device by a
-//     sync-free pass, and rows are processed in level order;
-//   * the factorisation is left-looking per row (same accumulation order as the
-//     reference's right-looking column loop) and the triangular sweeps are
-//     sync-free: each row publishes its value with an agent-scope atomic store
-//     over a signalling-NaN sentinel that dependants poll;
+//     sync-free pass;
+//   * the factorisation is level-synchronous (one launch per level, one row per
+//     lane, left-looking per row with the reference's accumulation order);
+//   * the triangular sweeps are sync-free persistent kernels over merged
+//     stencils (sweep_merge.h): each row publishes its value with an
+//     agent-scope atomic store over a signalling-NaN sentinel that dependants
+//     poll;
 //   * dots, axpys and the convergence test are fused into the SpMV / update /
 //     sweep kernels with wave64 reductions; scalars stay on the device.
 #pragma once
@@ -33,16 +35,32 @@ struct DevBuf {
     T* as() const { return static_cast<T*>(p); }
 };
 
-// Rows in padded level order, plus one gate row per 64-row segment.
+// Rows in padded level order (or padded super-level order, see
+// sweep_merge.h), plus one gate row per 64-row segment.
 struct SweepPlan {
     const int* order;  // P entries, -1 = padding
-    const int* gateF;  // per segment: position of a row of level L-dist (forward), -1 none
-    const int* gateB;  // per segment: position of a row of level L+dist (backward), -1 none
+    const int* gateF;  // per segment: position of a row of group G-dist (forward), -1 none
+    const int* gateB;  // per segment: position of a row of group G+dist (backward), -1 none
     int P;
     int nchunks;       // P / 256
     int pollCap;       // largest sleep between polling rounds (units of 64 cycles)
     int gateCap;       // largest sleep between gate polls (units of 128 cycles)
     long long* trace;  // optional per-row timeline (3 per row), null in normal runs
+};
+
+// Merged sweep stencil of one direction, by sweep position (sweep_merge.h).
+template <typename T>
+struct StencilView {
+    const int* start;  // P + 1
+    const int* nr;     // r-terms of the row (the first entries)
+    const int* idx;    // per entry: r-term key, y-term sweep position
+    const T* coef;     // per entry
+};
+struct DevStencil {
+    DevBuf start, nr, idx, ustart, upd, coef, coef32;
+    long long E = 0, R = 0, kept = 0;
+    int maxY = 0;  // largest number of y-terms of a row
+    int cap = 16;  // register capacity of the sweep kernel (16 or 32)
 };
 
 // Per-block partials of the reductions, consumed by the next kernel.
@@ -95,21 +113,29 @@ public:
 
 private:
     template <typename T>
-    int pcgInner(const T* A, const T* F, const T* invd, const T* b, T* x, double tolRel, int maxIt, int norm,
-                 int& iters, double& resOut, bool& conv, std::string& err);
+    int pcgInner(const T* A, const T* invd, const T* b, T* x, double tolRel, int maxIt, int norm, int& iters,
+                 double& resOut, bool& conv, std::string& err);
+    // Sweep k of a solve (0: the first preconditioning, it + 1: iteration it)
+    // writes y buffer k & 1; the backward sweep re-arms buffer (k + 1) & 1.
     template <typename T>
-    int launchFwd(const T* F, const T* invd, const T* r, T* y, int mode, int it, int maxIt, int norm,
-                  double tolRel, std::string& err);
+    int launchFwd(const T* invd, const T* r, int k, int mode, int it, int maxIt, int norm, double tolRel,
+                  std::string& err);
     template <typename T>
-    int launchBwd(const T* F, const T* invd, T* y, const T* r, T* z, int parityOut, std::string& err);
+    int launchBwd(const T* invd, const T* r, T* z, int k, int parityOut, std::string& err);
     int launchMic(double tau, double ratio, std::string& err);
+    int launchMerge(std::string& err);
+    int convert32(std::string& err);
+    template <typename T>
+    StencilView<T> view(const DevStencil& s) const;
+    template <typename T>
+    T* ybuf(int k) const { return y_.as<T>() + (size_t)(k & 1) * PS_; }
     int syncScalars(std::string& err);
     int checkErrors(const char* where, std::string& err);
     template <typename T>
     int fwdGrid() const { return sizeof(T) == 8 ? gF64_ : gF32_; }
     template <typename T>
     int bwdGrid() const { return sizeof(T) == 8 ? gB64_ : gB32_; }
-    SweepPlan plan() const;
+    SweepPlan splan() const;  // sweeps: rows in super-level order
     Parts parts(int gb) const;
 
     hipStream_t stream_;
@@ -117,21 +143,28 @@ private:
     long long nnz_ = 0;
     int depth_ = 0;
     std::vector<int> hrs_, hcol_;
-    DevBuf rs_, col_, lend_, ubeg_, tpos_, order_, lev_, gateF_, gateB_;
+    DevBuf rs_, col_, lend_, ubeg_, tpos_, order_, lev_;  // pattern split, level order (factorisation)
+    std::vector<int> levRange_;             // per level: [first, last + 1) position in the level order
+    DevBuf micStart_, micProg_, live_;      // factorisation programs (mic_plan.h), eliminated columns
+    DevBuf sorder_, sgateF_, sgateB_;       // sweep order (super-levels) and its gates
+    DevStencil stF_, stB_;                  // merged forward / backward stencils
     DevBuf val_, F_, invd_;                 // FP64 A, factor (symmetric copy), 1/pivot
-    DevBuf val32_, F32_, invd32_;           // FP32 copies for the mixed form
+    DevBuf val32_, invd32_;                 // FP32 copies for the mixed form
     DevBuf b_, x_;                          // FP64 rhs / solution
     DevBuf r_, z_, s_, q_;                  // work vectors (sized for FP64; reused as FP32)
-    DevBuf y_, zp_, colp_;                  // sweep vectors in padded level order (P), column -> position
+    DevBuf y_, zp_;                         // sweep vectors in sweep order (2 x PS and PS)
     DevBuf r64_, d64_;                      // mixed form outer vectors
     DevBuf scal_, cnt_, part_;              // device scalars, counters, partials
     double* hscal_ = nullptr;               // pinned mirror of the scalars
     int P_ = 0;                             // padded length of the level order
+    int PS_ = 0;                            // padded length of the sweep order
+    int merge_ = 2;                         // levels per super-level of the sweeps
+    int mergeMaxY_ = 32;                    // most y-terms a merged row keeps
     int cus_ = 0;                           // compute units
     int gs_ = 0;                            // grid of the streaming kernels
     int gb_ = 0;                            // largest backward-sweep grid (rho partial slots)
-    int gF64_ = 0, gB64_ = 0, gF32_ = 0, gB32_ = 0, gM_ = 0;  // persistent grids
-    int gateDist_ = 2;
+    int gF64_ = 0, gB64_ = 0, gF32_ = 0, gB32_ = 0;  // persistent grids
+    int gateDist_ = 2;                      // sweep gates: groups back (forward) / ahead
     int pollCap_ = 1;
     int gateCap_ = 8;
     long long* trace_ = nullptr;            // set only while a timeline is recorded

@@ -1,19 +1,21 @@
 // Device MIC(0)-PCG (see pcg.h). Kernels are written for gfx950 wave64.
 //
 // Synchronisation design
-//   * Rows are processed in dependency-level order. The order is padded so
-//     every 64-row wavefront segment holds rows of one level only; rows of one
-//     level are independent, so no wavefront ever waits on one of its lanes.
-//   * The sweeps and the factorisation are persistent cooperative kernels:
-//     all blocks are co-resident and block b takes chunks b, b+G, b+2G, ... in
-//     order. The smallest unfinished chunk therefore always belongs to a
-//     running block whose dependencies (earlier chunks) are complete, so the
-//     spin-waits always end. No ticket atomics are needed.
+//   * The sweeps process rows in order of groups of q consecutive dependency
+//     levels, every group padded so a 64-row wavefront segment holds rows of
+//     one group only. Within a group, rows depend only on earlier groups
+//     (dependencies inside the group are substituted away, sweep_merge.h).
+//   * The sweeps are persistent cooperative kernels: all blocks are
+//     co-resident and block b takes chunks b, b+G, b+2G, ... in order. The
+//     smallest unfinished chunk therefore always belongs to a running block
+//     whose dependencies (earlier chunks) are complete, so the spin-waits
+//     always end. No ticket atomics are needed.
 //   * A row publishes its value with one agent-scope atomic store over a
 //     signalling-NaN sentinel; dependants poll the values they need, all at
 //     once. A wavefront first waits (one lane, with backoff) on a single
-//     representative row of the level `dist` levels back, which keeps the
-//     number of polling lanes to a few levels' worth.
+//     representative row of the group `dist` groups back, which keeps the
+//     number of polling lanes to a few groups' worth.
+//   * The factorisation runs one plain launch per dependency level.
 //   * Reductions are deterministic and atomic-free: every block writes one
 //     partial, and the next kernel in the stream reduces the partials it needs
 //     in a fixed order.
@@ -25,6 +27,8 @@
 
 #include "device_common.h"
 #include "pcg.h"
+#include "mic_plan.h"
+#include "sweep_merge.h"
 
 #define HIPCHK(expr)                                                                 \
     do {                                                                             \
@@ -39,6 +43,22 @@ namespace astg {
 
 constexpr int BS = 256;
 constexpr int kLpr = 8;  // lanes per row in the SpMV kernels
+
+// Sweep rows keep up to 16 dependencies in registers; wider plans use the
+// LDS-backed row (coefficients in LDS, one column per lane).
+template <typename T, int N>
+struct SweepRow {
+    typedef DepRow<T, N> type;
+    __device__ static void bind(type&) {}
+};
+template <typename T>
+struct SweepRow<T, 32> {
+    typedef DepRowWide<T, 32, BS> type;
+    __device__ static void bind(type& d) {
+        __shared__ T fl[32 * BS];
+        d.fl = fl + threadIdx.x;
+    }
+};
 
 DevBuf::~DevBuf() {
     if (p) (void)hipFree(p);
@@ -164,74 +184,63 @@ __device__ inline void gate_row(const T* vec, int target, int cap) {
 }
 
 // ---------------------------------------------------------------------------
-// MIC(0) factorisation, one row per lane, rows in level order.
+// MIC(0) factorisation, level-synchronous: one launch per dependency level.
 // Row k computes its pivot and column k of L (stored at the upper positions of
-// row k and, through tpos, at the matching lower positions of the rows below).
-// Every factor entry is written exactly once, over a sentinel, so readers poll
-// the entries themselves: no flags, fences or cache invalidation. Accumulation
-// order follows the reference's right-looking loop (pcg_solver.h:129-189):
-// contributions to row/column k arrive in ascending m.
+// row k and, through tpos, at the matching lower positions of the rows below);
+// everything it reads was written by rows of lower levels, i.e. by earlier
+// launches, so plain loads suffice. A row takes 32 lanes: lane j < 16 gathers
+// the pivot term of lower entry j, lane 16 + j computes upper entry j, each
+// from its own operand list (mic_plan.h), so all gathers are independent.
+// Accumulation order follows the reference's right-looking loop
+// (pcg_solver.h:129-189): contributions to row/column k arrive in ascending m.
 // ---------------------------------------------------------------------------
-// Polls F[pos[j]], j < cnt, until all are published; every round issues all
-// its loads before examining any.
-__device__ inline bool poll_all(const double* F, const int* pos, double* out, int cnt) {
-    unsigned ready = 0;
-    const unsigned full = cnt >= 32 ? ~0u : ((1u << cnt) - 1u);
-    for (int spins = 0;; ++spins) {
-        unsigned long long b[kMaxDeps];
-#pragma unroll
-        for (int j = 0; j < kMaxDeps; ++j) {
-            b[j] = kSent64;
-            if (j < cnt && !((ready >> j) & 1u))
-                b[j] = ld_agent_u64(reinterpret_cast<const unsigned long long*>(&F[pos[j]]));
-        }
-#pragma unroll
-        for (int j = 0; j < kMaxDeps; ++j)
-            if (b[j] != kSent64) {
-                out[j] = __longlong_as_double((long long)b[j]);
-                ready |= 1u << j;
-            }
-        if (ready == full) return true;
-        if (spins > kSpinLimit) return false;
-        __builtin_amdgcn_s_sleep(1);
-    }
+constexpr int kMicLanes = 32;
+
+// Rows whose column is eliminated: a stored, nonzero diagonal.
+__global__ void k_mic_live(int n, const int* rs, const int* lend, const int* col, const double* A, uint8_t* live) {
+    const int m = blockIdx.x * BS + threadIdx.x;
+    if (m >= n) return;
+    const int le = lend[m];
+    live[m] = (le < rs[m + 1] && col[le] == m && A[le] != 0.0) ? 1 : 0;
 }
 
-__device__ inline bool mic_row(const int k, const int r0, const int le, const int ub, const int r1, const int* rs,
-                               const int* lend, const int* ubeg, const int* col, const double* A, const int* tpos,
-                               double* F, double* invd, double tau, double ratio) {
-    bool ok = true;
+// Rows at level-order positions [t0, t1): one dependency level.
+__global__ void __launch_bounds__(BS) k_mic_level(int t0, int t1, const int* order, const long long* pst,
+                                                  const int* prog, const int* rs, const int* lend, const int* ubeg,
+                                                  const int* col, const double* A, const uint8_t* live,
+                                                  const int* tpos, double* F, double* invd, double tau, double ratio) {
+    const int sub = threadIdx.x & (kMicLanes - 1);
+    const int t = t0 + (blockIdx.x * BS + threadIdx.x) / kMicLanes;
+    const int k = t < t1 ? order[t] : -1;  // lanes of absent rows still join the shuffles
+    int r0 = 0, le = 0, ub = 0, r1 = 0;
+    if (k >= 0) {
+        r0 = rs[k];
+        le = lend[k];
+        ub = ubeg[k];
+        r1 = rs[k + 1];
+    }
     const int nl = le - r0;
-    int pos[kMaxDeps];
-    double Lk[kMaxDeps], fv[kMaxDeps];
-#pragma unroll
-    for (int j = 0; j < kMaxDeps; ++j)
-        if (j < nl) pos[j] = r0 + j;
-    ok &= poll_all(F, pos, Lk, nl);  // L(k,m) for every m in lower(k): waits for those columns
-    const double akk = (le < r1 && col[le] == k) ? A[le] : 0.0;
+    // pivot term of lower entry sub: L(k,m) and column m's entries missing from row k
+    double lkm = 0.0, missing = 0.0;
+    int lv = 0;
+    if (sub < nl) {
+        const int a = r0 + sub;
+        lv = live[col[a]];
+        lkm = F[a];
+        for (long long u = pst[a], ue = pst[a + 1]; u < ue; ++u) missing += F[prog[u]];
+    }
+    const double akk = (k >= 0 && le < r1 && col[le] == k) ? A[le] : 0.0;
     double d = akk;
-    for (int j = 0; j < nl; ++j) {
-        const int m = col[r0 + j];
-        const int mle = lend[m];
-        if (!(mle < rs[m + 1] && col[mle] == m && A[mle] != 0.0)) continue;  // null column: never eliminated
-        // column m below the diagonal: its entries, then the missing fill with
-        // respect to row k's pattern, summed in ascending row order
-        const int a0 = ubeg[m], na = rs[m + 1] - a0;
+    unsigned liveMask = 0;  // lower entries whose column is eliminated (null columns never are)
 #pragma unroll
-        for (int q = 0; q < kMaxDeps; ++q)
-            if (q < na) pos[q] = a0 + q;
-        ok &= poll_all(F, pos, fv, na);
-        double missing = 0.0;
-        int b = r0;
-        for (int q = 0; q < na; ++q) {
-            const int c = col[a0 + q];
-            if (c == k) continue;
-            while (b < r1 && col[b] < c) ++b;
-            if (b < r1 && col[b] == c) continue;
-            missing += fv[q];
+    for (int j = 0; j < kMaxDeps; ++j) {
+        const double lj = __shfl(lkm, j, kMicLanes), mj = __shfl(missing, j, kMicLanes);
+        const int vj = __shfl(lv, j, kMicLanes);
+        if (j < nl && vj) {
+            liveMask |= 1u << j;
+            d -= lj * lj;
+            d -= tau * lj * mj;
         }
-        d -= Lk[j] * Lk[j];
-        d -= tau * Lk[j] * missing;
     }
     double inv;
     if (akk == 0.0) {
@@ -241,53 +250,19 @@ __device__ inline bool mic_row(const int k, const int r0, const int le, const in
     } else {
         inv = 1.0 / sqrt(d);
     }
-    // column k of L: entries (i,k), i in upper(k)
-    for (int p = ub; p < r1; ++p) {
-        const int i = col[p];
+    // entry (i,k) of column k, i = upper entry sub - 16 of row k
+    const int p = ub + sub - kMaxDeps;
+    if (k >= 0 && sub >= kMaxDeps && p < r1) {
         double v = A[p];
-        // common m in lower(k) and lower(i), ascending
-        int nc = 0, ja[kMaxDeps];
-        int a = 0, c = rs[i];
-        const int ce = lend[i];
-        while (a < nl && c < ce) {
-            const int ma = col[r0 + a], mc = col[c];
-            if (ma < mc) {
-                ++a;
-            } else if (mc < ma) {
-                ++c;
-            } else {
-                const int mr = lend[ma];
-                if (mr < rs[ma + 1] && col[mr] == ma && A[mr] != 0.0 && nc < kMaxDeps) {
-                    ja[nc] = a;
-                    pos[nc] = c;
-                    ++nc;
-                }
-                ++a;
-                ++c;
-            }
+        for (long long u = pst[p], ue = pst[p + 1]; u < ue; u += 2) {
+            const int j = prog[u], c = prog[u + 1];
+            if ((liveMask >> j) & 1u) v -= F[r0 + j] * F[c];
         }
-        ok &= poll_all(F, pos, fv, nc);
-        for (int q = 0; q < nc; ++q) v -= Lk[ja[q]] * fv[q];
         if (akk != 0.0) v *= inv;
-        st_agent_u64(reinterpret_cast<unsigned long long*>(&F[p]), (unsigned long long)__double_as_longlong(v));
-        st_agent_u64(reinterpret_cast<unsigned long long*>(&F[tpos[p]]), (unsigned long long)__double_as_longlong(v));
+        F[p] = v;
+        F[tpos[p]] = v;
     }
-    st_agent_u64(reinterpret_cast<unsigned long long*>(&invd[k]), (unsigned long long)__double_as_longlong(inv));
-    return ok;
-}
-
-__global__ void __launch_bounds__(BS) k_mic(SweepPlan S, const int* rs, const int* lend, const int* ubeg,
-                                            const int* col, const double* A, const int* tpos, double* F,
-                                            double* invd, double tau, double ratio, int* cnt) {
-    for (int c = blockIdx.x; c < S.nchunks; c += gridDim.x) {
-        const int t = c * BS + threadIdx.x;
-        const int gp = S.gateF[t >> 6];
-        gate_row(invd, gp >= 0 ? S.order[gp] : -1, S.gateCap);
-        const int k = S.order[t];
-        if (k < 0) continue;
-        if (!mic_row(k, rs[k], lend[k], ubeg[k], rs[k + 1], rs, lend, ubeg, col, A, tpos, F, invd, tau, ratio))
-            atomicOr(&cnt[C_ERR], E_MIC);
-    }
+    if (k >= 0 && sub == 0) invd[k] = inv;
 }
 
 // ---------------------------------------------------------------------------
@@ -344,26 +319,30 @@ __device__ inline bool fwd_prologue(const Parts& Pt, double* scal, int mode, int
     return conv || it + 1 >= maxIt;
 }
 
-template <typename T>
-__global__ void __launch_bounds__(BS) k_fwd(SweepPlan S, const int* rs, const int* lend, const int* col, const T* F,
-                                            const T* invd, const T* r, T* y, T* z, Parts Pt, int* cnt, double* scal,
-                                            int mode, int it, int maxIt, int norm, double tolRel) {
+// Forward sweep y = L^-1 r over the merged stencil:
+//   y_i = d_i (r_i - sum_j R_ij r_j - sum_m W_im y_m)
+// (with no merging R is empty and W is row i of L, in the reference's order).
+// The r-terms need no other row, so they are summed before the row waits.
+template <typename T, int N>
+__global__ void __launch_bounds__(BS) k_fwd(SweepPlan S, StencilView<T> St, const T* invd, const T* r, T* y, T* zp,
+                                            Parts Pt, int* cnt, double* scal, int mode, int it, int maxIt, int norm,
+                                            double tolRel) {
     if (scal[S_DONE] != 0.0) return;
     if (fwd_prologue(Pt, scal, mode, it, maxIt, norm, tolRel)) return;
     for (int c = blockIdx.x; c < S.nchunks; c += gridDim.x) {
         const int t = c * BS + threadIdx.x;
         const int i = S.order[t];
         // row data that does not depend on other rows is loaded before the gate
-        DepRow<T> d;
+        typename SweepRow<T, N>::type d;
+        SweepRow<T, N>::bind(d);
         T acc = 0, inv = 0;
-        int p0 = 0, pe = 0;
         if (i >= 0) {
-            p0 = rs[i];
-            pe = lend[i];
+            const int b = St.start[t], e = St.start[t + 1], br = b + St.nr[t];
             acc = r[i];
             inv = invd[i];
-            d.preload(col, F, p0, pe);
-            Sentinel<T>::storeSent(&z[t]);
+            for (int p = b; p < br; ++p) acc -= St.coef[p] * r[St.idx[p]];
+            d.preload(St.idx, St.coef, br, e);
+            Sentinel<T>::storeSent(&zp[t]);  // armed for the backward sweep
         }
         gate_row(y, S.gateF[t >> 6], S.gateCap);
         const long long tg = S.trace ? wall_clock64() : 0;
@@ -382,28 +361,31 @@ __global__ void __launch_bounds__(BS) k_fwd(SweepPlan S, const int* rs, const in
     }
 }
 
-// Backward sweep z = L^-T y in reverse level order, fused with the partial dot
-// z.r (one partial per block, fixed chunk assignment: deterministic).
-template <typename T>
-__global__ void __launch_bounds__(BS) k_bwd_dot(SweepPlan S, const int* rs, const int* ubeg, const int* col,
-                                                const T* F, const T* invd, T* y, const T* r, T* z, T* zp, Parts Pt,
-                                                int* cnt, const double* scal, int parityOut) {
+// Backward sweep z = L^-T y in reverse sweep order over the merged stencil
+//   z_k = d_k (y_k - sum_j R_kj y_j - sum_m W_km z_m),
+// fused with the partial dot z.r (one partial per block, fixed chunk
+// assignment: deterministic). y is complete (previous kernel); the other y
+// buffer is re-armed here for the next forward sweep.
+template <typename T, int N>
+__global__ void __launch_bounds__(BS) k_bwd_dot(SweepPlan S, StencilView<T> St, const T* invd, const T* y, T* ynext,
+                                                const T* r, T* z, T* zp, Parts Pt, int* cnt, const double* scal,
+                                                int parityOut) {
     if (scal[S_DONE] != 0.0) return;
     double v = 0.0;
     for (int c = blockIdx.x; c < S.nchunks; c += gridDim.x) {
         const int pos = S.P - 1 - (c * BS + threadIdx.x);
         const int k = S.order[pos];
-        DepRow<T> d;
+        typename SweepRow<T, N>::type d;
+        SweepRow<T, N>::bind(d);
         T acc = 0, inv = 0, rk = 0;
-        int p0 = 0, pe = 0;
         if (k >= 0) {
-            p0 = ubeg[k];
-            pe = rs[k + 1];
+            const int b = St.start[pos], e = St.start[pos + 1], br = b + St.nr[pos];
             acc = y[pos];
-            Sentinel<T>::storeSent(&y[pos]);  // re-armed for the next forward sweep
+            for (int p = b; p < br; ++p) acc -= St.coef[p] * y[St.idx[p]];
+            Sentinel<T>::storeSent(&ynext[pos]);
             inv = invd[k];
             rk = r[k];
-            d.preload(col, F, p0, pe);
+            d.preload(St.idx, St.coef, br, e);
         }
         gate_row(zp, S.gateB[pos >> 6], S.gateCap);
         const long long tg = S.trace ? wall_clock64() : 0;
@@ -426,6 +408,34 @@ __global__ void __launch_bounds__(BS) k_bwd_dot(SweepPlan S, const int* rs, cons
     }
     const double t = block_reduce<false>(v);
     if (threadIdx.x == 0) (parityOut ? Pt.rho1 : Pt.rho0)[blockIdx.x] = t;
+}
+
+// Coefficients of the merged stencils from the current factor: replays the
+// row's slot program (sweep_merge.h). Slots start at zero, receive the row's
+// own factor entries, then each substituted key j in order turns its slot into
+// the r-coefficient a = W_j d_j and subtracts a * (row j of the factor) from
+// the slots of j's dependencies. DIR 0: lower rows (forward), 1: upper.
+template <int DIR>
+__global__ void __launch_bounds__(BS) k_merge(int P, const int* order, const int* rs, const int* lend,
+                                              const int* ubeg, const double* F, const double* invd, const int* sb,
+                                              const int* nr, const int* idx, const int* us, const uint8_t* upd,
+                                              double* C) {
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= P) return;
+    const int i = order[t];
+    if (i < 0) return;
+    const int b = sb[t], e = sb[t + 1], nR = nr[t];
+    for (int s = b; s < e; ++s) C[s] = 0.0;
+    int u = us[t];
+    const int p0 = DIR == 0 ? rs[i] : ubeg[i], p1 = DIR == 0 ? lend[i] : rs[i + 1];
+    for (int p = p0; p < p1; ++p) C[b + upd[u++]] += F[p];
+    for (int x = 0; x < nR; ++x) {
+        const int j = DIR == 0 ? idx[b + x] : order[idx[b + x]];
+        const double a = C[b + x] * invd[j];
+        C[b + x] = a;
+        const int q0 = DIR == 0 ? rs[j] : ubeg[j], q1 = DIR == 0 ? lend[j] : rs[j + 1];
+        for (int p = q0; p < q1; ++p) C[b + upd[u++]] -= a * F[p];
+    }
 }
 
 // s = z + beta s with beta = rho(it+1) / rho(it); mode 0: s = z after the first
@@ -605,6 +615,12 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
     if (const char* g = getenv("ASTG_GATE_DIST")) gateDist_ = std::max(1, atoi(g));
     if (const char* g = getenv("ASTG_POLL_CAP")) pollCap_ = std::max(1, atoi(g));
     if (const char* g = getenv("ASTG_GATE_CAP")) gateCap_ = std::max(1, atoi(g));
+    // sweep merging: levels per group (1 = plain level order) and the most
+    // y-terms a merged row may keep (16: registers, 32: LDS-backed rows)
+    merge_ = 2;
+    mergeMaxY_ = 16;
+    if (const char* g = getenv("ASTG_MERGE")) merge_ = std::max(1, std::min(8, atoi(g)));
+    if (const char* g = getenv("ASTG_MERGE_MAXY")) mergeMaxY_ = atoi(g) > 16 ? 32 : 16;
     int dev = 0, cus = 0;
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -667,83 +683,148 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
     }
     depth_ = sortedLev[n - 1] + 1;
 
-    // Pad every level to a multiple of 64 rows so that a wavefront never holds
-    // two rows of different levels (rows of one level are independent).
-    std::vector<int> padded, levStart(depth_, 0), levRows(depth_, 0);
-    padded.reserve((size_t)n + (size_t)depth_ * 64 + BS);
-    for (int i = 0; i < n;) {
-        int j = i;
-        const int L = sortedLev[i];
-        levStart[L] = (int)padded.size();
-        while (j < n && sortedLev[j] == L) padded.push_back(sortedRow[j++]);
-        levRows[L] = j - i;
-        while (padded.size() % 64) padded.push_back(-1);
-        i = j;
-    }
-    while (padded.size() % BS) padded.push_back(-1);
-    P_ = (int)padded.size();
-    // gate rows: for each 64-row segment of level L, one row of level L-dist
-    // (forward) and one of level L+dist (backward), spread over the level
-    {
-        const int nseg = P_ / 64;
-        std::vector<int> gF(nseg, -1), gB(nseg, -1), segLev(nseg, -1);
-        for (int L = 0; L < depth_; ++L) {
-            const int s0 = levStart[L] / 64, s1 = (levStart[L] + levRows[L] + 63) / 64;
-            for (int sg = s0; sg < s1; ++sg) segLev[sg] = L;
+    // Rows in order of groups of q consecutive levels, every group padded to a
+    // multiple of 64 rows so that a wavefront never holds rows of two groups;
+    // gate rows: for each 64-row segment of group G, one row of group G-dist
+    // (forward) and one of group G+dist (backward), spread over that group.
+    auto buildOrder = [&](int q, int dist, std::vector<int>& padded, std::vector<int>& gF, std::vector<int>& gB) {
+        const int ng = (depth_ + q - 1) / q;
+        std::vector<int> gStart(ng, 0), gRows(ng, 0);
+        padded.clear();
+        padded.reserve((size_t)n + (size_t)ng * 64 + BS);
+        for (int i = 0; i < n;) {
+            int j = i;
+            const int G = sortedLev[i] / q;
+            gStart[G] = (int)padded.size();
+            while (j < n && sortedLev[j] / q == G) padded.push_back(sortedRow[j++]);
+            gRows[G] = j - i;
+            while (padded.size() % 64) padded.push_back(-1);
+            i = j;
+        }
+        while (padded.size() % BS) padded.push_back(-1);
+        const int nseg = (int)padded.size() / 64;
+        gF.assign(nseg, -1);
+        gB.assign(nseg, -1);
+        std::vector<int> segG(nseg, -1);
+        for (int G = 0; G < ng; ++G) {
+            const int s0 = gStart[G] / 64, s1 = (gStart[G] + gRows[G] + 63) / 64;
+            for (int sg = s0; sg < s1; ++sg) segG[sg] = G;
         }
         for (int sg = 0; sg < nseg; ++sg) {
-            const int L = segLev[sg];
-            if (L < 0) continue;
+            const int G = segG[sg];
+            if (G < 0) continue;
             const unsigned h = (unsigned)sg * 2654435761u;
-            const int lf = L - gateDist_, lb = L + gateDist_;
-            if (lf >= 0) gF[sg] = levStart[lf] + (int)(h % (unsigned)levRows[lf]);
-            if (lb < depth_) gB[sg] = levStart[lb] + (int)(h % (unsigned)levRows[lb]);
+            const int gf = G - dist, gb = G + dist;
+            if (gf >= 0) gF[sg] = gStart[gf] + (int)(h % (unsigned)gRows[gf]);
+            if (gb < ng) gB[sg] = gStart[gb] + (int)(h % (unsigned)gRows[gb]);
         }
-        HIPCHK(gateF_.alloc((size_t)nseg * 4));
-        HIPCHK(gateB_.alloc((size_t)nseg * 4));
-        HIPCHK(hipMemcpyAsync(gateF_.p, gF.data(), (size_t)nseg * 4, hipMemcpyHostToDevice, stream_));
-        HIPCHK(hipMemcpyAsync(gateB_.p, gB.data(), (size_t)nseg * 4, hipMemcpyHostToDevice, stream_));
-    }
-    HIPCHK(order_.alloc((size_t)P_ * 4));
-    HIPCHK(hipMemcpyAsync(order_.p, padded.data(), (size_t)P_ * 4, hipMemcpyHostToDevice, stream_));
-    // The sweep vectors live in padded level order: a wavefront's dependencies
-    // sit in the segments of the previous levels in the same relative order,
-    // so its polls touch few, shared cache lines. colp = column -> position.
+    };
+    auto upload = [&](DevBuf& d, const void* src, size_t bytes) -> hipError_t {
+        hipError_t e = d.alloc(bytes);
+        if (e != hipSuccess) return e;
+        return hipMemcpyAsync(d.p, src, bytes, hipMemcpyHostToDevice, stream_);
+    };
+    // factorisation: rows in level order, one launch per level
+    std::vector<int> padded, gF, gB;
+    buildOrder(1, 1, padded, gF, gB);
+    P_ = (int)padded.size();
+    HIPCHK(upload(order_, padded.data(), (size_t)P_ * 4));
     {
-        std::vector<int> pos(n), colp(nnz_);
+        std::vector<int> levOfRow(n);
+        for (int q = 0; q < n; ++q) levOfRow[sortedRow[q]] = sortedLev[q];
+        std::vector<int> first(depth_, -1), last(depth_, -1);
         for (int q = 0; q < P_; ++q)
-            if (padded[q] >= 0) pos[padded[q]] = q;
-        for (long long p = 0; p < nnz_; ++p) colp[p] = pos[col[p]];
-        HIPCHK(colp_.alloc((size_t)nnz_ * 4));
-        HIPCHK(y_.alloc((size_t)P_ * 8));
-        HIPCHK(zp_.alloc((size_t)P_ * 8));
-        HIPCHK(hipMemcpyAsync(colp_.p, colp.data(), (size_t)nnz_ * 4, hipMemcpyHostToDevice, stream_));
+            if (padded[q] >= 0) {
+                const int L = levOfRow[padded[q]];
+                if (first[L] < 0) first[L] = q;
+                last[L] = q;
+            }
+        levRange_.assign(2 * (size_t)depth_, 0);
+        for (int L = 0; L < depth_; ++L) {
+            levRange_[2 * L] = first[L];
+            levRange_[2 * L + 1] = last[L] + 1;
+        }
+    }
+    {
+        std::vector<long long> ms;
+        std::vector<int> mp;
+        planMic(n, rowstart, lend.data(), ubeg.data(), col, ms, mp);
+        HIPCHK(upload(micStart_, ms.data(), ms.size() * 8));
+        HIPCHK(upload(micProg_, mp.data(), mp.size() * 4));
+        HIPCHK(live_.alloc((size_t)n));
         HIPCHK(hipStreamSynchronize(stream_));
     }
+    // sweeps: q levels per group, merged stencils (sweep_merge.h)
+    buildOrder(merge_, gateDist_, padded, gF, gB);
+    PS_ = (int)padded.size();
+    HIPCHK(upload(sorder_, padded.data(), (size_t)PS_ * 4));
+    HIPCHK(upload(sgateF_, gF.data(), gF.size() * 4));
+    HIPCHK(upload(sgateB_, gB.data(), gB.size() * 4));
+    {
+        std::vector<int> lev(n), pos(n);
+        for (int q = 0; q < n; ++q) lev[sortedRow[q]] = sortedLev[q];
+        for (int q = 0; q < PS_; ++q)
+            if (padded[q] >= 0) pos[padded[q]] = q;
+        const int maxY = merge_ > 1 ? mergeMaxY_ : kMaxDeps;
+        DevStencil* dst[2] = {&stF_, &stB_};
+        for (int dir = 0; dir < 2; ++dir) {
+            SweepStencil st;
+            planStencil(dir, n, rowstart, lend.data(), ubeg.data(), col, lev.data(), merge_, maxY, padded.data(),
+                        PS_, pos.data(), st);
+            DevStencil& D = *dst[dir];
+            D.E = (long long)st.idx.size();
+            D.R = st.rTerms;
+            D.kept = st.kept;
+            D.maxY = 0;
+            for (int t = 0; t < PS_; ++t) D.maxY = std::max(D.maxY, st.start[t + 1] - st.start[t] - st.nr[t]);
+            D.cap = D.maxY <= 16 ? 16 : 32;
+            HIPCHK(upload(D.start, st.start.data(), st.start.size() * 4));
+            HIPCHK(upload(D.nr, st.nr.data(), st.nr.size() * 4));
+            HIPCHK(upload(D.idx, st.idx.data(), st.idx.size() * 4));
+            HIPCHK(upload(D.ustart, st.ustart.data(), st.ustart.size() * 4));
+            HIPCHK(upload(D.upd, st.upd.data(), st.upd.size()));
+            HIPCHK(D.coef.alloc((size_t)D.E * 8));
+            // the host vectors must outlive the copies
+            HIPCHK(hipStreamSynchronize(stream_));
+        }
+        HIPCHK(y_.alloc((size_t)PS_ * 2 * 8));
+        HIPCHK(zp_.alloc((size_t)PS_ * 8));
+    }
     // persistent grids (all blocks co-resident)
-    const int nchunks = P_ / BS;
-    gF64_ = coopBlocks(k_fwd<double>, cus, nchunks);
-    gB64_ = coopBlocks(k_bwd_dot<double>, cus, nchunks);
-    gF32_ = coopBlocks(k_fwd<float>, cus, nchunks);
-    gB32_ = coopBlocks(k_bwd_dot<float>, cus, nchunks);
-    gM_ = coopBlocks(k_mic, cus, nchunks);
+    const int schunks = PS_ / BS;
+    gF64_ = stF_.cap == 16 ? coopBlocks(k_fwd<double, 16>, cus, schunks) : coopBlocks(k_fwd<double, 32>, cus, schunks);
+    gF32_ = stF_.cap == 16 ? coopBlocks(k_fwd<float, 16>, cus, schunks) : coopBlocks(k_fwd<float, 32>, cus, schunks);
+    gB64_ = stB_.cap == 16 ? coopBlocks(k_bwd_dot<double, 16>, cus, schunks)
+                           : coopBlocks(k_bwd_dot<double, 32>, cus, schunks);
+    gB32_ = stB_.cap == 16 ? coopBlocks(k_bwd_dot<float, 16>, cus, schunks)
+                           : coopBlocks(k_bwd_dot<float, 32>, cus, schunks);
     gb_ = std::max(std::max(gB64_, gB32_), 1);
     HIPCHK(part_.alloc((size_t)(2 * gs_ + 2 * gb_ + 8) * 8));
     HIPCHK(hipStreamSynchronize(stream_));
     return ASTG_OK;
 }
 
-SweepPlan PcgDevice::plan() const {
+SweepPlan PcgDevice::splan() const {
     SweepPlan s;
-    s.order = order_.as<int>();
-    s.gateF = gateF_.as<int>();
-    s.gateB = gateB_.as<int>();
-    s.P = P_;
-    s.nchunks = P_ / BS;
+    s.order = sorder_.as<int>();
+    s.gateF = sgateF_.as<int>();
+    s.gateB = sgateB_.as<int>();
+    s.P = PS_;
+    s.nchunks = PS_ / BS;
     s.pollCap = pollCap_;
     s.gateCap = gateCap_;
     s.trace = trace_;
     return s;
+}
+
+template <typename T>
+StencilView<T> PcgDevice::view(const DevStencil& d) const {
+    StencilView<T> v;
+    v.start = d.start.as<int>();
+    v.nr = d.nr.as<int>();
+    v.idx = d.idx.as<int>();
+    v.coef = sizeof(T) == 8 ? reinterpret_cast<const T*>(d.coef.p) : reinterpret_cast<const T*>(d.coef32.p);
+    return v;
 }
 
 Parts PcgDevice::parts(int gb) const {
@@ -770,20 +851,52 @@ int PcgDevice::checkErrors(const char* where, std::string& err) {
 }
 
 int PcgDevice::launchMic(double tau, double ratio, std::string& err) {
-    k_fill_sent<double><<<blocks(n_), BS, 0, stream_>>>(invd_.as<double>(), n_);
-    k_fill_sent<double><<<blocks(nnz_), BS, 0, stream_>>>(F_.as<double>(), (int)nnz_);
-    HIPCHK(launchCoop(k_mic, gM_, stream_, plan(), (const int*)rs_.as<int>(), (const int*)lend_.as<int>(),
-                      (const int*)ubeg_.as<int>(), (const int*)col_.as<int>(), (const double*)val_.as<double>(),
-                      (const int*)tpos_.as<int>(), F_.as<double>(), invd_.as<double>(), tau, ratio, cnt_.as<int>()));
+    k_mic_live<<<blocks(n_), BS, 0, stream_>>>(n_, rs_.as<int>(), lend_.as<int>(), col_.as<int>(), val_.as<double>(),
+                                                 live_.as<uint8_t>());
+    for (int L = 0; L < depth_; ++L) {
+        const int t0 = levRange_[2 * L], t1 = levRange_[2 * L + 1];
+        k_mic_level<<<blocks((long)(t1 - t0) * kMicLanes), BS, 0, stream_>>>(
+            t0, t1, order_.as<int>(), micStart_.as<long long>(), micProg_.as<int>(), rs_.as<int>(), lend_.as<int>(),
+            ubeg_.as<int>(), col_.as<int>(), val_.as<double>(), live_.as<uint8_t>(), tpos_.as<int>(), F_.as<double>(),
+            invd_.as<double>(), tau, ratio);
+    }
+    HIPCHK(hipGetLastError());
+    return ASTG_OK;
+}
+
+int PcgDevice::launchMerge(std::string& err) {
+    const DevStencil* st[2] = {&stF_, &stB_};
+    for (int dir = 0; dir < 2; ++dir) {
+        const DevStencil& d = *st[dir];
+        auto k = dir == 0 ? k_merge<0> : k_merge<1>;
+        k<<<blocks(PS_), BS, 0, stream_>>>(PS_, sorder_.as<int>(), rs_.as<int>(), lend_.as<int>(), ubeg_.as<int>(),
+                                           F_.as<double>(), invd_.as<double>(), d.start.as<int>(), d.nr.as<int>(),
+                                           d.idx.as<int>(), d.ustart.as<int>(), d.upd.as<uint8_t>(),
+                                           d.coef.as<double>());
+    }
+    HIPCHK(hipGetLastError());
+    return ASTG_OK;
+}
+
+// FP32 copies of A, the pivots and the merged stencil coefficients.
+int PcgDevice::convert32(std::string& err) {
+    const long long nnz = nnz_;
+    HIPCHK(val32_.alloc((size_t)nnz * 4));
+    HIPCHK(invd32_.alloc((size_t)n_ * 4));
+    HIPCHK(stF_.coef32.alloc((size_t)stF_.E * 4));
+    HIPCHK(stB_.coef32.alloc((size_t)stB_.E * 4));
+    k_d2f<<<blocks(nnz), BS, 0, stream_>>>(val_.as<double>(), val32_.as<float>(), nnz);
+    k_d2f<<<blocks(n_), BS, 0, stream_>>>(invd_.as<double>(), invd32_.as<float>(), n_);
+    k_d2f<<<blocks(stF_.E), BS, 0, stream_>>>(stF_.coef.as<double>(), stF_.coef32.as<float>(), stF_.E);
+    k_d2f<<<blocks(stB_.E), BS, 0, stream_>>>(stB_.coef.as<double>(), stB_.coef32.as<float>(), stB_.E);
+    HIPCHK(hipGetLastError());
     return ASTG_OK;
 }
 
 int PcgDevice::factor(double tau, double ratio, std::string& err) {
-    k_zero_counters<<<1, 64, 0, stream_>>>(cnt_.as<int>());
     int rc = launchMic(tau, ratio, err);
     if (rc) return rc;
-    HIPCHK(hipGetLastError());
-    return checkErrors("MIC(0) factorisation", err);
+    return launchMerge(err);
 }
 
 int PcgDevice::syncScalars(std::string& err) {
@@ -793,34 +906,44 @@ int PcgDevice::syncScalars(std::string& err) {
 }
 
 template <typename T>
-int PcgDevice::launchFwd(const T* F, const T* invd, const T* r, T* y, int mode, int it, int maxIt, int norm,
-                         double tolRel, std::string& err) {
+int PcgDevice::launchFwd(const T* invd, const T* r, int k, int mode, int it, int maxIt, int norm, double tolRel,
+                         std::string& err) {
     const int g = fwdGrid<T>(), gb = bwdGrid<T>();
-    HIPCHK(launchCoop(k_fwd<T>, g, stream_, plan(), (const int*)rs_.as<int>(), (const int*)lend_.as<int>(),
-                      (const int*)colp_.as<int>(), F, invd, r, y, zp_.as<T>(), parts(gb), cnt_.as<int>(),
-                      scal_.as<double>(), mode, it, maxIt, norm, tolRel));
+    const StencilView<T> V = view<T>(stF_);
+    T* y = ybuf<T>(k);
+    if (stF_.cap == 16)
+        HIPCHK(launchCoop(k_fwd<T, 16>, g, stream_, splan(), V, invd, r, y, zp_.as<T>(), parts(gb), cnt_.as<int>(),
+                          scal_.as<double>(), mode, it, maxIt, norm, tolRel));
+    else
+        HIPCHK(launchCoop(k_fwd<T, 32>, g, stream_, splan(), V, invd, r, y, zp_.as<T>(), parts(gb), cnt_.as<int>(),
+                          scal_.as<double>(), mode, it, maxIt, norm, tolRel));
     return ASTG_OK;
 }
 
 template <typename T>
-int PcgDevice::launchBwd(const T* F, const T* invd, T* y, const T* r, T* z, int parityOut, std::string& err) {
+int PcgDevice::launchBwd(const T* invd, const T* r, T* z, int k, int parityOut, std::string& err) {
     const int gb = bwdGrid<T>();
-    HIPCHK(launchCoop(k_bwd_dot<T>, gb, stream_, plan(), (const int*)rs_.as<int>(), (const int*)ubeg_.as<int>(),
-                      (const int*)colp_.as<int>(), F, invd, y, r, z, zp_.as<T>(), parts(gb), cnt_.as<int>(),
-                      (const double*)scal_.as<double>(), parityOut));
+    const StencilView<T> V = view<T>(stB_);
+    const T* y = ybuf<T>(k);
+    T* ynext = ybuf<T>(k + 1);
+    if (stB_.cap == 16)
+        HIPCHK(launchCoop(k_bwd_dot<T, 16>, gb, stream_, splan(), V, invd, y, ynext, r, z, zp_.as<T>(), parts(gb),
+                          cnt_.as<int>(), (const double*)scal_.as<double>(), parityOut));
+    else
+        HIPCHK(launchCoop(k_bwd_dot<T, 32>, gb, stream_, splan(), V, invd, y, ynext, r, z, zp_.as<T>(), parts(gb),
+                          cnt_.as<int>(), (const double*)scal_.as<double>(), parityOut));
     return ASTG_OK;
 }
 
 template <typename T>
-int PcgDevice::pcgInner(const T* A, const T* F, const T* invd, const T* b, T* x, double tolRel, int maxIt, int norm,
-                        int& iters, double& resOut, bool& conv, std::string& err) {
+int PcgDevice::pcgInner(const T* A, const T* invd, const T* b, T* x, double tolRel, int maxIt, int norm, int& iters,
+                        double& resOut, bool& conv, std::string& err) {
     const int n = n_, gs = gs_;
     const int gb = bwdGrid<T>();
     T* r = r_.as<T>();
     T* z = z_.as<T>();
     T* s = s_.as<T>();
     T* q = q_.as<T>();
-    T* y = y_.as<T>();
     const int* rs = rs_.as<int>();
     const int* col = col_.as<int>();
     double* scal = scal_.as<double>();
@@ -828,9 +951,9 @@ int PcgDevice::pcgInner(const T* A, const T* F, const T* invd, const T* b, T* x,
     int rc;
     k_zero64<<<1, BS, 0, stream_>>>(32, scal);
     k_init<T><<<gs, BS, 0, stream_>>>(n, b, x, r, Pt, norm);
-    k_fill_sent<T><<<blocks(P_), BS, 0, stream_>>>(y, P_);
-    if ((rc = launchFwd<T>(F, invd, r, y, 0, 0, maxIt, norm, tolRel, err))) return rc;
-    if ((rc = launchBwd<T>(F, invd, y, r, z, 0, err))) return rc;
+    k_fill_sent<T><<<blocks(2 * PS_), BS, 0, stream_>>>(y_.as<T>(), 2 * PS_);
+    if ((rc = launchFwd<T>(invd, r, 0, 0, 0, maxIt, norm, tolRel, err))) return rc;
+    if ((rc = launchBwd<T>(invd, r, z, 0, 0, err))) return rc;
     k_xpby<T><<<gs, BS, 0, stream_>>>(n, z, s, Pt, scal, 0, 0);
     HIPCHK(hipGetLastError());
     int it = 0;
@@ -839,8 +962,8 @@ int PcgDevice::pcgInner(const T* A, const T* F, const T* invd, const T* b, T* x,
         for (int j = 0; j < batch && it < maxIt; ++j, ++it) {
             k_spmv_dot<T><<<gs, BS, 0, stream_>>>(n, rs, col, A, s, q, Pt, scal);
             k_axpy_res<T><<<gs, BS, 0, stream_>>>(n, s, q, x, r, Pt, scal, norm, it);
-            if ((rc = launchFwd<T>(F, invd, r, y, 1, it, maxIt, norm, tolRel, err))) return rc;
-            if ((rc = launchBwd<T>(F, invd, y, r, z, (it + 1) & 1, err))) return rc;
+            if ((rc = launchFwd<T>(invd, r, it + 1, 1, it, maxIt, norm, tolRel, err))) return rc;
+            if ((rc = launchBwd<T>(invd, r, z, it + 1, (it + 1) & 1, err))) return rc;
             k_xpby<T><<<gs, BS, 0, stream_>>>(n, z, s, Pt, scal, 1, it);
         }
         HIPCHK(hipGetLastError());
@@ -871,8 +994,8 @@ int PcgDevice::solve(const astg_pcg_params& prm, astg_pcg_result& res, std::stri
         int it = 0;
         double r = 0;
         bool conv = false;
-        int rc = pcgInner<double>(val_.as<double>(), F_.as<double>(), invd_.as<double>(), b_.as<double>(),
-                                  x_.as<double>(), prm.tol, prm.max_iterations, prm.norm, it, r, conv, err);
+        int rc = pcgInner<double>(val_.as<double>(), invd_.as<double>(), b_.as<double>(), x_.as<double>(), prm.tol,
+                                  prm.max_iterations, prm.norm, it, r, conv, err);
         if (rc) return rc;
         res.iterations = it;
         res.residual = r;
@@ -882,15 +1005,9 @@ int PcgDevice::solve(const astg_pcg_params& prm, astg_pcg_result& res, std::stri
         return ASTG_OK;
     }
     // mixed: FP32 inner PCG, FP64 outer residual
-    const long nnz = nnz_;
-    HIPCHK(val32_.alloc((size_t)nnz * 4));
-    HIPCHK(F32_.alloc((size_t)nnz * 4));
-    HIPCHK(invd32_.alloc((size_t)n * 4));
     HIPCHK(r64_.alloc((size_t)n * 8));
     HIPCHK(d64_.alloc((size_t)n * 8));
-    k_d2f<<<blocks(nnz), BS, 0, stream_>>>(val_.as<double>(), val32_.as<float>(), nnz);
-    k_d2f<<<blocks(nnz), BS, 0, stream_>>>(F_.as<double>(), F32_.as<float>(), nnz);
-    k_d2f<<<blocks(n), BS, 0, stream_>>>(invd_.as<double>(), invd32_.as<float>(), n);
+    if (int rc = convert32(err)) return rc;
     k_zero64<<<blocks(n), BS, 0, stream_>>>(n, x_.as<double>());
     float* rhs32 = reinterpret_cast<float*>(d64_.p);
     float* d32 = reinterpret_cast<float*>(d64_.as<char>() + (size_t)n * 4);
@@ -925,8 +1042,8 @@ int PcgDevice::solve(const astg_pcg_params& prm, astg_pcg_result& res, std::stri
         int it = 0;
         double ir = 0;
         bool ic = false;
-        rc = pcgInner<float>(val32_.as<float>(), F32_.as<float>(), invd32_.as<float>(), rhs32, d32, eta,
-                             prm.max_iterations - total, prm.norm, it, ir, ic, err);
+        rc = pcgInner<float>(val32_.as<float>(), invd32_.as<float>(), rhs32, d32, eta, prm.max_iterations - total,
+                             prm.norm, it, ir, ic, err);
         if (rc) return rc;
         total += it;
         ++outer;
@@ -975,13 +1092,9 @@ int PcgDevice::benchKernel(int which, int reps, double& avgMs, double& bytes, st
     const int* col = col_.as<int>();
     double* scal = scal_.as<double>();
     const bool f32 = which == ASTG_K_SPMV32 || which == ASTG_K_FWD32 || which == ASTG_K_BWD32;
-    if (f32 && !val32_.p) {
-        HIPCHK(val32_.alloc((size_t)nnz * 4));
-        HIPCHK(F32_.alloc((size_t)nnz * 4));
-        HIPCHK(invd32_.alloc((size_t)n * 4));
-        k_d2f<<<blocks(nnz), BS, 0, stream_>>>(val_.as<double>(), val32_.as<float>(), nnz);
-        k_d2f<<<blocks(nnz), BS, 0, stream_>>>(F_.as<double>(), F32_.as<float>(), nnz);
-        k_d2f<<<blocks(n), BS, 0, stream_>>>(invd_.as<double>(), invd32_.as<float>(), n);
+    if (f32 && (!val32_.p || stF_.coef32.bytes < (size_t)stF_.E * 4)) {
+        int rc = convert32(err);
+        if (rc) return rc;
     }
     const double w = f32 ? 4.0 : 8.0;
     switch (which) {
@@ -990,12 +1103,14 @@ int PcgDevice::benchKernel(int which, int reps, double& avgMs, double& bytes, st
             bytes = (w + 4) * nnz + 4.0 * (n + 1) + 3 * w * n;
             break;
         case ASTG_K_FWD64: case ASTG_K_FWD32:
-            // L values + columns, order + rowstart + diagonal split, r, invd, y written, y gathered once
-            bytes = (w + 4) * nl + (4.0 * w + 12) * n;
+            // stencil coefficients + indices; per row: order, start, r-term count, r, pivot,
+            // y written, y gathered once, backward vector armed
+            bytes = (w + 4) * stF_.E + (5.0 * w + 12) * n;
             break;
         case ASTG_K_BWD64: case ASTG_K_BWD32:
-            // L^T values + columns, order + rowstart + split, y, r, invd, z written, z gathered once
-            bytes = (w + 4) * nl + (5.0 * w + 12) * n;
+            // stencil coefficients + indices; per row: order, start, r-term count, y, next y
+            // armed, pivot, r, z written twice (sweep order and row order), z gathered once
+            bytes = (w + 4) * stB_.E + (7.0 * w + 12) * n;
             break;
         case ASTG_K_MIC:
             // A values + pattern read, factor written twice (both triangles), invd
@@ -1021,20 +1136,18 @@ int PcgDevice::benchKernel(int which, int reps, double& avgMs, double& bytes, st
         k_zero64<<<1, BS, 0, stream_>>>(32, scal);
         const Parts P64 = parts(bwdGrid<double>()), P32 = parts(bwdGrid<float>());
         if (which == ASTG_K_FWD64 || which == ASTG_K_BWD64) {
-            k_fill_sent<double><<<blocks(P_), BS, 0, stream_>>>(y_.as<double>(), P_);
-            k_fill_sent<double><<<blocks(P_), BS, 0, stream_>>>(zp_.as<double>(), P_);
+            k_fill_sent<double><<<blocks(2 * PS_), BS, 0, stream_>>>(y_.as<double>(), 2 * PS_);
+            k_fill_sent<double><<<blocks(PS_), BS, 0, stream_>>>(zp_.as<double>(), PS_);
         } else if (which == ASTG_K_FWD32 || which == ASTG_K_BWD32) {
-            k_fill_sent<float><<<blocks(P_), BS, 0, stream_>>>(y_.as<float>(), P_);
-            k_fill_sent<float><<<blocks(P_), BS, 0, stream_>>>(zp_.as<float>(), P_);
+            k_fill_sent<float><<<blocks(2 * PS_), BS, 0, stream_>>>(y_.as<float>(), 2 * PS_);
+            k_fill_sent<float><<<blocks(PS_), BS, 0, stream_>>>(zp_.as<float>(), PS_);
         }
         // the forward sweep starts in mode 0, which needs nonzero ||b|| partials
         if (sweep) k_fill64<<<blocks(gs_), BS, 0, stream_>>>(gs_, P64.res, 1.0);
         if (which == ASTG_K_BWD64)
-            rc = launchFwd<double>(F_.as<double>(), invd_.as<double>(), r_.as<double>(), y_.as<double>(),
-                                   0, 0, 1 << 30, ASTG_NORM_INF, 1e-30, err);
+            rc = launchFwd<double>(invd_.as<double>(), r_.as<double>(), 0, 0, 0, 1 << 30, ASTG_NORM_INF, 1e-30, err);
         else if (which == ASTG_K_BWD32)
-            rc = launchFwd<float>(F32_.as<float>(), invd32_.as<float>(), r_.as<float>(), y_.as<float>(),
-                                   0, 0, 1 << 30, ASTG_NORM_INF, 1e-30, err);
+            rc = launchFwd<float>(invd32_.as<float>(), r_.as<float>(), 0, 0, 0, 1 << 30, ASTG_NORM_INF, 1e-30, err);
         if (rc) break;
         HIPCHK(hipEventRecord(e0, stream_));
         switch (which) {
@@ -1047,20 +1160,18 @@ int PcgDevice::benchKernel(int which, int reps, double& avgMs, double& bytes, st
                                                           q_.as<float>(), P32, scal);
                 break;
             case ASTG_K_FWD64:
-                rc = launchFwd<double>(F_.as<double>(), invd_.as<double>(), r_.as<double>(), y_.as<double>(),
-                                       0, 0, 1 << 30, ASTG_NORM_INF, 1e-30, err);
+                rc = launchFwd<double>(invd_.as<double>(), r_.as<double>(), 0, 0, 0, 1 << 30, ASTG_NORM_INF, 1e-30,
+                                       err);
                 break;
             case ASTG_K_FWD32:
-                rc = launchFwd<float>(F32_.as<float>(), invd32_.as<float>(), r_.as<float>(), y_.as<float>(),
-                                       0, 0, 1 << 30, ASTG_NORM_INF, 1e-30, err);
+                rc = launchFwd<float>(invd32_.as<float>(), r_.as<float>(), 0, 0, 0, 1 << 30, ASTG_NORM_INF, 1e-30,
+                                      err);
                 break;
             case ASTG_K_BWD64:
-                rc = launchBwd<double>(F_.as<double>(), invd_.as<double>(), y_.as<double>(), r_.as<double>(),
-                                       z_.as<double>(), 0, err);
+                rc = launchBwd<double>(invd_.as<double>(), r_.as<double>(), z_.as<double>(), 0, 0, err);
                 break;
             case ASTG_K_BWD32:
-                rc = launchBwd<float>(F32_.as<float>(), invd32_.as<float>(), y_.as<float>(), r_.as<float>(),
-                                      z_.as<float>(), 0, err);
+                rc = launchBwd<float>(invd32_.as<float>(), r_.as<float>(), z_.as<float>(), 0, 0, err);
                 break;
             case ASTG_K_MIC:
                 rc = launchMic(0.97, 0.25, err);
@@ -1107,9 +1218,9 @@ int PcgDevice::sweepTimeline(int which, std::vector<long long>& t, std::vector<i
     return ASTG_OK;
 }
 
-template int PcgDevice::pcgInner<double>(const double*, const double*, const double*, const double*, double*, double,
-                                         int, int, int&, double&, bool&, std::string&);
-template int PcgDevice::pcgInner<float>(const float*, const float*, const float*, const float*, float*, double, int,
-                                        int, int&, double&, bool&, std::string&);
+template int PcgDevice::pcgInner<double>(const double*, const double*, const double*, double*, double, int, int, int&,
+                                         double&, bool&, std::string&);
+template int PcgDevice::pcgInner<float>(const float*, const float*, const float*, float*, double, int, int, int&,
+                                        double&, bool&, std::string&);
 
 }  // namespace astg
