@@ -90,7 +90,7 @@ constexpr int kGateLimit = 1 << 16;
 // A row's dependencies held in registers: column indices and factor values
 // are loaded before the row waits (they do not depend on other rows), then
 // all published values are polled together, one round trip per poll. N is
-// the register capacity (16 or 32 entries).
+// the register capacity.
 constexpr int kMaxDeps = 16;
 template <typename T, int N = kMaxDeps>
 struct DepRow {
@@ -146,63 +146,6 @@ struct DepRow {
 #pragma unroll
                     for (int j = 0; j < N; ++j)
                         if (j < nd) a -= f[j] * v[j];
-                    publish(a);
-                    pending = false;
-                } else if (spins > kSpinLimit) {
-                    publish((T)NAN);
-                    pending = false;
-                    ok = false;
-                }
-            }
-            if (!__any(pending)) break;
-            for (int k = 0; k < backoff; ++k) __builtin_amdgcn_s_sleep(1);
-            backoff = backoff < cap ? 2 * backoff : cap;
-        }
-        return ok;
-    }
-};
-
-// DepRow for rows with up to N > 16 dependencies: the coefficients wait in
-// LDS (f[j] of lane l at fl[j * STRIDE + l]) instead of registers, which
-// keeps the kernel at two waves per SIMD in FP64.
-template <typename T, int N, int STRIDE>
-struct DepRowWide {
-    int nd;
-    int c[N];
-    T* fl;
-    __device__ void preload(const int* col, const T* F, int p0, int pe) {
-        nd = pe - p0;
-#pragma unroll
-        for (int j = 0; j < N; ++j)
-            if (j < nd) {
-                c[j] = col[p0 + j];
-                fl[j * STRIDE] = F[p0 + j];
-            }
-    }
-    __device__ void poll_round(const T* vec, typename Sentinel<T>::bits_t* vb, unsigned& ready) {
-#pragma unroll
-        for (int j = 0; j < N; ++j)
-            if (j < nd && !((ready >> j) & 1u)) vb[j] = Sentinel<T>::load(&vec[c[j]]);
-#pragma unroll
-        for (int j = 0; j < N; ++j)
-            if (j < nd && vb[j] != Sentinel<T>::value) ready |= 1u << j;
-    }
-    // Same contract as DepRow::run.
-    template <class Pub>
-    __device__ bool run(const T* vec, T acc, int cap, Pub publish) {
-        typename Sentinel<T>::bits_t vb[N];
-        unsigned ready = 0;
-        const unsigned full = nd >= 32 ? ~0u : (1u << nd) - 1u;
-        int backoff = 1;
-        bool pending = true, ok = true;
-        for (int spins = 0;; ++spins) {
-            if (pending) {
-                poll_round(vec, vb, ready);
-                if (ready == full) {
-                    T a = acc;
-#pragma unroll
-                    for (int j = 0; j < N; ++j)
-                        if (j < nd) a -= fl[j * STRIDE] * Sentinel<T>::val(vb[j]);
                     publish(a);
                     pending = false;
                 } else if (spins > kSpinLimit) {

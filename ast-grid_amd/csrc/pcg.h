@@ -60,7 +60,6 @@ struct DevStencil {
     DevBuf start, nr, idx, ustart, upd, coef, coef32;
     long long E = 0, R = 0, kept = 0;
     int maxY = 0;  // largest number of y-terms of a row
-    int cap = 16;  // register capacity of the sweep kernel (16 or 32)
 };
 
 // Per-block partials of the reductions, consumed by the next kernel.
@@ -159,7 +158,6 @@ private:
     int P_ = 0;                             // padded length of the level order
     int PS_ = 0;                            // padded length of the sweep order
     int merge_ = 2;                         // levels per super-level of the sweeps
-    int mergeMaxY_ = 32;                    // most y-terms a merged row keeps
     int cus_ = 0;                           // compute units
     int gs_ = 0;                            // grid of the streaming kernels
     int gb_ = 0;                            // largest backward-sweep grid (rho partial slots)

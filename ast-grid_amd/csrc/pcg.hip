@@ -44,22 +44,6 @@ namespace astg {
 constexpr int BS = 256;
 constexpr int kLpr = 8;  // lanes per row in the SpMV kernels
 
-// Sweep rows keep up to 16 dependencies in registers; wider plans use the
-// LDS-backed row (coefficients in LDS, one column per lane).
-template <typename T, int N>
-struct SweepRow {
-    typedef DepRow<T, N> type;
-    __device__ static void bind(type&) {}
-};
-template <typename T>
-struct SweepRow<T, 32> {
-    typedef DepRowWide<T, 32, BS> type;
-    __device__ static void bind(type& d) {
-        __shared__ T fl[32 * BS];
-        d.fl = fl + threadIdx.x;
-    }
-};
-
 DevBuf::~DevBuf() {
     if (p) (void)hipFree(p);
 }
@@ -323,7 +307,7 @@ __device__ inline bool fwd_prologue(const Parts& Pt, double* scal, int mode, int
 //   y_i = d_i (r_i - sum_j R_ij r_j - sum_m W_im y_m)
 // (with no merging R is empty and W is row i of L, in the reference's order).
 // The r-terms need no other row, so they are summed before the row waits.
-template <typename T, int N>
+template <typename T>
 __global__ void __launch_bounds__(BS) k_fwd(SweepPlan S, StencilView<T> St, const T* invd, const T* r, T* y, T* zp,
                                             Parts Pt, int* cnt, double* scal, int mode, int it, int maxIt, int norm,
                                             double tolRel) {
@@ -333,8 +317,7 @@ __global__ void __launch_bounds__(BS) k_fwd(SweepPlan S, StencilView<T> St, cons
         const int t = c * BS + threadIdx.x;
         const int i = S.order[t];
         // row data that does not depend on other rows is loaded before the gate
-        typename SweepRow<T, N>::type d;
-        SweepRow<T, N>::bind(d);
+        DepRow<T> d;
         T acc = 0, inv = 0;
         if (i >= 0) {
             const int b = St.start[t], e = St.start[t + 1], br = b + St.nr[t];
@@ -366,7 +349,7 @@ __global__ void __launch_bounds__(BS) k_fwd(SweepPlan S, StencilView<T> St, cons
 // fused with the partial dot z.r (one partial per block, fixed chunk
 // assignment: deterministic). y is complete (previous kernel); the other y
 // buffer is re-armed here for the next forward sweep.
-template <typename T, int N>
+template <typename T>
 __global__ void __launch_bounds__(BS) k_bwd_dot(SweepPlan S, StencilView<T> St, const T* invd, const T* y, T* ynext,
                                                 const T* r, T* z, T* zp, Parts Pt, int* cnt, const double* scal,
                                                 int parityOut) {
@@ -375,8 +358,7 @@ __global__ void __launch_bounds__(BS) k_bwd_dot(SweepPlan S, StencilView<T> St, 
     for (int c = blockIdx.x; c < S.nchunks; c += gridDim.x) {
         const int pos = S.P - 1 - (c * BS + threadIdx.x);
         const int k = S.order[pos];
-        typename SweepRow<T, N>::type d;
-        SweepRow<T, N>::bind(d);
+        DepRow<T> d;
         T acc = 0, inv = 0, rk = 0;
         if (k >= 0) {
             const int b = St.start[pos], e = St.start[pos + 1], br = b + St.nr[pos];
@@ -615,12 +597,9 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
     if (const char* g = getenv("ASTG_GATE_DIST")) gateDist_ = std::max(1, atoi(g));
     if (const char* g = getenv("ASTG_POLL_CAP")) pollCap_ = std::max(1, atoi(g));
     if (const char* g = getenv("ASTG_GATE_CAP")) gateCap_ = std::max(1, atoi(g));
-    // sweep merging: levels per group (1 = plain level order) and the most
-    // y-terms a merged row may keep (16: registers, 32: LDS-backed rows)
+    // sweep merging: levels per group (1 = plain level order)
     merge_ = 2;
-    mergeMaxY_ = 16;
     if (const char* g = getenv("ASTG_MERGE")) merge_ = std::max(1, std::min(8, atoi(g)));
-    if (const char* g = getenv("ASTG_MERGE_MAXY")) mergeMaxY_ = atoi(g) > 16 ? 32 : 16;
     int dev = 0, cus = 0;
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -765,7 +744,7 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
         for (int q = 0; q < n; ++q) lev[sortedRow[q]] = sortedLev[q];
         for (int q = 0; q < PS_; ++q)
             if (padded[q] >= 0) pos[padded[q]] = q;
-        const int maxY = merge_ > 1 ? mergeMaxY_ : kMaxDeps;
+        const int maxY = kMaxDeps;
         DevStencil* dst[2] = {&stF_, &stB_};
         for (int dir = 0; dir < 2; ++dir) {
             SweepStencil st;
@@ -777,7 +756,6 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
             D.kept = st.kept;
             D.maxY = 0;
             for (int t = 0; t < PS_; ++t) D.maxY = std::max(D.maxY, st.start[t + 1] - st.start[t] - st.nr[t]);
-            D.cap = D.maxY <= 16 ? 16 : 32;
             HIPCHK(upload(D.start, st.start.data(), st.start.size() * 4));
             HIPCHK(upload(D.nr, st.nr.data(), st.nr.size() * 4));
             HIPCHK(upload(D.idx, st.idx.data(), st.idx.size() * 4));
@@ -792,12 +770,10 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
     }
     // persistent grids (all blocks co-resident)
     const int schunks = PS_ / BS;
-    gF64_ = stF_.cap == 16 ? coopBlocks(k_fwd<double, 16>, cus, schunks) : coopBlocks(k_fwd<double, 32>, cus, schunks);
-    gF32_ = stF_.cap == 16 ? coopBlocks(k_fwd<float, 16>, cus, schunks) : coopBlocks(k_fwd<float, 32>, cus, schunks);
-    gB64_ = stB_.cap == 16 ? coopBlocks(k_bwd_dot<double, 16>, cus, schunks)
-                           : coopBlocks(k_bwd_dot<double, 32>, cus, schunks);
-    gB32_ = stB_.cap == 16 ? coopBlocks(k_bwd_dot<float, 16>, cus, schunks)
-                           : coopBlocks(k_bwd_dot<float, 32>, cus, schunks);
+    gF64_ = coopBlocks(k_fwd<double>, cus, schunks);
+    gF32_ = coopBlocks(k_fwd<float>, cus, schunks);
+    gB64_ = coopBlocks(k_bwd_dot<double>, cus, schunks);
+    gB32_ = coopBlocks(k_bwd_dot<float>, cus, schunks);
     gb_ = std::max(std::max(gB64_, gB32_), 1);
     HIPCHK(part_.alloc((size_t)(2 * gs_ + 2 * gb_ + 8) * 8));
     HIPCHK(hipStreamSynchronize(stream_));
@@ -911,12 +887,8 @@ int PcgDevice::launchFwd(const T* invd, const T* r, int k, int mode, int it, int
     const int g = fwdGrid<T>(), gb = bwdGrid<T>();
     const StencilView<T> V = view<T>(stF_);
     T* y = ybuf<T>(k);
-    if (stF_.cap == 16)
-        HIPCHK(launchCoop(k_fwd<T, 16>, g, stream_, splan(), V, invd, r, y, zp_.as<T>(), parts(gb), cnt_.as<int>(),
-                          scal_.as<double>(), mode, it, maxIt, norm, tolRel));
-    else
-        HIPCHK(launchCoop(k_fwd<T, 32>, g, stream_, splan(), V, invd, r, y, zp_.as<T>(), parts(gb), cnt_.as<int>(),
-                          scal_.as<double>(), mode, it, maxIt, norm, tolRel));
+    HIPCHK(launchCoop(k_fwd<T>, g, stream_, splan(), V, invd, r, y, zp_.as<T>(), parts(gb), cnt_.as<int>(),
+                      scal_.as<double>(), mode, it, maxIt, norm, tolRel));
     return ASTG_OK;
 }
 
@@ -926,12 +898,8 @@ int PcgDevice::launchBwd(const T* invd, const T* r, T* z, int k, int parityOut, 
     const StencilView<T> V = view<T>(stB_);
     const T* y = ybuf<T>(k);
     T* ynext = ybuf<T>(k + 1);
-    if (stB_.cap == 16)
-        HIPCHK(launchCoop(k_bwd_dot<T, 16>, gb, stream_, splan(), V, invd, y, ynext, r, z, zp_.as<T>(), parts(gb),
-                          cnt_.as<int>(), (const double*)scal_.as<double>(), parityOut));
-    else
-        HIPCHK(launchCoop(k_bwd_dot<T, 32>, gb, stream_, splan(), V, invd, y, ynext, r, z, zp_.as<T>(), parts(gb),
-                          cnt_.as<int>(), (const double*)scal_.as<double>(), parityOut));
+    HIPCHK(launchCoop(k_bwd_dot<T>, gb, stream_, splan(), V, invd, y, ynext, r, z, zp_.as<T>(), parts(gb),
+                      cnt_.as<int>(), (const double*)scal_.as<double>(), parityOut));
     return ASTG_OK;
 }
 
