@@ -32,7 +32,7 @@ EXPORTS = [
     "astg_get_weights", "astg_set_velocity", "astg_get_velocity", "astg_project", "astg_get_system",
     "astg_get_mic_factor", "astg_fields_snapshot", "astg_kernel_time", "astg_sweep_timeline",
     "astg_set_scalar", "astg_get_scalar", "astg_post_project", "astg_scalar_update_ghost", "astg_get_dual",
-    "astg_sample_points", "astg_advect", "astg_step",
+    "astg_sample_points", "astg_adjust_e", "astg_set_tilt_adaptivity", "astg_advect", "astg_step",
 ]
 FIELD_DENSITY, FIELD_HEAT = 0, 1
 SAMPLE_VELOCITY, SAMPLE_MAC, SAMPLE_BACKTRACE, SAMPLE_DENSITY, SAMPLE_HEAT = range(5)
@@ -100,6 +100,8 @@ def lib():
         "astg_scalar_update_ghost": (I, [P, I]),
         "astg_get_dual": (I, [P, I, P, P]),
         "astg_sample_points": (I, [P, I, I, P, D, P]),
+        "astg_adjust_e": (I, [P, D, D, D, D]),
+        "astg_set_tilt_adaptivity": (I, [P, I]),
         "astg_advect": (I, [P, D]),
         "astg_step": (I, [P, D, D, ctypes.POINTER(PcgParams), ctypes.POINTER(PcgResult)]),
     }
@@ -311,6 +313,13 @@ class Context:
         out = np.empty((len(xyz), 3)) if what <= SAMPLE_BACKTRACE else np.empty(len(xyz))
         self._chk(self._L.astg_sample_points(self.h, what, len(xyz), _ptr(xyz), dt, _ptr(out)), "sample_points")
         return out
+
+    def adjust_e(self, d_threshold, dt, v_threshold, max_threshold):
+        """adjustE: tilt open/close decision and commit; fields are kept."""
+        self._chk(self._L.astg_adjust_e(self.h, d_threshold, dt, v_threshold, max_threshold), "adjust_e")
+
+    def set_tilt_adaptivity(self, on):
+        self._chk(self._L.astg_set_tilt_adaptivity(self.h, 1 if on else 0), "set_tilt_adaptivity")
 
     def advect(self, dt):
         self._chk(self._L.astg_advect(self.h, dt), "advect")

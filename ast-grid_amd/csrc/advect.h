@@ -17,6 +17,7 @@
 //   BackTrace                          include/advection.h:15-64 (midpoint branch)
 //   advect                             src/octree_omnigrid_solver3.cpp:850-914,
 //                                      include/advection.h:195-239
+//   adjustE (decision)                 src/octree_omnigrid_solver3.cpp:1125-1149
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -50,7 +51,9 @@ struct AdvVel {
 class AdvectDevice {
 public:
     explicit AdvectDevice(hipStream_t s) : stream_(s) {}
-    int upload(const Octree& o, const ProjectTables& t, std::string& err);
+    // keepFields: the node sets are unchanged (a tilt update); the fields stay
+    // and keep being sampled with the previous tilt state until advect.
+    int upload(const Octree& o, const ProjectTables& t, bool keepFields, std::string& err);
 
     // Scalar fields: 0 density, 1 heat; per level, octagon and tilt value per node.
     int setScalar(int field, int level, const double* oct, const double* tilt, std::string& err);
@@ -72,11 +75,17 @@ public:
     int samplePoints(ProjectDevice& pd, int what, int n, const double* xyz, double dt, double* out,
                      std::string& err);
     double advectBytes() const;
+    // adjustE's decision (octree_omnigrid_solver3.cpp:1131-1149): the new
+    // tilt_layout open flag of every leaf, levels ascending, leaf order.
+    int decideTilts(ProjectDevice& pd, double dThr, double dt, double vThr, double maxThr, std::vector<uint8_t>& open,
+                    std::string& err);
+    int leaves(int level) const { return nleaf_[level]; }
 
 private:
     int omniGhost(double* cell, double* vert, int nc, std::string& err);
     int checkBad(std::string& err);
-    AdvTopo topo() const;
+    AdvTopo topo() const;      // current tilt state
+    AdvTopo sampTopo() const;  // tilt state the fields were written with
     AdvVel vel(ProjectDevice& pd) const;
     hipStream_t stream_;
     int L_ = 0, R_ = 0, G_ = 0;
@@ -93,6 +102,8 @@ private:
     DevBuf dualOct_, dualTilt_;      // 3 per node
     DevBuf tMac_, tTilt_;            // advected velocity (swapped with the live one)
     DevBuf pts_, out_;               // point queries
+    DevBuf sampTe_, sampTb_, sampThd_, open_;  // tilt state before an update, decisions
+    bool sampValid_ = false;
 };
 
 }  // namespace astg

@@ -829,7 +829,10 @@ __device__ inline void tiltDir(int k, double* d) {
     d[2] = s;
 }
 
-__global__ void k_adv_tilt(Topo T, const uint8_t* level, Vel V, const int* list, int n, double dt, double* tilt1) {
+// T samples the source fields; teDst is the tilt size of the destination grid
+// (they differ in the step in which adjustE changed the tilts).
+__global__ void k_adv_tilt(Topo T, const double* teDst, const uint8_t* level, Vel V, const int* list, int n, double dt,
+                           double* tilt1) {
     const int t = blockIdx.x * BS + threadIdx.x;
     if (t >= 8 * n) return;
     const int g = list[t >> 3], k = t & 7;
@@ -839,15 +842,15 @@ __global__ void k_adv_tilt(Topo T, const uint8_t* level, Vel V, const int* list,
     nodeXYZ(T, g, x, y, z);
     const double c[3] = {x * h, y * h, z * h};
     double p[3], q[3], v[3], d[3];
-    tiltPos(c, k, T.te[g], p);
+    tiltPos(c, k, teDst[g], p);
     backTrace(T, V, p, dt, q);
     velSample(T, V, q[0], q[1], q[2], v);
     tiltDir(k, d);
     tilt1[(size_t)k * c_G + g] = v[0] * d[0] + v[1] * d[1] + v[2] * d[2];
 }
 
-__global__ void k_adv_half(Topo T, const uint8_t* level, Vel V, const int* halfList, int n, double dt,
-                           double* tilt1) {
+__global__ void k_adv_half(Topo T, const double* teDst, const uint8_t* level, Vel V, const int* halfList, int n,
+                           double dt, double* tilt1) {
     const int t = blockIdx.x * BS + threadIdx.x;
     if (t >= n) return;
     const int g = halfList[t];
@@ -867,7 +870,7 @@ __global__ void k_adv_half(Topo T, const uint8_t* level, Vel V, const int* halfL
             const int ax = (k == 0 || k == 1) ? 0 : (k == 2 || k == 4) ? 1 : 2;
             d[0] = ax == 0, d[1] = ax == 1, d[2] = ax == 2;
         } else {
-            tiltPos(c, k, T.te[g], p);
+            tiltPos(c, k, teDst[g], p);
             tiltDir(k, d);
         }
         backTrace(T, V, p, dt, q);
@@ -875,6 +878,39 @@ __global__ void k_adv_half(Topo T, const uint8_t* level, Vel V, const int* halfL
         nv[k] = v[0] * d[0] + v[1] * d[1] + v[2] * d[2];
     }
     for (int k = 0; k < 8; ++k) tilt1[(size_t)k * c_G + g] = nv[k];
+}
+
+// adjustE's decision (octree_omnigrid_solver3.cpp:1131-1149) for every leaf
+// tilt: an open one closes when its density leaves (dThr, maxThr); a closed
+// one opens when the velocity at the tilt exceeds vThr and the density
+// sampled at the point it came from lies inside (dThr, maxThr). Fixed tilts
+// keep their state.
+__global__ void k_adjust_e(Topo T, const uint8_t* level, Vel V, const double* dCell, const double* dVert,
+                           const int* leaf, int n, double dThr, double dt, double vThr, double maxThr,
+                           uint8_t* open) {
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= n) return;
+    const int g = leaf[t];
+    const uint8_t b = T.tb[g];
+    uint8_t o = (b & TB_LAYOUT_OPEN) ? 1 : 0;
+    if (!(b & TB_LAYOUT_FIXED)) {
+        if (o) {
+            const double d = dVert[g];
+            if (d < dThr || d > maxThr) o = 0;
+        } else {
+            const double h = c_hl[level[g]];
+            int x, y, z;
+            nodeXYZ(T, g, x, y, z);
+            const double p[3] = {x * h, y * h, z * h};
+            double v[3];
+            velSample(T, V, p[0], p[1], p[2], v);
+            if (sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) > vThr) {
+                const double d = scalarSample(T, dCell, dVert, p[0] - v[0] * dt, p[1] - v[1] * dt, p[2] - v[2] * dt);
+                if (d > dThr && d < maxThr) o = 1;
+            }
+        }
+    }
+    open[t] = o;
 }
 
 __global__ void k_scale2(long n, double s, double* a, double* b) {
@@ -948,7 +984,24 @@ void dualMatrices(double* out) {
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-int AdvectDevice::upload(const Octree& o, const ProjectTables& t, std::string& err) {
+int AdvectDevice::upload(const Octree& o, const ProjectTables& t, bool keepFields, std::string& err) {
+    if (keepFields && (t.G != G_ || o.L != L_)) {
+        err = "tilt update changed the node sets";
+        return ASTG_ERR_STATE;
+    }
+    if (keepFields && !sampValid_) {
+        // the fields keep the tilt state they were written with until the next
+        // advect has sampled them
+        HIPCHK(sampTe_.alloc((size_t)G_ * 8));
+        HIPCHK(sampTb_.alloc((size_t)G_));
+        HIPCHK(sampThd_.alloc((size_t)G_));
+        HIPCHK(hipMemcpyAsync(sampTe_.p, te_.p, (size_t)G_ * 8, hipMemcpyDeviceToDevice, stream_));
+        HIPCHK(hipMemcpyAsync(sampTb_.p, tb_.p, (size_t)G_, hipMemcpyDeviceToDevice, stream_));
+        HIPCHK(hipMemcpyAsync(sampThd_.p, thd_.p, (size_t)G_, hipMemcpyDeviceToDevice, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+        sampValid_ = true;
+    }
+    if (!keepFields) sampValid_ = false;
     if (o.L > kMaxLevels) {
         err = "advection supports at most 8 levels";
         return ASTG_ERR_ARG;
@@ -1076,6 +1129,10 @@ int AdvectDevice::upload(const Octree& o, const ProjectTables& t, std::string& e
     HIPCHK(up(hmS_, hmS.data(), hmS.size() * 4));
     HIPCHK(level_.alloc((size_t)G_));
     HIPCHK(hipMemcpyAsync(level_.p, t.level.data(), (size_t)G_, hipMemcpyHostToDevice, stream_));
+    if (keepFields) {
+        HIPCHK(hipStreamSynchronize(stream_));
+        return ASTG_OK;
+    }
     // fields: both buffers of each scalar start at zero, as the reference's
     // temporaries do
     for (int f = 0; f < 2; ++f)
@@ -1128,6 +1185,32 @@ AdvTopo AdvectDevice::topo() const {
     T.tb = static_cast<const uint8_t*>(tb_.p);
     T.thd = static_cast<const int8_t*>(thd_.p);
     return T;
+}
+
+AdvTopo AdvectDevice::sampTopo() const {
+    AdvTopo T = topo();
+    if (sampValid_) {
+        T.te = sampTe_.as<double>();
+        T.tb = static_cast<const uint8_t*>(sampTb_.p);
+        T.thd = static_cast<const int8_t*>(sampThd_.p);
+    }
+    return T;
+}
+
+int AdvectDevice::decideTilts(ProjectDevice& pd, double dThr, double dt, double vThr, double maxThr,
+                              std::vector<uint8_t>& open, std::string& err) {
+    open.assign(nLeaf_, 0);
+    if (!nLeaf_) return ASTG_OK;
+    HIPCHK(open_.alloc((size_t)nLeaf_));
+    const int d = cur_[0];
+    k_adjust_e<<<nblocks(nLeaf_), BS, 0, stream_>>>(sampTopo(), static_cast<const uint8_t*>(level_.p), vel(pd),
+                                                    scal_[0][d][0].as<double>(), scal_[0][d][1].as<double>(),
+                                                    leaf_.as<int>(), nLeaf_, dThr, dt, vThr, maxThr,
+                                                    open_.as<uint8_t>());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(open.data(), open_.p, (size_t)nLeaf_, hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    return ASTG_OK;
 }
 
 int AdvectDevice::setScalar(int field, int level, const double* oct, const double* tilt, std::string& err) {
@@ -1277,7 +1360,7 @@ AdvVel AdvectDevice::vel(ProjectDevice& pd) const {
 }
 
 int AdvectDevice::advect(ProjectDevice& pd, double dt, std::string& err) {
-    const Topo T = topo();
+    const Topo T = sampTopo();  // sources: the tilt state they were written with
     const uint8_t* lev = static_cast<const uint8_t*>(level_.p);
     const Vel V = vel(pd);
     const int d = cur_[0], h = cur_[1];
@@ -1309,10 +1392,11 @@ int AdvectDevice::advect(ProjectDevice& pd, double dt, std::string& err) {
     if (nLeaf_)
         k_adv_mac<<<nblocks(3L * nLeaf_), BS, 0, stream_>>>(T, lev, V, leaf_.as<int>(), nLeaf_, dt, tMac_.as<double>());
     if (nVelTilt_)
-        k_adv_tilt<<<nblocks(8L * nVelTilt_), BS, 0, stream_>>>(T, lev, V, velTilt_.as<int>(), nVelTilt_, dt,
-                                                                 tTilt_.as<double>());
+        k_adv_tilt<<<nblocks(8L * nVelTilt_), BS, 0, stream_>>>(T, te_.as<double>(), lev, V, velTilt_.as<int>(),
+                                                                 nVelTilt_, dt, tTilt_.as<double>());
     if (nHalf_)
-        k_adv_half<<<nblocks(nHalf_), BS, 0, stream_>>>(T, lev, V, half_.as<int>(), nHalf_, dt, tTilt_.as<double>());
+        k_adv_half<<<nblocks(nHalf_), BS, 0, stream_>>>(T, te_.as<double>(), lev, V, half_.as<int>(), nHalf_, dt,
+                                                         tTilt_.as<double>());
     HIPCHK(hipGetLastError());
     pd.swapVelocity(tMac_, tTilt_);
     // decay (density->scale, heat->scale)
@@ -1323,6 +1407,7 @@ int AdvectDevice::advect(ProjectDevice& pd, double dt, std::string& err) {
                                              scal_[1][cur_[1]][1].as<double>());
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(stream_));
+    sampValid_ = false;  // the fields now carry the current tilt state
     return ASTG_OK;
 }
 

@@ -4,6 +4,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -35,6 +36,7 @@ struct astg_ctx {
     std::vector<int> csrRs, csrCol;
     astg::PcgDevice* lastPcg = nullptr;
     hipEvent_t ev[6] = {};
+    bool tiltAdaptivity = false;                          // enableTiltAdaptivity
 };
 
 namespace {
@@ -88,7 +90,7 @@ int ensureTables(astg_ctx* c) {
         rc = c->proj->upload(c->tables, *c->tree, c->err);
         if (rc) return rc;
         if (!c->adv) c->adv.reset(new astg::AdvectDevice(c->stream));
-        rc = c->adv->upload(*c->tree, c->tables, c->err);
+        rc = c->adv->upload(*c->tree, c->tables, false, c->err);
         if (rc) return rc;
         rc = c->pcg->setPattern(c->tables.n, c->tables.rowstart.data(), c->tables.col.data(), c->err);
         if (rc) return rc;
@@ -102,6 +104,28 @@ int ensureTables(astg_ctx* c) {
         c->deviceReady = true;
     }
     return ASTG_OK;
+}
+
+// Tilt update with the fields kept: numbering, weights, tables, device
+// topology and the projection pattern follow the new tilt state; the node
+// sets, and so every field array, are unchanged.
+int retopologize(astg_ctx* c) {
+    astg::numberUnknowns(*c->tree, c->num);
+    astg::computeWeights(*c->tree, c->w);
+    astg::buildProjectTables(*c->tree, c->num, c->w, c->tables);
+    int rc = c->proj->upload(c->tables, *c->tree, c->err);
+    if (rc) return rc;
+    rc = c->adv->upload(*c->tree, c->tables, true, c->err);
+    if (rc) return rc;
+    return c->pcg->setPattern(c->tables.n, c->tables.rowstart.data(), c->tables.col.data(), c->err);
+}
+
+int adjustTilts(astg_ctx* c, double dThr, double dt, double vThr, double maxThr) {
+    std::vector<uint8_t> open;
+    int rc = c->adv->decideTilts(*c->proj, dThr, dt, vThr, maxThr, open, c->err);
+    if (rc) return rc;
+    if (!c->tree->setTiltOpenAll(open.data())) return ASTG_OK;
+    return retopologize(c);
 }
 
 int checkLevel(astg_ctx* c, int level) {
@@ -625,10 +649,31 @@ int astg_sample_points(astg_ctx* c, int what, int n, const double* xyz, double d
     return c->adv->samplePoints(*c->proj, what, n, xyz, dt, out, c->err);
 }
 
+int astg_adjust_e(astg_ctx* c, double d_threshold, double dt, double v_threshold, double max_threshold) {
+    if (!(dt >= 0)) return fail(c, ASTG_ERR_ARG, "dt must be non-negative");
+    int rc = needAdvect(c);
+    if (rc) return rc;
+    return guarded(c, [&] { return adjustTilts(c, d_threshold, dt, v_threshold, max_threshold); });
+}
+
+int astg_set_tilt_adaptivity(astg_ctx* c, int on) {
+    if (!c) return ASTG_ERR_ARG;
+    c->tiltAdaptivity = on != 0;
+    return ASTG_OK;
+}
+
 int astg_advect(astg_ctx* c, double dt) {
     if (!(dt > 0)) return fail(c, ASTG_ERR_ARG, "dt must be positive");
     int rc = needAdvect(c);
     if (rc) return rc;
+    // advect(dt) opens with adjustE(0.02, dt, EPSILON, 1000) when tilt
+    // adaptivity is on (octree_omnigrid_solver3.cpp:852-853)
+    if (c->tiltAdaptivity) {
+        rc = guarded(c, [&] {
+            return adjustTilts(c, 0.02, dt, std::numeric_limits<double>::epsilon(), 1000.0);
+        });
+        if (rc) return rc;
+    }
     return c->adv->advect(*c->proj, dt, c->err);
 }
 
@@ -647,7 +692,7 @@ int astg_step(astg_ctx* c, double dt, double rho, const astg_pcg_params* params,
     if (rc) return rc;
     rc = c->adv->postProject(*c->proj, c->err);
     if (rc) return rc;
-    return c->adv->advect(*c->proj, dt, c->err);
+    return astg_advect(c, dt);
 }
 
 }  // extern "C"

@@ -312,6 +312,29 @@ void Octree::setTiltOpen(int l, const uint8_t* openMask) {
     updateTiltGhost(tilt, false);
 }
 
+// adjustE's commit (octree_omnigrid_solver3.cpp:1150-1165) after its decision
+// loop (1131-1149) has set the layout flags of every level.
+bool Octree::setTiltOpenAll(const uint8_t* openMasks) {
+    bool changed = false;
+    for (int l = 0, base = 0; l < L; base += lv[l].nleaf, ++l) {
+        const Level& Lv = lv[l];
+        const double maxE = Lv.h * 0.5;
+        for (int i = 0; i < Lv.nleaf; ++i) {
+            TiltNode& n = tilt[l][i];
+            if (n.bits & TB_LAYOUT_FIXED) continue;
+            const bool open = openMasks[base + i] != 0;
+            if (open != ((n.bits & TB_LAYOUT_OPEN) != 0)) changed = true;
+            if (open) n.bits |= TB_LAYOUT_OPEN;
+            else n.bits &= ~TB_LAYOUT_OPEN;
+            if (open) setE(n, maxE, maxE);
+            else setE(n, 0);
+        }
+    }
+    updateTiltGhost(tilt, true);
+    updateTiltGhost(tilt, false);
+    return changed;
+}
+
 // updateIndexParallel (octree_omnigrid_solver3.cpp:292-347). The per-core
 // counts plus prefix sum give the sequential order of the leaf iterator
 // (levels ascending, leaf order), octagon before tilt at each node; then the

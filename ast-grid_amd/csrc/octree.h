@@ -80,6 +80,10 @@ public:
     // Opens (e = h/2, largest) or closes non-fixed leaf tilts of one level
     // following adjustE's commit step (octree_omnigrid_solver3.cpp:1150-1165).
     void setTiltOpen(int level, const uint8_t* openMask);
+    // The same for every level at once (masks: leaves of each level, levels
+    // ascending), ghosts updated once at the end; returns whether any leaf
+    // tilt changed.
+    bool setTiltOpenAll(const uint8_t* openMasks);
 
     int node(int l, int x, int y, int z) const { return (l < 0 || l >= L) ? -1 : lv[l].get(x, y, z); }
     lcoor_t getUntil(int l, int x, int y, int z, bool toFinest) const;

@@ -178,7 +178,9 @@ int astg_sweep_timeline(astg_ctx* ctx, int which, long long* t_out, int* level_o
 /* Scalar fields carried by the context: ASTG_FIELD_DENSITY, ASTG_FIELD_HEAT;
  * per level, one octagon (cell) and one tilt (vertex) value per node, in node
  * order, as OctreeOmniGrid3<REAL> stores them (include/octree_omni_grid3.h).
- * Scalars and velocity start at zero and are reset when the topology changes. */
+ * Scalars and velocity start at zero and are reset when the topology is set
+ * again (astg_set_topology_split, astg_set_tilt_open); tilt adaptivity
+ * (astg_adjust_e) keeps them. */
 #define ASTG_FIELD_DENSITY 0
 #define ASTG_FIELD_HEAT 1
 int astg_set_scalar(astg_ctx* ctx, int field, int level, const double* oct, const double* tilt);
@@ -208,9 +210,24 @@ int astg_get_dual(astg_ctx* ctx, int level, double* oct3, double* tilt3);
 #define ASTG_SAMPLE_HEAT 4
 int astg_sample_points(astg_ctx* ctx, int what, int n, const double* xyz, double dt, double* out);
 
-/* OctreeOmniGridSolver3::advect with tilt adaptivity off
- * (src/octree_omnigrid_solver3.cpp:850-914): density and heat on octagon, open
- * tilt and half-tilt DOFs, their ghosts, the velocity on MAC, tilt and
+/* OctreeOmniGridSolver3::adjustE (src/octree_omnigrid_solver3.cpp:1125-1166):
+ * every non-fixed leaf tilt closes when open and its density leaves
+ * (d_threshold, max_threshold), or opens when closed, the velocity at the tilt
+ * exceeds v_threshold and the density sampled at pos - v*dt lies inside the
+ * range; then the new tilt sizes (h/2 or 0) and ghost states are committed.
+ * The numbering, weights and projection pattern follow the new state; fields
+ * are kept and sampled with the previous tilt state until the next
+ * astg_advect, as the reference's tiltE/tTiltE pair does. */
+int astg_adjust_e(astg_ctx* ctx, double d_threshold, double dt, double v_threshold, double max_threshold);
+
+/* enableTiltAdaptivity (include/octree_omnigrid_solver3.h:56), off by
+ * default: when on, astg_advect and astg_step start the advection with
+ * adjustE(0.02, dt, DBL_EPSILON, 1000) (octree_omnigrid_solver3.cpp:852-853). */
+int astg_set_tilt_adaptivity(astg_ctx* ctx, int on);
+
+/* OctreeOmniGridSolver3::advect (src/octree_omnigrid_solver3.cpp:850-914):
+ * with tilt adaptivity on, adjustE first; then density and heat on octagon,
+ * open tilt and half-tilt DOFs, their ghosts, the velocity on MAC, tilt and
  * half-tilt slots (SemiLagrangianOctreeOmniFace3, advection.h:195-239), then
  * the 0.999 decay of both scalars. Needs astg_post_project first. */
 int astg_advect(astg_ctx* ctx, double dt);

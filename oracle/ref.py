@@ -65,6 +65,9 @@ def lib():
         "ref_sample_points": (I, [P, I, I, P, D, P]),
         "ref_sample_scalar": (I, [P, I, I, P, P]),
         "ref_advect": (I, [P, D]),
+        "ref_adjust_e": (I, [P, D, D, D, D]),
+        "ref_set_tilt_adaptivity": (None, [P, I]),
+        "ref_tilt_nodes_of": (None, [P, I, I, P, P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -111,12 +114,13 @@ class RefSolver:
         self.L.ref_nodes(self.h, l, _p(xyz), _p(fl))
         return xyz, fl
 
-    def tilt(self, l):
+    def tilt(self, l, which=0):
+        """Tilt state per layer node; which=1 reads the state adjustE writes."""
         n = self.level_size(l)
         e = np.empty(n, np.float64)
         bits = np.empty(n, np.int32)
         hd = np.empty(n, np.int32)
-        self.L.ref_tilt_nodes(self.h, l, _p(e), _p(bits), _p(hd))
+        self.L.ref_tilt_nodes_of(self.h, which, l, _p(e), _p(bits), _p(hd))
         return e, bits, hd
 
     def half_tilts(self):
@@ -202,6 +206,12 @@ class RefSolver:
 
     def advect(self, dt):
         self._chk(self.L.ref_advect(self.h, dt))
+
+    def adjust_e(self, d_threshold, dt, v_threshold, max_threshold):
+        self._chk(self.L.ref_adjust_e(self.h, d_threshold, dt, v_threshold, max_threshold))
+
+    def set_tilt_adaptivity(self, on):
+        self.L.ref_set_tilt_adaptivity(self.h, 1 if on else 0)
 
     def project(self, dt, tol=1e-6, maxit=3000):
         if self.L.ref_project(self.h, dt, tol, maxit):

@@ -90,6 +90,8 @@ public:
     void doUpdateVelocity(const VectorXd& p, double dt) { updateVelocityByPressure(p, dt); }
     int doNumber() { return updateIndexParallel(); }
     void doAdvect(double dt) { advect(dt); }
+    void doAdjustE(double dThr, double dt, double vThr, double maxThr) { adjustE(dThr, dt, vThr, maxThr); }
+    void setTiltAdaptivity(bool on) { enableTiltAdaptivity = on; }
     void doStep(double dt) { onUpdate(dt); }
 
 private:
@@ -169,10 +171,12 @@ void ref_nodes(void* p, int level, int* xyz, int* flags) {
 // Tilt state per layer node. bits: 0 is_largest, 1 is_closed, 2 is_fixed,
 // 3 is_T_joint, 4 is_closed_by_T_joint, 5 is_half_tilt, 6 layout.is_open,
 // 7 layout.is_fixed.
-void ref_tilt_nodes(void* p, int level, double* e, int* bits, int* halfDir) {
+// which = 0: tiltE (the state the solver's fields use), 1: tTiltE (the state
+// adjustE writes, swapped in at the end of advect).
+void ref_tilt_nodes_of(void* p, int which, int level, double* e, int* bits, int* halfDir) {
     auto* s = H(p);
     auto& layout = s->lay();
-    auto& data = s->tilt()->getData()[level];
+    auto& data = (which ? s->ttilt() : s->tilt())->getData()[level];
     int n = (int)data.size();
     for (int i = 0; i < n; ++i) {
         const TiltENode& t = data[i];
@@ -184,6 +188,10 @@ void ref_tilt_nodes(void* p, int level, double* e, int* bits, int* halfDir) {
                   (t.is_half_tilt ? 32 : 0) | (ts.is_open ? 64 : 0) | (ts.is_fixed ? 128 : 0);
         halfDir[i] = t.half_direction;
     }
+}
+
+void ref_tilt_nodes(void* p, int level, double* e, int* bits, int* halfDir) {
+    ref_tilt_nodes_of(p, 0, level, e, bits, halfDir);
 }
 
 // Returns the number of half-tilts; out (may be null) receives (level,x,y,z) quads.
@@ -318,6 +326,15 @@ int ref_sample_scalar(void* p, int field, int n, const double* xyz, double* out)
         for (int i = 0; i < n; ++i) out[i] = s->scalar(field)->sample(Vector3f(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]));
     });
 }
+
+// adjustE (octree_omnigrid_solver3.cpp:1125-1166) on the current fields.
+int ref_adjust_e(void* p, double dThr, double dt, double vThr, double maxThr) {
+    return guarded([&] { H(p)->doAdjustE(dThr, dt, vThr, maxThr); });
+}
+
+// enableTiltAdaptivity (octree_omnigrid_solver3.h:56): advect(dt) then starts
+// with adjustE(0.02, dt, EPSILON, 1000) (octree_omnigrid_solver3.cpp:852-853).
+void ref_set_tilt_adaptivity(void* p, int on) { H(p)->setTiltAdaptivity(on != 0); }
 
 int ref_advect(void* p, double dt) {
     return guarded([&] { H(p)->doAdvect(dt); });

@@ -10,9 +10,10 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "ast-grid_amd"))
 
-# projection fixtures (the adv_*.npz advection fixtures build on them)
+# projection fixtures (the adv_*.npz advection and adj_*.npz tilt-adaptivity
+# fixtures build on them)
 GOLDEN = sorted(p for p in glob.glob(os.path.join(ROOT, "tests", "golden", "*.npz"))
-                if not os.path.basename(p).startswith("adv_"))
+                if not os.path.basename(p).startswith(("adv_", "adj_")))
 
 
 def pytest_configure(config):

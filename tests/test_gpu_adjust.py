@@ -1,0 +1,101 @@
+"""Device tilt adaptivity (adjustE, octree_omnigrid_solver3.cpp:1125-1166)
+against the reference (tests/golden/adj_*.npz, written by
+tests/golden/make_golden_adjust.py from the reference build).
+
+Through the C ABI on device 0, from the inputs of the advection fixtures:
+  * astg_adjust_e after the post-projection chain: the committed tilt state
+    (tilt_layout open/fixed, TiltENode flags and e) per node, bit-exact;
+  * one advect(dt) with tilt adaptivity on: density, heat and velocity per
+    node (1e-12 of the largest magnitude; the reference is built with
+    -ffast-math) and the tilt state, bit-exact;
+  * the next projection on the updated tilts: velocity per node to 1e-6
+    relative (closed boxes modulo the pressure constant, as for the
+    projection fixtures).
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, load_golden, make_context
+
+pytestmark = pytest.mark.gpu
+
+ADJ = sorted(glob.glob(os.path.join(ROOT, "tests", "golden", "adj_*.npz")))
+
+
+@pytest.fixture(params=ADJ, ids=[os.path.basename(p)[4:-4] for p in ADJ])
+def adj(request):
+    j = load_golden(request.param)
+    g = load_golden(os.path.join(ROOT, "tests", "golden", str(j["scene"]) + ".npz"))
+    a = load_golden(os.path.join(ROOT, "tests", "golden", "adv_" + str(j["scene"]) + ".npz"))
+    return g, a, j
+
+
+def relmax(x, y):
+    x, y = np.asarray(x), np.asarray(y)
+    if x.size == 0:
+        return 0.0
+    return np.abs(x - y).max() / max(np.abs(y).max(), 1e-300)
+
+
+def rel2(x, y):
+    return np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-300)
+
+
+def prepared(g, a):
+    c = make_context(g, 0)
+    L = int(g["levels"])
+    for l in range(L):
+        c.set_velocity(l, g["u_mac_out_%d" % l], g["u_tilt_out_%d" % l])
+    for f in (0, 1):
+        for l in range(L):
+            c.set_scalar(f, l, a["s%d_in_oct_%d" % (f, l)], a["s%d_in_tilt_%d" % (f, l)])
+    c.post_project()
+    return c, L
+
+
+def test_adjust_e_matches_reference(adj):
+    g, a, j = adj
+    c, L = prepared(g, a)
+    c.adjust_e(float(j["d_threshold"]), float(j["dt"]), float(j["v_threshold"]), float(j["max_threshold"]))
+    for l in range(L):
+        e, bits, _ = c.tilt(l)
+        assert np.array_equal(bits, j["adj_bits_%d" % l]), l
+        assert np.array_equal(e, j["adj_e_%d" % l]), l
+    c.close()
+
+
+def test_adaptive_advect_matches_reference(adj):
+    g, a, j = adj
+    c, L = prepared(g, a)
+    c.set_tilt_adaptivity(True)
+    c.advect(float(j["dt"]))
+    for l in range(L):
+        e, bits, _ = c.tilt(l)
+        assert np.array_equal(bits, j["step_bits_%d" % l]), l
+        for f in (0, 1):
+            o, t = c.scalar(f, l)
+            assert relmax(o, j["s%d_step_oct_%d" % (f, l)]) < 1e-12, (f, l)
+            assert relmax(t, j["s%d_step_tilt_%d" % (f, l)]) < 1e-12, (f, l)
+        m, t = c.velocity(l)
+        assert relmax(m, j["vel_step_mac_%d" % l]) < 1e-12, l
+        assert relmax(t, j["vel_step_tilt_%d" % l]) < 1e-12, l
+    c.close()
+
+
+def test_projection_after_tilt_update_matches_reference(adj):
+    import astg
+    g, a, j = adj
+    c, L = prepared(g, a)
+    c.set_tilt_adaptivity(True)
+    c.advect(float(j["dt"]))
+    c.post_project()
+    res = c.project(float(g["dt"]), float(g["rho"]), astg.default_params(tol=float(g["tol"]), max_iterations=3000))
+    assert res.converged == 1
+    for l in range(L):
+        m, t = c.velocity(l)
+        assert rel2(m, j["vel_proj_mac_%d" % l]) < 1e-6, l
+        assert rel2(t, j["vel_proj_tilt_%d" % l]) < 1e-6, l
+    c.close()
