@@ -111,9 +111,12 @@ public:
     const std::vector<int>& hostCol() const { return hcol_; }
 
 private:
+    // x: FP64 solution (the FP32 iteration accumulates its steps in FP64).
+    // b64, replaceEvery: FP32 iteration only; every replaceEvery iterations
+    // the recurrence residual becomes b64 - A x in FP64 (0: never).
     template <typename T>
-    int pcgInner(const T* A, const T* invd, const T* b, T* x, double tolRel, int maxIt, int norm, int& iters,
-                 double& resOut, bool& conv, std::string& err);
+    int pcgInner(const T* A, const T* invd, const T* b, double* x, double tolRel, int maxIt, int norm, int& iters,
+                 double& resOut, bool& conv, std::string& err, const double* b64, int replaceEvery);
     // Sweep k of a solve (0: the first preconditioning, it + 1: iteration it)
     // writes y buffer k & 1; the backward sweep re-arms buffer (k + 1) & 1.
     template <typename T>
@@ -152,7 +155,9 @@ private:
     DevBuf b_, x_;                          // FP64 rhs / solution
     DevBuf r_, z_, s_, q_;                  // work vectors (sized for FP64; reused as FP32)
     DevBuf y_, zp_;                         // sweep vectors in sweep order (2 x PS and PS)
-    DevBuf r64_, d64_;                      // mixed form outer vectors
+    DevBuf r64_, d64_, rhs64_;              // mixed form outer vectors
+    int mixedReplace_ = 100;                // mixed form: residual replacement period
+    double mixedEta_ = 0.9;                 // mixed form: inner tolerance / outer target
     DevBuf scal_, cnt_, part_;              // device scalars, counters, partials
     double* hscal_ = nullptr;               // pinned mirror of the scalars
     int P_ = 0;                             // padded length of the level order

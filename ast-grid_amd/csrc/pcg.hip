@@ -261,12 +261,13 @@ __global__ void __launch_bounds__(BS) k_mic_level(int t0, int t1, const int* ord
 // ---------------------------------------------------------------------------
 __device__ inline const double* rho_buf(const Parts& P, int parity) { return parity ? P.rho1 : P.rho0; }
 
-template <typename T>
-__global__ void __launch_bounds__(BS) k_init(int n, const T* b, T* x, T* r, Parts Pt, int norm) {
+// X: type of the solution vector (FP64 also when the iteration runs in FP32).
+template <typename T, typename X>
+__global__ void __launch_bounds__(BS) k_init(int n, const T* b, X* x, T* r, Parts Pt, int norm) {
     double v = 0.0;
     for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
         const T bi = b[i];
-        x[i] = (T)0;
+        x[i] = (X)0;
         r[i] = bi;
         const double a = (double)bi;
         v = norm == ASTG_NORM_INF ? fmax(v, fabs(a)) : v + a * a;
@@ -477,16 +478,20 @@ __global__ void __launch_bounds__(BS) k_spmv_dot(int n, const int* rs, const int
     if (threadIdx.x == 0) Pt.sq[blockIdx.x] = t;
 }
 
-template <typename T>
-__global__ void __launch_bounds__(BS) k_axpy_res(int n, const T* s, const T* q, T* x, T* r, Parts Pt,
+// x accumulates in X (FP64 in the mixed form: the FP32 iteration's steps are
+// summed without FP32 rounding).
+template <typename T, typename X>
+__global__ void __launch_bounds__(BS) k_axpy_res(int n, const T* s, const T* q, X* x, T* r, Parts Pt,
                                                  const double* scal, int norm, int it) {
     if (scal[S_DONE] != 0.0) return;
     const double sq = reduce_prev<false>(Pt.sq, Pt.gs);
     const double rho = reduce_prev<false>(rho_buf(Pt, it & 1), Pt.gb);
-    const T alpha = (T)(rho / sq);
+    const double ad = rho / sq;
+    const T alpha = (T)ad;
+    const X ax = sizeof(X) > sizeof(T) ? (X)ad : (X)alpha;
     double v = 0.0;
     for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
-        x[i] += alpha * s[i];
+        x[i] += ax * (X)s[i];
         const T ri = r[i] - alpha * q[i];
         r[i] = ri;
         const double a = (double)ri;
@@ -519,11 +524,39 @@ __global__ void k_res_final(Parts Pt, double* scal, int norm) {
     const double t = norm == ASTG_NORM_INF ? reduce_prev<true>(Pt.res, Pt.gs) : reduce_prev<false>(Pt.res, Pt.gs);
     if (threadIdx.x == 0) scal[S_OUTER_RES] = norm == ASTG_NORM_INF ? t : sqrt(t);
 }
-__global__ void k_scale_f32(int n, const double* r, float* o, double inv) {
-    for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) o[i] = (float)(r[i] * inv);
+// Mixed form, inner right-hand side: r64 / |r64| in FP64 and FP32.
+__global__ void k_scale_rhs(int n, const double* r, double inv, double* o64, float* o32) {
+    for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
+        const double v = r[i] * inv;
+        o64[i] = v;
+        o32[i] = (float)v;
+    }
 }
-__global__ void k_add_corr(int n, double* x, const float* d, double scale) {
-    for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) x[i] += scale * (double)d[i];
+__global__ void k_add_corr(int n, double* x, const double* d, double scale) {
+    for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) x[i] += scale * d[i];
+}
+// Mixed form, reliable update: the FP32 recurrence residual is replaced by the
+// true residual b - A x of the inner system, computed in FP64 from the FP64
+// iterate, with its norm partials for the convergence test that follows.
+__global__ void __launch_bounds__(BS) k_replace_res(int n, const int* rs, const int* col, const double* A,
+                                                    const double* x, const double* b, float* r, Parts Pt,
+                                                    const double* scal, int norm) {
+    if (scal[S_DONE] != 0.0) return;
+    const int sub = threadIdx.x & (kLpr - 1);
+    double v = 0.0;
+    const int rowsPerIter = gridDim.x * (BS / kLpr);
+    for (int base = blockIdx.x * (BS / kLpr); base < n; base += rowsPerIter) {
+        const int ii = base + threadIdx.x / kLpr;
+        const int i = ii < n ? ii : -1;
+        const double acc = row_dot(rs, col, A, x, i, sub);
+        if (i >= 0 && sub == 0) {
+            const float ri = (float)(b[i] - acc);
+            r[i] = ri;
+            v = norm == ASTG_NORM_INF ? fmax(v, fabs((double)ri)) : v + (double)ri * (double)ri;
+        }
+    }
+    const double t = norm == ASTG_NORM_INF ? block_reduce<true>(v) : block_reduce<false>(v);
+    if (threadIdx.x == 0) Pt.res[blockIdx.x] = t;
 }
 
 // ---------------------------------------------------------------------------
@@ -600,6 +633,9 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
     // sweep merging: levels per group (1 = plain level order)
     merge_ = 2;
     if (const char* g = getenv("ASTG_MERGE")) merge_ = std::max(1, std::min(8, atoi(g)));
+    // mixed form: residual replacement period and inner tolerance factor
+    if (const char* g = getenv("ASTG_MIXED_REPLACE")) mixedReplace_ = std::max(0, atoi(g));
+    if (const char* g = getenv("ASTG_MIXED_ETA")) mixedEta_ = std::max(0.01, atof(g));
     int dev = 0, cus = 0;
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -904,8 +940,9 @@ int PcgDevice::launchBwd(const T* invd, const T* r, T* z, int k, int parityOut, 
 }
 
 template <typename T>
-int PcgDevice::pcgInner(const T* A, const T* invd, const T* b, T* x, double tolRel, int maxIt, int norm, int& iters,
-                        double& resOut, bool& conv, std::string& err) {
+int PcgDevice::pcgInner(const T* A, const T* invd, const T* b, double* x, double tolRel, int maxIt, int norm,
+                        int& iters, double& resOut, bool& conv, std::string& err, const double* b64,
+                        int replaceEvery) {
     const int n = n_, gs = gs_;
     const int gb = bwdGrid<T>();
     T* r = r_.as<T>();
@@ -918,7 +955,7 @@ int PcgDevice::pcgInner(const T* A, const T* invd, const T* b, T* x, double tolR
     const Parts Pt = parts(gb);
     int rc;
     k_zero64<<<1, BS, 0, stream_>>>(32, scal);
-    k_init<T><<<gs, BS, 0, stream_>>>(n, b, x, r, Pt, norm);
+    k_init<T, double><<<gs, BS, 0, stream_>>>(n, b, x, r, Pt, norm);
     k_fill_sent<T><<<blocks(2 * PS_), BS, 0, stream_>>>(y_.as<T>(), 2 * PS_);
     if ((rc = launchFwd<T>(invd, r, 0, 0, 0, maxIt, norm, tolRel, err))) return rc;
     if ((rc = launchBwd<T>(invd, r, z, 0, 0, err))) return rc;
@@ -929,7 +966,10 @@ int PcgDevice::pcgInner(const T* A, const T* invd, const T* b, T* x, double tolR
     for (;;) {
         for (int j = 0; j < batch && it < maxIt; ++j, ++it) {
             k_spmv_dot<T><<<gs, BS, 0, stream_>>>(n, rs, col, A, s, q, Pt, scal);
-            k_axpy_res<T><<<gs, BS, 0, stream_>>>(n, s, q, x, r, Pt, scal, norm, it);
+            k_axpy_res<T, double><<<gs, BS, 0, stream_>>>(n, s, q, x, r, Pt, scal, norm, it);
+            if (sizeof(T) == 4 && replaceEvery > 0 && (it + 1) % replaceEvery == 0)
+                k_replace_res<<<gs, BS, 0, stream_>>>(n, rs, col, val_.as<double>(), x, b64,
+                                                      reinterpret_cast<float*>(r), Pt, scal, norm);
             if ((rc = launchFwd<T>(invd, r, it + 1, 1, it, maxIt, norm, tolRel, err))) return rc;
             if ((rc = launchBwd<T>(invd, r, z, it + 1, (it + 1) & 1, err))) return rc;
             k_xpby<T><<<gs, BS, 0, stream_>>>(n, z, s, Pt, scal, 1, it);
@@ -963,7 +1003,7 @@ int PcgDevice::solve(const astg_pcg_params& prm, astg_pcg_result& res, std::stri
         double r = 0;
         bool conv = false;
         int rc = pcgInner<double>(val_.as<double>(), invd_.as<double>(), b_.as<double>(), x_.as<double>(), prm.tol,
-                                  prm.max_iterations, prm.norm, it, r, conv, err);
+                                  prm.max_iterations, prm.norm, it, r, conv, err, nullptr, 0);
         if (rc) return rc;
         res.iterations = it;
         res.residual = r;
@@ -972,13 +1012,17 @@ int PcgDevice::solve(const astg_pcg_params& prm, astg_pcg_result& res, std::stri
         if (hscal_[S_BNORM] == 0.0) res.converged = 1;
         return ASTG_OK;
     }
-    // mixed: FP32 inner PCG, FP64 outer residual
+    // mixed: FP32 inner PCG whose correction accumulates in FP64 and whose
+    // recurrence residual is replaced every `replace` iterations by the FP64
+    // residual of the inner system (reliable updates); FP64 outer residual.
     HIPCHK(r64_.alloc((size_t)n * 8));
     HIPCHK(d64_.alloc((size_t)n * 8));
+    HIPCHK(rhs64_.alloc((size_t)n * 12));
     if (int rc = convert32(err)) return rc;
     k_zero64<<<blocks(n), BS, 0, stream_>>>(n, x_.as<double>());
-    float* rhs32 = reinterpret_cast<float*>(d64_.p);
-    float* d32 = reinterpret_cast<float*>(d64_.as<char>() + (size_t)n * 4);
+    double* rhs64 = rhs64_.as<double>();
+    float* rhs32 = reinterpret_cast<float*>(rhs64 + n);
+    double* d64 = d64_.as<double>();
     double bnorm = -1, target = 0;
     int total = 0, outer = 0;
     const int maxOuter = prm.max_outer > 0 ? prm.max_outer : 20;
@@ -1001,21 +1045,20 @@ int PcgDevice::solve(const astg_pcg_params& prm, astg_pcg_result& res, std::stri
             break;
         }
         if (outer >= maxOuter || total >= prm.max_iterations) break;
-        // Inner relative tolerance: half the outer target. The FP32 recurrence
-        // residual keeps falling below FP32's attainable accuracy, so the first
-        // inner solve does almost all the work and one short second pass
-        // (from the FP64 residual) finishes; the floor only guards tiny targets.
-        const double eta = std::max(0.5 * target / rnorm, 5e-7);
-        k_scale_f32<<<gs, BS, 0, stream_>>>(n, r64_.as<double>(), rhs32, 1.0 / rnorm);
+        // Inner relative tolerance: a fraction of the outer target; with the
+        // residual replaced by the true one, the inner recurrence residual
+        // tracks the FP64 residual, so one inner solve normally suffices.
+        const double eta = std::max(mixedEta_ * target / rnorm, 1e-7);
+        k_scale_rhs<<<gs, BS, 0, stream_>>>(n, r64_.as<double>(), 1.0 / rnorm, rhs64, rhs32);
         int it = 0;
         double ir = 0;
         bool ic = false;
-        rc = pcgInner<float>(val32_.as<float>(), invd32_.as<float>(), rhs32, d32, eta, prm.max_iterations - total,
-                             prm.norm, it, ir, ic, err);
+        rc = pcgInner<float>(val32_.as<float>(), invd32_.as<float>(), rhs32, d64, eta, prm.max_iterations - total,
+                             prm.norm, it, ir, ic, err, rhs64, mixedReplace_);
         if (rc) return rc;
         total += it;
         ++outer;
-        k_add_corr<<<gs, BS, 0, stream_>>>(n, x_.as<double>(), d32, rnorm);
+        k_add_corr<<<gs, BS, 0, stream_>>>(n, x_.as<double>(), d64, rnorm);
         if (it == 0) break;
     }
     res.iterations = total;
@@ -1187,8 +1230,8 @@ int PcgDevice::sweepTimeline(int which, std::vector<long long>& t, std::vector<i
 }
 
 template int PcgDevice::pcgInner<double>(const double*, const double*, const double*, double*, double, int, int, int&,
-                                         double&, bool&, std::string&);
-template int PcgDevice::pcgInner<float>(const float*, const float*, const float*, float*, double, int, int, int&,
-                                        double&, bool&, std::string&);
+                                         double&, bool&, std::string&, const double*, int);
+template int PcgDevice::pcgInner<float>(const float*, const float*, const float*, double*, double, int, int, int&,
+                                        double&, bool&, std::string&, const double*, int);
 
 }  // namespace astg

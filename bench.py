@@ -24,9 +24,10 @@ KERNEL_NAMES = ("spmv64", "fwd64", "bwd64", "mic", "assemble", "spmv32", "fwd32"
 KERNELS = tuple((nm, i) for i, nm in enumerate(KERNEL_NAMES))
 
 # HBM bytes per launch from rocprofv3 counter runs (FETCH_SIZE + WRITE_SIZE,
-# scaled by the copy-kernel calibration); filled in from the committed
-# counter summaries, None where not collected.
-TRAFFIC = {}
+# scaled by the copy-kernel calibration) of `bench.py --kernels-only` on this
+# workload, as summarised in profiles/traffic_projection_256.md.
+TRAFFIC = {"spmv64": 644036001, "fwd64": 1137658278, "bwd64": 1336729770, "spmv32": 429851975,
+           "fwd32": 888763642, "bwd32": 1103413339, "copy": 56200384, "mic": 7376658496}
 
 
 def log(*a):
@@ -87,7 +88,7 @@ def advect_timing(ctx, info, prm, reps):
                     "2 velocity samples per back-trace"}
 
 
-def cpu_baseline(ctx, res64, budget_iters=12):
+def cpu_baseline(ctx, res64, budget_iters=150):
     """Reference MIC(0)-PCG (oracle/_ref, the reference's own code) or the
     oracle's port on the same C2 system: factor + a bounded number of PCG
     iterations, timed, then scaled to the full iteration count."""
