@@ -32,7 +32,8 @@ EXPORTS = [
     "astg_get_weights", "astg_set_velocity", "astg_get_velocity", "astg_project", "astg_get_system",
     "astg_get_mic_factor", "astg_fields_snapshot", "astg_kernel_time", "astg_sweep_timeline",
     "astg_set_scalar", "astg_get_scalar", "astg_post_project", "astg_scalar_update_ghost", "astg_get_dual",
-    "astg_sample_points", "astg_adjust_e", "astg_set_tilt_adaptivity", "astg_advect", "astg_step",
+    "astg_sample_points", "astg_apply_sphere_source", "astg_add_force", "astg_adjust_e",
+    "astg_set_tilt_adaptivity", "astg_advect", "astg_step",
 ]
 FIELD_DENSITY, FIELD_HEAT = 0, 1
 SAMPLE_VELOCITY, SAMPLE_MAC, SAMPLE_BACKTRACE, SAMPLE_DENSITY, SAMPLE_HEAT = range(5)
@@ -100,6 +101,8 @@ def lib():
         "astg_scalar_update_ghost": (I, [P, I]),
         "astg_get_dual": (I, [P, I, P, P]),
         "astg_sample_points": (I, [P, I, I, P, D, P]),
+        "astg_apply_sphere_source": (I, [P, D, D, D, D, D, D, D, I]),
+        "astg_add_force": (I, [P, D, D, D]),
         "astg_adjust_e": (I, [P, D, D, D, D]),
         "astg_set_tilt_adaptivity": (I, [P, I]),
         "astg_advect": (I, [P, D]),
@@ -313,6 +316,16 @@ class Context:
         out = np.empty((len(xyz), 3)) if what <= SAMPLE_BACKTRACE else np.empty(len(xyz))
         self._chk(self._L.astg_sample_points(self.h, what, len(xyz), _ptr(xyz), dt, _ptr(out)), "sample_points")
         return out
+
+    def apply_sphere_source(self, center, radius, density=1.0, heat=1.0, speed=1.0, update_ghost=False):
+        """The octree smoke example's sphere source (density, heat, +z inflow)."""
+        cx, cy, cz = center
+        self._chk(self._L.astg_apply_sphere_source(self.h, cx, cy, cz, radius, density, heat, speed,
+                                                   1 if update_ghost else 0), "apply_sphere_source")
+
+    def add_force(self, dt, smoke_density_factor=-0.000625, temperature_factor=5.0):
+        """addForce: buoyancy along +z (the reference's default factors)."""
+        self._chk(self._L.astg_add_force(self.h, dt, smoke_density_factor, temperature_factor), "add_force")
 
     def adjust_e(self, d_threshold, dt, v_threshold, max_threshold):
         """adjustE: tilt open/close decision and commit; fields are kept."""

@@ -9,9 +9,13 @@ a scene is a pure function of (name, seed).
   e = h0/2; face velocities N(0,1) sampled in FP32 on every MAC face and tilt
   slot; closed x and y walls, open z walls (the reference's default flags,
   octree_omnigrid_solver3.cpp:155-156); dt = 0.5/256, rho = 2.
-* ``smoke_64``: 64^3, 3 levels, one source sphere r=0.08 at (0.5,0.12,0.5),
+* ``smoke_64``: the plain smoke configuration: 64^3, 3 levels, one source
+  sphere r=0.08 rising along +z, the reference's buoyancy axis
+  (addForce, octree_omnigrid_solver3.cpp:917-972): centre (0.5,0.5,0.12);
   level 0 within 4 finest cells of the source, one level coarser every 8 cells
-  of that level further out.
+  of that level further out; closed x and y walls, open z walls; fields start
+  at rest; every step applies the source (density 1, inflow 1) and a buoyancy
+  of 1.0 x density (no heat term); dt = 0.5/64, rho = 2, tilt adaptivity on.
 """
 import numpy as np
 
@@ -65,15 +69,31 @@ def projection_256(device=0, seed=0, R=256, levels=4, band=BAND_256, velocities=
     return ctx, info
 
 
-def smoke_64(device=0, R=64, levels=3, seed=0):
+SMOKE_SOURCE = (0.5, 0.5, 0.12, 0.08)
+
+
+def smoke_64_split(R=64, levels=3):
     h0 = 1.0 / R
-    src = np.array([[0.5, 0.12, 0.5, 0.08]])
     band = [4 * h0]
     for l in range(1, levels - 1):
         band.append(band[-1] + 8 * h0 * (1 << l))
-    split = split_spheres(R, levels, src.ravel(), band)
+    return split_spheres(R, levels, np.array([SMOKE_SOURCE]).ravel(), band)
+
+
+def smoke_64(device=0, R=64, levels=3):
+    split = smoke_64_split(R, levels)
     ctx = Context(device)
-    ctx.set_topology(split, R, levels, WALL_XNEG | WALL_XPOS | WALL_ZNEG | WALL_ZPOS)
-    if device >= 0:
-        fill_velocity(ctx, levels, seed)
-    return ctx, {"R": R, "levels": levels, "dt": 0.5 / R, "rho": 2.0}
+    ctx.set_topology(split, R, levels, CLOSED_XY)
+    info = {"R": R, "levels": levels, "dt": 0.5 / R, "rho": 2.0, "split": split, "closed": CLOSED_XY,
+            "source": SMOKE_SOURCE, "buoyancy": (1.0, 0.0)}
+    return ctx, info
+
+
+def smoke_step(ctx, info, first, params):
+    """One step of the smoke configuration: sceneSetup (the source), addForce,
+    then onUpdate's constrain / project / advect chain (with tilt adaptivity)."""
+    cx, cy, cz, r = info["source"]
+    ctx.apply_sphere_source((cx, cy, cz), r, 1.0, 1.0, 1.0, update_ghost=first)
+    ctx.add_force(info["dt"], *info["buoyancy"])
+    ctx.set_tilt_adaptivity(True)
+    return ctx.step(info["dt"], info["rho"], params)

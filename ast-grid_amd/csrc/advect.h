@@ -18,6 +18,7 @@
 //   advect                             src/octree_omnigrid_solver3.cpp:850-914,
 //                                      include/advection.h:195-239
 //   adjustE (decision)                 src/octree_omnigrid_solver3.cpp:1125-1149
+//   addForce                           src/octree_omnigrid_solver3.cpp:917-972
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -80,6 +81,13 @@ public:
     int decideTilts(ProjectDevice& pd, double dThr, double dt, double vThr, double maxThr, std::vector<uint8_t>& open,
                     std::string& err);
     int leaves(int level) const { return nleaf_[level]; }
+    // Sphere smoke source of the octree smoke example (sphereSource,
+    // example/octree_smoke_3d/main.cpp:23-81) with parameters; ghost: update
+    // the scalar ghosts afterwards (the example does on its first call).
+    int applySource(ProjectDevice& pd, double cx, double cy, double cz, double r, double dval, double hval,
+                    double speed, bool ghost, std::string& err);
+    // addForce (octree_omnigrid_solver3.cpp:917-972): buoyancy along +z.
+    int addForce(ProjectDevice& pd, double dt, double fd, double ft, std::string& err);
 
 private:
     int omniGhost(double* cell, double* vert, int nc, std::string& err);

@@ -913,6 +913,141 @@ __global__ void k_adjust_e(Topo T, const uint8_t* level, Vel V, const double* dC
     open[t] = o;
 }
 
+// ---------------------------------------------------------------------------
+// sphere source and buoyancy
+// ---------------------------------------------------------------------------
+__device__ inline bool inSphere(double px, double py, double pz, double cx, double cy, double cz, double r) {
+    const double dx = px - cx, dy = py - cy, dz = pz - cz;
+    return sqrt(dx * dx + dy * dy + dz * dz) < r;
+}
+
+// Smoke source of the octree smoke example (example/octree_smoke_3d/main.cpp:
+// 23-81, sphereSource), with its centre, radius, values and inflow speed as
+// parameters: inside the sphere, density and heat on cell and vertex DOFs, +z
+// velocity on MAC z-faces, and speed/sqrt(3) on all 8 slots of every leaf tilt
+// whose centre is inside.
+__global__ void k_source(Topo T, const uint8_t* level, const int* leaf, int n, double cx, double cy, double cz,
+                         double r, double dval, double hval, double speed, double* dC, double* dV, double* hC,
+                         double* hV, double* mac, double* tilt) {
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= n) return;
+    const int g = leaf[t];
+    const double h = c_hl[level[g]];
+    int x, y, z;
+    nodeXYZ(T, g, x, y, z);
+    const double X = x * h, Y = y * h, Z = z * h;
+    if (inSphere(X + 0.5 * h, Y + 0.5 * h, Z + 0.5 * h, cx, cy, cz, r)) {
+        dC[g] = dval;
+        hC[g] = hval;
+    }
+    if (inSphere(X, Y, Z, cx, cy, cz, r)) {
+        dV[g] = dval;
+        hV[g] = hval;
+        for (int k = 0; k < 8; ++k) tilt[(size_t)k * c_G + g] = speed / kSqrt3;
+    }
+    if (inSphere(X + 0.5 * h, Y + 0.5 * h, Z, cx, cy, cz, r)) mac[2 * (size_t)c_G + g] = speed;
+}
+
+// The same source on half-tilts: density and heat at the vertex, and the four
+// tilt slots of the half-tilt (plus its face slot when the face is a z-face).
+__global__ void k_source_half(Topo T, const uint8_t* level, const int* half, int n, double cx, double cy, double cz,
+                              double r, double dval, double hval, double speed, double* dV, double* hV, double* tilt) {
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= n) return;
+    const int g = half[t];
+    const double h = c_hl[level[g]];
+    int x, y, z;
+    nodeXYZ(T, g, x, y, z);
+    if (!inSphere(x * h, y * h, z * h, cx, cy, cz, r)) return;
+    dV[g] = dval;
+    hV[g] = hval;
+    const int dir = T.thd[g];
+    double nv[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < 4; ++i) nv[c_halfSlot[dir][i]] = speed / kSqrt3;
+    if (dir % 3 == 2) nv[c_halfSlot[dir][4]] = speed;
+    for (int k = 0; k < 8; ++k) tilt[(size_t)k * c_G + g] = nv[k];
+}
+
+// Per-block sums of one level's cell values (mean temperature of addForce).
+__global__ void k_sum_range(const double* a, int off, int n, double* part) {
+    double v = 0.0;
+    for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) v += a[off + i];
+    v = wave_sum(v);
+    __shared__ double sm[BS / 64];
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < BS / 64; ++w) s += sm[w];
+        part[blockIdx.x] = s;
+    }
+}
+
+// addForce (octree_omnigrid_solver3.cpp:917-972): buoyancy f = fd * density +
+// ft * (heat - ambT) sampled at each DOF, added along +z over dt.
+__device__ inline double buoyancy(const Topo& T, const double* dC, const double* dV, const double* hC,
+                                  const double* hV, double fd, double ft, double ambT, const double* p) {
+    return fd * scalarSample(T, dC, dV, p[0], p[1], p[2]) + ft * (scalarSample(T, hC, hV, p[0], p[1], p[2]) - ambT);
+}
+__global__ void k_force_mac(Topo T, const uint8_t* level, const int* leaf, int n, const double* dC, const double* dV,
+                            const double* hC, const double* hV, double fd, double ft, double ambT, double dt,
+                            double* mac) {
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= n) return;
+    const int g = leaf[t];
+    const int l = level[g], res = c_res[l];
+    const double h = c_hl[l];
+    int x, y, z;
+    nodeXYZ(T, g, x, y, z);
+    if (!(x <= res - 2 && y <= res - 2 && z <= res - 1)) return;  // isValidZ
+    const double p[3] = {x * h + 0.5 * h, y * h + 0.5 * h, z * h};
+    double* u = mac + 2 * (size_t)c_G + g;
+    *u = *u + dt * buoyancy(T, dC, dV, hC, hV, fd, ft, ambT, p);
+}
+__global__ void k_force_tilt(Topo T, const uint8_t* level, const int* openTilt, int n, const double* dC,
+                             const double* dV, const double* hC, const double* hV, double fd, double ft, double ambT,
+                             double dt, double* tilt) {
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= 8 * n) return;
+    const int g = openTilt[t >> 3], k = t & 7;
+    const double h = c_hl[level[g]];
+    int x, y, z;
+    nodeXYZ(T, g, x, y, z);
+    const double c[3] = {x * h, y * h, z * h};
+    double p[3], d[3];
+    tiltPos(c, k, T.te[g], p);
+    tiltDir(k, d);
+    double* u = tilt + (size_t)k * c_G + g;
+    *u = *u + dt * (d[2] * buoyancy(T, dC, dV, hC, hV, fd, ft, ambT, p));
+}
+__global__ void k_force_half(Topo T, const uint8_t* level, const int* half, int n, const double* dC,
+                             const double* dV, const double* hC, const double* hV, double fd, double ft, double ambT,
+                             double dt, double* tilt) {
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= n) return;
+    const int g = half[t];
+    const int dir = T.thd[g];
+    const double h = c_hl[level[g]];
+    int x, y, z;
+    nodeXYZ(T, g, x, y, z);
+    const double c[3] = {x * h, y * h, z * h};
+    for (int i = 0; i < 5; ++i) {
+        const int k = c_halfSlot[dir][i];
+        double p[3], dz;
+        if (i == 4) {
+            p[0] = c[0], p[1] = c[1], p[2] = c[2];
+            dz = (k == 0 || k == 1 || k == 2 || k == 4) ? 0.0 : 1.0;  // HALF_TILT_XYZ_DIR_POSITIVE[k].z
+        } else {
+            double d[3];
+            tiltPos(c, k, T.te[g], p);
+            tiltDir(k, d);
+            dz = d[2];
+        }
+        double* u = tilt + (size_t)k * c_G + g;
+        *u = *u + dt * (buoyancy(T, dC, dV, hC, hV, fd, ft, ambT, p) * dz);
+    }
+}
+
 __global__ void k_scale2(long n, double s, double* a, double* b) {
     const long i = (long)blockIdx.x * BS + threadIdx.x;
     if (i < n) {
@@ -1408,6 +1543,70 @@ int AdvectDevice::advect(ProjectDevice& pd, double dt, std::string& err) {
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(stream_));
     sampValid_ = false;  // the fields now carry the current tilt state
+    return ASTG_OK;
+}
+
+int AdvectDevice::applySource(ProjectDevice& pd, double cx, double cy, double cz, double r, double dval, double hval,
+                              double speed, bool ghost, std::string& err) {
+    const Topo T = topo();
+    const uint8_t* lev = static_cast<const uint8_t*>(level_.p);
+    double* dC = scal_[0][cur_[0]][0].as<double>();
+    double* dV = scal_[0][cur_[0]][1].as<double>();
+    double* hC = scal_[1][cur_[1]][0].as<double>();
+    double* hV = scal_[1][cur_[1]][1].as<double>();
+    if (nLeaf_)
+        k_source<<<nblocks(nLeaf_), BS, 0, stream_>>>(T, lev, leaf_.as<int>(), nLeaf_, cx, cy, cz, r, dval, hval, speed,
+                                                      dC, dV, hC, hV, pd.uMac(), pd.uTilt());
+    if (nHalf_)
+        k_source_half<<<nblocks(nHalf_), BS, 0, stream_>>>(T, lev, half_.as<int>(), nHalf_, cx, cy, cz, r, dval, hval,
+                                                           speed, dV, hV, pd.uTilt());
+    HIPCHK(hipGetLastError());
+    if (ghost) {
+        int rc = scalarGhost(0, err);
+        if (rc) return rc;
+        rc = scalarGhost(1, err);
+        if (rc) return rc;
+    }
+    HIPCHK(hipStreamSynchronize(stream_));
+    return ASTG_OK;
+}
+
+int AdvectDevice::addForce(ProjectDevice& pd, double dt, double fd, double ft, std::string& err) {
+    const Topo T = topo();
+    const uint8_t* lev = static_cast<const uint8_t*>(level_.p);
+    const double* dC = scal_[0][cur_[0]][0].as<double>();
+    const double* dV = scal_[0][cur_[0]][1].as<double>();
+    const double* hC = scal_[1][cur_[1]][0].as<double>();
+    const double* hV = scal_[1][cur_[1]][1].as<double>();
+    // ambient temperature: mean of the cell heat over every node of every
+    // level, nodes of levels above 0 counted 8 times (as addForce weighs them)
+    const int nb = 64;
+    HIPCHK(out_.alloc((size_t)L_ * nb * 8));
+    for (int l = 0; l < L_; ++l)
+        k_sum_range<<<nb, BS, 0, stream_>>>(hC, off_[l], nnode_[l], out_.as<double>() + (size_t)l * nb);
+    std::vector<double> part((size_t)L_ * nb);
+    HIPCHK(hipMemcpyAsync(part.data(), out_.p, part.size() * 8, hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    double sum = 0.0, count = 0.0;
+    for (int l = 0; l < L_; ++l) {
+        double s = 0.0;
+        for (int b = 0; b < nb; ++b) s += part[(size_t)l * nb + b];
+        const double w = l == 0 ? 1.0 : 8.0;
+        sum += s * w;
+        count += (double)nnode_[l] * w;
+    }
+    const double ambT = count > 0 ? sum / count : 0.0;
+    if (nLeaf_)
+        k_force_mac<<<nblocks(nLeaf_), BS, 0, stream_>>>(T, lev, leaf_.as<int>(), nLeaf_, dC, dV, hC, hV, fd, ft, ambT,
+                                                         dt, pd.uMac());
+    if (nOpenTilt_)
+        k_force_tilt<<<nblocks(8L * nOpenTilt_), BS, 0, stream_>>>(T, lev, openTilt_.as<int>(), nOpenTilt_, dC, dV, hC,
+                                                                   hV, fd, ft, ambT, dt, pd.uTilt());
+    if (nHalf_)
+        k_force_half<<<nblocks(nHalf_), BS, 0, stream_>>>(T, lev, half_.as<int>(), nHalf_, dC, dV, hC, hV, fd, ft, ambT,
+                                                          dt, pd.uTilt());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(stream_));
     return ASTG_OK;
 }
 

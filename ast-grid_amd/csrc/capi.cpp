@@ -649,6 +649,21 @@ int astg_sample_points(astg_ctx* c, int what, int n, const double* xyz, double d
     return c->adv->samplePoints(*c->proj, what, n, xyz, dt, out, c->err);
 }
 
+int astg_apply_sphere_source(astg_ctx* c, double cx, double cy, double cz, double radius, double density,
+                             double heat, double speed, int update_ghost) {
+    if (!(radius > 0)) return fail(c, ASTG_ERR_ARG, "radius must be positive");
+    int rc = needAdvect(c);
+    if (rc) return rc;
+    return c->adv->applySource(*c->proj, cx, cy, cz, radius, density, heat, speed, update_ghost != 0, c->err);
+}
+
+int astg_add_force(astg_ctx* c, double dt, double smoke_density_factor, double temperature_factor) {
+    if (!(dt >= 0)) return fail(c, ASTG_ERR_ARG, "dt must be non-negative");
+    int rc = needAdvect(c);
+    if (rc) return rc;
+    return c->adv->addForce(*c->proj, dt, smoke_density_factor, temperature_factor, c->err);
+}
+
 int astg_adjust_e(astg_ctx* c, double d_threshold, double dt, double v_threshold, double max_threshold) {
     if (!(dt >= 0)) return fail(c, ASTG_ERR_ARG, "dt must be non-negative");
     int rc = needAdvect(c);

@@ -210,6 +210,25 @@ int astg_get_dual(astg_ctx* ctx, int level, double* oct3, double* tilt3);
 #define ASTG_SAMPLE_HEAT 4
 int astg_sample_points(astg_ctx* ctx, int what, int n, const double* xyz, double dt, double* out);
 
+/* Scene source of the reference's octree smoke example (sphereSource,
+ * example/octree_smoke_3d/main.cpp:23-81) with its constants as parameters:
+ * inside the sphere (centre, radius) density and heat are set on cell and
+ * vertex DOFs (and half-tilts), MAC z-faces get `speed`, every leaf tilt whose
+ * centre is inside gets speed/sqrt(3) on all 8 slots, half-tilts their 4 tilt
+ * slots (and their face slot for z-faces). update_ghost: then update the
+ * density and heat ghosts (the example does so on its first call). The
+ * example's solver calls it at the start of every step (sceneSetup). */
+int astg_apply_sphere_source(astg_ctx* ctx, double cx, double cy, double cz, double radius, double density,
+                             double heat, double speed, int update_ghost);
+
+/* OctreeOmniGridSolver3::addForce (src/octree_omnigrid_solver3.cpp:917-972):
+ * buoyancy f = smoke_density_factor * density + temperature_factor * (heat -
+ * ambT), ambT the mean cell heat (nodes of levels above 0 weighted 8), sampled
+ * at every valid MAC z-face, open tilt slot and half-tilt slot and added along
+ * +z over dt. The reference's defaults are -0.000625 and 5.0
+ * (octree_omnigrid_solver3.h:125-126). */
+int astg_add_force(astg_ctx* ctx, double dt, double smoke_density_factor, double temperature_factor);
+
 /* OctreeOmniGridSolver3::adjustE (src/octree_omnigrid_solver3.cpp:1125-1166):
  * every non-fixed leaf tilt closes when open and its density leaves
  * (d_threshold, max_threshold), or opens when closed, the velocity at the tilt
@@ -232,8 +251,9 @@ int astg_set_tilt_adaptivity(astg_ctx* ctx, int on);
  * the 0.999 decay of both scalars. Needs astg_post_project first. */
 int astg_advect(astg_ctx* ctx, double dt);
 
-/* One step of onUpdate without addForce: constrain + ghosts, astg_project,
- * astg_post_project, astg_advect. */
+/* One step of onUpdate after sceneSetup and addForce (which the caller runs
+ * first: astg_apply_sphere_source, astg_add_force): constrain + ghosts,
+ * astg_project, astg_post_project, astg_advect. */
 int astg_step(astg_ctx* ctx, double dt, double rho, const astg_pcg_params* params, astg_pcg_result* result);
 
 #ifdef __cplusplus

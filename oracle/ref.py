@@ -68,6 +68,9 @@ def lib():
         "ref_adjust_e": (I, [P, D, D, D, D]),
         "ref_set_tilt_adaptivity": (None, [P, I]),
         "ref_tilt_nodes_of": (None, [P, I, I, P, P, P]),
+        "ref_set_source": (None, [P, D, D, D, D, D, D, D]),
+        "ref_set_buoyancy": (None, [P, D, D]),
+        "ref_step": (I, [P, D, D, I]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -212,6 +215,17 @@ class RefSolver:
 
     def set_tilt_adaptivity(self, on):
         self.L.ref_set_tilt_adaptivity(self.h, 1 if on else 0)
+
+    def set_source(self, center, radius, density=1.0, heat=1.0, speed=1.0):
+        self.L.ref_set_source(self.h, center[0], center[1], center[2], radius, density, heat, speed)
+
+    def set_buoyancy(self, smoke_density_factor, temperature_factor):
+        self.L.ref_set_buoyancy(self.h, smoke_density_factor, temperature_factor)
+
+    def step(self, dt, tol=1e-6, maxit=3000):
+        """One onUpdate(dt): sceneSetup, addForce, constrain, project, advect."""
+        if self.L.ref_step(self.h, dt, tol, maxit):
+            raise RuntimeError(self.L.ref_last_error())
 
     def project(self, dt, tol=1e-6, maxit=3000):
         if self.L.ref_project(self.h, dt, tol, maxit):

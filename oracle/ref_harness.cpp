@@ -92,12 +92,76 @@ public:
     void doAdvect(double dt) { advect(dt); }
     void doAdjustE(double dThr, double dt, double vThr, double maxThr) { adjustE(dThr, dt, vThr, maxThr); }
     void setTiltAdaptivity(bool on) { enableTiltAdaptivity = on; }
+    void setBuoyancy(double fd, double ft) {
+        _buoyancySmokeDensityFactor = fd;
+        _buoyancyTemperatureFactor = ft;
+    }
+
+    // sceneSetup of the reference's octree smoke example (sphereSource,
+    // example/octree_smoke_3d/main.cpp:23-81) with its centre, radius, values
+    // and inflow speed as parameters; onUpdate calls it first in every step.
+    void setSource(double cx, double cy, double cz, double r, double dval, double hval, double speed) {
+        srcC_ = Vector3f(cx, cy, cz);
+        srcR_ = r;
+        srcD_ = dval;
+        srcH_ = hval;
+        srcV_ = speed;
+        srcFirst_ = true;
+        sceneSetup = [this](OctreeOmniGridSolver3*) { applySource(); };
+    }
+    void applySource() {
+        auto in = [&](const Vector3f& p) { return (p - srcC_).norm() < srcR_; };
+        auto& dOct = density->getOctagonGrid();
+        auto& dTilt = density->getTiltGrid();
+        auto& hOct = heat->getOctagonGrid();
+        auto& hTilt = heat->getTiltGrid();
+        ThreadPool::parallelIterateGrid(dOct->getParallelIteratorVec(), [&](const OctreeGridIterator3& it) {
+            if (in(dOct->position(it.levelCoor()))) {
+                dOct->set(it.levelCoor(), srcD_);
+                hOct->set(it.levelCoor(), srcH_);
+            }
+        });
+        ThreadPool::parallelIterateGrid(dTilt->getParallelIteratorVec(), [&](const OctreeGridIterator3& it) {
+            if (in(dTilt->position(it.levelCoor()))) {
+                dTilt->set(it.levelCoor(), srcD_);
+                hTilt->set(it.levelCoor(), srcH_);
+            }
+        });
+        parallel_iterate_half_tilt(layout, [&](LCOOR_T c) {
+            if (in(dTilt->position(c))) {
+                dTilt->set(c, srcD_);
+                hTilt->set(c, srcH_);
+            }
+        });
+        auto& mac = velocity->getMACGrid();
+        auto& tv = velocity->getTiltGrid();
+        ThreadPool::parallelIterateGrid(mac->getParallelIteratorVec(), [&](const OctreeGridIterator3& it) {
+            if (in(mac->positionZ(it.levelCoor()))) mac->setZ(it.levelCoor(), srcV_);
+            if (in(tv->positionCenter(it.levelCoor()))) tv->set(it.levelCoor(), Vector8f::Ones() * srcV_ * INV_SQRT3);
+        });
+        parallel_iterate_half_tilt(layout, [&](LCOOR_T c) {
+            if (!in(tv->positionCenter(c))) return;
+            Vector8f v = Vector8f::Zero();
+            const int dir = tiltE->get(c).half_direction;
+            for (int i = 0; i < 4; ++i) v(HALF_TILT_SLOT_INDEX[dir][i]) = srcV_ * INV_SQRT3;
+            if (dir % 3 == 2) v(HALF_TILT_SLOT_INDEX[dir][4]) = srcV_;
+            tv->set(c, v);
+        });
+        if (srcFirst_) {
+            density->updateGhost();
+            heat->updateGhost();
+            srcFirst_ = false;
+        }
+    }
     void doStep(double dt) { onUpdate(dt); }
 
 private:
     std::vector<int> split_;
     int R_;
     int closed_;
+    Vector3f srcC_ = Vector3f::Zero();
+    double srcR_ = 0, srcD_ = 0, srcH_ = 0, srcV_ = 0;
+    bool srcFirst_ = true;
 };
 
 HarnessSolver* H(void* p) { return static_cast<HarnessSolver*>(p); }
@@ -335,6 +399,23 @@ int ref_adjust_e(void* p, double dThr, double dt, double vThr, double maxThr) {
 // enableTiltAdaptivity (octree_omnigrid_solver3.h:56): advect(dt) then starts
 // with adjustE(0.02, dt, EPSILON, 1000) (octree_omnigrid_solver3.cpp:852-853).
 void ref_set_tilt_adaptivity(void* p, int on) { H(p)->setTiltAdaptivity(on != 0); }
+
+// Scene source for onUpdate's sceneSetup (see HarnessSolver::setSource) and
+// addForce's buoyancy factors.
+void ref_set_source(void* p, double cx, double cy, double cz, double r, double dval, double hval, double speed) {
+    H(p)->setSource(cx, cy, cz, r, dval, hval, speed);
+}
+void ref_set_buoyancy(void* p, double fd, double ft) { H(p)->setBuoyancy(fd, ft); }
+
+// One onUpdate(dt) (octree_omnigrid_solver3.cpp:222-249) with the PCG
+// tolerance factor and iteration cap given here.
+int ref_step(void* p, double dt, double tol, int maxit) {
+    auto* s = H(p);
+    return guarded([&] {
+        s->pcg().set_solver_parameters(tol, maxit);
+        s->doStep(dt);
+    });
+}
 
 int ref_advect(void* p, double dt) {
     return guarded([&] { H(p)->doAdvect(dt); });
