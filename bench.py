@@ -88,6 +88,33 @@ def advect_timing(ctx, info, prm, reps):
                     "2 velocity samples per back-trace"}
 
 
+def smoke_timing(steps=50):
+    """The plain smoke configuration (64^3, 3 levels, rising source, buoyancy,
+    tilt adaptivity): `steps` full steps from rest, each sceneSetup + addForce
+    + onUpdate's chain (projection to 1e-6 in FP64, advection); wall time per
+    step including the tilt updates (host retopology)."""
+    import torch
+    import astg
+    from astg import scenes
+    ctx, info = scenes.smoke_64(device=0)
+    prm = astg.default_params(tol=1e-6, max_iterations=3000)
+    torch.cuda.synchronize()
+    iters = []
+    t0 = time.perf_counter()
+    for k in range(steps):
+        iters.append(scenes.smoke_step(ctx, info, k == 0, prm).iterations)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    open_tilts = 0
+    for l in range(info["levels"]):
+        _, bits, _ = ctx.tilt(l)
+        open_tilts += int(((bits[:ctx.level_info(l)[1]] & 64) != 0).sum())
+    ctx.close()
+    return {"ms_per_step": round(ms, 3), "steps": steps, "pcg_iterations_mean": round(sum(iters) / len(iters), 1),
+            "open_tilts_at_end": open_tilts,
+            "note": "64^3, 3 levels; source + addForce + constrain/project/post chain/adjustE/advect per step"}
+
+
 def cpu_baseline(ctx, res64, budget_iters=150):
     """Reference MIC(0)-PCG (oracle/_ref, the reference's own code) or the
     oracle's port on the same C2 system: factor + a bounded number of PCG
@@ -188,6 +215,10 @@ def main():
     log("advect %.2f ms (post-project chain %.2f ms), %.1f M back-traces/s" % (
         adv["ms_advect"], adv["ms_post_project"], adv["backtraces_per_s"] / 1e6))
 
+    smoke = smoke_timing()
+    log("smoke_64: %.2f ms/step over %d steps (mean PCG iterations %.1f)" % (
+        smoke["ms_per_step"], smoke["steps"], smoke["pcg_iterations_mean"]))
+
     cpu = None
     if not a.no_cpu:
         try:
@@ -223,6 +254,7 @@ def main():
         "setup_s": round(setup_s, 2),
         "roofline": roofline,
         "advect": adv,
+        "smoke_64": smoke,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
