@@ -12,6 +12,7 @@
 
 #include "../../include/astg.h"
 #include "octree.h"
+#include "timing.h"
 #include "pcg.h"
 #include "advect.h"
 #include "project.h"
@@ -110,21 +111,33 @@ int ensureTables(astg_ctx* c) {
 // topology and the projection pattern follow the new tilt state; the node
 // sets, and so every field array, are unchanged.
 int retopologize(astg_ctx* c) {
+    astg::PhaseTimer tm("retopologize");
     astg::numberUnknowns(*c->tree, c->num);
+    tm.mark("numbering");
     astg::computeWeights(*c->tree, c->w);
+    tm.mark("weights");
     astg::buildProjectTables(*c->tree, c->num, c->w, c->tables);
+    tm.mark("tables");
     int rc = c->proj->upload(c->tables, *c->tree, c->err);
     if (rc) return rc;
+    tm.mark("project upload");
     rc = c->adv->upload(*c->tree, c->tables, true, c->err);
     if (rc) return rc;
-    return c->pcg->setPattern(c->tables.n, c->tables.rowstart.data(), c->tables.col.data(), c->err);
+    tm.mark("advect upload");
+    rc = c->pcg->setPattern(c->tables.n, c->tables.rowstart.data(), c->tables.col.data(), c->err);
+    tm.mark("solver pattern");
+    return rc;
 }
 
 int adjustTilts(astg_ctx* c, double dThr, double dt, double vThr, double maxThr) {
+    astg::PhaseTimer tm("adjustE");
     std::vector<uint8_t> open;
     int rc = c->adv->decideTilts(*c->proj, dThr, dt, vThr, maxThr, open, c->err);
     if (rc) return rc;
-    if (!c->tree->setTiltOpenAll(open.data())) return ASTG_OK;
+    tm.mark("decide");
+    const bool changed = c->tree->setTiltOpenAll(open.data());
+    tm.mark("commit");
+    if (!changed) return ASTG_OK;
     return retopologize(c);
 }
 

@@ -29,6 +29,7 @@
 #include "pcg.h"
 #include "mic_plan.h"
 #include "sweep_merge.h"
+#include "timing.h"
 
 #define HIPCHK(expr)                                                                 \
     do {                                                                             \
@@ -326,7 +327,6 @@ __global__ void __launch_bounds__(BS) k_fwd(SweepPlan S, StencilView<T> St, cons
             inv = invd[i];
             for (int p = b; p < br; ++p) acc -= St.coef[p] * r[St.idx[p]];
             d.preload(St.idx, St.coef, br, e);
-            Sentinel<T>::storeSent(&zp[t]);  // armed for the backward sweep
         }
         gate_row(y, S.gateF[t >> 6], S.gateCap);
         const long long tg = S.trace ? wall_clock64() : 0;
@@ -341,6 +341,8 @@ __global__ void __launch_bounds__(BS) k_fwd(SweepPlan S, StencilView<T> St, cons
                 }
             });
             if (!ok) atomicOr(&cnt[C_ERR], E_FWD);
+            // armed for the backward sweep; after publishing, off the dependency path
+            Sentinel<T>::storeSent(&zp[t]);
         }
     }
 }
@@ -365,9 +367,7 @@ __global__ void __launch_bounds__(BS) k_bwd_dot(SweepPlan S, StencilView<T> St, 
             const int b = St.start[pos], e = St.start[pos + 1], br = b + St.nr[pos];
             acc = y[pos];
             for (int p = b; p < br; ++p) acc -= St.coef[p] * y[St.idx[p]];
-            Sentinel<T>::storeSent(&ynext[pos]);
             inv = invd[k];
-            rk = r[k];
             d.preload(St.idx, St.coef, br, e);
         }
         gate_row(zp, S.gateB[pos >> 6], S.gateCap);
@@ -386,6 +386,10 @@ __global__ void __launch_bounds__(BS) k_bwd_dot(SweepPlan S, StencilView<T> St, 
                 }
             });
             if (!ok) atomicOr(&cnt[C_ERR], E_BWD);
+            // after publishing, off the dependency path: the dot with r and the
+            // re-arming of the next forward sweep's vector
+            rk = r[k];
+            Sentinel<T>::storeSent(&ynext[pos]);
             v += (double)zk * (double)rk;
         }
     }
@@ -586,6 +590,7 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
         err = "empty system";
         return ASTG_ERR_ARG;
     }
+    PhaseTimer tm("setPattern");
     n_ = n;
     nnz_ = rowstart[n];
     hrs_.assign(rowstart, rowstart + n + 1);
@@ -697,6 +702,7 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
         return ASTG_ERR_HIP;
     }
     depth_ = sortedLev[n - 1] + 1;
+    tm.mark("split, transpose map, levels, sort");
 
     // Rows in order of groups of q consecutive levels, every group padded to a
     // multiple of 64 rows so that a wavefront never holds rows of two groups;
@@ -742,6 +748,7 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
     // factorisation: rows in level order, one launch per level
     std::vector<int> padded, gF, gB;
     buildOrder(1, 1, padded, gF, gB);
+    tm.mark("level order");
     P_ = (int)padded.size();
     HIPCHK(upload(order_, padded.data(), (size_t)P_ * 4));
     {
@@ -769,6 +776,7 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
         HIPCHK(live_.alloc((size_t)n));
         HIPCHK(hipStreamSynchronize(stream_));
     }
+    tm.mark("factorisation plan");
     // sweeps: q levels per group, merged stencils (sweep_merge.h)
     buildOrder(merge_, gateDist_, padded, gF, gB);
     PS_ = (int)padded.size();
@@ -804,6 +812,7 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
         HIPCHK(y_.alloc((size_t)PS_ * 2 * 8));
         HIPCHK(zp_.alloc((size_t)PS_ * 8));
     }
+    tm.mark("sweep order and merged stencils");
     // persistent grids (all blocks co-resident)
     const int schunks = PS_ / BS;
     gF64_ = coopBlocks(k_fwd<double>, cus, schunks);
