@@ -27,7 +27,7 @@ NORM_INF, NORM_L2 = 0, 1
 # Header symbols, in the order include/astg.h declares them.
 EXPORTS = [
     "astg_pcg_params_default", "astg_create", "astg_destroy", "astg_last_error",
-    "astg_pcg_solve_csr", "astg_split_spheres", "astg_set_topology_split", "astg_set_tilt_open",
+    "astg_pcg_solve_csr", "astg_pcg_solve_batch", "astg_split_spheres", "astg_set_topology_split", "astg_set_tilt_open",
     "astg_level_info", "astg_get_nodes", "astg_get_tilt", "astg_get_half_tilts", "astg_get_numbering",
     "astg_get_weights", "astg_set_velocity", "astg_get_velocity", "astg_project", "astg_get_system",
     "astg_get_mic_factor", "astg_fields_snapshot", "astg_kernel_time", "astg_sweep_timeline",
@@ -78,6 +78,7 @@ def lib():
         "astg_destroy": (I, [P]),
         "astg_last_error": (ctypes.c_char_p, [P]),
         "astg_pcg_solve_csr": (I, [P, I, P, P, P, P, P, ctypes.POINTER(PcgParams), ctypes.POINTER(PcgResult)]),
+        "astg_pcg_solve_batch": (I, [P, I, P, P, P, P, P, P, ctypes.POINTER(PcgParams), ctypes.POINTER(PcgResult)]),
         "astg_split_spheres": (I, [I, I, I, P, P, P]),
         "astg_set_topology_split": (I, [P, I, D, I, P, I]),
         "astg_set_tilt_open": (I, [P, I, P]),
@@ -366,3 +367,22 @@ class Context:
                                              ctypes.byref(prm), ctypes.byref(res)), "pcg_solve_csr",
                   allow=(ASTG_ERR_NOT_CONVERGED,) if allow_unconverged else ())
         return x, res
+
+    def pcg_solve_batch(self, systems, params=None, allow_unconverged=False):
+        """Batched form of pcg_solve_csr: systems = [(rowstart, col, val, b), ...].
+        Returns ([x_k], [PcgResult_k])."""
+        K = len(systems)
+        keep = []
+        for rs, col, val, b in systems:
+            keep.append((np.ascontiguousarray(rs, np.int32), np.ascontiguousarray(col, np.int32),
+                         np.ascontiguousarray(val, np.float64), np.ascontiguousarray(b, np.float64)))
+        n = np.array([len(s[0]) - 1 for s in keep], np.int32)
+        xs = [np.empty(int(nk), np.float64) for nk in n]
+        arr = lambda items: (ctypes.c_void_p * K)(*[a.ctypes.data for a in items])
+        res = (PcgResult * K)()
+        prm = params if params is not None else default_params()
+        self._chk(self._L.astg_pcg_solve_batch(self.h, K, _ptr(n), arr([s[0] for s in keep]), arr([s[1] for s in keep]),
+                                               arr([s[2] for s in keep]), arr([s[3] for s in keep]), arr(xs),
+                                               ctypes.byref(prm), res), "pcg_solve_batch",
+                  allow=(ASTG_ERR_NOT_CONVERGED,) if allow_unconverged else ())
+        return xs, list(res)

@@ -9,6 +9,12 @@ a scene is a pure function of (name, seed).
   e = h0/2; face velocities N(0,1) sampled in FP32 on every MAC face and tilt
   slot; closed x and y walls, open z walls (the reference's default flags,
   octree_omnigrid_solver3.cpp:155-156); dt = 0.5/256, rho = 2.
+* ``batch_scene`` / ``batch_systems``: the batched-solve workload: 24
+  independent octree Poisson systems with leaf counts log-uniform in
+  [5e4, 2e6]; scene k has 1-6 random spheres, 3-5 levels and R = 64, 128 or
+  256 by its target count, its sphere radii and refinement bands scaled by one
+  factor found by a secant search on log(leaves) until within 10% of the
+  target; level-0 tilts inside the spheres opened; N(0,1) FP32 face velocities.
 * ``smoke_64``: the plain smoke configuration: 64^3, 3 levels, one source
   sphere r=0.08 rising along +z, the reference's buoyancy axis
   (addForce, octree_omnigrid_solver3.cpp:917-972): centre (0.5,0.5,0.12);
@@ -67,6 +73,66 @@ def projection_256(device=0, seed=0, R=256, levels=4, band=BAND_256, velocities=
     info = {"R": R, "levels": levels, "leaves": leaves, "spheres": spheres, "dt": 0.5 / R, "rho": 2.0,
             "seed": seed}
     return ctx, info
+
+
+def leaf_count(split):
+    """Leaves of a split map: a level-l leaf covers 8^l finest cells."""
+    c = np.bincount(split)
+    return int(round(sum(c[l] / 8.0 ** l for l in range(len(c)))))
+
+
+def batch_targets(seed=0, count=24, lo=5e4, hi=2e6):
+    rng = np.random.default_rng(seed + 104729)
+    return np.exp(rng.uniform(np.log(lo), np.log(hi), count))
+
+
+def batch_scene_params(seed, k, target):
+    """(R, levels, spheres, band) of batch scene k, leaves within 10% of target."""
+    rng = np.random.default_rng([seed, k, 7])
+    R = 64 if target < 1.2e5 else (128 if target < 6e5 else 256)
+    levels = int(rng.integers(3, 6))
+    if R == 64:
+        levels = min(levels, 4)
+    m = int(rng.integers(1, 7))
+    base = random_spheres(int(rng.integers(0, 2 ** 31)), count=m)
+
+    def build(sc):
+        sph = base.copy()
+        sph[:, 3] *= sc
+        band = [sc * 24.0 * (1 << l) / R for l in range(levels - 1)]
+        split = split_spheres(R, levels, sph.ravel(), band)
+        return sph, band, split
+
+    # leaves grow about like scale^2 (refined shells): secant on the logs
+    s0, s1 = 1.0, None
+    sph, band, split = build(s0)
+    l0 = leaf_count(split)
+    s1 = s0 * (target / l0) ** 0.5
+    for _ in range(8):
+        if abs(np.log(l0 / target)) < np.log(1.1):
+            break
+        sph, band, split = build(s1)
+        l1 = leaf_count(split)
+        if abs(np.log(l1 / target)) < np.log(1.1) or l1 == l0:
+            l0 = l1
+            break
+        slope = (np.log(l1) - np.log(l0)) / (np.log(s1) - np.log(s0))
+        slope = min(max(slope, 0.5), 4.0)
+        s0, l0 = s1, l1
+        s1 = s0 * (target / l0) ** (1.0 / slope)
+    return R, levels, sph, band, split
+
+
+def batch_scene(seed, k, target, device=0):
+    R, levels, sph, band, split = batch_scene_params(seed, k, target)
+    ctx = Context(device)
+    ctx.set_topology(split, R, levels, CLOSED_XY)
+    xyz, _ = ctx.nodes(0)
+    nleaf = ctx.level_info(0)[1]
+    ctx.set_tilt_open(0, inside_mask(xyz[:nleaf], R, sph))
+    fill_velocity(ctx, levels, seed * 1000 + k)
+    leaves = sum(ctx.level_info(l)[1] for l in range(levels))
+    return ctx, {"R": R, "levels": levels, "leaves": leaves, "spheres": len(sph), "dt": 0.5 / R, "rho": 2.0}
 
 
 SMOKE_SOURCE = (0.5, 0.5, 0.12, 0.08)

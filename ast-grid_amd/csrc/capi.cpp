@@ -35,6 +35,8 @@ struct astg_ctx {
     std::unique_ptr<astg::AdvectDevice> adv;              // post-projection step and advection
     std::unique_ptr<astg::PcgDevice> csr;                 // CSR seam
     std::vector<int> csrRs, csrCol;
+    std::unique_ptr<astg::PcgDevice> batch;               // batched CSR solves (stacked systems)
+    std::vector<int> batchRs, batchCol, batchSeg;
     astg::PcgDevice* lastPcg = nullptr;
     hipEvent_t ev[6] = {};
     bool tiltAdaptivity = false;                          // enableTiltAdaptivity
@@ -543,6 +545,115 @@ int astg_pcg_solve_csr(astg_ctx* c, int n, const int* rowstart, const int* colin
         c->lastPcg = c->csr.get();
         if (result) *result = res;
         if (!res.converged) return fail(c, ASTG_ERR_NOT_CONVERGED, "PCG did not reach the tolerance");
+        return ASTG_OK;
+    });
+}
+
+int astg_pcg_solve_batch(astg_ctx* c, int count, const int* n, const int* const* rowstart,
+                         const int* const* colindex, const double* const* value, const double* const* b,
+                         double* const* x, const astg_pcg_params* params, astg_pcg_result* results) {
+    if (!c) return ASTG_ERR_ARG;
+    if (count <= 0 || !n || !rowstart || !colindex || !value || !b || !x)
+        return fail(c, ASTG_ERR_ARG, "null or empty batch");
+    int rc = needDevice(c);
+    if (rc) return rc;
+    astg_pcg_params prm;
+    astg_pcg_params_default(&prm);
+    if (params) prm = *params;
+    return guarded(c, [&] {
+        constexpr int kChunk = 256;
+        // stacked layout: system k owns rows [seg[k], seg[k+1]), padded to 256
+        std::vector<int> seg(count + 1, 0);
+        long long rows = 0, nnzAll = 0;
+        for (int k = 0; k < count; ++k) {
+            if (n[k] <= 0 || !rowstart[k] || !colindex[k] || !value[k] || !b[k] || !x[k])
+                return fail(c, ASTG_ERR_ARG, "system " + std::to_string(k) + ": null or empty input");
+            const int* rs = rowstart[k];
+            if (rs[0] != 0) return fail(c, ASTG_ERR_ARG, "system " + std::to_string(k) + ": bad rowstart");
+            for (int i = 0; i < n[k]; ++i)
+                if (rs[i + 1] < rs[i]) return fail(c, ASTG_ERR_ARG, "system " + std::to_string(k) + ": rowstart not monotone");
+            const long long padded = ((long long)n[k] + kChunk - 1) / kChunk * kChunk;
+            nnzAll += rs[n[k]] + (padded - n[k]);
+            rows += padded;
+            if (rows >= (1LL << 31) - kChunk || nnzAll >= (1LL << 31))
+                return fail(c, ASTG_ERR_ARG, "batch exceeds 32-bit row or entry indices");
+            seg[k + 1] = (int)rows;
+        }
+        const int N = (int)rows;
+        std::vector<int> srs((size_t)N + 1), scol((size_t)nnzAll);
+        std::vector<double> sval((size_t)nnzAll), sb((size_t)N, 0.0);
+        long long p = 0;
+        for (int k = 0; k < count; ++k) {
+            const int* rs = rowstart[k];
+            const int* cl = colindex[k];
+            const int r0 = seg[k], nk = n[k];
+            for (int i = 0; i < nk; ++i) {
+                srs[(size_t)r0 + i] = (int)p;
+                for (int q = rs[i]; q < rs[i + 1]; ++q, ++p) {
+                    const int j = cl[q];
+                    if (j < 0 || j >= nk)
+                        return fail(c, ASTG_ERR_ARG, "system " + std::to_string(k) + ": column index out of range");
+                    scol[(size_t)p] = r0 + j;
+                    sval[(size_t)p] = value[k][q];
+                }
+            }
+            std::memcpy(sb.data() + r0, b[k], (size_t)nk * 8);
+            for (int i = r0 + nk; i < seg[k + 1]; ++i, ++p) {  // identity padding rows
+                srs[(size_t)i] = (int)p;
+                scol[(size_t)p] = i;
+                sval[(size_t)p] = 1.0;
+            }
+        }
+        srs[(size_t)N] = (int)p;
+        if (!c->batch) c->batch.reset(new astg::PcgDevice(c->stream));
+        const bool same = c->batchSeg == seg && c->batchRs == srs && c->batchCol == scol;
+        if (!same) {
+            c->batchRs.clear();
+            c->batchCol.clear();
+            c->batchSeg.clear();
+            int r = c->batch->setPattern(N, srs.data(), scol.data(), c->err);
+            if (r) return r;
+            if ((r = c->batch->setBatch(seg, c->err))) return r;
+            c->batchRs.swap(srs);
+            c->batchCol.swap(scol);
+            c->batchSeg = seg;
+        }
+        hipEvent_t* ev = c->ev;
+        if (hipMemcpyAsync(c->batch->values(), sval.data(), (size_t)nnzAll * 8, hipMemcpyHostToDevice, c->stream) !=
+                hipSuccess ||
+            hipMemcpyAsync(c->batch->rhs(), sb.data(), (size_t)N * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+            return fail(c, ASTG_ERR_HIP, "upload failed");
+        (void)hipEventRecord(ev[0], c->stream);
+        int r = c->batch->factor(prm.mic_tau, prm.min_diag_ratio, c->err);
+        if (r) return r;
+        (void)hipEventRecord(ev[1], c->stream);
+        std::vector<astg_pcg_result> res;
+        r = c->batch->solveBatch(prm, res, c->err);
+        if (r) return r;
+        (void)hipEventRecord(ev[2], c->stream);
+        const double* xd = c->batch->solution();
+        for (int k = 0; k < count; ++k)
+            if (hipMemcpyAsync(x[k], xd + seg[k], (size_t)n[k] * 8, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+                return fail(c, ASTG_ERR_HIP, "download failed");
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(c, ASTG_ERR_HIP, "download failed");
+        float ms0 = 0, ms1 = 0, ms2 = 0;
+        (void)hipEventElapsedTime(&ms0, ev[0], ev[1]);
+        (void)hipEventElapsedTime(&ms1, ev[1], ev[2]);
+        (void)hipEventElapsedTime(&ms2, ev[0], ev[2]);
+        int unconverged = 0;
+        for (int k = 0; k < count; ++k) {
+            astg_pcg_result& R = res[k];
+            R.n = n[k];
+            R.nnz = rowstart[k][n[k]];
+            R.ms_factor = ms0;
+            R.ms_solve = ms1;
+            R.ms_total = ms2;
+            if (!R.converged) ++unconverged;
+            if (results) results[k] = R;
+        }
+        if (unconverged)
+            return fail(c, ASTG_ERR_NOT_CONVERGED,
+                        std::to_string(unconverged) + " system(s) did not reach the tolerance");
         return ASTG_OK;
     });
 }

@@ -71,6 +71,32 @@ struct Parts {
     int gs, gb;
 };
 
+// Batched form: the layout of K stacked systems and their per-system state.
+enum BatchSlot {
+    B_BNORM = 0,  // ||b|| of the system (inner right-hand side)
+    B_TOL,        // absolute tolerance of the inner solve
+    B_RES,        // last residual norm
+    B_DONE,       // 0 running; 1 converged, 2 breakdown, 3 iteration limit, 4 not solved this time
+    B_ITER,       // iterations performed
+    B_RHO,        // z.r
+    B_ALPHA,      // rho / s.q
+    B_BETA,       // rho_new / rho
+    B_TOLREL,     // relative tolerance of the inner solve (set by the host)
+    B_MAXIT,      // iteration limit of the inner solve (set by the host)
+    B_SKIP,       // nonzero: the system takes no part in the inner solve (set by the host)
+    B_SCALE,      // mixed form: 1 / outer residual norm (inner right-hand side scaling)
+    B_OUTER,      // mixed form: FP64 outer residual norm
+    B_COUNT = 16
+};
+struct BatchView {
+    const int* chunkSeg;  // per 256-row chunk: its system
+    const int* segChunk;  // K + 1: first chunk of each system
+    double* st;           // K x B_COUNT
+    double* part;         // one partial per chunk
+    int* ndone;           // systems finished in the current inner solve
+    int K, nchunks;
+};
+
 class PcgDevice {
 public:
     explicit PcgDevice(hipStream_t s) : stream_(s) {}
@@ -107,6 +133,15 @@ public:
     // Reference-form copy of the factor (host arrays), for parity checks.
     int downloadFactor(std::vector<double>& invdiag, std::vector<unsigned>& colstart, std::vector<unsigned>& rowindex,
                        std::vector<double>& value);
+    // Batched form (pcg_batch.hip). The pattern set by setPattern is the
+    // block-diagonal stack of K systems; system k owns rows
+    // [segRow[k], segRow[k+1]), every segRow a multiple of 256 (rows past the
+    // system's own are identity rows with a zero right-hand side).
+    int setBatch(const std::vector<int>& segRow, std::string& err);
+    int batchCount() const { return K_; }
+    // Solves every system of the stack with its own PCG scalars, convergence
+    // test and iteration count (rhs(), solution(), values() hold the stack).
+    int solveBatch(const astg_pcg_params& prm, std::vector<astg_pcg_result>& res, std::string& err);
     const std::vector<int>& hostRowstart() const { return hrs_; }
     const std::vector<int>& hostCol() const { return hcol_; }
 
@@ -124,6 +159,13 @@ private:
                   std::string& err);
     template <typename T>
     int launchBwd(const T* invd, const T* r, T* z, int k, int parityOut, std::string& err);
+    // Batched inner PCG over every system not marked B_SKIP (state set by the
+    // host beforehand); per-system results stay in the device state.
+    template <typename T>
+    int pcgBatchInner(const T* A, const T* invd, const T* b, double* x, int maxIt, int norm, std::string& err,
+                      const double* b64, int replaceEvery);
+    int batchState(std::vector<double>& st, bool toDevice, std::string& err);
+    BatchView bview() const;
     int launchMic(double tau, double ratio, std::string& err);
     int launchMerge(std::string& err);
     int convert32(std::string& err);
@@ -171,6 +213,8 @@ private:
     int pollCap_ = 1;
     int gateCap_ = 8;
     long long* trace_ = nullptr;            // set only while a timeline is recorded
+    int K_ = 0;                             // batched form: systems in the stack (0: single system)
+    DevBuf chunkSeg_, segChunk_, bst_, bpart_, bcnt_;  // batched form: layout, state, partials, counter
 };
 
 }  // namespace astg

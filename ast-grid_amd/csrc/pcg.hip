@@ -27,23 +27,12 @@
 
 #include "device_common.h"
 #include "pcg.h"
+#include "pcg_common.h"
 #include "mic_plan.h"
 #include "sweep_merge.h"
 #include "timing.h"
 
-#define HIPCHK(expr)                                                                 \
-    do {                                                                             \
-        hipError_t e_ = (expr);                                                      \
-        if (e_ != hipSuccess) {                                                      \
-            err = std::string(#expr) + ": " + hipGetErrorString(e_);                 \
-            return ASTG_ERR_HIP;                                                     \
-        }                                                                            \
-    } while (0)
-
 namespace astg {
-
-constexpr int BS = 256;
-constexpr int kLpr = 8;  // lanes per row in the SpMV kernels
 
 DevBuf::~DevBuf() {
     if (p) (void)hipFree(p);
@@ -93,30 +82,6 @@ __global__ void k_fill64(int n, double* x, double v) {
 __global__ void __launch_bounds__(BS) k_copy64(int n, const double* __restrict__ a, double* __restrict__ b) {
     const int i = blockIdx.x * BS + threadIdx.x;
     if (i < n) b[i] = a[i];
-}
-
-// ---------------------------------------------------------------------------
-// deterministic block reductions
-// ---------------------------------------------------------------------------
-template <bool MAX>
-__device__ inline double block_reduce(double v) {
-    __shared__ double sm[BS / 64];
-    v = MAX ? wave_max(v) : wave_sum(v);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    __syncthreads();  // sm may still be read by a previous call
-    if (lane == 0) sm[wid] = v;
-    __syncthreads();
-    double t = sm[0];
-    for (int w = 1; w < BS / 64; ++w) t = MAX ? fmax(t, sm[w]) : t + sm[w];
-    return t;
-}
-// Sum (or max) of np partials written by the previous kernel, same order in
-// every block.
-template <bool MAX>
-__device__ inline double reduce_prev(const double* part, int np) {
-    double acc = 0.0;
-    for (int s = threadIdx.x; s < np; s += BS) acc = MAX ? fmax(acc, part[s]) : acc + part[s];
-    return block_reduce<MAX>(acc);
 }
 
 // ---------------------------------------------------------------------------
@@ -279,7 +244,8 @@ __global__ void __launch_bounds__(BS) k_init(int n, const T* b, X* x, T* r, Part
 
 // Forward sweep y = L^-1 r. mode 0 (start of the solve): ||b|| from the
 // partials, tolerance, zero-rhs exit. mode 1 (iteration `it`): the reference's
-// convergence test on the updated residual, before preconditioning.
+// convergence test on the updated residual, before preconditioning. mode 2:
+// no test here (the batched solve tests each system in its own kernel).
 // Returns true when the solve stops here (every block takes the same decision).
 __device__ inline bool fwd_prologue(const Parts& Pt, double* scal, int mode, int it, int maxIt, int norm,
                                     double tolRel) {
@@ -314,7 +280,7 @@ __global__ void __launch_bounds__(BS) k_fwd(SweepPlan S, StencilView<T> St, cons
                                             Parts Pt, int* cnt, double* scal, int mode, int it, int maxIt, int norm,
                                             double tolRel) {
     if (scal[S_DONE] != 0.0) return;
-    if (fwd_prologue(Pt, scal, mode, it, maxIt, norm, tolRel)) return;
+    if (mode != 2 && fwd_prologue(Pt, scal, mode, it, maxIt, norm, tolRel)) return;
     for (int c = blockIdx.x; c < S.nchunks; c += gridDim.x) {
         const int t = c * BS + threadIdx.x;
         const int i = S.order[t];
@@ -448,20 +414,7 @@ __global__ void __launch_bounds__(BS) k_xpby(int n, const T* z, T* s, Parts Pt, 
     for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) s[i] = z[i] + beta * s[i];
 }
 
-// q = A s with kLpr lanes per row (rows are short: 8 entries on average), so a
-// wavefront reads the entries of 64/kLpr consecutive rows contiguously; the
-// row sum is a shuffle reduction over the row's lanes. Fused with s.q.
-template <typename T>
-__device__ inline T row_dot(const int* rs, const int* col, const T* A, const T* s, int i, int sub) {
-    T acc = (T)0;
-    if (i >= 0) {
-        for (int p = rs[i] + sub, pe = rs[i + 1]; p < pe; p += kLpr) acc += A[p] * s[col[p]];
-    }
-#pragma unroll
-    for (int o = kLpr / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kLpr);
-    return acc;
-}
-
+// q = A s (row_dot, kLpr lanes per row), fused with s.q.
 template <typename T>
 __global__ void __launch_bounds__(BS) k_spmv_dot(int n, const int* rs, const int* col, const T* A, const T* s, T* q,
                                                  Parts Pt, const double* scal) {
@@ -1238,6 +1191,11 @@ int PcgDevice::sweepTimeline(int which, std::vector<long long>& t, std::vector<i
     return ASTG_OK;
 }
 
+template int PcgDevice::launchFwd<double>(const double*, const double*, int, int, int, int, int, double,
+                                          std::string&);
+template int PcgDevice::launchFwd<float>(const float*, const float*, int, int, int, int, int, double, std::string&);
+template int PcgDevice::launchBwd<double>(const double*, const double*, double*, int, int, std::string&);
+template int PcgDevice::launchBwd<float>(const float*, const float*, float*, int, int, std::string&);
 template int PcgDevice::pcgInner<double>(const double*, const double*, const double*, double*, double, int, int, int&,
                                          double&, bool&, std::string&, const double*, int);
 template int PcgDevice::pcgInner<float>(const float*, const float*, const float*, double*, double, int, int, int&,

@@ -1,0 +1,488 @@
+// Batched MIC(0)-PCG: many independent systems solved by one sequence of
+// launches.
+//
+// The systems are stacked into one block-diagonal matrix, every system's rows
+// padded with identity rows (zero right-hand side) to a multiple of 256, so
+// that every 256-row chunk belongs to one system. The pattern-only work
+// (dependency levels, factorisation plan, merged sweep stencils) and the
+// factorisation are PcgDevice's own on the stacked matrix. With no coupling
+// between the blocks, the MIC(0) factor of the stack is the stack of the
+// systems' factors and a row's dependency level is its level inside its own
+// system, so the sweeps compute what the per-system sweeps compute, and a
+// batch is as deep as its deepest system rather than the sum of the depths.
+//
+// Per system: the PCG scalars (rho, alpha, beta, the norms), the reference's
+// convergence test (pcg_solver.h:396-421) and the iteration count. Streaming
+// kernels write one partial per chunk; kb_reduce, one block per system, sums
+// the system's partials in chunk order (deterministic) and advances the
+// system's scalars. A finished system's chunks are skipped by the streaming
+// kernels. The sweeps run over all rows: a finished system's residual no
+// longer changes, so its rows recompute values nobody reads.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "pcg.h"
+#include "pcg_common.h"
+
+namespace astg {
+
+namespace {
+
+enum BatchPhase { P_INIT = 0, P_RHO0, P_SQ, P_RES, P_RHO, P_OUTER };
+
+__device__ inline bool chunk_live(const BatchView& V, int c) {
+    return V.st[V.chunkSeg[c] * B_COUNT + B_DONE] == 0.0;
+}
+__device__ inline double norm_part(double a, int norm) { return norm == ASTG_NORM_INF ? fabs(a) : a * a; }
+__device__ inline double norm_reduce(double v, int norm) {
+    return norm == ASTG_NORM_INF ? block_reduce<true>(v) : block_reduce<false>(v);
+}
+
+template <typename T, typename X>
+__global__ void __launch_bounds__(BS) kb_init(BatchView V, const T* b, X* x, T* r, int norm) {
+    for (int c = blockIdx.x; c < V.nchunks; c += gridDim.x) {
+        const int i = c * BS + threadIdx.x;
+        const T bi = b[i];
+        x[i] = (X)0;
+        r[i] = bi;
+        const double t = norm_reduce(norm_part((double)bi, norm), norm);
+        if (threadIdx.x == 0) V.part[c] = t;
+    }
+}
+
+// One block per system: the system's partials in chunk order, then the
+// phase's scalar update (thread 0). A system that finishes here is counted;
+// the last one to finish raises the all-done flag that ends the sweeps.
+__global__ void __launch_bounds__(BS) kb_reduce(BatchView V, int phase, int norm, double* scal) {
+    const int k = blockIdx.x;
+    double* s = V.st + (size_t)k * B_COUNT;
+    if (phase != P_OUTER && s[B_DONE] != 0.0) return;
+    const bool isNorm = phase == P_INIT || phase == P_RES || phase == P_OUTER;
+    const bool mx = isNorm && norm == ASTG_NORM_INF;
+    const int c0 = V.segChunk[k], c1 = V.segChunk[k + 1];
+    double acc = 0.0;
+    for (int c = c0 + threadIdx.x; c < c1; c += BS) acc = mx ? fmax(acc, V.part[c]) : acc + V.part[c];
+    double red = mx ? block_reduce<true>(acc) : block_reduce<false>(acc);
+    if (threadIdx.x != 0) return;
+    if (isNorm && !mx) red = sqrt(red);
+    bool fin = false;
+    switch (phase) {
+        case P_INIT:
+            s[B_ITER] = 0.0;
+            if (s[B_SKIP] != 0.0) {
+                s[B_DONE] = 4.0;
+                fin = true;
+                break;
+            }
+            s[B_BNORM] = red;
+            s[B_RES] = red;
+            s[B_TOL] = s[B_TOLREL] * red;
+            if (red == 0.0) {
+                s[B_DONE] = 1.0;
+                fin = true;
+            } else if (s[B_MAXIT] <= 0.0) {
+                s[B_DONE] = 3.0;
+                fin = true;
+            }
+            break;
+        case P_RHO0:  // the reference's breakdown test after the first preconditioning
+            if (red == 0.0 || red != red) {
+                s[B_DONE] = 2.0;
+                s[B_ITER] = 0.0;
+                fin = true;
+            } else {
+                s[B_RHO] = red;
+            }
+            break;
+        case P_SQ:
+            s[B_ALPHA] = s[B_RHO] / red;
+            break;
+        case P_RES: {
+            const double it1 = s[B_ITER] + 1.0;
+            s[B_RES] = red;
+            s[B_ITER] = it1;
+            if (red <= s[B_TOL]) {
+                s[B_DONE] = 1.0;
+                fin = true;
+            } else if (it1 >= s[B_MAXIT]) {
+                s[B_DONE] = 3.0;
+                fin = true;
+            }
+            break;
+        }
+        case P_RHO:
+            s[B_BETA] = red / s[B_RHO];
+            s[B_RHO] = red;
+            break;
+        case P_OUTER:
+            s[B_OUTER] = red;
+            break;
+    }
+    if (fin && atomicAdd(V.ndone, 1) + 1 == V.K) scal[S_DONE] = 1.0;
+}
+
+// Partials of z.r per chunk (the single-system form fuses this dot into the
+// backward sweep, whose rows are in sweep order and mix the systems).
+template <typename T>
+__global__ void __launch_bounds__(BS) kb_dot(BatchView V, const T* z, const T* r) {
+    for (int c = blockIdx.x; c < V.nchunks; c += gridDim.x) {
+        if (!chunk_live(V, c)) continue;
+        const int i = c * BS + threadIdx.x;
+        const double t = block_reduce<false>((double)z[i] * (double)r[i]);
+        if (threadIdx.x == 0) V.part[c] = t;
+    }
+}
+
+// q = A s and the partials of s.q, one chunk (256 rows) per block pass.
+template <typename T>
+__global__ void __launch_bounds__(BS) kb_spmv_dot(BatchView V, const int* rs, const int* col, const T* A, const T* s,
+                                                  T* q) {
+    const int sub = threadIdx.x & (kLpr - 1);
+    for (int c = blockIdx.x; c < V.nchunks; c += gridDim.x) {
+        if (!chunk_live(V, c)) continue;
+        double v = 0.0;
+        for (int j = 0; j < kLpr; ++j) {
+            const int i = c * BS + j * (BS / kLpr) + threadIdx.x / kLpr;
+            const T acc = row_dot(rs, col, A, s, i, sub);
+            if (sub == 0) {
+                q[i] = acc;
+                v += (double)s[i] * (double)acc;
+            }
+        }
+        const double t = block_reduce<false>(v);
+        if (threadIdx.x == 0) V.part[c] = t;
+    }
+}
+
+template <typename T, typename X>
+__global__ void __launch_bounds__(BS) kb_axpy_res(BatchView V, const T* s, const T* q, X* x, T* r, int norm) {
+    for (int c = blockIdx.x; c < V.nchunks; c += gridDim.x) {
+        if (!chunk_live(V, c)) continue;
+        const double ad = V.st[V.chunkSeg[c] * B_COUNT + B_ALPHA];
+        const T alpha = (T)ad;
+        const X ax = sizeof(X) > sizeof(T) ? (X)ad : (X)alpha;
+        const int i = c * BS + threadIdx.x;
+        x[i] += ax * (X)s[i];
+        const T ri = r[i] - alpha * q[i];
+        r[i] = ri;
+        const double t = norm_reduce(norm_part((double)ri, norm), norm);
+        if (threadIdx.x == 0) V.part[c] = t;
+    }
+}
+
+// s = z (mode 0) or s = z + beta s (mode 1).
+template <typename T>
+__global__ void __launch_bounds__(BS) kb_xpby(BatchView V, const T* z, T* s, int mode) {
+    for (int c = blockIdx.x; c < V.nchunks; c += gridDim.x) {
+        if (!chunk_live(V, c)) continue;
+        const int i = c * BS + threadIdx.x;
+        if (mode == 0) {
+            s[i] = z[i];
+        } else {
+            const T beta = (T)V.st[V.chunkSeg[c] * B_COUNT + B_BETA];
+            s[i] = z[i] + beta * s[i];
+        }
+    }
+}
+
+// Mixed form, reliable update: r = b64 - A x in FP64, stored in FP32.
+__global__ void __launch_bounds__(BS) kb_replace_res(BatchView V, const int* rs, const int* col, const double* A,
+                                                     const double* x, const double* b, float* r, int norm) {
+    const int sub = threadIdx.x & (kLpr - 1);
+    for (int c = blockIdx.x; c < V.nchunks; c += gridDim.x) {
+        if (!chunk_live(V, c)) continue;
+        double v = 0.0;
+        for (int j = 0; j < kLpr; ++j) {
+            const int i = c * BS + j * (BS / kLpr) + threadIdx.x / kLpr;
+            const double acc = row_dot(rs, col, A, x, i, sub);
+            if (sub == 0) {
+                const float ri = (float)(b[i] - acc);
+                r[i] = ri;
+                v = norm == ASTG_NORM_INF ? fmax(v, fabs((double)ri)) : v + (double)ri * (double)ri;
+            }
+        }
+        const double t = norm_reduce(v, norm);
+        if (threadIdx.x == 0) V.part[c] = t;
+    }
+}
+
+// Mixed form, outer residual r = b - A x (FP64) of every system.
+__global__ void __launch_bounds__(BS) kb_residual64(BatchView V, const int* rs, const int* col, const double* A,
+                                                    const double* x, const double* b, double* r, int norm) {
+    const int sub = threadIdx.x & (kLpr - 1);
+    for (int c = blockIdx.x; c < V.nchunks; c += gridDim.x) {
+        double v = 0.0;
+        for (int j = 0; j < kLpr; ++j) {
+            const int i = c * BS + j * (BS / kLpr) + threadIdx.x / kLpr;
+            const double acc = row_dot(rs, col, A, x, i, sub);
+            if (sub == 0) {
+                const double ri = b[i] - acc;
+                r[i] = ri;
+                v = norm == ASTG_NORM_INF ? fmax(v, fabs(ri)) : v + ri * ri;
+            }
+        }
+        const double t = norm_reduce(v, norm);
+        if (threadIdx.x == 0) V.part[c] = t;
+    }
+}
+
+// Mixed form: inner right-hand side r64 / ||r64|| (zero for systems not
+// solved this time), and the correction x += ||r64|| d of the solved ones.
+__global__ void __launch_bounds__(BS) kb_scale_rhs(BatchView V, const double* r, double* o64, float* o32) {
+    for (int c = blockIdx.x; c < V.nchunks; c += gridDim.x) {
+        const double* s = V.st + V.chunkSeg[c] * B_COUNT;
+        const int i = c * BS + threadIdx.x;
+        const double v = s[B_SKIP] != 0.0 ? 0.0 : r[i] * s[B_SCALE];
+        o64[i] = v;
+        o32[i] = (float)v;
+    }
+}
+__global__ void __launch_bounds__(BS) kb_add_corr(BatchView V, double* x, const double* d) {
+    for (int c = blockIdx.x; c < V.nchunks; c += gridDim.x) {
+        const double* s = V.st + V.chunkSeg[c] * B_COUNT;
+        if (s[B_SKIP] != 0.0) continue;
+        const int i = c * BS + threadIdx.x;
+        x[i] += s[B_OUTER] * d[i];
+    }
+}
+
+__global__ void kb_zero_count(int* c) {
+    if (threadIdx.x == 0) c[0] = 0;
+}
+__global__ void kb_zero64(int n, double* x) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] = 0.0;
+}
+template <typename T>
+__global__ void kb_fill_sent(T* p, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) Sentinel<T>::storeSent(&p[i]);
+}
+inline int nblocks(long n) { return (int)((n + BS - 1) / BS); }
+
+}  // namespace
+
+int PcgDevice::setBatch(const std::vector<int>& segRow, std::string& err) {
+    const int K = (int)segRow.size() - 1;
+    if (K < 1 || segRow[0] != 0 || segRow[K] != n_) {
+        err = "batch layout does not cover the stacked system";
+        return ASTG_ERR_ARG;
+    }
+    const int nch = n_ / BS;
+    std::vector<int> segChunk(K + 1), chunkSeg(nch);
+    for (int k = 0; k <= K; ++k) {
+        if (segRow[k] % BS || (k && segRow[k] <= segRow[k - 1])) {
+            err = "batch layout: system starts must be increasing multiples of 256";
+            return ASTG_ERR_ARG;
+        }
+        segChunk[k] = segRow[k] / BS;
+    }
+    for (int k = 0; k < K; ++k)
+        for (int c = segChunk[k]; c < segChunk[k + 1]; ++c) chunkSeg[c] = k;
+    HIPCHK(chunkSeg_.alloc((size_t)nch * 4));
+    HIPCHK(segChunk_.alloc((size_t)(K + 1) * 4));
+    HIPCHK(bst_.alloc((size_t)K * B_COUNT * 8));
+    HIPCHK(bpart_.alloc((size_t)nch * 8));
+    HIPCHK(bcnt_.alloc(64));
+    HIPCHK(hipMemcpyAsync(chunkSeg_.p, chunkSeg.data(), (size_t)nch * 4, hipMemcpyHostToDevice, stream_));
+    HIPCHK(hipMemcpyAsync(segChunk_.p, segChunk.data(), (size_t)(K + 1) * 4, hipMemcpyHostToDevice, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    K_ = K;
+    return ASTG_OK;
+}
+
+BatchView PcgDevice::bview() const {
+    BatchView v;
+    v.chunkSeg = chunkSeg_.as<int>();
+    v.segChunk = segChunk_.as<int>();
+    v.st = bst_.as<double>();
+    v.part = bpart_.as<double>();
+    v.ndone = bcnt_.as<int>();
+    v.K = K_;
+    v.nchunks = n_ / BS;
+    return v;
+}
+
+int PcgDevice::batchState(std::vector<double>& st, bool toDevice, std::string& err) {
+    const size_t bytes = (size_t)K_ * B_COUNT * 8;
+    if (toDevice) {
+        HIPCHK(hipMemcpyAsync(bst_.p, st.data(), bytes, hipMemcpyHostToDevice, stream_));
+    } else {
+        st.resize((size_t)K_ * B_COUNT);
+        HIPCHK(hipMemcpyAsync(st.data(), bst_.p, bytes, hipMemcpyDeviceToHost, stream_));
+    }
+    HIPCHK(hipStreamSynchronize(stream_));
+    return ASTG_OK;
+}
+
+template <typename T>
+int PcgDevice::pcgBatchInner(const T* A, const T* invd, const T* b, double* x, int maxIt, int norm, std::string& err,
+                             const double* b64, int replaceEvery) {
+    const BatchView V = bview();
+    const int g = std::max(1, std::min(gs_, V.nchunks));
+    T* r = r_.as<T>();
+    T* z = z_.as<T>();
+    T* s = s_.as<T>();
+    T* q = q_.as<T>();
+    const int* rs = rs_.as<int>();
+    const int* col = col_.as<int>();
+    double* scal = scal_.as<double>();
+    int rc;
+    kb_zero64<<<1, BS, 0, stream_>>>(32, scal);
+    kb_zero_count<<<1, 64, 0, stream_>>>(V.ndone);
+    kb_init<T, double><<<g, BS, 0, stream_>>>(V, b, x, r, norm);
+    kb_reduce<<<V.K, BS, 0, stream_>>>(V, P_INIT, norm, scal);
+    kb_fill_sent<T><<<nblocks(2 * (long)PS_), BS, 0, stream_>>>(y_.as<T>(), 2 * (long)PS_);
+    if ((rc = launchFwd<T>(invd, r, 0, 2, 0, maxIt, norm, 0.0, err))) return rc;
+    if ((rc = launchBwd<T>(invd, r, z, 0, 0, err))) return rc;
+    kb_dot<T><<<g, BS, 0, stream_>>>(V, z, r);
+    kb_reduce<<<V.K, BS, 0, stream_>>>(V, P_RHO0, norm, scal);
+    kb_xpby<T><<<g, BS, 0, stream_>>>(V, z, s, 0);
+    HIPCHK(hipGetLastError());
+    int it = 0, batch = 4;
+    for (;;) {
+        for (int j = 0; j < batch && it < maxIt; ++j, ++it) {
+            kb_spmv_dot<T><<<g, BS, 0, stream_>>>(V, rs, col, A, s, q);
+            kb_reduce<<<V.K, BS, 0, stream_>>>(V, P_SQ, norm, scal);
+            kb_axpy_res<T, double><<<g, BS, 0, stream_>>>(V, s, q, x, r, norm);
+            if (sizeof(T) == 4 && replaceEvery > 0 && (it + 1) % replaceEvery == 0)
+                kb_replace_res<<<g, BS, 0, stream_>>>(V, rs, col, val_.as<double>(), x, b64,
+                                                      reinterpret_cast<float*>(r), norm);
+            kb_reduce<<<V.K, BS, 0, stream_>>>(V, P_RES, norm, scal);
+            if ((rc = launchFwd<T>(invd, r, it + 1, 2, it, maxIt, norm, 0.0, err))) return rc;
+            if ((rc = launchBwd<T>(invd, r, z, it + 1, (it + 1) & 1, err))) return rc;
+            kb_dot<T><<<g, BS, 0, stream_>>>(V, z, r);
+            kb_reduce<<<V.K, BS, 0, stream_>>>(V, P_RHO, norm, scal);
+            kb_xpby<T><<<g, BS, 0, stream_>>>(V, z, s, 1);
+        }
+        HIPCHK(hipGetLastError());
+        if ((rc = syncScalars(err))) return rc;
+        if ((rc = checkErrors("batched PCG", err))) return rc;
+        if (hscal_[S_DONE] != 0.0 || it >= maxIt) break;
+        batch = std::min(batch * 2, 32);
+    }
+    return ASTG_OK;
+}
+
+int PcgDevice::solveBatch(const astg_pcg_params& prm, std::vector<astg_pcg_result>& res, std::string& err) {
+    const int K = K_;
+    if (K < 1) {
+        err = "no batch layout";
+        return ASTG_ERR_ARG;
+    }
+    const BatchView V = bview();
+    const int g = std::max(1, std::min(gs_, V.nchunks));
+    const int maxIt = std::max(0, prm.max_iterations);
+    res.assign(K, astg_pcg_result());
+    for (auto& r : res) {
+        std::memset(&r, 0, sizeof(r));
+        r.depth = depth_;
+    }
+    std::vector<double> st((size_t)K * B_COUNT, 0.0);
+    auto S = [&](int k, int slot) -> double& { return st[(size_t)k * B_COUNT + slot]; };
+    if (prm.precision == ASTG_PREC_FP64) {
+        for (int k = 0; k < K; ++k) {
+            S(k, B_TOLREL) = prm.tol;
+            S(k, B_MAXIT) = maxIt;
+        }
+        int rc = batchState(st, true, err);
+        if (rc) return rc;
+        rc = pcgBatchInner<double>(val_.as<double>(), invd_.as<double>(), b_.as<double>(), x_.as<double>(), maxIt,
+                                   prm.norm, err, nullptr, 0);
+        if (rc) return rc;
+        if ((rc = batchState(st, false, err))) return rc;
+        for (int k = 0; k < K; ++k) {
+            astg_pcg_result& R = res[k];
+            const double done = S(k, B_DONE), bn = S(k, B_BNORM);
+            R.iterations = done == 2.0 ? 0 : (int)S(k, B_ITER);
+            R.residual = S(k, B_RES);
+            R.rel_residual = bn > 0 ? R.residual / bn : 0.0;
+            R.converged = (done == 1.0 || bn == 0.0) ? 1 : 0;
+        }
+        return ASTG_OK;
+    }
+    // Mixed form, per system as PcgDevice::solve: FP64 outer residual, FP32
+    // inner PCG on r / ||r|| with FP64 accumulation and reliable updates,
+    // correction x += ||r|| d; systems drop out as they converge.
+    const int n = n_;
+    HIPCHK(r64_.alloc((size_t)n * 8));
+    HIPCHK(d64_.alloc((size_t)n * 8));
+    HIPCHK(rhs64_.alloc((size_t)n * 12));
+    if (int rc = convert32(err)) return rc;
+    kb_zero64<<<nblocks(n), BS, 0, stream_>>>(n, x_.as<double>());
+    double* rhs64 = rhs64_.as<double>();
+    float* rhs32 = reinterpret_cast<float*>(rhs64 + n);
+    double* d64 = d64_.as<double>();
+    const int maxOuter = prm.max_outer > 0 ? prm.max_outer : 20;
+    std::vector<double> bnorm(K, -1.0), target(K, 0.0), rnorm(K, 0.0);
+    std::vector<int> total(K, 0), outer(K, 0);
+    std::vector<char> conv(K, 0), stopped(K, 0);
+    for (;;) {
+        kb_residual64<<<g, BS, 0, stream_>>>(V, rs_.as<int>(), col_.as<int>(), val_.as<double>(), x_.as<double>(),
+                                             b_.as<double>(), r64_.as<double>(), prm.norm);
+        kb_reduce<<<K, BS, 0, stream_>>>(V, P_OUTER, prm.norm, scal_.as<double>());
+        HIPCHK(hipGetLastError());
+        int rc = batchState(st, false, err);
+        if (rc) return rc;
+        int active = 0, innerMax = 0;
+        for (int k = 0; k < K; ++k) {
+            const double rn = S(k, B_OUTER);
+            if (!stopped[k]) rnorm[k] = rn;
+            if (bnorm[k] < 0) {
+                bnorm[k] = rn;
+                target[k] = prm.tol * rn;
+            }
+            if (!stopped[k]) {
+                if (rn <= target[k]) {
+                    conv[k] = 1;
+                    stopped[k] = 1;
+                } else if (outer[k] >= maxOuter || total[k] >= maxIt) {
+                    stopped[k] = 1;
+                }
+            }
+            std::fill(st.begin() + (size_t)k * B_COUNT, st.begin() + (size_t)(k + 1) * B_COUNT, 0.0);
+            S(k, B_OUTER) = rn;
+            if (stopped[k]) {
+                S(k, B_SKIP) = 1.0;
+                continue;
+            }
+            ++active;
+            S(k, B_TOLREL) = std::max(mixedEta_ * target[k] / rn, 1e-7);
+            S(k, B_MAXIT) = maxIt - total[k];
+            S(k, B_SCALE) = 1.0 / rn;
+            innerMax = std::max(innerMax, maxIt - total[k]);
+        }
+        if (!active) break;
+        if ((rc = batchState(st, true, err))) return rc;
+        kb_scale_rhs<<<g, BS, 0, stream_>>>(V, r64_.as<double>(), rhs64, rhs32);
+        rc = pcgBatchInner<float>(val32_.as<float>(), invd32_.as<float>(), rhs32, d64, innerMax, prm.norm, err, rhs64,
+                                  mixedReplace_);
+        if (rc) return rc;
+        kb_add_corr<<<g, BS, 0, stream_>>>(V, x_.as<double>(), d64);
+        HIPCHK(hipGetLastError());
+        if ((rc = batchState(st, false, err))) return rc;
+        for (int k = 0; k < K; ++k) {
+            if (S(k, B_SKIP) != 0.0) continue;
+            const int it = S(k, B_DONE) == 2.0 ? 0 : (int)S(k, B_ITER);
+            total[k] += it;
+            ++outer[k];
+            if (it == 0) stopped[k] = 1;
+        }
+    }
+    for (int k = 0; k < K; ++k) {
+        astg_pcg_result& R = res[k];
+        R.iterations = total[k];
+        R.outer = outer[k];
+        R.residual = rnorm[k];
+        R.rel_residual = bnorm[k] > 0 ? rnorm[k] / bnorm[k] : 0.0;
+        R.converged = conv[k];
+    }
+    return ASTG_OK;
+}
+
+template int PcgDevice::pcgBatchInner<double>(const double*, const double*, const double*, double*, int, int,
+                                              std::string&, const double*, int);
+template int PcgDevice::pcgBatchInner<float>(const float*, const float*, const float*, double*, int, int,
+                                             std::string&, const double*, int);
+
+}  // namespace astg

@@ -115,6 +115,55 @@ def smoke_timing(steps=50):
             "note": "64^3, 3 levels; source + addForce + constrain/project/post chain/adjustE/advect per step"}
 
 
+def batched_timing(seed=0, count=24):
+    """Batched solves: `count` independent octree systems (leaf counts
+    log-uniform in [5e4, 2e6]) solved by one astg_pcg_solve_batch call, FP64
+    and mixed, against the same systems solved one after another (each its own
+    factor + solve through the single-system path). Device times (factor +
+    solve), second call of each (the first builds the stacked pattern)."""
+    import numpy as np
+    import astg
+    from astg import scenes
+    prm = {p: astg.default_params(tol=1e-6, max_iterations=3000, precision=p)
+           for p in (astg.PREC_FP64, astg.PREC_MIXED)}
+    t0 = time.perf_counter()
+    systems, leaves, seq, single_its = [], [], {p: 0.0 for p in prm}, {p: [] for p in prm}
+    for k, target in enumerate(scenes.batch_targets(seed, count)):
+        ctx, info = scenes.batch_scene(seed, k, target)
+        ctx.snapshot(restore=False)
+        for p in prm:
+            ctx.snapshot(restore=True)
+            ctx.project(info["dt"], info["rho"], prm[p])  # first call: pattern and buffers
+            ctx.snapshot(restore=True)
+            r = ctx.project(info["dt"], info["rho"], prm[p])
+            seq[p] += r.ms_factor + r.ms_solve
+            single_its[p].append(r.iterations)
+        rs, col, val, b, _ = ctx.system(r.n, r.nnz)
+        systems.append((rs, col, val, b))
+        leaves.append(info["leaves"])
+        ctx.close()
+    build_s = time.perf_counter() - t0
+    c = astg.Context(0)
+    out = {"systems": count, "seed": seed, "leaves": leaves, "leaves_total": int(sum(leaves)),
+           "n_total": int(sum(len(s[3]) for s in systems)), "nnz_total": int(sum(len(s[1]) for s in systems)),
+           "scene_build_s": round(build_s, 1)}
+    for p, name in ((astg.PREC_FP64, "fp64"), (astg.PREC_MIXED, "mixed")):
+        t1 = time.perf_counter()
+        c.pcg_solve_batch(systems, prm[p])
+        first_s = time.perf_counter() - t1
+        xs, res = c.pcg_solve_batch(systems, prm[p])
+        ms = res[0].ms_factor + res[0].ms_solve
+        its = [r.iterations for r in res]
+        out[name] = {"ms_batch": round(ms, 3), "ms_sequential": round(seq[p], 3),
+                     "speedup_vs_sequential": round(seq[p] / ms, 3), "iterations": its,
+                     "single_iterations": single_its[p],
+                     "max_iteration_diff": int(max(abs(a - b) for a, b in zip(its, single_its[p]))),
+                     "all_converged": bool(all(r.converged for r in res)), "depth": res[0].depth,
+                     "first_call_s": round(first_s, 2)}
+    c.close()
+    return out
+
+
 def cpu_baseline(ctx, res64, budget_iters=150):
     """Reference MIC(0)-PCG (oracle/_ref, the reference's own code) or the
     oracle's port on the same C2 system: factor + a bounded number of PCG
@@ -158,6 +207,8 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=5)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-solve workload")
+    ap.add_argument("--batch-only", action="store_true", help="run only the batched-solve workload")
     ap.add_argument("--kernels-only", action="store_true",
                     help="one projection, then per-kernel timings only (for counter collection)")
     a = ap.parse_args()
@@ -171,6 +222,10 @@ def main():
     from astg import scenes
 
     assert torch.cuda.is_available(), "no GPU visible"
+    if a.batch_only:
+        bt = batched_timing()
+        log("batched: %s" % json.dumps(bt))
+        return
     t0 = time.perf_counter()
     ctx, info = scenes.projection_256(device=0, seed=a.seed)
     prm64 = astg.default_params(tol=1e-6, max_iterations=3000, precision=astg.PREC_FP64)
@@ -219,6 +274,13 @@ def main():
     log("smoke_64: %.2f ms/step over %d steps (mean PCG iterations %.1f)" % (
         smoke["ms_per_step"], smoke["steps"], smoke["pcg_iterations_mean"]))
 
+    batched = None
+    if not a.no_batch:
+        batched = batched_timing()
+        log("batched (%d systems, %d leaves): mixed %.1f ms vs %.1f ms one after another; fp64 %.1f vs %.1f ms" % (
+            batched["systems"], batched["leaves_total"], batched["mixed"]["ms_batch"],
+            batched["mixed"]["ms_sequential"], batched["fp64"]["ms_batch"], batched["fp64"]["ms_sequential"]))
+
     cpu = None
     if not a.no_cpu:
         try:
@@ -255,6 +317,7 @@ def main():
         "roofline": roofline,
         "advect": adv,
         "smoke_64": smoke,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -92,6 +92,21 @@ const char* astg_last_error(const astg_ctx* ctx);
 int astg_pcg_solve_csr(astg_ctx* ctx, int n, const int* rowstart, const int* colindex, const double* value,
                        const double* b, double* x, const astg_pcg_params* params, astg_pcg_result* result);
 
+/* Batched form: `count` independent systems, each given as to
+ * astg_pcg_solve_csr (system k: n[k] rows, rowstart[k], colindex[k],
+ * value[k], b[k], output x[k]), solved together. The systems are stacked
+ * block-diagonally and factored in one pass; every system keeps its own PCG
+ * scalars, convergence test and iteration count, so each follows the
+ * recurrence astg_pcg_solve_csr would run on it alone (up to the summation
+ * order of its dots). results[k] (may be null) per system; its device times
+ * are those of the whole batch. Replaces a loop of CudaSolver::solve calls,
+ * one per scene (octree_omnigrid_solver3.cpp:666-669); the reference has no
+ * batched solver. Returns ASTG_ERR_NOT_CONVERGED if any system missed the
+ * tolerance (every x[k] still holds its last iterate). */
+int astg_pcg_solve_batch(astg_ctx* ctx, int count, const int* n, const int* const* rowstart,
+                         const int* const* colindex, const double* const* value, const double* const* b,
+                         double* const* x, const astg_pcg_params* params, astg_pcg_result* results);
+
 /* ---- scene helpers ---------------------------------------------------- */
 /* Split map (R^3 ints, x fastest, value = leaf level) refined toward a union
  * of spheres (n x {cx,cy,cz,r}); band[l] = refinement distance for level l+1 -> l;
