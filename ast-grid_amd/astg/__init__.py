@@ -27,7 +27,8 @@ NORM_INF, NORM_L2 = 0, 1
 # Header symbols, in the order include/astg.h declares them.
 EXPORTS = [
     "astg_pcg_params_default", "astg_create", "astg_destroy", "astg_last_error",
-    "astg_pcg_solve_csr", "astg_pcg_solve_batch", "astg_split_spheres", "astg_set_topology_split", "astg_set_tilt_open",
+    "astg_pcg_solve_csr", "astg_pcg_solve_batch", "astg_batch_tuning_default", "astg_pcg_set_batch_tuning",
+    "astg_pcg_tune_batch", "astg_split_spheres", "astg_set_topology_split", "astg_set_tilt_open",
     "astg_level_info", "astg_get_nodes", "astg_get_tilt", "astg_get_half_tilts", "astg_get_numbering",
     "astg_get_weights", "astg_set_velocity", "astg_get_velocity", "astg_project", "astg_get_system",
     "astg_get_mic_factor", "astg_fields_snapshot", "astg_kernel_time", "astg_sweep_timeline",
@@ -44,6 +45,14 @@ class PcgParams(ctypes.Structure):
     _fields_ = [("tol", ctypes.c_double), ("max_iterations", ctypes.c_int), ("norm", ctypes.c_int),
                 ("mic_tau", ctypes.c_double), ("min_diag_ratio", ctypes.c_double),
                 ("precision", ctypes.c_int), ("max_outer", ctypes.c_int)]
+
+
+class BatchTuning(ctypes.Structure):
+    _fields_ = [("merge_levels", ctypes.c_int), ("stream_blocks_per_cu", ctypes.c_int),
+                ("lanes_per_row", ctypes.c_int), ("ms_per_iteration", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class PcgResult(ctypes.Structure):
@@ -79,6 +88,10 @@ def lib():
         "astg_last_error": (ctypes.c_char_p, [P]),
         "astg_pcg_solve_csr": (I, [P, I, P, P, P, P, P, ctypes.POINTER(PcgParams), ctypes.POINTER(PcgResult)]),
         "astg_pcg_solve_batch": (I, [P, I, P, P, P, P, P, P, ctypes.POINTER(PcgParams), ctypes.POINTER(PcgResult)]),
+        "astg_batch_tuning_default": (None, [ctypes.POINTER(BatchTuning)]),
+        "astg_pcg_set_batch_tuning": (I, [P, ctypes.POINTER(BatchTuning)]),
+        "astg_pcg_tune_batch": (I, [P, I, P, P, P, P, P, ctypes.POINTER(PcgParams), I,
+                                    ctypes.POINTER(BatchTuning)]),
         "astg_split_spheres": (I, [I, I, I, P, P, P]),
         "astg_set_topology_split": (I, [P, I, D, I, P, I]),
         "astg_set_tilt_open": (I, [P, I, P]),
@@ -368,21 +381,38 @@ class Context:
                   allow=(ASTG_ERR_NOT_CONVERGED,) if allow_unconverged else ())
         return x, res
 
-    def pcg_solve_batch(self, systems, params=None, allow_unconverged=False):
-        """Batched form of pcg_solve_csr: systems = [(rowstart, col, val, b), ...].
-        Returns ([x_k], [PcgResult_k])."""
+    @staticmethod
+    def _batch_args(systems):
         K = len(systems)
         keep = []
         for rs, col, val, b in systems:
             keep.append((np.ascontiguousarray(rs, np.int32), np.ascontiguousarray(col, np.int32),
                          np.ascontiguousarray(val, np.float64), np.ascontiguousarray(b, np.float64)))
         n = np.array([len(s[0]) - 1 for s in keep], np.int32)
-        xs = [np.empty(int(nk), np.float64) for nk in n]
         arr = lambda items: (ctypes.c_void_p * K)(*[a.ctypes.data for a in items])
+        cols = [arr([s[j] for s in keep]) for j in range(4)]
+        return K, keep, n, arr, cols
+
+    def pcg_solve_batch(self, systems, params=None, allow_unconverged=False):
+        """Batched form of pcg_solve_csr: systems = [(rowstart, col, val, b), ...].
+        Returns ([x_k], [PcgResult_k])."""
+        K, keep, n, arr, cols = self._batch_args(systems)
+        xs = [np.empty(int(nk), np.float64) for nk in n]
         res = (PcgResult * K)()
         prm = params if params is not None else default_params()
-        self._chk(self._L.astg_pcg_solve_batch(self.h, K, _ptr(n), arr([s[0] for s in keep]), arr([s[1] for s in keep]),
-                                               arr([s[2] for s in keep]), arr([s[3] for s in keep]), arr(xs),
-                                               ctypes.byref(prm), res), "pcg_solve_batch",
-                  allow=(ASTG_ERR_NOT_CONVERGED,) if allow_unconverged else ())
+        self._chk(self._L.astg_pcg_solve_batch(self.h, K, _ptr(n), *cols, arr(xs), ctypes.byref(prm), res),
+                  "pcg_solve_batch", allow=(ASTG_ERR_NOT_CONVERGED,) if allow_unconverged else ())
         return xs, list(res)
+
+    def tune_batch(self, systems, params=None, iterations=20):
+        """Autotuner over the batched launch shape; sets and returns the best."""
+        K, keep, n, arr, cols = self._batch_args(systems)
+        best = BatchTuning()
+        prm = params if params is not None else default_params()
+        self._chk(self._L.astg_pcg_tune_batch(self.h, K, _ptr(n), *cols, ctypes.byref(prm), iterations,
+                                              ctypes.byref(best)), "tune_batch")
+        return best
+
+    def set_batch_tuning(self, merge_levels=2, stream_blocks_per_cu=8, lanes_per_row=8):
+        t = BatchTuning(merge_levels, stream_blocks_per_cu, lanes_per_row, 0.0)
+        self._chk(self._L.astg_pcg_set_batch_tuning(self.h, ctypes.byref(t)), "set_batch_tuning")

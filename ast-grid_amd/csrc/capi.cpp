@@ -37,6 +37,8 @@ struct astg_ctx {
     std::vector<int> csrRs, csrCol;
     std::unique_ptr<astg::PcgDevice> batch;               // batched CSR solves (stacked systems)
     std::vector<int> batchRs, batchCol, batchSeg;
+    int batchMerge = 0;                                   // levels per sweep group of the cached pattern
+    astg_batch_tuning batchTune = {2, 8, 8, 0.0};         // launch shape of batched solves
     astg::PcgDevice* lastPcg = nullptr;
     hipEvent_t ev[6] = {};
     bool tiltAdaptivity = false;                          // enableTiltAdaptivity
@@ -549,111 +551,230 @@ int astg_pcg_solve_csr(astg_ctx* c, int n, const int* rowstart, const int* colin
     });
 }
 
+namespace {
+
+// Stacked block-diagonal form of a batch: system k owns rows [seg[k], seg[k+1]),
+// padded with identity rows (zero right-hand side) to a multiple of 256.
+struct StackedBatch {
+    std::vector<int> seg, rs, col;
+    std::vector<double> val, b;
+    int N = 0;
+    long long nnz = 0;
+};
+
+int stackBatch(astg_ctx* c, int count, const int* n, const int* const* rowstart, const int* const* colindex,
+               const double* const* value, const double* const* b, StackedBatch& S) {
+    constexpr int kChunk = 256;
+    S.seg.assign(count + 1, 0);
+    long long rows = 0, nnzAll = 0;
+    for (int k = 0; k < count; ++k) {
+        if (n[k] <= 0 || !rowstart[k] || !colindex[k] || !value[k] || !b[k])
+            return fail(c, ASTG_ERR_ARG, "system " + std::to_string(k) + ": null or empty input");
+        const int* rs = rowstart[k];
+        if (rs[0] != 0) return fail(c, ASTG_ERR_ARG, "system " + std::to_string(k) + ": bad rowstart");
+        for (int i = 0; i < n[k]; ++i)
+            if (rs[i + 1] < rs[i])
+                return fail(c, ASTG_ERR_ARG, "system " + std::to_string(k) + ": rowstart not monotone");
+        const long long padded = ((long long)n[k] + kChunk - 1) / kChunk * kChunk;
+        nnzAll += rs[n[k]] + (padded - n[k]);
+        rows += padded;
+        if (rows >= (1LL << 31) - kChunk || nnzAll >= (1LL << 31))
+            return fail(c, ASTG_ERR_ARG, "batch exceeds 32-bit row or entry indices");
+        S.seg[k + 1] = (int)rows;
+    }
+    S.N = (int)rows;
+    S.nnz = nnzAll;
+    S.rs.assign((size_t)S.N + 1, 0);
+    S.col.assign((size_t)nnzAll, 0);
+    S.val.assign((size_t)nnzAll, 0.0);
+    S.b.assign((size_t)S.N, 0.0);
+    long long p = 0;
+    for (int k = 0; k < count; ++k) {
+        const int* rs = rowstart[k];
+        const int* cl = colindex[k];
+        const int r0 = S.seg[k], nk = n[k];
+        for (int i = 0; i < nk; ++i) {
+            S.rs[(size_t)r0 + i] = (int)p;
+            for (int q = rs[i]; q < rs[i + 1]; ++q, ++p) {
+                const int j = cl[q];
+                if (j < 0 || j >= nk)
+                    return fail(c, ASTG_ERR_ARG, "system " + std::to_string(k) + ": column index out of range");
+                S.col[(size_t)p] = r0 + j;
+                S.val[(size_t)p] = value[k][q];
+            }
+        }
+        std::memcpy(S.b.data() + r0, b[k], (size_t)nk * 8);
+        for (int i = r0 + nk; i < S.seg[k + 1]; ++i, ++p) {  // identity padding rows
+            S.rs[(size_t)i] = (int)p;
+            S.col[(size_t)p] = i;
+            S.val[(size_t)p] = 1.0;
+        }
+    }
+    S.rs[(size_t)S.N] = (int)p;
+    return ASTG_OK;
+}
+
+// Pattern-only set-up of the stacked system with `merge` levels per sweep
+// group, skipped when the cached pattern is the same.
+int batchPattern(astg_ctx* c, const StackedBatch& S, int merge, bool cache) {
+    if (!c->batch) c->batch.reset(new astg::PcgDevice(c->stream));
+    if (c->batchMerge == merge && c->batchSeg == S.seg && c->batchRs == S.rs && c->batchCol == S.col)
+        return ASTG_OK;
+    c->batchRs.clear();
+    c->batchCol.clear();
+    c->batchSeg.clear();
+    c->batchMerge = 0;
+    c->batch->setMergeLevels(merge);
+    int r = c->batch->setPattern(S.N, S.rs.data(), S.col.data(), c->err);
+    if (r) return r;
+    if ((r = c->batch->setBatch(S.seg, c->err))) return r;
+    if (cache) {
+        c->batchRs = S.rs;
+        c->batchCol = S.col;
+        c->batchSeg = S.seg;
+        c->batchMerge = merge;
+    }
+    return ASTG_OK;
+}
+
+// Uploads values and right-hand side, factors and solves; device times in ms.
+int batchRun(astg_ctx* c, const StackedBatch& S, const astg_pcg_params& prm, std::vector<astg_pcg_result>& res,
+             float& msFactor, float& msSolve) {
+    hipEvent_t* ev = c->ev;
+    if (hipMemcpyAsync(c->batch->values(), S.val.data(), (size_t)S.nnz * 8, hipMemcpyHostToDevice, c->stream) !=
+            hipSuccess ||
+        hipMemcpyAsync(c->batch->rhs(), S.b.data(), (size_t)S.N * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        return fail(c, ASTG_ERR_HIP, "upload failed");
+    (void)hipEventRecord(ev[0], c->stream);
+    int r = c->batch->factor(prm.mic_tau, prm.min_diag_ratio, c->err);
+    if (r) return r;
+    (void)hipEventRecord(ev[1], c->stream);
+    r = c->batch->solveBatch(prm, res, c->err);
+    if (r) return r;
+    (void)hipEventRecord(ev[2], c->stream);
+    if (hipEventSynchronize(ev[2]) != hipSuccess) return fail(c, ASTG_ERR_HIP, "batch solve failed");
+    (void)hipEventElapsedTime(&msFactor, ev[0], ev[1]);
+    (void)hipEventElapsedTime(&msSolve, ev[1], ev[2]);
+    return ASTG_OK;
+}
+
+bool validTuning(const astg_batch_tuning& t) {
+    return t.merge_levels >= 1 && t.merge_levels <= 8 && t.stream_blocks_per_cu >= 0 &&
+           t.stream_blocks_per_cu <= 64 && (t.lanes_per_row == 4 || t.lanes_per_row == 8 || t.lanes_per_row == 16);
+}
+
+}  // namespace
+
 int astg_pcg_solve_batch(astg_ctx* c, int count, const int* n, const int* const* rowstart,
                          const int* const* colindex, const double* const* value, const double* const* b,
                          double* const* x, const astg_pcg_params* params, astg_pcg_result* results) {
     if (!c) return ASTG_ERR_ARG;
     if (count <= 0 || !n || !rowstart || !colindex || !value || !b || !x)
         return fail(c, ASTG_ERR_ARG, "null or empty batch");
+    for (int k = 0; k < count; ++k)
+        if (!x[k]) return fail(c, ASTG_ERR_ARG, "system " + std::to_string(k) + ": null output");
     int rc = needDevice(c);
     if (rc) return rc;
     astg_pcg_params prm;
     astg_pcg_params_default(&prm);
     if (params) prm = *params;
     return guarded(c, [&] {
-        constexpr int kChunk = 256;
-        // stacked layout: system k owns rows [seg[k], seg[k+1]), padded to 256
-        std::vector<int> seg(count + 1, 0);
-        long long rows = 0, nnzAll = 0;
-        for (int k = 0; k < count; ++k) {
-            if (n[k] <= 0 || !rowstart[k] || !colindex[k] || !value[k] || !b[k] || !x[k])
-                return fail(c, ASTG_ERR_ARG, "system " + std::to_string(k) + ": null or empty input");
-            const int* rs = rowstart[k];
-            if (rs[0] != 0) return fail(c, ASTG_ERR_ARG, "system " + std::to_string(k) + ": bad rowstart");
-            for (int i = 0; i < n[k]; ++i)
-                if (rs[i + 1] < rs[i]) return fail(c, ASTG_ERR_ARG, "system " + std::to_string(k) + ": rowstart not monotone");
-            const long long padded = ((long long)n[k] + kChunk - 1) / kChunk * kChunk;
-            nnzAll += rs[n[k]] + (padded - n[k]);
-            rows += padded;
-            if (rows >= (1LL << 31) - kChunk || nnzAll >= (1LL << 31))
-                return fail(c, ASTG_ERR_ARG, "batch exceeds 32-bit row or entry indices");
-            seg[k + 1] = (int)rows;
-        }
-        const int N = (int)rows;
-        std::vector<int> srs((size_t)N + 1), scol((size_t)nnzAll);
-        std::vector<double> sval((size_t)nnzAll), sb((size_t)N, 0.0);
-        long long p = 0;
-        for (int k = 0; k < count; ++k) {
-            const int* rs = rowstart[k];
-            const int* cl = colindex[k];
-            const int r0 = seg[k], nk = n[k];
-            for (int i = 0; i < nk; ++i) {
-                srs[(size_t)r0 + i] = (int)p;
-                for (int q = rs[i]; q < rs[i + 1]; ++q, ++p) {
-                    const int j = cl[q];
-                    if (j < 0 || j >= nk)
-                        return fail(c, ASTG_ERR_ARG, "system " + std::to_string(k) + ": column index out of range");
-                    scol[(size_t)p] = r0 + j;
-                    sval[(size_t)p] = value[k][q];
-                }
-            }
-            std::memcpy(sb.data() + r0, b[k], (size_t)nk * 8);
-            for (int i = r0 + nk; i < seg[k + 1]; ++i, ++p) {  // identity padding rows
-                srs[(size_t)i] = (int)p;
-                scol[(size_t)p] = i;
-                sval[(size_t)p] = 1.0;
-            }
-        }
-        srs[(size_t)N] = (int)p;
-        if (!c->batch) c->batch.reset(new astg::PcgDevice(c->stream));
-        const bool same = c->batchSeg == seg && c->batchRs == srs && c->batchCol == scol;
-        if (!same) {
-            c->batchRs.clear();
-            c->batchCol.clear();
-            c->batchSeg.clear();
-            int r = c->batch->setPattern(N, srs.data(), scol.data(), c->err);
-            if (r) return r;
-            if ((r = c->batch->setBatch(seg, c->err))) return r;
-            c->batchRs.swap(srs);
-            c->batchCol.swap(scol);
-            c->batchSeg = seg;
-        }
-        hipEvent_t* ev = c->ev;
-        if (hipMemcpyAsync(c->batch->values(), sval.data(), (size_t)nnzAll * 8, hipMemcpyHostToDevice, c->stream) !=
-                hipSuccess ||
-            hipMemcpyAsync(c->batch->rhs(), sb.data(), (size_t)N * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess)
-            return fail(c, ASTG_ERR_HIP, "upload failed");
-        (void)hipEventRecord(ev[0], c->stream);
-        int r = c->batch->factor(prm.mic_tau, prm.min_diag_ratio, c->err);
+        StackedBatch S;
+        int r = stackBatch(c, count, n, rowstart, colindex, value, b, S);
         if (r) return r;
-        (void)hipEventRecord(ev[1], c->stream);
+        if ((r = batchPattern(c, S, c->batchTune.merge_levels, true))) return r;
+        c->batch->setBatchTuning(c->batchTune.stream_blocks_per_cu, c->batchTune.lanes_per_row);
         std::vector<astg_pcg_result> res;
-        r = c->batch->solveBatch(prm, res, c->err);
-        if (r) return r;
-        (void)hipEventRecord(ev[2], c->stream);
+        float msF = 0, msS = 0;
+        if ((r = batchRun(c, S, prm, res, msF, msS))) return r;
         const double* xd = c->batch->solution();
         for (int k = 0; k < count; ++k)
-            if (hipMemcpyAsync(x[k], xd + seg[k], (size_t)n[k] * 8, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+            if (hipMemcpyAsync(x[k], xd + S.seg[k], (size_t)n[k] * 8, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
                 return fail(c, ASTG_ERR_HIP, "download failed");
         if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(c, ASTG_ERR_HIP, "download failed");
-        float ms0 = 0, ms1 = 0, ms2 = 0;
-        (void)hipEventElapsedTime(&ms0, ev[0], ev[1]);
-        (void)hipEventElapsedTime(&ms1, ev[1], ev[2]);
-        (void)hipEventElapsedTime(&ms2, ev[0], ev[2]);
         int unconverged = 0;
         for (int k = 0; k < count; ++k) {
             astg_pcg_result& R = res[k];
             R.n = n[k];
             R.nnz = rowstart[k][n[k]];
-            R.ms_factor = ms0;
-            R.ms_solve = ms1;
-            R.ms_total = ms2;
+            R.ms_factor = msF;
+            R.ms_solve = msS;
+            R.ms_total = msF + msS;
             if (!R.converged) ++unconverged;
             if (results) results[k] = R;
         }
         if (unconverged)
             return fail(c, ASTG_ERR_NOT_CONVERGED,
                         std::to_string(unconverged) + " system(s) did not reach the tolerance");
+        return ASTG_OK;
+    });
+}
+
+void astg_batch_tuning_default(astg_batch_tuning* t) {
+    if (!t) return;
+    t->merge_levels = 2;
+    t->stream_blocks_per_cu = 8;
+    t->lanes_per_row = 8;
+    t->ms_per_iteration = 0.0;
+}
+
+int astg_pcg_set_batch_tuning(astg_ctx* c, const astg_batch_tuning* t) {
+    if (!c) return ASTG_ERR_ARG;
+    if (!t || !validTuning(*t)) return fail(c, ASTG_ERR_ARG, "invalid batch tuning");
+    c->batchTune = *t;
+    return ASTG_OK;
+}
+
+int astg_pcg_tune_batch(astg_ctx* c, int count, const int* n, const int* const* rowstart, const int* const* colindex,
+                        const double* const* value, const double* const* b, const astg_pcg_params* params,
+                        int iterations, astg_batch_tuning* best) {
+    if (!c) return ASTG_ERR_ARG;
+    if (count <= 0 || !n || !rowstart || !colindex || !value || !b || !best)
+        return fail(c, ASTG_ERR_ARG, "null or empty batch");
+    if (iterations < 2 || iterations > 1000) return fail(c, ASTG_ERR_ARG, "iterations must be in [2, 1000]");
+    int rc = needDevice(c);
+    if (rc) return rc;
+    astg_pcg_params prm;
+    astg_pcg_params_default(&prm);
+    if (params) prm = *params;
+    // a fixed amount of work per configuration: no system converges
+    prm.tol = 0.0;
+    prm.max_iterations = iterations;
+    prm.max_outer = 1;
+    return guarded(c, [&] {
+        StackedBatch S;
+        int r = stackBatch(c, count, n, rowstart, colindex, value, b, S);
+        if (r) return r;
+        static const int kMerge[] = {1, 2, 3};
+        static const int kBlocks[] = {4, 8, 16};
+        static const int kLanes[] = {4, 8, 16};
+        astg_batch_tuning t;
+        astg_batch_tuning_default(&t);
+        double bestMs = 1e300;
+        for (int q : kMerge) {
+            if ((r = batchPattern(c, S, q, false))) return r;
+            for (int bl : kBlocks)
+                for (int ln : kLanes) {
+                    c->batch->setBatchTuning(bl, ln);
+                    std::vector<astg_pcg_result> res;
+                    float msF = 0, msS = 0;
+                    double ms = 1e300;
+                    for (int rep = 0; rep < 3; ++rep) {  // first run warms; best of the other two
+                        if ((r = batchRun(c, S, prm, res, msF, msS))) return r;
+                        if (rep) ms = std::min(ms, (double)msS);
+                    }
+                    if (ms < bestMs) {
+                        bestMs = ms;
+                        t.merge_levels = q;
+                        t.stream_blocks_per_cu = bl;
+                        t.lanes_per_row = ln;
+                    }
+                }
+        }
+        t.ms_per_iteration = bestMs / iterations;
+        c->batchTune = t;
+        if ((r = batchPattern(c, S, t.merge_levels, true))) return r;
+        *best = t;
         return ASTG_OK;
     });
 }

@@ -46,6 +46,10 @@ struct SweepPlan {
     int pollCap;       // largest sleep between polling rounds (units of 64 cycles)
     int gateCap;       // largest sleep between gate polls (units of 128 cycles)
     long long* trace;  // optional per-row timeline (3 per row), null in normal runs
+    // batched form (null for one system): the system of every 256-row chunk
+    // of the stacked rows, and the per-system state (BatchSlot layout)
+    const int* chunkSys;
+    const double* sysState;
 };
 
 // Merged sweep stencil of one direction, by sweep position (sweep_merge.h).
@@ -139,6 +143,12 @@ public:
     // system's own are identity rows with a zero right-hand side).
     int setBatch(const std::vector<int>& segRow, std::string& err);
     int batchCount() const { return K_; }
+    // Launch shape of the batched streaming kernels: blocks per compute unit
+    // (0: the single-system default) and SpMV lanes per row (4, 8 or 16).
+    void setBatchTuning(int blocksPerCu, int lanesPerRow);
+    // Levels per merged sweep group used by the next setPattern (0: default).
+    void setMergeLevels(int q) { mergeLevels_ = q; }
+    int mergeLevels() const { return merge_; }
     // Solves every system of the stack with its own PCG scalars, convergence
     // test and iteration count (rhs(), solution(), values() hold the stack).
     int solveBatch(const astg_pcg_params& prm, std::vector<astg_pcg_result>& res, std::string& err);
@@ -166,6 +176,7 @@ private:
                       const double* b64, int replaceEvery);
     int batchState(std::vector<double>& st, bool toDevice, std::string& err);
     BatchView bview() const;
+    int batchGrid() const;
     int launchMic(double tau, double ratio, std::string& err);
     int launchMerge(std::string& err);
     int convert32(std::string& err);
@@ -215,6 +226,9 @@ private:
     long long* trace_ = nullptr;            // set only while a timeline is recorded
     int K_ = 0;                             // batched form: systems in the stack (0: single system)
     DevBuf chunkSeg_, segChunk_, bst_, bpart_, bcnt_;  // batched form: layout, state, partials, counter
+    int blocksPerCu_ = 0;                   // batched form: streaming grid (0: gs_)
+    int lpr_ = 8;                           // batched form: SpMV lanes per row
+    int mergeLevels_ = 0;                   // requested levels per sweep group (0: default)
 };
 
 }  // namespace astg

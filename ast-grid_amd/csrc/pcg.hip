@@ -271,6 +271,11 @@ __device__ inline bool fwd_prologue(const Parts& Pt, double* scal, int mode, int
     return conv || it + 1 >= maxIt;
 }
 
+// Batched form: a row of a system that has finished takes no part in a sweep.
+__device__ inline bool row_finished(const SweepPlan& S, int i) {
+    return S.chunkSys && S.sysState[S.chunkSys[i / BS] * B_COUNT + B_DONE] != 0.0;
+}
+
 // Forward sweep y = L^-1 r over the merged stencil:
 //   y_i = d_i (r_i - sum_j R_ij r_j - sum_m W_im y_m)
 // (with no merging R is empty and W is row i of L, in the reference's order).
@@ -283,7 +288,9 @@ __global__ void __launch_bounds__(BS) k_fwd(SweepPlan S, StencilView<T> St, cons
     if (mode != 2 && fwd_prologue(Pt, scal, mode, it, maxIt, norm, tolRel)) return;
     for (int c = blockIdx.x; c < S.nchunks; c += gridDim.x) {
         const int t = c * BS + threadIdx.x;
-        const int i = S.order[t];
+        const int i0 = S.order[t];
+        const bool fin = i0 >= 0 && row_finished(S, i0);
+        const int i = fin ? -1 : i0;  // live row, or -1
         // row data that does not depend on other rows is loaded before the gate
         DepRow<T> d;
         T acc = 0, inv = 0;
@@ -309,6 +316,11 @@ __global__ void __launch_bounds__(BS) k_fwd(SweepPlan S, StencilView<T> St, cons
             if (!ok) atomicOr(&cnt[C_ERR], E_FWD);
             // armed for the backward sweep; after publishing, off the dependency path
             Sentinel<T>::storeSent(&zp[t]);
+        } else if (fin) {
+            // nobody reads a finished system's rows; published so that gates
+            // placed on them pass
+            Sentinel<T>::store(&y[t], (T)0);
+            Sentinel<T>::storeSent(&zp[t]);
         }
     }
 }
@@ -326,7 +338,9 @@ __global__ void __launch_bounds__(BS) k_bwd_dot(SweepPlan S, StencilView<T> St, 
     double v = 0.0;
     for (int c = blockIdx.x; c < S.nchunks; c += gridDim.x) {
         const int pos = S.P - 1 - (c * BS + threadIdx.x);
-        const int k = S.order[pos];
+        const int k0 = S.order[pos];
+        const bool fin = k0 >= 0 && row_finished(S, k0);
+        const int k = fin ? -1 : k0;  // live row, or -1
         DepRow<T> d;
         T acc = 0, inv = 0, rk = 0;
         if (k >= 0) {
@@ -357,6 +371,9 @@ __global__ void __launch_bounds__(BS) k_bwd_dot(SweepPlan S, StencilView<T> St, 
             rk = r[k];
             Sentinel<T>::storeSent(&ynext[pos]);
             v += (double)zk * (double)rk;
+        } else if (fin) {
+            Sentinel<T>::store(&zp[pos], (T)0);
+            Sentinel<T>::storeSent(&ynext[pos]);
         }
     }
     const double t = block_reduce<false>(v);
@@ -591,6 +608,7 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
     // sweep merging: levels per group (1 = plain level order)
     merge_ = 2;
     if (const char* g = getenv("ASTG_MERGE")) merge_ = std::max(1, std::min(8, atoi(g)));
+    if (mergeLevels_ > 0) merge_ = std::min(8, mergeLevels_);
     // mixed form: residual replacement period and inner tolerance factor
     if (const char* g = getenv("ASTG_MIXED_REPLACE")) mixedReplace_ = std::max(0, atoi(g));
     if (const char* g = getenv("ASTG_MIXED_ETA")) mixedEta_ = std::max(0.01, atof(g));
@@ -788,6 +806,8 @@ SweepPlan PcgDevice::splan() const {
     s.pollCap = pollCap_;
     s.gateCap = gateCap_;
     s.trace = trace_;
+    s.chunkSys = K_ ? chunkSeg_.as<int>() : nullptr;
+    s.sysState = K_ ? bst_.as<double>() : nullptr;
     return s;
 }
 

@@ -134,17 +134,18 @@ __global__ void __launch_bounds__(BS) kb_dot(BatchView V, const T* z, const T* r
     }
 }
 
-// q = A s and the partials of s.q, one chunk (256 rows) per block pass.
-template <typename T>
+// q = A s and the partials of s.q, one chunk (256 rows) per block pass, LPR
+// lanes per row.
+template <typename T, int LPR>
 __global__ void __launch_bounds__(BS) kb_spmv_dot(BatchView V, const int* rs, const int* col, const T* A, const T* s,
                                                   T* q) {
-    const int sub = threadIdx.x & (kLpr - 1);
+    const int sub = threadIdx.x & (LPR - 1);
     for (int c = blockIdx.x; c < V.nchunks; c += gridDim.x) {
         if (!chunk_live(V, c)) continue;
         double v = 0.0;
-        for (int j = 0; j < kLpr; ++j) {
-            const int i = c * BS + j * (BS / kLpr) + threadIdx.x / kLpr;
-            const T acc = row_dot(rs, col, A, s, i, sub);
+        for (int j = 0; j < LPR; ++j) {
+            const int i = c * BS + j * (BS / LPR) + threadIdx.x / LPR;
+            const T acc = row_dot<T, LPR>(rs, col, A, s, i, sub);
             if (sub == 0) {
                 q[i] = acc;
                 v += (double)s[i] * (double)acc;
@@ -187,15 +188,16 @@ __global__ void __launch_bounds__(BS) kb_xpby(BatchView V, const T* z, T* s, int
 }
 
 // Mixed form, reliable update: r = b64 - A x in FP64, stored in FP32.
+template <int LPR>
 __global__ void __launch_bounds__(BS) kb_replace_res(BatchView V, const int* rs, const int* col, const double* A,
                                                      const double* x, const double* b, float* r, int norm) {
-    const int sub = threadIdx.x & (kLpr - 1);
+    const int sub = threadIdx.x & (LPR - 1);
     for (int c = blockIdx.x; c < V.nchunks; c += gridDim.x) {
         if (!chunk_live(V, c)) continue;
         double v = 0.0;
-        for (int j = 0; j < kLpr; ++j) {
-            const int i = c * BS + j * (BS / kLpr) + threadIdx.x / kLpr;
-            const double acc = row_dot(rs, col, A, x, i, sub);
+        for (int j = 0; j < LPR; ++j) {
+            const int i = c * BS + j * (BS / LPR) + threadIdx.x / LPR;
+            const double acc = row_dot<double, LPR>(rs, col, A, x, i, sub);
             if (sub == 0) {
                 const float ri = (float)(b[i] - acc);
                 r[i] = ri;
@@ -208,14 +210,15 @@ __global__ void __launch_bounds__(BS) kb_replace_res(BatchView V, const int* rs,
 }
 
 // Mixed form, outer residual r = b - A x (FP64) of every system.
+template <int LPR>
 __global__ void __launch_bounds__(BS) kb_residual64(BatchView V, const int* rs, const int* col, const double* A,
                                                     const double* x, const double* b, double* r, int norm) {
-    const int sub = threadIdx.x & (kLpr - 1);
+    const int sub = threadIdx.x & (LPR - 1);
     for (int c = blockIdx.x; c < V.nchunks; c += gridDim.x) {
         double v = 0.0;
-        for (int j = 0; j < kLpr; ++j) {
-            const int i = c * BS + j * (BS / kLpr) + threadIdx.x / kLpr;
-            const double acc = row_dot(rs, col, A, x, i, sub);
+        for (int j = 0; j < LPR; ++j) {
+            const int i = c * BS + j * (BS / LPR) + threadIdx.x / LPR;
+            const double acc = row_dot<double, LPR>(rs, col, A, x, i, sub);
             if (sub == 0) {
                 const double ri = b[i] - acc;
                 r[i] = ri;
@@ -291,6 +294,16 @@ int PcgDevice::setBatch(const std::vector<int>& segRow, std::string& err) {
     return ASTG_OK;
 }
 
+int PcgDevice::batchGrid() const {
+    const int want = blocksPerCu_ > 0 ? blocksPerCu_ * cus_ : gs_;
+    return std::max(1, std::min(want, n_ / BS));
+}
+
+void PcgDevice::setBatchTuning(int blocksPerCu, int lanesPerRow) {
+    blocksPerCu_ = std::max(0, blocksPerCu);
+    lpr_ = (lanesPerRow == 4 || lanesPerRow == 16) ? lanesPerRow : 8;
+}
+
 BatchView PcgDevice::bview() const {
     BatchView v;
     v.chunkSeg = chunkSeg_.as<int>();
@@ -319,7 +332,7 @@ template <typename T>
 int PcgDevice::pcgBatchInner(const T* A, const T* invd, const T* b, double* x, int maxIt, int norm, std::string& err,
                              const double* b64, int replaceEvery) {
     const BatchView V = bview();
-    const int g = std::max(1, std::min(gs_, V.nchunks));
+    const int g = batchGrid();
     T* r = r_.as<T>();
     T* z = z_.as<T>();
     T* s = s_.as<T>();
@@ -342,12 +355,23 @@ int PcgDevice::pcgBatchInner(const T* A, const T* invd, const T* b, double* x, i
     int it = 0, batch = 4;
     for (;;) {
         for (int j = 0; j < batch && it < maxIt; ++j, ++it) {
-            kb_spmv_dot<T><<<g, BS, 0, stream_>>>(V, rs, col, A, s, q);
+            if (lpr_ == 4)
+                kb_spmv_dot<T, 4><<<g, BS, 0, stream_>>>(V, rs, col, A, s, q);
+            else if (lpr_ == 16)
+                kb_spmv_dot<T, 16><<<g, BS, 0, stream_>>>(V, rs, col, A, s, q);
+            else
+                kb_spmv_dot<T, 8><<<g, BS, 0, stream_>>>(V, rs, col, A, s, q);
             kb_reduce<<<V.K, BS, 0, stream_>>>(V, P_SQ, norm, scal);
             kb_axpy_res<T, double><<<g, BS, 0, stream_>>>(V, s, q, x, r, norm);
-            if (sizeof(T) == 4 && replaceEvery > 0 && (it + 1) % replaceEvery == 0)
-                kb_replace_res<<<g, BS, 0, stream_>>>(V, rs, col, val_.as<double>(), x, b64,
-                                                      reinterpret_cast<float*>(r), norm);
+            if (sizeof(T) == 4 && replaceEvery > 0 && (it + 1) % replaceEvery == 0) {
+                float* r32 = reinterpret_cast<float*>(r);
+                if (lpr_ == 4)
+                    kb_replace_res<4><<<g, BS, 0, stream_>>>(V, rs, col, val_.as<double>(), x, b64, r32, norm);
+                else if (lpr_ == 16)
+                    kb_replace_res<16><<<g, BS, 0, stream_>>>(V, rs, col, val_.as<double>(), x, b64, r32, norm);
+                else
+                    kb_replace_res<8><<<g, BS, 0, stream_>>>(V, rs, col, val_.as<double>(), x, b64, r32, norm);
+            }
             kb_reduce<<<V.K, BS, 0, stream_>>>(V, P_RES, norm, scal);
             if ((rc = launchFwd<T>(invd, r, it + 1, 2, it, maxIt, norm, 0.0, err))) return rc;
             if ((rc = launchBwd<T>(invd, r, z, it + 1, (it + 1) & 1, err))) return rc;
@@ -371,7 +395,7 @@ int PcgDevice::solveBatch(const astg_pcg_params& prm, std::vector<astg_pcg_resul
         return ASTG_ERR_ARG;
     }
     const BatchView V = bview();
-    const int g = std::max(1, std::min(gs_, V.nchunks));
+    const int g = batchGrid();
     const int maxIt = std::max(0, prm.max_iterations);
     res.assign(K, astg_pcg_result());
     for (auto& r : res) {
@@ -418,8 +442,20 @@ int PcgDevice::solveBatch(const astg_pcg_params& prm, std::vector<astg_pcg_resul
     std::vector<int> total(K, 0), outer(K, 0);
     std::vector<char> conv(K, 0), stopped(K, 0);
     for (;;) {
-        kb_residual64<<<g, BS, 0, stream_>>>(V, rs_.as<int>(), col_.as<int>(), val_.as<double>(), x_.as<double>(),
-                                             b_.as<double>(), r64_.as<double>(), prm.norm);
+        {
+            const int* rs = rs_.as<int>();
+            const int* col = col_.as<int>();
+            const double* A = val_.as<double>();
+            if (lpr_ == 4)
+                kb_residual64<4><<<g, BS, 0, stream_>>>(V, rs, col, A, x_.as<double>(), b_.as<double>(),
+                                                        r64_.as<double>(), prm.norm);
+            else if (lpr_ == 16)
+                kb_residual64<16><<<g, BS, 0, stream_>>>(V, rs, col, A, x_.as<double>(), b_.as<double>(),
+                                                         r64_.as<double>(), prm.norm);
+            else
+                kb_residual64<8><<<g, BS, 0, stream_>>>(V, rs, col, A, x_.as<double>(), b_.as<double>(),
+                                                        r64_.as<double>(), prm.norm);
+        }
         kb_reduce<<<K, BS, 0, stream_>>>(V, P_OUTER, prm.norm, scal_.as<double>());
         HIPCHK(hipGetLastError());
         int rc = batchState(st, false, err);

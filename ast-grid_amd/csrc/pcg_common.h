@@ -46,17 +46,17 @@ __device__ inline double reduce_prev(const double* part, int np) {
     return block_reduce<MAX>(acc);
 }
 
-// (A s)_i with kLpr lanes per row (rows are short: 8 entries on average), so
-// a wavefront reads the entries of 64/kLpr consecutive rows contiguously; the
+// (A s)_i with LPR lanes per row (rows are short: 8 entries on average), so
+// a wavefront reads the entries of 64/LPR consecutive rows contiguously; the
 // row sum is a shuffle reduction over the row's lanes. i < 0: no row.
-template <typename T>
+template <typename T, int LPR = kLpr>
 __device__ inline T row_dot(const int* rs, const int* col, const T* A, const T* s, int i, int sub) {
     T acc = (T)0;
     if (i >= 0) {
-        for (int p = rs[i] + sub, pe = rs[i + 1]; p < pe; p += kLpr) acc += A[p] * s[col[p]];
+        for (int p = rs[i] + sub, pe = rs[i + 1]; p < pe; p += LPR) acc += A[p] * s[col[p]];
     }
 #pragma unroll
-    for (int o = kLpr / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kLpr);
+    for (int o = LPR / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, LPR);
     return acc;
 }
 

@@ -115,7 +115,12 @@ def smoke_timing(steps=50):
             "note": "64^3, 3 levels; source + addForce + constrain/project/post chain/adjustE/advect per step"}
 
 
-def batched_timing(seed=0, count=24):
+# Launch shape of the batched solves, frozen from `bench.py --batch-only --tune`
+# on seed 0 (the autotuner's choice; profiles/batched_24.md).
+BATCH_TUNING = {"merge_levels": 1, "stream_blocks_per_cu": 8, "lanes_per_row": 4}
+
+
+def batched_timing(seed=0, count=24, tune=False):
     """Batched solves: `count` independent octree systems (leaf counts
     log-uniform in [5e4, 2e6]) solved by one astg_pcg_solve_batch call, FP64
     and mixed, against the same systems solved one after another (each its own
@@ -147,6 +152,14 @@ def batched_timing(seed=0, count=24):
     out = {"systems": count, "seed": seed, "leaves": leaves, "leaves_total": int(sum(leaves)),
            "n_total": int(sum(len(s[3]) for s in systems)), "nnz_total": int(sum(len(s[1]) for s in systems)),
            "scene_build_s": round(build_s, 1)}
+    if tune:
+        t1 = time.perf_counter()
+        best = c.tune_batch(systems, prm[astg.PREC_MIXED], iterations=20)
+        out["tuner"] = dict(best.as_dict(), seconds=round(time.perf_counter() - t1, 1))
+        log("tuner: %s" % out["tuner"])
+    else:
+        c.set_batch_tuning(**BATCH_TUNING)
+        out["tuning"] = dict(BATCH_TUNING)
     for p, name in ((astg.PREC_FP64, "fp64"), (astg.PREC_MIXED, "mixed")):
         t1 = time.perf_counter()
         c.pcg_solve_batch(systems, prm[p])
@@ -209,6 +222,7 @@ def main():
     ap.add_argument("--kernel-reps", type=int, default=5)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-solve workload")
     ap.add_argument("--batch-only", action="store_true", help="run only the batched-solve workload")
+    ap.add_argument("--tune", action="store_true", help="with --batch-only: run the batch autotuner first")
     ap.add_argument("--kernels-only", action="store_true",
                     help="one projection, then per-kernel timings only (for counter collection)")
     a = ap.parse_args()
@@ -223,7 +237,7 @@ def main():
 
     assert torch.cuda.is_available(), "no GPU visible"
     if a.batch_only:
-        bt = batched_timing()
+        bt = batched_timing(tune=a.tune)
         log("batched: %s" % json.dumps(bt))
         return
     t0 = time.perf_counter()

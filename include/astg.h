@@ -107,6 +107,25 @@ int astg_pcg_solve_batch(astg_ctx* ctx, int count, const int* n, const int* cons
                          const int* const* colindex, const double* const* value, const double* const* b,
                          double* const* x, const astg_pcg_params* params, astg_pcg_result* results);
 
+/* Launch shape of batched solves (no counterpart in the reference). */
+typedef struct astg_batch_tuning {
+    int merge_levels;         /* dependency levels per merged sweep group, 1-8 (default 2) */
+    int stream_blocks_per_cu; /* grid of the streaming kernels, blocks per compute unit, 1-64 (default 8) */
+    int lanes_per_row;        /* SpMV lanes per row: 4, 8 (default) or 16 */
+    double ms_per_iteration;  /* astg_pcg_tune_batch: one iteration of the chosen shape, ms */
+} astg_batch_tuning;
+void astg_batch_tuning_default(astg_batch_tuning* t);
+/* Sets the shape used by the context's next astg_pcg_solve_batch calls. */
+int astg_pcg_set_batch_tuning(astg_ctx* ctx, const astg_batch_tuning* t);
+/* Autotuner: times `iterations` PCG iterations (params' precision; no system
+ * converges) of the given batch for merge_levels in {1,2,3} x blocks per CU
+ * in {4,8,16} x lanes per row in {4,8,16}, best of two runs each, then sets
+ * the fastest shape on the context and returns it in *best. Results of the
+ * solves are discarded. */
+int astg_pcg_tune_batch(astg_ctx* ctx, int count, const int* n, const int* const* rowstart,
+                        const int* const* colindex, const double* const* value, const double* const* b,
+                        const astg_pcg_params* params, int iterations, astg_batch_tuning* best);
+
 /* ---- scene helpers ---------------------------------------------------- */
 /* Split map (R^3 ints, x fastest, value = leaf level) refined toward a union
  * of spheres (n x {cx,cy,cz,r}); band[l] = refinement distance for level l+1 -> l;

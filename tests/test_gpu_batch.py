@@ -169,3 +169,29 @@ def test_batch_scene_systems_match_single():
                 assert rel(xs[k], x1) < (1e-8 if tol < 1e-8 else 1e-5)
             else:
                 assert rel(xs[k], x1) < 1e-4
+
+
+def test_batch_tuning_shapes_agree_and_tuner_sets_one(goldens):
+    """Every launch shape the autotuner can choose gives the reference's
+    solutions; the tuner returns one of its candidates and sets it."""
+    import astg
+    c = astg.Context(0)
+    prm = astg.default_params(tol=1e-6, max_iterations=3000)
+    systems = [csr_system(g) for g in goldens]
+    for q, blk, lanes in ((1, 4, 4), (3, 16, 16), (2, 8, 8)):
+        c.set_batch_tuning(q, blk, lanes)
+        xs, res = c.pcg_solve_batch(systems, prm)
+        for k, g in enumerate(goldens):
+            x, xr = xs[k], g["p"]
+            if closed(g):
+                x, xr = x - x.mean(), xr - xr.mean()
+            assert res[k].converged == 1 and abs(res[k].iterations - int(g["pcg_iters"])) <= 1
+            assert rel(x, xr) < 1e-6, (q, blk, lanes, k)
+    best = c.tune_batch(systems, astg.default_params(precision=astg.PREC_MIXED), iterations=5)
+    print("tuned:", best.as_dict(), flush=True)
+    assert best.merge_levels in (1, 2, 3) and best.stream_blocks_per_cu in (4, 8, 16)
+    assert best.lanes_per_row in (4, 8, 16) and best.ms_per_iteration > 0
+    xs, res = c.pcg_solve_batch(systems, prm)
+    assert all(r.converged == 1 for r in res)
+    with pytest.raises(astg.AstgError):
+        c.set_batch_tuning(2, 8, 6)
