@@ -177,6 +177,28 @@ def batched_timing(seed=0, count=24, tune=False):
     return out
 
 
+def cpu_smoke_baseline(steps=50):
+    """The reference's own onUpdate (oracle/_ref) on the smoke configuration,
+    the same `steps` steps as smoke_timing, on the host's cores."""
+    from oracle import ref
+    from astg import scenes
+    if not ref.available():
+        return None
+    R, L = 64, 3
+    cx, cy, cz, rad = scenes.SMOKE_SOURCE
+    r = ref.RefSolver(scenes.smoke_64_split(R, L), R, L, scenes.CLOSED_XY)
+    r.set_source((cx, cy, cz), rad, 1.0, 1.0, 1.0)
+    r.set_buoyancy(1.0, 0.0)
+    r.set_tilt_adaptivity(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r.step(0.5 / R, tol=1e-6)
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    r.close()
+    return {"ms_per_step": round(ms, 2), "steps": steps, "cores": int(ref.lib().ref_num_workers()),
+            "kind": "reference"}
+
+
 def cpu_baseline(ctx, res64, budget_iters=150):
     """Reference MIC(0)-PCG (oracle/_ref, the reference's own code) or the
     oracle's port on the same C2 system: factor + a bounded number of PCG
@@ -301,6 +323,13 @@ def main():
             cpu = cpu_baseline(ctx, last64)
         except Exception as e:  # the baseline is reported, never required
             log("cpu baseline failed: %s" % e)
+        try:
+            cs = cpu_smoke_baseline()
+            if cs is not None and cpu is not None:
+                cpu["smoke_64"] = cs
+                log("smoke_64 reference CPU: %.1f ms/step (%d cores)" % (cs["ms_per_step"], cs["cores"]))
+        except Exception as e:
+            log("cpu smoke baseline failed: %s" % e)
 
     out = {
         "metric": "projection ms/step to 1e-6 rel. residual @256^3 octree (mixed vs FP64); HBM GB/s",
