@@ -56,20 +56,35 @@ void planMic(int n, const int* rs, const int* lend, const int* ubeg, const int* 
     const long long nnz = rs[n];
     const int nt = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
     std::vector<std::vector<int>> pr(nt);
-    std::vector<long long> len(nnz + 1, 0);
-    std::vector<std::thread> th;
+    start.resize(nnz + 1);
+    long long* len = start.data() + 1;  // segment lengths first, prefix sums in place afterwards
     std::vector<int> k0(nt + 1);
     for (int w = 0; w <= nt; ++w) k0[w] = (int)((long long)n * w / nt);
-    for (int w = 0; w < nt; ++w)
-        th.emplace_back(planRows, rs, lend, ubeg, col, k0[w], k0[w + 1], len.data(), std::ref(pr[w]));
-    for (auto& x : th) x.join();
-    start.assign(nnz + 1, 0);
-    for (long long a = 0; a < nnz; ++a) start[a + 1] = start[a] + len[a];
-    prog.clear();
-    prog.reserve((size_t)start[nnz]);
-    for (int w = 0; w < nt; ++w) {
-        prog.insert(prog.end(), pr[w].begin(), pr[w].end());
-        std::vector<int>().swap(pr[w]);
+    {
+        std::vector<std::thread> th;
+        for (int w = 0; w < nt; ++w)
+            th.emplace_back(planRows, rs, lend, ubeg, col, k0[w], k0[w + 1], len, std::ref(pr[w]));
+        for (auto& x : th) x.join();
+    }
+    // thread w's segments are contiguous (rows k0[w]..k0[w+1]), so its prefix
+    // sums start at the total of the threads before it
+    std::vector<long long> base(nt + 1, 0);
+    for (int w = 0; w < nt; ++w) base[w + 1] = base[w] + (long long)pr[w].size();
+    start[0] = 0;
+    prog.resize((size_t)base[nt]);
+    {
+        std::vector<std::thread> th;
+        for (int w = 0; w < nt; ++w)
+            th.emplace_back([&, w] {
+                long long acc = base[w];
+                for (long long a = rs[k0[w]]; a < rs[k0[w + 1]]; ++a) {
+                    acc += len[a];
+                    len[a] = acc;  // start[a + 1]
+                }
+                std::copy(pr[w].begin(), pr[w].end(), prog.begin() + base[w]);
+                std::vector<int>().swap(pr[w]);
+            });
+        for (auto& x : th) x.join();
     }
 }
 

@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "device_common.h"
 #include "pcg.h"
@@ -31,6 +32,7 @@
 #include "mic_plan.h"
 #include "sweep_merge.h"
 #include "timing.h"
+#include "parallel.h"
 
 namespace astg {
 
@@ -565,43 +567,75 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
     nnz_ = rowstart[n];
     hrs_.assign(rowstart, rowstart + n + 1);
     hcol_.assign(col, col + nnz_);
-    // split each row at the diagonal; build the transpose map and check symmetry
+    // split each row at the diagonal; build the transpose map and check
+    // symmetry (rows in parallel; the first failing row is reported)
     std::vector<int> lend(n), ubeg(n), tpos(nnz_, 0);
-    for (int i = 0; i < n; ++i) {
-        int p = rowstart[i];
-        const int e = rowstart[i + 1];
-        for (int q = p + 1; q < e; ++q)
-            if (col[q] <= col[q - 1]) {
-                err = "row " + std::to_string(i) + " is not sorted by column";
-                return ASTG_ERR_ARG;
+    int badRow = n, badKind = 0, badCol = 0;
+    {
+        std::mutex mu;
+        parallelFor(n, [&](long i0, long i1) {
+            int row = n, kind = 0, cb = 0;
+            for (int i = (int)i0; i < (int)i1 && row == n; ++i) {
+                int p = rowstart[i];
+                const int e = rowstart[i + 1];
+                for (int q = p + 1; q < e; ++q)
+                    if (col[q] <= col[q - 1]) {
+                        row = i;
+                        kind = 1;
+                        break;
+                    }
+                if (row != n) break;
+                while (p < e && col[p] < i) ++p;
+                lend[i] = p;
+                ubeg[i] = (p < e && col[p] == i) ? p + 1 : p;
+                for (int q = rowstart[i]; q < e; ++q) {
+                    const int j = col[q];
+                    if (j < 0 || j >= n) {
+                        row = i;
+                        kind = 2;
+                        break;
+                    }
+                    if (j == i) continue;
+                    const int* b = col + rowstart[j];
+                    const int* f = std::lower_bound(b, col + rowstart[j + 1], i);
+                    if (f == col + rowstart[j + 1] || *f != i) {
+                        row = i;
+                        kind = 3;
+                        cb = j;
+                        break;
+                    }
+                    tpos[q] = (int)(f - col);
+                }
+                if (row == n && (lend[i] - rowstart[i] > kMaxDeps || rowstart[i + 1] - ubeg[i] > kMaxDeps)) {
+                    row = i;
+                    kind = 4;
+                }
             }
-        while (p < e && col[p] < i) ++p;
-        lend[i] = p;
-        ubeg[i] = (p < e && col[p] == i) ? p + 1 : p;
+            if (row != n) {
+                std::lock_guard<std::mutex> g(mu);
+                if (row < badRow) {
+                    badRow = row;
+                    badKind = kind;
+                    badCol = cb;
+                }
+            }
+        });
     }
-    for (int i = 0; i < n; ++i)
-        for (int p = rowstart[i]; p < rowstart[i + 1]; ++p) {
-            const int j = col[p];
-            if (j < 0 || j >= n) {
-                err = "column index out of range";
-                return ASTG_ERR_ARG;
-            }
-            if (j == i) continue;
-            const int* b = col + rowstart[j];
-            const int* e = col + rowstart[j + 1];
-            const int* f = std::lower_bound(b, e, i);
-            if (f == e || *f != i) {
-                err = "matrix is not structurally symmetric at (" + std::to_string(i) + "," + std::to_string(j) + ")";
-                return ASTG_ERR_ARG;
-            }
-            tpos[p] = (int)(f - col);
+    if (badRow < n) {
+        const int i = badRow;
+        switch (badKind) {
+            case 1: err = "row " + std::to_string(i) + " is not sorted by column"; break;
+            case 2: err = "column index out of range"; break;
+            case 3:
+                err = "matrix is not structurally symmetric at (" + std::to_string(i) + "," +
+                      std::to_string(badCol) + ")";
+                break;
+            default:
+                err = "row " + std::to_string(i) + " has more than " + std::to_string(kMaxDeps) +
+                      " entries on one side of the diagonal (unsupported)";
         }
-    for (int i = 0; i < n; ++i)
-        if (lend[i] - rowstart[i] > kMaxDeps || rowstart[i + 1] - ubeg[i] > kMaxDeps) {
-            err = "row " + std::to_string(i) + " has more than " + std::to_string(kMaxDeps) +
-                  " entries on one side of the diagonal (unsupported)";
-            return ASTG_ERR_ARG;
-        }
+        return ASTG_ERR_ARG;
+    }
     if (const char* g = getenv("ASTG_GATE_DIST")) gateDist_ = std::max(1, atoi(g));
     if (const char* g = getenv("ASTG_POLL_CAP")) pollCap_ = std::max(1, atoi(g));
     if (const char* g = getenv("ASTG_GATE_CAP")) gateCap_ = std::max(1, atoi(g));

@@ -6,6 +6,8 @@
 // an entry exists for every (row, column) the assembly touches, whatever its
 // value, so explicit zeros are kept exactly where the reference keeps them.
 #include "project_host.h"
+#include "parallel.h"
+#include "timing.h"
 
 #include <algorithm>
 #include <set>
@@ -47,6 +49,7 @@ struct RowSet {
 }  // namespace
 
 void buildProjectTables(const Octree& o, const Numbering& num, const Weights& w, ProjectTables& t) {
+    PhaseTimer tm("tables");
     const int L = o.L;
     t = ProjectTables();
     t.off.assign(L + 1, 0);
@@ -61,17 +64,20 @@ void buildProjectTables(const Octree& o, const Numbering& num, const Weights& w,
     t.w_mac.resize((size_t)3 * G);
     t.w_tilt.resize((size_t)8 * G);
     for (int l = 0; l < L; ++l)
-        for (int i = 0; i < o.lv[l].nnode; ++i) {
-            const int g = t.off[l] + i;
-            t.level[g] = (uint8_t)l;
-            t.tilt_e[g] = o.tilt[l][i].e;
-            t.tilt_bits[g] = o.tilt[l][i].bits;
-            t.tilt_hd[g] = o.tilt[l][i].half_dir;
-            t.pid_oct[g] = num.oct[l][i];
-            t.pid_tilt[g] = num.tilt[l][i];
-            for (int a = 0; a < 3; ++a) t.w_mac[(size_t)a * G + g] = w.mac[l][(size_t)i * 3 + a];
-            for (int k = 0; k < 8; ++k) t.w_tilt[(size_t)k * G + g] = w.tilt[l][(size_t)i * 8 + k];
-        }
+        parallelFor(o.lv[l].nnode, [&](long i0, long i1) {
+            for (int i = (int)i0; i < (int)i1; ++i) {
+                const int g = t.off[l] + i;
+                t.level[g] = (uint8_t)l;
+                t.tilt_e[g] = o.tilt[l][i].e;
+                t.tilt_bits[g] = o.tilt[l][i].bits;
+                t.tilt_hd[g] = o.tilt[l][i].half_dir;
+                t.pid_oct[g] = num.oct[l][i];
+                t.pid_tilt[g] = num.tilt[l][i];
+                for (int a = 0; a < 3; ++a) t.w_mac[(size_t)a * G + g] = w.mac[l][(size_t)i * 3 + a];
+                for (int k = 0; k < 8; ++k) t.w_tilt[(size_t)k * G + g] = w.tilt[l][(size_t)i * 8 + k];
+            }
+        });
+    tm.mark("nodes");
     t.n = num.n;
     RowSet rs;
     rs.cols.assign(t.n, {});
@@ -90,7 +96,10 @@ void buildProjectTables(const Octree& o, const Numbering& num, const Weights& w,
     t.oct_corner.resize((size_t)8 * no);
     t.oct_face.resize((size_t)6 * no);
     t.oct_pos.assign((size_t)15 * no, 0xff);
-    for (int j = 0; j < no; ++j) {
+    // every octagon row and every tilt row is its own unknown: the rows of the
+    // pattern are written by one thread each
+    parallelFor(no, [&](long j0, long j1) {
+    for (int j = (int)j0; j < (int)j1; ++j) {
         const int g = octJ[j], l = t.level[g], i = g - t.off[l];
         const Level& Lv = o.lv[l];
         const int x = Lv.x[i], y = Lv.y[i], z = Lv.z[i];
@@ -124,13 +133,15 @@ void buildProjectTables(const Octree& o, const Numbering& num, const Weights& w,
         }
         rs.touch(r, r);
     }
+    });
     const int nt = t.n_tilt = (int)tiltJ.size();
     t.tilt_row.resize(nt);
     t.tilt_node.resize(nt);
     t.tilt_nb.resize((size_t)8 * nt);
     t.tilt_nblev.resize((size_t)8 * nt);
     t.tilt_pos.assign((size_t)9 * nt, 0xff);
-    for (int j = 0; j < nt; ++j) {
+    parallelFor(nt, [&](long j0, long j1) {
+    for (int j = (int)j0; j < (int)j1; ++j) {
         const int g = tiltJ[j], l = t.level[g], i = g - t.off[l];
         const Level& Lv = o.lv[l];
         const int x = Lv.x[i], y = Lv.y[i], z = Lv.z[i];
@@ -162,7 +173,9 @@ void buildProjectTables(const Octree& o, const Numbering& num, const Weights& w,
         }
         rs.touch(r, r);
     }
+    });
 
+    tm.mark("octagon and tilt rows");
     // ---- half-tilt rows (project 554-649), serial order ----
     const int nh = t.nh = (int)o.half.size();
     t.half_node.resize(nh);
@@ -222,11 +235,15 @@ void buildProjectTables(const Octree& o, const Numbering& num, const Weights& w,
         }
     }
 
+    tm.mark("half-tilt rows");
     // ---- CSR ----
     t.rowstart.assign(t.n + 1, 0);
     for (int r = 0; r < t.n; ++r) t.rowstart[r + 1] = t.rowstart[r] + (int)rs.cols[r].size();
     t.col.resize(t.rowstart[t.n]);
-    for (int r = 0; r < t.n; ++r) std::copy(rs.cols[r].begin(), rs.cols[r].end(), t.col.begin() + t.rowstart[r]);
+    parallelFor(t.n, [&](long r0, long r1) {
+        for (int r = (int)r0; r < (int)r1; ++r)
+            std::copy(rs.cols[r].begin(), rs.cols[r].end(), t.col.begin() + t.rowstart[r]);
+    });
     auto posOf = [&](int r, int c) -> uint8_t {
         auto& v = rs.cols[r];
         auto it = std::lower_bound(v.begin(), v.end(), c);
@@ -235,7 +252,8 @@ void buildProjectTables(const Octree& o, const Numbering& num, const Weights& w,
         if (p > 254) throw std::runtime_error("row too long");
         return (uint8_t)p;
     };
-    for (int j = 0; j < no; ++j) {
+    parallelFor(no, [&](long j0, long j1) {
+    for (int j = (int)j0; j < (int)j1; ++j) {
         const int r = t.oct_row[j];
         for (int f = 0; f < 6; ++f) {
             int code = t.oct_face[(size_t)f * no + j];
@@ -249,7 +267,9 @@ void buildProjectTables(const Octree& o, const Numbering& num, const Weights& w,
         }
         t.oct_pos[(size_t)14 * no + j] = posOf(r, r);
     }
-    for (int j = 0; j < nt; ++j) {
+    });
+    parallelFor(nt, [&](long j0, long j1) {
+    for (int j = (int)j0; j < (int)j1; ++j) {
         const int r = t.tilt_row[j];
         for (int k = 0; k < 8; ++k) {
             int code = t.tilt_nb[(size_t)k * nt + j];
@@ -257,18 +277,24 @@ void buildProjectTables(const Octree& o, const Numbering& num, const Weights& w,
         }
         t.tilt_pos[(size_t)8 * nt + j] = posOf(r, r);
     }
+    });
+    tm.mark("CSR and positions");
     // coarse-target lists and half-row occurrence lists, both in q order
     {
-        std::vector<std::vector<int>> byT(t.n), byC(t.n);
+        // (row, q) pairs sorted by row then q: rows ascending, q order within a row
+        std::vector<std::pair<int, int>> byT, byC;
         for (int q = 0; q < nh; ++q) {
-            if (t.half_t[q] >= 0 && t.half_c[q] >= 0) byT[t.half_t[q]].push_back(q);
-            if (t.half_c[q] >= 0) byC[t.half_c[q]].push_back(q);
+            if (t.half_t[q] >= 0 && t.half_c[q] >= 0) byT.push_back({t.half_t[q], q});
+            if (t.half_c[q] >= 0) byC.push_back({t.half_c[q], q});
         }
+        std::sort(byT.begin(), byT.end());
+        std::sort(byC.begin(), byC.end());
         t.hc_start.push_back(0);
-        for (int r = 0; r < t.n; ++r) {
-            if (byT[r].empty()) continue;
+        for (size_t a = 0; a < byT.size();) {
+            const int r = byT[a].first;
             t.hc_row.push_back(r);
-            for (int q : byT[r]) {
+            for (; a < byT.size() && byT[a].first == r; ++a) {
+                const int q = byT[a].second;
                 t.hc_q.push_back(q);
                 t.hc_pos_diag.push_back(posOf(r, r));
                 t.hc_pos_off.push_back(posOf(r, t.half_c[q]));
@@ -276,10 +302,11 @@ void buildProjectTables(const Octree& o, const Numbering& num, const Weights& w,
             t.hc_start.push_back((int)t.hc_q.size());
         }
         t.hr_start.push_back(0);
-        for (int r = 0; r < t.n; ++r) {
-            if (byC[r].empty()) continue;
+        for (size_t a = 0; a < byC.size();) {
+            const int r = byC[a].first;
             t.hr_row.push_back(r);
-            for (int q : byC[r]) {
+            for (; a < byC.size() && byC[a].first == r; ++a) {
+                const int q = byC[a].second;
                 t.hr_q.push_back(q);
                 t.hr_pos.push_back(t.half_t[q] >= 0 ? posOf(r, t.half_t[q]) : 0xff);
                 for (int m = 0; m < 4; ++m) {
@@ -293,79 +320,84 @@ void buildProjectTables(const Octree& o, const Numbering& num, const Weights& w,
         t.n_half_rows = (int)t.hr_row.size();
     }
 
+    tm.mark("half lists");
     // ---- velocity correction tables (updateVelocityByPressure 681-847) ----
-    for (int l = 0; l < L; ++l) {
-        const Level& Lv = o.lv[l];
-        const int res = Lv.res;
-        for (int i = 0; i < Lv.nleaf; ++i) {
-            const int g = t.off[l] + i, x = Lv.x[i], y = Lv.y[i], z = Lv.z[i];
-            t.vm_node.push_back(g);
-            t.vm_c.push_back(octValid(Lv, x, y, z) ? num.oct[l][i] : -1);
-            for (int a = 0; a < 3; ++a) {
-                // MAC validity: axis a may reach res-1, the other two res-2
-                int lim[3] = {res - 1, res - 1, res - 1};
-                lim[a] = res;
-                const int v[3] = {x, y, z};
-                bool macValid = v[0] < lim[0] && v[1] < lim[1] && v[2] < lim[2];
-                int32_t code = VNB_SKIP;
-                if (macValid) {
-                    int nb[3] = {x, y, z};
-                    nb[a] -= 1;
-                    code = VNB_P0;
-                    if (octValid(Lv, nb[0], nb[1], nb[2])) {
-                        int ni = Lv.get(nb[0], nb[1], nb[2]);
-                        if (ni < 0) throw std::runtime_error("MAC neighbour node missing");
-                        if (num.oct[l][ni] >= 0) code = num.oct[l][ni];
+    // MAC: every leaf of every level, in level then leaf order; neighbour
+    // codes stored axis-major
+    {
+        std::vector<int> base(L + 1, 0);
+        for (int l = 0; l < L; ++l) base[l + 1] = base[l] + o.lv[l].nleaf;
+        const int nvm = base[L];
+        t.vm_node.resize(nvm);
+        t.vm_c.resize(nvm);
+        t.vm_nb.resize((size_t)3 * nvm);
+        for (int l = 0; l < L; ++l) {
+            const Level& Lv = o.lv[l];
+            const int res = Lv.res;
+            parallelFor(Lv.nleaf, [&](long i0, long i1) {
+                for (int i = (int)i0; i < (int)i1; ++i) {
+                    const int j = base[l] + i;
+                    const int g = t.off[l] + i, x = Lv.x[i], y = Lv.y[i], z = Lv.z[i];
+                    t.vm_node[j] = g;
+                    t.vm_c[j] = octValid(Lv, x, y, z) ? num.oct[l][i] : -1;
+                    for (int a = 0; a < 3; ++a) {
+                        // MAC validity: axis a may reach res-1, the other two res-2
+                        int lim[3] = {res - 1, res - 1, res - 1};
+                        lim[a] = res;
+                        const int v[3] = {x, y, z};
+                        bool macValid = v[0] < lim[0] && v[1] < lim[1] && v[2] < lim[2];
+                        int32_t code = VNB_SKIP;
+                        if (macValid) {
+                            int nb[3] = {x, y, z};
+                            nb[a] -= 1;
+                            code = VNB_P0;
+                            if (octValid(Lv, nb[0], nb[1], nb[2])) {
+                                int ni = Lv.get(nb[0], nb[1], nb[2]);
+                                if (ni < 0) throw std::runtime_error("MAC neighbour node missing");
+                                if (num.oct[l][ni] >= 0) code = num.oct[l][ni];
+                            }
+                        }
+                        t.vm_nb[(size_t)a * nvm + j] = code;
                     }
                 }
-                t.vm_nb.push_back(code);
-            }
+            });
         }
     }
+    // open tilts, in level then leaf order; codes and levels slot-major
     {
-        const int nvm = (int)t.vm_node.size();
-        std::vector<int32_t> tr((size_t)3 * nvm);
-        for (int j = 0; j < nvm; ++j)
-            for (int a = 0; a < 3; ++a) tr[(size_t)a * nvm + j] = t.vm_nb[(size_t)3 * j + a];
-        t.vm_nb.swap(tr);
-    }
-    for (int l = 0; l < L; ++l) {
-        const Level& Lv = o.lv[l];
-        for (int i = 0; i < Lv.nleaf; ++i) {
-            if (!(o.tilt[l][i].bits & TB_LAYOUT_OPEN)) continue;
-            const int g = t.off[l] + i, x = Lv.x[i], y = Lv.y[i], z = Lv.z[i];
-            t.vt_node.push_back(g);
-            t.vt_c.push_back(num.tilt[l][i]);
-            for (int k = 0; k < 8; ++k) {
-                lcoor_t c = tiltNeighbor(o, l, x, y, z, k);
-                int32_t code = VNB_ZERO;
-                uint8_t lev = (uint8_t)l;
-                if (c != -1) {
-                    const int cl = lc_level(c);
-                    lev = (uint8_t)cl;
-                    code = VNB_P0;
-                    if (octValid(o.lv[cl], lc_x(c), lc_y(c), lc_z(c))) {
-                        int ci = o.lv[cl].get(lc_x(c), lc_y(c), lc_z(c));
-                        if (ci >= 0 && num.oct[cl][ci] >= 0) code = num.oct[cl][ci];
-                    }
-                }
-                t.vt_nb.push_back(code);
-                t.vt_lev.push_back(lev);
-            }
-        }
-    }
-    {
+        for (int l = 0; l < L; ++l)
+            for (int i = 0; i < o.lv[l].nleaf; ++i)
+                if (o.tilt[l][i].bits & TB_LAYOUT_OPEN) t.vt_node.push_back(t.off[l] + i);
         const int nvt = (int)t.vt_node.size();
-        std::vector<int32_t> tr((size_t)8 * nvt);
-        std::vector<uint8_t> tl((size_t)8 * nvt);
-        for (int j = 0; j < nvt; ++j)
-            for (int k = 0; k < 8; ++k) {
-                tr[(size_t)k * nvt + j] = t.vt_nb[(size_t)8 * j + k];
-                tl[(size_t)k * nvt + j] = t.vt_lev[(size_t)8 * j + k];
+        t.vt_c.resize(nvt);
+        t.vt_nb.resize((size_t)8 * nvt);
+        t.vt_lev.resize((size_t)8 * nvt);
+        parallelFor(nvt, [&](long j0, long j1) {
+            for (int j = (int)j0; j < (int)j1; ++j) {
+                const int g = t.vt_node[j], l = t.level[g], i = g - t.off[l];
+                const Level& Lv = o.lv[l];
+                const int x = Lv.x[i], y = Lv.y[i], z = Lv.z[i];
+                t.vt_c[j] = num.tilt[l][i];
+                for (int k = 0; k < 8; ++k) {
+                    lcoor_t c = tiltNeighbor(o, l, x, y, z, k);
+                    int32_t code = VNB_ZERO;
+                    uint8_t lev = (uint8_t)l;
+                    if (c != -1) {
+                        const int cl = lc_level(c);
+                        lev = (uint8_t)cl;
+                        code = VNB_P0;
+                        if (octValid(o.lv[cl], lc_x(c), lc_y(c), lc_z(c))) {
+                            int ci = o.lv[cl].get(lc_x(c), lc_y(c), lc_z(c));
+                            if (ci >= 0 && num.oct[cl][ci] >= 0) code = num.oct[cl][ci];
+                        }
+                    }
+                    t.vt_nb[(size_t)k * nvt + j] = code;
+                    t.vt_lev[(size_t)k * nvt + j] = lev;
+                }
             }
-        t.vt_nb.swap(tr);
-        t.vt_lev.swap(tl);
+        });
     }
+    tm.mark("velocity tables");
 }
 
 }  // namespace astg
