@@ -15,6 +15,10 @@ a scene is a pure function of (name, seed).
   256 by its target count, its sphere radii and refinement bands scaled by one
   factor found by a secant search on log(leaves) until within 10% of the
   target; level-0 tilts inside the spheres opened; N(0,1) FP32 face velocities.
+* ``full_512``: the full-scale size: 512^3-effective octree, 5 levels, the
+  projection_256 spheres with radii x 1.2 and bands 1.2 * 24 * 2^l / 512
+  (seed 0: 18.0 M leaves), level-0 tilts inside the spheres opened, N(0,1)
+  FP32 face velocities; closed x and y walls, open z walls; dt = 0.5/512.
 * ``smoke_64``: the plain smoke configuration: 64^3, 3 levels, one source
   sphere r=0.08 rising along +z, the reference's buoyancy axis
   (addForce, octree_omnigrid_solver3.cpp:917-972): centre (0.5,0.5,0.12);
@@ -73,6 +77,25 @@ def projection_256(device=0, seed=0, R=256, levels=4, band=BAND_256, velocities=
     info = {"R": R, "levels": levels, "leaves": leaves, "spheres": spheres, "dt": 0.5 / R, "rho": 2.0,
             "seed": seed}
     return ctx, info
+
+
+def full_512(device=0, seed=0, scale=1.2):
+    R, levels = 512, 5
+    spheres = random_spheres(seed)
+    spheres[:, 3] *= scale
+    band = [scale * 24.0 * (1 << l) / R for l in range(levels - 1)]
+    split = split_spheres(R, levels, spheres.ravel(), band)
+    ctx = Context(device)
+    ctx.set_topology(split, R, levels, CLOSED_XY)
+    del split
+    xyz, _ = ctx.nodes(0)
+    nleaf = ctx.level_info(0)[1]
+    ctx.set_tilt_open(0, inside_mask(xyz[:nleaf], R, spheres))
+    del xyz
+    fill_velocity(ctx, levels, seed)
+    leaves = sum(ctx.level_info(l)[1] for l in range(levels))
+    return ctx, {"R": R, "levels": levels, "leaves": leaves, "spheres": spheres, "dt": 0.5 / R, "rho": 2.0,
+                 "seed": seed}
 
 
 def leaf_count(split):

@@ -177,6 +177,67 @@ def batched_timing(seed=0, count=24, tune=False):
     return out
 
 
+def full_scale_timing(reps=2):
+    """The full-scale size (scenes.full_512: 512^3-effective, 5 levels, 18 M
+    leaves): set-up, one projection in FP64 and one mixed (best of `reps`
+    after a first call that builds the solver pattern), the mixed result's
+    error against FP64 per field, the post-projection chain and one advect."""
+    import numpy as np
+    import torch
+    import astg
+    from astg import scenes
+    t0 = time.perf_counter()
+    ctx, info = scenes.full_512(device=0)
+    scene_s = time.perf_counter() - t0
+    log("full_512: scene %.1f s, %d leaves" % (scene_s, info["leaves"]))
+    L = info["levels"]
+    out = {"workload": "full_512: 512^3-effective AST octree, 5 levels, %d leaves" % info["leaves"],
+           "scene_s": round(scene_s, 1)}
+    ctx.snapshot(restore=False)
+    fields = {}
+    for p, name in ((astg.PREC_FP64, "fp64"), (astg.PREC_MIXED, "mixed")):
+        prm = astg.default_params(tol=1e-6, max_iterations=3000, precision=p)
+        best = None
+        for k in range(reps + 1):
+            ctx.snapshot(restore=True)
+            t1 = time.perf_counter()
+            r = ctx.project(info["dt"], info["rho"], prm)
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t1
+            if k == 0:
+                out["first_call_s_" + name] = round(wall, 2)
+            elif best is None or r.ms_total < best.ms_total:
+                best = r
+        log("full_512 %s: %.1f ms, %d iterations" % (name, best.ms_total, best.iterations))
+        out[name] = {"ms_per_step": round(best.ms_total, 2), "iterations": best.iterations, "outer": best.outer,
+                     "ms_factor": round(best.ms_factor, 2), "ms_solve": round(best.ms_solve, 2),
+                     "rel_residual": best.rel_residual}
+        out["n"], out["nnz"], out["depth"] = best.n, best.nnz, best.depth
+        _, _, _, _, pr = ctx.system(best.n, best.nnz)
+        fields[name] = {"p": pr, "u": [ctx.velocity(l) for l in range(L)]}
+    rel = lambda a, b: float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+    f64, fmx = fields["fp64"], fields["mixed"]
+    out["mixed_vs_fp64"] = {
+        "pressure_rel_l2": rel(fmx["p"], f64["p"]),
+        "mac_velocity_rel_l2": rel(np.concatenate([u[0].ravel() for u in fmx["u"]]),
+                                   np.concatenate([u[0].ravel() for u in f64["u"]])),
+        "tilt_velocity_rel_l2": rel(np.concatenate([u[1].ravel() for u in fmx["u"]]),
+                                    np.concatenate([u[1].ravel() for u in f64["u"]]))}
+    out["speedup_mixed_vs_fp64"] = round(out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"], 3)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ctx.post_project()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    ctx.advect(info["dt"])
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    out["ms_post_project"] = round((t2 - t1) * 1e3, 2)
+    out["ms_advect"] = round((t3 - t2) * 1e3, 2)
+    ctx.close()
+    return out
+
+
 def cpu_smoke_baseline(steps=50):
     """The reference's own onUpdate (oracle/_ref) on the smoke configuration,
     the same `steps` steps as smoke_timing, on the host's cores."""
@@ -245,6 +306,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-solve workload")
     ap.add_argument("--batch-only", action="store_true", help="run only the batched-solve workload")
     ap.add_argument("--tune", action="store_true", help="with --batch-only: run the batch autotuner first")
+    ap.add_argument("--full-scale", action="store_true",
+                    help="run only the full-scale size (512^3-effective, 5 levels, 18 M leaves)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="one projection, then per-kernel timings only (for counter collection)")
     a = ap.parse_args()
@@ -258,6 +321,10 @@ def main():
     from astg import scenes
 
     assert torch.cuda.is_available(), "no GPU visible"
+    if a.full_scale:
+        fs = full_scale_timing()
+        log("full_scale: %s" % json.dumps(fs))
+        return
     if a.batch_only:
         bt = batched_timing(tune=a.tune)
         log("batched: %s" % json.dumps(bt))
