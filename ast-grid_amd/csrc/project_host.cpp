@@ -10,6 +10,8 @@
 #include "timing.h"
 
 #include <algorithm>
+#include <climits>
+#include <memory>
 #include <set>
 #include <stdexcept>
 
@@ -37,13 +39,28 @@ lcoor_t tiltNeighbor(const Octree& o, int l, int x, int y, int z, int k) {
     return c;
 }
 
+// Pattern rows under construction. Octagon and tilt rows (each its own
+// unknown, at most 15 entries: 6 faces, 8 tilt corners, the diagonal) are
+// sorted lists in a flat array with stride kRowCap, each written by one
+// thread; the half-tilt couplings, which several half-tilts add to one
+// coarse row, are (row, column) pairs merged in when the CSR is formed.
+constexpr int kRowCap = 16;
 struct RowSet {
-    std::vector<std::vector<int32_t>> cols;
-    void touch(int r, int c) {
-        auto& v = cols[r];
-        auto it = std::lower_bound(v.begin(), v.end(), c);
-        if (it == v.end() || *it != c) v.insert(it, c);
+    std::unique_ptr<int32_t[]> c;    // n * kRowCap, only the first k[r] of a row are defined
+    std::vector<uint8_t> k;          // n
+    std::vector<std::pair<int, int>> extra;
+    explicit RowSet(int n) : c(new int32_t[(size_t)n * kRowCap]), k((size_t)n, 0) {}
+    void touch(int r, int col) {
+        int32_t* v = c.get() + (size_t)r * kRowCap;
+        const int m = k[r];
+        const int p = (int)(std::lower_bound(v, v + m, col) - v);
+        if (p < m && v[p] == col) return;
+        if (m == kRowCap) throw std::runtime_error("pattern row longer than expected");
+        for (int q = m; q > p; --q) v[q] = v[q - 1];
+        v[p] = col;
+        k[r] = (uint8_t)(m + 1);
     }
+    void touchShared(int r, int col) { extra.push_back({r, col}); }
 };
 
 }  // namespace
@@ -79,8 +96,7 @@ void buildProjectTables(const Octree& o, const Numbering& num, const Weights& w,
         });
     tm.mark("nodes");
     t.n = num.n;
-    RowSet rs;
-    rs.cols.assign(t.n, {});
+    RowSet rs(t.n);
 
     // ---- octagon and tilt rows (project 367-552) ----
     std::vector<int> octJ, tiltJ;
@@ -224,31 +240,66 @@ void buildProjectTables(const Octree& o, const Numbering& num, const Weights& w,
             t.half_nb[(size_t)4 * q + m] = code;
         }
         if (tc >= 0 && c >= 0) {
-            rs.touch(tc, tc);
-            rs.touch(tc, c);
-            rs.touch(c, tc);
+            rs.touchShared(tc, tc);
+            rs.touchShared(tc, c);
+            rs.touchShared(c, tc);
         }
         if (c >= 0) {
             for (int m = 0; m < 4; ++m)
-                if (t.half_nb[(size_t)4 * q + m] >= 0) rs.touch(c, t.half_nb[(size_t)4 * q + m]);
-            rs.touch(c, c);
+                if (t.half_nb[(size_t)4 * q + m] >= 0) rs.touchShared(c, t.half_nb[(size_t)4 * q + m]);
+            rs.touchShared(c, c);
         }
     }
 
     tm.mark("half-tilt rows");
-    // ---- CSR ----
+    // ---- CSR: each row's own list merged with its shared couplings ----
+    std::sort(rs.extra.begin(), rs.extra.end());
+    rs.extra.erase(std::unique(rs.extra.begin(), rs.extra.end()), rs.extra.end());
+    const auto& ex = rs.extra;
+    // merge of row r's list and its shared entries [e0, e1); emit(col) per entry
+    auto mergeRow = [&](int r, size_t e0, size_t e1, auto&& emit) {
+        const int32_t* v = rs.c.get() + (size_t)r * kRowCap;
+        const int m = rs.k[r];
+        int a = 0;
+        size_t b = e0;
+        while (a < m || b < e1) {
+            if (b == e1 || (a < m && v[a] < ex[b].second)) emit(v[a++]);
+            else if (a == m || ex[b].second < v[a]) emit(ex[b++].second);
+            else {
+                emit(v[a++]);
+                ++b;
+            }
+        }
+    };
+    // rows in slices; each slice finds its first shared entry once
+    auto sliceRows = [&](auto&& body) {
+        parallelFor(t.n, [&](long r0, long r1) {
+            size_t e = (size_t)(std::lower_bound(ex.begin(), ex.end(), std::make_pair((int)r0, INT32_MIN)) - ex.begin());
+            for (int r = (int)r0; r < (int)r1; ++r) {
+                const size_t e0 = e;
+                while (e < ex.size() && ex[e].first == r) ++e;
+                body(r, e0, e);
+            }
+        });
+    };
     t.rowstart.assign(t.n + 1, 0);
-    for (int r = 0; r < t.n; ++r) t.rowstart[r + 1] = t.rowstart[r] + (int)rs.cols[r].size();
+    sliceRows([&](int r, size_t e0, size_t e1) {
+        int cnt = 0;
+        mergeRow(r, e0, e1, [&](int) { ++cnt; });
+        t.rowstart[r + 1] = cnt;
+    });
+    for (int r = 0; r < t.n; ++r) t.rowstart[r + 1] += t.rowstart[r];
     t.col.resize(t.rowstart[t.n]);
-    parallelFor(t.n, [&](long r0, long r1) {
-        for (int r = (int)r0; r < (int)r1; ++r)
-            std::copy(rs.cols[r].begin(), rs.cols[r].end(), t.col.begin() + t.rowstart[r]);
+    sliceRows([&](int r, size_t e0, size_t e1) {
+        int32_t* o = t.col.data() + t.rowstart[r];
+        mergeRow(r, e0, e1, [&](int c) { *o++ = c; });
     });
     auto posOf = [&](int r, int c) -> uint8_t {
-        auto& v = rs.cols[r];
-        auto it = std::lower_bound(v.begin(), v.end(), c);
-        if (it == v.end() || *it != c) throw std::runtime_error("pattern lookup failed");
-        size_t p = (size_t)(it - v.begin());
+        const int32_t* b = t.col.data() + t.rowstart[r];
+        const int32_t* e = t.col.data() + t.rowstart[r + 1];
+        const int32_t* it = std::lower_bound(b, e, c);
+        if (it == e || *it != c) throw std::runtime_error("pattern lookup failed");
+        size_t p = (size_t)(it - b);
         if (p > 254) throw std::runtime_error("row too long");
         return (uint8_t)p;
     };

@@ -106,27 +106,38 @@ void planStencil(int dir, int n, const int* rs, const int* lend, const int* ubeg
         th.emplace_back(planRange, dir, rs, lend, ubeg, col, lev, q, maxY, order, pos, t0, t1, std::ref(parts[w]));
     }
     for (auto& x : th) x.join();
-    st.start.clear();
-    st.nr.clear();
-    st.idx.clear();
-    st.ustart.clear();
-    st.upd.clear();
+    // concatenate the parts in parallel at their final offsets
+    std::vector<size_t> pS(nt + 1, 0), pE(nt + 1, 0), pU(nt + 1, 0);
     st.rTerms = st.yTerms = st.kept = 0;
-    st.start.reserve(P + 1);
-    st.ustart.reserve(P + 1);
-    for (auto& pt : parts) {
-        const int e0 = (int)st.idx.size(), u0 = (int)st.upd.size();
-        for (int s : pt.start) st.start.push_back(e0 + s);
-        for (int s : pt.ustart) st.ustart.push_back(u0 + s);
-        st.nr.insert(st.nr.end(), pt.nr.begin(), pt.nr.end());
-        st.idx.insert(st.idx.end(), pt.idx.begin(), pt.idx.end());
-        st.upd.insert(st.upd.end(), pt.upd.begin(), pt.upd.end());
-        st.rTerms += pt.rTerms;
-        st.yTerms += pt.yTerms;
-        st.kept += pt.kept;
+    for (int w = 0; w < nt; ++w) {
+        pS[w + 1] = pS[w] + parts[w].start.size();
+        pE[w + 1] = pE[w] + parts[w].idx.size();
+        pU[w + 1] = pU[w] + parts[w].upd.size();
+        st.rTerms += parts[w].rTerms;
+        st.yTerms += parts[w].yTerms;
+        st.kept += parts[w].kept;
     }
-    st.start.push_back((int)st.idx.size());
-    st.ustart.push_back((int)st.upd.size());
+    st.start.resize(pS[nt] + 1);
+    st.ustart.resize(pS[nt] + 1);
+    st.nr.resize(pS[nt]);
+    st.idx.resize(pE[nt]);
+    st.upd.resize(pU[nt]);
+    th.clear();
+    for (int w = 0; w < nt; ++w)
+        th.emplace_back([&, w] {
+            Part& pt = parts[w];
+            for (size_t a = 0; a < pt.start.size(); ++a) {
+                st.start[pS[w] + a] = (int)pE[w] + pt.start[a];
+                st.ustart[pS[w] + a] = (int)pU[w] + pt.ustart[a];
+            }
+            std::copy(pt.nr.begin(), pt.nr.end(), st.nr.begin() + pS[w]);
+            std::copy(pt.idx.begin(), pt.idx.end(), st.idx.begin() + pE[w]);
+            std::copy(pt.upd.begin(), pt.upd.end(), st.upd.begin() + pU[w]);
+            pt = Part();
+        });
+    for (auto& x : th) x.join();
+    st.start[pS[nt]] = (int)st.idx.size();
+    st.ustart[pS[nt]] = (int)st.upd.size();
 }
 
 }  // namespace astg
