@@ -19,6 +19,7 @@ ASTG_ERR_HIP = -2
 ASTG_ERR_NOT_CONVERGED = -3
 ASTG_ERR_TOPOLOGY = -4
 ASTG_ERR_STATE = -5
+ASTG_ERR_IO = -6
 
 WALL_XNEG, WALL_XPOS, WALL_ZNEG, WALL_ZPOS, WALL_YNEG, WALL_YPOS = 1, 2, 4, 8, 16, 32
 PREC_FP64, PREC_MIXED = 0, 1
@@ -28,7 +29,7 @@ NORM_INF, NORM_L2 = 0, 1
 EXPORTS = [
     "astg_pcg_params_default", "astg_create", "astg_destroy", "astg_last_error",
     "astg_pcg_solve_csr", "astg_pcg_solve_batch", "astg_batch_tuning_default", "astg_pcg_set_batch_tuning",
-    "astg_pcg_tune_batch", "astg_split_spheres", "astg_set_topology_split", "astg_set_tilt_open",
+    "astg_pcg_tune_batch", "astg_write_frame", "astg_read_frame", "astg_split_spheres", "astg_set_topology_split", "astg_set_tilt_open",
     "astg_level_info", "astg_get_nodes", "astg_get_tilt", "astg_get_half_tilts", "astg_get_numbering",
     "astg_get_weights", "astg_set_velocity", "astg_get_velocity", "astg_project", "astg_get_system",
     "astg_get_mic_factor", "astg_fields_snapshot", "astg_kernel_time", "astg_sweep_timeline",
@@ -93,6 +94,10 @@ def lib():
         "astg_pcg_tune_batch": (I, [P, I, P, P, P, P, P, ctypes.POINTER(PcgParams), I,
                                     ctypes.POINTER(BatchTuning)]),
         "astg_split_spheres": (I, [I, I, I, P, P, P]),
+        "astg_write_frame": (I, [P, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
+                                 ctypes.c_char_p]),
+        "astg_read_frame": (I, [P, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
+                                ctypes.c_char_p]),
         "astg_set_topology_split": (I, [P, I, D, I, P, I]),
         "astg_set_tilt_open": (I, [P, I, P]),
         "astg_level_info": (I, [P, I, P, P]),
@@ -366,6 +371,18 @@ class Context:
         self._chk(self._L.astg_sweep_timeline(self.h, which, t.ctypes.data_as(ctypes.c_void_p),
                                               lev.ctypes.data_as(ctypes.c_void_p)), "sweep_timeline")
         return t, lev
+
+    def write_frame(self, layout=None, tilt=None, density=None, heat=None, velocity=None):
+        """Writes the reference's binary frame files (None skips a file)."""
+        enc = lambda p: None if p is None else os.fsencode(p)
+        self._chk(self._L.astg_write_frame(self.h, enc(layout), enc(tilt), enc(density), enc(heat), enc(velocity)),
+                  "write_frame")
+
+    def read_frame(self, layout=None, tilt=None, density=None, heat=None, velocity=None):
+        """restart from the reference's binary frame files (None skips a file)."""
+        enc = lambda p: None if p is None else os.fsencode(p)
+        self._chk(self._L.astg_read_frame(self.h, enc(layout), enc(tilt), enc(density), enc(heat), enc(velocity)),
+                  "read_frame")
 
     def pcg_solve_csr(self, rowstart, col, val, b, params=None, allow_unconverged=False):
         rowstart = np.ascontiguousarray(rowstart, np.int32)

@@ -62,10 +62,16 @@ public:
     int getDual(int level, double* oct3, double* tilt3, std::string& err);
 
     // MAC wall constraint and velocity ghosts (onUpdate before project).
-    int constrainAndGhost(ProjectDevice& pd, std::string& err);
+    int constrainAndGhost(ProjectDevice& pd, std::string& err, bool constrain = true);
     // onUpdate between project and advect: the above, then the dual grid and
     // its ghosts.
-    int postProject(ProjectDevice& pd, std::string& err);
+    int postProject(ProjectDevice& pd, std::string& err, bool constrain = true);
+    // The fields now carry the current tilt state (they were just written).
+    void fieldsCurrent() { sampValid_ = false; }
+    // Copies the current scalar fields (field f when scalars[f]) and, when
+    // velocity is set, the current velocity into the advection destinations,
+    // as restart loads tDensity / tHeat / tVelocity with the same values.
+    int mirrorTemporaries(ProjectDevice& pd, const bool scalars[2], bool velocity, std::string& err);
     // Ghost update of one scalar field (omni grid).
     int scalarGhost(int field, std::string& err);
     // advect(dt) with tilt adaptivity off: scalars, their ghosts, velocity,

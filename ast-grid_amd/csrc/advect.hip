@@ -1417,13 +1417,13 @@ int AdvectDevice::omniGhost(double* cell, double* vert, int nc, std::string& err
     return ASTG_OK;
 }
 
-int AdvectDevice::constrainAndGhost(ProjectDevice& pd, std::string& err) {
+int AdvectDevice::constrainAndGhost(ProjectDevice& pd, std::string& err, bool constrain) {
     const Topo T = topo();
     const uint8_t* lev = static_cast<const uint8_t*>(level_.p);
     double* mac = pd.uMac();
     double* tilt = pd.uTilt();
     // constrain (MAC walls; tilt and dual constraints need a collider)
-    k_wall<<<nblocks(G_), BS, 0, stream_>>>(T, lev, mac, closed_);
+    if (constrain) k_wall<<<nblocks(G_), BS, 0, stream_>>>(T, lev, mac, closed_);
     // velocity ghosts (octree_omni_face_centered_grid3.h:97-120)
     if (nGhost_) {
         k_mac_ghost_clear<<<nblocks(nGhost_), BS, 0, stream_>>>(ghost_.as<int>(), nGhost_, mac);
@@ -1441,13 +1441,13 @@ int AdvectDevice::constrainAndGhost(ProjectDevice& pd, std::string& err) {
     return ASTG_OK;
 }
 
-int AdvectDevice::postProject(ProjectDevice& pd, std::string& err) {
+int AdvectDevice::postProject(ProjectDevice& pd, std::string& err, bool constrain) {
     const Topo T = topo();
     const uint8_t* lev = static_cast<const uint8_t*>(level_.p);
     double* mac = pd.uMac();
     double* tilt = pd.uTilt();
     HIPCHK(hipMemsetAsync(bad_.p, 0, 4, stream_));
-    int rc = constrainAndGhost(pd, err);
+    int rc = constrainAndGhost(pd, err, constrain);
     if (rc) return rc;
     // dual grid
     if (nLeaf_) {
@@ -1543,6 +1543,22 @@ int AdvectDevice::advect(ProjectDevice& pd, double dt, std::string& err) {
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(stream_));
     sampValid_ = false;  // the fields now carry the current tilt state
+    return ASTG_OK;
+}
+
+int AdvectDevice::mirrorTemporaries(ProjectDevice& pd, const bool scalars[2], bool velocity, std::string& err) {
+    const size_t b = (size_t)G_ * 8;
+    for (int f = 0; f < 2; ++f) {
+        if (!scalars[f]) continue;
+        const int c = cur_[f];
+        for (int k = 0; k < 2; ++k)
+            HIPCHK(hipMemcpyAsync(scal_[f][1 - c][k].p, scal_[f][c][k].p, b, hipMemcpyDeviceToDevice, stream_));
+    }
+    if (velocity) {
+        HIPCHK(hipMemcpyAsync(tMac_.p, pd.uMac(), 3 * b, hipMemcpyDeviceToDevice, stream_));
+        HIPCHK(hipMemcpyAsync(tTilt_.p, pd.uTilt(), 8 * b, hipMemcpyDeviceToDevice, stream_));
+    }
+    HIPCHK(hipStreamSynchronize(stream_));
     return ASTG_OK;
 }
 

@@ -14,6 +14,7 @@
 #include "octree.h"
 #include "timing.h"
 #include "pcg.h"
+#include "frame_io.h"
 #include "advect.h"
 #include "project.h"
 #include "project_host.h"
@@ -956,3 +957,100 @@ int astg_step(astg_ctx* c, double dt, double rho, const astg_pcg_params* params,
 }
 
 }  // extern "C"
+
+int astg_write_frame(astg_ctx* c, const char* layout, const char* tilt, const char* density, const char* heat,
+                     const char* velocity) {
+    if (!c) return ASTG_ERR_ARG;
+    if (!c->tree) return fail(c, ASTG_ERR_STATE, "no topology set");
+    return guarded(c, [&] {
+        const astg::Octree& o = *c->tree;
+        if (layout && !astg::writeLayoutFile(layout, o, c->err)) return ASTG_ERR_IO;
+        if (tilt && !astg::writeTiltFile(tilt, o, c->err)) return ASTG_ERR_IO;
+        const char* sc[2] = {density, heat};
+        for (int f = 0; f < 2; ++f) {
+            if (!sc[f]) continue;
+            astg::OmniScalarFrame fr;
+            fr.oct.assign(o.L, {});
+            fr.tilt.assign(o.L, {});
+            for (int l = 0; l < o.L; ++l) {
+                fr.oct[l].resize(o.lv[l].nnode);
+                fr.tilt[l].resize(o.lv[l].nnode);
+                int r = astg_get_scalar(c, f, l, fr.oct[l].data(), fr.tilt[l].data());
+                if (r) return r;
+            }
+            if (!astg::writeScalarFile(sc[f], fr, c->err)) return ASTG_ERR_IO;
+        }
+        if (velocity) {
+            astg::OmniVelocityFrame fr;
+            fr.mac.assign(o.L, {});
+            fr.tilt8.assign(o.L, {});
+            for (int l = 0; l < o.L; ++l) {
+                fr.mac[l].resize((size_t)o.lv[l].nnode * 3);
+                fr.tilt8[l].resize((size_t)o.lv[l].nnode * 8);
+                int r = astg_get_velocity(c, l, fr.mac[l].data(), fr.tilt8[l].data());
+                if (r) return r;
+            }
+            if (!astg::writeVelocityFile(velocity, fr, c->err)) return ASTG_ERR_IO;
+        }
+        return ASTG_OK;
+    });
+}
+
+int astg_read_frame(astg_ctx* c, const char* layout, const char* tilt, const char* density, const char* heat,
+                    const char* velocity) {
+    if (!c) return ASTG_ERR_ARG;
+    if (!c->tree) return fail(c, ASTG_ERR_STATE, "no topology set");
+    if ((density || heat || velocity) && c->device < 0)
+        return fail(c, ASTG_ERR_STATE, "fields need a device context");
+    return guarded(c, [&] {
+        astg::Octree& o = *c->tree;
+        if (layout && !astg::checkLayoutFile(layout, o, c->err)) return ASTG_ERR_IO;
+        // every file is read and checked before any state changes
+        astg::Octree staged;
+        const bool newTilt = tilt != nullptr;
+        std::vector<std::vector<astg::TiltNode>> tiltState;
+        if (newTilt) {
+            staged.L = o.L;
+            staged.lv = o.lv;
+            staged.tilt = o.tilt;
+            if (!astg::readTiltFile(tilt, staged, c->err)) return ASTG_ERR_IO;
+            tiltState.swap(staged.tilt);
+        }
+        const char* sc[2] = {density, heat};
+        astg::OmniScalarFrame sf[2];
+        for (int f = 0; f < 2; ++f)
+            if (sc[f] && !astg::readScalarFile(sc[f], o, sf[f], c->err)) return ASTG_ERR_IO;
+        astg::OmniVelocityFrame vf;
+        if (velocity && !astg::readVelocityFile(velocity, o, vf, c->err)) return ASTG_ERR_IO;
+        if (newTilt) {
+            o.tilt.swap(tiltState);
+            if (c->deviceReady) {
+                int r = retopologize(c);
+                if (r) return r;
+            } else {
+                c->tablesReady = false;
+            }
+        }
+        if (c->device < 0) return ASTG_OK;
+        for (int f = 0; f < 2; ++f)
+            if (sc[f])
+                for (int l = 0; l < o.L; ++l) {
+                    int r = astg_set_scalar(c, f, l, sf[f].oct[l].data(), sf[f].tilt[l].data());
+                    if (r) return r;
+                }
+        if (velocity) {
+            for (int l = 0; l < o.L; ++l) {
+                int r = astg_set_velocity(c, l, vf.mac[l].data(), vf.tilt8[l].data());
+                if (r) return r;
+            }
+            // restart's velocity ghosts, dual grid and dual ghosts, no constraint
+            int r = c->adv->postProject(*c->proj, c->err, false);
+            if (r) return r;
+        }
+        // restart loads the advection temporaries from the same files
+        const bool sl[2] = {density != nullptr, heat != nullptr};
+        if (int r = c->adv->mirrorTemporaries(*c->proj, sl, velocity != nullptr, c->err)) return r;
+        if (newTilt || density || heat || velocity) c->adv->fieldsCurrent();
+        return ASTG_OK;
+    });
+}

@@ -35,6 +35,7 @@ extern "C" {
 #define ASTG_ERR_NOT_CONVERGED (-3) /* PCG hit max_iterations, or rho == 0 / NaN */
 #define ASTG_ERR_TOPOLOGY (-4)      /* topology missing or inconsistent */
 #define ASTG_ERR_STATE (-5)         /* call order violated (e.g. project before topology) */
+#define ASTG_ERR_IO (-6)            /* file missing, truncated or not matching the context */
 
 /* Reference wall flags (include/constants.h:11-26). */
 #define ASTG_WALL_XNEG (1 << 0)
@@ -289,6 +290,26 @@ int astg_advect(astg_ctx* ctx, double dt);
  * first: astg_apply_sphere_source, astg_add_force): constrain + ghosts,
  * astg_project, astg_post_project, astg_advect. */
 int astg_step(astg_ctx* ctx, double dt, double rho, const astg_pcg_params* params, astg_pcg_result* result);
+
+/* ---- frames on disk (the reference's binary formats) ------------------- */
+/* Writes the context's state as the reference's frame dumps
+ * (src/viewer/grid_dump.cpp): layout (dumpOctreeLayout3dBinary, 280-327),
+ * tilt state (dumpOctreeOmniTiltE3dBinary, 448-467), density and heat
+ * (dumpOctreeOmniScalar3dBinary, 379-410), velocity
+ * (dumpOctreeOmniFaceCentered3dBinary, 492-549). A null path skips that file;
+ * layout and tilt need only a topology, the fields a device context. */
+int astg_write_frame(astg_ctx* ctx, const char* layout, const char* tilt, const char* density, const char* heat,
+                     const char* velocity);
+/* restart (src/octree_omnigrid_solver3.cpp:1015-1057) on a context whose
+ * topology was set from the same split map: the layout file must describe
+ * the context's layers exactly (ASTG_ERR_IO otherwise; the reference reads it
+ * into the layers it already built); the tilt state is replaced as restart
+ * replaces tiltE, keeping the numbering's own open / fixed state as restart
+ * keeps tilt_layout; density, heat and velocity are loaded, then velocity
+ * ghosts, the dual grid and its ghosts are recomputed (no constraint), as
+ * restart does. Null paths skip; every file is checked before state changes. */
+int astg_read_frame(astg_ctx* ctx, const char* layout, const char* tilt, const char* density, const char* heat,
+                    const char* velocity);
 
 #ifdef __cplusplus
 }

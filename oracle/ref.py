@@ -71,6 +71,8 @@ def lib():
         "ref_set_source": (None, [P, D, D, D, D, D, D, D]),
         "ref_set_buoyancy": (None, [P, D, D]),
         "ref_step": (I, [P, D, D, I]),
+        "ref_write_frame": (I, [P] + [ctypes.c_char_p] * 5),
+        "ref_restart": (I, [P] + [ctypes.c_char_p] * 5),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -221,6 +223,17 @@ class RefSolver:
 
     def set_buoyancy(self, smoke_density_factor, temperature_factor):
         self.L.ref_set_buoyancy(self.h, smoke_density_factor, temperature_factor)
+
+    def write_frame(self, layout=None, tilt=None, density=None, heat=None, velocity=None):
+        """The reference's frame dumps (grid_dump.cpp); None skips a file."""
+        enc = lambda q: None if q is None else os.fsencode(q)
+        self._chk(self.L.ref_write_frame(self.h, enc(layout), enc(tilt), enc(density), enc(heat), enc(velocity)))
+
+    def restart(self, layout, tilt, density, heat, velocity):
+        """The reference's restart from frame files (writes a preview image
+        under ./astg_ref_harness/ in the working directory)."""
+        enc = os.fsencode
+        self._chk(self.L.ref_restart(self.h, enc(layout), enc(tilt), enc(density), enc(heat), enc(velocity)))
 
     def step(self, dt, tol=1e-6, maxit=3000):
         """One onUpdate(dt): sceneSetup, addForce, constrain, project, advect."""

@@ -29,6 +29,7 @@
 #include "octree_omnigrid_solver3.h"
 #include "octree_util.h"
 #include "pcgsolver/pcg_solver.h"
+#include "viewer/grid_dump.h"
 
 using namespace Omni;
 
@@ -389,6 +390,34 @@ int ref_sample_scalar(void* p, int field, int n, const double* xyz, double* out)
     return guarded([&] {
         for (int i = 0; i < n; ++i) out[i] = s->scalar(field)->sample(Vector3f(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]));
     });
+}
+
+// The reference's frame dumps (src/viewer/grid_dump.cpp) of the current
+// state; a null path skips that file.
+int ref_write_frame(void* p, const char* layout, const char* tilt, const char* density, const char* heat,
+                    const char* velocity) {
+    return guarded([&] {
+        auto* s = H(p);
+        if (layout) dumpOctreeLayout3dBinary(layout, *s->lay());
+        if (tilt) dumpOctreeOmniTiltE3dBinary(tilt, *s->tilt());
+        if (density) dumpOctreeOmniScalar3dBinary(density, *s->scalar(0));
+        if (heat) dumpOctreeOmniScalar3dBinary(heat, *s->scalar(1));
+        if (velocity) dumpOctreeOmniFaceCentered3dBinary(velocity, *s->getVelocity());
+    });
+}
+
+// The reference's restart (octree_omnigrid_solver3.cpp:1015-1057) from frame
+// files; it also writes a preview image under ./astg_ref_harness/.
+int ref_restart(void* p, const char* layout, const char* tilt, const char* density, const char* heat,
+                const char* velocity) {
+    int ok = 0;
+    int rc = guarded([&] { ok = H(p)->restart(0, layout, "", tilt, density, heat, velocity) ? 1 : 0; });
+    if (rc) return rc;
+    if (!ok) {
+        g_err = "restart failed to read a file";
+        return -1;
+    }
+    return 0;
 }
 
 // adjustE (octree_omnigrid_solver3.cpp:1125-1166) on the current fields.
