@@ -177,6 +177,7 @@ private:
     int batchState(std::vector<double>& st, bool toDevice, std::string& err);
     BatchView bview() const;
     int batchGrid() const;
+    bool isSweepKernel(const void* f) const;  // k_fwd / k_bwd_dot (cooperative graph nodes)
     int launchMic(double tau, double ratio, std::string& err);
     int launchMerge(std::string& err);
     int convert32(std::string& err);
@@ -229,6 +230,7 @@ private:
     int blocksPerCu_ = 0;                   // batched form: streaming grid (0: gs_)
     int lpr_ = 8;                           // batched form: SpMV lanes per row
     int mergeLevels_ = 0;                   // requested levels per sweep group (0: default)
+    bool batchGraph_ = true;                // batched form: iterations replayed from hipGraphs
 };
 
 }  // namespace astg

@@ -643,6 +643,7 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
     merge_ = 2;
     if (const char* g = getenv("ASTG_MERGE")) merge_ = std::max(1, std::min(8, atoi(g)));
     if (mergeLevels_ > 0) merge_ = std::min(8, mergeLevels_);
+    if (const char* g = getenv("ASTG_BATCH_GRAPH")) batchGraph_ = atoi(g) != 0;
     // mixed form: residual replacement period and inner tolerance factor
     if (const char* g = getenv("ASTG_MIXED_REPLACE")) mixedReplace_ = std::max(0, atoi(g));
     if (const char* g = getenv("ASTG_MIXED_ETA")) mixedEta_ = std::max(0.01, atof(g));
@@ -828,6 +829,11 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
     HIPCHK(part_.alloc((size_t)(2 * gs_ + 2 * gb_ + 8) * 8));
     HIPCHK(hipStreamSynchronize(stream_));
     return ASTG_OK;
+}
+
+bool PcgDevice::isSweepKernel(const void* f) const {
+    return f == (const void*)k_fwd<double> || f == (const void*)k_fwd<float> || f == (const void*)k_bwd_dot<double> ||
+           f == (const void*)k_bwd_dot<float>;
 }
 
 SweepPlan PcgDevice::splan() const {

@@ -20,6 +20,8 @@
 // longer changes, so its rows recompute values nobody reads.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "pcg.h"
@@ -352,39 +354,121 @@ int PcgDevice::pcgBatchInner(const T* A, const T* invd, const T* b, double* x, i
     kb_reduce<<<V.K, BS, 0, stream_>>>(V, P_RHO0, norm, scal);
     kb_xpby<T><<<g, BS, 0, stream_>>>(V, z, s, 0);
     HIPCHK(hipGetLastError());
-    int it = 0, batch = 4;
-    for (;;) {
-        for (int j = 0; j < batch && it < maxIt; ++j, ++it) {
+    // one PCG iteration of every running system
+    auto iteration = [&](int itx, bool replace) -> int {
+        if (lpr_ == 4)
+            kb_spmv_dot<T, 4><<<g, BS, 0, stream_>>>(V, rs, col, A, s, q);
+        else if (lpr_ == 16)
+            kb_spmv_dot<T, 16><<<g, BS, 0, stream_>>>(V, rs, col, A, s, q);
+        else
+            kb_spmv_dot<T, 8><<<g, BS, 0, stream_>>>(V, rs, col, A, s, q);
+        kb_reduce<<<V.K, BS, 0, stream_>>>(V, P_SQ, norm, scal);
+        kb_axpy_res<T, double><<<g, BS, 0, stream_>>>(V, s, q, x, r, norm);
+        if (replace) {
+            float* r32 = reinterpret_cast<float*>(r);
             if (lpr_ == 4)
-                kb_spmv_dot<T, 4><<<g, BS, 0, stream_>>>(V, rs, col, A, s, q);
+                kb_replace_res<4><<<g, BS, 0, stream_>>>(V, rs, col, val_.as<double>(), x, b64, r32, norm);
             else if (lpr_ == 16)
-                kb_spmv_dot<T, 16><<<g, BS, 0, stream_>>>(V, rs, col, A, s, q);
+                kb_replace_res<16><<<g, BS, 0, stream_>>>(V, rs, col, val_.as<double>(), x, b64, r32, norm);
             else
-                kb_spmv_dot<T, 8><<<g, BS, 0, stream_>>>(V, rs, col, A, s, q);
-            kb_reduce<<<V.K, BS, 0, stream_>>>(V, P_SQ, norm, scal);
-            kb_axpy_res<T, double><<<g, BS, 0, stream_>>>(V, s, q, x, r, norm);
-            if (sizeof(T) == 4 && replaceEvery > 0 && (it + 1) % replaceEvery == 0) {
-                float* r32 = reinterpret_cast<float*>(r);
-                if (lpr_ == 4)
-                    kb_replace_res<4><<<g, BS, 0, stream_>>>(V, rs, col, val_.as<double>(), x, b64, r32, norm);
-                else if (lpr_ == 16)
-                    kb_replace_res<16><<<g, BS, 0, stream_>>>(V, rs, col, val_.as<double>(), x, b64, r32, norm);
-                else
-                    kb_replace_res<8><<<g, BS, 0, stream_>>>(V, rs, col, val_.as<double>(), x, b64, r32, norm);
-            }
-            kb_reduce<<<V.K, BS, 0, stream_>>>(V, P_RES, norm, scal);
-            if ((rc = launchFwd<T>(invd, r, it + 1, 2, it, maxIt, norm, 0.0, err))) return rc;
-            if ((rc = launchBwd<T>(invd, r, z, it + 1, (it + 1) & 1, err))) return rc;
-            kb_dot<T><<<g, BS, 0, stream_>>>(V, z, r);
-            kb_reduce<<<V.K, BS, 0, stream_>>>(V, P_RHO, norm, scal);
-            kb_xpby<T><<<g, BS, 0, stream_>>>(V, z, s, 1);
+                kb_replace_res<8><<<g, BS, 0, stream_>>>(V, rs, col, val_.as<double>(), x, b64, r32, norm);
         }
-        HIPCHK(hipGetLastError());
-        if ((rc = syncScalars(err))) return rc;
-        if ((rc = checkErrors("batched PCG", err))) return rc;
+        kb_reduce<<<V.K, BS, 0, stream_>>>(V, P_RES, norm, scal);
+        int e = launchFwd<T>(invd, r, itx + 1, 2, itx, maxIt, norm, 0.0, err);
+        if (!e) e = launchBwd<T>(invd, r, z, itx + 1, (itx + 1) & 1, err);
+        if (e) return e;
+        kb_dot<T><<<g, BS, 0, stream_>>>(V, z, r);
+        kb_reduce<<<V.K, BS, 0, stream_>>>(V, P_RHO, norm, scal);
+        kb_xpby<T><<<g, BS, 0, stream_>>>(V, z, s, 1);
+        return ASTG_OK;
+    };
+    auto replaceAt = [&](int itx) {
+        return sizeof(T) == 4 && replaceEvery > 0 && (itx + 1) % replaceEvery == 0;
+    };
+    // Two consecutive iterations captured as one hipGraph: the sweep vectors
+    // alternate every iteration and nothing else depends on the iteration
+    // number. Variant 1 replaces the residual in its second iteration. The
+    // sweep nodes keep the cooperative attribute (all blocks co-resident).
+    hipGraphExec_t pair[2] = {nullptr, nullptr};
+    bool useGraph = batchGraph_;
+    auto buildPair = [&](int variant) -> bool {
+        hipGraph_t gr = nullptr;
+        if (hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            (void)hipGetLastError();
+            return false;
+        }
+        const int c1 = iteration(0, false), c2 = iteration(1, variant == 1);
+        const hipError_t ce = hipStreamEndCapture(stream_, &gr);
+        bool ok = c1 == 0 && c2 == 0 && ce == hipSuccess && gr != nullptr;
+        if (ok) {
+            size_t nn = 0;
+            ok = hipGraphGetNodes(gr, nullptr, &nn) == hipSuccess;
+            std::vector<hipGraphNode_t> nodes(nn);
+            if (ok) ok = hipGraphGetNodes(gr, nodes.data(), &nn) == hipSuccess;
+            for (size_t a = 0; ok && a < nn; ++a) {
+                hipGraphNodeType ty;
+                if (hipGraphNodeGetType(nodes[a], &ty) != hipSuccess || ty != hipGraphNodeTypeKernel) continue;
+                hipKernelNodeParams kp;
+                if (hipGraphKernelNodeGetParams(nodes[a], &kp) != hipSuccess) {
+                    ok = false;
+                    break;
+                }
+                if (!isSweepKernel(kp.func)) continue;
+                hipKernelNodeAttrValue v;
+                std::memset(&v, 0, sizeof v);
+                v.cooperative = 1;
+                ok = hipGraphKernelNodeSetAttribute(nodes[a], hipKernelNodeAttributeCooperative, &v) == hipSuccess;
+            }
+        }
+        if (ok) ok = hipGraphInstantiate(&pair[variant], gr, nullptr, nullptr, 0) == hipSuccess;
+        if (gr) (void)hipGraphDestroy(gr);
+        if (!ok) {
+            pair[variant] = nullptr;
+            (void)hipGetLastError();
+        }
+        return ok;
+    };
+    auto destroyPairs = [&] {
+        for (auto& ex : pair)
+            if (ex) (void)hipGraphExecDestroy(ex);
+    };
+    int it = 0, batch = 4, replays = 0;
+    for (;;) {
+        for (int j = 0; j < batch && it < maxIt;) {
+            const int variant = replaceAt(it + 1) ? 1 : 0;
+            if (useGraph && it % 2 == 0 && it + 2 <= maxIt && j + 2 <= batch && !replaceAt(it)) {
+                if (!pair[variant] && !buildPair(variant)) {
+                    useGraph = false;
+                    continue;
+                }
+                if (hipGraphLaunch(pair[variant], stream_) != hipSuccess) {
+                    destroyPairs();
+                    err = "batched PCG: graph launch failed";
+                    return ASTG_ERR_HIP;
+                }
+                it += 2;
+                j += 2;
+                ++replays;
+                continue;
+            }
+            if ((rc = iteration(it, replaceAt(it)))) {
+                destroyPairs();
+                return rc;
+            }
+            ++it;
+            ++j;
+        }
+        if (hipGetLastError() != hipSuccess || (rc = syncScalars(err)) || (rc = checkErrors("batched PCG", err))) {
+            destroyPairs();
+            return rc ? rc : ASTG_ERR_HIP;
+        }
         if (hscal_[S_DONE] != 0.0 || it >= maxIt) break;
         batch = std::min(batch * 2, 32);
     }
+    destroyPairs();
+    if (getenv("ASTG_TIMING"))
+        fprintf(stderr, "[astg timing] batched PCG: %d iterations, %d two-iteration graph replays%s\n", it, replays,
+                batchGraph_ && !useGraph ? " (capture failed: eager)" : "");
     return ASTG_OK;
 }
 

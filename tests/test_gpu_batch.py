@@ -195,3 +195,39 @@ def test_batch_tuning_shapes_agree_and_tuner_sets_one(goldens):
     assert all(r.converged == 1 for r in res)
     with pytest.raises(astg.AstgError):
         c.set_batch_tuning(2, 8, 6)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_batch_graph_replay_equals_eager(goldens, prec):
+    """Iterations replayed from the captured two-iteration hipGraphs (with the
+    host's convergence checks between replays, and in the mixed form the
+    residual-replacement variant) give the eager loop's results bit for bit."""
+    import os
+    import astg
+    precision = astg.PREC_FP64 if prec == "fp64" else astg.PREC_MIXED
+    # a long inner solve (tolerance 1e-12) so that replacement steps occur
+    prm = astg.default_params(tol=1e-12, max_iterations=3000, precision=precision)
+    systems = [csr_system(g) for g in goldens]
+    out = {}
+    keys = ("ASTG_BATCH_GRAPH", "ASTG_MIXED_REPLACE")
+    old = {k: os.environ.get(k) for k in keys}
+    try:
+        # mixed: replace the residual every 10 iterations, so the graph
+        # variant with a replacement step is replayed too
+        os.environ["ASTG_MIXED_REPLACE"] = "10"
+        for mode in ("0", "1"):
+            os.environ["ASTG_BATCH_GRAPH"] = mode
+            c = astg.Context(0)
+            out[mode] = c.pcg_solve_batch(systems, prm, allow_unconverged=True)
+            c.close()
+    finally:
+        for k in keys:
+            if old[k] is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = old[k]
+    (xe, re_), (xg, rg) = out["0"], out["1"]
+    print("iterations eager %s graph %s" % ([r.iterations for r in re_], [r.iterations for r in rg]), flush=True)
+    for k in range(len(systems)):
+        assert re_[k].iterations == rg[k].iterations and re_[k].converged == rg[k].converged
+        assert np.array_equal(xe[k], xg[k]), k
