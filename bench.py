@@ -312,8 +312,8 @@ def main():
                     help="one projection, then per-kernel timings only (for counter collection)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py: this build runs on one GPU only; multi-GPU sharding of the projection is design-only "
-              "(see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py: runs on one GPU only (the projection's multi-GPU sharding is not built); "
+              "run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import torch  # device memory / stream plumbing only

@@ -2,18 +2,18 @@
 against files the reference itself wrote (tests/golden/frame32.npz, made by
 make_golden_frame.py from the reference build): host-side parts, no GPU.
 
-* The reference's layout file describes exactly the layers this build makes
+* The reference's layout file describes exactly the layers the library makes
   from the same split map (node indices, coordinates, flags, leaf and ghost
   orders, half-tilt list), and a layout from a different split map is refused.
 * Reading the reference's tilt file gives the reference's tilt state, and
   writing it back reproduces every record of the reference's file (the
   reference leaves the records' padding bytes uninitialised; they are not
   compared).
-* The layout this build writes carries the same records and lists as the
+* The layout the library writes carries the same records and lists as the
   reference's (whose map order is its hash map's, plus default records left
   by lookups).
 * Truncated and mismatched files are refused without changing the state.
-* Where the reference build is present, the reference reads this build's
+* Where the reference build is present, the reference reads the library's
   layout and tilt files in its own restart and ends in the same tilt state.
 """
 import os

@@ -6,7 +6,7 @@ astg_pcg_solve_csr on the reference's own matrices.
 
 Tolerances: A, b and the MIC factor within 1e-12 relative (same expression
 trees; the reference is built with -ffast-math, so bit equality is not
-expected); pressure and velocities within 1e-6 relative L2 (north-star bar for
+expected); pressure and velocities within 1e-6 relative L2 (the parity bar for
 the FP64 form); PCG iterations within 1 of the reference's.
 """
 import numpy as np
