@@ -123,6 +123,8 @@ public:
     // MIC(0) factorisation of the current values (FP64).
     int factor(double tau, double ratio, std::string& err);
     // Solves with b = rhs(), x = solution(); both device FP64.
+    // (Both solves retry once with cooperative sweep launches if an ordinary
+    // launch left a sweep block waiting for one that was not resident.)
     int solve(const astg_pcg_params& prm, astg_pcg_result& res, std::string& err);
 
     // Times `reps` launches of one kernel (ASTG_K_*) with events on the stream.
@@ -152,6 +154,7 @@ public:
     // Solves every system of the stack with its own PCG scalars, convergence
     // test and iteration count (rhs(), solution(), values() hold the stack).
     int solveBatch(const astg_pcg_params& prm, std::vector<astg_pcg_result>& res, std::string& err);
+    bool cooperativeSweeps() const { return coop_; }
     const std::vector<int>& hostRowstart() const { return hrs_; }
     const std::vector<int>& hostCol() const { return hcol_; }
 
@@ -178,6 +181,11 @@ private:
     BatchView bview() const;
     int batchGrid() const;
     bool isSweepKernel(const void* f) const;  // k_fwd / k_bwd_dot (cooperative graph nodes)
+    int solveOnce(const astg_pcg_params& prm, astg_pcg_result& res, std::string& err);
+    int solveBatchOnce(const astg_pcg_params& prm, std::vector<astg_pcg_result>& res, std::string& err);
+    // After a spin-limit failure under ordinary sweep launches: cooperative
+    // launches from now on, error bits cleared; true if the solve should be retried.
+    bool fallBackToCooperative(std::string& err);
     int launchMic(double tau, double ratio, std::string& err);
     int launchMerge(std::string& err);
     int convert32(std::string& err);
@@ -231,6 +239,9 @@ private:
     int lpr_ = 8;                           // batched form: SpMV lanes per row
     int mergeLevels_ = 0;                   // requested levels per sweep group (0: default)
     bool batchGraph_ = true;                // batched form: iterations replayed from hipGraphs
+    bool coop_ = false;                     // sweeps through the cooperative launch API (ASTG_COOP=1)
+    bool spinFailed_ = false;               // a sweep hit its spin limit since the last solve started
+    bool forceSpinError_ = false;           // test hook (ASTG_TEST_SPIN_ERROR_ONCE): one forced spin error
 };
 
 }  // namespace astg

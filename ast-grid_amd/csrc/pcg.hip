@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <mutex>
 
@@ -68,6 +69,9 @@ __global__ void k_fill_sent(T* p, int n) {
 __global__ void k_d2f(const double* a, float* b, long n) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) b[i] = (float)a[i];
+}
+__global__ void k_set_err(int* c, int bit) {
+    if (threadIdx.x == 0) atomicOr(&c[C_ERR], bit);
 }
 __global__ void k_zero_counters(int* c) {
     if (threadIdx.x < C_COUNT) c[threadIdx.x] = 0;
@@ -551,8 +555,17 @@ static int coopBlocks(K kernel, int cus, int want) {
     return std::max(1, std::min(want, per * cus));
 }
 
+// Persistent sweep launch. coop: the cooperative API, which guarantees that
+// all blocks are co-resident; otherwise an ordinary launch of the same grid
+// (at most the occupancy limit, so on an otherwise idle device every block is
+// resident at once; if one were not, the sweeps' bounded spin ends the kernel
+// with an error flag instead of waiting for it).
 template <typename K, typename... Args>
-static hipError_t launchCoop(K kernel, int grid, hipStream_t s, Args... args) {
+static hipError_t launchCoop(bool coop, K kernel, int grid, hipStream_t s, Args... args) {
+    if (!coop) {
+        kernel<<<grid, BS, 0, s>>>(args...);
+        return hipGetLastError();
+    }
     void* ptrs[] = {(void*)&args...};
     return hipLaunchCooperativeKernel((const void*)kernel, dim3(grid), dim3(BS), ptrs, 0, s);
 }
@@ -644,6 +657,8 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
     if (const char* g = getenv("ASTG_MERGE")) merge_ = std::max(1, std::min(8, atoi(g)));
     if (mergeLevels_ > 0) merge_ = std::min(8, mergeLevels_);
     if (const char* g = getenv("ASTG_BATCH_GRAPH")) batchGraph_ = atoi(g) != 0;
+    if (const char* g = getenv("ASTG_COOP")) coop_ = atoi(g) != 0;
+    if (const char* g = getenv("ASTG_TEST_SPIN_ERROR_ONCE")) forceSpinError_ = atoi(g) != 0;
     // mixed form: residual replacement period and inner tolerance factor
     if (const char* g = getenv("ASTG_MIXED_REPLACE")) mixedReplace_ = std::max(0, atoi(g));
     if (const char* g = getenv("ASTG_MIXED_ETA")) mixedEta_ = std::max(0.01, atof(g));
@@ -878,10 +893,23 @@ int PcgDevice::checkErrors(const char* where, std::string& err) {
     HIPCHK(hipMemcpyAsync(&bits, cnt_.as<int>() + C_ERR, 4, hipMemcpyDeviceToHost, stream_));
     HIPCHK(hipStreamSynchronize(stream_));
     if (bits) {
+        spinFailed_ = true;
         err = std::string(where) + ": a sync-free kernel hit its spin limit (error bits " + std::to_string(bits) + ")";
         return ASTG_ERR_HIP;
     }
     return ASTG_OK;
+}
+
+bool PcgDevice::fallBackToCooperative(std::string& err) {
+    if (!spinFailed_ || coop_) return false;
+    std::fprintf(stderr, "astg: a sweep block was not resident under an ordinary launch (%s); "
+                         "retrying with cooperative launches\n", err.c_str());
+    coop_ = true;
+    spinFailed_ = false;
+    k_zero_counters<<<1, 64, 0, stream_>>>(cnt_.as<int>());
+    if (hipStreamSynchronize(stream_) != hipSuccess) return false;
+    err.clear();
+    return true;
 }
 
 int PcgDevice::launchMic(double tau, double ratio, std::string& err) {
@@ -945,7 +973,7 @@ int PcgDevice::launchFwd(const T* invd, const T* r, int k, int mode, int it, int
     const int g = fwdGrid<T>(), gb = bwdGrid<T>();
     const StencilView<T> V = view<T>(stF_);
     T* y = ybuf<T>(k);
-    HIPCHK(launchCoop(k_fwd<T>, g, stream_, splan(), V, invd, r, y, zp_.as<T>(), parts(gb), cnt_.as<int>(),
+    HIPCHK(launchCoop(coop_, k_fwd<T>, g, stream_, splan(), V, invd, r, y, zp_.as<T>(), parts(gb), cnt_.as<int>(),
                       scal_.as<double>(), mode, it, maxIt, norm, tolRel));
     return ASTG_OK;
 }
@@ -956,7 +984,7 @@ int PcgDevice::launchBwd(const T* invd, const T* r, T* z, int k, int parityOut, 
     const StencilView<T> V = view<T>(stB_);
     const T* y = ybuf<T>(k);
     T* ynext = ybuf<T>(k + 1);
-    HIPCHK(launchCoop(k_bwd_dot<T>, gb, stream_, splan(), V, invd, y, ynext, r, z, zp_.as<T>(), parts(gb),
+    HIPCHK(launchCoop(coop_, k_bwd_dot<T>, gb, stream_, splan(), V, invd, y, ynext, r, z, zp_.as<T>(), parts(gb),
                       cnt_.as<int>(), (const double*)scal_.as<double>(), parityOut));
     return ASTG_OK;
 }
@@ -981,6 +1009,10 @@ int PcgDevice::pcgInner(const T* A, const T* invd, const T* b, double* x, double
     k_fill_sent<T><<<blocks(2 * PS_), BS, 0, stream_>>>(y_.as<T>(), 2 * PS_);
     if ((rc = launchFwd<T>(invd, r, 0, 0, 0, maxIt, norm, tolRel, err))) return rc;
     if ((rc = launchBwd<T>(invd, r, z, 0, 0, err))) return rc;
+    if (forceSpinError_ && !coop_) {  // test hook: as if a sweep had hit its spin limit
+        forceSpinError_ = false;
+        k_set_err<<<1, 64, 0, stream_>>>(cnt_.as<int>(), E_FWD);
+    }
     k_xpby<T><<<gs, BS, 0, stream_>>>(n, z, s, Pt, scal, 0, 0);
     HIPCHK(hipGetLastError());
     int it = 0;
@@ -1015,6 +1047,13 @@ int PcgDevice::pcgInner(const T* A, const T* invd, const T* b, double* x, double
 }
 
 int PcgDevice::solve(const astg_pcg_params& prm, astg_pcg_result& res, std::string& err) {
+    spinFailed_ = false;
+    int rc = solveOnce(prm, res, err);
+    if (rc != ASTG_OK && fallBackToCooperative(err)) rc = solveOnce(prm, res, err);
+    return rc;
+}
+
+int PcgDevice::solveOnce(const astg_pcg_params& prm, astg_pcg_result& res, std::string& err) {
     const int n = n_, gs = gs_;
     res.n = n;
     res.nnz = nnz_;

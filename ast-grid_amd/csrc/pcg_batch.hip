@@ -473,6 +473,13 @@ int PcgDevice::pcgBatchInner(const T* A, const T* invd, const T* b, double* x, i
 }
 
 int PcgDevice::solveBatch(const astg_pcg_params& prm, std::vector<astg_pcg_result>& res, std::string& err) {
+    spinFailed_ = false;
+    int rc = solveBatchOnce(prm, res, err);
+    if (rc != ASTG_OK && fallBackToCooperative(err)) rc = solveBatchOnce(prm, res, err);
+    return rc;
+}
+
+int PcgDevice::solveBatchOnce(const astg_pcg_params& prm, std::vector<astg_pcg_result>& res, std::string& err) {
     const int K = K_;
     if (K < 1) {
         err = "no batch layout";
