@@ -5,11 +5,14 @@
 //     levels, every group padded so a 64-row wavefront segment holds rows of
 //     one group only. Within a group, rows depend only on earlier groups
 //     (dependencies inside the group are substituted away, sweep_merge.h).
-//   * The sweeps are persistent cooperative kernels: all blocks are
-//     co-resident and block b takes chunks b, b+G, b+2G, ... in order. The
-//     smallest unfinished chunk therefore always belongs to a running block
-//     whose dependencies (earlier chunks) are complete, so the spin-waits
-//     always end. No ticket atomics are needed.
+//   * The sweeps are persistent kernels with a grid at the occupancy limit:
+//     all blocks are co-resident and block b takes chunks b, b+G, b+2G, ...
+//     in order. The smallest unfinished chunk therefore always belongs to a
+//     running block whose dependencies (earlier chunks) are complete, so the
+//     spin-waits end. No ticket atomics are needed. They are launched as
+//     ordinary kernels (an idle device makes such a grid resident at once);
+//     a spin-limit error makes the solve retry with cooperative launches,
+//     which guarantee co-residency.
 //   * A row publishes its value with one agent-scope atomic store over a
 //     signalling-NaN sentinel; dependants poll the values they need, all at
 //     once. A wavefront first waits (one lane, with backoff) on a single

@@ -387,8 +387,8 @@ int PcgDevice::pcgBatchInner(const T* A, const T* invd, const T* b, double* x, i
     };
     // Two consecutive iterations captured as one hipGraph: the sweep vectors
     // alternate every iteration and nothing else depends on the iteration
-    // number. Variant 1 replaces the residual in its second iteration. The
-    // sweep nodes keep the cooperative attribute (all blocks co-resident).
+    // number. Variant 1 replaces the residual in its second iteration. Under
+    // cooperative launches the sweep nodes carry the cooperative attribute.
     hipGraphExec_t pair[2] = {nullptr, nullptr};
     bool useGraph = batchGraph_;
     auto buildPair = [&](int variant) -> bool {
@@ -413,7 +413,7 @@ int PcgDevice::pcgBatchInner(const T* A, const T* invd, const T* b, double* x, i
                     ok = false;
                     break;
                 }
-                if (!isSweepKernel(kp.func)) continue;
+                if (!coop_ || !isSweepKernel(kp.func)) continue;
                 hipKernelNodeAttrValue v;
                 std::memset(&v, 0, sizeof v);
                 v.cooperative = 1;
