@@ -849,35 +849,40 @@ __global__ void k_adv_tilt(Topo T, const double* teDst, const uint8_t* level, Ve
     tilt1[(size_t)k * c_G + g] = v[0] * d[0] + v[1] * d[1] + v[2] * d[2];
 }
 
+// One lane per (half-tilt, slot): the five slots' back-traces are independent
+// chains of gathers; the first slot's lane also clears the three slots a
+// half-tilt does not carry.
 __global__ void k_adv_half(Topo T, const double* teDst, const uint8_t* level, Vel V, const int* halfList, int n,
                            double dt, double* tilt1) {
-    const int t = blockIdx.x * BS + threadIdx.x;
-    if (t >= n) return;
-    const int g = halfList[t];
+    const long t = (long)blockIdx.x * BS + threadIdx.x;
+    if (t >= 5L * n) return;
+    const int g = halfList[t / 5], i = (int)(t % 5);
     const int dir = T.thd[g];
     const int l = level[g];
     const double h = c_hl[l];
     int x, y, z;
     nodeXYZ(T, g, x, y, z);
     const double c[3] = {x * h, y * h, z * h};
-    double nv[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int i = 0; i < 5; ++i) {
-        const int k = c_halfSlot[dir][i];
-        double p[3], q[3], v[3], d[3];
-        if (i == 4) {
-            p[0] = c[0], p[1] = c[1], p[2] = c[2];
-            // HALF_TILT_XYZ_DIR_POSITIVE[k]
-            const int ax = (k == 0 || k == 1) ? 0 : (k == 2 || k == 4) ? 1 : 2;
-            d[0] = ax == 0, d[1] = ax == 1, d[2] = ax == 2;
-        } else {
-            tiltPos(c, k, teDst[g], p);
-            tiltDir(k, d);
-        }
-        backTrace(T, V, p, dt, q);
-        velSample(T, V, q[0], q[1], q[2], v);
-        nv[k] = v[0] * d[0] + v[1] * d[1] + v[2] * d[2];
+    const int k = c_halfSlot[dir][i];
+    double p[3], q[3], v[3], d[3];
+    if (i == 4) {
+        p[0] = c[0], p[1] = c[1], p[2] = c[2];
+        // HALF_TILT_XYZ_DIR_POSITIVE[k]
+        const int ax = (k == 0 || k == 1) ? 0 : (k == 2 || k == 4) ? 1 : 2;
+        d[0] = ax == 0, d[1] = ax == 1, d[2] = ax == 2;
+    } else {
+        tiltPos(c, k, teDst[g], p);
+        tiltDir(k, d);
     }
-    for (int k = 0; k < 8; ++k) tilt1[(size_t)k * c_G + g] = nv[k];
+    backTrace(T, V, p, dt, q);
+    velSample(T, V, q[0], q[1], q[2], v);
+    tilt1[(size_t)k * c_G + g] = v[0] * d[0] + v[1] * d[1] + v[2] * d[2];
+    if (i == 0) {
+        unsigned used = 0;
+        for (int j = 0; j < 5; ++j) used |= 1u << c_halfSlot[dir][j];
+        for (int kk = 0; kk < 8; ++kk)
+            if (!((used >> kk) & 1u)) tilt1[(size_t)kk * c_G + g] = 0.0;
+    }
 }
 
 // adjustE's decision (octree_omnigrid_solver3.cpp:1131-1149) for every leaf
@@ -1023,29 +1028,28 @@ __global__ void k_force_tilt(Topo T, const uint8_t* level, const int* openTilt, 
 __global__ void k_force_half(Topo T, const uint8_t* level, const int* half, int n, const double* dC,
                              const double* dV, const double* hC, const double* hV, double fd, double ft, double ambT,
                              double dt, double* tilt) {
-    const int t = blockIdx.x * BS + threadIdx.x;
-    if (t >= n) return;
-    const int g = half[t];
+    // one lane per (half-tilt, slot), as in k_adv_half
+    const long t = (long)blockIdx.x * BS + threadIdx.x;
+    if (t >= 5L * n) return;
+    const int g = half[t / 5], i = (int)(t % 5);
     const int dir = T.thd[g];
     const double h = c_hl[level[g]];
     int x, y, z;
     nodeXYZ(T, g, x, y, z);
     const double c[3] = {x * h, y * h, z * h};
-    for (int i = 0; i < 5; ++i) {
-        const int k = c_halfSlot[dir][i];
-        double p[3], dz;
-        if (i == 4) {
-            p[0] = c[0], p[1] = c[1], p[2] = c[2];
-            dz = (k == 0 || k == 1 || k == 2 || k == 4) ? 0.0 : 1.0;  // HALF_TILT_XYZ_DIR_POSITIVE[k].z
-        } else {
-            double d[3];
-            tiltPos(c, k, T.te[g], p);
-            tiltDir(k, d);
-            dz = d[2];
-        }
-        double* u = tilt + (size_t)k * c_G + g;
-        *u = *u + dt * (buoyancy(T, dC, dV, hC, hV, fd, ft, ambT, p) * dz);
+    const int k = c_halfSlot[dir][i];
+    double p[3], dz;
+    if (i == 4) {
+        p[0] = c[0], p[1] = c[1], p[2] = c[2];
+        dz = (k == 0 || k == 1 || k == 2 || k == 4) ? 0.0 : 1.0;  // HALF_TILT_XYZ_DIR_POSITIVE[k].z
+    } else {
+        double d[3];
+        tiltPos(c, k, T.te[g], p);
+        tiltDir(k, d);
+        dz = d[2];
     }
+    double* u = tilt + (size_t)k * c_G + g;
+    *u = *u + dt * (buoyancy(T, dC, dV, hC, hV, fd, ft, ambT, p) * dz);
 }
 
 __global__ void k_scale2(long n, double s, double* a, double* b) {
@@ -1530,7 +1534,7 @@ int AdvectDevice::advect(ProjectDevice& pd, double dt, std::string& err) {
         k_adv_tilt<<<nblocks(8L * nVelTilt_), BS, 0, stream_>>>(T, te_.as<double>(), lev, V, velTilt_.as<int>(),
                                                                  nVelTilt_, dt, tTilt_.as<double>());
     if (nHalf_)
-        k_adv_half<<<nblocks(nHalf_), BS, 0, stream_>>>(T, te_.as<double>(), lev, V, half_.as<int>(), nHalf_, dt,
+        k_adv_half<<<nblocks(5L * nHalf_), BS, 0, stream_>>>(T, te_.as<double>(), lev, V, half_.as<int>(), nHalf_, dt,
                                                          tTilt_.as<double>());
     HIPCHK(hipGetLastError());
     pd.swapVelocity(tMac_, tTilt_);
@@ -1619,7 +1623,7 @@ int AdvectDevice::addForce(ProjectDevice& pd, double dt, double fd, double ft, s
         k_force_tilt<<<nblocks(8L * nOpenTilt_), BS, 0, stream_>>>(T, lev, openTilt_.as<int>(), nOpenTilt_, dC, dV, hC,
                                                                    hV, fd, ft, ambT, dt, pd.uTilt());
     if (nHalf_)
-        k_force_half<<<nblocks(nHalf_), BS, 0, stream_>>>(T, lev, half_.as<int>(), nHalf_, dC, dV, hC, hV, fd, ft, ambT,
+        k_force_half<<<nblocks(5L * nHalf_), BS, 0, stream_>>>(T, lev, half_.as<int>(), nHalf_, dC, dV, hC, hV, fd, ft, ambT,
                                                           dt, pd.uTilt());
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(stream_));
