@@ -5,6 +5,8 @@
 
 #include <cstdint>
 
+#include "sweep_merge.h"
+
 namespace astg {
 
 // Unpublished-value sentinels: signalling-NaN bit patterns. GPU arithmetic
@@ -90,8 +92,7 @@ constexpr int kGateLimit = 1 << 16;
 // A row's dependencies held in registers: column indices and factor values
 // are loaded before the row waits (they do not depend on other rows), then
 // all published values are polled together, one round trip per poll. N is
-// the register capacity.
-constexpr int kMaxDeps = 16;
+// the register capacity (kMaxDeps, sweep_merge.h, by default).
 template <typename T, int N = kMaxDeps>
 struct DepRow {
     int nd;

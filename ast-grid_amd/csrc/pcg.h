@@ -62,8 +62,7 @@ struct StencilView {
 };
 struct DevStencil {
     DevBuf start, nr, idx, ustart, upd, coef, coef32;
-    long long E = 0, R = 0, kept = 0;
-    int maxY = 0;  // largest number of y-terms of a row
+    long long E = 0;  // entries (r-terms plus y-terms) over all rows
 };
 
 // Per-block partials of the reductions, consumed by the next kernel.
@@ -186,6 +185,10 @@ private:
     // After a spin-limit failure under ordinary sweep launches: cooperative
     // launches from now on, error bits cleared; true if the solve should be retried.
     bool fallBackToCooperative(std::string& err);
+    // symbolic plans built on the device from the uploaded pattern
+    // (plan_device.hip); the host planners serve when hostPlan_ is set
+    int planMicDevice(std::string& err);
+    int planStencilDevice(int dir, int maxY, DevStencil& D, std::string& err);
     int launchMic(double tau, double ratio, std::string& err);
     int launchMerge(std::string& err);
     int convert32(std::string& err);
@@ -238,6 +241,7 @@ private:
     int blocksPerCu_ = 0;                   // batched form: streaming grid (0: gs_)
     int lpr_ = 8;                           // batched form: SpMV lanes per row
     int mergeLevels_ = 0;                   // requested levels per sweep group (0: default)
+    bool hostPlan_ = false;                 // ASTG_HOST_PLAN: symbolic plans on the host
     bool batchGraph_ = true;                // batched form: iterations replayed from hipGraphs
     bool coop_ = false;                     // sweeps through the cooperative launch API (ASTG_COOP=1)
     bool spinFailed_ = false;               // a sweep hit its spin limit since the last solve started

@@ -662,6 +662,8 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
     if (const char* g = getenv("ASTG_BATCH_GRAPH")) batchGraph_ = atoi(g) != 0;
     if (const char* g = getenv("ASTG_COOP")) coop_ = atoi(g) != 0;
     if (const char* g = getenv("ASTG_TEST_SPIN_ERROR_ONCE")) forceSpinError_ = atoi(g) != 0;
+    hostPlan_ = false;
+    if (const char* g = getenv("ASTG_HOST_PLAN")) hostPlan_ = atoi(g) != 0;
     // mixed form: residual replacement period and inner tolerance factor
     if (const char* g = getenv("ASTG_MIXED_REPLACE")) mixedReplace_ = std::max(0, atoi(g));
     if (const char* g = getenv("ASTG_MIXED_ETA")) mixedEta_ = std::max(0.01, atof(g));
@@ -791,15 +793,18 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
             levRange_[2 * L + 1] = last[L] + 1;
         }
     }
-    {
+    if (hostPlan_) {
         std::vector<long long> ms;
         std::vector<int> mp;
         planMic(n, rowstart, lend.data(), ubeg.data(), col, ms, mp);
         HIPCHK(upload(micStart_, ms.data(), ms.size() * 8));
         HIPCHK(upload(micProg_, mp.data(), mp.size() * 4));
-        HIPCHK(live_.alloc((size_t)n));
         HIPCHK(hipStreamSynchronize(stream_));
+    } else {
+        const int rc = planMicDevice(err);
+        if (rc) return rc;
     }
+    HIPCHK(live_.alloc((size_t)n));
     tm.mark("factorisation plan");
     // sweeps: q levels per group, merged stencils (sweep_merge.h)
     buildOrder(merge_, gateDist_, padded, gF, gB);
@@ -807,23 +812,24 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
     HIPCHK(upload(sorder_, padded.data(), (size_t)PS_ * 4));
     HIPCHK(upload(sgateF_, gF.data(), gF.size() * 4));
     HIPCHK(upload(sgateB_, gB.data(), gB.size() * 4));
-    {
+    const int maxY = kMaxDeps;
+    DevStencil* dst[2] = {&stF_, &stB_};
+    if (!hostPlan_) {
+        for (int dir = 0; dir < 2; ++dir) {
+            const int rc = planStencilDevice(dir, maxY, *dst[dir], err);
+            if (rc) return rc;
+        }
+    } else {
         std::vector<int> lev(n), pos(n);
         for (int q = 0; q < n; ++q) lev[sortedRow[q]] = sortedLev[q];
         for (int q = 0; q < PS_; ++q)
             if (padded[q] >= 0) pos[padded[q]] = q;
-        const int maxY = kMaxDeps;
-        DevStencil* dst[2] = {&stF_, &stB_};
         for (int dir = 0; dir < 2; ++dir) {
             SweepStencil st;
             planStencil(dir, n, rowstart, lend.data(), ubeg.data(), col, lev.data(), merge_, maxY, padded.data(),
                         PS_, pos.data(), st);
             DevStencil& D = *dst[dir];
             D.E = (long long)st.idx.size();
-            D.R = st.rTerms;
-            D.kept = st.kept;
-            D.maxY = 0;
-            for (int t = 0; t < PS_; ++t) D.maxY = std::max(D.maxY, st.start[t + 1] - st.start[t] - st.nr[t]);
             HIPCHK(upload(D.start, st.start.data(), st.start.size() * 4));
             HIPCHK(upload(D.nr, st.nr.data(), st.nr.size() * 4));
             HIPCHK(upload(D.idx, st.idx.data(), st.idx.size() * 4));
@@ -833,9 +839,9 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
             // the host vectors must outlive the copies
             HIPCHK(hipStreamSynchronize(stream_));
         }
-        HIPCHK(y_.alloc((size_t)PS_ * 2 * 8));
-        HIPCHK(zp_.alloc((size_t)PS_ * 8));
     }
+    HIPCHK(y_.alloc((size_t)PS_ * 2 * 8));
+    HIPCHK(zp_.alloc((size_t)PS_ * 8));
     tm.mark("sweep order and merged stencils");
     // persistent grids (all blocks co-resident)
     const int schunks = PS_ / BS;

@@ -35,6 +35,9 @@ struct SweepStencil {
     long long rTerms = 0, yTerms = 0, kept = 0;
 };
 
+// Most dependencies of a row on one side of the diagonal, and so most y-terms
+// of a merged row: the sweep kernels hold a row's dependencies in registers.
+constexpr int kMaxDeps = 16;
 // Most r-terms per row; with at most maxY y-terms, a row's slots fit in a byte.
 constexpr int kMaxRTerms = 64;
 
