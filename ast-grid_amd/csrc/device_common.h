@@ -36,7 +36,11 @@ enum ScalarSlot {
     S_COUNT = 16
 };
 // int counters in the counter block
-enum CounterSlot { C_TICKET_F = 0, C_TICKET_B, C_TICKET_L, C_TICKET_M, C_DONE_SQ, C_DONE_RES, C_DONE_RHO, C_DONE_B, C_ERR, C_COUNT = 16 };
+enum CounterSlot {
+    C_TICKET_F = 0, C_TICKET_B, C_TICKET_L, C_TICKET_M, C_DONE_SQ, C_DONE_RES, C_DONE_RHO, C_DONE_B, C_ERR,
+    C_PATTERN,  // set up: the pattern (or a symbolic plan) breaks a structural rule
+    C_COUNT = 16
+};
 // Sticky error bits in cnt[C_ERR] (atomicOr): a spin limit was hit.
 enum ErrBit { E_LEVELS = 1, E_MIC = 2, E_FWD = 4, E_BWD = 8 };
 

@@ -185,10 +185,10 @@ private:
     // After a spin-limit failure under ordinary sweep launches: cooperative
     // launches from now on, error bits cleared; true if the solve should be retried.
     bool fallBackToCooperative(std::string& err);
-    // symbolic plans built on the device from the uploaded pattern
-    // (plan_device.hip); the host planners serve when hostPlan_ is set
-    int planMicDevice(std::string& err);
-    int planStencilDevice(int dir, int maxY, DevStencil& D, std::string& err);
+    // symbolic plans (factorisation programs, both merged stencils) built on
+    // the device from the uploaded pattern and sweep order (plan_device.hip);
+    // the host planners serve when hostPlan_ is set
+    int planDevice(int maxY, std::string& err);
     int launchMic(double tau, double ratio, std::string& err);
     int launchMerge(std::string& err);
     int convert32(std::string& err);
@@ -225,6 +225,9 @@ private:
     double mixedEta_ = 0.9;                 // mixed form: inner tolerance / outer target
     DevBuf scal_, cnt_, part_;              // device scalars, counters, partials
     double* hscal_ = nullptr;               // pinned mirror of the scalars
+    long long* hplan_ = nullptr;            // pinned: set-up counters and plan sizes
+    DevBuf sortKeys_, sortIdx_;             // set-up: level sort keys and row indices
+    DevBuf planLen_, planPos_, planCnt_, planTmp_;  // set-up: plan counts, sweep positions, scan scratch
     int P_ = 0;                             // padded length of the level order
     int PS_ = 0;                            // padded length of the sweep order
     int merge_ = 2;                         // levels per super-level of the sweeps

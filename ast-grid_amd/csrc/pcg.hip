@@ -101,6 +101,7 @@ __global__ void __launch_bounds__(BS) k_copy64(int n, const double* __restrict__
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(BS) k_levels(int n, const int* rs, const int* lend, const int* col, int* lev,
                                                int* cnt) {
+    if (cnt[C_PATTERN]) return;  // invalid pattern (k_split_rows ran earlier in the stream)
     __shared__ int chunk;
     if (threadIdx.x == 0) chunk = atomicAdd(&cnt[C_TICKET_L], 1);
     __syncthreads();
@@ -127,6 +128,50 @@ __global__ void __launch_bounds__(BS) k_levels(int n, const int* rs, const int* 
     }
     for (int o = 32; o > 0; o >>= 1) L = max(L, __shfl_xor(L, o, 64));
     if (lane == 0) st_agent_i32(&lev[i], L);
+}
+
+// Split of row i at the diagonal (lend, ubeg), transpose map (tpos: position
+// of (j,i) for every off-diagonal (i,j)) and structural checks: sorted rows,
+// columns in range, symmetry, at most kMaxDeps entries on each side of the
+// diagonal. A failing row sets C_PATTERN. One lane per row.
+__global__ void __launch_bounds__(BS) k_split_rows(int n, const int* __restrict__ rs, const int* __restrict__ col,
+                                                   int* __restrict__ lend, int* __restrict__ ubeg,
+                                                   int* __restrict__ tpos, int* cnt) {
+    const int i = (int)(blockIdx.x * (long)blockDim.x + threadIdx.x);
+    if (i >= n) return;
+    const int p0 = rs[i], e = rs[i + 1];
+    bool bad = false;
+    for (int q = p0 + 1; q < e; ++q)
+        if (col[q] <= col[q - 1]) bad = true;
+    int p = p0;
+    while (p < e && col[p] < i) ++p;
+    const int ub = (p < e && col[p] == i) ? p + 1 : p;
+    lend[i] = p;
+    ubeg[i] = ub;
+    for (int q = p0; q < e && !bad; ++q) {
+        const int j = col[q];
+        if (j < 0 || j >= n) {
+            bad = true;
+            break;
+        }
+        if (j == i) continue;
+        int lo = rs[j], hi = rs[j + 1];
+        const int end = hi;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (col[mid] < i)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        if (lo == end || col[lo] != i) {
+            bad = true;
+            break;
+        }
+        tpos[q] = lo;
+    }
+    if (p - p0 > kMaxDeps || e - ub > kMaxDeps) bad = true;
+    if (bad) atomicOr(&cnt[C_PATTERN], 1);
 }
 
 // One lane of a wavefront polls the published value of a gate row (advisory
@@ -547,6 +592,7 @@ __global__ void __launch_bounds__(BS) k_replace_res(int n, const int* rs, const 
 // ---------------------------------------------------------------------------
 PcgDevice::~PcgDevice() {
     if (hscal_) (void)hipHostFree(hscal_);
+    if (hplan_) (void)hipHostFree(hplan_);
 }
 
 static inline int blocks(long n) { return (int)((n + BS - 1) / BS); }
@@ -573,70 +619,63 @@ static hipError_t launchCoop(bool coop, K kernel, int grid, hipStream_t s, Args.
     return hipLaunchCooperativeKernel((const void*)kernel, dim3(grid), dim3(BS), ptrs, 0, s);
 }
 
-int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::string& err) {
-    if (n <= 0) {
-        err = "empty system";
-        return ASTG_ERR_ARG;
-    }
-    PhaseTimer tm("setPattern");
-    n_ = n;
-    nnz_ = rowstart[n];
-    hrs_.assign(rowstart, rowstart + n + 1);
-    hcol_.assign(col, col + nnz_);
-    // split each row at the diagonal; build the transpose map and check
-    // symmetry (rows in parallel; the first failing row is reported)
-    std::vector<int> lend(n), ubeg(n), tpos(nnz_, 0);
+// Split at the diagonal and structural checks on the host threads: sorted
+// rows, columns in range, structural symmetry, at most kMaxDeps entries on
+// each side of the diagonal. The first failing row is reported. Serves the
+// host plans (which need lend / ubeg) and words the error when the device
+// check (k_split_rows) fails.
+static int splitPatternHost(int n, const int* rowstart, const int* col, std::vector<int>& lend,
+                            std::vector<int>& ubeg, std::string& err) {
+    lend.assign(n, 0);
+    ubeg.assign(n, 0);
     int badRow = n, badKind = 0, badCol = 0;
-    {
-        std::mutex mu;
-        parallelFor(n, [&](long i0, long i1) {
-            int row = n, kind = 0, cb = 0;
-            for (int i = (int)i0; i < (int)i1 && row == n; ++i) {
-                int p = rowstart[i];
-                const int e = rowstart[i + 1];
-                for (int q = p + 1; q < e; ++q)
-                    if (col[q] <= col[q - 1]) {
-                        row = i;
-                        kind = 1;
-                        break;
-                    }
-                if (row != n) break;
-                while (p < e && col[p] < i) ++p;
-                lend[i] = p;
-                ubeg[i] = (p < e && col[p] == i) ? p + 1 : p;
-                for (int q = rowstart[i]; q < e; ++q) {
-                    const int j = col[q];
-                    if (j < 0 || j >= n) {
-                        row = i;
-                        kind = 2;
-                        break;
-                    }
-                    if (j == i) continue;
-                    const int* b = col + rowstart[j];
-                    const int* f = std::lower_bound(b, col + rowstart[j + 1], i);
-                    if (f == col + rowstart[j + 1] || *f != i) {
-                        row = i;
-                        kind = 3;
-                        cb = j;
-                        break;
-                    }
-                    tpos[q] = (int)(f - col);
-                }
-                if (row == n && (lend[i] - rowstart[i] > kMaxDeps || rowstart[i + 1] - ubeg[i] > kMaxDeps)) {
+    std::mutex mu;
+    parallelFor(n, [&](long i0, long i1) {
+        int row = n, kind = 0, cb = 0;
+        for (int i = (int)i0; i < (int)i1 && row == n; ++i) {
+            int p = rowstart[i];
+            const int e = rowstart[i + 1];
+            for (int q = p + 1; q < e; ++q)
+                if (col[q] <= col[q - 1]) {
                     row = i;
-                    kind = 4;
+                    kind = 1;
+                    break;
+                }
+            if (row != n) break;
+            while (p < e && col[p] < i) ++p;
+            lend[i] = p;
+            ubeg[i] = (p < e && col[p] == i) ? p + 1 : p;
+            for (int q = rowstart[i]; q < e; ++q) {
+                const int j = col[q];
+                if (j < 0 || j >= n) {
+                    row = i;
+                    kind = 2;
+                    break;
+                }
+                if (j == i) continue;
+                const int* b = col + rowstart[j];
+                const int* f = std::lower_bound(b, col + rowstart[j + 1], i);
+                if (f == col + rowstart[j + 1] || *f != i) {
+                    row = i;
+                    kind = 3;
+                    cb = j;
+                    break;
                 }
             }
-            if (row != n) {
-                std::lock_guard<std::mutex> g(mu);
-                if (row < badRow) {
-                    badRow = row;
-                    badKind = kind;
-                    badCol = cb;
-                }
+            if (row == n && (lend[i] - rowstart[i] > kMaxDeps || rowstart[i + 1] - ubeg[i] > kMaxDeps)) {
+                row = i;
+                kind = 4;
             }
-        });
-    }
+        }
+        if (row != n) {
+            std::lock_guard<std::mutex> g(mu);
+            if (row < badRow) {
+                badRow = row;
+                badKind = kind;
+                badCol = cb;
+            }
+        }
+    });
     if (badRow < n) {
         const int i = badRow;
         switch (badKind) {
@@ -652,6 +691,19 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
         }
         return ASTG_ERR_ARG;
     }
+    return ASTG_OK;
+}
+
+int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::string& err) {
+    if (n <= 0) {
+        err = "empty system";
+        return ASTG_ERR_ARG;
+    }
+    PhaseTimer tm("setPattern");
+    n_ = n;
+    nnz_ = rowstart[n];
+    hrs_.assign(rowstart, rowstart + n + 1);
+    hcol_.assign(col, col + nnz_);
     if (const char* g = getenv("ASTG_GATE_DIST")) gateDist_ = std::max(1, atoi(g));
     if (const char* g = getenv("ASTG_POLL_CAP")) pollCap_ = std::max(1, atoi(g));
     if (const char* g = getenv("ASTG_GATE_CAP")) gateCap_ = std::max(1, atoi(g));
@@ -694,36 +746,42 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
     HIPCHK(scal_.alloc(32 * 8));
     HIPCHK(cnt_.alloc(C_COUNT * 4));
     if (!hscal_) HIPCHK(hipHostMalloc((void**)&hscal_, 32 * 8, hipHostMallocDefault));
+    if (!hplan_) HIPCHK(hipHostMalloc((void**)&hplan_, 16 * 8, hipHostMallocDefault));
     HIPCHK(hipMemcpyAsync(rs_.p, rowstart, ((size_t)n + 1) * 4, hipMemcpyHostToDevice, stream_));
     HIPCHK(hipMemcpyAsync(col_.p, col, (size_t)nnz_ * 4, hipMemcpyHostToDevice, stream_));
-    HIPCHK(hipMemcpyAsync(lend_.p, lend.data(), n4, hipMemcpyHostToDevice, stream_));
-    HIPCHK(hipMemcpyAsync(ubeg_.p, ubeg.data(), n4, hipMemcpyHostToDevice, stream_));
-    HIPCHK(hipMemcpyAsync(tpos_.p, tpos.data(), (size_t)nnz_ * 4, hipMemcpyHostToDevice, stream_));
     k_zero_counters<<<1, 64, 0, stream_>>>(cnt_.as<int>());
     k_zero64<<<1, BS, 0, stream_>>>(32, scal_.as<double>());
-    // dependency levels on the device
+    // split at the diagonal, transpose map and structural checks
+    k_split_rows<<<blocks(n), BS, 0, stream_>>>(n, rs_.as<int>(), col_.as<int>(), lend_.as<int>(), ubeg_.as<int>(),
+                                                tpos_.as<int>(), cnt_.as<int>());
+    // dependency levels on the device (skipped if the check failed)
     k_fill_i32<<<blocks(n), BS, 0, stream_>>>(lev_.as<int>(), kSentLevel, n);
     k_levels<<<(n + BS / 64 - 1) / (BS / 64), BS, 0, stream_>>>(n, rs_.as<int>(), lend_.as<int>(), col_.as<int>(),
                                                                 lev_.as<int>(), cnt_.as<int>());
     HIPCHK(hipGetLastError());
     // stable sort of (level, row): rows in level order, ties by index
-    DevBuf keysOut, idxIn, tmp;
-    HIPCHK(keysOut.alloc(n4));
-    HIPCHK(idxIn.alloc(n4));
-    k_iota<<<blocks(n), BS, 0, stream_>>>(idxIn.as<int>(), n);
+    HIPCHK(sortKeys_.alloc(n4));
+    HIPCHK(sortIdx_.alloc(n4));
+    k_iota<<<blocks(n), BS, 0, stream_>>>(sortIdx_.as<int>(), n);
     size_t tmpBytes = 0;
-    HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmpBytes, lev_.as<int>(), keysOut.as<int>(), idxIn.as<int>(),
-                                              order_.as<int>(), n, 0, 31, stream_));
-    HIPCHK(tmp.alloc(tmpBytes));
-    HIPCHK(hipcub::DeviceRadixSort::SortPairs(tmp.p, tmpBytes, lev_.as<int>(), keysOut.as<int>(), idxIn.as<int>(),
-                                              order_.as<int>(), n, 0, 31, stream_));
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmpBytes, lev_.as<int>(), sortKeys_.as<int>(),
+                                              sortIdx_.as<int>(), order_.as<int>(), n, 0, 31, stream_));
+    HIPCHK(planTmp_.alloc(tmpBytes));
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(planTmp_.p, tmpBytes, lev_.as<int>(), sortKeys_.as<int>(),
+                                              sortIdx_.as<int>(), order_.as<int>(), n, 0, 31, stream_));
     std::vector<int> sortedLev(n), sortedRow(n);
-    HIPCHK(hipMemcpyAsync(sortedLev.data(), keysOut.p, n4, hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipMemcpyAsync(sortedLev.data(), sortKeys_.p, n4, hipMemcpyDeviceToHost, stream_));
     HIPCHK(hipMemcpyAsync(sortedRow.data(), order_.p, n4, hipMemcpyDeviceToHost, stream_));
-    int errBits = 0;
-    HIPCHK(hipMemcpyAsync(&errBits, cnt_.as<int>() + C_ERR, 4, hipMemcpyDeviceToHost, stream_));
+    int* hcnt = reinterpret_cast<int*>(hplan_);
+    HIPCHK(hipMemcpyAsync(hcnt, cnt_.p, C_COUNT * 4, hipMemcpyDeviceToHost, stream_));
     HIPCHK(hipStreamSynchronize(stream_));
-    if (errBits) {
+    if (hcnt[C_PATTERN]) {
+        std::vector<int> lend, ubeg;
+        if (splitPatternHost(n, rowstart, col, lend, ubeg, err)) return ASTG_ERR_ARG;
+        err = "the device pattern check failed where the host check passes";
+        return ASTG_ERR_HIP;
+    }
+    if (hcnt[C_ERR]) {
         err = "level computation did not complete";
         return ASTG_ERR_HIP;
     }
@@ -771,10 +829,10 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
         if (e != hipSuccess) return e;
         return hipMemcpyAsync(d.p, src, bytes, hipMemcpyHostToDevice, stream_);
     };
-    // factorisation: rows in level order, one launch per level
+    // factorisation: rows in level order, one launch per level. The host
+    // vectors uploaded below live until the synchronisation at the end.
     std::vector<int> padded, gF, gB;
     buildOrder(1, 1, padded, gF, gB);
-    tm.mark("level order");
     P_ = (int)padded.size();
     HIPCHK(upload(order_, padded.data(), (size_t)P_ * 4));
     {
@@ -793,40 +851,36 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
             levRange_[2 * L + 1] = last[L] + 1;
         }
     }
-    if (hostPlan_) {
+    // sweeps: q levels per group
+    std::vector<int> spadded, sgF, sgB;
+    buildOrder(merge_, gateDist_, spadded, sgF, sgB);
+    PS_ = (int)spadded.size();
+    HIPCHK(upload(sorder_, spadded.data(), (size_t)PS_ * 4));
+    HIPCHK(upload(sgateF_, sgF.data(), sgF.size() * 4));
+    HIPCHK(upload(sgateB_, sgB.data(), sgB.size() * 4));
+    tm.mark("level and sweep orders");
+    // symbolic plans: factorisation programs and merged stencils
+    const int maxY = kMaxDeps;
+    if (!hostPlan_) {
+        const int rc = planDevice(maxY, err);
+        if (rc) return rc;
+    } else {
+        std::vector<int> lend, ubeg;
+        if (splitPatternHost(n, rowstart, col, lend, ubeg, err)) return ASTG_ERR_ARG;
         std::vector<long long> ms;
         std::vector<int> mp;
         planMic(n, rowstart, lend.data(), ubeg.data(), col, ms, mp);
         HIPCHK(upload(micStart_, ms.data(), ms.size() * 8));
         HIPCHK(upload(micProg_, mp.data(), mp.size() * 4));
         HIPCHK(hipStreamSynchronize(stream_));
-    } else {
-        const int rc = planMicDevice(err);
-        if (rc) return rc;
-    }
-    HIPCHK(live_.alloc((size_t)n));
-    tm.mark("factorisation plan");
-    // sweeps: q levels per group, merged stencils (sweep_merge.h)
-    buildOrder(merge_, gateDist_, padded, gF, gB);
-    PS_ = (int)padded.size();
-    HIPCHK(upload(sorder_, padded.data(), (size_t)PS_ * 4));
-    HIPCHK(upload(sgateF_, gF.data(), gF.size() * 4));
-    HIPCHK(upload(sgateB_, gB.data(), gB.size() * 4));
-    const int maxY = kMaxDeps;
-    DevStencil* dst[2] = {&stF_, &stB_};
-    if (!hostPlan_) {
-        for (int dir = 0; dir < 2; ++dir) {
-            const int rc = planStencilDevice(dir, maxY, *dst[dir], err);
-            if (rc) return rc;
-        }
-    } else {
         std::vector<int> lev(n), pos(n);
         for (int q = 0; q < n; ++q) lev[sortedRow[q]] = sortedLev[q];
         for (int q = 0; q < PS_; ++q)
-            if (padded[q] >= 0) pos[padded[q]] = q;
+            if (spadded[q] >= 0) pos[spadded[q]] = q;
+        DevStencil* dst[2] = {&stF_, &stB_};
         for (int dir = 0; dir < 2; ++dir) {
             SweepStencil st;
-            planStencil(dir, n, rowstart, lend.data(), ubeg.data(), col, lev.data(), merge_, maxY, padded.data(),
+            planStencil(dir, n, rowstart, lend.data(), ubeg.data(), col, lev.data(), merge_, maxY, spadded.data(),
                         PS_, pos.data(), st);
             DevStencil& D = *dst[dir];
             D.E = (long long)st.idx.size();
@@ -840,9 +894,10 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
             HIPCHK(hipStreamSynchronize(stream_));
         }
     }
+    HIPCHK(live_.alloc((size_t)n));
     HIPCHK(y_.alloc((size_t)PS_ * 2 * 8));
     HIPCHK(zp_.alloc((size_t)PS_ * 8));
-    tm.mark("sweep order and merged stencils");
+    tm.mark("symbolic plans");
     // persistent grids (all blocks co-resident)
     const int schunks = PS_ / BS;
     gF64_ = coopBlocks(k_fwd<double>, cus, schunks);

@@ -25,82 +25,85 @@ inline int grid(long n, int bs) { return (int)((n + bs - 1) / bs); }
 
 using namespace plan;
 
-int PcgDevice::planMicDevice(std::string& err) {
-    const int n = n_;
-    const long long nnz = nnz_;
-    DevBuf len, tmp;
-    HIPCHK(len.alloc((size_t)nnz * 8));
-    HIPCHK(micStart_.alloc(((size_t)nnz + 1) * 8));
-    long long* start = micStart_.as<long long>();
-    k_plan_mic<false><<<grid(n, 256), 256, 0, stream_>>>(n, rs_.as<int>(), lend_.as<int>(), ubeg_.as<int>(),
-                                                         col_.as<int>(), len.as<long long>(), nullptr, nullptr);
-    HIPCHK(hipGetLastError());
-    size_t tmpBytes = 0;
-    HIPCHK(hipcub::DeviceScan::InclusiveSum(nullptr, tmpBytes, len.as<long long>(), start + 1, (int)nnz, stream_));
-    HIPCHK(tmp.alloc(tmpBytes));
-    HIPCHK(hipcub::DeviceScan::InclusiveSum(tmp.p, tmpBytes, len.as<long long>(), start + 1, (int)nnz, stream_));
-    k_set_i64<<<1, 64, 0, stream_>>>(start, 0);
-    long long total = 0;
-    HIPCHK(hipMemcpyAsync(&total, start + nnz, 8, hipMemcpyDeviceToHost, stream_));
-    HIPCHK(hipStreamSynchronize(stream_));
-    HIPCHK(micProg_.alloc((size_t)total * 4));
-    k_plan_mic<true><<<grid(n, 256), 256, 0, stream_>>>(n, rs_.as<int>(), lend_.as<int>(), ubeg_.as<int>(),
-                                                        col_.as<int>(), nullptr, start, micProg_.as<int>());
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(stream_));  // the temporaries are freed on return
-    return ASTG_OK;
-}
-
-int PcgDevice::planStencilDevice(int dir, int maxY, DevStencil& D, std::string& err) {
+int PcgDevice::planDevice(int maxY, std::string& err) {
     if (maxY > kMaxDeps) {
         err = "sweep stencil: maxY above the dependency limit";
         return ASTG_ERR_ARG;
     }
-    const int P = PS_, n = n_;
-    DevBuf pos, cI, cU, tmp, ebuf;
-    HIPCHK(pos.alloc((size_t)n * 4));
-    HIPCHK(cI.alloc((size_t)P * 4));
-    HIPCHK(cU.alloc((size_t)P * 4));
-    HIPCHK(ebuf.alloc(4));
-    HIPCHK(D.start.alloc(((size_t)P + 1) * 4));
-    HIPCHK(D.ustart.alloc(((size_t)P + 1) * 4));
-    HIPCHK(D.nr.alloc((size_t)P * 4));
-    k_set_i32<<<1, 64, 0, stream_>>>(ebuf.as<int>(), 0);
-    k_sweep_pos<<<grid(P, 256), 256, 0, stream_>>>(P, sorder_.as<int>(), pos.as<int>());
-    auto kc = dir == 0 ? k_plan_stencil<0, false> : k_plan_stencil<1, false>;
-    kc<<<grid(P, 64), 64, 0, stream_>>>(P, rs_.as<int>(), lend_.as<int>(), ubeg_.as<int>(), col_.as<int>(),
-                                        lev_.as<int>(), merge_, maxY, sorder_.as<int>(), pos.as<int>(), cI.as<int>(),
-                                        cU.as<int>(), D.nr.as<int>(), nullptr, nullptr, nullptr, nullptr,
-                                        ebuf.as<int>());
+    const int n = n_, P = PS_;
+    const long long nnz = nnz_;
+    const int* rs = rs_.as<int>();
+    const int* lend = lend_.as<int>();
+    const int* ubeg = ubeg_.as<int>();
+    const int* col = col_.as<int>();
+    const int* lev = lev_.as<int>();
+    const int* order = sorder_.as<int>();
+    int* cnt = cnt_.as<int>();
+    HIPCHK(planLen_.alloc((size_t)nnz * 8));
+    HIPCHK(planPos_.alloc((size_t)n * 4));
+    HIPCHK(planCnt_.alloc((size_t)P * 16));  // per direction: entry and slot counts
+    HIPCHK(micStart_.alloc(((size_t)nnz + 1) * 8));
+    DevStencil* dst[2] = {&stF_, &stB_};
+    int* cI[2] = {planCnt_.as<int>(), planCnt_.as<int>() + 2 * (size_t)P};
+    int* cU[2] = {planCnt_.as<int>() + (size_t)P, planCnt_.as<int>() + 3 * (size_t)P};
+    for (DevStencil* D : dst) {
+        HIPCHK(D->start.alloc(((size_t)P + 1) * 4));
+        HIPCHK(D->ustart.alloc(((size_t)P + 1) * 4));
+        HIPCHK(D->nr.alloc((size_t)P * 4));
+    }
+    long long* mstart = micStart_.as<long long>();
+    size_t b64 = 0, b32 = 0;
+    HIPCHK(hipcub::DeviceScan::InclusiveSum(nullptr, b64, planLen_.as<long long>(), mstart + 1, (int)nnz, stream_));
+    HIPCHK(hipcub::DeviceScan::InclusiveSum(nullptr, b32, cI[0], dst[0]->start.as<int>() + 1, P, stream_));
+    size_t tb = std::max(b64, b32);
+    HIPCHK(planTmp_.alloc(tb));
+    // count passes and prefix sums
+    k_plan_mic<false><<<grid(n, 256), 256, 0, stream_>>>(n, rs, lend, ubeg, col, planLen_.as<long long>(), nullptr,
+                                                         nullptr);
+    HIPCHK(hipcub::DeviceScan::InclusiveSum(planTmp_.p, tb, planLen_.as<long long>(), mstart + 1, (int)nnz,
+                                            stream_));
+    k_set_i64<<<1, 64, 0, stream_>>>(mstart, 0);
+    k_sweep_pos<<<grid(P, 256), 256, 0, stream_>>>(P, order, planPos_.as<int>());
+    for (int dir = 0; dir < 2; ++dir) {
+        DevStencil& D = *dst[dir];
+        auto kc = dir == 0 ? k_plan_stencil<0, false> : k_plan_stencil<1, false>;
+        kc<<<grid(P, 64), 64, 0, stream_>>>(P, rs, lend, ubeg, col, lev, merge_, maxY, order, planPos_.as<int>(),
+                                            cI[dir], cU[dir], D.nr.as<int>(), nullptr, nullptr, nullptr, nullptr,
+                                            cnt + C_PATTERN);
+        HIPCHK(hipcub::DeviceScan::InclusiveSum(planTmp_.p, tb, cI[dir], D.start.as<int>() + 1, P, stream_));
+        HIPCHK(hipcub::DeviceScan::InclusiveSum(planTmp_.p, tb, cU[dir], D.ustart.as<int>() + 1, P, stream_));
+        k_set_i32<<<1, 64, 0, stream_>>>(D.start.as<int>(), 0);
+        k_set_i32<<<1, 64, 0, stream_>>>(D.ustart.as<int>(), 0);
+    }
     HIPCHK(hipGetLastError());
-    size_t b1 = 0, b2 = 0;
-    HIPCHK(hipcub::DeviceScan::InclusiveSum(nullptr, b1, cI.as<int>(), D.start.as<int>() + 1, P, stream_));
-    HIPCHK(hipcub::DeviceScan::InclusiveSum(nullptr, b2, cU.as<int>(), D.ustart.as<int>() + 1, P, stream_));
-    HIPCHK(tmp.alloc(std::max(b1, b2)));
-    HIPCHK(hipcub::DeviceScan::InclusiveSum(tmp.p, b1, cI.as<int>(), D.start.as<int>() + 1, P, stream_));
-    HIPCHK(hipcub::DeviceScan::InclusiveSum(tmp.p, b2, cU.as<int>(), D.ustart.as<int>() + 1, P, stream_));
-    k_set_i32<<<1, 64, 0, stream_>>>(D.start.as<int>(), 0);
-    k_set_i32<<<1, 64, 0, stream_>>>(D.ustart.as<int>(), 0);
-    int tot[3] = {0, 0, 0};
-    HIPCHK(hipMemcpyAsync(&tot[0], D.start.as<int>() + P, 4, hipMemcpyDeviceToHost, stream_));
-    HIPCHK(hipMemcpyAsync(&tot[1], D.ustart.as<int>() + P, 4, hipMemcpyDeviceToHost, stream_));
-    HIPCHK(hipMemcpyAsync(&tot[2], ebuf.p, 4, hipMemcpyDeviceToHost, stream_));
+    // sizes: program length, entries and slots per direction, error flag
+    int* hi = reinterpret_cast<int*>(hplan_ + 1);
+    HIPCHK(hipMemcpyAsync(hplan_, mstart + nnz, 8, hipMemcpyDeviceToHost, stream_));
+    for (int dir = 0; dir < 2; ++dir) {
+        HIPCHK(hipMemcpyAsync(hi + 2 * dir, dst[dir]->start.as<int>() + P, 4, hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipMemcpyAsync(hi + 2 * dir + 1, dst[dir]->ustart.as<int>() + P, 4, hipMemcpyDeviceToHost, stream_));
+    }
+    HIPCHK(hipMemcpyAsync(hi + 4, cnt + C_PATTERN, 4, hipMemcpyDeviceToHost, stream_));
     HIPCHK(hipStreamSynchronize(stream_));
-    if (tot[2]) {
+    if (hi[4]) {
         err = "sweep stencil: a row exceeds the key bound";
         return ASTG_ERR_HIP;
     }
-    D.E = tot[0];
-    HIPCHK(D.idx.alloc((size_t)D.E * 4));
-    HIPCHK(D.upd.alloc((size_t)tot[1]));
-    HIPCHK(D.coef.alloc((size_t)D.E * 8));
-    auto kf = dir == 0 ? k_plan_stencil<0, true> : k_plan_stencil<1, true>;
-    kf<<<grid(P, 64), 64, 0, stream_>>>(P, rs_.as<int>(), lend_.as<int>(), ubeg_.as<int>(), col_.as<int>(),
-                                        lev_.as<int>(), merge_, maxY, sorder_.as<int>(), pos.as<int>(), nullptr,
-                                        nullptr, nullptr, D.start.as<int>(), D.ustart.as<int>(), D.idx.as<int>(),
-                                        D.upd.as<uint8_t>(), ebuf.as<int>());
+    // fill passes
+    HIPCHK(micProg_.alloc((size_t)hplan_[0] * 4));
+    k_plan_mic<true><<<grid(n, 256), 256, 0, stream_>>>(n, rs, lend, ubeg, col, nullptr, mstart, micProg_.as<int>());
+    for (int dir = 0; dir < 2; ++dir) {
+        DevStencil& D = *dst[dir];
+        D.E = hi[2 * dir];
+        HIPCHK(D.idx.alloc((size_t)D.E * 4));
+        HIPCHK(D.upd.alloc((size_t)hi[2 * dir + 1]));
+        HIPCHK(D.coef.alloc((size_t)D.E * 8));
+        auto kf = dir == 0 ? k_plan_stencil<0, true> : k_plan_stencil<1, true>;
+        kf<<<grid(P, 64), 64, 0, stream_>>>(P, rs, lend, ubeg, col, lev, merge_, maxY, order, planPos_.as<int>(),
+                                            nullptr, nullptr, nullptr, D.start.as<int>(), D.ustart.as<int>(),
+                                            D.idx.as<int>(), D.upd.as<uint8_t>(), cnt + C_PATTERN);
+    }
     HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(stream_));  // the temporaries are freed on return
     return ASTG_OK;
 }
 

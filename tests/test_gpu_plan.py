@@ -94,3 +94,55 @@ def test_adaptive_smoke_steps_identical_with_host_plans(monkeypatch):
     assert h[0] == d[0]
     for a, b in zip(h[1:], d[1:]):
         assert _same_arrays(a, b)
+
+
+def _tridiag():
+    rs = [0, 2, 5, 8, 10]
+    col = [0, 1, 0, 1, 2, 1, 2, 3, 2, 3]
+    return rs, col
+
+
+def _wide():
+    # row 17 couples to all 17 rows before it: more lower entries than a sweep row holds
+    rs, col = [0], []
+    for j in range(17):
+        col += [j, 17]
+        rs.append(len(col))
+    col += list(range(18))
+    rs.append(len(col))
+    return rs, col
+
+
+@pytest.mark.parametrize("case,message", [
+    ("unsorted", "row 0 is not sorted by column"),
+    ("range", "column index out of range"),
+    ("asymmetric", "matrix is not structurally symmetric at (1,0)"),
+    ("wide", "row 17 has more than 16 entries on one side of the diagonal"),
+])
+def test_invalid_patterns_are_rejected(case, message):
+    """The structural checks run on the device; the host words the error."""
+    import astg
+    if case == "wide":
+        rs, col = _wide()
+    else:
+        rs, col = _tridiag()
+        if case == "unsorted":
+            col[0:2] = [1, 0]
+        elif case == "range":
+            col[9] = 4
+        else:
+            rs = [0, 1, 4, 7, 9]
+            col = [0, 0, 1, 2, 1, 2, 3, 2, 3]
+    n = len(rs) - 1
+    val = np.where(np.array(col) == np.repeat(np.arange(n), np.diff(rs)), 4.0, -1.0)
+    c = astg.Context(0)
+    with pytest.raises(astg.AstgError) as e:
+        c.pcg_solve_csr(np.array(rs), np.array(col), val, np.ones(n))
+    assert e.value.code == astg.ASTG_ERR_ARG
+    assert message in str(e.value)
+    # the context stays usable: a valid system solves afterwards
+    rs, col = _tridiag()
+    val = np.where(np.array(col) == np.repeat(np.arange(4), np.diff(rs)), 4.0, -1.0)
+    x, res = c.pcg_solve_csr(np.array(rs), np.array(col), val, np.ones(4))
+    assert res.converged == 1
+    c.close()
