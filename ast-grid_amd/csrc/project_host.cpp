@@ -11,7 +11,6 @@
 
 #include <algorithm>
 #include <climits>
-#include <memory>
 #include <set>
 #include <stdexcept>
 
@@ -45,13 +44,20 @@ lcoor_t tiltNeighbor(const Octree& o, int l, int x, int y, int z, int k) {
 // thread; the half-tilt couplings, which several half-tilts add to one
 // coarse row, are (row, column) pairs merged in when the CSR is formed.
 constexpr int kRowCap = 16;
+// rows per thread at least, for the loops that look neighbours up per row
+constexpr long kRowSlice = 2048;
 struct RowSet {
-    std::unique_ptr<int32_t[]> c;    // n * kRowCap, only the first k[r] of a row are defined
-    std::vector<uint8_t> k;          // n
-    std::vector<std::pair<int, int>> extra;
-    explicit RowSet(int n) : c(new int32_t[(size_t)n * kRowCap]), k((size_t)n, 0) {}
+    int32_t* c;                                // n * kRowCap, only the first k[r] of a row are defined
+    std::vector<uint8_t>& k;                   // n
+    std::vector<std::pair<int, int>>& extra;
+    RowSet(int n, ProjectTables& t) : k(t.scratch_len), extra(t.scratch_extra) {
+        if (t.scratch_rows.size() < (size_t)n * kRowCap) t.scratch_rows.resize((size_t)n * kRowCap);
+        c = t.scratch_rows.data();
+        k.assign((size_t)n, 0);
+        extra.clear();
+    }
     void touch(int r, int col) {
-        int32_t* v = c.get() + (size_t)r * kRowCap;
+        int32_t* v = c + (size_t)r * kRowCap;
         const int m = k[r];
         const int p = (int)(std::lower_bound(v, v + m, col) - v);
         if (p < m && v[p] == col) return;
@@ -68,7 +74,12 @@ struct RowSet {
 void buildProjectTables(const Octree& o, const Numbering& num, const Weights& w, ProjectTables& t) {
     PhaseTimer tm("tables");
     const int L = o.L;
-    t = ProjectTables();
+    // A rebuild reuses the previous tables' memory: every array below is
+    // either filled (assign), cleared before it is appended to, or resized
+    // and then written at every element.
+    for (auto* v : {&t.hc_row, &t.hc_start, &t.hc_q, &t.hr_row, &t.hr_start, &t.hr_q, &t.vt_node}) v->clear();
+    for (auto* v : {&t.hc_pos_diag, &t.hc_pos_off, &t.hr_pos}) v->clear();
+    t.half_duplicates = false;
     t.off.assign(L + 1, 0);
     for (int l = 0; l < L; ++l) t.off[l + 1] = t.off[l] + o.lv[l].nnode;
     const int G = t.G = t.off[L];
@@ -93,10 +104,10 @@ void buildProjectTables(const Octree& o, const Numbering& num, const Weights& w,
                 for (int a = 0; a < 3; ++a) t.w_mac[(size_t)a * G + g] = w.mac[l][(size_t)i * 3 + a];
                 for (int k = 0; k < 8; ++k) t.w_tilt[(size_t)k * G + g] = w.tilt[l][(size_t)i * 8 + k];
             }
-        });
+        }, 16 * kRowSlice);
     tm.mark("nodes");
     t.n = num.n;
-    RowSet rs(t.n);
+    RowSet rs(t.n, t);
 
     // ---- octagon and tilt rows (project 367-552) ----
     std::vector<int> octJ, tiltJ;
@@ -149,7 +160,7 @@ void buildProjectTables(const Octree& o, const Numbering& num, const Weights& w,
         }
         rs.touch(r, r);
     }
-    });
+    }, kRowSlice);
     const int nt = t.n_tilt = (int)tiltJ.size();
     t.tilt_row.resize(nt);
     t.tilt_node.resize(nt);
@@ -189,7 +200,7 @@ void buildProjectTables(const Octree& o, const Numbering& num, const Weights& w,
         }
         rs.touch(r, r);
     }
-    });
+    }, kRowSlice);
 
     tm.mark("octagon and tilt rows");
     // ---- half-tilt rows (project 554-649), serial order ----
@@ -258,7 +269,7 @@ void buildProjectTables(const Octree& o, const Numbering& num, const Weights& w,
     const auto& ex = rs.extra;
     // merge of row r's list and its shared entries [e0, e1); emit(col) per entry
     auto mergeRow = [&](int r, size_t e0, size_t e1, auto&& emit) {
-        const int32_t* v = rs.c.get() + (size_t)r * kRowCap;
+        const int32_t* v = rs.c + (size_t)r * kRowCap;
         const int m = rs.k[r];
         int a = 0;
         size_t b = e0;
@@ -280,7 +291,7 @@ void buildProjectTables(const Octree& o, const Numbering& num, const Weights& w,
                 while (e < ex.size() && ex[e].first == r) ++e;
                 body(r, e0, e);
             }
-        });
+        }, kRowSlice);
     };
     t.rowstart.assign(t.n + 1, 0);
     sliceRows([&](int r, size_t e0, size_t e1) {
@@ -318,7 +329,7 @@ void buildProjectTables(const Octree& o, const Numbering& num, const Weights& w,
         }
         t.oct_pos[(size_t)14 * no + j] = posOf(r, r);
     }
-    });
+    }, kRowSlice);
     parallelFor(nt, [&](long j0, long j1) {
     for (int j = (int)j0; j < (int)j1; ++j) {
         const int r = t.tilt_row[j];
@@ -328,7 +339,7 @@ void buildProjectTables(const Octree& o, const Numbering& num, const Weights& w,
         }
         t.tilt_pos[(size_t)8 * nt + j] = posOf(r, r);
     }
-    });
+    }, kRowSlice);
     tm.mark("CSR and positions");
     // coarse-target lists and half-row occurrence lists, both in q order
     {
@@ -411,7 +422,7 @@ void buildProjectTables(const Octree& o, const Numbering& num, const Weights& w,
                         t.vm_nb[(size_t)a * nvm + j] = code;
                     }
                 }
-            });
+            }, 4 * kRowSlice);
         }
     }
     // open tilts, in level then leaf order; codes and levels slot-major
@@ -446,7 +457,7 @@ void buildProjectTables(const Octree& o, const Numbering& num, const Weights& w,
                     t.vt_lev[(size_t)k * nvt + j] = lev;
                 }
             }
-        });
+        }, 4 * kRowSlice);
     }
     tm.mark("velocity tables");
 }

@@ -8,6 +8,7 @@
 #pragma once
 
 #include <cstdint>
+#include <utility>
 #include <vector>
 
 #include "octree.h"
@@ -82,6 +83,12 @@ struct ProjectTables {
     std::vector<int32_t> vt_nb;              // 8 * n_vt
     std::vector<uint8_t> vt_lev;             // 8 * n_vt
     bool half_duplicates = false;
+
+    // builder scratch: pattern rows under construction, kept with the tables
+    // so that a rebuild (every tilt update) reuses their memory
+    std::vector<int32_t> scratch_rows;               // n * 16
+    std::vector<uint8_t> scratch_len;                // n
+    std::vector<std::pair<int, int>> scratch_extra;  // shared (row, column) pairs
 };
 
 void buildProjectTables(const Octree& o, const Numbering& num, const Weights& w, ProjectTables& t);

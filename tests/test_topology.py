@@ -48,3 +48,21 @@ def test_matrix_pattern(golden):
     rs, col = c.pattern()
     assert np.array_equal(rs, golden["A_rowstart"])
     assert np.array_equal(col, golden["A_col"])
+
+
+def test_matrix_pattern_after_rebuild(golden):
+    """A tilt change rebuilds the tables into the previous tables' memory: the
+    pattern after a rebuild equals the fresh build's (the golden pattern)."""
+    import astg
+    c = astg.Context(-1)
+    R, L = int(golden["R"]), int(golden["levels"])
+    c.set_topology(golden["split"].astype(np.int32), R, L, int(golden["closed"]))
+    _, nleaf = c.level_info(0)
+    rng = np.random.default_rng(3)
+    c.set_tilt_open(0, (rng.random(nleaf) < 0.5).astype(np.uint8))
+    c.pattern()  # first build, with a different tilt state (and size)
+    final = golden["open_mask_l0"] if int(golden["open_tilts"]) else np.zeros(nleaf, np.uint8)
+    c.set_tilt_open(0, final)
+    rs, col = c.pattern()
+    assert np.array_equal(rs, golden["A_rowstart"])
+    assert np.array_equal(col, golden["A_col"])
