@@ -618,7 +618,10 @@ int stackBatch(astg_ctx* c, int count, const int* n, const int* const* rowstart,
 // Pattern-only set-up of the stacked system with `merge` levels per sweep
 // group, skipped when the cached pattern is the same.
 int batchPattern(astg_ctx* c, const StackedBatch& S, int merge, bool cache) {
-    if (!c->batch) c->batch.reset(new astg::PcgDevice(c->stream));
+    if (!c->batch) {
+        c->batch.reset(new astg::PcgDevice(c->stream));
+        c->batch->setRowOrder(true);  // per-system streaming kernels work on stacked rows
+    }
     if (c->batchMerge == merge && c->batchSeg == S.seg && c->batchRs == S.rs && c->batchCol == S.col)
         return ASTG_OK;
     c->batchRs.clear();

@@ -50,6 +50,9 @@ struct SweepPlan {
     // of the stacked rows, and the per-system state (BatchSlot layout)
     const int* chunkSys;
     const double* sysState;
+    // 1: the vectors r and z and the pivots are stored in sweep order (index
+    // = sweep position), and forward r-term keys are sweep positions; 0: row order
+    int vecPos;
 };
 
 // Merged sweep stencil of one direction, by sweep position (sweep_merge.h).
@@ -147,6 +150,9 @@ public:
     // Launch shape of the batched streaming kernels: blocks per compute unit
     // (0: the single-system default) and SpMV lanes per row (4, 8 or 16).
     void setBatchTuning(int blocksPerCu, int lanesPerRow);
+    // Keep the PCG vectors in row order (the batched form, whose streaming
+    // kernels work per system on stacked rows). Takes effect at setPattern.
+    void setRowOrder(bool on) { rowOrderForced_ = on; }
     // Levels per merged sweep group used by the next setPattern (0: default).
     void setMergeLevels(int q) { mergeLevels_ = q; }
     int mergeLevels() const { return merge_; }
@@ -189,6 +195,17 @@ private:
     // the device from the uploaded pattern and sweep order (plan_device.hip);
     // the host planners serve when hostPlan_ is set
     int planDevice(int maxY, std::string& err);
+    // Sweep-ordered solve: permuted CSR, forward r-term keys as positions
+    // (once per pattern); values, pivots and rhs gathered per solve.
+    int permuteSetup(std::string& err);
+    int prepareSolve(std::string& err);
+    int vlen() const { return perm_ ? PS_ : n_; }  // length of the PCG vectors
+    const int* sRs() const { return perm_ ? prs_.as<int>() : rs_.as<int>(); }
+    const int* sCol() const { return perm_ ? pcol_.as<int>() : col_.as<int>(); }
+    double* sVal64() const { return perm_ ? valP_.as<double>() : val_.as<double>(); }
+    double* sInvd64() const { return perm_ ? invdP_.as<double>() : invd_.as<double>(); }
+    double* sB() const { return perm_ ? bP_.as<double>() : b_.as<double>(); }
+    double* sX() const { return perm_ ? xP_.as<double>() : x_.as<double>(); }
     int launchMic(double tau, double ratio, std::string& err);
     int launchMerge(std::string& err);
     int convert32(std::string& err);
@@ -245,6 +262,12 @@ private:
     int lpr_ = 8;                           // batched form: SpMV lanes per row
     int mergeLevels_ = 0;                   // requested levels per sweep group (0: default)
     bool hostPlan_ = false;                 // ASTG_HOST_PLAN: symbolic plans on the host
+    bool perm_ = false;                     // PCG vectors in sweep order (single-system solves)
+    bool rowOrderForced_ = false;           // setRowOrder (batched form)
+    DevBuf pos_;                            // row -> sweep position
+    DevBuf prs_, pcol_, pmap_;              // CSR in sweep order: rowstart, columns as positions, source entry
+    DevBuf valP_, invdP_, bP_, xP_;         // values, pivots, rhs and solution in sweep order
+    DevBuf permLen_;                        // set-up: row lengths in sweep order
     bool batchGraph_ = true;                // batched form: iterations replayed from hipGraphs
     bool coop_ = false;                     // sweeps through the cooperative launch API (ASTG_COOP=1)
     bool spinFailed_ = false;               // a sweep hit its spin limit since the last solve started

@@ -350,8 +350,9 @@ __global__ void __launch_bounds__(BS) k_fwd(SweepPlan S, StencilView<T> St, cons
         T acc = 0, inv = 0;
         if (i >= 0) {
             const int b = St.start[t], e = St.start[t + 1], br = b + St.nr[t];
-            acc = r[i];
-            inv = invd[i];
+            const int vi = S.vecPos ? t : i;  // vectors in sweep or row order
+            acc = r[vi];
+            inv = invd[vi];
             for (int p = b; p < br; ++p) acc -= St.coef[p] * r[St.idx[p]];
             d.preload(St.idx, St.coef, br, e);
         }
@@ -401,7 +402,7 @@ __global__ void __launch_bounds__(BS) k_bwd_dot(SweepPlan S, StencilView<T> St, 
             const int b = St.start[pos], e = St.start[pos + 1], br = b + St.nr[pos];
             acc = y[pos];
             for (int p = b; p < br; ++p) acc -= St.coef[p] * y[St.idx[p]];
-            inv = invd[k];
+            inv = invd[S.vecPos ? pos : k];
             d.preload(St.idx, St.coef, br, e);
         }
         gate_row(zp, S.gateB[pos >> 6], S.gateCap);
@@ -412,7 +413,7 @@ __global__ void __launch_bounds__(BS) k_bwd_dot(SweepPlan S, StencilView<T> St, 
                 const long long td = S.trace ? wall_clock64() : 0;
                 zk = a * inv;
                 Sentinel<T>::store(&zp[pos], zk);
-                z[k] = zk;
+                z[S.vecPos ? pos : k] = zk;
                 if (S.trace) {
                     S.trace[3 * (long)k] = tg;
                     S.trace[3 * (long)k + 1] = td;
@@ -422,12 +423,14 @@ __global__ void __launch_bounds__(BS) k_bwd_dot(SweepPlan S, StencilView<T> St, 
             if (!ok) atomicOr(&cnt[C_ERR], E_BWD);
             // after publishing, off the dependency path: the dot with r and the
             // re-arming of the next forward sweep's vector
-            rk = r[k];
+            rk = r[S.vecPos ? pos : k];
             Sentinel<T>::storeSent(&ynext[pos]);
             v += (double)zk * (double)rk;
         } else if (fin) {
             Sentinel<T>::store(&zp[pos], (T)0);
             Sentinel<T>::storeSent(&ynext[pos]);
+        } else if (S.vecPos) {
+            z[pos] = (T)0;  // padding position: z stays zero there
         }
     }
     const double t = block_reduce<false>(v);
@@ -443,7 +446,7 @@ template <int DIR>
 __global__ void __launch_bounds__(BS) k_merge(int P, const int* order, const int* rs, const int* lend,
                                               const int* ubeg, const double* F, const double* invd, const int* sb,
                                               const int* nr, const int* idx, const int* us, const uint8_t* upd,
-                                              double* C) {
+                                              double* C, int fwdPos) {
     const int t = blockIdx.x * BS + threadIdx.x;
     if (t >= P) return;
     const int i = order[t];
@@ -454,12 +457,79 @@ __global__ void __launch_bounds__(BS) k_merge(int P, const int* order, const int
     const int p0 = DIR == 0 ? rs[i] : ubeg[i], p1 = DIR == 0 ? lend[i] : rs[i + 1];
     for (int p = p0; p < p1; ++p) C[b + upd[u++]] += F[p];
     for (int x = 0; x < nR; ++x) {
-        const int j = DIR == 0 ? idx[b + x] : order[idx[b + x]];
+        const int j = (DIR == 1 || fwdPos) ? order[idx[b + x]] : idx[b + x];
         const double a = C[b + x] * invd[j];
         C[b + x] = a;
         const int q0 = DIR == 0 ? rs[j] : ubeg[j], q1 = DIR == 0 ? lend[j] : rs[j + 1];
         for (int p = q0; p < q1; ++p) C[b + upd[u++]] -= a * F[p];
     }
+}
+
+// ---------------------------------------------------------------------------
+// Sweep-ordered solves: the PCG vectors are stored by sweep position, so the
+// sweeps read r and the pivots and write z contiguously. Padding positions
+// are empty rows whose vector entries stay zero.
+// ---------------------------------------------------------------------------
+__global__ void k_row_pos(int P, const int* __restrict__ order, int* __restrict__ pos) {
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= P) return;
+    const int i = order[t];
+    if (i >= 0) pos[i] = t;
+}
+
+// Forward r-term keys: rows -> sweep positions.
+__global__ void k_rterm_pos(int P, const int* __restrict__ start, const int* __restrict__ nr, int* __restrict__ idx,
+                            const int* __restrict__ pos) {
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= P) return;
+    for (int x = start[t], e = start[t] + nr[t]; x < e; ++x) idx[x] = pos[idx[x]];
+}
+
+__global__ void k_perm_len(int P, const int* __restrict__ order, const int* __restrict__ rs, int* __restrict__ len) {
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= P) return;
+    const int k = order[t];
+    len[t] = k >= 0 ? rs[k + 1] - rs[k] : 0;
+}
+
+// Row t of the sweep-ordered CSR: row order[t]'s entries in their stored
+// order (so every row sum keeps its order), columns as sweep positions, and
+// the source entry of each (for the values of every new assembly).
+__global__ void k_perm_fill(int P, const int* __restrict__ order, const int* __restrict__ rs,
+                            const int* __restrict__ col, const int* __restrict__ pos, const int* __restrict__ prs,
+                            int* __restrict__ pcol, int* __restrict__ pmap) {
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= P) return;
+    const int k = order[t];
+    if (k < 0) return;
+    int w = prs[t];
+    for (int p = rs[k], e = rs[k + 1]; p < e; ++p, ++w) {
+        pcol[w] = pos[col[p]];
+        pmap[w] = p;
+    }
+}
+
+__global__ void k_gather_vals(long nnz, const int* __restrict__ pmap, const double* __restrict__ val,
+                              double* __restrict__ out) {
+    for (long e = (long)blockIdx.x * BS + threadIdx.x; e < nnz; e += (long)gridDim.x * BS) out[e] = val[pmap[e]];
+}
+
+// out[t] = in[order[t]], zero at padding positions.
+__global__ void k_to_sweep(int P, const int* __restrict__ order, const double* __restrict__ in,
+                           double* __restrict__ out) {
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= P) return;
+    const int k = order[t];
+    out[t] = k >= 0 ? in[k] : 0.0;
+}
+
+// out[order[t]] = in[t].
+__global__ void k_from_sweep(int P, const int* __restrict__ order, const double* __restrict__ in,
+                             double* __restrict__ out) {
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= P) return;
+    const int k = order[t];
+    if (k >= 0) out[k] = in[t];
 }
 
 // s = z + beta s with beta = rho(it+1) / rho(it); mode 0: s = z after the first
@@ -739,10 +809,6 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
     HIPCHK(invd_.alloc(n8));
     HIPCHK(b_.alloc(n8));
     HIPCHK(x_.alloc(n8));
-    HIPCHK(r_.alloc(n8));
-    HIPCHK(z_.alloc(n8));
-    HIPCHK(s_.alloc(n8));
-    HIPCHK(q_.alloc(n8));
     HIPCHK(scal_.alloc(32 * 8));
     HIPCHK(cnt_.alloc(C_COUNT * 4));
     if (!hscal_) HIPCHK(hipHostMalloc((void**)&hscal_, 32 * 8, hipHostMallocDefault));
@@ -895,6 +961,18 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
         }
     }
     HIPCHK(live_.alloc((size_t)n));
+    // PCG work vectors: sweep-ordered (PS entries) unless the row order is kept
+    perm_ = !rowOrderForced_;
+    if (const char* g = getenv("ASTG_ROW_ORDER")) perm_ = perm_ && atoi(g) == 0;
+    if (perm_) {
+        const int rc = permuteSetup(err);
+        if (rc) return rc;
+    }
+    const size_t nv8 = (size_t)vlen() * 8;
+    HIPCHK(r_.alloc(nv8));
+    HIPCHK(z_.alloc(nv8));
+    HIPCHK(s_.alloc(nv8));
+    HIPCHK(q_.alloc(nv8));
     HIPCHK(y_.alloc((size_t)PS_ * 2 * 8));
     HIPCHK(zp_.alloc((size_t)PS_ * 8));
     tm.mark("symbolic plans");
@@ -907,6 +985,46 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
     gb_ = std::max(std::max(gB64_, gB32_), 1);
     HIPCHK(part_.alloc((size_t)(2 * gs_ + 2 * gb_ + 8) * 8));
     HIPCHK(hipStreamSynchronize(stream_));
+    return ASTG_OK;
+}
+
+int PcgDevice::permuteSetup(std::string& err) {
+    const int n = n_, P = PS_;
+    const long long nnz = nnz_;
+    HIPCHK(pos_.alloc((size_t)n * 4));
+    HIPCHK(prs_.alloc(((size_t)P + 1) * 4));
+    HIPCHK(pcol_.alloc((size_t)nnz * 4));
+    HIPCHK(pmap_.alloc((size_t)nnz * 4));
+    HIPCHK(valP_.alloc((size_t)nnz * 8));
+    HIPCHK(invdP_.alloc((size_t)P * 8));
+    HIPCHK(bP_.alloc((size_t)P * 8));
+    HIPCHK(xP_.alloc((size_t)P * 8));
+    HIPCHK(permLen_.alloc((size_t)P * 4));
+    const int* order = sorder_.as<int>();
+    k_row_pos<<<blocks(P), BS, 0, stream_>>>(P, order, pos_.as<int>());
+    k_rterm_pos<<<blocks(P), BS, 0, stream_>>>(P, stF_.start.as<int>(), stF_.nr.as<int>(), stF_.idx.as<int>(),
+                                               pos_.as<int>());
+    k_perm_len<<<blocks(P), BS, 0, stream_>>>(P, order, rs_.as<int>(), permLen_.as<int>());
+    size_t tb = 0;
+    HIPCHK(hipcub::DeviceScan::InclusiveSum(nullptr, tb, permLen_.as<int>(), prs_.as<int>() + 1, P, stream_));
+    HIPCHK(planTmp_.alloc(tb));
+    HIPCHK(hipcub::DeviceScan::InclusiveSum(planTmp_.p, tb, permLen_.as<int>(), prs_.as<int>() + 1, P, stream_));
+    k_fill_i32<<<1, BS, 0, stream_>>>(prs_.as<int>(), 0, 1);
+    k_perm_fill<<<blocks(P), BS, 0, stream_>>>(P, order, rs_.as<int>(), col_.as<int>(), pos_.as<int>(),
+                                               prs_.as<int>(), pcol_.as<int>(), pmap_.as<int>());
+    HIPCHK(hipGetLastError());
+    return ASTG_OK;
+}
+
+int PcgDevice::prepareSolve(std::string& err) {
+    if (!perm_) return ASTG_OK;
+    const int P = PS_;
+    const int* order = sorder_.as<int>();
+    k_gather_vals<<<std::max(1, std::min(8 * cus_, blocks(nnz_))), BS, 0, stream_>>>(
+        (long)nnz_, pmap_.as<int>(), val_.as<double>(), valP_.as<double>());
+    k_to_sweep<<<blocks(P), BS, 0, stream_>>>(P, order, invd_.as<double>(), invdP_.as<double>());
+    k_to_sweep<<<blocks(P), BS, 0, stream_>>>(P, order, b_.as<double>(), bP_.as<double>());
+    HIPCHK(hipGetLastError());
     return ASTG_OK;
 }
 
@@ -927,6 +1045,7 @@ SweepPlan PcgDevice::splan() const {
     s.trace = trace_;
     s.chunkSys = K_ ? chunkSeg_.as<int>() : nullptr;
     s.sysState = K_ ? bst_.as<double>() : nullptr;
+    s.vecPos = perm_ ? 1 : 0;
     return s;
 }
 
@@ -998,7 +1117,7 @@ int PcgDevice::launchMerge(std::string& err) {
         k<<<blocks(PS_), BS, 0, stream_>>>(PS_, sorder_.as<int>(), rs_.as<int>(), lend_.as<int>(), ubeg_.as<int>(),
                                            F_.as<double>(), invd_.as<double>(), d.start.as<int>(), d.nr.as<int>(),
                                            d.idx.as<int>(), d.ustart.as<int>(), d.upd.as<uint8_t>(),
-                                           d.coef.as<double>());
+                                           d.coef.as<double>(), perm_ ? 1 : 0);
     }
     HIPCHK(hipGetLastError());
     return ASTG_OK;
@@ -1008,11 +1127,11 @@ int PcgDevice::launchMerge(std::string& err) {
 int PcgDevice::convert32(std::string& err) {
     const long long nnz = nnz_;
     HIPCHK(val32_.alloc((size_t)nnz * 4));
-    HIPCHK(invd32_.alloc((size_t)n_ * 4));
+    HIPCHK(invd32_.alloc((size_t)vlen() * 4));
     HIPCHK(stF_.coef32.alloc((size_t)stF_.E * 4));
     HIPCHK(stB_.coef32.alloc((size_t)stB_.E * 4));
-    k_d2f<<<blocks(nnz), BS, 0, stream_>>>(val_.as<double>(), val32_.as<float>(), nnz);
-    k_d2f<<<blocks(n_), BS, 0, stream_>>>(invd_.as<double>(), invd32_.as<float>(), n_);
+    k_d2f<<<blocks(nnz), BS, 0, stream_>>>(sVal64(), val32_.as<float>(), nnz);
+    k_d2f<<<blocks(vlen()), BS, 0, stream_>>>(sInvd64(), invd32_.as<float>(), vlen());
     k_d2f<<<blocks(stF_.E), BS, 0, stream_>>>(stF_.coef.as<double>(), stF_.coef32.as<float>(), stF_.E);
     k_d2f<<<blocks(stB_.E), BS, 0, stream_>>>(stB_.coef.as<double>(), stB_.coef32.as<float>(), stB_.E);
     HIPCHK(hipGetLastError());
@@ -1057,14 +1176,14 @@ template <typename T>
 int PcgDevice::pcgInner(const T* A, const T* invd, const T* b, double* x, double tolRel, int maxIt, int norm,
                         int& iters, double& resOut, bool& conv, std::string& err, const double* b64,
                         int replaceEvery) {
-    const int n = n_, gs = gs_;
+    const int n = vlen(), gs = gs_;
     const int gb = bwdGrid<T>();
     T* r = r_.as<T>();
     T* z = z_.as<T>();
     T* s = s_.as<T>();
     T* q = q_.as<T>();
-    const int* rs = rs_.as<int>();
-    const int* col = col_.as<int>();
+    const int* rs = sRs();
+    const int* col = sCol();
     double* scal = scal_.as<double>();
     const Parts Pt = parts(gb);
     int rc;
@@ -1086,7 +1205,7 @@ int PcgDevice::pcgInner(const T* A, const T* invd, const T* b, double* x, double
             k_spmv_dot<T><<<gs, BS, 0, stream_>>>(n, rs, col, A, s, q, Pt, scal);
             k_axpy_res<T, double><<<gs, BS, 0, stream_>>>(n, s, q, x, r, Pt, scal, norm, it);
             if (sizeof(T) == 4 && replaceEvery > 0 && (it + 1) % replaceEvery == 0)
-                k_replace_res<<<gs, BS, 0, stream_>>>(n, rs, col, val_.as<double>(), x, b64,
+                k_replace_res<<<gs, BS, 0, stream_>>>(n, rs, col, sVal64(), x, b64,
                                                       reinterpret_cast<float*>(r), Pt, scal, norm);
             if ((rc = launchFwd<T>(invd, r, it + 1, 1, it, maxIt, norm, tolRel, err))) return rc;
             if ((rc = launchBwd<T>(invd, r, z, it + 1, (it + 1) & 1, err))) return rc;
@@ -1118,18 +1237,26 @@ int PcgDevice::solve(const astg_pcg_params& prm, astg_pcg_result& res, std::stri
 }
 
 int PcgDevice::solveOnce(const astg_pcg_params& prm, astg_pcg_result& res, std::string& err) {
-    const int n = n_, gs = gs_;
-    res.n = n;
+    const int n = vlen(), gs = gs_;
+    res.n = n_;
     res.nnz = nnz_;
     res.depth = depth_;
     res.outer = 0;
+    if (int rc = prepareSolve(err)) return rc;
+    // the solution in row order (x_) from the sweep-ordered one
+    auto finish = [&]() -> int {
+        if (perm_) k_from_sweep<<<blocks(PS_), BS, 0, stream_>>>(PS_, sorder_.as<int>(), sX(), x_.as<double>());
+        HIPCHK(hipGetLastError());
+        return ASTG_OK;
+    };
     if (prm.precision == ASTG_PREC_FP64) {
         int it = 0;
         double r = 0;
         bool conv = false;
-        int rc = pcgInner<double>(val_.as<double>(), invd_.as<double>(), b_.as<double>(), x_.as<double>(), prm.tol,
-                                  prm.max_iterations, prm.norm, it, r, conv, err, nullptr, 0);
+        int rc = pcgInner<double>(sVal64(), sInvd64(), sB(), sX(), prm.tol, prm.max_iterations, prm.norm, it, r,
+                                  conv, err, nullptr, 0);
         if (rc) return rc;
+        if ((rc = finish())) return rc;
         res.iterations = it;
         res.residual = r;
         res.rel_residual = hscal_[S_BNORM] > 0 ? r / hscal_[S_BNORM] : 0.0;
@@ -1144,7 +1271,7 @@ int PcgDevice::solveOnce(const astg_pcg_params& prm, astg_pcg_result& res, std::
     HIPCHK(d64_.alloc((size_t)n * 8));
     HIPCHK(rhs64_.alloc((size_t)n * 12));
     if (int rc = convert32(err)) return rc;
-    k_zero64<<<blocks(n), BS, 0, stream_>>>(n, x_.as<double>());
+    k_zero64<<<blocks(n), BS, 0, stream_>>>(n, sX());
     double* rhs64 = rhs64_.as<double>();
     float* rhs32 = reinterpret_cast<float*>(rhs64 + n);
     double* d64 = d64_.as<double>();
@@ -1155,8 +1282,8 @@ int PcgDevice::solveOnce(const astg_pcg_params& prm, astg_pcg_result& res, std::
     bool conv = false;
     for (;;) {
         const Parts Pt = parts(gB64_);
-        k_residual64<<<gs, BS, 0, stream_>>>(n, rs_.as<int>(), col_.as<int>(), val_.as<double>(), x_.as<double>(),
-                                             b_.as<double>(), r64_.as<double>(), Pt, prm.norm);
+        k_residual64<<<gs, BS, 0, stream_>>>(n, sRs(), sCol(), sVal64(), sX(), sB(), r64_.as<double>(), Pt,
+                                             prm.norm);
         k_res_final<<<1, BS, 0, stream_>>>(Pt, scal_.as<double>(), prm.norm);
         int rc = syncScalars(err);
         if (rc) return rc;
@@ -1183,9 +1310,10 @@ int PcgDevice::solveOnce(const astg_pcg_params& prm, astg_pcg_result& res, std::
         if (rc) return rc;
         total += it;
         ++outer;
-        k_add_corr<<<gs, BS, 0, stream_>>>(n, x_.as<double>(), d64, rnorm);
+        k_add_corr<<<gs, BS, 0, stream_>>>(n, sX(), d64, rnorm);
         if (it == 0) break;
     }
+    if (int rc = finish()) return rc;
     res.iterations = total;
     res.outer = outer;
     res.residual = rnorm;
@@ -1222,10 +1350,12 @@ int PcgDevice::downloadFactor(std::vector<double>& invdiag, std::vector<unsigned
 }
 
 int PcgDevice::benchKernel(int which, int reps, double& avgMs, double& bytes, std::string& err) {
-    const int n = n_, gs = gs_;
+    const int n = n_, gs = gs_, nv = vlen();
     const long long nnz = nnz_, nl = nnzL();
-    const int* rs = rs_.as<int>();
-    const int* col = col_.as<int>();
+    // the solve's own layout: sweep-ordered vectors and CSR unless the row order is kept
+    if (int rc = prepareSolve(err)) return rc;
+    const int* rs = sRs();
+    const int* col = sCol();
     double* scal = scal_.as<double>();
     const bool f32 = which == ASTG_K_SPMV32 || which == ASTG_K_FWD32 || which == ASTG_K_BWD32;
     if (f32 && (!val32_.p || stF_.coef32.bytes < (size_t)stF_.E * 4)) {
@@ -1281,30 +1411,29 @@ int PcgDevice::benchKernel(int which, int reps, double& avgMs, double& bytes, st
         // the forward sweep starts in mode 0, which needs nonzero ||b|| partials
         if (sweep) k_fill64<<<blocks(gs_), BS, 0, stream_>>>(gs_, P64.res, 1.0);
         if (which == ASTG_K_BWD64)
-            rc = launchFwd<double>(invd_.as<double>(), r_.as<double>(), 0, 0, 0, 1 << 30, ASTG_NORM_INF, 1e-30, err);
+            rc = launchFwd<double>(sInvd64(), r_.as<double>(), 0, 0, 0, 1 << 30, ASTG_NORM_INF, 1e-30, err);
         else if (which == ASTG_K_BWD32)
             rc = launchFwd<float>(invd32_.as<float>(), r_.as<float>(), 0, 0, 0, 1 << 30, ASTG_NORM_INF, 1e-30, err);
         if (rc) break;
         HIPCHK(hipEventRecord(e0, stream_));
         switch (which) {
             case ASTG_K_SPMV64:
-                k_spmv_dot<double><<<gs, BS, 0, stream_>>>(n, rs, col, val_.as<double>(), s_.as<double>(),
+                k_spmv_dot<double><<<gs, BS, 0, stream_>>>(nv, rs, col, sVal64(), s_.as<double>(),
                                                            q_.as<double>(), P64, scal);
                 break;
             case ASTG_K_SPMV32:
-                k_spmv_dot<float><<<gs, BS, 0, stream_>>>(n, rs, col, val32_.as<float>(), s_.as<float>(),
+                k_spmv_dot<float><<<gs, BS, 0, stream_>>>(nv, rs, col, val32_.as<float>(), s_.as<float>(),
                                                           q_.as<float>(), P32, scal);
                 break;
             case ASTG_K_FWD64:
-                rc = launchFwd<double>(invd_.as<double>(), r_.as<double>(), 0, 0, 0, 1 << 30, ASTG_NORM_INF, 1e-30,
-                                       err);
+                rc = launchFwd<double>(sInvd64(), r_.as<double>(), 0, 0, 0, 1 << 30, ASTG_NORM_INF, 1e-30, err);
                 break;
             case ASTG_K_FWD32:
                 rc = launchFwd<float>(invd32_.as<float>(), r_.as<float>(), 0, 0, 0, 1 << 30, ASTG_NORM_INF, 1e-30,
                                       err);
                 break;
             case ASTG_K_BWD64:
-                rc = launchBwd<double>(invd_.as<double>(), r_.as<double>(), z_.as<double>(), 0, 0, err);
+                rc = launchBwd<double>(sInvd64(), r_.as<double>(), z_.as<double>(), 0, 0, err);
                 break;
             case ASTG_K_BWD32:
                 rc = launchBwd<float>(invd32_.as<float>(), r_.as<float>(), z_.as<float>(), 0, 0, err);

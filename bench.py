@@ -26,8 +26,8 @@ KERNELS = tuple((nm, i) for i, nm in enumerate(KERNEL_NAMES))
 # HBM bytes per launch from rocprofv3 counter runs (FETCH_SIZE + WRITE_SIZE,
 # scaled by the copy-kernel calibration) of `bench.py --kernels-only` on this
 # workload, as summarised in profiles/traffic_projection_256.md.
-TRAFFIC = {"spmv64": 644036001, "fwd64": 1137658278, "bwd64": 1336729770, "spmv32": 429851975,
-           "fwd32": 888763642, "bwd32": 1103413339, "copy": 56200384, "mic": 7376658496}
+TRAFFIC = {"spmv64": 682078527, "fwd64": 629650857, "bwd64": 716365013, "spmv32": 432905557,
+           "fwd32": 434983656, "bwd32": 513409061, "copy": 56200384, "mic": 7376752364}
 
 
 def log(*a):

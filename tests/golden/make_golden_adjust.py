@@ -11,7 +11,8 @@ from its own sources by oracle/ref.mk). Each fixture is data only:
   * after post_project and one adaptive advect(dt): density, heat and
     velocity per node and the tilt state the solver's fields then use;
   * then the next step's projection on the new tilt state (constraint and
-    ghosts, project(dt) to 1e-6 with rho = 2): velocity per node.
+    ghosts, project(dt) to 1e-6 with rho = 2): velocity per node, and the
+    PCG iteration count and residual of that solve.
 
     python tests/golden/make_golden_adjust.py
 """
@@ -73,6 +74,10 @@ def make(name):
     for l in range(L):
         m, t = r.velocity(l)
         out["vel_proj_mac_%d" % l], out["vel_proj_tilt_%d" % l] = m[:sizes[l]], t[:sizes[l]]
+    # the iteration count and residual that project() only logs, from the same
+    # PCGSolver on the same system
+    _, res, it = r.resolve(r.L.ref_matrix_n(r.h))
+    out["proj_iters"], out["proj_residual"] = np.array(it), np.array(res)
     r.close()
     np.savez_compressed(os.path.join(HERE, "adj_" + name + ".npz"), **out)
     return out

@@ -8,9 +8,9 @@ Through the C ABI on device 0, from the inputs of the advection fixtures:
   * one advect(dt) with tilt adaptivity on: density, heat and velocity per
     node (1e-12 of the largest magnitude; the reference is built with
     -ffast-math) and the tilt state, bit-exact;
-  * the next projection on the updated tilts: velocity per node to 1e-6
-    relative (closed boxes modulo the pressure constant, as for the
-    projection fixtures).
+  * the next projection on the updated tilts: PCG iterations within 1 of the
+    reference's, velocity per node to 1e-6 relative after the reference's
+    number of iterations.
 """
 import glob
 import os
@@ -92,8 +92,19 @@ def test_projection_after_tilt_update_matches_reference(adj):
     c.set_tilt_adaptivity(True)
     c.advect(float(j["dt"]))
     c.post_project()
+    c.snapshot()
     res = c.project(float(g["dt"]), float(g["rho"]), astg.default_params(tol=float(g["tol"]), max_iterations=3000))
     assert res.converged == 1
+    ref_it = int(j["proj_iters"])
+    assert abs(res.iterations - ref_it) <= 1, (res.iterations, ref_it)
+    if res.iterations != ref_it:
+        # Where the residual meets the tolerance within rounding, another
+        # summation order of the dots can end the solve one iteration
+        # earlier or later than the reference; compare after the reference's
+        # own number of iterations.
+        c.snapshot(restore=True)
+        c.project(float(g["dt"]), float(g["rho"]), astg.default_params(tol=0.0, max_iterations=ref_it),
+                  allow_unconverged=True)
     for l in range(L):
         m, t = c.velocity(l)
         assert rel2(m, j["vel_proj_mac_%d" % l]) < 1e-6, l

