@@ -109,8 +109,10 @@ public:
     ~PcgDevice();
 
     // Uploads a CSR pattern (host arrays) and builds everything that depends on
-    // the pattern only: diagonal / upper split, transpose map, dependency
-    // levels (computed on the device) and the padded level order.
+    // the pattern only, on the device: diagonal / upper split, transpose map
+    // and structural checks, dependency levels, the padded level and sweep
+    // orders, the symbolic plans and, for single-system solves, A's pattern
+    // permuted to sweep order.
     int setPattern(int n, const int* rowstart, const int* col, std::string& err);
     // Device pointer to the FP64 value array (nnz) for in-place assembly.
     double* values() { return val_.as<double>(); }
@@ -124,7 +126,8 @@ public:
 
     // MIC(0) factorisation of the current values (FP64).
     int factor(double tau, double ratio, std::string& err);
-    // Solves with b = rhs(), x = solution(); both device FP64.
+    // Solves with b = rhs(), x = solution(); both device FP64 in row order
+    // (the solve gathers b into sweep order and scatters x back).
     // (Both solves retry once with cooperative sweep launches if an ordinary
     // launch left a sweep block waiting for one that was not resident.)
     int solve(const astg_pcg_params& prm, astg_pcg_result& res, std::string& err);
@@ -235,7 +238,7 @@ private:
     DevBuf val_, F_, invd_;                 // FP64 A, factor (symmetric copy), 1/pivot
     DevBuf val32_, invd32_;                 // FP32 copies for the mixed form
     DevBuf b_, x_;                          // FP64 rhs / solution
-    DevBuf r_, z_, s_, q_;                  // work vectors (sized for FP64; reused as FP32)
+    DevBuf r_, z_, s_, q_;                  // work vectors, vlen() entries (sized for FP64; reused as FP32)
     DevBuf y_, zp_;                         // sweep vectors in sweep order (2 x PS and PS)
     DevBuf r64_, d64_, rhs64_;              // mixed form outer vectors
     int mixedReplace_ = 100;                // mixed form: residual replacement period
