@@ -209,6 +209,9 @@ private:
     double* sInvd64() const { return perm_ ? invdP_.as<double>() : invd_.as<double>(); }
     double* sB() const { return perm_ ? bP_.as<double>() : b_.as<double>(); }
     double* sX() const { return perm_ ? xP_.as<double>() : x_.as<double>(); }
+    template <typename T>
+    void launchSpmv(int n, const int* rs, const int* col, const T* A, const T* s, T* q, const Parts& Pt,
+                    const double* scal);
     int launchMic(double tau, double ratio, std::string& err);
     int launchMerge(std::string& err);
     int convert32(std::string& err);
@@ -266,6 +269,7 @@ private:
     int mergeLevels_ = 0;                   // requested levels per sweep group (0: default)
     bool hostPlan_ = false;                 // ASTG_HOST_PLAN: symbolic plans on the host
     bool perm_ = false;                     // PCG vectors in sweep order (single-system solves)
+    int spmvLpr_ = 4;                       // ASTG_SPMV_LPR: lanes per row of the single-system SpMV
     bool rowOrderForced_ = false;           // setRowOrder (batched form)
     DevBuf pos_;                            // row -> sweep position
     DevBuf prs_, pcol_, pmap_;              // CSR in sweep order: rowstart, columns as positions, source entry

@@ -555,18 +555,18 @@ __global__ void __launch_bounds__(BS) k_xpby(int n, const T* z, T* s, Parts Pt, 
     for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) s[i] = z[i] + beta * s[i];
 }
 
-// q = A s (row_dot, kLpr lanes per row), fused with s.q.
-template <typename T>
+// q = A s (row_dot, LPR lanes per row), fused with s.q.
+template <typename T, int LPR = kLpr>
 __global__ void __launch_bounds__(BS) k_spmv_dot(int n, const int* rs, const int* col, const T* A, const T* s, T* q,
                                                  Parts Pt, const double* scal) {
     if (scal[S_DONE] != 0.0) return;
-    const int sub = threadIdx.x & (kLpr - 1);
+    const int sub = threadIdx.x & (LPR - 1);
     double v = 0.0;
-    const int rowsPerIter = gridDim.x * (BS / kLpr);
-    for (int base = blockIdx.x * (BS / kLpr); base < n; base += rowsPerIter) {
-        const int ii = base + threadIdx.x / kLpr;
+    const int rowsPerIter = gridDim.x * (BS / LPR);
+    for (int base = blockIdx.x * (BS / LPR); base < n; base += rowsPerIter) {
+        const int ii = base + threadIdx.x / LPR;
         const int i = ii < n ? ii : -1;
-        const T acc = row_dot(rs, col, A, s, i, sub);
+        const T acc = row_dot<T, LPR>(rs, col, A, s, i, sub);
         if (i >= 0 && sub == 0) {
             q[i] = acc;
             v += (double)s[i] * (double)acc;
@@ -786,6 +786,9 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
     if (const char* g = getenv("ASTG_TEST_SPIN_ERROR_ONCE")) forceSpinError_ = atoi(g) != 0;
     hostPlan_ = false;
     if (const char* g = getenv("ASTG_HOST_PLAN")) hostPlan_ = atoi(g) != 0;
+    // 4 lanes per row: 16 rows per wavefront, 2 entries per lane on average
+    spmvLpr_ = 4;
+    if (const char* g = getenv("ASTG_SPMV_LPR")) spmvLpr_ = (atoi(g) == 8 || atoi(g) == 16) ? atoi(g) : 4;
     // mixed form: residual replacement period and inner tolerance factor
     if (const char* g = getenv("ASTG_MIXED_REPLACE")) mixedReplace_ = std::max(0, atoi(g));
     if (const char* g = getenv("ASTG_MIXED_ETA")) mixedEta_ = std::max(0.01, atof(g));
@@ -986,6 +989,17 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
     HIPCHK(part_.alloc((size_t)(2 * gs_ + 2 * gb_ + 8) * 8));
     HIPCHK(hipStreamSynchronize(stream_));
     return ASTG_OK;
+}
+
+template <typename T>
+void PcgDevice::launchSpmv(int n, const int* rs, const int* col, const T* A, const T* s, T* q, const Parts& Pt,
+                           const double* scal) {
+    if (spmvLpr_ == 4)
+        k_spmv_dot<T, 4><<<gs_, BS, 0, stream_>>>(n, rs, col, A, s, q, Pt, scal);
+    else if (spmvLpr_ == 16)
+        k_spmv_dot<T, 16><<<gs_, BS, 0, stream_>>>(n, rs, col, A, s, q, Pt, scal);
+    else
+        k_spmv_dot<T, 8><<<gs_, BS, 0, stream_>>>(n, rs, col, A, s, q, Pt, scal);
 }
 
 int PcgDevice::permuteSetup(std::string& err) {
@@ -1202,7 +1216,7 @@ int PcgDevice::pcgInner(const T* A, const T* invd, const T* b, double* x, double
     int batch = 4;
     for (;;) {
         for (int j = 0; j < batch && it < maxIt; ++j, ++it) {
-            k_spmv_dot<T><<<gs, BS, 0, stream_>>>(n, rs, col, A, s, q, Pt, scal);
+            launchSpmv<T>(n, rs, col, A, s, q, Pt, scal);
             k_axpy_res<T, double><<<gs, BS, 0, stream_>>>(n, s, q, x, r, Pt, scal, norm, it);
             if (sizeof(T) == 4 && replaceEvery > 0 && (it + 1) % replaceEvery == 0)
                 k_replace_res<<<gs, BS, 0, stream_>>>(n, rs, col, sVal64(), x, b64,
@@ -1418,11 +1432,11 @@ int PcgDevice::benchKernel(int which, int reps, double& avgMs, double& bytes, st
         HIPCHK(hipEventRecord(e0, stream_));
         switch (which) {
             case ASTG_K_SPMV64:
-                k_spmv_dot<double><<<gs, BS, 0, stream_>>>(nv, rs, col, sVal64(), s_.as<double>(),
+                launchSpmv<double>(nv, rs, col, sVal64(), s_.as<double>(),
                                                            q_.as<double>(), P64, scal);
                 break;
             case ASTG_K_SPMV32:
-                k_spmv_dot<float><<<gs, BS, 0, stream_>>>(nv, rs, col, val32_.as<float>(), s_.as<float>(),
+                launchSpmv<float>(nv, rs, col, val32_.as<float>(), s_.as<float>(),
                                                           q_.as<float>(), P32, scal);
                 break;
             case ASTG_K_FWD64:

@@ -5,15 +5,17 @@ host's cores) solving the same assembled system to the same tolerance.
 
 Bar: the FP64 solve stops within max(2, 1 %) iterations of the reference
 (the inf-norm stopping test sits on a residual plateau, where the summation
-order of the dots moves it by an iteration or two). Run for exactly the
-reference's number of iterations, its pressure differs from the reference's
-by less than a quarter of what one more iteration changes, and by less than
-1e-5 relative L2: after hundreds of iterations, dots summed in another order
-(the reference sums serially) leave the iterates apart by a small fraction of
-one iteration's progress, not by a fixed amount. The mixed form within 1e-4
-relative L2 of the reference (its stated bound against FP64). Skipped where
-the reference build is absent. The reference solve takes about 15 s on 16
-cores.
+order of the dots moves it by an iteration or two) with its relative
+residual, computed on the host, within the 1e-6 tolerance; and its pressure
+is as close to the exact solution as the reference's own (at most 1.5x the
+reference's distance, the exact solution being the device solve to 1e-12).
+Two solves that differ only in rounding (dot and row-sum orders) follow
+trajectories that drift apart by up to a fraction of one iteration's
+progress over hundreds of iterations, so the distance between the two 1e-6
+solutions is compared with the size of that tolerance, not with zero. The
+mixed form within 1e-4 relative L2 of the reference (its stated bound against
+FP64). Skipped where the reference build is absent. The reference solve takes
+about 15 s on 16 cores.
 """
 import numpy as np
 import pytest
@@ -38,19 +40,19 @@ def test_full_size_pressure_matches_reference_solver():
         print("%s: n=%d iterations %d" % (name, r.n, r.iterations), flush=True)
     xr, _, it, ok = ref.pcg_csr(rs, col, val, b, tol=1e-6, maxit=3000)
     assert ok
-    rel = lambda a: np.linalg.norm(a - xr) / np.linalg.norm(xr)
-    print("reference: %d iterations; rel L2 fp64 %.3e, mixed %.3e" % (it, rel(sols["fp64"][0]), rel(sols["mixed"][0])),
+    rel = lambda a, y: np.linalg.norm(a - y) / np.linalg.norm(y)
+    ctx.snapshot(restore=True)
+    ctx.project(info["dt"], info["rho"], astg.default_params(tol=1e-12, max_iterations=3000))
+    xt = ctx.system(r.n, r.nnz)[4]
+    rows = np.repeat(np.arange(len(rs) - 1), np.diff(rs))
+    res = np.max(np.abs(b - np.bincount(rows, weights=val * sols["fp64"][0][col], minlength=len(b)))) / np.max(np.abs(b))
+    e_ref, e_fp64 = rel(xr, xt), rel(sols["fp64"][0], xt)
+    print("reference: %d iterations; fp64 %d iterations, relative residual %.3e; distance to the 1e-12 solution: "
+          "reference %.3e, fp64 %.3e; fp64 to reference %.3e, mixed to reference %.3e" % (
+              it, sols["fp64"][1], res, e_ref, e_fp64, rel(sols["fp64"][0], xr), rel(sols["mixed"][0], xr)),
           flush=True)
     assert abs(sols["fp64"][1] - it) <= max(2, it // 100)
-    assert rel(sols["mixed"][0]) < 1e-4
-    xs = {}
-    for k in (it, it + 1):
-        ctx.snapshot(restore=True)
-        ctx.project(info["dt"], info["rho"], astg.default_params(tol=0.0, max_iterations=k), allow_unconverged=True)
-        xs[k] = ctx.system(r.n, r.nnz)[4]
-    step = np.linalg.norm(xs[it + 1] - xs[it]) / np.linalg.norm(xs[it])
-    print("fp64 after the reference's %d iterations: rel L2 %.3e (one more iteration moves it by %.3e)" % (
-        it, rel(xs[it]), step), flush=True)
-    assert rel(xs[it]) < 0.25 * step
-    assert rel(xs[it]) < 1e-5
+    assert res <= 1e-6
+    assert e_fp64 <= 1.5 * e_ref
+    assert rel(sols["mixed"][0], xr) < 1e-4
     ctx.close()
