@@ -413,7 +413,8 @@ __global__ void __launch_bounds__(BS) k_bwd_dot(SweepPlan S, StencilView<T> St, 
                 const long long td = S.trace ? wall_clock64() : 0;
                 zk = a * inv;
                 Sentinel<T>::store(&zp[pos], zk);
-                z[S.vecPos ? pos : k] = zk;
+                // in sweep order the PCG reads z from zp itself (same position)
+                if (!S.vecPos) z[k] = zk;
                 if (S.trace) {
                     S.trace[3 * (long)k] = tg;
                     S.trace[3 * (long)k + 1] = td;
@@ -430,7 +431,7 @@ __global__ void __launch_bounds__(BS) k_bwd_dot(SweepPlan S, StencilView<T> St, 
             Sentinel<T>::store(&zp[pos], (T)0);
             Sentinel<T>::storeSent(&ynext[pos]);
         } else if (S.vecPos) {
-            z[pos] = (T)0;  // padding position: z stays zero there
+            Sentinel<T>::store(&zp[pos], (T)0);  // padding: z (read from zp) stays zero there
         }
     }
     const double t = block_reduce<false>(v);
@@ -1210,7 +1211,10 @@ int PcgDevice::pcgInner(const T* A, const T* invd, const T* b, double* x, double
         forceSpinError_ = false;
         k_set_err<<<1, 64, 0, stream_>>>(cnt_.as<int>(), E_FWD);
     }
-    k_xpby<T><<<gs, BS, 0, stream_>>>(n, z, s, Pt, scal, 0, 0);
+    // z as the PCG reads it: the backward sweep's published copy in sweep
+    // order (valid until the next forward sweep re-arms it)
+    const T* zv = perm_ ? zp_.as<T>() : z;
+    k_xpby<T><<<gs, BS, 0, stream_>>>(n, zv, s, Pt, scal, 0, 0);
     HIPCHK(hipGetLastError());
     int it = 0;
     int batch = 4;
@@ -1223,7 +1227,7 @@ int PcgDevice::pcgInner(const T* A, const T* invd, const T* b, double* x, double
                                                       reinterpret_cast<float*>(r), Pt, scal, norm);
             if ((rc = launchFwd<T>(invd, r, it + 1, 1, it, maxIt, norm, tolRel, err))) return rc;
             if ((rc = launchBwd<T>(invd, r, z, it + 1, (it + 1) & 1, err))) return rc;
-            k_xpby<T><<<gs, BS, 0, stream_>>>(n, z, s, Pt, scal, 1, it);
+            k_xpby<T><<<gs, BS, 0, stream_>>>(n, zv, s, Pt, scal, 1, it);
         }
         HIPCHK(hipGetLastError());
         rc = syncScalars(err);
@@ -1389,8 +1393,9 @@ int PcgDevice::benchKernel(int which, int reps, double& avgMs, double& bytes, st
             break;
         case ASTG_K_BWD64: case ASTG_K_BWD32:
             // stencil coefficients + indices; per row: order, start, r-term count, y, next y
-            // armed, pivot, r, z written twice (sweep order and row order), z gathered once
-            bytes = (w + 4) * stB_.E + (7.0 * w + 12) * n;
+            // armed, pivot, r, z published (and, in row order, written to the PCG vector),
+            // z gathered once
+            bytes = (w + 4) * stB_.E + ((perm_ ? 6.0 : 7.0) * w + 12) * n;
             break;
         case ASTG_K_MIC:
             // A values + pattern read, factor written twice (both triangles), invd
