@@ -26,8 +26,8 @@ KERNELS = tuple((nm, i) for i, nm in enumerate(KERNEL_NAMES))
 # HBM bytes per launch from rocprofv3 counter runs (FETCH_SIZE + WRITE_SIZE,
 # scaled by the copy-kernel calibration) of `bench.py --kernels-only` on this
 # workload, as summarised in profiles/traffic_projection_256.md.
-TRAFFIC = {"spmv64": 688332025, "fwd64": 628833848, "bwd64": 715745359, "spmv32": 416772589,
-           "fwd32": 434982115, "bwd32": 513363590, "copy": 56200384, "mic": 7374273153}
+TRAFFIC = {"spmv64": 688377572, "fwd64": 629399413, "bwd64": 685888379, "spmv32": 416803946,
+           "fwd32": 434900715, "bwd32": 499123025, "copy": 56200384, "mic": 7374325768}
 
 
 def log(*a):
