@@ -108,6 +108,7 @@ private:
     std::vector<int> off_, nleaf_, nnode_;
     int nLeaf_ = 0, nGhost_ = 0, nHalf_ = 0, nTJ_ = 0, nCBH_ = 0, nOpenTilt_ = 0, nVelTilt_ = 0, nHalfMac_ = 0;
     DevBuf map_, nx_, ny_, nz_, nflag_, te_, tb_, thd_, level_, inHalf_;
+    std::vector<uint16_t> hnx_, hny_, hnz_;  // node coordinates on the host (kept across tilt updates)
     DevBuf leaf_, ghost_, half_, tj_, cbh_, openTilt_, velTilt_;
     DevBuf hmG_, hmT_, hmA_, hmS_;   // coarse MAC faces written from half-tilts
     DevBuf bad_;

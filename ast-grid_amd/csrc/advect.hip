@@ -1168,19 +1168,33 @@ int AdvectDevice::upload(const Octree& o, const ProjectTables& t, bool keepField
         err = "index maps too large";
         return ASTG_ERR_ARG;
     }
-    std::vector<int> map((size_t)mtot);
-    std::vector<uint16_t> nx(G_), ny(G_), nz(G_);
-    std::vector<uint8_t> nf(G_), inHalf(G_, 0);
+    // A tilt update (keepFields) keeps the node sets: the index maps, node
+    // coordinates and flags, leaf and ghost lists and levels on the device
+    // stay as they are; only the tilt-state dependent lists are rebuilt.
+    const bool sameNodes = keepFields;
+    std::vector<int> map(sameNodes ? 0 : (size_t)mtot);
+    std::vector<uint16_t>& nx = hnx_;
+    std::vector<uint16_t>& ny = hny_;
+    std::vector<uint16_t>& nz = hnz_;
+    if (!sameNodes) {
+        nx.assign(G_, 0);
+        ny.assign(G_, 0);
+        nz.assign(G_, 0);
+    }
+    std::vector<uint8_t> nf(sameNodes ? 0 : G_), inHalf(G_, 0);
     std::vector<int> leaf, ghostL, openTilt, half, tj, cbh;
     for (int l = 0; l < L_; ++l) {
         const Level& Lv = o.lv[l];
-        for (size_t q = 0; q < Lv.map.size(); ++q) map[moff[l] + q] = Lv.map[q] >= 0 ? Lv.map[q] + off_[l] : -1;
+        if (!sameNodes)
+            for (size_t q = 0; q < Lv.map.size(); ++q) map[moff[l] + q] = Lv.map[q] >= 0 ? Lv.map[q] + off_[l] : -1;
         for (int i = 0; i < Lv.nnode; ++i) {
             const int g = off_[l] + i;
-            nx[g] = Lv.x[i];
-            ny[g] = Lv.y[i];
-            nz[g] = Lv.z[i];
-            nf[g] = Lv.flag[i];
+            if (!sameNodes) {
+                nx[g] = Lv.x[i];
+                ny[g] = Lv.y[i];
+                nz[g] = Lv.z[i];
+                nf[g] = Lv.flag[i];
+            }
             if (i < Lv.nleaf) {
                 leaf.push_back(g);
                 const uint8_t b = o.tilt[l][i].bits;
@@ -1244,17 +1258,21 @@ int AdvectDevice::upload(const Octree& o, const ProjectTables& t, bool keepField
         if (e != hipSuccess || !bytes) return e;
         return hipMemcpyAsync(d.p, src, bytes, hipMemcpyHostToDevice, stream_);
     };
-    HIPCHK(up(map_, map.data(), map.size() * 4));
-    HIPCHK(up(nx_, nx.data(), (size_t)G_ * 2));
-    HIPCHK(up(ny_, ny.data(), (size_t)G_ * 2));
-    HIPCHK(up(nz_, nz.data(), (size_t)G_ * 2));
-    HIPCHK(up(nflag_, nf.data(), (size_t)G_));
+    if (!sameNodes) {
+        HIPCHK(up(map_, map.data(), map.size() * 4));
+        HIPCHK(up(nx_, nx.data(), (size_t)G_ * 2));
+        HIPCHK(up(ny_, ny.data(), (size_t)G_ * 2));
+        HIPCHK(up(nz_, nz.data(), (size_t)G_ * 2));
+        HIPCHK(up(nflag_, nf.data(), (size_t)G_));
+        HIPCHK(up(leaf_, leaf.data(), leaf.size() * 4));
+        HIPCHK(up(ghost_, ghostL.data(), ghostL.size() * 4));
+        HIPCHK(level_.alloc((size_t)G_));
+        HIPCHK(hipMemcpyAsync(level_.p, t.level.data(), (size_t)G_, hipMemcpyHostToDevice, stream_));
+    }
     HIPCHK(up(te_, t.tilt_e.data(), (size_t)G_ * 8));
     HIPCHK(up(tb_, t.tilt_bits.data(), (size_t)G_));
     HIPCHK(up(thd_, t.tilt_hd.data(), (size_t)G_));
     HIPCHK(up(inHalf_, inHalf.data(), (size_t)G_));
-    HIPCHK(up(leaf_, leaf.data(), leaf.size() * 4));
-    HIPCHK(up(ghost_, ghostL.data(), ghostL.size() * 4));
     HIPCHK(up(half_, half.data(), half.size() * 4));
     HIPCHK(up(tj_, tj.data(), tj.size() * 4));
     HIPCHK(up(cbh_, cbh.data(), cbh.size() * 4));
@@ -1266,8 +1284,6 @@ int AdvectDevice::upload(const Octree& o, const ProjectTables& t, bool keepField
     HIPCHK(up(hmT_, hmT.data(), hmT.size() * 4));
     HIPCHK(up(hmA_, hmA.data(), hmA.size() * 4));
     HIPCHK(up(hmS_, hmS.data(), hmS.size() * 4));
-    HIPCHK(level_.alloc((size_t)G_));
-    HIPCHK(hipMemcpyAsync(level_.p, t.level.data(), (size_t)G_, hipMemcpyHostToDevice, stream_));
     if (keepFields) {
         HIPCHK(hipStreamSynchronize(stream_));
         return ASTG_OK;
