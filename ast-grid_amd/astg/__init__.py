@@ -5,8 +5,10 @@ The library is ``libastg.so`` next to this file, built by
 arrays are passed as host pointers. Loading fails loudly when the library is
 missing: there is no fallback implementation.
 """
+import atexit
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -36,9 +38,11 @@ EXPORTS = [
     "astg_set_scalar", "astg_get_scalar", "astg_post_project", "astg_scalar_update_ghost", "astg_get_dual",
     "astg_sample_points", "astg_apply_sphere_source", "astg_add_force", "astg_adjust_e",
     "astg_set_tilt_adaptivity", "astg_advect", "astg_step",
+    "astg_set_boundary_sdf", "astg_sphere_sdf", "astg_wipe_boundary",
+    "astg_field_layout", "astg_fields_upload", "astg_fields_download", "astg_project_fields",
 ]
 FIELD_DENSITY, FIELD_HEAT = 0, 1
-SAMPLE_VELOCITY, SAMPLE_MAC, SAMPLE_BACKTRACE, SAMPLE_DENSITY, SAMPLE_HEAT = range(5)
+SAMPLE_VELOCITY, SAMPLE_MAC, SAMPLE_BACKTRACE, SAMPLE_DENSITY, SAMPLE_HEAT, SAMPLE_SDF = range(6)
 K_SPMV64, K_FWD64, K_BWD64, K_MIC, K_ASSEMBLE, K_SPMV32, K_FWD32, K_BWD32, K_COPY = range(9)
 
 
@@ -54,6 +58,13 @@ class BatchTuning(ctypes.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Fields(ctypes.Structure):
+    """astg_fields: caller-owned field arrays in the context's flat layout."""
+    _fields_ = [("mac", ctypes.c_void_p), ("tilt", ctypes.c_void_p), ("density_cell", ctypes.c_void_p),
+                ("density_vert", ctypes.c_void_p), ("heat_cell", ctypes.c_void_p), ("heat_vert", ctypes.c_void_p),
+                ("device_pointers", ctypes.c_int), ("stream", ctypes.c_void_p)]
 
 
 class PcgResult(ctypes.Structure):
@@ -78,6 +89,14 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError("libastg.so not built: run `python ast-grid_amd/build.py` (or __graft_entry__.build())")
+    # One HIP runtime per process: PyTorch-ROCm ships its own libamdhip64
+    # under the same soname, and a second runtime loaded beside it makes the
+    # one loaded later see no GPU. Loading torch first lets the library bind
+    # to torch's runtime, so torch tensors and streams are valid here.
+    try:
+        import torch  # noqa: F401
+    except Exception:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     P = ctypes.c_void_p
     I = ctypes.c_int
@@ -126,6 +145,14 @@ def lib():
         "astg_set_tilt_adaptivity": (I, [P, I]),
         "astg_advect": (I, [P, D]),
         "astg_step": (I, [P, D, D, ctypes.POINTER(PcgParams), ctypes.POINTER(PcgResult)]),
+        "astg_set_boundary_sdf": (I, [P, P, P]),
+        "astg_sphere_sdf": (I, [I, D, D, D, D, D, P]),
+        "astg_wipe_boundary": (I, [P]),
+        "astg_field_layout": (I, [P, P, P]),
+        "astg_fields_upload": (I, [P, ctypes.POINTER(Fields)]),
+        "astg_fields_download": (I, [P, ctypes.POINTER(Fields)]),
+        "astg_project_fields": (I, [P, D, D, ctypes.POINTER(PcgParams), ctypes.POINTER(Fields),
+                                    ctypes.POINTER(PcgResult)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -164,6 +191,34 @@ def split_spheres(R, levels, spheres, band):
     return out
 
 
+# Contexts still open when the interpreter exits are closed by an exit hook,
+# in creation order, while the HIP runtime and every module are still loaded
+# (not by __del__ during interpreter teardown, whose order is undefined).
+_live = weakref.WeakValueDictionary()
+_next_id = [0]
+
+
+@atexit.register
+def _close_live_contexts():
+    for key in sorted(_live.keys()):
+        c = _live.get(key)
+        if c is not None:
+            try:
+                c.close()
+            except Exception:
+                pass
+
+
+def sphere_sdf(R, center, radius, h0=None):
+    """R^3 signed distances of a sphere at the finest cell centres (x fastest)."""
+    out = np.empty(R ** 3, np.float64)
+    cx, cy, cz = center
+    rc = lib().astg_sphere_sdf(R, 1.0 / R if h0 is None else h0, cx, cy, cz, radius, _ptr(out))
+    if rc:
+        raise AstgError(rc, "sphere_sdf")
+    return out
+
+
 class Context:
     """One library context (device >= 0: HIP stream + device buffers; -1: host only)."""
 
@@ -175,6 +230,9 @@ class Context:
             raise AstgError(rc, "astg_create(device=%d)" % device)
         self.h = h
         self.levels = 0
+        self.device = device
+        _next_id[0] += 1
+        _live[_next_id[0]] = self
 
     def close(self):
         if self.h:
@@ -202,6 +260,81 @@ class Context:
         self.levels = levels
         self.R = R
         self.closed = closed_flags
+
+    def set_boundary_sdf(self, sdf, slip_center=None):
+        """Collider SDF on the finest grid (R^3, x fastest, cell centres), or None
+        to remove it; slip_center defaults to the reference's (0.5, 0.5, 0.45)."""
+        if sdf is None:
+            self._chk(self._L.astg_set_boundary_sdf(self.h, None, None), "set_boundary_sdf")
+            return
+        a = np.ascontiguousarray(sdf, np.float64).ravel()
+        assert a.size == self.R ** 3
+        c = None if slip_center is None else np.ascontiguousarray(slip_center, np.float64)
+        self._chk(self._L.astg_set_boundary_sdf(self.h, _ptr(a), _ptr(c)), "set_boundary_sdf")
+
+    # ---- fields in the caller's memory (flat layout, see field_layout) ----
+    def field_layout(self):
+        """(G, offsets): every level's nodes concatenated; level l at [offsets[l], offsets[l+1])."""
+        G = ctypes.c_longlong()
+        off = np.empty(self.levels + 1, np.int64)
+        self._chk(self._L.astg_field_layout(self.h, ctypes.byref(G), _ptr(off)), "field_layout")
+        return G.value, off
+
+    def _fields(self, arrays, stream):
+        f = Fields()
+        dev = None
+        for k in ("mac", "tilt", "density_cell", "density_vert", "heat_cell", "heat_vert"):
+            a = arrays.get(k)
+            if a is None:
+                continue
+            if hasattr(a, "data_ptr"):  # a torch tensor: device memory, float64, contiguous
+                if a.dtype.__str__() != "torch.float64" or not a.is_contiguous():
+                    raise ValueError("field %s must be a contiguous float64 tensor" % k)
+                setattr(f, k, a.data_ptr())
+                d = bool(a.is_cuda)
+            else:
+                if a.dtype != np.float64 or not a.flags.c_contiguous:
+                    raise ValueError("field %s must be a contiguous float64 array" % k)
+                setattr(f, k, a.ctypes.data)
+                d = False
+            if dev is not None and dev != d:
+                raise ValueError("fields mix host and device memory")
+            dev = d
+        f.device_pointers = 1 if dev else 0
+        f.stream = stream if dev else None
+        return f
+
+    @staticmethod
+    def _torch_stream(stream):
+        if stream is not None:
+            return stream
+        try:
+            import torch
+            return torch.cuda.current_stream().cuda_stream
+        except Exception:
+            return None
+
+    def upload_fields(self, stream=None, **arrays):
+        """Copies caller fields (numpy arrays or cuda torch tensors, flat layout) in;
+        device copies run in the order of `stream` (default: torch's current stream)."""
+        f = self._fields(arrays, self._torch_stream(stream))
+        self._chk(self._L.astg_fields_upload(self.h, ctypes.byref(f)), "fields_upload")
+
+    def download_fields(self, stream=None, **arrays):
+        f = self._fields(arrays, self._torch_stream(stream))
+        self._chk(self._L.astg_fields_download(self.h, ctypes.byref(f)), "fields_download")
+
+    def project_fields(self, dt, rho, params, mac, tilt, stream=None, allow_unconverged=False):
+        """project(dt) on caller-owned velocity arrays (updated in place)."""
+        f = self._fields({"mac": mac, "tilt": tilt}, self._torch_stream(stream))
+        res = PcgResult()
+        rc = self._L.astg_project_fields(self.h, dt, rho, ctypes.byref(params), ctypes.byref(f), ctypes.byref(res))
+        self._chk(rc, "project_fields", allow=(ASTG_ERR_NOT_CONVERGED,) if allow_unconverged else ())
+        return res
+
+    def wipe_boundary(self):
+        """wipeBoundary: density and heat inside the collider set to 0."""
+        self._chk(self._L.astg_wipe_boundary(self.h), "wipe_boundary")
 
     def set_tilt_open(self, level, mask):
         m = np.ascontiguousarray(mask, np.uint8)

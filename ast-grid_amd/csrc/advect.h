@@ -26,6 +26,7 @@
 #include <string>
 #include <vector>
 
+#include "consts.h"
 #include "octree.h"
 #include "pcg.h"
 #include "project.h"
@@ -41,6 +42,7 @@ struct AdvTopo {
     const double* te;               // tilt size e
     const uint8_t* tb;              // TB_* tilt bits
     const int8_t* thd;              // half-tilt direction
+    const double* sdf;              // boundary SDF on the finest grid (R^3 cell centres), null: no collider
 };
 struct AdvVel {
     const double* mac;    // 3G axis-major
@@ -87,6 +89,8 @@ public:
     int decideTilts(ProjectDevice& pd, double dThr, double dt, double vThr, double maxThr, std::vector<uint8_t>& open,
                     std::string& err);
     int leaves(int level) const { return nleaf_[level]; }
+    // Current buffer of a scalar field: field 0 density, 1 heat; kind 0 cell, 1 vertex (G values).
+    double* scalarPtr(int field, int kind) { return scal_[field][cur_[field]][kind].as<double>(); }
     // Sphere smoke source of the octree smoke example (sphereSource,
     // example/octree_smoke_3d/main.cpp:23-81) with parameters; ghost: update
     // the scalar ghosts afterwards (the example does on its first call).
@@ -94,8 +98,20 @@ public:
                     double speed, bool ghost, std::string& err);
     // addForce (octree_omnigrid_solver3.cpp:917-972): buoyancy along +z.
     int addForce(ProjectDevice& pd, double dt, double fd, double ft, std::string& err);
+    // wipeBoundary (octree_omnigrid_solver3.cpp:1240-1292): density and heat
+    // are zeroed where the boundary SDF is negative (no-op without a collider).
+    int wipeBoundary(std::string& err);
+    bool hasCollider() const { return hasSdf_; }
 
 private:
+    // This context's values of the kernels' constant tables (level
+    // resolutions, map offsets, spacings, slip centre, dual-grid matrices):
+    // uploaded by upload(), again whenever another context used the tables.
+    int bindConstants(bool force, std::string& err);
+    uint64_t constOwner_ = newConstOwner();
+    std::vector<int> cRes_, cMoff_;
+    std::vector<double> cHl_, cDual_;
+    double cSlip_[3] = {0.5, 0.5, 0.45};
     int omniGhost(double* cell, double* vert, int nc, std::string& err);
     int checkBad(std::string& err);
     AdvTopo topo() const;      // current tilt state
@@ -119,6 +135,13 @@ private:
     DevBuf pts_, out_;               // point queries
     DevBuf sampTe_, sampTb_, sampThd_, open_;  // tilt state before an update, decisions
     bool sampValid_ = false;
+    // collider: SDF grid, MAC faces inside it in the reference's visiting order
+    // (faceG node, faceA axis), their order index per (node, axis) (-1 if
+    // outside), and the published new values of the serial slip constraint
+    bool hasSdf_ = false;
+    DevBuf sdf_, slipFaceG_, slipFaceA_, slipOrd_, slipPub_, slipErr_;
+    int nSlip_ = 0, slipGrid_ = 0;
+    int slipMac(ProjectDevice& pd, std::string& err);
 };
 
 }  // namespace astg

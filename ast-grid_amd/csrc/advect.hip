@@ -8,9 +8,11 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "advect.h"
 #include "device_common.h"
+#include "linrec.h"
 
 #define HIPCHK(expr)                                                                 \
     do {                                                                             \
@@ -40,6 +42,7 @@ __constant__ double c_hl[kMaxLevels];
 // least-squares dual-graph matrices [x][y][z][n0][n1][n2][n3] (3 x 7 each)
 __constant__ double c_dual[128 * 21];
 __constant__ double c_dualTilt[12];  // 3 x 4
+__constant__ double c_slip[3];       // centre of the slip normals (omni_boundary_solver3.h:27,57,77)
 
 // OCT_NB_TILT_DXDYDZ (tilt corner k of a cell), restated from the corner order
 __constant__ int c_nbt[8][3] = {{1, 1, 1}, {0, 1, 1}, {0, 0, 1}, {1, 0, 1},
@@ -95,39 +98,114 @@ __device__ int getUntil(const Topo& T, int l, int x, int y, int z, bool toFinest
     }
 }
 
+// ---- sampler arithmetic, generic over the position type P ------------------
+// P = double for the forward samplers, P = D3 (value and gradient with
+// respect to the sampled position) for the adjoint's position derivatives.
+// Field values are always double: every sampler is linear in them, with
+// weights that depend on the position only.
+struct D3 {
+    double v, d[3];
+    __device__ D3() {}
+    __device__ D3(double x) : v(x), d{0.0, 0.0, 0.0} {}
+};
+__device__ inline double val(double x) { return x; }
+__device__ inline double val(const D3& x) { return x.v; }
+__device__ inline D3 operator+(const D3& a, const D3& b) {
+    D3 r;
+    r.v = a.v + b.v;
+    for (int k = 0; k < 3; ++k) r.d[k] = a.d[k] + b.d[k];
+    return r;
+}
+__device__ inline D3 operator-(const D3& a, const D3& b) {
+    D3 r;
+    r.v = a.v - b.v;
+    for (int k = 0; k < 3; ++k) r.d[k] = a.d[k] - b.d[k];
+    return r;
+}
+__device__ inline D3 operator*(const D3& a, const D3& b) {
+    D3 r;
+    r.v = a.v * b.v;
+    for (int k = 0; k < 3; ++k) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
+    return r;
+}
+__device__ inline D3 operator/(const D3& a, const D3& b) {
+    D3 r;
+    r.v = a.v / b.v;
+    for (int k = 0; k < 3; ++k) r.d[k] = (a.d[k] * b.v - a.v * b.d[k]) / (b.v * b.v);
+    return r;
+}
+__device__ inline D3 operator+(const D3& a, double b) { return a + D3(b); }
+__device__ inline D3 operator+(double a, const D3& b) { return D3(a) + b; }
+__device__ inline D3 operator-(const D3& a, double b) { return a - D3(b); }
+__device__ inline D3 operator-(double a, const D3& b) { return D3(a) - b; }
+__device__ inline D3 operator*(const D3& a, double b) {
+    D3 r;
+    r.v = a.v * b;
+    for (int k = 0; k < 3; ++k) r.d[k] = a.d[k] * b;
+    return r;
+}
+__device__ inline D3 operator*(double a, const D3& b) { return b * a; }
+__device__ inline D3 operator/(const D3& a, double b) {
+    D3 r;
+    r.v = a.v / b;
+    for (int k = 0; k < 3; ++k) r.d[k] = a.d[k] / b;
+    return r;
+}
+__device__ inline D3& operator+=(D3& a, const D3& b) { return a = a + b; }
+__device__ inline D3& operator-=(D3& a, double b) { return a = a - b; }
+__device__ inline D3& operator*=(D3& a, double b) { return a = a * b; }
+// min / max keep the operand the value picks (ties: the first)
+__device__ inline double pmin(double a, double b) { return fmin(a, b); }
+__device__ inline double pmax(double a, double b) { return fmax(a, b); }
+__device__ inline D3 pmin(const D3& a, const D3& b) { return b.v < a.v ? b : a; }
+__device__ inline D3 pmax(const D3& a, const D3& b) { return b.v > a.v ? b : a; }
+
+// Loads of field values. PlainLd reads memory; the adjoint's probe policy
+// returns a one-hot weight per load instead (transposed scatters).
+struct PlainLd {
+    __device__ double operator()(const double* base, size_t i) const { return base[i]; }
+};
+
 // getBarycentric (util.h:259-282) with iLow = 0
-__device__ inline void bary(double x, int iHigh, int& i, double& f) {
-    const double s = floor(x);
+template <class P>
+__device__ inline void bary(P x, int iHigh, int& i, P& f) {
+    const double s = floor(val(x));
     i = (int)s;
     if (iHigh == 0) {
         i = 0;
-        f = 0;
+        f = P(0.0);
     } else if (i < 0) {
         i = 0;
-        f = 0;
+        f = P(0.0);
     } else if (i > iHigh - 1) {
         i = iHigh - 1;
-        f = 1;
+        f = P(1.0);
     } else {
         f = x - s;
     }
 }
 // getBaryCentric, vector form (util.h:244-257): truncation, clamps on i and f
-__device__ inline void baryVec(double x, int high, int& i, double& f) {
-    i = (int)x;
+template <class P>
+__device__ inline void baryVec(P x, int high, int& i, P& f) {
+    i = (int)val(x);
     i = max(i, 0);
     i = min(i, high - 1);
     f = x - (double)i;
-    f = fmax(f, 0.0);
-    f = fmin(f, 1.0);
+    f = pmax(f, P(0.0));
+    f = pmin(f, P(1.0));
 }
 
-__device__ inline double lerp(double v0, double v1, double f) { return v0 * (1 - f) + v1 * f; }
-__device__ inline double bilerp(double v00, double v01, double v11, double v10, double fx, double fy) {
+template <class V, class F>
+__device__ inline auto lerp(const V& v0, const V& v1, const F& f) {
+    return v0 * (1 - f) + v1 * f;
+}
+template <class V, class F>
+__device__ inline auto bilerp(const V& v00, const V& v01, const V& v11, const V& v10, const F& fx, const F& fy) {
     return lerp(lerp(v00, v10, fx), lerp(v01, v11, fx), fy);
 }
-__device__ inline double trilerp(double f000, double f100, double f010, double f110, double f001, double f101,
-                                 double f011, double f111, double tx, double ty, double fz) {
+template <class V, class F>
+__device__ inline auto trilerp(const V& f000, const V& f100, const V& f010, const V& f110, const V& f001,
+                               const V& f101, const V& f011, const V& f111, const F& tx, const F& ty, const F& fz) {
     return lerp(bilerp(f000, f010, f110, f100, tx, ty), bilerp(f001, f011, f111, f101, tx, ty), fz);
 }
 
@@ -149,11 +227,12 @@ __device__ int locate(const Topo& T, double px, double py, double pz) {
 }
 
 // OctreeFaceCenteredGrid3::sampleX/Y/Z (octree_face_centered_grid3.cpp:47-113)
-__device__ double macAxis(const Topo& T, const double* mac, int a, int l, double rx, double ry, double rz) {
+template <class P, class LD>
+__device__ P macAxisR(const Topo& T, LD& ld, const double* mac, int a, int l, P rx, P ry, P rz) {
     const int r = c_res[l];
-    const double rr[3] = {rx, ry, rz};
+    const P rr[3] = {rx, ry, rz};
     int i0[3], i1[3];
-    double f[3];
+    P f[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         if (d == a) {
@@ -164,10 +243,9 @@ __device__ double macAxis(const Topo& T, const double* mac, int a, int l, double
             i1[d] = min(i0[d] + 1, r - 2);
         }
     }
-    const double* u = mac + (size_t)a * c_G;
     auto U = [&](int x, int y, int z) {
         const int g = nd(T, l, x, y, z);
-        return g >= 0 ? u[g] : 0.0;
+        return g >= 0 ? ld(mac, (size_t)a * c_G + g) : 0.0;
     };
     return trilerp(U(i0[0], i0[1], i0[2]), U(i1[0], i0[1], i0[2]), U(i0[0], i1[1], i0[2]), U(i1[0], i1[1], i0[2]),
                    U(i0[0], i0[1], i1[2]), U(i1[0], i0[1], i1[2]), U(i0[0], i1[1], i1[2]), U(i1[0], i1[1], i1[2]),
@@ -175,26 +253,52 @@ __device__ double macAxis(const Topo& T, const double* mac, int a, int l, double
 }
 
 // OctreeFaceCenteredGrid3::sample (octree_face_centered_grid3.cpp:36-45)
-__device__ bool macSample(const Topo& T, const double* mac, double px, double py, double pz, double* out) {
-    const int l = locate(T, px, py, pz);
+template <class P, class LD>
+__device__ bool macSampleR(const Topo& T, LD& ld, const double* mac, P px, P py, P pz, P* out) {
+    const int l = locate(T, val(px), val(py), val(pz));
     if (l < 0) {
-        out[0] = out[1] = out[2] = NAN;
+        out[0] = out[1] = out[2] = P(NAN);
         return false;
     }
     const double dh = c_hl[l];
-    const double rx = px / dh, ry = py / dh, rz = pz / dh;
+    const P rx = px / dh, ry = py / dh, rz = pz / dh;
 #pragma unroll
-    for (int a = 0; a < 3; ++a) out[a] = macAxis(T, mac, a, l, rx, ry, rz);
+    for (int a = 0; a < 3; ++a) out[a] = macAxisR(T, ld, mac, a, l, rx, ry, rz);
     return true;
+}
+__device__ inline bool macSample(const Topo& T, const double* mac, double px, double py, double pz, double* out) {
+    PlainLd ld;
+    return macSampleR(T, ld, mac, px, py, pz, out);
+}
+
+// CellCenteredGrid::sample (cell_centered_grid.h:40-54) of the boundary SDF:
+// cell-centred trilinear with clamped indices (getBaryCentric, vector form).
+// No collider: the reference's constant positive SDF.
+template <class P>
+__device__ P sdfSampleT(const Topo& T, P px, P py, P pz) {
+    if (!T.sdf) return P(1e4);
+    int i0, i1, i2;
+    P f0, f1, f2;
+    baryVec(px / c_h0 - 0.5, c_R - 1, i0, f0);
+    baryVec(py / c_h0 - 0.5, c_R - 1, i1, f1);
+    baryVec(pz / c_h0 - 0.5, c_R - 1, i2, f2);
+    const int j0 = min(i0 + 1, c_R - 1), j1 = min(i1 + 1, c_R - 1), j2 = min(i2 + 1, c_R - 1);
+    const size_t R = c_R;
+    auto g = [&](int a, int b, int c) { return T.sdf[((size_t)c * R + b) * R + a]; };
+    return trilerp(g(i0, i1, i2), g(j0, i1, i2), g(i0, j1, i2), g(j0, j1, i2), g(i0, i1, j2), g(j0, i1, j2),
+                   g(i0, j1, j2), g(j0, j1, j2), f0, f1, f2);
+}
+__device__ inline double sdfSample(const Topo& T, double px, double py, double pz) {
+    return sdfSampleT<double>(T, px, py, pz);
 }
 
 // cell-centred trilinear at a given level (sampleGivenRelativePosition,
 // octree_cell_centered_grid3.h:156-177); NC values per node
-template <int NC>
-__device__ void cellRel(const Topo& T, const double* cell, int l, double rx, double ry, double rz, double* out) {
+template <int NC, class P, class LD>
+__device__ void cellRelT(const Topo& T, LD& ld, const double* cell, int l, P rx, P ry, P rz, P* out) {
     const int r = c_res[l];
     int x0, y0, z0;
-    double fx, fy, fz;
+    P fx, fy, fz;
     bary(rx - 0.5, r - 2, x0, fx);
     bary(ry - 0.5, r - 2, y0, fy);
     bary(rz - 0.5, r - 2, z0, fz);
@@ -205,36 +309,46 @@ __device__ void cellRel(const Topo& T, const double* cell, int l, double rx, dou
     for (int c = 0; c < NC; ++c) {
         double v[8];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) v[k] = g[k] >= 0 ? cell[(size_t)NC * g[k] + c] : 0.0;
+        for (int k = 0; k < 8; ++k) v[k] = g[k] >= 0 ? ld(cell, (size_t)NC * g[k] + c) : 0.0;
         out[c] = trilerp(v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], fx, fy, fz);
     }
 }
 
 // OctreeCellCenteredGrid3::sample (octree_cell_centered_grid3.h:97-123)
-template <int NC>
-__device__ void cellSample(const Topo& T, const double* cell, double px, double py, double pz, double* out) {
-    const int l = locate(T, px, py, pz);
+template <int NC, class P, class LD>
+__device__ void cellSampleT(const Topo& T, LD& ld, const double* cell, P px, P py, P pz, P* out) {
+    const int l = locate(T, val(px), val(py), val(pz));
     if (l < 0) {
-        for (int c = 0; c < NC; ++c) out[c] = NAN;
+        for (int c = 0; c < NC; ++c) out[c] = P(NAN);
         return;
     }
     const double dh = c_hl[l];
-    cellRel<NC>(T, cell, l, px / dh, py / dh, pz / dh, out);
+    cellRelT<NC>(T, ld, cell, l, px / dh, py / dh, pz / dh, out);
+}
+template <int NC>
+__device__ inline void cellSample(const Topo& T, const double* cell, double px, double py, double pz, double* out) {
+    PlainLd ld;
+    cellSampleT<NC>(T, ld, cell, px, py, pz, out);
 }
 
-__device__ inline double cellv(const Topo& T, const double* cell, int NC, int l, int x, int y, int z, int c) {
+template <class LD>
+__device__ inline double cellvT(const Topo& T, LD& ld, const double* cell, int NC, int l, int x, int y, int z, int c) {
     const int g = nd(T, l, x, y, z);
-    return g >= 0 ? cell[(size_t)NC * g + c] : 0.0;
+    return g >= 0 ? ld(cell, (size_t)NC * g + c) : 0.0;
+}
+__device__ inline double cellv(const Topo& T, const double* cell, int NC, int l, int x, int y, int z, int c) {
+    PlainLd ld;
+    return cellvT(T, ld, cell, NC, l, x, y, z, c);
 }
 
 // getNeighborCubeValue (octree_util.h:124-185)
-template <int NC>
-__device__ void neighborCube(const Topo& T, const double* cell, const double* vert, int l, int xi, int yi, int zi,
-                             int dir, double (*v)[NC], bool tjoint) {
+template <int NC, class LD>
+__device__ void neighborCube(const Topo& T, LD& ld, const double* cell, const double* vert, int l, int xi, int yi,
+                             int zi, int dir, double (*v)[NC], bool tjoint) {
     const int reDir = ((dir & 1) ? 8 : 6) - dir;
     double centerV[NC];
     const int gc = nd(T, l, xi, yi, zi);
-    for (int c = 0; c < NC; ++c) centerV[c] = gc >= 0 ? cell[(size_t)NC * gc + c] : 0.0;
+    for (int c = 0; c < NC; ++c) centerV[c] = gc >= 0 ? ld(cell, (size_t)NC * gc + c) : 0.0;
     for (int c = 0; c < NC; ++c) v[reDir][c] = centerV[c];
     const int r = c_res[l];
     for (int i = 0; i < 8; ++i) {
@@ -255,7 +369,7 @@ __device__ void neighborCube(const Topo& T, const double* cell, const double* ve
                     id[k2] += 1;
                     const int g = nd(T, l - 1, id[0], id[1], id[2]);
                     if (g >= 0 && (T.tb[g] & TB_HALF)) {
-                        for (int c = 0; c < NC; ++c) v[i][c] = vert[(size_t)NC * g + c];
+                        for (int c = 0; c < NC; ++c) v[i][c] = ld(vert, (size_t)NC * g + c);
                         flag = true;
                     }
                 }
@@ -265,7 +379,7 @@ __device__ void neighborCube(const Topo& T, const double* cell, const double* ve
                     const int id2 = (zi << 1) + (dir < 4);
                     const int g = nd(T, l - 1, id0 + c_nbt[i][0], id1 + c_nbt[i][1], id2 + c_nbt[i][2]);
                     if (g >= 0 && !ghost(T, g) && (T.tb[g] & TB_CLOSED_BY_TJ)) {
-                        for (int c = 0; c < NC; ++c) v[i][c] = vert[(size_t)NC * g + c];
+                        for (int c = 0; c < NC; ++c) v[i][c] = ld(vert, (size_t)NC * g + c);
                         flag = true;
                     }
                 }
@@ -282,33 +396,33 @@ __device__ void neighborCube(const Topo& T, const double* cell, const double* ve
                     // OctreeCellCenteredGrid3::clamp: indices into [0, res-2]
                     const int cx = min(max(0, xi + x), r - 2), cy = min(max(0, yi + y), r - 2),
                               cz = min(max(0, zi + z), r - 2);
-                    for (int c = 0; c < NC; ++c) tv[c] += cellv(T, cell, NC, l, cx, cy, cz, c);
+                    for (int c = 0; c < NC; ++c) tv[c] += cellvT(T, ld, cell, NC, l, cx, cy, cz, c);
                     count += 1;
                 }
         for (int c = 0; c < NC; ++c) v[i][c] = tv[c] / count;
     }
 }
 
-__device__ inline int region3D(double x, double y, double z) {
-    int dir = z >= 0.5 ? 0 : 4;
-    dir += (x >= 0.5) ? (y >= 0.5 ? 0 : 3) : (y >= 0.5 ? 1 : 2);
+template <class P>
+__device__ inline int region3D(const P& x, const P& y, const P& z) {
+    int dir = val(z) >= 0.5 ? 0 : 4;
+    dir += (val(x) >= 0.5) ? (val(y) >= 0.5 ? 0 : 3) : (val(y) >= 0.5 ? 1 : 2);
     return dir;
 }
 
 // OctreeOmniGrid3<T>::sampleSub (octree_omni_grid3.h:242-286)
-template <int NC>
-__device__ void sampleSub(const Topo& T, const double* cell, const double* vert, double px, double py, double pz,
-                          double* out) {
-    const int l = locate(T, px, py, pz);
+template <int NC, class P, class LD>
+__device__ void sampleSubT(const Topo& T, LD& ld, const double* cell, const double* vert, P px, P py, P pz, P* out) {
+    const int l = locate(T, val(px), val(py), val(pz));
     if (l < 0) {
-        for (int c = 0; c < NC; ++c) out[c] = NAN;
+        for (int c = 0; c < NC; ++c) out[c] = P(NAN);
         return;
     }
     const double dh = c_hl[l];
     const int r = c_res[l];
-    const double rx = px / dh, ry = py / dh, rz = pz / dh;
+    const P rx = px / dh, ry = py / dh, rz = pz / dh;
     int xi, yi, zi;
-    double fx, fy, fz;
+    P fx, fy, fz;
     bary(rx, r - 1, xi, fx);
     bary(ry, r - 1, yi, fy);
     bary(rz, r - 1, zi, fz);
@@ -317,13 +431,13 @@ __device__ void sampleSub(const Topo& T, const double* cell, const double* vert,
     const uint8_t b = tbits(T, gt);
     if (!(b & TB_HALF) && !(b & TB_CLOSED_BY_TJ)) {
         if (b & TB_CLOSED) {
-            cellRel<NC>(T, cell, l, rx, ry, rz, out);
+            cellRelT<NC>(T, ld, cell, l, rx, ry, rz, out);
             return;
         }
     }
     double v[8][NC];
-    for (int c = 0; c < NC; ++c) v[reg][c] = gt >= 0 ? vert[(size_t)NC * gt + c] : 0.0;
-    neighborCube<NC>(T, cell, vert, l, xi, yi, zi, reg, v, (b & TB_TJOINT) != 0);
+    for (int c = 0; c < NC; ++c) v[reg][c] = gt >= 0 ? ld(vert, (size_t)NC * gt + c) : 0.0;
+    neighborCube<NC>(T, ld, cell, vert, l, xi, yi, zi, reg, v, (b & TB_TJOINT) != 0);
     switch (reg) {
         case 0: fx -= 0.5, fy -= 0.5, fz -= 0.5; break;
         case 1: fy -= 0.5, fz -= 0.5; break;
@@ -340,6 +454,12 @@ __device__ void sampleSub(const Topo& T, const double* cell, const double* vert,
     for (int c = 0; c < NC; ++c)
         out[c] = trilerp(v[6][c], v[7][c], v[5][c], v[4][c], v[2][c], v[3][c], v[1][c], v[0][c], fx, fy, fz);
 }
+template <int NC>
+__device__ inline void sampleSub(const Topo& T, const double* cell, const double* vert, double px, double py, double pz,
+                                 double* out) {
+    PlainLd ld;
+    sampleSubT<NC>(T, ld, cell, vert, px, py, pz, out);
+}
 
 // level-transition test shared by the two correction-based samplers
 __device__ inline bool nearTransition(const Topo& T, int l, int xi, int yi, int zi) {
@@ -351,75 +471,86 @@ __device__ inline bool nearTransition(const Topo& T, int l, int xi, int yi, int 
 
 // OctreeOmniFaceCenteredGrid3::sample, correction-based branch
 // (octree_omni_face_centered_grid3.h:37-80)
-__device__ void velSample(const Topo& T, const Vel& V, double px, double py, double pz, double* out) {
-    const int l = locate(T, px, py, pz);
+template <class P, class LD>
+__device__ void velSampleT(const Topo& T, const Vel& V, LD& ld, P px, P py, P pz, P* out) {
+    const int l = locate(T, val(px), val(py), val(pz));
     if (l < 0) {
-        out[0] = out[1] = out[2] = NAN;
+        out[0] = out[1] = out[2] = P(NAN);
         return;
     }
     const double dh = c_hl[l];
     const int r = c_res[l];
     int xi, yi, zi;
-    double fx, fy, fz;
+    P fx, fy, fz;
     baryVec(px / dh - 0.5, r - 1, xi, fx);
     baryVec(py / dh - 0.5, r - 1, yi, fy);
     baryVec(pz / dh - 0.5, r - 1, zi, fz);
     const int gt = nd(T, l, xi + 1, yi + 1, zi + 1);
     const uint8_t b = tbits(T, gt);
     if (nearTransition(T, l, xi, yi, zi)) {
-        sampleSub<3>(T, V.doct, V.dtilt, px, py, pz, out);
+        sampleSubT<3>(T, ld, V.doct, V.dtilt, px, py, pz, out);
         return;
     }
-    double v[3];
-    macSample(T, V.mac, px, py, pz, v);
+    P v[3];
+    macSampleR(T, ld, V.mac, px, py, pz, v);
     if (!(b & TB_HALF) && (b & TB_CLOSED)) {
         out[0] = v[0], out[1] = v[1], out[2] = v[2];
         return;
     }
-    double mt[3];
-    macSample(T, V.mac, (double)(xi + 1) * dh, (double)(yi + 1) * dh, (double)(zi + 1) * dh, mt);
-    const double w = fmin(fmin(fmin(fx, fy), fz), fmin(fmin(1 - fx, 1 - fy), 1 - fz));
+    P mt[3];
+    macSampleR(T, ld, V.mac, P((double)(xi + 1) * dh), P((double)(yi + 1) * dh), P((double)(zi + 1) * dh), mt);
+    const P w = pmin(pmin(pmin(fx, fy), fz), pmin(pmin(1 - fx, 1 - fy), 1 - fz));
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        const double diff = (gt >= 0 ? V.dtilt[3 * (size_t)gt + c] : 0.0) - mt[c];
+        const P diff = (gt >= 0 ? ld(V.dtilt, 3 * (size_t)gt + c) : 0.0) - mt[c];
         out[c] = v[c] + diff * 2 * w;
     }
 }
+__device__ inline void velSample(const Topo& T, const Vel& V, double px, double py, double pz, double* out) {
+    PlainLd ld;
+    velSampleT(T, V, ld, px, py, pz, out);
+}
 
 // OctreeOmniGrid3<REAL>::sample, correction-based branch (octree_omni_grid3.h:146-191)
-__device__ double scalarSample(const Topo& T, const double* cell, const double* vert, double px, double py,
-                               double pz) {
-    const int l = locate(T, px, py, pz);
-    if (l < 0) return NAN;
+template <class P, class LD>
+__device__ P scalarSampleT(const Topo& T, LD& ld, const double* cell, const double* vert, P px, P py, P pz) {
+    const int l = locate(T, val(px), val(py), val(pz));
+    if (l < 0) return P(NAN);
     const double dh = c_hl[l];
     const int r = c_res[l];
     int xi, yi, zi;
-    double fx, fy, fz;
+    P fx, fy, fz;
     bary(px / dh - 0.5, r - 2, xi, fx);
     bary(py / dh - 0.5, r - 2, yi, fy);
     bary(pz / dh - 0.5, r - 2, zi, fz);
     const int x1 = min(xi + 1, r - 2), y1 = min(yi + 1, r - 2), z1 = min(zi + 1, r - 2);
-    const double o000 = cellv(T, cell, 1, l, xi, yi, zi, 0), o100 = cellv(T, cell, 1, l, x1, yi, zi, 0),
-                 o010 = cellv(T, cell, 1, l, xi, y1, zi, 0), o110 = cellv(T, cell, 1, l, x1, y1, zi, 0),
-                 o001 = cellv(T, cell, 1, l, xi, yi, z1, 0), o101 = cellv(T, cell, 1, l, x1, yi, z1, 0),
-                 o011 = cellv(T, cell, 1, l, xi, y1, z1, 0), o111 = cellv(T, cell, 1, l, x1, y1, z1, 0);
-    const double ret = trilerp(o000, o100, o010, o110, o001, o101, o011, o111, fx, fy, fz);
+    const double o000 = cellvT(T, ld, cell, 1, l, xi, yi, zi, 0), o100 = cellvT(T, ld, cell, 1, l, x1, yi, zi, 0),
+                 o010 = cellvT(T, ld, cell, 1, l, xi, y1, zi, 0), o110 = cellvT(T, ld, cell, 1, l, x1, y1, zi, 0),
+                 o001 = cellvT(T, ld, cell, 1, l, xi, yi, z1, 0), o101 = cellvT(T, ld, cell, 1, l, x1, yi, z1, 0),
+                 o011 = cellvT(T, ld, cell, 1, l, xi, y1, z1, 0), o111 = cellvT(T, ld, cell, 1, l, x1, y1, z1, 0);
+    const P ret = trilerp(o000, o100, o010, o110, o001, o101, o011, o111, fx, fy, fz);
     const int gt = nd(T, l, xi + 1, yi + 1, zi + 1);
     const uint8_t b = tbits(T, gt);
     if (nearTransition(T, l, xi, yi, zi)) {
-        double s;
-        sampleSub<1>(T, cell, vert, px, py, pz, &s);
+        P s;
+        sampleSubT<1>(T, ld, cell, vert, px, py, pz, &s);
         return s;
     }
     if ((b & TB_CLOSED) && !(b & TB_HALF)) return ret;
-    const double diff = (gt >= 0 ? vert[gt] : 0.0) -
+    const double diff = (gt >= 0 ? ld(vert, (size_t)gt) : 0.0) -
                         (1 / 8.0) * (o000 + o100 + o010 + o110 + o001 + o101 + o011 + o111);
-    const double w = fmin(fmin(fmin(fx, fy), fz), fmin(fmin(1 - fx, 1 - fy), 1 - fz));
+    const P w = pmin(pmin(pmin(fx, fy), fz), pmin(pmin(1 - fx, 1 - fy), 1 - fz));
     return ret + diff * 2.0 * w;
 }
+__device__ inline double scalarSample(const Topo& T, const double* cell, const double* vert, double px, double py,
+                                      double pz) {
+    PlainLd ld;
+    return scalarSampleT(T, ld, cell, vert, px, py, pz);
+}
 
-// BackTrace, midpoint branch (advection.h:45-62); no collider, so the SDF
-// sign test never fires
+// BackTrace, midpoint branch (advection.h:45-62): where the boundary SDF
+// changes sign between p and the traced point, the point is pulled back to
+// the blend w p + (1 - w) ret, w = |phi1| / (|phi0| + |phi1|).
 __device__ void backTrace(const Topo& T, const Vel& V, const double* p, double dt, double* out) {
     double v[3], mid[3];
     velSample(T, V, p[0], p[1], p[2], v);
@@ -428,6 +559,15 @@ __device__ void backTrace(const Topo& T, const Vel& V, const double* p, double d
     velSample(T, V, mid[0], mid[1], mid[2], v);
 #pragma unroll
     for (int c = 0; c < 3; ++c) out[c] = p[c] - v[c] * dt;
+    if (T.sdf) {
+        const double phi0 = sdfSample(T, p[0], p[1], p[2]);
+        const double phi1 = sdfSample(T, out[0], out[1], out[2]);
+        if (phi0 * phi1 < 0.0) {
+            const double w = fabs(phi1) / (fabs(phi0) + fabs(phi1));
+#pragma unroll
+            for (int c = 0; c < 3; ++c) out[c] = w * p[c] + (1.0 - w) * out[c];
+        }
+    }
 }
 
 __device__ inline void nodeXYZ(const Topo& T, int g, int& x, int& y, int& z) {
@@ -440,97 +580,310 @@ __device__ inline void nodeXYZ(const Topo& T, int g, int& x, int& y, int& z) {
 // kernels
 // ---------------------------------------------------------------------------
 // constrainVelocity(MAC) without collider: normal faces on closed walls -> 0
-__global__ void k_wall(Topo T, const uint8_t* level, double* mac, int closed) {
-    const int g = blockIdx.x * BS + threadIdx.x;
-    if (g >= c_G) return;
-    const int l = level[g], r = c_res[l];
-    int x, y, z;
-    nodeXYZ(T, g, x, y, z);
-    if (((closed & ASTG_WALL_XPOS) && x == r - 1) || ((closed & ASTG_WALL_XNEG) && x == 0)) mac[g] = 0.0;
-    if (((closed & ASTG_WALL_YPOS) && y == r - 1) || ((closed & ASTG_WALL_YNEG) && y == 0)) mac[c_G + g] = 0.0;
-    if (((closed & ASTG_WALL_ZPOS) && z == r - 1) || ((closed & ASTG_WALL_ZNEG) && z == 0)) mac[2 * c_G + g] = 0.0;
-}
-
-__global__ void k_mac_ghost_clear(const int* ghostList, int n, double* mac) {
-    const int t = blockIdx.x * BS + threadIdx.x;
-    if (t >= n) return;
-    const int g = ghostList[t];
-    mac[g] = 0.0;
-    mac[c_G + g] = 0.0;
-    mac[2 * c_G + g] = 0.0;
-}
-
-// ghost MAC faces resampled at their positions; fromCoarse selects the pass
-__global__ void k_mac_ghost_sample(Topo T, const uint8_t* level, const int* ghostList, int n, double* mac,
-                                   int fromCoarse) {
-    const int t = blockIdx.x * BS + threadIdx.x;
-    if (t >= n) return;
-    const int g = ghostList[t];
-    if (((T.nf[g] & NF_FROM_COARSE) != 0) != (fromCoarse != 0)) return;
-    const int l = level[g];
-    const double h = c_hl[l];
-    int x, y, z;
-    nodeXYZ(T, g, x, y, z);
-    const double bx = x * h, by = y * h, bz = z * h;
-    double v[3];
-    for (int a = 0; a < 3; ++a) {
-        const double px = bx + (a == 0 ? 0.0 : 0.5) * h, py = by + (a == 1 ? 0.0 : 0.5) * h,
-                     pz = bz + (a == 2 ? 0.0 : 0.5) * h;
-        macSample(T, mac, px, py, pz, v);
-        mac[(size_t)a * c_G + g] = v[a];
+// The linear passes below are bodies run by k_run (forward) or recorded
+// for their transposes (linrec.h): field reads and writes go through the
+// policy P.
+struct BWall {
+    Topo T;
+    const uint8_t* level;
+    double* mac;
+    int closed;
+    template <class Pol>
+    __device__ void operator()(int g, Pol& P) const {
+        const int l = level[g], r = c_res[l];
+        int x, y, z;
+        nodeXYZ(T, g, x, y, z);
+        if (((closed & ASTG_WALL_XPOS) && x == r - 1) || ((closed & ASTG_WALL_XNEG) && x == 0)) P.st(mac, g, 0.0);
+        if (((closed & ASTG_WALL_YPOS) && y == r - 1) || ((closed & ASTG_WALL_YNEG) && y == 0))
+            P.st(mac, (size_t)c_G + g, 0.0);
+        if (((closed & ASTG_WALL_ZPOS) && z == r - 1) || ((closed & ASTG_WALL_ZNEG) && z == 0))
+            P.st(mac, 2 * (size_t)c_G + g, 0.0);
     }
-}
+};
 
-// updateGhostFromHalfTilt: coarse face <- half-tilt face slot (one writer per face)
-__global__ void k_mac_from_half(const int* halfG, const int* target, const int* axis, const int* slot, int n,
-                                const double* tilt, double* mac) {
-    const int t = blockIdx.x * BS + threadIdx.x;
-    if (t >= n) return;
-    mac[(size_t)axis[t] * c_G + target[t]] = tilt[(size_t)slot[t] * c_G + halfG[t]];
-}
-
-// OctreeOmniTiltGrid3::updateGhost, half-tilts kept
-__global__ void k_tilt_ghost(Topo T, const uint8_t* level, const int* ghostList, int n, const uint8_t* inHalf,
-                             double* tilt) {
-    const int t = blockIdx.x * BS + threadIdx.x;
-    if (t >= n) return;
-    const int g = ghostList[t];
-    if ((T.tb[g] & TB_HALF) || inHalf[g]) return;
-    const int l = level[g];
-    int x, y, z;
-    nodeXYZ(T, g, x, y, z);
-    int src;
-    if (T.nf[g] & NF_FROM_COARSE) {
-        if ((x & 1) || (y & 1) || (z & 1)) {
-            for (int k = 0; k < 8; ++k) tilt[(size_t)k * c_G + g] = 0.0;
-            return;
+// ---- collider (omni_boundary_solver3.h:26-100, wipeBoundary 1240-1292)
+// Slip constraint of the MAC faces inside the collider (sdf <= 0), in the
+// reference's visiting order: face k (faces in order: levels ascending, leaf
+// order, axis x, y, z) samples the MAC velocity at its position reading the
+// new values of the inside faces before it and the old values of every other
+// face, and keeps u - (u . n) n along its axis, n the unit vector from the
+// slip centre (0 if the face sits on the centre). A face publishes its value
+// with an agent-scope store over a signalling-NaN sentinel; the faces after
+// it poll for it. Persistent (cooperative launch): block b takes chunks b,
+// b + grid, ... so the lowest unfinished face always belongs to a running lane.
+struct SlipReader {
+    const double* mac;
+    const int* ord;
+    const double* pub;
+    int k;
+    bool ok;
+    // i = axis * G + node, as the MAC sampler addresses the array
+    __device__ double operator()(const double* base, size_t i) {
+        const int a = (int)(i / (size_t)c_G), g = (int)(i - (size_t)a * c_G);
+        const int o = ord[(size_t)g * 3 + a];
+        if (o >= 0 && o < k) {
+            const unsigned long long b = Sentinel<double>::load(&pub[o]);
+            if (b == Sentinel<double>::value) {
+                ok = false;
+                return 0.0;
+            }
+            return Sentinel<double>::val(b);
         }
-        src = nd(T, l + 1, x >> 1, y >> 1, z >> 1);
-    } else {
-        src = nd(T, l - 1, x << 1, y << 1, z << 1);
+        return mac[(size_t)a * c_G + g];
     }
-    for (int k = 0; k < 8; ++k) tilt[(size_t)k * c_G + g] = src >= 0 ? tilt[(size_t)k * c_G + src] : 0.0;
+};
+
+__device__ inline void facePos(const Topo& T, const uint8_t* level, int g, int a, double* p) {
+    const double h = c_hl[level[g]];
+    int x, y, z;
+    nodeXYZ(T, g, x, y, z);
+    p[0] = x * h + (a == 0 ? 0.0 : 0.5) * h;
+    p[1] = y * h + (a == 1 ? 0.0 : 0.5) * h;
+    p[2] = z * h + (a == 2 ? 0.0 : 0.5) * h;
 }
 
-// updateDualGrid, tilt part
-__global__ void k_dual_tilt(Topo T, const int* leafList, int n, const double* tilt, double* dtilt) {
+__global__ void __launch_bounds__(BS) k_slip_mac(Topo T, const uint8_t* level, const double* mac, const int* ord,
+                                                 const int* faceG, const uint8_t* faceA, int nf, double* pub,
+                                                 int* err) {
+    const int nchunks = (nf + BS - 1) / BS;
+    for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        const int k = c * BS + threadIdx.x;
+        bool pending = k < nf;
+        double p[3] = {0, 0, 0};
+        int a = 0;
+        if (pending) {
+            a = faceA[k];
+            facePos(T, level, faceG[k], a, p);
+        }
+        int backoff = 1;
+        for (int spins = 0;; ++spins) {
+            if (pending) {
+                SlipReader rd{mac, ord, pub, k, true};
+                double v[3];
+                macSampleR(T, rd, mac, p[0], p[1], p[2], v);
+                if (rd.ok) {
+                    double n[3] = {p[0] - c_slip[0], p[1] - c_slip[1], p[2] - c_slip[2]};
+                    const double len = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+                    double val = 0.0;
+                    if (!(len < kEps)) {
+                        n[0] /= len;
+                        n[1] /= len;
+                        n[2] /= len;
+                        const double d = v[0] * n[0] + v[1] * n[1] + v[2] * n[2];
+                        val = v[a] - d * n[a];
+                    }
+                    Sentinel<double>::store(&pub[k], val);
+                    pending = false;
+                } else if (spins > kSpinLimit) {
+                    Sentinel<double>::store(&pub[k], (double)NAN);
+                    atomicOr(err, 1);
+                    pending = false;
+                }
+            }
+            if (!__any(pending)) break;
+            for (int q = 0; q < backoff; ++q) __builtin_amdgcn_s_sleep(1);
+            backoff = backoff < 4 ? 2 * backoff : 4;
+        }
+    }
+}
+
+__global__ void k_fill_sent64(double* a, int n) {
+    const int i = blockIdx.x * BS + threadIdx.x;
+    if (i < n) Sentinel<double>::storeSent(&a[i]);
+}
+
+__global__ void k_slip_scatter(const int* faceG, const uint8_t* faceA, int nf, const double* pub, double* mac) {
+    const int k = blockIdx.x * BS + threadIdx.x;
+    if (k < nf) mac[(size_t)faceA[k] * c_G + faceG[k]] = pub[k];
+}
+
+// constrainVelocity(tilt, MAC) (omni_boundary_solver3.h:26-50) as the
+// reference's code behaves: the projected value is assigned to a shadowing
+// local 3-vector, so the only change to the tilt velocity is a slot set to 0
+// when its position coincides with the slip centre (open tilts whose centre
+// lies within h of the collider, slots inside it).
+__global__ void k_slip_tilt(Topo T, const uint8_t* level, const int* leafList, int n, double* tilt) {
     const int t = blockIdx.x * BS + threadIdx.x;
     if (t >= n) return;
     const int g = leafList[t];
-    double* o = dtilt + 3 * (size_t)g;
-    if (T.tb[g] & TB_CLOSED) {
-        o[0] = o[1] = o[2] = 0.0;
-        return;
-    }
-    double gv[8];
-    for (int k = 0; k < 8; ++k) gv[k] = tilt[(size_t)k * c_G + g];
-    const double v[4] = {(gv[0] + gv[6]) * 0.5, (gv[1] + gv[7]) * 0.5, (gv[2] + gv[4]) * 0.5, (gv[3] + gv[5]) * 0.5};
-    for (int r = 0; r < 3; ++r) {
-        double s = 0;
-        for (int j = 0; j < 4; ++j) s += c_dualTilt[r * 4 + j] * v[j];
-        o[r] = s;
+    if (T.tb[g] & TB_CLOSED) return;
+    const double h = c_hl[level[g]];
+    int x, y, z;
+    nodeXYZ(T, g, x, y, z);
+    const double c[3] = {x * h, y * h, z * h};
+    if (!(sdfSample(T, c[0], c[1], c[2]) - h <= 0)) return;
+    const double e = h * 0.5;
+    for (int k = 0; k < 8; ++k) {
+        const int k4 = k & 0x3;
+        const double p[3] = {c[0] + ((k4 == 0 || k4 == 3) ? 1 : -1) * e / 3.0, c[1] + (k4 < 2 ? 1 : -1) * e / 3.0,
+                             c[2] + (k < 4 ? 1 : -1) * e / 3.0};
+        if (!(sdfSample(T, p[0], p[1], p[2]) <= 0)) continue;
+        const double d[3] = {p[0] - c_slip[0], p[1] - c_slip[1], p[2] - c_slip[2]};
+        if (sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) < kEps) tilt[(size_t)k * c_G + g] = 0.0;
     }
 }
+
+// constrainVelocity(dual grid) (omni_boundary_solver3.h:52-68): the dual
+// vertex vectors inside the collider keep their tangential part.
+__global__ void k_slip_dual(Topo T, const uint8_t* level, const int* leafList, int n, double* dtilt) {
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= n) return;
+    const int g = leafList[t];
+    const double h = c_hl[level[g]];
+    int x, y, z;
+    nodeXYZ(T, g, x, y, z);
+    const double p[3] = {x * h, y * h, z * h};
+    if (!(sdfSample(T, p[0], p[1], p[2]) <= 0)) return;
+    double nrm[3] = {p[0] - c_slip[0], p[1] - c_slip[1], p[2] - c_slip[2]};
+    const double len = sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+    double* v = dtilt + 3 * (size_t)g;
+    if (len < kEps) {
+        v[0] = v[1] = v[2] = 0.0;
+        return;
+    }
+    nrm[0] /= len;
+    nrm[1] /= len;
+    nrm[2] /= len;
+    const double d = v[0] * nrm[0] + v[1] * nrm[1] + v[2] * nrm[2];
+    const double o[3] = {v[0] - d * nrm[0], v[1] - d * nrm[1], v[2] - d * nrm[2]};
+    v[0] = o[0];
+    v[1] = o[1];
+    v[2] = o[2];
+}
+
+// wipeBoundary: density and heat at leaf cell centres and leaf vertices (and
+// half-tilt vertices, list kind 1) where the SDF is negative become 0.
+__global__ void k_wipe(Topo T, const uint8_t* level, const int* list, int n, int vertexOnly, double* dC, double* dV,
+                       double* hC, double* hV) {
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= n) return;
+    const int g = list[t];
+    const double h = c_hl[level[g]];
+    int x, y, z;
+    nodeXYZ(T, g, x, y, z);
+    if (!vertexOnly && sdfSample(T, (x + 0.5) * h, (y + 0.5) * h, (z + 0.5) * h) < 0) {
+        dC[g] = 0.0;
+        hC[g] = 0.0;
+    }
+    if (sdfSample(T, x * h, y * h, z * h) < 0) {
+        dV[g] = 0.0;
+        hV[g] = 0.0;
+    }
+}
+
+struct BMacGhostClear {
+    const int* ghostList;
+    double* mac;
+    template <class Pol>
+    __device__ void operator()(int t, Pol& P) const {
+        const int g = ghostList[t];
+        P.st(mac, g, 0.0);
+        P.st(mac, (size_t)c_G + g, 0.0);
+        P.st(mac, 2 * (size_t)c_G + g, 0.0);
+    }
+};
+
+// a policy's loads as the samplers' load functor
+template <class Pol>
+struct PolLd {
+    Pol& P;
+    __device__ double operator()(const double* base, size_t i) const { return P.ld(base, i); }
+};
+
+// ghost MAC faces resampled at their positions; fromCoarse selects the pass
+struct BMacGhostSample {
+    Topo T;
+    const uint8_t* level;
+    const int* ghostList;
+    double* mac;
+    int fromCoarse;
+    template <class Pol>
+    __device__ void operator()(int t, Pol& P) const {
+        const int g = ghostList[t];
+        if (((T.nf[g] & NF_FROM_COARSE) != 0) != (fromCoarse != 0)) return;
+        const int l = level[g];
+        const double h = c_hl[l];
+        int x, y, z;
+        nodeXYZ(T, g, x, y, z);
+        const double bx = x * h, by = y * h, bz = z * h;
+        double v[3];
+        PolLd<Pol> ld{P};
+        for (int a = 0; a < 3; ++a) {
+            const double px = bx + (a == 0 ? 0.0 : 0.5) * h, py = by + (a == 1 ? 0.0 : 0.5) * h,
+                         pz = bz + (a == 2 ? 0.0 : 0.5) * h;
+            macSampleR(T, ld, (const double*)mac, px, py, pz, v);
+            P.st(mac, (size_t)a * c_G + g, v[a]);
+        }
+    }
+};
+
+// updateGhostFromHalfTilt: coarse face <- half-tilt face slot (one writer per face)
+struct BMacFromHalf {
+    const int *halfG, *target, *axis, *slot;
+    const double* tilt;
+    double* mac;
+    template <class Pol>
+    __device__ void operator()(int t, Pol& P) const {
+        P.st(mac, (size_t)axis[t] * c_G + target[t], P.ld(tilt, (size_t)slot[t] * c_G + halfG[t]));
+    }
+};
+
+// OctreeOmniTiltGrid3::updateGhost, half-tilts kept
+struct BTiltGhost {
+    Topo T;
+    const uint8_t* level;
+    const int* ghostList;
+    const uint8_t* inHalf;
+    double* tilt;
+    template <class Pol>
+    __device__ void operator()(int t, Pol& P) const {
+        const int g = ghostList[t];
+        if ((T.tb[g] & TB_HALF) || inHalf[g]) return;
+        const int l = level[g];
+        int x, y, z;
+        nodeXYZ(T, g, x, y, z);
+        int src;
+        if (T.nf[g] & NF_FROM_COARSE) {
+            if ((x & 1) || (y & 1) || (z & 1)) {
+                for (int k = 0; k < 8; ++k) P.st(tilt, (size_t)k * c_G + g, 0.0);
+                return;
+            }
+            src = nd(T, l + 1, x >> 1, y >> 1, z >> 1);
+        } else {
+            src = nd(T, l - 1, x << 1, y << 1, z << 1);
+        }
+        for (int k = 0; k < 8; ++k)
+            P.st(tilt, (size_t)k * c_G + g, src >= 0 ? P.ld(tilt, (size_t)k * c_G + src) : 0.0);
+    }
+};
+
+// updateDualGrid, tilt part
+struct BDualTilt {
+    Topo T;
+    const int* leafList;
+    const double* tilt;
+    double* dtilt;
+    template <class Pol>
+    __device__ void operator()(int t, Pol& P) const {
+        const int g = leafList[t];
+        const size_t o = 3 * (size_t)g;
+        if (T.tb[g] & TB_CLOSED) {
+            P.st(dtilt, o, 0.0);
+            P.st(dtilt, o + 1, 0.0);
+            P.st(dtilt, o + 2, 0.0);
+            return;
+        }
+        double gv[8];
+        for (int k = 0; k < 8; ++k) gv[k] = P.ld(tilt, (size_t)k * c_G + g);
+        const double v[4] = {(gv[0] + gv[6]) * 0.5, (gv[1] + gv[7]) * 0.5, (gv[2] + gv[4]) * 0.5,
+                             (gv[3] + gv[5]) * 0.5};
+        for (int r = 0; r < 3; ++r) {
+            double s = 0;
+            for (int j = 0; j < 4; ++j) s += c_dualTilt[r * 4 + j] * v[j];
+            P.st(dtilt, o + r, s);
+        }
+    }
+};
 
 __device__ inline void dualApply(int idx, const double* v, double* o) {
     const double* M = c_dual + idx * 21;
@@ -545,17 +898,24 @@ __device__ inline int dualIndex(int x, int y, int z, int n0, int n1, int n2, int
 }
 
 // updateDualGrid, octagon part (octree_omni_face_centered_grid3.cpp:50-128)
-__global__ void k_dual_oct(Topo T, const uint8_t* level, const int* leafList, int n, const double* mac,
-                           const double* tilt, double* doct) {
-    const int t = blockIdx.x * BS + threadIdx.x;
-    if (t >= n) return;
+struct BDualOct {
+    Topo T;
+    const uint8_t* level;
+    const int* leafList;
+    const double* mac;
+    const double* tilt;
+    double* doct;
+    template <class Pol>
+    __device__ void operator()(int t, Pol& P) const {
     const int g = leafList[t];
     const int l = level[g], r = c_res[l];
     int x, y, z;
     nodeXYZ(T, g, x, y, z);
-    double* o = doct + 3 * (size_t)g;
+    const size_t o = 3 * (size_t)g;
     if (!(x < r - 1 && y < r - 1 && z < r - 1)) {
-        o[0] = o[1] = o[2] = 0.0;
+        P.st(doct, o, 0.0);
+        P.st(doct, o + 1, 0.0);
+        P.st(doct, o + 2, 0.0);
         return;
     }
     const double dh = c_hl[l];
@@ -571,7 +931,7 @@ __global__ void k_dual_oct(Topo T, const uint8_t* level, const int* leafList, in
     int cnt[7] = {0, 0, 0, 0, 0, 0, 0};
     auto MX = [&](int a, int xx, int yy, int zz) {
         const int q = nd(T, l, xx, yy, zz);
-        return q >= 0 ? mac[(size_t)a * c_G + q] : 0.0;
+        return q >= 0 ? P.ld(mac, (size_t)a * c_G + q) : 0.0;
     };
     if (!((nj[1] && nj[6]) || (nj[2] && nj[5]))) v[0] += MX(0, x, y, z), cnt[0]++;
     if (!((nj[0] && nj[7]) || (nj[3] && nj[4]))) v[0] += MX(0, x + 1, y, z), cnt[0]++;
@@ -585,11 +945,11 @@ __global__ void k_dual_oct(Topo T, const uint8_t* level, const int* leafList, in
         const uint8_t bn = tbits(T, gn), br = tbits(T, gr);
         const double en = gn >= 0 ? T.te[gn] : 0.0, er = gr >= 0 ? T.te[gr] : 0.0;
         if (!(bn & TB_CLOSED) || (bn & TB_HALF)) {
-            v[vs] += gn >= 0 ? tilt[(size_t)reSlot * c_G + gn] : 0.0;
+            v[vs] += gn >= 0 ? P.ld(tilt, (size_t)reSlot * c_G + gn) : 0.0;
             cnt[vs]++;
         }
         if (!(br & TB_CLOSED) || (br & TB_HALF)) {
-            const double tv = gr >= 0 ? tilt[(size_t)slot * c_G + gr] : 0.0;
+            const double tv = gr >= 0 ? P.ld(tilt, (size_t)slot * c_G + gr) : 0.0;
             // INV_SQRT3 is the unparenthesised macro 1.0 / SQRT3: e * INV_SQRT3 == e * 1.0 / SQRT3
             v[vs] = cnt[vs] == 0 ? tv
                                  : ((v[vs] * (hs3h - er * 1.0 / kSqrt3) + tv * (hs3h - en * 1.0 / kSqrt3)) /
@@ -600,120 +960,153 @@ __global__ void k_dual_oct(Topo T, const uint8_t* level, const int* leafList, in
     for (int i = 0; i < 3; ++i) v[i] = v[i] / (cnt[i] + kEps);
     const int idx = dualIndex(cnt[0] > 0, cnt[1] > 0, cnt[2] > 0, cnt[3] > 1 || nj[0] || nj[6],
                               cnt[4] > 1 || nj[1] || nj[7], cnt[5] > 1 || nj[2] || nj[4], cnt[6] > 1 || nj[3] || nj[5]);
-    dualApply(idx, v, o);
-}
+    double oo[3];
+    dualApply(idx, v, oo);
+    for (int k = 0; k < 3; ++k) P.st(doct, o + k, oo[k]);
+    }
+};
 
 // updateDualGrid, half-tilt part
-__global__ void k_dual_half(Topo T, const int* halfList, int n, const double* tilt, double* dtilt) {
-    const int t = blockIdx.x * BS + threadIdx.x;
-    if (t >= n) return;
-    const int g = halfList[t];
-    const int dir = T.thd[g];
-    const int axis = dir % 3;
-    double v[7] = {0, 0, 0, 0, 0, 0, 0};
-    v[axis] = tilt[(size_t)c_halfSlot[dir][4] * c_G + g];
-    for (int i = 0; i < 4; ++i) {
-        const int slot = c_halfSlot[dir][i];
-        const int reSlot = ((slot & 1) ? 8 : 6) - slot;
-        const int id = slot < reSlot ? slot : reSlot;
-        v[id + 3] = tilt[(size_t)slot * c_G + g];
+struct BDualHalf {
+    Topo T;
+    const int* halfList;
+    const double* tilt;
+    double* dtilt;
+    template <class Pol>
+    __device__ void operator()(int t, Pol& P) const {
+        const int g = halfList[t];
+        const int dir = T.thd[g];
+        const int axis = dir % 3;
+        double v[7] = {0, 0, 0, 0, 0, 0, 0};
+        v[axis] = P.ld(tilt, (size_t)c_halfSlot[dir][4] * c_G + g);
+        for (int i = 0; i < 4; ++i) {
+            const int slot = c_halfSlot[dir][i];
+            const int reSlot = ((slot & 1) ? 8 : 6) - slot;
+            const int id = slot < reSlot ? slot : reSlot;
+            v[id + 3] = P.ld(tilt, (size_t)slot * c_G + g);
+        }
+        double oo[3];
+        dualApply(dualIndex(axis == 0, axis == 1, axis == 2, 1, 1, 1, 1), v, oo);
+        for (int k = 0; k < 3; ++k) P.st(dtilt, 3 * (size_t)g + k, oo[k]);
     }
-    dualApply(dualIndex(axis == 0, axis == 1, axis == 2, 1, 1, 1, 1), v, dtilt + 3 * (size_t)g);
-}
+};
 
 // ---- omni grid ghost chain (octree_omni_grid3.h:136-143), NC values per node
 template <int NC>
-__global__ void k_cell_ghost_fine(Topo T, const uint8_t* level, const int* ghostList, int n, double* cell) {
-    const int t = blockIdx.x * BS + threadIdx.x;
-    if (t >= n) return;
-    const int g = ghostList[t];
-    if (T.nf[g] & NF_FROM_COARSE) return;
-    const int l = level[g];
-    int x, y, z;
-    nodeXYZ(T, g, x, y, z);
-    x <<= 1;
-    y <<= 1;
-    z <<= 1;
-    for (int c = 0; c < NC; ++c) {
-        double v = 0;
-        for (int i = 0; i <= 1; ++i)
-            for (int j = 0; j <= 1; ++j)
-                for (int k = 0; k <= 1; ++k) v += cellv(T, cell, NC, l - 1, x + i, y + j, z + k, c);
-        cell[(size_t)NC * g + c] = v * (1 / 8.0);
+struct BCellGhostFine {
+    Topo T;
+    const uint8_t* level;
+    const int* ghostList;
+    double* cell;
+    template <class Pol>
+    __device__ void operator()(int t, Pol& P) const {
+        const int g = ghostList[t];
+        if (T.nf[g] & NF_FROM_COARSE) return;
+        const int l = level[g];
+        int x, y, z;
+        nodeXYZ(T, g, x, y, z);
+        x <<= 1;
+        y <<= 1;
+        z <<= 1;
+        PolLd<Pol> ld{P};
+        for (int c = 0; c < NC; ++c) {
+            double v = 0;
+            for (int i = 0; i <= 1; ++i)
+                for (int j = 0; j <= 1; ++j)
+                    for (int k = 0; k <= 1; ++k) v += cellvT(T, ld, cell, NC, l - 1, x + i, y + j, z + k, c);
+            P.st(cell, (size_t)NC * g + c, v * (1 / 8.0));
+        }
     }
-}
+};
 
 template <int NC>
-__global__ void k_cell_ghost_coarse(Topo T, const uint8_t* level, const int* ghostList, int n, double* cell) {
-    const int t = blockIdx.x * BS + threadIdx.x;
-    if (t >= n) return;
-    const int g = ghostList[t];
-    if (!(T.nf[g] & NF_FROM_COARSE)) return;
-    const int l = level[g];
-    const double h = c_hl[l];
-    int x, y, z;
-    nodeXYZ(T, g, x, y, z);
-    double out[NC];
-    cellSample<NC>(T, cell, x * h + 0.5 * h, y * h + 0.5 * h, z * h + 0.5 * h, out);
-    for (int c = 0; c < NC; ++c) cell[(size_t)NC * g + c] = out[c];
-}
+struct BCellGhostCoarse {
+    Topo T;
+    const uint8_t* level;
+    const int* ghostList;
+    double* cell;
+    template <class Pol>
+    __device__ void operator()(int t, Pol& P) const {
+        const int g = ghostList[t];
+        if (!(T.nf[g] & NF_FROM_COARSE)) return;
+        const int l = level[g];
+        const double h = c_hl[l];
+        int x, y, z;
+        nodeXYZ(T, g, x, y, z);
+        double out[NC];
+        PolLd<Pol> ld{P};
+        cellSampleT<NC>(T, ld, (const double*)cell, x * h + 0.5 * h, y * h + 0.5 * h, z * h + 0.5 * h, out);
+        for (int c = 0; c < NC; ++c) P.st(cell, (size_t)NC * g + c, out[c]);
+    }
+};
 
 template <int NC>
-__global__ void k_vert_ghost(Topo T, const uint8_t* level, const int* ghostList, int n, const uint8_t* inHalf,
-                             double* vert) {
-    const int t = blockIdx.x * BS + threadIdx.x;
-    if (t >= n) return;
-    const int g = ghostList[t];
-    if (inHalf[g]) return;
-    const int l = level[g];
-    int x, y, z;
-    nodeXYZ(T, g, x, y, z);
-    if (T.nf[g] & NF_FROM_COARSE) {
-        double v[NC];
-        for (int c = 0; c < NC; ++c) v[c] = 0;
-        int count = 0;
-        for (int dx = 0; dx <= (x & 1); ++dx)
-            for (int dy = 0; dy <= (y & 1); ++dy)
-                for (int dz = 0; dz <= (z & 1); ++dz) {
-                    const int q = getUntil(T, l + 1, (x >> 1) + dx, (y >> 1) + dy, (z >> 1) + dz, true);
-                    if (q >= 0) {
-                        for (int c = 0; c < NC; ++c) v[c] += vert[(size_t)NC * q + c];
-                        count += 1;
+struct BVertGhost {
+    Topo T;
+    const uint8_t* level;
+    const int* ghostList;
+    const uint8_t* inHalf;
+    double* vert;
+    template <class Pol>
+    __device__ void operator()(int t, Pol& P) const {
+        const int g = ghostList[t];
+        if (inHalf[g]) return;
+        const int l = level[g];
+        int x, y, z;
+        nodeXYZ(T, g, x, y, z);
+        if (T.nf[g] & NF_FROM_COARSE) {
+            double v[NC];
+            for (int c = 0; c < NC; ++c) v[c] = 0;
+            int count = 0;
+            for (int dx = 0; dx <= (x & 1); ++dx)
+                for (int dy = 0; dy <= (y & 1); ++dy)
+                    for (int dz = 0; dz <= (z & 1); ++dz) {
+                        const int q = getUntil(T, l + 1, (x >> 1) + dx, (y >> 1) + dy, (z >> 1) + dz, true);
+                        if (q >= 0) {
+                            for (int c = 0; c < NC; ++c) v[c] += P.ld(vert, (size_t)NC * q + c);
+                            count += 1;
+                        }
                     }
-                }
-        for (int c = 0; c < NC; ++c) vert[(size_t)NC * g + c] = v[c] * (1.0 / (count + kEps));
-    } else {
-        const int q = getUntil(T, l - 1, x << 1, y << 1, z << 1, true);
-        for (int c = 0; c < NC; ++c) vert[(size_t)NC * g + c] = q >= 0 ? vert[(size_t)NC * q + c] : 0.0;
+            for (int c = 0; c < NC; ++c) P.st(vert, (size_t)NC * g + c, v[c] * (1.0 / (count + kEps)));
+        } else {
+            const int q = getUntil(T, l - 1, x << 1, y << 1, z << 1, true);
+            for (int c = 0; c < NC; ++c)
+                P.st(vert, (size_t)NC * g + c, q >= 0 ? P.ld(vert, (size_t)NC * q + c) : 0.0);
+        }
     }
-}
+};
 
 // updateCellGhost (octree_omni_grid3.h:287-302)
 template <int NC>
-__global__ void k_cell_ghost_tj(Topo T, const uint8_t* level, const int* ghostList, int n, const double* vert,
-                                double* cell) {
-    const int t = blockIdx.x * BS + threadIdx.x;
-    if (t >= n) return;
-    const int g = ghostList[t];
-    if (!(T.nf[g] & NF_FROM_COARSE)) return;
-    const int l = level[g], r = c_res[l];
-    int x, y, z;
-    nodeXYZ(T, g, x, y, z);
-    if (!(x < r - 1 && y < r - 1 && z < r - 1)) return;
-    const int gd = nd(T, l, x + (x & 1), y + (y & 1), z + (z & 1));
-    const int gp = nd(T, l + 1, x >> 1, y >> 1, z >> 1);
-    for (int c = 0; c < NC; ++c) {
-        const double a = gd >= 0 ? vert[(size_t)NC * gd + c] : 0.0;
-        const double b = gp >= 0 ? cell[(size_t)NC * gp + c] : 0.0;
-        cell[(size_t)NC * g + c] = 0.5 * (a + b);
+struct BCellGhostTJ {
+    Topo T;
+    const uint8_t* level;
+    const int* ghostList;
+    const double* vert;
+    double* cell;
+    template <class Pol>
+    __device__ void operator()(int t, Pol& P) const {
+        const int g = ghostList[t];
+        if (!(T.nf[g] & NF_FROM_COARSE)) return;
+        const int l = level[g], r = c_res[l];
+        int x, y, z;
+        nodeXYZ(T, g, x, y, z);
+        if (!(x < r - 1 && y < r - 1 && z < r - 1)) return;
+        const int gd = nd(T, l, x + (x & 1), y + (y & 1), z + (z & 1));
+        const int gp = nd(T, l + 1, x >> 1, y >> 1, z >> 1);
+        for (int c = 0; c < NC; ++c) {
+            const double a = gd >= 0 ? P.ld(vert, (size_t)NC * gd + c) : 0.0;
+            const double b = gp >= 0 ? P.ld(cell, (size_t)NC * gp + c) : 0.0;
+            P.st(cell, (size_t)NC * g + c, 0.5 * (a + b));
+        }
     }
-}
+};
 
 // getTiltFromOctagonNeighbor3 (octree_util.h:94-112)
-template <int NC>
-__device__ void tiltFromOctagons(const Topo& T, const double* cell, int l, int x, int y, int z, double* out) {
+template <int NC, class LD>
+__device__ void tiltFromOctagons(const Topo& T, LD& ld, const double* cell, int l, int x, int y, int z, double* out) {
     const int r = c_res[l];
-    for (int c = 0; c < NC; ++c) out[c] = cellv(T, cell, NC, l, x, y, z, c);
+    for (int c = 0; c < NC; ++c) out[c] = cellvT(T, ld, cell, NC, l, x, y, z, c);
     double count = 1;
     for (int m = -1; m <= 0; ++m)
         for (int nn = -1; nn <= 0; ++nn)
@@ -722,7 +1115,7 @@ __device__ void tiltFromOctagons(const Topo& T, const double* cell, int l, int x
                 const int xx = x + m, yy = y + nn, zz = z + k;
                 if (xx >= 0 && yy >= 0 && zz >= 0 && xx < r - 1 && yy < r - 1 && zz < r - 1 &&
                     nd(T, l, xx, yy, zz) >= 0) {
-                    for (int c = 0; c < NC; ++c) out[c] += cellv(T, cell, NC, l, xx, yy, zz, c);
+                    for (int c = 0; c < NC; ++c) out[c] += cellvT(T, ld, cell, NC, l, xx, yy, zz, c);
                     count += 1;
                 }
             }
@@ -731,47 +1124,60 @@ __device__ void tiltFromOctagons(const Topo& T, const double* cell, int l, int x
 
 // interpolateClosedJoint (octree_omni_grid3.h:305-337), T-joint part
 template <int NC>
-__global__ void k_closed_joint(Topo T, const uint8_t* level, const int* tjList, int n, const double* cell,
-                               double* vert) {
-    const int t = blockIdx.x * BS + threadIdx.x;
-    if (t >= n) return;
-    const int g = tjList[t];
-    int l = level[g];
-    int x, y, z;
-    nodeXYZ(T, g, x, y, z);
-    if (T.te[g] < c_hl[l]) {
-        l -= 1;
-        x <<= 1;
-        y <<= 1;
-        z <<= 1;
-    }
-    for (int i = 0; i < 6; ++i) {
-        const int xx = x + c_nb6[i][0], yy = y + c_nb6[i][1], zz = z + c_nb6[i][2];
-        const int q = nd(T, l, xx, yy, zz);
-        if (q >= 0) {
-            double v[NC];
-            tiltFromOctagons<NC>(T, cell, l, xx, yy, zz, v);
-            for (int c = 0; c < NC; ++c) vert[(size_t)NC * q + c] = v[c];
+struct BClosedJoint {
+    Topo T;
+    const uint8_t* level;
+    const int* tjList;
+    const double* cell;
+    double* vert;
+    template <class Pol>
+    __device__ void operator()(int t, Pol& P) const {
+        const int g = tjList[t];
+        int l = level[g];
+        int x, y, z;
+        nodeXYZ(T, g, x, y, z);
+        if (T.te[g] < c_hl[l]) {
+            l -= 1;
+            x <<= 1;
+            y <<= 1;
+            z <<= 1;
+        }
+        PolLd<Pol> ld{P};
+        for (int i = 0; i < 6; ++i) {
+            const int xx = x + c_nb6[i][0], yy = y + c_nb6[i][1], zz = z + c_nb6[i][2];
+            const int q = nd(T, l, xx, yy, zz);
+            if (q >= 0) {
+                double v[NC];
+                tiltFromOctagons<NC>(T, ld, cell, l, xx, yy, zz, v);
+                for (int c = 0; c < NC; ++c) P.st(vert, (size_t)NC * q + c, v[c]);
+            }
         }
     }
-}
+};
 
 template <int NC>
-__global__ void k_closed_by_half(Topo T, const uint8_t* level, const int* list, int n, const double* cell,
-                                 double* vert, int* bad) {
-    const int t = blockIdx.x * BS + threadIdx.x;
-    if (t >= n) return;
-    const int g = list[t];
-    if (!((T.tb[g] & TB_CLOSED) && T.thd[g] == -1)) {
-        atomicOr(bad, 1);
-        return;
+struct BClosedByHalf {
+    Topo T;
+    const uint8_t* level;
+    const int* list;
+    const double* cell;
+    double* vert;
+    int* bad;
+    template <class Pol>
+    __device__ void operator()(int t, Pol& P) const {
+        const int g = list[t];
+        if (!((T.tb[g] & TB_CLOSED) && T.thd[g] == -1)) {
+            if (Pol::kAux) atomicOr(bad, 1);
+            return;
+        }
+        int x, y, z;
+        nodeXYZ(T, g, x, y, z);
+        double v[NC];
+        PolLd<Pol> ld{P};
+        tiltFromOctagons<NC>(T, ld, cell, level[g], x, y, z, v);
+        for (int c = 0; c < NC; ++c) P.st(vert, (size_t)NC * g + c, v[c]);
     }
-    int x, y, z;
-    nodeXYZ(T, g, x, y, z);
-    double v[NC];
-    tiltFromOctagons<NC>(T, cell, level[g], x, y, z, v);
-    for (int c = 0; c < NC; ++c) vert[(size_t)NC * g + c] = v[c];
-}
+};
 
 // ---- advection (octree_omnigrid_solver3.cpp:850-914, advection.h:195-239)
 // kind 0: octagon DOF (cell centre), 1: tilt / half-tilt DOF (vertex)
@@ -1068,6 +1474,7 @@ __global__ void k_points(Topo T, Vel V, const double* cell, const double* vert, 
     if (what == 0) velSample(T, V, p[0], p[1], p[2], out + 3 * (size_t)t);
     else if (what == 1) macSample(T, V.mac, p[0], p[1], p[2], out + 3 * (size_t)t);
     else if (what == 2) backTrace(T, V, p, dt, out + 3 * (size_t)t);
+    else if (what == 5) out[t] = sdfSample(T, p[0], p[1], p[2]);
     else out[t] = scalarSample(T, cell, vert, p[0], p[1], p[2]);
 }
 
@@ -1124,9 +1531,16 @@ void dualMatrices(double* out) {
 // host side
 // ---------------------------------------------------------------------------
 int AdvectDevice::upload(const Octree& o, const ProjectTables& t, bool keepFields, std::string& err) {
-    if (keepFields && (t.G != G_ || o.L != L_)) {
-        err = "tilt update changed the node sets";
-        return ASTG_ERR_STATE;
+    if (keepFields) {
+        // a tilt update keeps the node sets; anything else would leave the
+        // index maps, node lists and coordinates on the device stale
+        bool same = t.G == G_ && o.L == L_ && o.R == R_ && (int)nnode_.size() == o.L;
+        for (int l = 0; same && l < o.L; ++l)
+            same = o.lv[l].nnode == nnode_[l] && o.lv[l].nleaf == nleaf_[l] && t.off[l] == off_[l];
+        if (!same) {
+            err = "tilt update changed the node sets";
+            return ASTG_ERR_STATE;
+        }
     }
     if (keepFields && !sampValid_) {
         // the fields keep the tilt state they were written with until the next
@@ -1167,6 +1581,19 @@ int AdvectDevice::upload(const Octree& o, const ProjectTables& t, bool keepField
     if (mtot >= (1L << 31)) {
         err = "index maps too large";
         return ASTG_ERR_ARG;
+    }
+    cRes_ = res;
+    cMoff_ = moff;
+    cHl_ = hl;
+    if (cDual_.empty()) {
+        cDual_.resize(128 * 21);
+        dualMatrices(cDual_.data());
+    }
+    for (int k = 0; k < 3; ++k) cSlip_[k] = o.sdf.slip[k];
+    std::lock_guard<std::recursive_mutex> lk(constMutex());
+    {
+        int rc = bindConstants(true, err);
+        if (rc) return rc;
     }
     // A tilt update (keepFields) keeps the node sets: the index maps, node
     // coordinates and flags, leaf and ghost lists and levels on the device
@@ -1280,6 +1707,50 @@ int AdvectDevice::upload(const Octree& o, const ProjectTables& t, bool keepField
     HIPCHK(up(velTilt_, velTilt.data(), velTilt.size() * 4));
     nVelTilt_ = (int)velTilt.size();
     nHalfMac_ = (int)hmG.size();
+    // collider: the SDF grid and the MAC faces inside it (sdf <= 0 at the face
+    // position, isValidX/Y/Z), in the reference's visiting order
+    hasSdf_ = o.sdf.present();
+    if (hasSdf_ && !sameNodes) {
+        if (o.sdf.R != R_ || (long)o.sdf.phi.size() != (long)R_ * R_ * R_) {
+            err = "boundary SDF does not match the finest resolution";
+            return ASTG_ERR_ARG;
+        }
+        HIPCHK(up(sdf_, o.sdf.phi.data(), o.sdf.phi.size() * 8));
+        std::vector<int> fg, ord((size_t)G_ * 3, -1);
+        std::vector<uint8_t> fa;
+        for (int l = 0; l < L_; ++l) {
+            const Level& Lv = o.lv[l];
+            const double h = Lv.h;
+            const int r = Lv.res;
+            for (int i = 0; i < Lv.nleaf; ++i) {
+                const int x = Lv.x[i], y = Lv.y[i], z = Lv.z[i];
+                for (int a = 0; a < 3; ++a) {
+                    const bool valid = (a == 0 ? x <= r - 1 : x <= r - 2) && (a == 1 ? y <= r - 1 : y <= r - 2) &&
+                                       (a == 2 ? z <= r - 1 : z <= r - 2);
+                    if (!valid) continue;
+                    const double px = x * h + (a == 0 ? 0.0 : 0.5) * h, py = y * h + (a == 1 ? 0.0 : 0.5) * h,
+                                 pz = z * h + (a == 2 ? 0.0 : 0.5) * h;
+                    if (!(o.sdf.sample(px, py, pz) <= 0)) continue;
+                    const int g = off_[l] + i;
+                    ord[(size_t)g * 3 + a] = (int)fg.size();
+                    fg.push_back(g);
+                    fa.push_back((uint8_t)a);
+                }
+            }
+        }
+        nSlip_ = (int)fg.size();
+        HIPCHK(up(slipFaceG_, fg.data(), fg.size() * 4));
+        HIPCHK(up(slipFaceA_, fa.data(), fa.size()));
+        HIPCHK(up(slipOrd_, ord.data(), ord.size() * 4));
+        HIPCHK(slipPub_.alloc((size_t)std::max(nSlip_, 1) * 8));
+        HIPCHK(slipErr_.alloc(4));
+        int perCu = 0, dev = 0, cus = 0;
+        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, k_slip_mac, BS, 0));
+        HIPCHK(hipGetDevice(&dev));
+        HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        slipGrid_ = std::max(1, std::min(perCu * cus, (nSlip_ + BS - 1) / BS));
+    }
+    if (!hasSdf_) nSlip_ = 0;
     HIPCHK(up(hmG_, hmG.data(), hmG.size() * 4));
     HIPCHK(up(hmT_, hmT.data(), hmT.size() * 4));
     HIPCHK(up(hmA_, hmA.data(), hmA.size() * 4));
@@ -1306,9 +1777,12 @@ int AdvectDevice::upload(const Octree& o, const ProjectTables& t, bool keepField
     HIPCHK(hipMemsetAsync(tMac_.p, 0, (size_t)G_ * 24, stream_));
     HIPCHK(hipMemsetAsync(tTilt_.p, 0, (size_t)G_ * 64, stream_));
     HIPCHK(bad_.alloc(4));
-    // constants
-    std::vector<double> dm(128 * 21);
-    dualMatrices(dm.data());
+    HIPCHK(hipStreamSynchronize(stream_));
+    return ASTG_OK;
+}
+
+int AdvectDevice::bindConstants(bool force, std::string& err) {
+    if (!takeConstants(CT_ADVECT, constOwner_, force)) return ASTG_OK;
     const double s = 1.0 / 1.7320508075688772;
     double dt4[12] = {s, -s, -s, s, s, s, -s, -s, s, s, s, s};
     for (double& v : dt4) v *= 3.0 / 4.0;
@@ -1317,15 +1791,16 @@ int AdvectDevice::upload(const Octree& o, const ProjectTables& t, bool keepField
     HIPCHK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_R), &RR, sizeof(int), 0, hipMemcpyHostToDevice, stream_));
     HIPCHK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_G), &GG, sizeof(int), 0, hipMemcpyHostToDevice, stream_));
     HIPCHK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_h0), &h0_, sizeof(double), 0, hipMemcpyHostToDevice, stream_));
-    HIPCHK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_res), res.data(), sizeof(int) * L_, 0, hipMemcpyHostToDevice, stream_));
-    HIPCHK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_moff), moff.data(), sizeof(int) * L_, 0, hipMemcpyHostToDevice,
+    HIPCHK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_res), cRes_.data(), sizeof(int) * L_, 0, hipMemcpyHostToDevice,
                                   stream_));
-    HIPCHK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_hl), hl.data(), sizeof(double) * L_, 0, hipMemcpyHostToDevice,
+    HIPCHK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_moff), cMoff_.data(), sizeof(int) * L_, 0, hipMemcpyHostToDevice,
                                   stream_));
-    HIPCHK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_dual), dm.data(), sizeof(double) * dm.size(), 0,
+    HIPCHK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_hl), cHl_.data(), sizeof(double) * L_, 0, hipMemcpyHostToDevice,
+                                  stream_));
+    HIPCHK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_dual), cDual_.data(), sizeof(double) * cDual_.size(), 0,
                                   hipMemcpyHostToDevice, stream_));
     HIPCHK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_dualTilt), dt4, sizeof(dt4), 0, hipMemcpyHostToDevice, stream_));
-    HIPCHK(hipStreamSynchronize(stream_));
+    HIPCHK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_slip), cSlip_, sizeof(double) * 3, 0, hipMemcpyHostToDevice, stream_));
     return ASTG_OK;
 }
 
@@ -1339,6 +1814,7 @@ AdvTopo AdvectDevice::topo() const {
     T.te = te_.as<double>();
     T.tb = static_cast<const uint8_t*>(tb_.p);
     T.thd = static_cast<const int8_t*>(thd_.p);
+    T.sdf = hasSdf_ ? sdf_.as<double>() : nullptr;
     return T;
 }
 
@@ -1354,6 +1830,8 @@ AdvTopo AdvectDevice::sampTopo() const {
 
 int AdvectDevice::decideTilts(ProjectDevice& pd, double dThr, double dt, double vThr, double maxThr,
                               std::vector<uint8_t>& open, std::string& err) {
+    std::lock_guard<std::recursive_mutex> lk_(constMutex());
+    if (int rc_ = bindConstants(false, err)) return rc_;
     open.assign(nLeaf_, 0);
     if (!nLeaf_) return ASTG_OK;
     HIPCHK(open_.alloc((size_t)nLeaf_));
@@ -1412,56 +1890,68 @@ int AdvectDevice::omniGhost(double* cell, double* vert, int nc, std::string& err
     const int gb = nblocks(nGhost_);
     if (nc == 1) {
         if (nGhost_) {
-            k_cell_ghost_fine<1><<<gb, BS, 0, stream_>>>(T, lev, gl, nGhost_, cell);
-            k_cell_ghost_coarse<1><<<gb, BS, 0, stream_>>>(T, lev, gl, nGhost_, cell);
-            k_vert_ghost<1><<<gb, BS, 0, stream_>>>(T, lev, gl, nGhost_, ih, vert);
-            k_cell_ghost_tj<1><<<gb, BS, 0, stream_>>>(T, lev, gl, nGhost_, vert, cell);
+            k_run<<<gb, BS, 0, stream_>>>(BCellGhostFine<1>{T, lev, gl, cell}, nGhost_);
+            k_run<<<gb, BS, 0, stream_>>>(BCellGhostCoarse<1>{T, lev, gl, cell}, nGhost_);
+            k_run<<<gb, BS, 0, stream_>>>(BVertGhost<1>{T, lev, gl, ih, vert}, nGhost_);
+            k_run<<<gb, BS, 0, stream_>>>(BCellGhostTJ<1>{T, lev, gl, vert, cell}, nGhost_);
         }
-        if (nTJ_) k_closed_joint<1><<<nblocks(nTJ_), BS, 0, stream_>>>(T, lev, tj_.as<int>(), nTJ_, cell, vert);
+        if (nTJ_) k_run<<<nblocks(nTJ_), BS, 0, stream_>>>(BClosedJoint<1>{T, lev, tj_.as<int>(), cell, vert}, nTJ_);
         if (nCBH_)
-            k_closed_by_half<1><<<nblocks(nCBH_), BS, 0, stream_>>>(T, lev, cbh_.as<int>(), nCBH_, cell, vert,
-                                                                     bad_.as<int>());
+            k_run<<<nblocks(nCBH_), BS, 0, stream_>>>(BClosedByHalf<1>{T, lev, cbh_.as<int>(), cell, vert,
+                                                                         bad_.as<int>()}, nCBH_);
     } else {
         if (nGhost_) {
-            k_cell_ghost_fine<3><<<gb, BS, 0, stream_>>>(T, lev, gl, nGhost_, cell);
-            k_cell_ghost_coarse<3><<<gb, BS, 0, stream_>>>(T, lev, gl, nGhost_, cell);
-            k_vert_ghost<3><<<gb, BS, 0, stream_>>>(T, lev, gl, nGhost_, ih, vert);
-            k_cell_ghost_tj<3><<<gb, BS, 0, stream_>>>(T, lev, gl, nGhost_, vert, cell);
+            k_run<<<gb, BS, 0, stream_>>>(BCellGhostFine<3>{T, lev, gl, cell}, nGhost_);
+            k_run<<<gb, BS, 0, stream_>>>(BCellGhostCoarse<3>{T, lev, gl, cell}, nGhost_);
+            k_run<<<gb, BS, 0, stream_>>>(BVertGhost<3>{T, lev, gl, ih, vert}, nGhost_);
+            k_run<<<gb, BS, 0, stream_>>>(BCellGhostTJ<3>{T, lev, gl, vert, cell}, nGhost_);
         }
-        if (nTJ_) k_closed_joint<3><<<nblocks(nTJ_), BS, 0, stream_>>>(T, lev, tj_.as<int>(), nTJ_, cell, vert);
+        if (nTJ_) k_run<<<nblocks(nTJ_), BS, 0, stream_>>>(BClosedJoint<3>{T, lev, tj_.as<int>(), cell, vert}, nTJ_);
         if (nCBH_)
-            k_closed_by_half<3><<<nblocks(nCBH_), BS, 0, stream_>>>(T, lev, cbh_.as<int>(), nCBH_, cell, vert,
-                                                                     bad_.as<int>());
+            k_run<<<nblocks(nCBH_), BS, 0, stream_>>>(BClosedByHalf<3>{T, lev, cbh_.as<int>(), cell, vert,
+                                                                         bad_.as<int>()}, nCBH_);
     }
     HIPCHK(hipGetLastError());
     return ASTG_OK;
 }
 
 int AdvectDevice::constrainAndGhost(ProjectDevice& pd, std::string& err, bool constrain) {
+    std::lock_guard<std::recursive_mutex> lk_(constMutex());
+    if (int rc_ = bindConstants(false, err)) return rc_;
     const Topo T = topo();
     const uint8_t* lev = static_cast<const uint8_t*>(level_.p);
     double* mac = pd.uMac();
     double* tilt = pd.uTilt();
-    // constrain (MAC walls; tilt and dual constraints need a collider)
-    if (constrain) k_wall<<<nblocks(G_), BS, 0, stream_>>>(T, lev, mac, closed_);
+    // constrainVelocity: MAC slip on the collider, then the closed walls, then
+    // the tilt slots (omni_boundary_solver3.h:26-161)
+    if (constrain) {
+        if (hasSdf_ && nSlip_) {
+            int rc = slipMac(pd, err);
+            if (rc) return rc;
+        }
+        k_run<<<nblocks(G_), BS, 0, stream_>>>(BWall{T, lev, mac, closed_}, G_);
+        if (hasSdf_ && nLeaf_) k_slip_tilt<<<nblocks(nLeaf_), BS, 0, stream_>>>(T, lev, leaf_.as<int>(), nLeaf_, tilt);
+    }
     // velocity ghosts (octree_omni_face_centered_grid3.h:97-120)
     if (nGhost_) {
-        k_mac_ghost_clear<<<nblocks(nGhost_), BS, 0, stream_>>>(ghost_.as<int>(), nGhost_, mac);
-        k_mac_ghost_sample<<<nblocks(nGhost_), BS, 0, stream_>>>(T, lev, ghost_.as<int>(), nGhost_, mac, 0);
+        k_run<<<nblocks(nGhost_), BS, 0, stream_>>>(BMacGhostClear{ghost_.as<int>(), mac}, nGhost_);
+        k_run<<<nblocks(nGhost_), BS, 0, stream_>>>(BMacGhostSample{T, lev, ghost_.as<int>(), mac, 0}, nGhost_);
     }
     if (nHalfMac_)
-        k_mac_from_half<<<nblocks(nHalfMac_), BS, 0, stream_>>>(hmG_.as<int>(), hmT_.as<int>(), hmA_.as<int>(),
-                                                                 hmS_.as<int>(), nHalfMac_, tilt, mac);
+        k_run<<<nblocks(nHalfMac_), BS, 0, stream_>>>(
+            BMacFromHalf{hmG_.as<int>(), hmT_.as<int>(), hmA_.as<int>(), hmS_.as<int>(), tilt, mac}, nHalfMac_);
     if (nGhost_) {
-        k_mac_ghost_sample<<<nblocks(nGhost_), BS, 0, stream_>>>(T, lev, ghost_.as<int>(), nGhost_, mac, 1);
-        k_tilt_ghost<<<nblocks(nGhost_), BS, 0, stream_>>>(T, lev, ghost_.as<int>(), nGhost_,
-                                                            static_cast<const uint8_t*>(inHalf_.p), tilt);
+        k_run<<<nblocks(nGhost_), BS, 0, stream_>>>(BMacGhostSample{T, lev, ghost_.as<int>(), mac, 1}, nGhost_);
+        k_run<<<nblocks(nGhost_), BS, 0, stream_>>>(
+            BTiltGhost{T, lev, ghost_.as<int>(), static_cast<const uint8_t*>(inHalf_.p), tilt}, nGhost_);
     }
     HIPCHK(hipGetLastError());
     return ASTG_OK;
 }
 
 int AdvectDevice::postProject(ProjectDevice& pd, std::string& err, bool constrain) {
+    std::lock_guard<std::recursive_mutex> lk_(constMutex());
+    if (int rc_ = bindConstants(false, err)) return rc_;
     const Topo T = topo();
     const uint8_t* lev = static_cast<const uint8_t*>(level_.p);
     double* mac = pd.uMac();
@@ -1471,16 +1961,66 @@ int AdvectDevice::postProject(ProjectDevice& pd, std::string& err, bool constrai
     if (rc) return rc;
     // dual grid
     if (nLeaf_) {
-        k_dual_tilt<<<nblocks(nLeaf_), BS, 0, stream_>>>(T, leaf_.as<int>(), nLeaf_, tilt, dualTilt_.as<double>());
-        k_dual_oct<<<nblocks(nLeaf_), BS, 0, stream_>>>(T, lev, leaf_.as<int>(), nLeaf_, mac, tilt,
-                                                         dualOct_.as<double>());
+        k_run<<<nblocks(nLeaf_), BS, 0, stream_>>>(BDualTilt{T, leaf_.as<int>(), tilt, dualTilt_.as<double>()},
+                                                    nLeaf_);
+        k_run<<<nblocks(nLeaf_), BS, 0, stream_>>>(
+            BDualOct{T, lev, leaf_.as<int>(), mac, tilt, dualOct_.as<double>()}, nLeaf_);
     }
     if (nHalf_)
-        k_dual_half<<<nblocks(nHalf_), BS, 0, stream_>>>(T, half_.as<int>(), nHalf_, tilt, dualTilt_.as<double>());
+        k_run<<<nblocks(nHalf_), BS, 0, stream_>>>(BDualHalf{T, half_.as<int>(), tilt, dualTilt_.as<double>()},
+                                                    nHalf_);
+    // constrainVelocity(dual grid): the vertex vectors inside the collider
+    if (constrain && hasSdf_ && nLeaf_)
+        k_slip_dual<<<nblocks(nLeaf_), BS, 0, stream_>>>(T, lev, leaf_.as<int>(), nLeaf_, dualTilt_.as<double>());
     HIPCHK(hipGetLastError());
     rc = omniGhost(dualOct_.as<double>(), dualTilt_.as<double>(), 3, err);
     if (rc) return rc;
     return checkBad(err);
+}
+
+int AdvectDevice::slipMac(ProjectDevice& pd, std::string& err) {
+    const Topo T = topo();
+    const uint8_t* lev = static_cast<const uint8_t*>(level_.p);
+    double* mac = pd.uMac();
+    const int* ord = slipOrd_.as<int>();
+    const int* fg = slipFaceG_.as<int>();
+    const uint8_t* fa = static_cast<const uint8_t*>(slipFaceA_.p);
+    double* pub = slipPub_.as<double>();
+    int* e = slipErr_.as<int>();
+    int nf = nSlip_;
+    k_fill_sent64<<<nblocks(nf), BS, 0, stream_>>>(pub, nf);
+    HIPCHK(hipMemsetAsync(e, 0, 4, stream_));
+    // every block must be resident at once (faces wait on earlier faces)
+    void* args[] = {(void*)&T, (void*)&lev, (void*)&mac, (void*)&ord, (void*)&fg, (void*)&fa, (void*)&nf, (void*)&pub,
+                    (void*)&e};
+    HIPCHK(hipLaunchCooperativeKernel((const void*)k_slip_mac, dim3(slipGrid_), dim3(BS), args, 0, stream_));
+    k_slip_scatter<<<nblocks(nf), BS, 0, stream_>>>(fg, fa, nf, pub, mac);
+    HIPCHK(hipGetLastError());
+    int bad = 0;
+    HIPCHK(hipMemcpyAsync(&bad, e, 4, hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    if (bad) {
+        err = "slip constraint: a face waited past its spin limit";
+        return ASTG_ERR_HIP;
+    }
+    return ASTG_OK;
+}
+
+int AdvectDevice::wipeBoundary(std::string& err) {
+    std::lock_guard<std::recursive_mutex> lk_(constMutex());
+    if (int rc_ = bindConstants(false, err)) return rc_;
+    if (!hasSdf_) return ASTG_OK;
+    const Topo T = topo();
+    const uint8_t* lev = static_cast<const uint8_t*>(level_.p);
+    double* dC = scal_[0][cur_[0]][0].as<double>();
+    double* dV = scal_[0][cur_[0]][1].as<double>();
+    double* hC = scal_[1][cur_[1]][0].as<double>();
+    double* hV = scal_[1][cur_[1]][1].as<double>();
+    if (nLeaf_) k_wipe<<<nblocks(nLeaf_), BS, 0, stream_>>>(T, lev, leaf_.as<int>(), nLeaf_, 0, dC, dV, hC, hV);
+    if (nHalf_) k_wipe<<<nblocks(nHalf_), BS, 0, stream_>>>(T, lev, half_.as<int>(), nHalf_, 1, dC, dV, hC, hV);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(stream_));
+    return ASTG_OK;
 }
 
 int AdvectDevice::checkBad(std::string& err) {
@@ -1495,6 +2035,8 @@ int AdvectDevice::checkBad(std::string& err) {
 }
 
 int AdvectDevice::scalarGhost(int field, std::string& err) {
+    std::lock_guard<std::recursive_mutex> lk_(constMutex());
+    if (int rc_ = bindConstants(false, err)) return rc_;
     if (field < 0 || field > 1) {
         err = "bad field";
         return ASTG_ERR_ARG;
@@ -1515,6 +2057,8 @@ AdvVel AdvectDevice::vel(ProjectDevice& pd) const {
 }
 
 int AdvectDevice::advect(ProjectDevice& pd, double dt, std::string& err) {
+    std::lock_guard<std::recursive_mutex> lk_(constMutex());
+    if (int rc_ = bindConstants(false, err)) return rc_;
     const Topo T = sampTopo();  // sources: the tilt state they were written with
     const uint8_t* lev = static_cast<const uint8_t*>(level_.p);
     const Vel V = vel(pd);
@@ -1584,6 +2128,8 @@ int AdvectDevice::mirrorTemporaries(ProjectDevice& pd, const bool scalars[2], bo
 
 int AdvectDevice::applySource(ProjectDevice& pd, double cx, double cy, double cz, double r, double dval, double hval,
                               double speed, bool ghost, std::string& err) {
+    std::lock_guard<std::recursive_mutex> lk_(constMutex());
+    if (int rc_ = bindConstants(false, err)) return rc_;
     const Topo T = topo();
     const uint8_t* lev = static_cast<const uint8_t*>(level_.p);
     double* dC = scal_[0][cur_[0]][0].as<double>();
@@ -1608,6 +2154,8 @@ int AdvectDevice::applySource(ProjectDevice& pd, double cx, double cy, double cz
 }
 
 int AdvectDevice::addForce(ProjectDevice& pd, double dt, double fd, double ft, std::string& err) {
+    std::lock_guard<std::recursive_mutex> lk_(constMutex());
+    if (int rc_ = bindConstants(false, err)) return rc_;
     const Topo T = topo();
     const uint8_t* lev = static_cast<const uint8_t*>(level_.p);
     const double* dC = scal_[0][cur_[0]][0].as<double>();
@@ -1648,7 +2196,9 @@ int AdvectDevice::addForce(ProjectDevice& pd, double dt, double fd, double ft, s
 
 int AdvectDevice::samplePoints(ProjectDevice& pd, int what, int n, const double* xyz, double dt, double* out,
                                std::string& err) {
-    if (what < 0 || what > 4 || n < 0) {
+    std::lock_guard<std::recursive_mutex> lk_(constMutex());
+    if (int rc_ = bindConstants(false, err)) return rc_;
+    if (what < 0 || what > 5 || n < 0) {
         err = "bad query";
         return ASTG_ERR_ARG;
     }

@@ -64,6 +64,7 @@ int guarded(astg_ctx* c, F&& f) {
 }
 
 int runProject(astg_ctx* c, double dt, double rho, const astg_pcg_params& prm, astg_pcg_result* result);
+int needAdvect(astg_ctx* c);
 
 int needDevice(astg_ctx* c) {
     if (c->device < 0) return fail(c, ASTG_ERR_STATE, "context has no device (created with device < 0)");
@@ -114,8 +115,27 @@ int ensureTables(astg_ctx* c) {
 
 // Tilt update with the fields kept: numbering, weights, tables, device
 // topology and the projection pattern follow the new tilt state; the node
-// sets, and so every field array, are unchanged.
+// sets, and so every field array, are unchanged. On any failure the cached
+// tables and device state are dropped (tablesReady = deviceReady = false), so
+// the next call rebuilds them from the tree's tilt state (fields restart at
+// zero) instead of running on tables and buffers of two different states.
+int retopologizeSteps(astg_ctx* c);
 int retopologize(astg_ctx* c) {
+    int rc;
+    try {
+        rc = retopologizeSteps(c);
+    } catch (...) {
+        c->tablesReady = false;
+        c->deviceReady = false;
+        throw;
+    }
+    if (rc) {
+        c->tablesReady = false;
+        c->deviceReady = false;
+    }
+    return rc;
+}
+int retopologizeSteps(astg_ctx* c) {
     astg::PhaseTimer tm("retopologize");
     astg::numberUnknowns(*c->tree, c->num);
     tm.mark("numbering");
@@ -199,9 +219,14 @@ int astg_destroy(astg_ctx* c) {
     if (c->device >= 0) {
         (void)hipSetDevice(c->device);
         if (c->stream) (void)hipStreamSynchronize(c->stream);
+        // every device buffer (and pinned host block) is released while the
+        // stream and the runtime are still alive, before the stream goes
+        c->adv.reset();
         c->pcg.reset();
         c->proj.reset();
         c->csr.reset();
+        c->batch.reset();
+        c->lastPcg = nullptr;
         for (auto& e : c->ev)
             if (e) (void)hipEventDestroy(e);
         if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -252,6 +277,40 @@ int astg_set_tilt_open(astg_ctx* c, int level, const unsigned char* mask) {
     c->tree->setTiltOpen(level, mask);
     c->tablesReady = false;
     c->deviceReady = false;
+    return ASTG_OK;
+}
+
+int astg_set_boundary_sdf(astg_ctx* c, const double* sdf, const double* slip_center) {
+    if (!c) return ASTG_ERR_ARG;
+    if (!c->tree) return fail(c, ASTG_ERR_STATE, "no topology set");
+    return guarded(c, [&] {
+        astg::BoundarySdf b;
+        if (sdf) {
+            const size_t n = (size_t)c->tree->R * c->tree->R * c->tree->R;
+            for (size_t i = 0; i < n; ++i)
+                if (!std::isfinite(sdf[i])) return fail(c, ASTG_ERR_ARG, "boundary SDF holds a non-finite value");
+            b.R = c->tree->R;
+            b.h0 = c->tree->h0;
+            b.phi.assign(sdf, sdf + n);
+            if (slip_center)
+                for (int k = 0; k < 3; ++k) b.slip[k] = slip_center[k];
+        }
+        c->tree->sdf = std::move(b);
+        // numbering, weights, tables and every device array follow
+        c->tablesReady = false;
+        c->deviceReady = false;
+        return ASTG_OK;
+    });
+}
+
+int astg_sphere_sdf(int R, double h0, double cx, double cy, double cz, double radius, double* out) {
+    if (R <= 0 || !(h0 > 0) || !(radius >= 0) || !out) return ASTG_ERR_ARG;
+    for (int z = 0; z < R; ++z)
+        for (int y = 0; y < R; ++y)
+            for (int x = 0; x < R; ++x) {
+                const double dx = (x + 0.5) * h0 - cx, dy = (y + 0.5) * h0 - cy, dz = (z + 0.5) * h0 - cz;
+                out[((size_t)z * R + y) * R + x] = std::sqrt(dx * dx + dy * dy + dz * dz) - radius;
+            }
     return ASTG_OK;
 }
 
@@ -414,6 +473,9 @@ int runProject(astg_ctx* c, double dt, double rho, const astg_pcg_params& prm, a
         if (r) return r;
         if (c->tables.half_duplicates)
             return fail(c, ASTG_ERR_TOPOLOGY, "half-tilt list holds duplicates (the reference races on them)");
+        // the solver's pattern must be the tables' (a failed tilt update drops both)
+        if (c->pcg->n() != c->tables.n || c->pcg->nnz() != (long long)c->tables.col.size())
+            return fail(c, ASTG_ERR_STATE, "solver pattern does not match the topology tables");
         astg_pcg_result res;
         std::memset(&res, 0, sizeof(res));
         hipEvent_t* ev = c->ev;
@@ -451,6 +513,94 @@ int runProject(astg_ctx* c, double dt, double rho, const astg_pcg_params& prm, a
 }  // namespace
 
 extern "C" {
+
+int astg_field_layout(astg_ctx* c, long long* G, long long* offsets) {
+    if (!c) return ASTG_ERR_ARG;
+    if (!c->tree) return fail(c, ASTG_ERR_STATE, "no topology set");
+    long long g = 0;
+    for (int l = 0; l < c->tree->L; ++l) {
+        if (offsets) offsets[l] = g;
+        g += c->tree->lv[l].nnode;
+    }
+    if (offsets) offsets[c->tree->L] = g;
+    if (G) *G = g;
+    return ASTG_OK;
+}
+
+namespace {
+// Copies between the caller's fields and the context's, ordered after the
+// caller's stream and before its later work (device pointers), or blocking
+// (host pointers). toContext: upload.
+int moveFields(astg_ctx* c, const astg_fields* f, bool toContext) {
+    if (!f) return fail(c, ASTG_ERR_ARG, "null fields");
+    int rc = needAdvect(c);
+    if (rc) return rc;
+    const size_t G = c->tables.G;
+    struct Pair {
+        double* user;
+        double* mine;
+        size_t n;
+    } pairs[6] = {{f->mac, c->proj->uMac(), 3 * G},
+                  {f->tilt, c->proj->uTilt(), 8 * G},
+                  {f->density_cell, c->adv->scalarPtr(0, 0), G},
+                  {f->density_vert, c->adv->scalarPtr(0, 1), G},
+                  {f->heat_cell, c->adv->scalarPtr(1, 0), G},
+                  {f->heat_vert, c->adv->scalarPtr(1, 1), G}};
+    if (!f->device_pointers) {
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(c, ASTG_ERR_HIP, "stream sync failed");
+        for (const Pair& p : pairs) {
+            if (!p.user) continue;
+            if (hipMemcpy(toContext ? p.mine : p.user, toContext ? p.user : p.mine, p.n * 8,
+                          toContext ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost) != hipSuccess)
+                return fail(c, ASTG_ERR_HIP, "field copy failed");
+        }
+        return ASTG_OK;
+    }
+    hipStream_t us = static_cast<hipStream_t>(f->stream);
+    // the context's stream waits for the caller's queued work ...
+    if (hipEventRecord(c->ev[5], us) != hipSuccess || hipStreamWaitEvent(c->stream, c->ev[5], 0) != hipSuccess)
+        return fail(c, ASTG_ERR_HIP, "stream ordering failed");
+    for (const Pair& p : pairs) {
+        if (!p.user) continue;
+        hipPointerAttribute_t at;
+        if (hipPointerGetAttributes(&at, p.user) != hipSuccess || at.type != hipMemoryTypeDevice ||
+            at.device != c->device)
+            return fail(c, ASTG_ERR_ARG, "a field pointer is not device memory of the context's GPU");
+        if (hipMemcpyAsync(toContext ? p.mine : p.user, toContext ? p.user : p.mine, p.n * 8,
+                           hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+            return fail(c, ASTG_ERR_HIP, "field copy failed");
+    }
+    // ... and the caller's later work waits for the copies
+    if (hipEventRecord(c->ev[5], c->stream) != hipSuccess || hipStreamWaitEvent(us, c->ev[5], 0) != hipSuccess)
+        return fail(c, ASTG_ERR_HIP, "stream ordering failed");
+    if (toContext) c->adv->fieldsCurrent();
+    return ASTG_OK;
+}
+}  // namespace
+
+int astg_fields_upload(astg_ctx* c, const astg_fields* f) {
+    if (!c) return ASTG_ERR_ARG;
+    return guarded(c, [&] { return moveFields(c, f, true); });
+}
+
+int astg_fields_download(astg_ctx* c, const astg_fields* f) {
+    if (!c) return ASTG_ERR_ARG;
+    return guarded(c, [&] { return moveFields(c, f, false); });
+}
+
+int astg_project_fields(astg_ctx* c, double dt, double rho, const astg_pcg_params* params, astg_fields* inout,
+                        astg_pcg_result* result) {
+    if (!c) return ASTG_ERR_ARG;
+    if (!inout) return fail(c, ASTG_ERR_ARG, "null fields");
+    astg_fields vel = *inout;  // the projection reads and writes the velocity only
+    vel.density_cell = vel.density_vert = vel.heat_cell = vel.heat_vert = nullptr;
+    int rc = astg_fields_upload(c, &vel);
+    if (rc) return rc;
+    rc = astg_project(c, dt, rho, params, result);
+    if (rc && rc != ASTG_ERR_NOT_CONVERGED) return rc;
+    const int rc2 = astg_fields_download(c, &vel);
+    return rc2 ? rc2 : rc;
+}
 
 int astg_get_system(astg_ctx* c, int* rowstart, int* col, double* val, double* b, double* p) {
     if (!c) return ASTG_ERR_ARG;
@@ -906,6 +1056,12 @@ int astg_apply_sphere_source(astg_ctx* c, double cx, double cy, double cz, doubl
     return c->adv->applySource(*c->proj, cx, cy, cz, radius, density, heat, speed, update_ghost != 0, c->err);
 }
 
+int astg_wipe_boundary(astg_ctx* c) {
+    int rc = needAdvect(c);
+    if (rc) return rc;
+    return c->adv->wipeBoundary(c->err);
+}
+
 int astg_add_force(astg_ctx* c, double dt, double smoke_density_factor, double temperature_factor) {
     if (!(dt >= 0)) return fail(c, ASTG_ERR_ARG, "dt must be non-negative");
     int rc = needAdvect(c);
@@ -1028,8 +1184,19 @@ int astg_read_frame(astg_ctx* c, const char* layout, const char* tilt, const cha
         if (newTilt) {
             o.tilt.swap(tiltState);
             if (c->deviceReady) {
-                int r = retopologize(c);
-                if (r) return r;
+                // a failed update puts the previous tilt state back (the
+                // cached tables are dropped and rebuilt on the next call)
+                int r;
+                try {
+                    r = retopologize(c);
+                } catch (...) {
+                    o.tilt.swap(tiltState);
+                    throw;
+                }
+                if (r) {
+                    o.tilt.swap(tiltState);
+                    return r;
+                }
             } else {
                 c->tablesReady = false;
             }

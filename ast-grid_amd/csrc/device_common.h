@@ -133,9 +133,12 @@ struct DepRow {
     // the lane's own dependencies became ready, so a lane never waits for the
     // other lanes of its wavefront before publishing. Between polling rounds
     // the lane sleeps 1, 2, 4, ... up to `cap` units of 64 cycles. Returns
-    // false (after publishing NaN) if the spin limit was hit.
+    // false (after publishing NaN) if the spin limit was hit, or if another
+    // row of the launch has raised an error bit in *errBits (checked every
+    // 256 rounds): once one row has given up, the others stop polling too, so
+    // one non-resident block costs one spin budget, not one per waiting row.
     template <class Pub>
-    __device__ bool run(const T* vec, T acc, int cap, Pub publish) {
+    __device__ bool run(const T* vec, T acc, int cap, const int* errBits, Pub publish) {
         T v[N];
         unsigned ready = 0;
         const unsigned full = nd >= 32 ? ~0u : (1u << nd) - 1u;
@@ -153,7 +156,8 @@ struct DepRow {
                         if (j < nd) a -= f[j] * v[j];
                     publish(a);
                     pending = false;
-                } else if (spins > kSpinLimit) {
+                } else if (spins > kSpinLimit ||
+                           ((spins & 255) == 255 && ld_agent_i32(errBits) != 0)) {
                     publish((T)NAN);
                     pending = false;
                     ok = false;

@@ -236,6 +236,9 @@ bool readTiltFile(const std::string& path, Octree& o, std::string& err) {
             TiltRec r;
             if (!in.get(r)) return failWith(err, path, "truncated");
             if (r.halfDir < -128 || r.halfDir > 127) return failWith(err, path, "half direction out of range");
+            // a half-tilt's face direction indexes the six faces (HALF_TILT_SLOT_INDEX)
+            if (r.half && (r.halfDir < 0 || r.halfDir > 5))
+                return failWith(err, path, "half-tilt with a half direction outside 0..5");
             uint8_t b = t[l][i].bits & (TB_LAYOUT_OPEN | TB_LAYOUT_FIXED);
             if (r.largest) b |= TB_LARGEST;
             if (r.closed) b |= TB_CLOSED;

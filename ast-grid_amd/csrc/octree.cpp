@@ -340,24 +340,61 @@ bool Octree::setTiltOpenAll(const uint8_t* openMasks) {
 // (levels ascending, leaf order), octagon before tilt at each node; then the
 // half-tilts in halfGhostIdx order; then the int ghost rule
 // (octree_vertex_centered_grid3.cpp:47-64) on the tilt indices.
+// getBaryCentric, vector form (util.h:244-257): truncation, index and
+// fraction clamped.
+static inline void baryClamp(double x, int high, int& i, double& f) {
+    i = (int)x;
+    i = std::max(i, 0);
+    i = std::min(i, high - 1);
+    f = x - (double)i;
+    f = std::max(f, 0.0);
+    f = std::min(f, 1.0);
+}
+
+double BoundarySdf::sample(double x, double y, double z) const {
+    if (phi.empty()) return 1e4;
+    int i[3];
+    double f[3];
+    const double p[3] = {x / h0 - 0.5, y / h0 - 0.5, z / h0 - 0.5};
+    for (int d = 0; d < 3; ++d) baryClamp(p[d], R - 1, i[d], f[d]);
+    const int j[3] = {std::min(i[0] + 1, R - 1), std::min(i[1] + 1, R - 1), std::min(i[2] + 1, R - 1)};
+    auto g = [&](int a, int b, int c) { return phi[((size_t)c * R + b) * R + a]; };
+    auto lerp = [](double v0, double v1, double t) { return v0 * (1 - t) + v1 * t; };
+    // trilerp (util.h:165-198): x within each (y, z) pair, then y, then z
+    const double c00 = lerp(g(i[0], i[1], i[2]), g(j[0], i[1], i[2]), f[0]);
+    const double c10 = lerp(g(i[0], j[1], i[2]), g(j[0], j[1], i[2]), f[0]);
+    const double c01 = lerp(g(i[0], i[1], j[2]), g(j[0], i[1], j[2]), f[0]);
+    const double c11 = lerp(g(i[0], j[1], j[2]), g(j[0], j[1], j[2]), f[0]);
+    return lerp(lerp(c00, c10, f[1]), lerp(c01, c11, f[1]), f[2]);
+}
+
 void numberUnknowns(const Octree& o, Numbering& num) {
     num.oct.assign(o.L, {});
     num.tilt.assign(o.L, {});
+    const BoundarySdf& B = o.sdf;
+    // the collider cut: sdf(position) > -3 h_l (octree_omnigrid_solver3.cpp:294-315)
+    auto keep = [&](int l, double x, double y, double z) {
+        return !B.present() || B.sample(x, y, z) > -o.lv[l].h * 3;
+    };
     int c = 0;
     for (int l = 0; l < o.L; ++l) {
         const Level& Lv = o.lv[l];
+        const double h = Lv.h;
         num.oct[l].assign(Lv.nnode, -1);
         num.tilt[l].assign(Lv.nnode, -1);
         for (int i = 0; i < Lv.nleaf; ++i) {
-            if (octValid(Lv, Lv.x[i], Lv.y[i], Lv.z[i])) num.oct[l][i] = c++;
-            if (o.tilt[l][i].bits & TB_LAYOUT_OPEN) num.tilt[l][i] = c++;
+            const int x = Lv.x[i], y = Lv.y[i], z = Lv.z[i];
+            if (octValid(Lv, x, y, z) && keep(l, (x + 0.5) * h, (y + 0.5) * h, (z + 0.5) * h)) num.oct[l][i] = c++;
+            if ((o.tilt[l][i].bits & TB_LAYOUT_OPEN) && keep(l, x * h, y * h, z * h)) num.tilt[l][i] = c++;
         }
     }
     num.n_oct_tilt = c;
-    for (lcoor_t h : o.half) {
-        int l = lc_level(h);
-        int i = o.node(l, lc_x(h), lc_y(h), lc_z(h));
-        num.tilt[l][i] = c++;
+    for (lcoor_t hc : o.half) {
+        int l = lc_level(hc);
+        int i = o.node(l, lc_x(hc), lc_y(hc), lc_z(hc));
+        const double h = o.lv[l].h;
+        if (keep(l, lc_x(hc) * h, lc_y(hc) * h, lc_z(hc) * h)) num.tilt[l][i] = c++;
+        else num.tilt[l][i] = -1;
     }
     num.n = c;
     for (int l = 0; l < o.L; ++l) {

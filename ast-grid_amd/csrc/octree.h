@@ -57,6 +57,24 @@ inline int lc_x(lcoor_t c) { return (int)((c >> 40) & 0xfffff); }
 inline int lc_y(lcoor_t c) { return (int)((c >> 20) & 0xfffff); }
 inline int lc_z(lcoor_t c) { return (int)(c & 0xfffff); }
 
+// Boundary (collider) signed distance on the finest grid, as the reference's
+// colliderSdf (CellCenteredScalarGrid3 of resolution R and spacing h0,
+// omni_boundary_solver3.cpp:11-31): R^3 values at the cell centres
+// ((i+0.5) h0), x fastest; negative inside the collider. Absent (empty) means
+// no collider: the reference then samples a constant positive SDF.
+struct BoundarySdf {
+    int R = 0;
+    double h0 = 0;
+    std::vector<double> phi;
+    // centre the slip constraint takes its normals from (the reference
+    // hard-codes (0.5, 0.5, 0.45), omni_boundary_solver3.h:27,57,77)
+    double slip[3] = {0.5, 0.5, 0.45};
+    bool present() const { return !phi.empty(); }
+    // CellCenteredGrid::sample (cell_centered_grid.h:40-54): clamped
+    // trilinear interpolation of the cell-centred values.
+    double sample(double x, double y, double z) const;
+};
+
 struct TiltNode {
     double e = 0;
     uint8_t bits = TB_CLOSED;  // default TiltENode(0,false,true,false): closed, e=0
@@ -72,6 +90,7 @@ public:
     std::vector<std::vector<TiltNode>> tilt;  // tiltE + tilt_layout bits per node
     std::vector<lcoor_t> half;                // halfGhostIdx order
     std::vector<lcoor_t> closed_by_half;      // closedByHalfTiltArr order
+    BoundarySdf sdf;                          // collider SDF (empty: none)
 
     // Builds layers, ghosts, T-joint / half-tilt state and the initData
     // boundary fix-ups from a split map (R^3 ints, x fastest).
@@ -98,8 +117,9 @@ private:
     void initFixups();
 };
 
-// Unknown numbering (reference updateIndexParallel, octree_omnigrid_solver3.cpp:292-347)
-// with no collider (boundary SDF > -3h everywhere).
+// Unknown numbering (reference updateIndexParallel, octree_omnigrid_solver3.cpp:292-347):
+// a cell (octagon), open tilt or half-tilt is an unknown only where the
+// boundary SDF at its position exceeds -3 h_l (always, with no collider).
 struct Numbering {
     std::vector<std::vector<int32_t>> oct, tilt;  // per level, per node
     int n_oct_tilt = 0;  // rows before the half-tilt block

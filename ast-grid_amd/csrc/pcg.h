@@ -279,6 +279,7 @@ private:
     bool coop_ = false;                     // sweeps through the cooperative launch API (ASTG_COOP=1)
     bool spinFailed_ = false;               // a sweep hit its spin limit since the last solve started
     bool forceSpinError_ = false;           // test hook (ASTG_TEST_SPIN_ERROR_ONCE): one forced spin error
+    bool f32Valid_ = false;                 // FP32 copies match the current factor (convert32 since factor)
 };
 
 }  // namespace astg

@@ -338,7 +338,8 @@ template <typename T>
 __global__ void __launch_bounds__(BS) k_fwd(SweepPlan S, StencilView<T> St, const T* invd, const T* r, T* y, T* zp,
                                             Parts Pt, int* cnt, double* scal, int mode, int it, int maxIt, int norm,
                                             double tolRel) {
-    if (scal[S_DONE] != 0.0) return;
+    // a sweep of this solve already failed: the host retries the whole solve
+    if (scal[S_DONE] != 0.0 || ld_agent_i32(&cnt[C_ERR]) != 0) return;
     if (mode != 2 && fwd_prologue(Pt, scal, mode, it, maxIt, norm, tolRel)) return;
     for (int c = blockIdx.x; c < S.nchunks; c += gridDim.x) {
         const int t = c * BS + threadIdx.x;
@@ -359,7 +360,7 @@ __global__ void __launch_bounds__(BS) k_fwd(SweepPlan S, StencilView<T> St, cons
         gate_row(y, S.gateF[t >> 6], S.gateCap);
         const long long tg = S.trace ? wall_clock64() : 0;
         if (i >= 0) {
-            const bool ok = d.run(y, acc, S.pollCap, [&](T a) {
+            const bool ok = d.run(y, acc, S.pollCap, &cnt[C_ERR], [&](T a) {
                 const long long td = S.trace ? wall_clock64() : 0;
                 Sentinel<T>::store(&y[t], a * inv);
                 if (S.trace) {
@@ -389,7 +390,7 @@ template <typename T>
 __global__ void __launch_bounds__(BS) k_bwd_dot(SweepPlan S, StencilView<T> St, const T* invd, const T* y, T* ynext,
                                                 const T* r, T* z, T* zp, Parts Pt, int* cnt, const double* scal,
                                                 int parityOut) {
-    if (scal[S_DONE] != 0.0) return;
+    if (scal[S_DONE] != 0.0 || ld_agent_i32(&cnt[C_ERR]) != 0) return;
     double v = 0.0;
     for (int c = blockIdx.x; c < S.nchunks; c += gridDim.x) {
         const int pos = S.P - 1 - (c * BS + threadIdx.x);
@@ -409,7 +410,7 @@ __global__ void __launch_bounds__(BS) k_bwd_dot(SweepPlan S, StencilView<T> St, 
         const long long tg = S.trace ? wall_clock64() : 0;
         if (k >= 0) {
             T zk = 0;
-            const bool ok = d.run(zp, acc, S.pollCap, [&](T a) {
+            const bool ok = d.run(zp, acc, S.pollCap, &cnt[C_ERR], [&](T a) {
                 const long long td = S.trace ? wall_clock64() : 0;
                 zk = a * inv;
                 Sentinel<T>::store(&zp[pos], zk);
@@ -766,6 +767,7 @@ static int splitPatternHost(int n, const int* rowstart, const int* col, std::vec
 }
 
 int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::string& err) {
+    f32Valid_ = false;
     if (n <= 0) {
         err = "empty system";
         return ASTG_ERR_ARG;
@@ -1150,10 +1152,12 @@ int PcgDevice::convert32(std::string& err) {
     k_d2f<<<blocks(stF_.E), BS, 0, stream_>>>(stF_.coef.as<double>(), stF_.coef32.as<float>(), stF_.E);
     k_d2f<<<blocks(stB_.E), BS, 0, stream_>>>(stB_.coef.as<double>(), stB_.coef32.as<float>(), stB_.E);
     HIPCHK(hipGetLastError());
+    f32Valid_ = true;
     return ASTG_OK;
 }
 
 int PcgDevice::factor(double tau, double ratio, std::string& err) {
+    f32Valid_ = false;  // new values and pivots: the FP32 copies are stale
     int rc = launchMic(tau, ratio, err);
     if (rc) return rc;
     return launchMerge(err);
@@ -1211,6 +1215,10 @@ int PcgDevice::pcgInner(const T* A, const T* invd, const T* b, double* x, double
         forceSpinError_ = false;
         k_set_err<<<1, 64, 0, stream_>>>(cnt_.as<int>(), E_FWD);
     }
+    // the first sweep pair shows whether every sweep block was resident: a
+    // failure is caught here, before any iteration is spent on it
+    rc = checkErrors("PCG", err);
+    if (rc) return rc;
     // z as the PCG reads it: the backward sweep's published copy in sweep
     // order (valid until the next forward sweep re-arms it)
     const T* zv = perm_ ? zp_.as<T>() : z;
@@ -1376,7 +1384,7 @@ int PcgDevice::benchKernel(int which, int reps, double& avgMs, double& bytes, st
     const int* col = sCol();
     double* scal = scal_.as<double>();
     const bool f32 = which == ASTG_K_SPMV32 || which == ASTG_K_FWD32 || which == ASTG_K_BWD32;
-    if (f32 && (!val32_.p || stF_.coef32.bytes < (size_t)stF_.E * 4)) {
+    if (f32 && !f32Valid_) {
         int rc = convert32(err);
         if (rc) return rc;
     }

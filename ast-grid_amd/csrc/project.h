@@ -7,10 +7,14 @@
 #include <string>
 #include <utility>
 
+#include "consts.h"
 #include "pcg.h"
 #include "project_host.h"
 
 namespace astg {
+
+struct AsmPasses;
+struct VelPasses;
 
 class ProjectDevice {
 public:
@@ -33,9 +37,18 @@ public:
     }
     double* uMac() { return u_mac_.as<double>(); }    // 3*G, axis-major
     double* uTilt() { return u_tilt_.as<double>(); }  // 8*G, slot-major
+    // The assembly's b part and the velocity correction as linear passes
+    // (project.hip): their bodies with this context's arrays.
+    void asmPasses(double* A, double* b, const int* rs, double rho, double dt, AsmPasses& ps);
+    void velPasses(const double* p, double rho, double dt, VelPasses& ps);
 
 private:
+    // level spacings in the kernels' constant table: uploaded by upload(),
+    // again whenever another context used the table in between
+    int bindConstants(bool force, std::string& err);
     hipStream_t stream_;
+    uint64_t constOwner_ = newConstOwner();
+    double hConst_[16] = {};
     const ProjectTables* T_ = nullptr;
     DevBuf level_, tilt_e_, tilt_bits_, tilt_hd_, pid_tilt_, w_mac_, w_tilt_;
     DevBuf oct_row_, oct_corner_, oct_face_, oct_pos_;

@@ -9,6 +9,7 @@
 #include <cstring>
 
 #include "device_common.h"
+#include "linrec.h"
 #include "project.h"
 
 #define HIPCHK(expr)                                                                 \
@@ -38,16 +39,23 @@ inline int blocks(long n) { return (int)((n + BS - 1) / BS); }
 // ---------------------------------------------------------------------------
 // octagon rows (project 367-477)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(BS) k_asm_oct(int no, const int* __restrict__ row, const int* __restrict__ corner,
-                                                const int* __restrict__ face, const uint8_t* __restrict__ pos,
-                                                const int* __restrict__ rs, const double* __restrict__ tilt_e,
-                                                const uint8_t* __restrict__ tilt_bits, const int* __restrict__ pid_tilt,
-                                                const double* __restrict__ w_mac, const double* __restrict__ w_tilt,
-                                                const double* __restrict__ u_mac, const double* __restrict__ u_tilt,
-                                                const uint8_t* __restrict__ level, long G, double* __restrict__ A,
-                                                double* __restrict__ b) {
-    const int j = blockIdx.x * BS + threadIdx.x;
-    if (j >= no) return;
+// The assembly and velocity-correction kernels are bodies run by k_run
+// (forward) or recorded for their transposes (linrec.h): velocities, b and p
+// go through the policy P; matrix values are written only by the forward.
+struct BAsmOct {
+    int no;
+    const int *row, *corner, *face;
+    const uint8_t* pos;
+    const int* rs;
+    const double* tilt_e;
+    const uint8_t* tilt_bits;
+    const int* pid_tilt;
+    const double *w_mac, *w_tilt, *u_mac, *u_tilt;
+    const uint8_t* level;
+    long G;
+    double *A, *b;
+    template <class Pol>
+    __device__ void operator()(int j, Pol& P) const {
     const int r = row[j];
     int cg[8];
     double e[8];
@@ -59,7 +67,8 @@ __global__ void __launch_bounds__(BS) k_asm_oct(int no, const int* __restrict__ 
     const int g = cg[6];
     const double h = c_h[level[g]];
     const int base = rs[r], len = rs[r + 1] - base;
-    for (int p = 0; p < len; ++p) A[base + p] = 0.0;
+    if (Pol::kAux)
+        for (int p = 0; p < len; ++p) A[base + p] = 0.0;
     double center = 0.0, tb = 0.0;
 #pragma unroll
     for (int f = 0; f < 6; ++f) {
@@ -78,7 +87,7 @@ __global__ void __launch_bounds__(BS) k_asm_oct(int no, const int* __restrict__ 
             const double term = w_mac[axis * G + fn];
             tCenter = term * area / h;
             tCoeff = area * term;
-            A[base + pos[(size_t)f * no + j]] = -tCenter;
+            if (Pol::kAux) A[base + pos[(size_t)f * no + j]] = -tCenter;
         } else if (code == FACE_EXCLUDED || code == FACE_CLOSED) {
             tCenter = 0.0;
             tCoeff = 0.0;
@@ -87,7 +96,7 @@ __global__ void __launch_bounds__(BS) k_asm_oct(int no, const int* __restrict__ 
             tCenter = tCoeff / h;
         }
         center += tCenter;
-        tb += ((f & 1) ? 1 : -1) * tCoeff * u_mac[axis * G + fn];
+        tb += ((f & 1) ? 1 : -1) * tCoeff * P.ld(u_mac, axis * G + fn);
     }
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
@@ -102,34 +111,39 @@ __global__ void __launch_bounds__(BS) k_asm_oct(int no, const int* __restrict__ 
         if (t >= 0) {
             tCenter = term * area / (h * kSqrt3 * 0.5);
             tCoeff = area * term;
-            A[base + pos[(size_t)(6 + k) * no + j]] += -tCenter;
+            if (Pol::kAux) A[base + pos[(size_t)(6 + k) * no + j]] += -tCenter;
         }
         center += tCenter;
-        tb += (k < 4 ? -1 : 1) * tCoeff * u_tilt[re * G + gk];
+        tb += (k < 4 ? -1 : 1) * tCoeff * P.ld(u_tilt, re * G + gk);
     }
     if (fabs(center) < kEps) {
         center = 1.0;
         tb = 0.0;
     }
-    A[base + pos[(size_t)14 * no + j]] = center;
-    b[r] = tb;
-}
+    if (Pol::kAux) A[base + pos[(size_t)14 * no + j]] = center;
+    P.st(b, r, tb);
+    }
+};
 
 // ---------------------------------------------------------------------------
 // tilt rows (project 479-552)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(BS) k_asm_tilt(int nt, const int* __restrict__ row, const int* __restrict__ node,
-                                                 const int* __restrict__ nb, const uint8_t* __restrict__ nblev,
-                                                 const uint8_t* __restrict__ pos, const int* __restrict__ rs,
-                                                 const double* __restrict__ tilt_e, const double* __restrict__ w_tilt,
-                                                 const double* __restrict__ u_tilt, const uint8_t* __restrict__ level,
-                                                 long G, double* __restrict__ A, double* __restrict__ b) {
-    const int j = blockIdx.x * BS + threadIdx.x;
-    if (j >= nt) return;
+struct BAsmTilt {
+    int nt;
+    const int *row, *node, *nb;
+    const uint8_t *nblev, *pos;
+    const int* rs;
+    const double *tilt_e, *w_tilt, *u_tilt;
+    const uint8_t* level;
+    long G;
+    double *A, *b;
+    template <class Pol>
+    __device__ void operator()(int j, Pol& P) const {
     const int r = row[j], g = node[j];
     const double h = c_h[level[g]];
     const int base = rs[r], len = rs[r + 1] - base;
-    for (int p = 0; p < len; ++p) A[base + p] = 0.0;
+    if (Pol::kAux)
+        for (int p = 0; p < len; ++p) A[base + p] = 0.0;
     const double e = tilt_e[g];
     const double area = e * e * kSqrt3 * 0.5;
     double center = 0.0, tb = 0.0;
@@ -146,67 +160,75 @@ __global__ void __launch_bounds__(BS) k_asm_tilt(int nt, const int* __restrict__
             const double hn = c_h[nblev[(size_t)k * nt + j]];
             tCenter = tw * area / (hn * kSqrt3 * 0.5);
             tCoeff = area * tw;
-            A[base + pos[(size_t)k * nt + j]] = -tCenter;
+            if (Pol::kAux) A[base + pos[(size_t)k * nt + j]] = -tCenter;
         } else {
             tCenter = 0.0;
             tCoeff = 0.0;
         }
         center += tCenter;
-        tb += (k < 4 ? -1 : 1) * tCoeff * u_tilt[k * G + g];
+        tb += (k < 4 ? -1 : 1) * tCoeff * P.ld(u_tilt, k * G + g);
     }
     if (fabs(center) < kEps) center = 1.0;
-    A[base + pos[(size_t)8 * nt + j]] = center;
-    b[r] = tb;
-}
+    if (Pol::kAux) A[base + pos[(size_t)8 * nt + j]] = center;
+    P.st(b, r, tb);
+    }
+};
 
 // ---------------------------------------------------------------------------
 // half-tilt face terms added into coarse octagon rows (project 572-581), in
 // halfGhostIdx order per target row
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(BS) k_asm_half_coarse(int nrows, const int* __restrict__ hrow,
-                                                        const int* __restrict__ hstart, const int* __restrict__ hq,
-                                                        const uint8_t* __restrict__ pdiag,
-                                                        const uint8_t* __restrict__ poff, const int* __restrict__ hnode,
-                                                        const int* __restrict__ rs, const double* __restrict__ tilt_e,
-                                                        const int8_t* __restrict__ tilt_hd,
-                                                        const double* __restrict__ w_tilt,
-                                                        const double* __restrict__ u_tilt,
-                                                        const uint8_t* __restrict__ level, long G,
-                                                        double* __restrict__ A, double* __restrict__ b) {
-    const int j = blockIdx.x * BS + threadIdx.x;
-    if (j >= nrows) return;
+struct BAsmHalfCoarse {
+    int nrows;
+    const int *hrow, *hstart, *hq;
+    const uint8_t *pdiag, *poff;
+    const int *hnode, *rs;
+    const double* tilt_e;
+    const int8_t* tilt_hd;
+    const double *w_tilt, *u_tilt;
+    const uint8_t* level;
+    long G;
+    double *A, *b;
+    template <class Pol>
+    __device__ void operator()(int j, Pol& P) const {
     const int r = hrow[j], base = rs[r];
-    double br = b[r];
+    double br = P.ld(b, r);
     for (int idx = hstart[j]; idx < hstart[j + 1]; ++idx) {
         const int q = hq[idx], g = hnode[q], hd = tilt_hd[g];
         const double h = c_h[level[g]];
         const double e = tilt_e[g];
         const double area = e * e * 2;
         const int os = c_halfSlot[hd][4];
-        const double w = w_tilt[os * G + g], u = u_tilt[os * G + g];
-        A[base + pdiag[idx]] += w * area / h;
-        A[base + poff[idx]] += -w * area / h;
+        const double w = w_tilt[os * G + g], u = P.ld(u_tilt, os * G + g);
+        if (Pol::kAux) {
+            A[base + pdiag[idx]] += w * area / h;
+            A[base + poff[idx]] += -w * area / h;
+        }
         br += (hd < 3 ? -1 : 1) * u * w * area;
     }
-    b[r] = br;
-}
+    P.st(b, r, br);
+    }
+};
 
 // half-tilt rows (project 556-649), occurrences in halfGhostIdx order
-__global__ void __launch_bounds__(BS) k_asm_half_rows(int nrows, const int* __restrict__ hrow,
-                                                      const int* __restrict__ hstart, const int* __restrict__ hq,
-                                                      const uint8_t* __restrict__ hpos, const int* __restrict__ hnode,
-                                                      const int* __restrict__ ht, const uint8_t* __restrict__ hclosed,
-                                                      const int* __restrict__ hnb, const int* __restrict__ rs,
-                                                      const double* __restrict__ tilt_e,
-                                                      const int8_t* __restrict__ tilt_hd,
-                                                      const double* __restrict__ w_tilt,
-                                                      const double* __restrict__ u_tilt,
-                                                      const uint8_t* __restrict__ level, long G,
-                                                      double* __restrict__ A, double* __restrict__ b) {
-    const int j = blockIdx.x * BS + threadIdx.x;
-    if (j >= nrows) return;
+struct BAsmHalfRows {
+    int nrows;
+    const int *hrow, *hstart, *hq;
+    const uint8_t* hpos;
+    const int *hnode, *ht;
+    const uint8_t* hclosed;
+    const int *hnb, *rs;
+    const double* tilt_e;
+    const int8_t* tilt_hd;
+    const double *w_tilt, *u_tilt;
+    const uint8_t* level;
+    long G;
+    double *A, *b;
+    template <class Pol>
+    __device__ void operator()(int j, Pol& P) const {
     const int c = hrow[j], base = rs[c], len = rs[c + 1] - base;
-    for (int p = 0; p < len; ++p) A[base + p] = 0.0;
+    if (Pol::kAux)
+        for (int p = 0; p < len; ++p) A[base + p] = 0.0;
     double bc = 0.0;
     for (int idx = hstart[j]; idx < hstart[j + 1]; ++idx) {
         const int q = hq[idx], g = hnode[q], hd = tilt_hd[g];
@@ -215,13 +237,13 @@ __global__ void __launch_bounds__(BS) k_asm_half_rows(int nrows, const int* __re
         const double e = tilt_e[g];
         double area = e * e * 2;
         const int os = c_halfSlot[hd][4];
-        const double wf = w_tilt[os * G + g], uf = u_tilt[os * G + g];
+        const double wf = w_tilt[os * G + g], uf = P.ld(u_tilt, os * G + g);
         double center = 0.0, tb = 0.0;
         const int t = ht[q];
         if (t != -20) {
             if (t >= 0) {
                 center += wf * area / h;
-                A[base + pp[0]] += -wf * area / h;
+                if (Pol::kAux) A[base + pp[0]] += -wf * area / h;
                 tb += (hd < 3 ? 1 : -1) * uf * wf * area;
             }
         } else {
@@ -238,108 +260,139 @@ __global__ void __launch_bounds__(BS) k_asm_half_rows(int nrows, const int* __re
             if (code >= 0) {
                 tCenter = area * tw / (h * kSqrt3 * 0.5);
                 tCoeff = area * tw;
-                A[base + pp[1 + m]] = -tCenter;
+                if (Pol::kAux) A[base + pp[1 + m]] = -tCenter;
             }
             center += tCenter;
-            tb += (k < 4 ? -1 : 1) * tCoeff * u_tilt[k * G + g];
+            tb += (k < 4 ? -1 : 1) * tCoeff * P.ld(u_tilt, k * G + g);
         }
-        A[base + pp[5]] += center;
+        if (Pol::kAux) A[base + pp[5]] += center;
         bc = tb;
     }
-    b[c] = bc;
-}
+    P.st(b, c, bc);
+    }
+};
 
-__global__ void k_scale(int n, double* b, double s) {
-    const int i = blockIdx.x * BS + threadIdx.x;
-    if (i < n) b[i] *= s;
-}
+struct BScale {
+    double* b;
+    double s;
+    template <class Pol>
+    __device__ void operator()(int i, Pol& P) const { P.st(b, i, P.ld(b, i) * s); }
+};
 
 // ---------------------------------------------------------------------------
 // velocity correction (updateVelocityByPressure 681-847)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(BS) k_vel_mac(int nvm, const int* __restrict__ node, const int* __restrict__ cidx,
-                                                const int* __restrict__ nb, const double* __restrict__ w_mac,
-                                                const double* __restrict__ p, const uint8_t* __restrict__ level,
-                                                long G, double dt, double rho, double* __restrict__ u_mac) {
-    const int j = blockIdx.x * BS + threadIdx.x;
-    if (j >= nvm) return;
-    const int g = node[j], c = cidx[j];
-    const double h = c_h[level[g]];
-    const double p0 = c >= 0 ? p[c] : 0.0;
+struct BVelMac {
+    int nvm;
+    const int *node, *cidx, *nb;
+    const double *w_mac, *p;
+    const uint8_t* level;
+    long G;
+    double dt, rho;
+    double* u_mac;
+    template <class Pol>
+    __device__ void operator()(int j, Pol& P) const {
+        const int g = node[j], c = cidx[j];
+        const double h = c_h[level[g]];
+        const double p0 = c >= 0 ? P.ld(p, c) : 0.0;
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        const int code = nb[(size_t)a * nvm + j];
-        if (code == VNB_SKIP) continue;
-        double& u = u_mac[a * G + g];
-        if (w_mac[a * G + g] <= 0) {
-            u = 0.0;
-        } else {
-            const double p1 = code >= 0 ? p[code] : 0.0;
-            u = u - dt / rho * (p0 - p1) / h;
-        }
-    }
-}
-
-__global__ void __launch_bounds__(BS) k_vel_tilt(int nvt, const int* __restrict__ node, const int* __restrict__ cidx,
-                                                 const int* __restrict__ nb, const uint8_t* __restrict__ lev,
-                                                 const double* __restrict__ w_tilt, const double* __restrict__ p, long G,
-                                                 double dt, double rho, double* __restrict__ u_tilt) {
-    const int j = blockIdx.x * BS + threadIdx.x;
-    if (j >= nvt) return;
-    const int g = node[j], c = cidx[j];
-    const double p0 = c >= 0 ? p[c] : 0.0;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const int code = nb[(size_t)k * nvt + j];
-        double& u = u_tilt[k * G + g];
-        if (w_tilt[k * G + g] > 0 && code != VNB_ZERO) {
-            const double hn = c_h[lev[(size_t)k * nvt + j]];
-            const double p1 = code >= 0 ? p[code] : 0.0;
-            u = u - dt / rho * (k < 4 ? 1 : -1) / (hn * kSqrt3 * 0.5) * (p1 - p0);
-        } else {
-            u = 0.0;
-        }
-    }
-}
-
-__global__ void __launch_bounds__(BS) k_vel_half(int nh, const int* __restrict__ hnode, const int* __restrict__ hc,
-                                                 const int* __restrict__ ht, const int* __restrict__ hnb,
-                                                 const int8_t* __restrict__ tilt_hd, const double* __restrict__ w_tilt,
-                                                 const double* __restrict__ p, const uint8_t* __restrict__ level,
-                                                 long G, double dt, double rho, double* __restrict__ u_tilt) {
-    const int q = blockIdx.x * BS + threadIdx.x;
-    if (q >= nh) return;
-    const int g = hnode[q], c = hc[q], hd = tilt_hd[g];
-    const double h = c_h[level[g]];
-    const double p0 = c >= 0 ? p[c] : 0.0;
-    const int fs = c_halfSlot[hd][4];
-    {
-        double& u = u_tilt[fs * G + g];
-        if (w_tilt[fs * G + g] > 0) {
-            const int t = ht[q];
-            const double p1 = t >= 0 ? p[t] : 0.0;
-            u -= dt / rho * (hd < 3 ? 1 : -1) / h * (p0 - p1);
-        } else {
-            u = 0.0;
-        }
-    }
-    int m = 0;
-    for (int k = 0; k < 8; ++k) {
-        if (k == fs) continue;
-        double& u = u_tilt[k * G + g];
-        const bool slot = (m < 4 && c_halfSlot[hd][m] == k);
-        if (slot) {
-            const int code = hnb[(size_t)4 * q + m];
-            ++m;
-            if (w_tilt[k * G + g] > 0) {
-                const double p1 = code >= 0 ? p[code] : 0.0;
-                u = u - dt / rho * (k < 4 ? 1 : -1) / (h * kSqrt3 * 0.5) * (p1 - p0);
-                continue;
+        for (int a = 0; a < 3; ++a) {
+            const int code = nb[(size_t)a * nvm + j];
+            if (code == VNB_SKIP) continue;
+            const size_t ui = a * G + g;
+            if (w_mac[a * G + g] <= 0) {
+                P.st(u_mac, ui, 0.0);
+            } else {
+                const double p1 = code >= 0 ? P.ld(p, code) : 0.0;
+                P.st(u_mac, ui, P.ld(u_mac, ui) - dt / rho * (p0 - p1) / h);
             }
         }
-        u = 0.0;
     }
-}
+};
+
+struct BVelTilt {
+    int nvt;
+    const int *node, *cidx, *nb;
+    const uint8_t* lev;
+    const double *w_tilt, *p;
+    long G;
+    double dt, rho;
+    double* u_tilt;
+    template <class Pol>
+    __device__ void operator()(int j, Pol& P) const {
+        const int g = node[j], c = cidx[j];
+        const double p0 = c >= 0 ? P.ld(p, c) : 0.0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int code = nb[(size_t)k * nvt + j];
+            const size_t ui = k * G + g;
+            if (w_tilt[k * G + g] > 0 && code != VNB_ZERO) {
+                const double hn = c_h[lev[(size_t)k * nvt + j]];
+                const double p1 = code >= 0 ? P.ld(p, code) : 0.0;
+                P.st(u_tilt, ui, P.ld(u_tilt, ui) - dt / rho * (k < 4 ? 1 : -1) / (hn * kSqrt3 * 0.5) * (p1 - p0));
+            } else {
+                P.st(u_tilt, ui, 0.0);
+            }
+        }
+    }
+};
+
+struct BVelHalf {
+    int nh;
+    const int *hnode, *hc, *ht, *hnb;
+    const int8_t* tilt_hd;
+    const double *w_tilt, *p;
+    const uint8_t* level;
+    long G;
+    double dt, rho;
+    double* u_tilt;
+    template <class Pol>
+    __device__ void operator()(int q, Pol& P) const {
+        const int g = hnode[q], c = hc[q], hd = tilt_hd[g];
+        const double h = c_h[level[g]];
+        const double p0 = c >= 0 ? P.ld(p, c) : 0.0;
+        const int fs = c_halfSlot[hd][4];
+        {
+            const size_t ui = fs * G + g;
+            if (w_tilt[fs * G + g] > 0) {
+                const int t = ht[q];
+                const double p1 = t >= 0 ? P.ld(p, t) : 0.0;
+                P.st(u_tilt, ui, P.ld(u_tilt, ui) - dt / rho * (hd < 3 ? 1 : -1) / h * (p0 - p1));
+            } else {
+                P.st(u_tilt, ui, 0.0);
+            }
+        }
+        int m = 0;
+        for (int k = 0; k < 8; ++k) {
+            if (k == fs) continue;
+            const size_t ui = k * G + g;
+            const bool slot = (m < 4 && c_halfSlot[hd][m] == k);
+            if (slot) {
+                const int code = hnb[(size_t)4 * q + m];
+                ++m;
+                if (w_tilt[k * G + g] > 0) {
+                    const double p1 = code >= 0 ? P.ld(p, code) : 0.0;
+                    P.st(u_tilt, ui, P.ld(u_tilt, ui) - dt / rho * (k < 4 ? 1 : -1) / (h * kSqrt3 * 0.5) * (p1 - p0));
+                    continue;
+                }
+            }
+            P.st(u_tilt, ui, 0.0);
+        }
+    }
+};
+
+struct AsmPasses {
+    BAsmOct oct;
+    BAsmTilt tilt;
+    BAsmHalfCoarse hc;
+    BAsmHalfRows hr;
+    BScale scale;
+};
+struct VelPasses {
+    BVelMac mac;
+    BVelTilt tilt;
+    BVelHalf half;
+};
 
 // ---------------------------------------------------------------------------
 // host driver
@@ -420,7 +473,9 @@ int ProjectDevice::upload(const ProjectTables& t, const Octree& o, std::string& 
             if (kOctNbTilt[k][0] == kOctNbFace[f][0] && kOctNbTilt[k][1] == kOctNbFace[f][1] &&
                 kOctNbTilt[k][2] == kOctNbFace[f][2])
                 faceNode[f] = k;
-    HIPCHK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_h), h, sizeof(h), 0, hipMemcpyHostToDevice, stream_));
+    std::memcpy(hConst_, h, sizeof(h));
+    std::lock_guard<std::recursive_mutex> lk(constMutex());
+    if ((rc = bindConstants(true, err))) return rc;
     HIPCHK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_faceCorner), faceCorner, sizeof(faceCorner), 0, hipMemcpyHostToDevice,
                                   stream_));
     HIPCHK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_faceNode), faceNode, sizeof(faceNode), 0, hipMemcpyHostToDevice, stream_));
@@ -430,56 +485,73 @@ int ProjectDevice::upload(const ProjectTables& t, const Octree& o, std::string& 
     return ASTG_OK;
 }
 
+int ProjectDevice::bindConstants(bool force, std::string& err) {
+    if (!takeConstants(CT_PROJECT, constOwner_, force)) return ASTG_OK;
+    HIPCHK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_h), hConst_, sizeof(hConst_), 0, hipMemcpyHostToDevice, stream_));
+    return ASTG_OK;
+}
+
 int ProjectDevice::assemble(double* A, double* b, const int* rs, double rho, double dt, std::string& err) {
+    std::lock_guard<std::recursive_mutex> lk(constMutex());
+    if (int rc = bindConstants(false, err)) return rc;
     const ProjectTables& t = *T_;
     const long G = t.G;
-    if (t.n_oct)
-        k_asm_oct<<<blocks(t.n_oct), BS, 0, stream_>>>(
-            t.n_oct, oct_row_.as<int>(), oct_corner_.as<int>(), oct_face_.as<int>(), oct_pos_.as<uint8_t>(), rs,
-            tilt_e_.as<double>(), tilt_bits_.as<uint8_t>(), pid_tilt_.as<int>(), w_mac_.as<double>(),
-            w_tilt_.as<double>(), u_mac_.as<double>(), u_tilt_.as<double>(), level_.as<uint8_t>(), G, A, b);
-    if (t.n_tilt)
-        k_asm_tilt<<<blocks(t.n_tilt), BS, 0, stream_>>>(t.n_tilt, tilt_row_.as<int>(), tilt_node_.as<int>(),
-                                                         tilt_nb_.as<int>(), tilt_nblev_.as<uint8_t>(),
-                                                         tilt_pos_.as<uint8_t>(), rs, tilt_e_.as<double>(),
-                                                         w_tilt_.as<double>(), u_tilt_.as<double>(),
-                                                         level_.as<uint8_t>(), G, A, b);
-    const int nhc = (int)t.hc_row.size();
-    if (nhc)
-        k_asm_half_coarse<<<blocks(nhc), BS, 0, stream_>>>(
-            nhc, hc_row_.as<int>(), hc_start_.as<int>(), hc_q_.as<int>(), hc_pos_diag_.as<uint8_t>(),
-            hc_pos_off_.as<uint8_t>(), half_node_.as<int>(), rs, tilt_e_.as<double>(), tilt_hd_.as<int8_t>(),
-            w_tilt_.as<double>(), u_tilt_.as<double>(), level_.as<uint8_t>(), G, A, b);
-    if (t.n_half_rows)
-        k_asm_half_rows<<<blocks(t.n_half_rows), BS, 0, stream_>>>(
-            t.n_half_rows, hr_row_.as<int>(), hr_start_.as<int>(), hr_q_.as<int>(), hr_pos_.as<uint8_t>(),
-            half_node_.as<int>(), half_t_.as<int>(), half_closed_.as<uint8_t>(), half_nb_.as<int>(), rs,
-            tilt_e_.as<double>(), tilt_hd_.as<int8_t>(), w_tilt_.as<double>(), u_tilt_.as<double>(),
-            level_.as<uint8_t>(), G, A, b);
-    k_scale<<<blocks(t.n), BS, 0, stream_>>>(t.n, b, rho / dt);
+    AsmPasses ps;
+    asmPasses(A, b, rs, rho, dt, ps);
+    if (ps.oct.no) k_run<<<blocks(ps.oct.no), BS, 0, stream_>>>(ps.oct, ps.oct.no);
+    if (ps.tilt.nt) k_run<<<blocks(ps.tilt.nt), BS, 0, stream_>>>(ps.tilt, ps.tilt.nt);
+    if (ps.hc.nrows) k_run<<<blocks(ps.hc.nrows), BS, 0, stream_>>>(ps.hc, ps.hc.nrows);
+    if (ps.hr.nrows) k_run<<<blocks(ps.hr.nrows), BS, 0, stream_>>>(ps.hr, ps.hr.nrows);
+    k_run<<<blocks(t.n), BS, 0, stream_>>>(ps.scale, t.n);
     HIPCHK(hipGetLastError());
     return ASTG_OK;
 }
 
-int ProjectDevice::correct(const double* p, double rho, double dt, std::string& err) {
+void ProjectDevice::asmPasses(double* A, double* b, const int* rs, double rho, double dt, AsmPasses& ps) {
     const ProjectTables& t = *T_;
     const long G = t.G;
-    const int nvm = (int)t.vm_node.size(), nvt = (int)t.vt_node.size();
-    if (nvm)
-        k_vel_mac<<<blocks(nvm), BS, 0, stream_>>>(nvm, vm_node_.as<int>(), vm_c_.as<int>(), vm_nb_.as<int>(),
-                                                   w_mac_.as<double>(), p, level_.as<uint8_t>(), G, dt, rho,
-                                                   u_mac_.as<double>());
-    if (nvt)
-        k_vel_tilt<<<blocks(nvt), BS, 0, stream_>>>(nvt, vt_node_.as<int>(), vt_c_.as<int>(), vt_nb_.as<int>(),
-                                                    vt_lev_.as<uint8_t>(), w_tilt_.as<double>(), p, G, dt, rho,
-                                                    u_tilt_.as<double>());
-    if (t.nh)
-        k_vel_half<<<blocks(t.nh), BS, 0, stream_>>>(t.nh, half_node_.as<int>(), half_c_.as<int>(),
-                                                     half_t_.as<int>(), half_nb_.as<int>(), tilt_hd_.as<int8_t>(),
-                                                     w_tilt_.as<double>(), p, level_.as<uint8_t>(), G, dt, rho,
-                                                     u_tilt_.as<double>());
+    ps.oct = BAsmOct{t.n_oct, oct_row_.as<int>(), oct_corner_.as<int>(), oct_face_.as<int>(), oct_pos_.as<uint8_t>(),
+                     rs, tilt_e_.as<double>(), tilt_bits_.as<uint8_t>(), pid_tilt_.as<int>(), w_mac_.as<double>(),
+                     w_tilt_.as<double>(), u_mac_.as<double>(), u_tilt_.as<double>(), level_.as<uint8_t>(), G, A, b};
+    ps.tilt = BAsmTilt{t.n_tilt, tilt_row_.as<int>(), tilt_node_.as<int>(), tilt_nb_.as<int>(),
+                       tilt_nblev_.as<uint8_t>(), tilt_pos_.as<uint8_t>(), rs, tilt_e_.as<double>(),
+                       w_tilt_.as<double>(), u_tilt_.as<double>(), level_.as<uint8_t>(), G, A, b};
+    ps.hc = BAsmHalfCoarse{(int)t.hc_row.size(), hc_row_.as<int>(), hc_start_.as<int>(), hc_q_.as<int>(),
+                           hc_pos_diag_.as<uint8_t>(), hc_pos_off_.as<uint8_t>(), half_node_.as<int>(), rs,
+                           tilt_e_.as<double>(), tilt_hd_.as<int8_t>(), w_tilt_.as<double>(), u_tilt_.as<double>(),
+                           level_.as<uint8_t>(), G, A, b};
+    ps.hr = BAsmHalfRows{t.n_half_rows, hr_row_.as<int>(), hr_start_.as<int>(), hr_q_.as<int>(), hr_pos_.as<uint8_t>(),
+                         half_node_.as<int>(), half_t_.as<int>(), half_closed_.as<uint8_t>(), half_nb_.as<int>(), rs,
+                         tilt_e_.as<double>(), tilt_hd_.as<int8_t>(), w_tilt_.as<double>(), u_tilt_.as<double>(),
+                         level_.as<uint8_t>(), G, A, b};
+    ps.scale = BScale{b, rho / dt};
+}
+
+int ProjectDevice::correct(const double* p, double rho, double dt, std::string& err) {
+    std::lock_guard<std::recursive_mutex> lk(constMutex());
+    if (int rc = bindConstants(false, err)) return rc;
+    const ProjectTables& t = *T_;
+    const long G = t.G;
+    (void)G;
+    VelPasses ps;
+    velPasses(p, rho, dt, ps);
+    if (ps.mac.nvm) k_run<<<blocks(ps.mac.nvm), BS, 0, stream_>>>(ps.mac, ps.mac.nvm);
+    if (ps.tilt.nvt) k_run<<<blocks(ps.tilt.nvt), BS, 0, stream_>>>(ps.tilt, ps.tilt.nvt);
+    if (ps.half.nh) k_run<<<blocks(ps.half.nh), BS, 0, stream_>>>(ps.half, ps.half.nh);
     HIPCHK(hipGetLastError());
     return ASTG_OK;
+}
+
+void ProjectDevice::velPasses(const double* p, double rho, double dt, VelPasses& ps) {
+    const ProjectTables& t = *T_;
+    const long G = t.G;
+    ps.mac = BVelMac{(int)t.vm_node.size(), vm_node_.as<int>(), vm_c_.as<int>(), vm_nb_.as<int>(), w_mac_.as<double>(),
+                     p, level_.as<uint8_t>(), G, dt, rho, u_mac_.as<double>()};
+    ps.tilt = BVelTilt{(int)t.vt_node.size(), vt_node_.as<int>(), vt_c_.as<int>(), vt_nb_.as<int>(),
+                       vt_lev_.as<uint8_t>(), w_tilt_.as<double>(), p, G, dt, rho, u_tilt_.as<double>()};
+    ps.half = BVelHalf{t.nh, half_node_.as<int>(), half_c_.as<int>(), half_t_.as<int>(), half_nb_.as<int>(),
+                       tilt_hd_.as<int8_t>(), w_tilt_.as<double>(), p, level_.as<uint8_t>(), G, dt, rho,
+                       u_tilt_.as<double>()};
 }
 
 int ProjectDevice::snapshot(bool restore, std::string& err) {

@@ -17,8 +17,12 @@
  *   - `PCGSolver<T>::solve` / `factor_modified_incomplete_cholesky0`
  *     (include/pcgsolver/pcg_solver.h:89-190, 366-426) -> astg_pcg_solve_csr
  *
- * Threading: one context per host thread; a context owns one HIP stream and
- * its device buffers. Distinct contexts may be used concurrently.
+ * Threading: a context owns one HIP stream and its device buffers; use each
+ * context from one host thread at a time. Distinct contexts may be used from
+ * different threads: the octree kernels read per-topology tables from
+ * constant memory that exists once per GPU, so calls that launch them take a
+ * process-wide lock and a change of context re-uploads the tables (after
+ * synchronising the GPU).
  */
 #ifndef ASTG_H
 #define ASTG_H
@@ -137,9 +141,25 @@ int astg_split_spheres(int R, int levels, int nspheres, const double* spheres, c
 /* Builds layers, ghosts, T-joint / half-tilt state (constrainOctreeTiltE3,
  * octree_util.cpp:38-169) and initData's fix-ups (octree_omnigrid_solver3.cpp:128-145)
  * from a split map. closed_flags: ASTG_WALL_* bits of closed walls. No collider
- * (boundary SDF > 0 everywhere, so all cutting-cell weights are 1 except
- * half-tilt slots, initCuttingCellWeights 1169-1238). */
+ * until astg_set_boundary_sdf (boundary SDF > 0 everywhere, so all cutting-cell
+ * weights are 1 except half-tilt slots, initCuttingCellWeights 1169-1238). */
 int astg_set_topology_split(astg_ctx* ctx, int R, double h0, int levels, const int* split, int closed_flags);
+
+/* Boundary (collider) SDF, as the reference's initObstacle installs one
+ * (octree_omnigrid_solver3.cpp:152-194, omni_boundary_solver3.cpp:11-31): sdf
+ * holds R^3 signed distances at the finest cell centres ((i+0.5) h0, x
+ * fastest), negative inside the collider; NULL removes the collider. It sets
+ * the unknown numbering's -3h cut (updateIndexParallel 292-347), the
+ * cutting-cell weights (initCuttingCellWeights 1169-1238), the slip constraint
+ * (constrainVelocity, omni_boundary_solver3.h:26-100), the back-trace clip
+ * (BackTrace, advection.h:45-62) and astg_wipe_boundary. slip_center (3
+ * doubles; NULL keeps the reference's hard-coded (0.5, 0.5, 0.45)) is where
+ * the slip normals start. Like astg_set_topology_split, the fields restart at
+ * zero; astg_set_topology_split removes the collider. */
+int astg_set_boundary_sdf(astg_ctx* ctx, const double* sdf, const double* slip_center);
+/* R^3 signed distances |p - c| - radius of a sphere at the finest cell
+ * centres (the reference's Sphere3 collider; its own grid agrees to 1 ulp). */
+int astg_sphere_sdf(int R, double h0, double cx, double cy, double cz, double radius, double* out);
 
 /* Opens non-fixed level-l leaf tilts where mask[i] != 0 (e = h_l/2), closes the
  * others, as adjustE commits its decision (octree_omnigrid_solver3.cpp:1150-1165). */
@@ -179,6 +199,42 @@ int astg_get_system(astg_ctx* ctx, int* rowstart, int* col, double* val, double*
  * rowindex / value nnz_L. Pass NULL arrays to query nnz_L. */
 int astg_get_mic_factor(astg_ctx* ctx, double* invdiag, unsigned* colstart, unsigned* rowindex, double* value,
                         long long* nnz_l);
+
+/* ---- fields in the caller's memory (PyTorch-ROCm and other device users) */
+/* The context's field layout: every level's nodes concatenated (level l holds
+ * nodes [offset[l], offset[l+1]) in node order, G = offset[levels]). MAC
+ * velocity 3*G axis-major (u_x of node g at g, u_y at G+g, u_z at 2G+g), tilt
+ * velocity 8*G slot-major, each scalar one cell and one vertex array of G.
+ * offsets: levels+1 ints (may be NULL). */
+int astg_field_layout(astg_ctx* ctx, long long* G, long long* offsets);
+
+/* Pointers to caller-owned fields in that layout. A NULL array is skipped.
+ * device_pointers = 1: device memory of the context's GPU, copied device to
+ * device in the order of `stream` (the caller's hipStream_t, e.g. torch's
+ * current stream; NULL = the null stream): the copies wait for the work
+ * already queued on `stream`, and work queued on `stream` afterwards waits
+ * for them; the host is not blocked. device_pointers = 0: host memory,
+ * copied synchronously (stream ignored). */
+typedef struct astg_fields {
+    double* mac;          /* 3*G */
+    double* tilt;         /* 8*G */
+    double* density_cell; /* G */
+    double* density_vert; /* G */
+    double* heat_cell;    /* G */
+    double* heat_vert;    /* G */
+    int device_pointers;
+    void* stream;
+} astg_fields;
+/* Copies the caller's fields into the context (before project / advect). */
+int astg_fields_upload(astg_ctx* ctx, const astg_fields* f);
+/* Copies the context's current fields out to the caller's arrays. */
+int astg_fields_download(astg_ctx* ctx, const astg_fields* f);
+/* project(dt) on caller-owned fields (the matrix-free seam of
+ * OctreeOmniGridSolver3::project, src/octree_omnigrid_solver3.cpp:350-679,
+ * with updateVelocityByPressure 681-847): the velocity in `inout` is copied
+ * in, projected and copied back in place, in `inout->stream` order. */
+int astg_project_fields(astg_ctx* ctx, double dt, double rho, const astg_pcg_params* params, astg_fields* inout,
+                        astg_pcg_result* result);
 
 /* ---- workload replay and diagnostics ---------------------------------- */
 /* Saves (restore = 0) or restores (restore = 1) a device-side copy of the
@@ -237,12 +293,14 @@ int astg_get_dual(astg_ctx* ctx, int level, double* oct3, double* tilt3);
  *   ASTG_SAMPLE_MAC       OctreeFaceCenteredGrid3::sample (face_centered_grid3.cpp:36-45)
  *   ASTG_SAMPLE_BACKTRACE BackTrace over dt (advection.h:15-64)
  *   ASTG_SAMPLE_DENSITY / ASTG_SAMPLE_HEAT  OctreeOmniGrid3<REAL>::sample (omni_grid3.h:146-191)
- * out holds 3 values per point for the first three, 1 for the scalars. */
+ *   ASTG_SAMPLE_SDF       boundary SDF, CellCenteredGrid::sample (cell_centered_grid.h:40-54)
+ * out holds 3 values per point for the first three, 1 for the others. */
 #define ASTG_SAMPLE_VELOCITY 0
 #define ASTG_SAMPLE_MAC 1
 #define ASTG_SAMPLE_BACKTRACE 2
 #define ASTG_SAMPLE_DENSITY 3
 #define ASTG_SAMPLE_HEAT 4
+#define ASTG_SAMPLE_SDF 5
 int astg_sample_points(astg_ctx* ctx, int what, int n, const double* xyz, double dt, double* out);
 
 /* Scene source of the reference's octree smoke example (sphereSource,
@@ -255,6 +313,12 @@ int astg_sample_points(astg_ctx* ctx, int what, int n, const double* xyz, double
  * example's solver calls it at the start of every step (sceneSetup). */
 int astg_apply_sphere_source(astg_ctx* ctx, double cx, double cy, double cz, double radius, double density,
                              double heat, double speed, int update_ghost);
+
+/* OctreeOmniGridSolver3::wipeBoundary (src/octree_omnigrid_solver3.cpp:1240-1292):
+ * density and heat at leaf cell centres, leaf vertices and half-tilts where
+ * the boundary SDF is negative become 0 (onUpdate runs it after sceneSetup,
+ * before addForce). A no-op without a collider. */
+int astg_wipe_boundary(astg_ctx* ctx);
 
 /* OctreeOmniGridSolver3::addForce (src/octree_omnigrid_solver3.cpp:917-972):
  * buoyancy f = smoke_density_factor * density + temperature_factor * (heat -

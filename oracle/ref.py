@@ -73,6 +73,12 @@ def lib():
         "ref_step": (I, [P, D, D, I]),
         "ref_write_frame": (I, [P] + [ctypes.c_char_p] * 5),
         "ref_restart": (I, [P] + [ctypes.c_char_p] * 5),
+        "ref_create_collider": (P, [I, D, I, P, I, D, D, D, D]),
+        "ref_get_sdf": (None, [P, P]),
+        "ref_sample_sdf": (I, [P, I, P, P]),
+        "ref_constrain_and_ghost_serial": (I, [P]),
+        "ref_post_project_serial": (I, [P]),
+        "ref_wipe_boundary": (I, [P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -90,13 +96,40 @@ def _p(a):
 class RefSolver:
     """The reference OctreeOmniGridSolver3 driven through the harness."""
 
-    def __init__(self, split, R, levels, closed_flags):
+    def __init__(self, split, R, levels, closed_flags, collider=None):
+        """collider: (cx, cy, cz, r) of the reference's sphere collider, or None
+        for a constant positive boundary SDF (no collider)."""
         self.L = lib()
         split = np.ascontiguousarray(split, np.int32)
-        self.h = self.L.ref_create(R, 1.0 / R, levels, _p(split), closed_flags)
+        if collider is None:
+            self.h = self.L.ref_create(R, 1.0 / R, levels, _p(split), closed_flags)
+        else:
+            cx, cy, cz, r = collider
+            self.h = self.L.ref_create_collider(R, 1.0 / R, levels, _p(split), closed_flags, cx, cy, cz, r)
         if not self.h:
             raise RuntimeError("ref_create failed: %s" % self.L.ref_last_error())
         self.levels = levels
+        self.R = R
+
+    def sdf(self):
+        out = np.empty(self.R ** 3, np.float64)
+        self.L.ref_get_sdf(self.h, _p(out))
+        return out
+
+    def sample_sdf(self, xyz):
+        xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+        out = np.empty(len(xyz))
+        self._chk(self.L.ref_sample_sdf(self.h, len(xyz), _p(xyz), _p(out)))
+        return out
+
+    def constrain_and_ghost_serial(self):
+        self._chk(self.L.ref_constrain_and_ghost_serial(self.h))
+
+    def post_project_serial(self):
+        self._chk(self.L.ref_post_project_serial(self.h))
+
+    def wipe_boundary(self):
+        self._chk(self.L.ref_wipe_boundary(self.h))
 
     def close(self):
         if self.h:

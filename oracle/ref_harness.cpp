@@ -26,6 +26,8 @@
 #include <spdlog/spdlog.h>
 
 #include "advection.h"
+#include "geometry/rigid_body_collider3.h"
+#include "geometry/sphere3.h"
 #include "octree_omnigrid_solver3.h"
 #include "octree_util.h"
 #include "pcgsolver/pcg_solver.h"
@@ -57,10 +59,83 @@ public:
 
     void initObstacle() override {
         boundarySolver->setClosedDomainBoundaryFlag(closed_);
-        auto sdf = std::make_shared<CellCenteredScalarGrid3>(resolution, delta_h, 10000);
-        boundarySolver->updateSdf(sdf);
+        if (collider_) {
+            // the reference's own initObstacle collider (octree_omnigrid_solver3.cpp:152-157)
+            // with the sphere's centre and radius as parameters
+            Geometry::Sphere3Ptr sphere = std::make_shared<Geometry::Sphere3>(
+                colR_, Geometry::Transform3(Vector3f(colC_[0], colC_[1], colC_[2]), Geometry::Quaternion()));
+            boundarySolver->updateCollider(std::make_shared<Geometry::RigidBodyCollider3>(sphere));
+        } else {
+            auto sdf = std::make_shared<CellCenteredScalarGrid3>(resolution, delta_h, 10000);
+            boundarySolver->updateSdf(sdf);
+        }
         boundarySdf = boundarySolver->colliderSdf();
     }
+    void setCollider(double cx, double cy, double cz, double r) {
+        collider_ = true;
+        colC_[0] = cx;
+        colC_[1] = cy;
+        colC_[2] = cz;
+        colR_ = r;
+    }
+    CellCenteredScalarGrid3Ptr& sdf() { return boundarySdf; }
+
+    // constrainVelocity(MAC) (omni_boundary_solver3.h:72-161) restated with the
+    // reference's own grid calls, the faces visited one after another in the
+    // iteration order (levels ascending, leaf order, axis x, y, z). The
+    // reference's loop runs on its thread pool and updates faces in place
+    // while sampling neighbours, so its result depends on where the pool cuts
+    // the iteration into chunks; this serial form is the order-defined version
+    // of the same loop (the slip part), followed by the closed-wall part.
+    void constrainMacSerial() {
+        auto& vel = velocity->getMACGrid();
+        auto& sdfG = boundarySolver->colliderSdf();
+        const Vector3f sCenter(0.5, 0.5, 0.45);  // omni_boundary_solver3.h:77
+        for (auto it : vel->getParallelIteratorVec())
+            for (; it.isValid(); it.next())
+                for (int axis = 0; axis < 3; ++axis) {
+                    if (!vel->isValid(axis, it.level, it.xIdx(), it.yIdx(), it.zIdx())) continue;
+                    auto pos = vel->position(axis, it.level, it.xIdx(), it.yIdx(), it.zIdx());
+                    if (!isInsideSdf(sdfG->sample(pos))) continue;
+                    Vector3f n = pos - sCenter;
+                    if (n.norm() < EPSILON) {
+                        vel->set(axis, it.level, it.xIdx(), it.yIdx(), it.zIdx(), 0);
+                    } else {
+                        auto v = vel->sample(pos);
+                        n.normalize();
+                        vel->set(axis, it.levelCoor(), (v - v.dot(n) * n)[axis]);
+                    }
+                }
+        auto& layout = vel->getLayout();
+        const int f = boundarySolver->closedDomainBoundaryFlag();
+        for (int i = 0; i < layout->level(); ++i) {
+            const auto& res = layout->levelResolution(i);
+            for (int a = 0; a < res.x(); ++a)
+                for (int b = 0; b < res.x(); ++b) {
+                    if (f & kDirectionXPositive) vel->trySetX(i, res.x() - 1, b, a, 0);
+                    if (f & kDirectionXNegative) vel->trySetX(i, 0, b, a, 0);
+                    if (f & kDirectionYPositive) vel->trySetY(i, b, res.y() - 1, a, 0);
+                    if (f & kDirectionYNegative) vel->trySetY(i, b, 0, a, 0);
+                    if (f & kDirectionZPositive) vel->trySetZ(i, b, a, res.z() - 1, 0);
+                    if (f & kDirectionZNegative) vel->trySetZ(i, b, a, 0, 0);
+                }
+        }
+    }
+    // onUpdate's constraint and ghost update before project (222-232), with
+    // the serial MAC constraint
+    void constrainAndGhostSerial() {
+        constrainMacSerial();
+        boundarySolver->constrainVelocity(velocity->getTiltGrid(), velocity->getMACGrid());
+        velocity->updateGhost();
+    }
+    // the chain between project and advect (234-241), serial MAC constraint
+    void postProjectSerial() {
+        constrainAndGhostSerial();
+        velocity->updateDualGrid();
+        boundarySolver->constrainVelocity(velocity->getDualGrid());
+        velocity->getDualGrid()->updateGhost();
+    }
+    void doWipeBoundary() { wipeBoundary(); }
 
     // --- accessors -------------------------------------------------------
     OctreeGridLayout3Ptr& lay() { return layout; }
@@ -160,6 +235,8 @@ private:
     std::vector<int> split_;
     int R_;
     int closed_;
+    bool collider_ = false;
+    double colC_[3] = {0, 0, 0}, colR_ = 0;
     Vector3f srcC_ = Vector3f::Zero();
     double srcR_ = 0, srcD_ = 0, srcH_ = 0, srcV_ = 0;
     bool srcFirst_ = true;
@@ -209,6 +286,50 @@ void* ref_create(int R, double h, int levels, const int* split, int closedFlags)
         return nullptr;
     }
     return s;
+}
+
+// The same with the reference's sphere collider (centre, radius) in place of
+// the constant boundary SDF.
+void* ref_create_collider(int R, double h, int levels, const int* split, int closedFlags, double cx, double cy,
+                          double cz, double r) {
+    HarnessSolver* s = nullptr;
+    int rc = guarded([&] {
+        s = new HarnessSolver(R, h, levels, split, closedFlags);
+        s->setCollider(cx, cy, cz, r);
+        s->initialize();
+    });
+    if (rc != 0) {
+        delete s;
+        return nullptr;
+    }
+    return s;
+}
+
+// The boundary SDF grid (R^3 cell-centred values, x fastest).
+void ref_get_sdf(void* p, double* out) {
+    auto& g = H(p)->sdf();
+    const auto res = g->resolution();
+    for (int z = 0; z < res.z(); ++z)
+        for (int y = 0; y < res.y(); ++y)
+            for (int x = 0; x < res.x(); ++x)
+                out[((size_t)z * res.y() + y) * res.x() + x] = g->get(Size3(x, y, z));
+}
+
+// Boundary SDF sampled at points (CellCenteredGrid::sample, cell_centered_grid.h:40-54).
+int ref_sample_sdf(void* p, int n, const double* xyz, double* out) {
+    return guarded([&] {
+        for (int i = 0; i < n; ++i) out[i] = H(p)->sdf()->sample(Vector3f(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]));
+    });
+}
+
+int ref_constrain_and_ghost_serial(void* p) {
+    return guarded([&] { H(p)->constrainAndGhostSerial(); });
+}
+int ref_post_project_serial(void* p) {
+    return guarded([&] { H(p)->postProjectSerial(); });
+}
+int ref_wipe_boundary(void* p) {
+    return guarded([&] { H(p)->doWipeBoundary(); });
 }
 
 void ref_destroy(void* p) { delete H(p); }

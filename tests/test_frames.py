@@ -159,6 +159,33 @@ def test_mismatched_and_truncated_files_are_refused(frame, tmp_path):
         c.read_frame(layout=str(tmp_path / "missing.bin"))
 
 
+def bad_half_direction(g, tmp_path):
+    """The reference's tilt file with its first record turned into a half-tilt
+    whose half direction is -1 (no face: not a state the reference writes)."""
+    b = bytearray(g["file_tilt"].tobytes())
+    off = 8  # level count, then the first level's record count
+    rec = np.frombuffer(bytes(b[off:off + TILT.itemsize]), TILT).copy()
+    rec["flags"][0][5] = 1
+    rec["hd"][0] = -1
+    b[off:off + TILT.itemsize] = rec.tobytes()
+    path = tmp_path / "tilt_bad_half.bin"
+    path.write_bytes(bytes(b))
+    return str(path)
+
+
+def test_half_tilt_without_direction_is_refused(frame, tmp_path):
+    import astg
+    g = frame
+    L = int(g["levels"])
+    c = host_context(g)
+    before = tilt_of(c, L)
+    with pytest.raises(astg.AstgError) as ei:
+        c.read_frame(tilt=bad_half_direction(g, tmp_path))
+    assert ei.value.code == astg.ASTG_ERR_IO
+    for (e0, b0, h0), (e1, b1, h1) in zip(before, tilt_of(c, L)):
+        assert np.array_equal(e0, e1) and np.array_equal(b0, b1) and np.array_equal(h0, h1)
+
+
 def test_reference_restarts_from_our_files(frame, tmp_path, monkeypatch):
     from oracle import ref
     if not ref.available():

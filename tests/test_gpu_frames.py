@@ -102,3 +102,25 @@ def test_frame_round_trip(frame, tmp_path):
         m, t8 = c2.velocity(l)
         assert close(m, g["restart_mac_%d" % l], 1e-12), l
         assert close(t8, g["restart_tilt8_%d" % l], 1e-12), l
+
+
+def test_bad_tilt_file_leaves_a_working_context(frame, tmp_path):
+    """A tilt file that names a half-tilt without a face is refused before any
+    state changes; the context keeps its tilt state and the next projection
+    runs on consistent tables."""
+    import astg
+    from test_frames import bad_half_direction
+    g = frame
+    L = int(g["levels"])
+    c = restarted(g, write_files(g, tmp_path))
+    before = [c.tilt(l) for l in range(L)]
+    prm = astg.default_params(tol=1e-8)
+    r0 = c.project(0.5 / int(g["R"]), 2.0, prm)
+    with pytest.raises(astg.AstgError) as ei:
+        c.read_frame(tilt=bad_half_direction(g, tmp_path))
+    assert ei.value.code == astg.ASTG_ERR_IO
+    for (e0, b0, h0), l in zip(before, range(L)):
+        e1, b1, h1 = c.tilt(l)
+        assert np.array_equal(e0, e1) and np.array_equal(b0, b1) and np.array_equal(h0, h1)
+    r1 = c.project(0.5 / int(g["R"]), 2.0, prm)
+    assert r1.converged and r1.n == r0.n and r1.nnz == r0.nnz
