@@ -40,6 +40,8 @@ EXPORTS = [
     "astg_set_tilt_adaptivity", "astg_advect", "astg_step",
     "astg_set_boundary_sdf", "astg_sphere_sdf", "astg_wipe_boundary",
     "astg_field_layout", "astg_fields_upload", "astg_fields_download", "astg_project_fields",
+    "astg_diff_state_size", "astg_diff_get_state", "astg_diff_set_state", "astg_diff_step",
+    "astg_diff_step_adjoint", "astg_diff_apply",
 ]
 FIELD_DENSITY, FIELD_HEAT = 0, 1
 SAMPLE_VELOCITY, SAMPLE_MAC, SAMPLE_BACKTRACE, SAMPLE_DENSITY, SAMPLE_HEAT, SAMPLE_SDF = range(6)

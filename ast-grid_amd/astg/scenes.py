@@ -186,3 +186,39 @@ def smoke_step(ctx, info, first, params):
     ctx.add_force(info["dt"], *info["buoyancy"])
     ctx.set_tilt_adaptivity(True)
     return ctx.step(info["dt"], info["rho"], params)
+
+
+def diff_smoke(device=0, R=128, levels=3, seed=0):
+    """The differentiable smoke configuration: the smoke topology at R^3
+    effective (level 0 near the source, coarser outward), frozen; density 1
+    in the source sphere (cells and vertices); the parameter is the MAC
+    velocity u0 = 0.1 N(0,1) (seeded); the target is 3 Gaussian blobs
+    (sigma 0.06, centres U[0.25, 0.75]^3) at the leaf cell centres. Closed x
+    and y walls, open z walls, no collider; dt = 0.5/R, rho = 2."""
+    split = smoke_64_split(R, levels)
+    ctx = Context(device)
+    ctx.set_topology(split, R, levels, CLOSED_XY)
+    ctx.set_tilt_adaptivity(False)
+    cx, cy, cz, r = SMOKE_SOURCE
+    ctx.apply_sphere_source((cx, cy, cz), r, density=1.0, heat=0.0, speed=0.0, update_ghost=True)
+    G, off = ctx.field_layout()
+    rng = np.random.default_rng(seed + 31337)
+    centres = rng.uniform(0.25, 0.75, size=(3, 3))
+    target = np.zeros(G)
+    leaf = np.zeros(G)
+    for l in range(levels):
+        xyz, _ = ctx.nodes(l)
+        nleaf = ctx.level_info(l)[1]
+        res = R >> l
+        h = 1.0 / res
+        pos = (xyz[:nleaf].astype(np.float64) + 0.5) * h
+        valid = np.all(xyz[:nleaf] < res - 1, axis=1)
+        t = np.zeros(nleaf)
+        for c in centres:
+            t += np.exp(-np.sum((pos - c) ** 2, axis=1) / (2 * 0.06 ** 2))
+        target[off[l]:off[l] + nleaf] = t * valid
+        leaf[off[l]:off[l] + nleaf] = valid
+    u0 = 0.1 * np.random.default_rng(seed + 4242).standard_normal(3 * G)
+    info = {"R": R, "levels": levels, "dt": 0.5 / R, "rho": 2.0, "G": G, "target": target, "leaf_cells": leaf,
+            "u0": u0, "centres": centres, "leaves": int(sum(ctx.level_info(l)[1] for l in range(levels)))}
+    return ctx, info

@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "consts.h"
+#include "linrec.h"
 #include "octree.h"
 #include "pcg.h"
 #include "project.h"
@@ -103,6 +104,42 @@ public:
     int wipeBoundary(std::string& err);
     bool hasCollider() const { return hasSdf_; }
 
+    // ---- differentiable step (frozen topology, no collider) ----
+    // Flat state of a step: MAC 3G | tilt 8G | dual cell 3G | dual vertex 3G |
+    // density cell G | density vertex G; the adjoint work vector appends the
+    // projection's b and p (n each).
+    long long diffStateLen() const { return 19LL * G_; }
+    // Records the transposes of the step's fixed linear passes (once per
+    // topology); pcg holds the projection system (n unknowns, b, p).
+    int diffPrepare(ProjectDevice& pd, PcgDevice& pcg, double dt, double rho, std::string& err);
+    bool diffReady() const { return diffReady_; }
+    void diffInvalidate() { diffReady_ = false; }
+    // Copies the current state to / from a flat device vector (stream order).
+    int diffGetState(ProjectDevice& pd, double* dst, std::string& err);
+    int diffSetState(ProjectDevice& pd, const double* src, std::string& err);
+    // advect(dt) in which the entries the step does not write keep their
+    // current value (the differentiable step's forward).
+    int advectCarry(ProjectDevice& pd, double dt, std::string& err);
+    // Adjoint of advect (with the decay and the density ghosts): x holds the
+    // output adjoint on entry (first 19G entries) and the adjoint of the
+    // post-projection state (velocity, dual grid) and of the input density
+    // on exit. Needs the post-projection fields of the step in the context.
+    int diffAdjointAdvect(ProjectDevice& pd, double dt, std::string& err);
+    // Applies the transposes of the post-projection chain / the constraint
+    // and ghosts before the projection to the work vector.
+    int diffAdjointPost(std::string& err);
+    int diffAdjointPre(std::string& err);
+    // Transposes of the velocity correction (p's adjoint gathers in the p
+    // region) and of the assembly of b (b's adjoint spreads to the velocity).
+    int diffAdjointVel(std::string& err) { return applyList(recVel_, err); }
+    int diffAdjointAsm(std::string& err) { return applyList(recAsm_, err); }
+    int diffAdjointGhost1(std::string& err) { return applyList(recG1_, err); }
+    // The work vector and the offsets of the regions of b and p in it.
+    double* diffWork() { return work_.as<double>(); }
+    long long diffOffB() const { return offB_; }
+    long long diffOffP() const { return offP_; }
+    double* diffTmp() { return wtmp_.as<double>(); }
+
 private:
     // This context's values of the kernels' constant tables (level
     // resolutions, map offsets, spacings, slip centre, dual-grid matrices):
@@ -139,6 +176,13 @@ private:
     // (faceG node, faceA axis), their order index per (node, axis) (-1 if
     // outside), and the published new values of the serial slip constraint
     bool hasSdf_ = false;
+    // differentiable step: recorded passes, masks of written entries, work vectors
+    bool diffReady_ = false;
+    std::vector<PassRecord> recCG_, recDual_, recG3_, recG1_, recAsm_, recVel_;
+    DevBuf wMac_, wTilt_, wCell_, wVert_;  // 1 where advect writes the entry
+    DevBuf work_, wtmp_, wadv_, recClaim_, recScratch_;
+    long long offB_ = 0, offP_ = 0, workLen_ = 0;
+    int applyList(std::vector<PassRecord>& recs, std::string& err);
     DevBuf sdf_, slipFaceG_, slipFaceA_, slipOrd_, slipPub_, slipErr_;
     int nSlip_ = 0, slipGrid_ = 0;
     int slipMac(ProjectDevice& pd, std::string& err);

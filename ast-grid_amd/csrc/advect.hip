@@ -1478,6 +1478,226 @@ __global__ void k_points(Topo T, Vel V, const double* cell, const double* vert, 
     else out[t] = scalarSample(T, cell, vert, p[0], p[1], p[2]);
 }
 
+// ---------------------------------------------------------------------------
+// adjoint of the advection (frozen topology, no collider)
+// ---------------------------------------------------------------------------
+// Every sampler is linear in the field values with weights that depend on the
+// sampled position. The transpose of a sampler at a point is applied by
+// probing: run k of the sampler returns 1 for its k-th load and 0 for the
+// others, so its output is that load's weight, which is scattered (atomic
+// add) into the adjoint of the loaded entry. The derivative with respect to
+// the position comes from the same sampler evaluated on D3 positions.
+struct ProbeLd {
+    int target;
+    int cnt = 0;
+    const double* tb = nullptr;
+    size_t ti = 0;
+    __device__ explicit ProbeLd(int k) : target(k) {}
+    __device__ double operator()(const double* b, size_t i) {
+        const int c = cnt++;
+        if (c == target) {
+            tb = b;
+            ti = i;
+            return 1.0;
+        }
+        return 0.0;
+    }
+};
+
+// Adjoint arrays of the sampled fields: the forward array a load came from ->
+// the adjoint array the weight goes to.
+struct AdjMap {
+    const double* src[4];
+    double* adj[4];
+    int n;
+    __device__ double* of(const double* b) const {
+        for (int r = 0; r < n; ++r)
+            if (src[r] == b) return adj[r];
+        return nullptr;
+    }
+};
+
+__device__ inline D3 seed(double v, int k) {
+    D3 r(v);
+    r.d[k] = 1.0;
+    return r;
+}
+
+// Scatters sum_c ubar[c] * d out[c] / d field into the adjoints, out = U(p).
+__device__ void velSampleTranspose(const Topo& T, const Vel& V, const AdjMap& A, const double* p, const double* ubar) {
+    if (ubar[0] == 0.0 && ubar[1] == 0.0 && ubar[2] == 0.0) return;
+    double o[3];
+    ProbeLd c0(-1);
+    velSampleT(T, V, c0, p[0], p[1], p[2], o);
+    for (int k = 0; k < c0.cnt; ++k) {
+        ProbeLd pk(k);
+        velSampleT(T, V, pk, p[0], p[1], p[2], o);
+        const double w = ubar[0] * o[0] + ubar[1] * o[1] + ubar[2] * o[2];
+        if (w != 0.0 && isfinite(w)) {
+            double* a = A.of(pk.tb);
+            if (a) atomicAdd(a + pk.ti, w);
+        }
+    }
+}
+
+// J_U(p)^T ubar: the position gradient of ubar . U(p).
+__device__ void velSampleGradT(const Topo& T, const Vel& V, const double* p, const double* ubar, double* pbar) {
+    PlainLd ld;
+    D3 o[3];
+    velSampleT(T, V, ld, seed(p[0], 0), seed(p[1], 1), seed(p[2], 2), o);
+    for (int k = 0; k < 3; ++k) {
+        double s = 0.0;
+        for (int c = 0; c < 3; ++c) s += ubar[c] * o[c].d[k];
+        pbar[k] = isfinite(s) ? s : 0.0;
+    }
+}
+
+// Back-trace of the forward (advection.h:45-62, no collider) and its adjoint:
+// given qbar = d loss / d q, scatters the velocity-field adjoints of both
+// velocity samples (at p and at the midpoint).
+__device__ void backTraceAdjoint(const Topo& T, const Vel& V, const AdjMap& A, const double* p, double dt,
+                                 const double* mid, const double* qbar) {
+    double v2bar[3], mbar[3], v1bar[3];
+    for (int c = 0; c < 3; ++c) v2bar[c] = -dt * qbar[c];
+    velSampleTranspose(T, V, A, mid, v2bar);
+    velSampleGradT(T, V, mid, v2bar, mbar);
+    for (int c = 0; c < 3; ++c) v1bar[c] = -dt * 0.5 * mbar[c];
+    velSampleTranspose(T, V, A, p, v1bar);
+}
+
+__device__ inline void traceFwd(const Topo& T, const Vel& V, const double* p, double dt, double* mid, double* q) {
+    double v[3];
+    velSample(T, V, p[0], p[1], p[2], v);
+    for (int c = 0; c < 3; ++c) mid[c] = p[c] - v[c] * dt * 0.5;
+    velSample(T, V, mid[0], mid[1], mid[2], v);
+    for (int c = 0; c < 3; ++c) q[c] = p[c] - v[c] * dt;
+}
+
+// Adjoint of the density advection (k_adv_scalar): DOF g received
+// S(rho; q(g)); rbar = the adjoint of that value.
+__global__ void k_adj_adv_scalar(Topo T, const uint8_t* level, Vel V, AdjMap A, const int* list, int n, int kind,
+                                 double dt, const double* d0c, const double* d0v, const double* rbarAdv, double* rbarC,
+                                 double* rbarV) {
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= n) return;
+    const int g = list[t];
+    const double ab = rbarAdv[g];
+    if (ab == 0.0) return;
+    const int l = level[g];
+    const double h = c_hl[l];
+    int x, y, z;
+    nodeXYZ(T, g, x, y, z);
+    double p[3] = {x * h, y * h, z * h};
+    if (kind == 0) {
+        p[0] += 0.5 * h;
+        p[1] += 0.5 * h;
+        p[2] += 0.5 * h;
+    }
+    double mid[3], q[3];
+    traceFwd(T, V, p, dt, mid, q);
+    // S^T at q into the density adjoints
+    {
+        ProbeLd c0(-1);
+        scalarSampleT(T, c0, d0c, d0v, q[0], q[1], q[2]);
+        for (int k = 0; k < c0.cnt; ++k) {
+            ProbeLd pk(k);
+            const double w = ab * scalarSampleT(T, pk, d0c, d0v, q[0], q[1], q[2]);
+            if (w != 0.0 && isfinite(w)) atomicAdd((pk.tb == d0c ? rbarC : rbarV) + pk.ti, w);
+        }
+    }
+    // position gradient at q, then the back-trace
+    PlainLd ld;
+    const D3 sv = scalarSampleT(T, ld, d0c, d0v, seed(q[0], 0), seed(q[1], 1), seed(q[2], 2));
+    double qbar[3];
+    for (int k = 0; k < 3; ++k) qbar[k] = isfinite(sv.d[k]) ? ab * sv.d[k] : 0.0;
+    backTraceAdjoint(T, V, A, p, dt, mid, qbar);
+}
+
+// Adjoint of the velocity advection: kind 0 MAC face (axis = sub), 1 tilt
+// slot (slot = sub), 2 half-tilt slot (slot = sub, face slot at i == 4).
+__device__ void advVelDof(const Topo& T, const double* teDst, const uint8_t* level, const Vel& V, const AdjMap& A,
+                          int g, int kind, int sub, double dt, double ab) {
+    const int l = level[g];
+    const double h = c_hl[l];
+    int x, y, z;
+    nodeXYZ(T, g, x, y, z);
+    double p[3], d[3] = {0, 0, 0};
+    if (kind == 0) {
+        const int a = sub;
+        p[0] = x * h + (a == 0 ? 0.0 : 0.5) * h;
+        p[1] = y * h + (a == 1 ? 0.0 : 0.5) * h;
+        p[2] = z * h + (a == 2 ? 0.0 : 0.5) * h;
+        d[a] = 1.0;
+    } else {
+        const double c[3] = {x * h, y * h, z * h};
+        const int dir = T.thd[g];
+        const int k = kind == 1 ? sub : c_halfSlot[dir][sub];
+        if (kind == 2 && sub == 4) {
+            p[0] = c[0], p[1] = c[1], p[2] = c[2];
+            const int ax = (k == 0 || k == 1) ? 0 : (k == 2 || k == 4) ? 1 : 2;
+            d[ax] = 1.0;
+        } else {
+            tiltPos(c, k, teDst[g], p);
+            tiltDir(k, d);
+        }
+    }
+    double mid[3], q[3];
+    traceFwd(T, V, p, dt, mid, q);
+    double ubar[3] = {ab * d[0], ab * d[1], ab * d[2]}, qbar[3];
+    velSampleTranspose(T, V, A, q, ubar);
+    velSampleGradT(T, V, q, ubar, qbar);
+    backTraceAdjoint(T, V, A, p, dt, mid, qbar);
+}
+
+__global__ void k_adj_adv_mac(Topo T, const double* teDst, const uint8_t* level, Vel V, AdjMap A, const int* leafList,
+                              int n, double dt, const double* ubarMac) {
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= 3 * n) return;
+    const int a = t / n, g = leafList[t - a * n];
+    const double ab = ubarMac[(size_t)a * c_G + g];
+    if (ab != 0.0) advVelDof(T, teDst, level, V, A, g, 0, a, dt, ab);
+}
+
+__global__ void k_adj_adv_tilt(Topo T, const double* teDst, const uint8_t* level, Vel V, AdjMap A, const int* list,
+                               int n, double dt, const double* ubarTilt) {
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= 8 * n) return;
+    const int g = list[t >> 3], k = t & 7;
+    const double ab = ubarTilt[(size_t)k * c_G + g];
+    if (ab != 0.0) advVelDof(T, teDst, level, V, A, g, 1, k, dt, ab);
+}
+
+__global__ void k_adj_adv_half(Topo T, const double* teDst, const uint8_t* level, Vel V, AdjMap A, const int* halfList,
+                               int n, double dt, const double* ubarTilt) {
+    const long t = (long)blockIdx.x * BS + threadIdx.x;
+    if (t >= 5L * n) return;
+    const int g = halfList[t / 5], i = (int)(t % 5);
+    const int k = c_halfSlot[T.thd[g]][i];
+    const double ab = ubarTilt[(size_t)k * c_G + g];
+    if (ab != 0.0) advVelDof(T, teDst, level, V, A, g, 2, i, dt, ab);
+}
+
+// Entries an advection step writes (1) or carries over (0): MAC of leaves,
+// tilt slots of velocity tilts and half-tilts, density of leaf cells and of
+// open-tilt and half-tilt vertices.
+__global__ void k_mark(const int* list, int n, int width, int G, uint8_t* mask) {
+    const long t = (long)blockIdx.x * BS + threadIdx.x;
+    if (t >= (long)width * n) return;
+    const int k = (int)(t / n), g = list[t - (long)k * n];
+    mask[(size_t)k * G + g] = 1;
+}
+
+// out = in where the entry is carried over (mask 0), 0 where written.
+__global__ void k_keep_unwritten(long n, const uint8_t* mask, const double* in, double* out) {
+    const long i = (long)blockIdx.x * BS + threadIdx.x;
+    if (i < n) out[i] = mask[i] ? 0.0 : in[i];
+}
+
+__global__ void k_scale1(long n, double s, double* a) {
+    const long i = (long)blockIdx.x * BS + threadIdx.x;
+    if (i < n) a[i] *= s;
+}
+
 // (M^T M)^-1 M^T of the rows selected by the 7 flags, in long double with
 // partial pivoting (Constants::init, constants.cpp:170-256)
 void dualMatrices(double* out) {
@@ -1531,6 +1751,7 @@ void dualMatrices(double* out) {
 // host side
 // ---------------------------------------------------------------------------
 int AdvectDevice::upload(const Octree& o, const ProjectTables& t, bool keepFields, std::string& err) {
+    diffReady_ = false;  // recorded passes belong to the previous topology
     if (keepFields) {
         // a tilt update keeps the node sets; anything else would leave the
         // index maps, node lists and coordinates on the device stale
@@ -2109,6 +2330,235 @@ int AdvectDevice::advect(ProjectDevice& pd, double dt, std::string& err) {
     sampValid_ = false;  // the fields now carry the current tilt state
     return ASTG_OK;
 }
+
+// ---- differentiable step --------------------------------------------------
+int AdvectDevice::applyList(std::vector<PassRecord>& recs, std::string& err) {
+    for (auto it = recs.rbegin(); it != recs.rend(); ++it) {
+        int rc = applyTransposed(*it, work_.as<double>(), wtmp_.as<double>(), stream_, err);
+        if (rc) return rc;
+    }
+    return ASTG_OK;
+}
+
+int AdvectDevice::diffPrepare(ProjectDevice& pd, PcgDevice& pcg, double dt, double rho, std::string& err) {
+    std::lock_guard<std::recursive_mutex> lk_(constMutex());
+    if (int rc_ = bindConstants(false, err)) return rc_;
+    if (hasSdf_) {
+        err = "the differentiable step has no form with a collider";
+        return ASTG_ERR_STATE;
+    }
+    const long long G = G_, n = pcg.n();
+    double* dC = scal_[0][cur_[0]][0].as<double>();
+    double* dV = scal_[0][cur_[0]][1].as<double>();
+    double* doct = dualOct_.as<double>();
+    double* dtil = dualTilt_.as<double>();
+    StateMap M;
+    const double* bases[8] = {pd.uMac(), pd.uTilt(), doct, dtil, dC, dV, pcg.rhs(), pcg.solution()};
+    const long long lens[8] = {3 * G, 8 * G, 3 * G, 3 * G, G, G, n, n};
+    long long off = 0;
+    M.n = 8;
+    for (int r = 0; r < 8; ++r) {
+        M.base[r] = bases[r];
+        M.len[r] = lens[r];
+        M.off[r] = off;
+        off += lens[r];
+    }
+    if (off >= (1LL << 31)) {
+        err = "differentiable step: state exceeds 32-bit indices";
+        return ASTG_ERR_ARG;
+    }
+    offB_ = M.off[6];
+    offP_ = M.off[7];
+    workLen_ = off;
+    HIPCHK(work_.alloc((size_t)workLen_ * 8));
+    HIPCHK(wtmp_.alloc((size_t)workLen_ * 8));
+    HIPCHK(wadv_.alloc((size_t)13 * G * 8));
+    const Topo T = topo();
+    const uint8_t* lev = static_cast<const uint8_t*>(level_.p);
+    const int* gl = ghost_.as<int>();
+    const uint8_t* ih = static_cast<const uint8_t*>(inHalf_.p);
+    double* mac = pd.uMac();
+    double* tilt = pd.uTilt();
+    int rc;
+    auto rec = [&](auto body, int cnt, std::vector<PassRecord>& v) {
+        v.emplace_back();
+        return recordPass(body, cnt, M, workLen_, recClaim_, recScratch_, stream_, v.back(), err);
+    };
+    recCG_.clear();
+    recDual_.clear();
+    recG3_.clear();
+    recG1_.clear();
+    recCG_.reserve(6);
+    recDual_.reserve(3);
+    recG3_.reserve(6);
+    recG1_.reserve(6);
+    if ((rc = rec(BWall{T, lev, mac, closed_}, G_, recCG_)) ||
+        (rc = rec(BMacGhostClear{gl, mac}, nGhost_, recCG_)) ||
+        (rc = rec(BMacGhostSample{T, lev, gl, mac, 0}, nGhost_, recCG_)) ||
+        (rc = rec(BMacFromHalf{hmG_.as<int>(), hmT_.as<int>(), hmA_.as<int>(), hmS_.as<int>(), tilt, mac}, nHalfMac_,
+                  recCG_)) ||
+        (rc = rec(BMacGhostSample{T, lev, gl, mac, 1}, nGhost_, recCG_)) ||
+        (rc = rec(BTiltGhost{T, lev, gl, ih, tilt}, nGhost_, recCG_)))
+        return rc;
+    if ((rc = rec(BDualTilt{T, leaf_.as<int>(), tilt, dtil}, nLeaf_, recDual_)) ||
+        (rc = rec(BDualOct{T, lev, leaf_.as<int>(), mac, tilt, doct}, nLeaf_, recDual_)) ||
+        (rc = rec(BDualHalf{T, half_.as<int>(), tilt, dtil}, nHalf_, recDual_)))
+        return rc;
+    if ((rc = rec(BCellGhostFine<3>{T, lev, gl, doct}, nGhost_, recG3_)) ||
+        (rc = rec(BCellGhostCoarse<3>{T, lev, gl, doct}, nGhost_, recG3_)) ||
+        (rc = rec(BVertGhost<3>{T, lev, gl, ih, dtil}, nGhost_, recG3_)) ||
+        (rc = rec(BCellGhostTJ<3>{T, lev, gl, dtil, doct}, nGhost_, recG3_)) ||
+        (rc = rec(BClosedJoint<3>{T, lev, tj_.as<int>(), doct, dtil}, nTJ_, recG3_)) ||
+        (rc = rec(BClosedByHalf<3>{T, lev, cbh_.as<int>(), doct, dtil, bad_.as<int>()}, nCBH_, recG3_)))
+        return rc;
+    if ((rc = rec(BCellGhostFine<1>{T, lev, gl, dC}, nGhost_, recG1_)) ||
+        (rc = rec(BCellGhostCoarse<1>{T, lev, gl, dC}, nGhost_, recG1_)) ||
+        (rc = rec(BVertGhost<1>{T, lev, gl, ih, dV}, nGhost_, recG1_)) ||
+        (rc = rec(BCellGhostTJ<1>{T, lev, gl, dV, dC}, nGhost_, recG1_)) ||
+        (rc = rec(BClosedJoint<1>{T, lev, tj_.as<int>(), dC, dV}, nTJ_, recG1_)) ||
+        (rc = rec(BClosedByHalf<1>{T, lev, cbh_.as<int>(), dC, dV, bad_.as<int>()}, nCBH_, recG1_)))
+        return rc;
+    if ((rc = pd.recordPasses(M, workLen_, pcg.values(), pcg.rhs(), pcg.devRowstart(), pcg.solution(), rho, dt,
+                              recClaim_, recScratch_, recAsm_, recVel_, err)))
+        return rc;
+    // entries advect writes
+    HIPCHK(wMac_.alloc((size_t)3 * G));
+    HIPCHK(wTilt_.alloc((size_t)8 * G));
+    HIPCHK(wCell_.alloc((size_t)G));
+    HIPCHK(wVert_.alloc((size_t)G));
+    HIPCHK(hipMemsetAsync(wMac_.p, 0, (size_t)3 * G, stream_));
+    HIPCHK(hipMemsetAsync(wTilt_.p, 0, (size_t)8 * G, stream_));
+    HIPCHK(hipMemsetAsync(wCell_.p, 0, (size_t)G, stream_));
+    HIPCHK(hipMemsetAsync(wVert_.p, 0, (size_t)G, stream_));
+    auto mark = [&](const DevBuf& list, int cnt, int width, DevBuf& m) {
+        if (cnt) k_mark<<<nblocks((long)width * cnt), BS, 0, stream_>>>(list.as<int>(), cnt, width, G_, m.as<uint8_t>());
+    };
+    mark(leaf_, nLeaf_, 3, wMac_);
+    mark(velTilt_, nVelTilt_, 8, wTilt_);
+    mark(half_, nHalf_, 8, wTilt_);
+    mark(leaf_, nLeaf_, 1, wCell_);
+    mark(openTilt_, nOpenTilt_, 1, wVert_);
+    mark(half_, nHalf_, 1, wVert_);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(stream_));
+    diffReady_ = true;
+    return ASTG_OK;
+}
+
+int AdvectDevice::diffGetState(ProjectDevice& pd, double* dst, std::string& err) {
+    const size_t G = G_;
+    const double* src[6] = {pd.uMac(), pd.uTilt(), dualOct_.as<double>(), dualTilt_.as<double>(),
+                            scal_[0][cur_[0]][0].as<double>(), scal_[0][cur_[0]][1].as<double>()};
+    const size_t len[6] = {3 * G, 8 * G, 3 * G, 3 * G, G, G};
+    size_t off = 0;
+    for (int r = 0; r < 6; ++r) {
+        HIPCHK(hipMemcpyAsync(dst + off, src[r], len[r] * 8, hipMemcpyDeviceToDevice, stream_));
+        off += len[r];
+    }
+    return ASTG_OK;
+}
+
+int AdvectDevice::diffSetState(ProjectDevice& pd, const double* src, std::string& err) {
+    const size_t G = G_;
+    double* dst[6] = {pd.uMac(), pd.uTilt(), dualOct_.as<double>(), dualTilt_.as<double>(),
+                      scal_[0][cur_[0]][0].as<double>(), scal_[0][cur_[0]][1].as<double>()};
+    const size_t len[6] = {3 * G, 8 * G, 3 * G, 3 * G, G, G};
+    size_t off = 0;
+    for (int r = 0; r < 6; ++r) {
+        HIPCHK(hipMemcpyAsync(dst[r], src + off, len[r] * 8, hipMemcpyDeviceToDevice, stream_));
+        off += len[r];
+    }
+    sampValid_ = false;
+    return ASTG_OK;
+}
+
+int AdvectDevice::advectCarry(ProjectDevice& pd, double dt, std::string& err) {
+    const size_t b = (size_t)G_ * 8;
+    // the destinations start as copies of the sources: entries advect does
+    // not write keep their value
+    for (int f = 0; f < 2; ++f)
+        for (int k = 0; k < 2; ++k)
+            HIPCHK(hipMemcpyAsync(scal_[f][1 - cur_[f]][k].p, scal_[f][cur_[f]][k].p, b, hipMemcpyDeviceToDevice,
+                                  stream_));
+    HIPCHK(hipMemcpyAsync(tMac_.p, pd.uMac(), 3 * b, hipMemcpyDeviceToDevice, stream_));
+    HIPCHK(hipMemcpyAsync(tTilt_.p, pd.uTilt(), 8 * b, hipMemcpyDeviceToDevice, stream_));
+    return advect(pd, dt, err);
+}
+
+int AdvectDevice::diffAdjointAdvect(ProjectDevice& pd, double dt, std::string& err) {
+    std::lock_guard<std::recursive_mutex> lk_(constMutex());
+    if (int rc_ = bindConstants(false, err)) return rc_;
+    if (!diffReady_) {
+        err = "differentiable step not prepared";
+        return ASTG_ERR_STATE;
+    }
+    const long G = G_;
+    double* x = work_.as<double>();
+    double* xMac = x;
+    double* xTilt = x + 3 * G;
+    double* xDoct = x + 11 * G;
+    double* xDtilt = x + 14 * G;
+    double* xCell = x + 17 * G;
+    double* xVert = x + 18 * G;
+    double* a = wadv_.as<double>();  // adjoints of the advected values: MAC 3G, tilt 8G, cell G, vertex G
+    // the decay, then the density ghosts
+    k_scale1<<<nblocks(2 * G), BS, 0, stream_>>>(2 * G, 0.999, xCell);
+    int rc = applyList(recG1_, err);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(a, x, (size_t)11 * G * 8, hipMemcpyDeviceToDevice, stream_));
+    HIPCHK(hipMemcpyAsync(a + 11 * G, xCell, (size_t)2 * G * 8, hipMemcpyDeviceToDevice, stream_));
+    // entries the step carried over pass their adjoint straight through
+    k_keep_unwritten<<<nblocks(3 * G), BS, 0, stream_>>>(3 * G, wMac_.as<uint8_t>(), a, xMac);
+    k_keep_unwritten<<<nblocks(8 * G), BS, 0, stream_>>>(8 * G, wTilt_.as<uint8_t>(), a + 3 * G, xTilt);
+    k_keep_unwritten<<<nblocks(G), BS, 0, stream_>>>(G, wCell_.as<uint8_t>(), a + 11 * G, xCell);
+    k_keep_unwritten<<<nblocks(G), BS, 0, stream_>>>(G, wVert_.as<uint8_t>(), a + 12 * G, xVert);
+    const Topo T = sampTopo();
+    const uint8_t* lev = static_cast<const uint8_t*>(level_.p);
+    const Vel V = vel(pd);
+    AdjMap A;
+    A.n = 3;
+    A.src[0] = V.mac;
+    A.adj[0] = xMac;
+    A.src[1] = V.doct;
+    A.adj[1] = xDoct;
+    A.src[2] = V.dtilt;
+    A.adj[2] = xDtilt;
+    const double* d0c = scal_[0][cur_[0]][0].as<double>();
+    const double* d0v = scal_[0][cur_[0]][1].as<double>();
+    const int BSA = 64;  // heavy lanes: small blocks spread them over the CUs
+    auto nbA = [&](long n) { return (int)((n + BSA - 1) / BSA); };
+    (void)nbA;
+    if (nLeaf_)
+        k_adj_adv_scalar<<<nblocks(nLeaf_), BS, 0, stream_>>>(T, lev, V, A, leaf_.as<int>(), nLeaf_, 0, dt, d0c, d0v,
+                                                              a + 11 * G, xCell, xVert);
+    if (nOpenTilt_)
+        k_adj_adv_scalar<<<nblocks(nOpenTilt_), BS, 0, stream_>>>(T, lev, V, A, openTilt_.as<int>(), nOpenTilt_, 1,
+                                                                  dt, d0c, d0v, a + 12 * G, xCell, xVert);
+    if (nHalf_)
+        k_adj_adv_scalar<<<nblocks(nHalf_), BS, 0, stream_>>>(T, lev, V, A, half_.as<int>(), nHalf_, 1, dt, d0c, d0v,
+                                                              a + 12 * G, xCell, xVert);
+    if (nLeaf_)
+        k_adj_adv_mac<<<nblocks(3L * nLeaf_), BS, 0, stream_>>>(T, te_.as<double>(), lev, V, A, leaf_.as<int>(),
+                                                                nLeaf_, dt, a);
+    if (nVelTilt_)
+        k_adj_adv_tilt<<<nblocks(8L * nVelTilt_), BS, 0, stream_>>>(T, te_.as<double>(), lev, V, A,
+                                                                     velTilt_.as<int>(), nVelTilt_, dt, a + 3 * G);
+    if (nHalf_)
+        k_adj_adv_half<<<nblocks(5L * nHalf_), BS, 0, stream_>>>(T, te_.as<double>(), lev, V, A, half_.as<int>(),
+                                                                 nHalf_, dt, a + 3 * G);
+    HIPCHK(hipGetLastError());
+    return ASTG_OK;
+}
+
+int AdvectDevice::diffAdjointPost(std::string& err) {
+    int rc = applyList(recG3_, err);
+    if (rc) return rc;
+    rc = applyList(recDual_, err);
+    if (rc) return rc;
+    return applyList(recCG_, err);
+}
+
+int AdvectDevice::diffAdjointPre(std::string& err) { return applyList(recCG_, err); }
 
 int AdvectDevice::mirrorTemporaries(ProjectDevice& pd, const bool scalars[2], bool velocity, std::string& err) {
     const size_t b = (size_t)G_ * 8;

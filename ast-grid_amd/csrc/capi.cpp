@@ -602,6 +602,243 @@ int astg_project_fields(astg_ctx* c, double dt, double rho, const astg_pcg_param
     return rc2 ? rc2 : rc;
 }
 
+// ---- differentiable step ---------------------------------------------------
+namespace {
+// Copies n doubles between caller memory (host or device, in the caller's
+// stream order) and device memory of the context.
+int callerCopy(astg_ctx* c, double* dst, const double* src, size_t n, bool toContext, int devicePtrs, void* stream) {
+    if (devicePtrs) {
+        hipStream_t us = static_cast<hipStream_t>(stream);
+        if (hipEventRecord(c->ev[5], toContext ? us : c->stream) != hipSuccess ||
+            hipStreamWaitEvent(toContext ? c->stream : us, c->ev[5], 0) != hipSuccess)
+            return fail(c, ASTG_ERR_HIP, "stream ordering failed");
+        if (!toContext &&
+            (hipMemcpyAsync(dst, src, n * 8, hipMemcpyDeviceToDevice, c->stream) != hipSuccess ||
+             hipEventRecord(c->ev[5], c->stream) != hipSuccess || hipStreamWaitEvent(us, c->ev[5], 0) != hipSuccess))
+            return fail(c, ASTG_ERR_HIP, "copy failed");
+        if (toContext && hipMemcpyAsync(dst, src, n * 8, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+            return fail(c, ASTG_ERR_HIP, "copy failed");
+        return ASTG_OK;
+    }
+    if (hipStreamSynchronize(c->stream) != hipSuccess ||
+        hipMemcpy(dst, src, n * 8, toContext ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(c, ASTG_ERR_HIP, "copy failed");
+    return ASTG_OK;
+}
+
+int diffReady(astg_ctx* c, double dt, double rho) {
+    int rc = needAdvect(c);
+    if (rc) return rc;
+    if (c->adv->hasCollider()) return fail(c, ASTG_ERR_STATE, "the differentiable step has no collider form");
+    if (!c->adv->diffReady()) {
+        rc = c->adv->diffPrepare(*c->proj, *c->pcg, dt, rho, c->err);
+        if (rc) return rc;
+    }
+    return ASTG_OK;
+}
+
+// The forward of one differentiable step on the context's fields.
+int diffForward(astg_ctx* c, double dt, double rho, const astg_pcg_params& prm, astg_pcg_result* result,
+                bool advect) {
+    int rc = c->adv->constrainAndGhost(*c->proj, c->err);
+    if (rc) return rc;
+    rc = runProject(c, dt, rho, prm, result);
+    if (rc) return rc;
+    rc = c->adv->postProject(*c->proj, c->err);
+    if (rc) return rc;
+    return advect ? c->adv->advectCarry(*c->proj, dt, c->err) : ASTG_OK;
+}
+}  // namespace
+
+int astg_diff_state_size(astg_ctx* c, long long* n) {
+    if (!c || !n) return ASTG_ERR_ARG;
+    int rc = needAdvect(c);
+    if (rc) return rc;
+    *n = c->adv->diffStateLen();
+    return ASTG_OK;
+}
+
+int astg_diff_get_state(astg_ctx* c, double* state, int device_pointers, void* stream) {
+    if (!c || !state) return ASTG_ERR_ARG;
+    int rc = needAdvect(c);
+    if (rc) return rc;
+    return guarded(c, [&] {
+        const size_t n = (size_t)c->adv->diffStateLen();
+        astg::DevBuf stage;
+        double* dst = state;
+        if (!device_pointers) {
+            if (stage.alloc(n * 8) != hipSuccess) return fail(c, ASTG_ERR_HIP, "allocation failed");
+            dst = stage.as<double>();
+        } else if (hipEventRecord(c->ev[5], static_cast<hipStream_t>(stream)) != hipSuccess ||
+                   hipStreamWaitEvent(c->stream, c->ev[5], 0) != hipSuccess) {
+            return fail(c, ASTG_ERR_HIP, "stream ordering failed");
+        }
+        int r = c->adv->diffGetState(*c->proj, dst, c->err);
+        if (r) return r;
+        if (!device_pointers) return callerCopy(c, state, dst, n, false, 0, nullptr);
+        if (hipEventRecord(c->ev[5], c->stream) != hipSuccess ||
+            hipStreamWaitEvent(static_cast<hipStream_t>(stream), c->ev[5], 0) != hipSuccess)
+            return fail(c, ASTG_ERR_HIP, "stream ordering failed");
+        return ASTG_OK;
+    });
+}
+
+int astg_diff_set_state(astg_ctx* c, const double* state, int device_pointers, void* stream) {
+    if (!c || !state) return ASTG_ERR_ARG;
+    int rc = needAdvect(c);
+    if (rc) return rc;
+    return guarded(c, [&] {
+        const size_t n = (size_t)c->adv->diffStateLen();
+        astg::DevBuf stage;
+        const double* src = state;
+        if (!device_pointers) {
+            if (stage.alloc(n * 8) != hipSuccess) return fail(c, ASTG_ERR_HIP, "allocation failed");
+            int r = callerCopy(c, stage.as<double>(), state, n, true, 0, nullptr);
+            if (r) return r;
+            src = stage.as<double>();
+        } else if (hipEventRecord(c->ev[5], static_cast<hipStream_t>(stream)) != hipSuccess ||
+                   hipStreamWaitEvent(c->stream, c->ev[5], 0) != hipSuccess) {
+            return fail(c, ASTG_ERR_HIP, "stream ordering failed");
+        }
+        int r = c->adv->diffSetState(*c->proj, src, c->err);
+        if (r) return r;
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(c, ASTG_ERR_HIP, "stream sync failed");
+        return ASTG_OK;
+    });
+}
+
+int astg_diff_step(astg_ctx* c, double dt, double rho, const astg_pcg_params* params, astg_pcg_result* result) {
+    if (!c) return ASTG_ERR_ARG;
+    if (!(dt > 0) || !(rho > 0)) return fail(c, ASTG_ERR_ARG, "dt and rho must be positive");
+    astg_pcg_params prm;
+    astg_pcg_params_default(&prm);
+    if (params) prm = *params;
+    return guarded(c, [&] {
+        int rc = diffReady(c, dt, rho);
+        if (rc) return rc;
+        return diffForward(c, dt, rho, prm, result, true);
+    });
+}
+
+int astg_diff_step_adjoint(astg_ctx* c, double dt, double rho, const astg_pcg_params* params, const double* state_in,
+                           const double* adj_out, double* adj_in, int device_pointers, void* stream) {
+    if (!c) return ASTG_ERR_ARG;
+    if (!state_in || !adj_out || !adj_in) return fail(c, ASTG_ERR_ARG, "null array");
+    if (!(dt > 0) || !(rho > 0)) return fail(c, ASTG_ERR_ARG, "dt and rho must be positive");
+    astg_pcg_params prm;
+    astg_pcg_params_default(&prm);
+    if (params) prm = *params;
+    return guarded(c, [&] {
+        int rc = diffReady(c, dt, rho);
+        if (rc) return rc;
+        astg::AdvectDevice& adv = *c->adv;
+        const size_t ns = (size_t)adv.diffStateLen();
+        // the step's forward up to the advection, from its input state
+        rc = astg_diff_set_state(c, state_in, device_pointers, stream);
+        if (rc) return rc;
+        astg_pcg_result res;
+        rc = diffForward(c, dt, rho, prm, &res, false);
+        if (rc) return rc;
+        // the work vector: the output adjoint, then zeros for b and p
+        double* x = adv.diffWork();
+        const size_t n = (size_t)c->pcg->n();
+        if (hipMemsetAsync(x + ns, 0, 2 * n * 8, c->stream) != hipSuccess)
+            return fail(c, ASTG_ERR_HIP, "clear failed");
+        if ((rc = callerCopy(c, x, adj_out, ns, true, device_pointers, stream))) return rc;
+        // advect, post-projection chain, velocity correction (gathers p's adjoint)
+        if ((rc = adv.diffAdjointAdvect(*c->proj, dt, c->err)) || (rc = adv.diffAdjointPost(c->err)) ||
+            (rc = adv.diffAdjointVel(c->err)))
+            return rc;
+        // b's adjoint = A^-1 p's adjoint (A is symmetric)
+        if (hipMemcpyAsync(c->pcg->rhs(), x + adv.diffOffP(), n * 8, hipMemcpyDeviceToDevice, c->stream) !=
+                hipSuccess ||
+            hipMemsetAsync(x + adv.diffOffP(), 0, n * 8, c->stream) != hipSuccess)
+            return fail(c, ASTG_ERR_HIP, "copy failed");
+        astg_pcg_result r2;
+        std::memset(&r2, 0, sizeof(r2));
+        rc = c->pcg->solve(prm, r2, c->err);
+        if (rc) return rc;
+        if (!r2.converged) return fail(c, ASTG_ERR_NOT_CONVERGED, "adjoint PCG did not reach the tolerance");
+        if (hipMemcpyAsync(x + adv.diffOffB(), c->pcg->solution(), n * 8, hipMemcpyDeviceToDevice, c->stream) !=
+            hipSuccess)
+            return fail(c, ASTG_ERR_HIP, "copy failed");
+        // assembly of b, then the constraint and ghosts before the projection
+        if ((rc = adv.diffAdjointAsm(c->err)) || (rc = adv.diffAdjointPre(c->err))) return rc;
+        rc = callerCopy(c, adj_in, x, ns, false, device_pointers, stream);
+        if (rc) return rc;
+        if (!device_pointers && hipStreamSynchronize(c->stream) != hipSuccess)
+            return fail(c, ASTG_ERR_HIP, "stream sync failed");
+        return ASTG_OK;
+    });
+}
+
+int astg_diff_apply(astg_ctx* c, int op, int transpose, double dt, double rho, const astg_pcg_params* params,
+                    const double* x, double* y) {
+    if (!c) return ASTG_ERR_ARG;
+    if (!x || !y || op < 0 || op > 4) return fail(c, ASTG_ERR_ARG, "bad operator or null array");
+    astg_pcg_params prm;
+    astg_pcg_params_default(&prm);
+    if (params) prm = *params;
+    return guarded(c, [&] {
+        int rc = diffReady(c, dt, rho);
+        if (rc) return rc;
+        astg::AdvectDevice& adv = *c->adv;
+        const size_t ns = (size_t)adv.diffStateLen();
+        if (!transpose) {
+            rc = astg_diff_set_state(c, x, 0, nullptr);
+            if (rc) return rc;
+            astg_pcg_result res;
+            switch (op) {
+                case ASTG_OP_CONSTRAIN: rc = adv.constrainAndGhost(*c->proj, c->err); break;
+                case ASTG_OP_POST: rc = adv.postProject(*c->proj, c->err); break;
+                case ASTG_OP_SCALAR_GHOST: rc = adv.scalarGhost(0, c->err); break;
+                case ASTG_OP_PROJECT: rc = runProject(c, dt, rho, prm, &res); break;
+                case ASTG_OP_ADVECT: rc = adv.advectCarry(*c->proj, dt, c->err); break;
+            }
+            if (rc) return rc;
+            return astg_diff_get_state(c, y, 0, nullptr);
+        }
+        // transpose: the work vector holds x (b and p regions cleared)
+        double* w = adv.diffWork();
+        const size_t n = (size_t)c->pcg->n();
+        if (hipMemsetAsync(w + ns, 0, 2 * n * 8, c->stream) != hipSuccess) return fail(c, ASTG_ERR_HIP, "clear failed");
+        if ((rc = callerCopy(c, w, x, ns, true, 0, nullptr))) return rc;
+        switch (op) {
+            case ASTG_OP_CONSTRAIN: rc = adv.diffAdjointPre(c->err); break;
+            case ASTG_OP_POST: rc = adv.diffAdjointPost(c->err); break;
+            case ASTG_OP_SCALAR_GHOST: rc = adv.diffAdjointGhost1(c->err); break;
+            case ASTG_OP_ADVECT: rc = adv.diffAdjointAdvect(*c->proj, dt, c->err); break;
+            case ASTG_OP_PROJECT: {
+                // the projection's factor must be current: assemble and factor first
+                astg_pcg_result res;
+                double* tmp = adv.diffTmp();
+                if (hipMemcpyAsync(tmp, w, ns * 8, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+                    return fail(c, ASTG_ERR_HIP, "copy failed");
+                rc = runProject(c, dt, rho, prm, &res);
+                if (rc) return rc;
+                if (hipMemcpyAsync(w, tmp, ns * 8, hipMemcpyDeviceToDevice, c->stream) != hipSuccess ||
+                    hipMemsetAsync(w + ns, 0, 2 * n * 8, c->stream) != hipSuccess)
+                    return fail(c, ASTG_ERR_HIP, "copy failed");
+                if ((rc = adv.diffAdjointVel(c->err))) return rc;
+                if (hipMemcpyAsync(c->pcg->rhs(), w + adv.diffOffP(), n * 8, hipMemcpyDeviceToDevice, c->stream) !=
+                        hipSuccess ||
+                    hipMemsetAsync(w + adv.diffOffP(), 0, n * 8, c->stream) != hipSuccess)
+                    return fail(c, ASTG_ERR_HIP, "copy failed");
+                astg_pcg_result r2;
+                std::memset(&r2, 0, sizeof(r2));
+                if ((rc = c->pcg->solve(prm, r2, c->err))) return rc;
+                if (hipMemcpyAsync(w + adv.diffOffB(), c->pcg->solution(), n * 8, hipMemcpyDeviceToDevice,
+                                   c->stream) != hipSuccess)
+                    return fail(c, ASTG_ERR_HIP, "copy failed");
+                rc = adv.diffAdjointAsm(c->err);
+                break;
+            }
+        }
+        if (rc) return rc;
+        return callerCopy(c, y, w, ns, false, 0, nullptr);
+    });
+}
+
 int astg_get_system(astg_ctx* c, int* rowstart, int* col, double* val, double* b, double* p) {
     if (!c) return ASTG_ERR_ARG;
     if (c->device < 0) {

@@ -6,8 +6,10 @@
 
 #include <string>
 #include <utility>
+#include <vector>
 
 #include "consts.h"
+#include "linrec.h"
 #include "pcg.h"
 #include "project_host.h"
 
@@ -41,6 +43,11 @@ public:
     // (project.hip): their bodies with this context's arrays.
     void asmPasses(double* A, double* b, const int* rs, double rho, double dt, AsmPasses& ps);
     void velPasses(const double* p, double rho, double dt, VelPasses& ps);
+    // Records the transposes of the assembly's b part and of the velocity
+    // correction (linrec.h) in the flat state M.
+    int recordPasses(const StateMap& M, long long stateLen, double* A, double* b, const int* rs, const double* p,
+                     double rho, double dt, DevBuf& claim, DevBuf& scratch, std::vector<PassRecord>& asmRec,
+                     std::vector<PassRecord>& velRec, std::string& err);
 
 private:
     // level spacings in the kernels' constant table: uploaded by upload(),

@@ -554,6 +554,33 @@ void ProjectDevice::velPasses(const double* p, double rho, double dt, VelPasses&
                        u_tilt_.as<double>()};
 }
 
+int ProjectDevice::recordPasses(const StateMap& M, long long stateLen, double* A, double* b, const int* rs,
+                                const double* p, double rho, double dt, DevBuf& claim, DevBuf& scratch,
+                                std::vector<PassRecord>& asmRec, std::vector<PassRecord>& velRec, std::string& err) {
+    std::lock_guard<std::recursive_mutex> lk(constMutex());
+    if (int rc = bindConstants(false, err)) return rc;
+    const ProjectTables& t = *T_;
+    AsmPasses a;
+    asmPasses(A, b, rs, rho, dt, a);
+    VelPasses v;
+    velPasses(p, rho, dt, v);
+    asmRec.clear();
+    asmRec.resize(5);
+    velRec.clear();
+    velRec.resize(3);
+    int rc;
+    if ((rc = recordPass(a.oct, a.oct.no, M, stateLen, claim, scratch, stream_, asmRec[0], err)) ||
+        (rc = recordPass(a.tilt, a.tilt.nt, M, stateLen, claim, scratch, stream_, asmRec[1], err)) ||
+        (rc = recordPass(a.hc, a.hc.nrows, M, stateLen, claim, scratch, stream_, asmRec[2], err)) ||
+        (rc = recordPass(a.hr, a.hr.nrows, M, stateLen, claim, scratch, stream_, asmRec[3], err)) ||
+        (rc = recordPass(a.scale, t.n, M, stateLen, claim, scratch, stream_, asmRec[4], err)) ||
+        (rc = recordPass(v.mac, v.mac.nvm, M, stateLen, claim, scratch, stream_, velRec[0], err)) ||
+        (rc = recordPass(v.tilt, v.tilt.nvt, M, stateLen, claim, scratch, stream_, velRec[1], err)) ||
+        (rc = recordPass(v.half, v.half.nh, M, stateLen, claim, scratch, stream_, velRec[2], err)))
+        return rc;
+    return ASTG_OK;
+}
+
 int ProjectDevice::snapshot(bool restore, std::string& err) {
     const size_t G = T_->G;
     if (!restore) {

@@ -355,6 +355,42 @@ int astg_advect(astg_ctx* ctx, double dt);
  * astg_project, astg_post_project, astg_advect. */
 int astg_step(astg_ctx* ctx, double dt, double rho, const astg_pcg_params* params, astg_pcg_result* result);
 
+/* ---- differentiable step ------------------------------------------------ */
+/* One step of onUpdate as a differentiable map on a frozen topology without a
+ * collider (no source, no force, no tilt adaptivity): constrain + ghosts,
+ * project, post-projection chain, advect (density and velocity; the heat is
+ * advected too but does not enter the state). Its state is the flat vector
+ * MAC 3G | tilt 8G | dual cell 3G | dual vertex 3G | density cell G |
+ * density vertex G (G and node order as astg_field_layout); entries a step
+ * does not write carry over. No counterpart in the reference (the reference
+ * has no adjoint). */
+int astg_diff_state_size(astg_ctx* ctx, long long* n);
+/* State of the context to / from caller memory (device_pointers as in
+ * astg_fields, in `stream` order). */
+int astg_diff_get_state(astg_ctx* ctx, double* state, int device_pointers, void* stream);
+int astg_diff_set_state(astg_ctx* ctx, const double* state, int device_pointers, void* stream);
+/* The step's forward on the context's state. */
+int astg_diff_step(astg_ctx* ctx, double dt, double rho, const astg_pcg_params* params, astg_pcg_result* result);
+/* Adjoint of one step: adj_in = J^T adj_out, J the step's Jacobian at
+ * state_in. The transposed solve reuses the MIC(0)-PCG (A is symmetric);
+ * the fixed linear passes are applied transposed from records made once per
+ * topology; the advection's transpose scatters through its interpolation
+ * weights and back-propagates through the back-traced positions. Leaves the
+ * context's state at state_in after its projection. */
+int astg_diff_step_adjoint(astg_ctx* ctx, double dt, double rho, const astg_pcg_params* params,
+                           const double* state_in, const double* adj_out, double* adj_in, int device_pointers,
+                           void* stream);
+/* One operator of the step, forward (transpose = 0: y = op(x)) or transposed
+ * (y = op^T x), on host state vectors; for checks. ASTG_OP_ADVECT is
+ * linearised at the context's current state (its transpose is J^T there). */
+#define ASTG_OP_CONSTRAIN 0    /* wall constraint and velocity ghosts */
+#define ASTG_OP_POST 1         /* post-projection chain (constraint, ghosts, dual grid, dual ghosts) */
+#define ASTG_OP_SCALAR_GHOST 2 /* density ghosts */
+#define ASTG_OP_PROJECT 3      /* projection (assembly of b, solve, velocity correction) */
+#define ASTG_OP_ADVECT 4       /* advection of density and velocity, density ghosts, decay */
+int astg_diff_apply(astg_ctx* ctx, int op, int transpose, double dt, double rho, const astg_pcg_params* params,
+                    const double* x, double* y);
+
 /* ---- frames on disk (the reference's binary formats) ------------------- */
 /* Writes the context's state as the reference's frame dumps
  * (src/viewer/grid_dump.cpp): layout (dumpOctreeLayout3dBinary, 280-327),
