@@ -1575,12 +1575,15 @@ __device__ inline void traceFwd(const Topo& T, const Vel& V, const double* p, do
 
 // Adjoint of the density advection (k_adv_scalar): DOF g received
 // S(rho; q(g)); rbar = the adjoint of that value.
+// skipHalf: the list's half-tilt nodes are left to the half-tilt pass, which
+// writes the same entries after this one (its write is the one that stays).
 __global__ void k_adj_adv_scalar(Topo T, const uint8_t* level, Vel V, AdjMap A, const int* list, int n, int kind,
                                  double dt, const double* d0c, const double* d0v, const double* rbarAdv, double* rbarC,
-                                 double* rbarV) {
+                                 double* rbarV, const uint8_t* skipHalf) {
     const int t = blockIdx.x * BS + threadIdx.x;
     if (t >= n) return;
     const int g = list[t];
+    if (skipHalf && skipHalf[g]) return;
     const double ab = rbarAdv[g];
     if (ab == 0.0) return;
     const int l = level[g];
@@ -1658,11 +1661,14 @@ __global__ void k_adj_adv_mac(Topo T, const double* teDst, const uint8_t* level,
     if (ab != 0.0) advVelDof(T, teDst, level, V, A, g, 0, a, dt, ab);
 }
 
+// Half-tilt nodes among the velocity tilts are left to k_adj_adv_half: the
+// forward's half-tilt pass overwrites every slot k_adv_tilt wrote for them.
 __global__ void k_adj_adv_tilt(Topo T, const double* teDst, const uint8_t* level, Vel V, AdjMap A, const int* list,
-                               int n, double dt, const double* ubarTilt) {
+                               int n, double dt, const double* ubarTilt, const uint8_t* inHalf) {
     const int t = blockIdx.x * BS + threadIdx.x;
     if (t >= 8 * n) return;
     const int g = list[t >> 3], k = t & 7;
+    if (inHalf[g]) return;
     const double ab = ubarTilt[(size_t)k * c_G + g];
     if (ab != 0.0) advVelDof(T, teDst, level, V, A, g, 1, k, dt, ab);
 }
@@ -2528,21 +2534,22 @@ int AdvectDevice::diffAdjointAdvect(ProjectDevice& pd, double dt, std::string& e
     const int BSA = 64;  // heavy lanes: small blocks spread them over the CUs
     auto nbA = [&](long n) { return (int)((n + BSA - 1) / BSA); };
     (void)nbA;
+    const uint8_t* ih = static_cast<const uint8_t*>(inHalf_.p);
     if (nLeaf_)
         k_adj_adv_scalar<<<nblocks(nLeaf_), BS, 0, stream_>>>(T, lev, V, A, leaf_.as<int>(), nLeaf_, 0, dt, d0c, d0v,
-                                                              a + 11 * G, xCell, xVert);
+                                                              a + 11 * G, xCell, xVert, nullptr);
     if (nOpenTilt_)
         k_adj_adv_scalar<<<nblocks(nOpenTilt_), BS, 0, stream_>>>(T, lev, V, A, openTilt_.as<int>(), nOpenTilt_, 1,
-                                                                  dt, d0c, d0v, a + 12 * G, xCell, xVert);
+                                                                  dt, d0c, d0v, a + 12 * G, xCell, xVert, ih);
     if (nHalf_)
         k_adj_adv_scalar<<<nblocks(nHalf_), BS, 0, stream_>>>(T, lev, V, A, half_.as<int>(), nHalf_, 1, dt, d0c, d0v,
-                                                              a + 12 * G, xCell, xVert);
+                                                              a + 12 * G, xCell, xVert, nullptr);
     if (nLeaf_)
         k_adj_adv_mac<<<nblocks(3L * nLeaf_), BS, 0, stream_>>>(T, te_.as<double>(), lev, V, A, leaf_.as<int>(),
                                                                 nLeaf_, dt, a);
     if (nVelTilt_)
         k_adj_adv_tilt<<<nblocks(8L * nVelTilt_), BS, 0, stream_>>>(T, te_.as<double>(), lev, V, A,
-                                                                     velTilt_.as<int>(), nVelTilt_, dt, a + 3 * G);
+                                                                     velTilt_.as<int>(), nVelTilt_, dt, a + 3 * G, ih);
     if (nHalf_)
         k_adj_adv_half<<<nblocks(5L * nHalf_), BS, 0, stream_>>>(T, te_.as<double>(), lev, V, A, half_.as<int>(),
                                                                  nHalf_, dt, a + 3 * G);
