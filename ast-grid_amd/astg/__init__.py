@@ -36,7 +36,7 @@ EXPORTS = [
     "astg_get_weights", "astg_set_velocity", "astg_get_velocity", "astg_project", "astg_get_system",
     "astg_get_mic_factor", "astg_fields_snapshot", "astg_kernel_time", "astg_sweep_timeline",
     "astg_set_scalar", "astg_get_scalar", "astg_post_project", "astg_scalar_update_ghost", "astg_get_dual",
-    "astg_sample_points", "astg_apply_sphere_source", "astg_add_force", "astg_adjust_e",
+    "astg_sample_points", "astg_sample_points_noalloc", "astg_apply_sphere_source", "astg_add_force", "astg_adjust_e",
     "astg_set_tilt_adaptivity", "astg_advect", "astg_step",
     "astg_set_boundary_sdf", "astg_sphere_sdf", "astg_wipe_boundary",
     "astg_field_layout", "astg_fields_upload", "astg_fields_download", "astg_project_fields",
@@ -141,6 +141,7 @@ def lib():
         "astg_scalar_update_ghost": (I, [P, I]),
         "astg_get_dual": (I, [P, I, P, P]),
         "astg_sample_points": (I, [P, I, I, P, D, P]),
+        "astg_sample_points_noalloc": (I, [P, I, I, P, D, P]),
         "astg_apply_sphere_source": (I, [P, D, D, D, D, D, D, D, I]),
         "astg_add_force": (I, [P, D, D, D]),
         "astg_adjust_e": (I, [P, D, D, D, D]),
@@ -465,10 +466,13 @@ class Context:
         self._chk(self._L.astg_get_dual(self.h, level, _ptr(o), _ptr(t)), "get_dual")
         return o, t
 
-    def sample_points(self, what, xyz, dt=0.0):
+    def sample_points(self, what, xyz, dt=0.0, noalloc=False):
+        """Point queries; noalloc=True stages them through the library's fixed
+        device array (astg_sample_points_noalloc, at most 262144 points)."""
         xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
         out = np.empty((len(xyz), 3)) if what <= SAMPLE_BACKTRACE else np.empty(len(xyz))
-        self._chk(self._L.astg_sample_points(self.h, what, len(xyz), _ptr(xyz), dt, _ptr(out)), "sample_points")
+        f = self._L.astg_sample_points_noalloc if noalloc else self._L.astg_sample_points
+        self._chk(f(self.h, what, len(xyz), _ptr(xyz), dt, _ptr(out)), "sample_points")
         return out
 
     def apply_sphere_source(self, center, radius, density=1.0, heat=1.0, speed=1.0, update_ghost=False):

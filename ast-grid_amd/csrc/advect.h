@@ -35,6 +35,16 @@
 
 namespace astg {
 
+// astg_sample_points_noalloc: largest point count of one call, and the device
+// ordinals that can hold the staging array's address.
+constexpr int kNoAllocPoints = 1 << 18;
+constexpr int kMaxDevices = 64;
+// Looks up this GPU's copy of the staging array (hipGetSymbolAddress) once per
+// device, with `device` current; later calls return at once.
+int noallocScratchInit(int device, std::string& err);
+// The address recorded for `device`, or null before noallocScratchInit.
+double* noallocScratch(int device);
+
 // Device views passed to the kernels.
 struct AdvTopo {
     const int* map;                 // concatenated dense index maps -> global node, -1
@@ -84,6 +94,10 @@ public:
     // 2 back-trace over dt, 3 density, 4 heat (out: 3 per point, or 1).
     int samplePoints(ProjectDevice& pd, int what, int n, const double* xyz, double dt, double* out,
                      std::string& err);
+    // The same queries staged through `stage` (device memory of at least
+    // 6 * kNoAllocPoints doubles: points, then results); allocates nothing.
+    int samplePointsStaged(ProjectDevice& pd, int what, int n, const double* xyz, double dt, double* out,
+                           double* stage, std::string& err);
     double advectBytes() const;
     // adjustE's decision (octree_omnigrid_solver3.cpp:1131-1149): the new
     // tilt_layout open flag of every leaf, levels ascending, leaf order.

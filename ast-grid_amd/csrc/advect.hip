@@ -1478,6 +1478,11 @@ __global__ void k_points(Topo T, Vel V, const double* cell, const double* vert, 
     else out[t] = scalarSample(T, cell, vert, p[0], p[1], p[2]);
 }
 
+// Staging of astg_sample_points_noalloc: query points (3 per point) then
+// results (3 per point), at most kNoAllocPoints points. One copy per GPU, held
+// until the process ends; every call on a GPU shares it.
+static __device__ __align__(256) double scratch[(size_t)kNoAllocPoints * 6];
+
 // ---------------------------------------------------------------------------
 // adjoint of the advection (frozen topology, no collider)
 // ---------------------------------------------------------------------------
@@ -2028,6 +2033,61 @@ int AdvectDevice::bindConstants(bool force, std::string& err) {
                                   hipMemcpyHostToDevice, stream_));
     HIPCHK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_dualTilt), dt4, sizeof(dt4), 0, hipMemcpyHostToDevice, stream_));
     HIPCHK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_slip), cSlip_, sizeof(double) * 3, 0, hipMemcpyHostToDevice, stream_));
+    return ASTG_OK;
+}
+
+namespace {
+std::mutex g_scratchMu;
+double* g_scratchPtr[kMaxDevices] = {};
+}  // namespace
+
+int noallocScratchInit(int device, std::string& err) {
+    if (device < 0 || device >= kMaxDevices) {
+        err = "device ordinal beyond the scratch table";
+        return ASTG_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> g(g_scratchMu);
+    if (g_scratchPtr[device]) return ASTG_OK;
+    // the caller made `device` current; the symbol's address is this GPU's copy
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratchPtr[device] = static_cast<double*>(p);
+    return ASTG_OK;
+}
+
+double* noallocScratch(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> g(g_scratchMu);
+    return g_scratchPtr[device];
+}
+
+int AdvectDevice::samplePointsStaged(ProjectDevice& pd, int what, int n, const double* xyz, double dt, double* out,
+                                     double* stage, std::string& err) {
+    std::lock_guard<std::recursive_mutex> lk_(constMutex());
+    if (int rc_ = bindConstants(false, err)) return rc_;
+    if (what < 0 || what > 5 || n < 0) {
+        err = "bad query";
+        return ASTG_ERR_ARG;
+    }
+    if (n > kNoAllocPoints) {
+        err = "at most " + std::to_string(kNoAllocPoints) + " points per call";
+        return ASTG_ERR_ARG;
+    }
+    if (!stage) {
+        err = "scratch not set up on this device";
+        return ASTG_ERR_STATE;
+    }
+    if (n == 0) return ASTG_OK;
+    const int w = what <= 2 ? 3 : 1;
+    double* dPts = stage;
+    double* dOut = stage + (size_t)kNoAllocPoints * 3;
+    HIPCHK(hipMemcpyAsync(dPts, xyz, (size_t)n * 24, hipMemcpyHostToDevice, stream_));
+    const int f = what == 4 ? 1 : 0;
+    k_points<<<nblocks(n), BS, 0, stream_>>>(topo(), vel(pd), scal_[f][cur_[f]][0].as<double>(),
+                                             scal_[f][cur_[f]][1].as<double>(), what, n, dPts, dt, dOut);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, dOut, (size_t)n * 8 * w, hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
     return ASTG_OK;
 }
 

@@ -204,6 +204,12 @@ int astg_create(astg_ctx** out, int device, uint32_t flags) {
             delete c;
             return ASTG_ERR_HIP;
         }
+        // this GPU's address of the no-alloc staging array (once per device)
+        if (astg::noallocScratchInit(device, c->err) != ASTG_OK) {
+            (void)hipStreamDestroy(c->stream);
+            delete c;
+            return ASTG_ERR_HIP;
+        }
         for (auto& e : c->ev)
             if (hipEventCreate(&e) != hipSuccess) {
                 delete c;
@@ -1283,6 +1289,17 @@ int astg_sample_points(astg_ctx* c, int what, int n, const double* xyz, double d
     int rc = needAdvect(c);
     if (rc) return rc;
     return c->adv->samplePoints(*c->proj, what, n, xyz, dt, out, c->err);
+}
+
+int astg_sample_points_noalloc(astg_ctx* c, int what, int n, const double* xyz, double dt, double* out) {
+    if (!c) return ASTG_ERR_ARG;
+    if (n < 0 || (n > 0 && (!xyz || !out))) return fail(c, ASTG_ERR_ARG, "bad point arrays");
+    int rc = needDevice(c);
+    if (rc) return rc;
+    // no set-up here: that would allocate
+    if (!c->tablesReady || !c->deviceReady || !c->adv || !c->proj)
+        return fail(c, ASTG_ERR_STATE, "fields not on the device yet (set or project a velocity first)");
+    return c->adv->samplePointsStaged(*c->proj, what, n, xyz, dt, out, astg::noallocScratch(c->device), c->err);
 }
 
 int astg_apply_sphere_source(astg_ctx* c, double cx, double cy, double cz, double radius, double density,

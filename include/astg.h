@@ -303,6 +303,17 @@ int astg_get_dual(astg_ctx* ctx, int level, double* oct3, double* tilt3);
 #define ASTG_SAMPLE_SDF 5
 int astg_sample_points(astg_ctx* ctx, int what, int n, const double* xyz, double dt, double* out);
 
+/* astg_sample_points without any allocation: the same queries, results and
+ * errors, with the points and results staged through a fixed device array
+ * (one per GPU, 12 MiB, held until the process ends) instead of buffers the
+ * context allocates on demand. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (tight serving loops, harnesses
+ * that forbid hipMalloc). At most 262144 points per call (ASTG_ERR_ARG
+ * beyond); the context's fields must already be on the device (ASTG_ERR_STATE
+ * otherwise). All calls on one GPU share the array: calls that overlap in
+ * time on one GPU, from any context or thread, race. */
+int astg_sample_points_noalloc(astg_ctx* ctx, int what, int n, const double* xyz, double dt, double* out);
+
 /* Scene source of the reference's octree smoke example (sphereSource,
  * example/octree_smoke_3d/main.cpp:23-81) with its constants as parameters:
  * inside the sphere (centre, radius) density and heat are set on cell and
