@@ -53,19 +53,31 @@ def inside_mask(xyz, res, spheres):
     return m.astype(np.uint8)
 
 
-def fill_velocity(ctx, levels, seed):
-    """N(0,1) face velocities drawn in FP32 (widened to FP64), every MAC face and tilt slot."""
+def velocity_frames(sizes, seed):
+    """(mac (n,3), tilt (n,8)) per level for level node counts `sizes`: N(0,1)
+    drawn in FP32 and widened to FP64, every MAC face and tilt slot."""
     rng = np.random.default_rng(seed + 7919)
-    for l in range(levels):
-        nn, _ = ctx.level_info(l)
+    for nn in sizes:
         mac = rng.standard_normal((nn, 3), dtype=np.float32).astype(np.float64)
         til = rng.standard_normal((nn, 8), dtype=np.float32).astype(np.float64)
+        yield mac, til
+
+
+def fill_velocity(ctx, levels, seed):
+    """N(0,1) face velocities drawn in FP32 (widened to FP64), every MAC face and tilt slot."""
+    sizes = [ctx.level_info(l)[0] for l in range(levels)]
+    for l, (mac, til) in enumerate(velocity_frames(sizes, seed)):
         ctx.set_velocity(l, mac, til)
 
 
-def projection_256(device=0, seed=0, R=256, levels=4, band=BAND_256, velocities=True):
+def projection_256_split(seed=0, R=256, levels=4, band=BAND_256):
+    """(split map, spheres) of projection_256."""
     spheres = random_spheres(seed)
-    split = split_spheres(R, levels, spheres.ravel(), band)
+    return split_spheres(R, levels, spheres.ravel(), band), spheres
+
+
+def projection_256(device=0, seed=0, R=256, levels=4, band=BAND_256, velocities=True):
+    split, spheres = projection_256_split(seed, R, levels, band)
     ctx = Context(device)
     ctx.set_topology(split, R, levels, CLOSED_XY)
     xyz, _ = ctx.nodes(0)

@@ -25,9 +25,33 @@ KERNELS = tuple((nm, i) for i, nm in enumerate(KERNEL_NAMES))
 
 # HBM bytes per launch from rocprofv3 counter runs (FETCH_SIZE + WRITE_SIZE,
 # scaled by the copy-kernel calibration) of `bench.py --kernels-only` on this
-# workload, as summarised in profiles/traffic_projection_256.md.
+# workload, as summarised in profiles/traffic_projection_256.md. Constants from
+# that file, not measured by this run.
+TRAFFIC_SOURCE = "profiles/traffic_projection_256.md (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated)"
 TRAFFIC = {"spmv64": 688377572, "fwd64": 629399413, "bwd64": 685888379, "spmv32": 416803946,
            "fwd32": 434900715, "bwd32": 499123025, "copy": 56200384, "mic": 7374325768}
+
+
+def algorithmic_bytes(name, n, nnz_a):
+    """Compulsory HBM bytes of one launch, each array streamed once, 4-byte
+    indices, value width w: SpMV (w+4) nnz_A + (2w+4) n; forward sweep
+    (w+4) nnz_L + (3w+4) n; backward sweep with z.r (w+4) nnz_L + (4w+4) n;
+    MIC(0) factorisation (FP64) 12 (nnz_A + nnz_L) + 16 n; copy 16 n. nnz_L is
+    the strict lower triangle, (nnz_A - n) / 2. None where no such figure is
+    defined (assembly: its own table count is used)."""
+    nnz_l = (nnz_a - n) / 2
+    w = 4 if name.endswith("32") else 8
+    if name.startswith("spmv"):
+        return (w + 4) * nnz_a + (2 * w + 4) * n
+    if name.startswith("fwd"):
+        return (w + 4) * nnz_l + (3 * w + 4) * n
+    if name.startswith("bwd"):
+        return (w + 4) * nnz_l + (4 * w + 4) * n
+    if name == "mic":
+        return 12 * (nnz_a + nnz_l) + 16 * n
+    if name == "copy":
+        return 16 * n
+    return None
 
 
 def log(*a):
@@ -177,6 +201,22 @@ def batched_timing(seed=0, count=24, tune=False):
     return out
 
 
+def projected_fields(ctx, res, levels):
+    """Pressure and per-level (MAC, tilt) velocities after a projection."""
+    _, _, _, _, p = ctx.system(res.n, res.nnz)
+    return {"p": p, "u": [ctx.velocity(l) for l in range(levels)]}
+
+
+def field_errors(fmx, f64):
+    """Relative L2 error of the mixed form's fields against the FP64 form's."""
+    import numpy as np
+    rel = lambda a, b: float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+    cat = lambda f, i: np.concatenate([u[i].ravel() for u in f["u"]])
+    return {"pressure_rel_l2": rel(fmx["p"], f64["p"]),
+            "mac_velocity_rel_l2": rel(cat(fmx, 0), cat(f64, 0)),
+            "tilt_velocity_rel_l2": rel(cat(fmx, 1), cat(f64, 1))}
+
+
 def full_scale_timing(reps=2):
     """The full-scale size (scenes.full_512: 512^3-effective, 5 levels, 18 M
     leaves): set-up, one projection in FP64 and one mixed (best of `reps`
@@ -213,16 +253,8 @@ def full_scale_timing(reps=2):
                      "ms_factor": round(best.ms_factor, 2), "ms_solve": round(best.ms_solve, 2),
                      "rel_residual": best.rel_residual}
         out["n"], out["nnz"], out["depth"] = best.n, best.nnz, best.depth
-        _, _, _, _, pr = ctx.system(best.n, best.nnz)
-        fields[name] = {"p": pr, "u": [ctx.velocity(l) for l in range(L)]}
-    rel = lambda a, b: float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
-    f64, fmx = fields["fp64"], fields["mixed"]
-    out["mixed_vs_fp64"] = {
-        "pressure_rel_l2": rel(fmx["p"], f64["p"]),
-        "mac_velocity_rel_l2": rel(np.concatenate([u[0].ravel() for u in fmx["u"]]),
-                                   np.concatenate([u[0].ravel() for u in f64["u"]])),
-        "tilt_velocity_rel_l2": rel(np.concatenate([u[1].ravel() for u in fmx["u"]]),
-                                    np.concatenate([u[1].ravel() for u in f64["u"]]))}
+        fields[name] = projected_fields(ctx, best, L)
+    out["mixed_vs_fp64"] = field_errors(fields["mixed"], fields["fp64"])
     out["speedup_mixed_vs_fp64"] = round(out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"], 3)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
@@ -260,39 +292,133 @@ def cpu_smoke_baseline(steps=50):
             "kind": "reference"}
 
 
-def cpu_baseline(ctx, res64, budget_iters=150):
-    """Reference MIC(0)-PCG (oracle/_ref, the reference's own code) or the
-    oracle's port on the same C2 system: factor + a bounded number of PCG
-    iterations, timed, then scaled to the full iteration count."""
-    import numpy as np
-    n, nnz = res64.n, res64.nnz
-    rs, col, val, b, p = ctx.system(n, nnz)
+def host_cpu():
+    """(CPU model, logical CPUs in this process's affinity mask)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count()
+    return model, n
+
+
+def cpu_baseline(ctx, info, res64, seed):
+    """The reference's own project() (octree_omnigrid_solver3.cpp:350-679,
+    compiled from its sources into oracle/_ref) on the same 256^3 scene: the
+    split map, opened level-0 tilts and FP32-drawn face velocities of the GPU
+    run. One call, timed whole (numbering, assembly, MIC(0) factor, PCG to 1e-6,
+    velocity update) and through the reference's own BENCHMARK sections.
+    Without the reference library, the oracle's port of its MIC(0)-PCG on the
+    device-assembled system, timed once to convergence (solve only)."""
     from oracle import ref
-    kind = "reference" if ref.available() else "port"
-    if kind == "reference":
-        lib = ref.lib()
-        cores = lib.ref_num_workers()
+    from astg import scenes
+    model, nproc = host_cpu()
+    R, L = info["R"], info["levels"]
+    n, nnz = res64.n, res64.nnz
+    if ref.available():
         t0 = time.perf_counter()
-        ref.pcg_csr(rs, col, val, b, tol=1e-6, maxit=0)
-        t_factor = time.perf_counter() - t0
+        split, spheres = scenes.projection_256_split(seed, R, L)
+        r = ref.RefSolver(split, R, L, scenes.CLOSED_XY)
+        del split
+        xyz, _ = r.nodes(0)
+        nl = r.level_size(0, ghost=False)
+        r.set_tilt_open(0, scenes.inside_mask(xyz[:nl], R, spheres))
+        sizes = [r.level_size(l, ghost=True) for l in range(L)]
+        assert sizes == [ctx.level_info(l)[0] for l in range(L)], "reference scene differs from the GPU scene"
+        for l, (mac, til) in enumerate(scenes.velocity_frames(sizes, seed)):
+            r.set_velocity(l, mac, til)
+        build_s = time.perf_counter() - t0
+        ref.benchmark_reset()
         t0 = time.perf_counter()
-        ref.pcg_csr(rs, col, val, b, tol=1e-6, maxit=budget_iters)
-        t_run = time.perf_counter() - t0
-    else:
-        from oracle import oracle as orc
-        cores = 1
+        r.project(info["dt"], tol=1e-6, maxit=3000)
+        wall = time.perf_counter() - t0
+        sections = {k: round(v[0] * 1e3, 1) for k, v in ref.benchmark_sections().items()}
+        r.close()
+        cores = int(ref.lib().ref_num_workers())
+        return {"value": round(wall * 1e3, 1), "unit": "ms/step", "cores": cores, "kind": "reference",
+                "sample": "one whole project() call of the reference on the same 256^3 scene (n=%d, nnz=%d): "
+                          "numbering, assembly, MIC(0) factor, PCG to 1e-6, velocity update; the reference's "
+                          "scene build (%.1f s) not included" % (n, nnz, build_s),
+                "sections_ms": sections, "cpu_model": model, "nproc": nproc}
+    from oracle import oracle as orc
+    rs, col, val, b, _ = ctx.system(n, nnz)
+    t0 = time.perf_counter()
+    orc.pcg(rs, col, val, b, tol=1e-6, maxit=3000)
+    wall = time.perf_counter() - t0
+    return {"value": round(wall * 1e3, 1), "unit": "ms/step (solve only)", "cores": 1, "kind": "port",
+            "sample": "the oracle's MIC(0)-PCG on the device-assembled 256^3 system (n=%d, nnz=%d), factor + "
+                      "solve to 1e-6" % (n, nnz), "cpu_model": model, "nproc": nproc}
+
+
+def c3_timing(fd_coords=8, steps=20, every=5):
+    """The differentiable smoke configuration (astg.scenes.diff_smoke: 128^3
+    effective, 3 levels, frozen topology; loss 0.5 |rho_T - target|^2 after
+    `steps` steps, gradient with respect to the initial MAC velocity through
+    the adjoint kernels, checkpoints every `every` steps): wall time per step
+    of the forward and of forward + backward, in the mixed form (FP32 inner
+    PCG, FP64 refinement, 1e-6) and in FP64 (1e-12); the gradients' relative
+    L2 difference; central differences (eps 1e-5, FP64 forward) on
+    `fd_coords` seeded face coordinates against both gradients."""
+    import numpy as np
+    import torch
+    import astg
+    from astg import scenes
+    from astg.diff import DiffSim, rollout
+    ctx, info = scenes.diff_smoke(R=128, levels=3, seed=0)
+    state0 = DiffSim(ctx, info["dt"], info["rho"], astg.default_params()).get_state().clone()
+    target = torch.tensor(info["target"], device="cuda")
+    mask = torch.tensor(info["leaf_cells"], device="cuda")
+
+    def loss(sim, u, n):
+        return 0.5 * torch.sum(mask * (rollout(u, sim, state0, n, every) - target) ** 2)
+
+    out = {"workload": "diff_smoke: 128^3-effective, 3 levels, %d leaves, frozen topology, %d steps, "
+                       "checkpoints every %d" % (info["leaves"], steps, every)}
+    grads, sims = {}, {}
+    for name, prec, tol in (("mixed", astg.PREC_MIXED, 1e-6), ("fp64", astg.PREC_FP64, 1e-12)):
+        sim = DiffSim(ctx, info["dt"], info["rho"], astg.default_params(tol=tol, max_iterations=5000,
+                                                                          precision=prec))
+        u = torch.tensor(info["u0"], device="cuda", requires_grad=True)
+        loss(sim, u, 1).backward()  # first call: the transposes' records
+        u = torch.tensor(info["u0"], device="cuda", requires_grad=True)
+        torch.cuda.synchronize()
         t0 = time.perf_counter()
-        orc.pcg(rs, col, val, b, tol=1e-6, maxit=0)
-        t_factor = time.perf_counter() - t0
-        t0 = time.perf_counter()
-        orc.pcg(rs, col, val, b, tol=1e-6, maxit=budget_iters)
-        t_run = time.perf_counter() - t0
-    t_iter = max(t_run - t_factor, 0.0) / budget_iters
-    est_ms = (t_factor + t_iter * res64.iterations) * 1e3
-    return {"value": round(est_ms, 1), "unit": "ms/step (solve only)", "cores": int(cores), "kind": kind,
-            "sample": "MIC(0) factor + %d PCG iterations on the same 256^3 system (n=%d, nnz=%d), %.1f s measured, "
-                      "scaled to the FP64 iteration count %d; assembly and velocity update not included"
-                      % (budget_iters, n, nnz, t_run + t_factor, res64.iterations)}
+        L = loss(sim, u, steps)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        L.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        grads[name], sims[name] = u.grad.detach().cpu().numpy(), sim
+        out[name] = {"tol": tol, "ms_forward_per_step": round((t1 - t0) * 1e3 / steps, 2),
+                     "ms_fwd_bwd_per_step": round((t2 - t0) * 1e3 / steps, 2), "loss": float(L.detach())}
+    g64, gmx = grads["fp64"], grads["mixed"]
+    out["grad_mixed_vs_fp64_rel_l2"] = float(np.linalg.norm(gmx - g64) / np.linalg.norm(g64))
+    leaf = np.concatenate([info["leaf_cells"]] * 3) > 0
+    cand = np.flatnonzero(leaf & (np.abs(g64) > 1e-2 * np.abs(g64).max()))
+    coords = np.random.default_rng(8).choice(cand, size=fd_coords, replace=False)
+    worst64 = worstmx = 0.0
+    eps = 1e-5
+    with torch.no_grad():
+        for k in coords:
+            up = torch.tensor(info["u0"], device="cuda")
+            up[k] += eps
+            um = torch.tensor(info["u0"], device="cuda")
+            um[k] -= eps
+            fd = (float(loss(sims["fp64"], up, steps)) - float(loss(sims["fp64"], um, steps))) / (2 * eps)
+            worst64 = max(worst64, abs(fd - g64[k]) / abs(g64[k]))
+            worstmx = max(worstmx, abs(fd - gmx[k]) / abs(g64[k]))
+    out["fd_check"] = {"coords": fd_coords, "eps": eps, "max_rel_err_fp64": worst64, "max_rel_err_mixed": worstmx}
+    ctx.close()
+    return out
 
 
 def main():
@@ -349,9 +475,13 @@ def main():
 
     ms64, r64 = time_steps(ctx, info, prm64, a.steps, a.warmup)
     log("fp64  %.2f ms/step iters %d depth %d" % (ms64, r64[-1].iterations, r64[-1].depth))
+    f64 = projected_fields(ctx, r64[-1], info["levels"])
     msmx, rmx = time_steps(ctx, info, prmmx, a.steps, a.warmup)
     log("mixed %.2f ms/step iters %d outer %d" % (msmx, rmx[-1].iterations, rmx[-1].outer))
     last64, lastmx = r64[-1], rmx[-1]
+    errs = field_errors(projected_fields(ctx, lastmx, info["levels"]), f64)
+    del f64
+    log("mixed vs fp64: %s" % errs)
 
     # per-kernel device times (HIP events on the context stream)
     ctx.snapshot(restore=True)
@@ -359,7 +489,14 @@ def main():
     kern = {}
     for name, k in KERNELS:
         ms, by = ctx.kernel_time(k, a.kernel_reps)
-        kern[name] = {"ms": round(ms, 4), "bytes": by, "GBs": round(by / ms / 1e6, 1)}
+        e = {"ms": round(ms, 4), "kernel_bytes": by, "kernel_GBs": round(by / ms / 1e6, 1)}
+        alg = algorithmic_bytes(name, last64.n, last64.nnz)
+        if alg is not None:
+            e["alg_bytes"] = int(alg)
+            e["GBs"] = round(alg / ms / 1e6, 1)
+        else:
+            e["GBs"] = e["kernel_GBs"]
+        kern[name] = e
     # dominant kernel of the timed (mixed) form: largest share of a step
     share = {k: kern[k]["ms"] * lastmx.iterations for k in ("spmv32", "fwd32", "bwd32")}
     share["mic"] = kern["mic"]["ms"]  # one FP64 factorisation per step
@@ -367,6 +504,15 @@ def main():
     ach = kern[dom]["GBs"]
     roofline = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": MI355X_HBM_GBS, "unit": "GB/s",
                 "frac": round(ach / MI355X_HBM_GBS, 4), "traffic": TRAFFIC.get(dom),
+                "traffic_source": TRAFFIC_SOURCE,
+                "bytes_per_launch": kern[dom].get("alg_bytes"),
+                "bytes_basis": "compulsory bytes of one launch from n and nnz, each array streamed once "
+                               "(sweeps: (w+4) nnz_L + (3w+4) n forward, (w+4) nnz_L + (4w+4) n backward)",
+                "merged_stencil": {"achieved": kern[dom]["kernel_GBs"],
+                                   "frac": round(kern[dom]["kernel_GBs"] / MI355X_HBM_GBS, 4),
+                                   "bytes_per_launch": kern[dom]["kernel_bytes"],
+                                   "note": "bytes of the merged-level stencil tables the sweep streams "
+                                           "(more entries than nnz_L), not the compulsory figure"},
                 "kernels": kern}
 
     adv = advect_timing(ctx, info, prm64, a.kernel_reps)
@@ -384,10 +530,17 @@ def main():
             batched["systems"], batched["leaves_total"], batched["mixed"]["ms_batch"],
             batched["mixed"]["ms_sequential"], batched["fp64"]["ms_batch"], batched["fp64"]["ms_sequential"]))
 
+    c3 = c3_timing()
+    log("c3: mixed %.1f ms per fwd+bwd step, fp64 %.1f; gradient mixed vs fp64 %.2e; fd %s" % (
+        c3["mixed"]["ms_fwd_bwd_per_step"], c3["fp64"]["ms_fwd_bwd_per_step"], c3["grad_mixed_vs_fp64_rel_l2"],
+        c3["fd_check"]))
+
     cpu = None
     if not a.no_cpu:
         try:
-            cpu = cpu_baseline(ctx, last64)
+            cpu = cpu_baseline(ctx, info, last64, a.seed)
+            log("reference project() on the CPU: %.1f ms (%d workers), sections %s" % (
+                cpu["value"], cpu["cores"], cpu.get("sections_ms")))
         except Exception as e:  # the baseline is reported, never required
             log("cpu baseline failed: %s" % e)
         try:
@@ -422,12 +575,14 @@ def main():
         "mixed": {"ms_per_step": round(msmx, 3), "iterations": lastmx.iterations, "outer": lastmx.outer,
                   "ms_solve": round(lastmx.ms_solve, 3), "rel_residual": lastmx.rel_residual},
         "speedup_mixed_vs_fp64": round(ms64 / msmx, 3),
+        "mixed_vs_fp64": errs,
         "sweep_depth": last64.depth,
         "setup_s": round(setup_s, 2),
         "roofline": roofline,
         "advect": adv,
         "smoke_64": smoke,
         "batched": batched,
+        "c3": c3,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -79,6 +79,8 @@ def lib():
         "ref_constrain_and_ghost_serial": (I, [P]),
         "ref_post_project_serial": (I, [P]),
         "ref_wipe_boundary": (I, [P]),
+        "ref_benchmark_reset": (None, []),
+        "ref_benchmark_report": (I, [ctypes.c_char_p, I]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -87,6 +89,33 @@ def lib():
     L.ref_quiet()
     _lib = L
     return L
+
+
+def benchmark_reset():
+    """Clears the reference's section timers (global_benchmark.cpp)."""
+    lib().ref_benchmark_reset()
+
+
+def benchmark_sections():
+    """The reference's section timers since the last reset, as
+    {"project": (seconds, calls), "project:A setup": (...), ...}: the report of
+    BenchmarkTimer::report (timer.h:214-225), whose nesting is shown by 4
+    spaces of indent per level, turned back into ':'-joined paths."""
+    L = lib()
+    n = L.ref_benchmark_report(None, 0)
+    buf = ctypes.create_string_buffer(n + 1)
+    L.ref_benchmark_report(buf, n + 1)
+    out, stack = {}, []
+    for line in buf.value.decode().splitlines():
+        parts = line.split("\t")
+        if len(parts) < 2 or parts[0] == "Full time":
+            continue
+        name = parts[0]
+        depth = (len(name) - len(name.lstrip(" "))) // 4
+        stack = stack[:depth] + [name.strip()]
+        calls = int(parts[2]) if len(parts) > 2 else 1
+        out[":".join(stack)] = (float(parts[1]), calls)
+    return out
 
 
 def _p(a):

@@ -19,13 +19,16 @@
 #include <cstdint>
 #include <cstring>
 #include <exception>
+#include <iostream>
 #include <memory>
+#include <sstream>
 #include <string>
 #include <vector>
 
 #include <spdlog/spdlog.h>
 
 #include "advection.h"
+#include "global_benchmark.h"
 #include "geometry/rigid_body_collider3.h"
 #include "geometry/sphere3.h"
 #include "octree_omnigrid_solver3.h"
@@ -732,4 +735,26 @@ int ref_pcg_csr(int n, const int* rs, const int* ci, const double* v, const doub
     return rc == 0 ? ok : -1;
 }
 
+// The reference's own section timers (global_benchmark.cpp, built with
+// GLOBALBENCHMARK): cleared here, read back as the text BenchmarkTimer::report
+// writes (timer.h:214-225: "name<TAB>seconds<TAB>invocations" per line, a
+// nested section indented 4 spaces per level, then "Full time").
+void ref_benchmark_reset() { BENCHMARK_RESET(); }
+
+// Copies the report into buf (at most cap-1 bytes and a NUL); returns its length.
+int ref_benchmark_report(char* buf, int cap) {
+    std::ostringstream os;
+    std::streambuf* old = std::cout.rdbuf(os.rdbuf());
+    BENCHMARK_REPORT_NO_MESSAGES();
+    std::cout.rdbuf(old);
+    const std::string s = os.str();
+    if (buf && cap > 0) {
+        const size_t n = std::min(s.size(), (size_t)cap - 1);
+        std::memcpy(buf, s.data(), n);
+        buf[n] = 0;
+    }
+    return (int)s.size();
+}
+
 }  // extern "C"
+
