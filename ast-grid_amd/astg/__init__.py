@@ -25,6 +25,7 @@ ASTG_ERR_IO = -6
 
 WALL_XNEG, WALL_XPOS, WALL_ZNEG, WALL_ZPOS, WALL_YNEG, WALL_YPOS = 1, 2, 4, 8, 16, 32
 PREC_FP64, PREC_MIXED = 0, 1
+ORDER_AUTO, ORDER_REFERENCE, ORDER_BLOCK_COLOUR = 0, 1, 2
 NORM_INF, NORM_L2 = 0, 1
 
 # Header symbols, in the order include/astg.h declares them.
@@ -51,7 +52,7 @@ K_SPMV64, K_FWD64, K_BWD64, K_MIC, K_ASSEMBLE, K_SPMV32, K_FWD32, K_BWD32, K_COP
 class PcgParams(ctypes.Structure):
     _fields_ = [("tol", ctypes.c_double), ("max_iterations", ctypes.c_int), ("norm", ctypes.c_int),
                 ("mic_tau", ctypes.c_double), ("min_diag_ratio", ctypes.c_double),
-                ("precision", ctypes.c_int), ("max_outer", ctypes.c_int)]
+                ("precision", ctypes.c_int), ("max_outer", ctypes.c_int), ("ordering", ctypes.c_int)]
 
 
 class BatchTuning(ctypes.Structure):

@@ -19,7 +19,8 @@ OUT = os.path.join(HERE, "astg", "libastg.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["octree.cpp", "scene.cpp", "weights.cpp", "project_host.cpp", "capi.cpp", "sweep_merge.cpp", "mic_plan.cpp", "frame_io.cpp",
-           "consts.cpp", "pcg.hip", "pcg_batch.hip", "plan_device.hip", "project.hip", "advect.hip", "linrec.hip"]
+           "consts.cpp", "reorder.cpp", "pcg.hip", "pcg_batch.hip", "plan_device.hip", "project.hip", "advect.hip",
+           "linrec.hip", "reorder.hip"]
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

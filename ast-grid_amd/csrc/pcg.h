@@ -29,8 +29,27 @@ namespace astg {
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    // One owner per allocation: copies are not allowed, moves hand it over.
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) {
+        o.p = nullptr;
+        o.bytes = 0;
+    }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p;
+            bytes = o.bytes;
+            o.p = nullptr;
+            o.bytes = 0;
+        }
+        return *this;
+    }
     ~DevBuf();
     hipError_t alloc(size_t b);
+    void release();  // frees now (the destructor does the same)
     template <typename T>
     T* as() const { return static_cast<T*>(p); }
 };

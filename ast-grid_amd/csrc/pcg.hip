@@ -40,8 +40,11 @@
 
 namespace astg {
 
-DevBuf::~DevBuf() {
+DevBuf::~DevBuf() { release(); }
+void DevBuf::release() {
     if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
 }
 hipError_t DevBuf::alloc(size_t b) {
     if (b <= bytes && p) return hipSuccess;

@@ -436,6 +436,9 @@ def main():
                     help="run only the full-scale size (512^3-effective, 5 levels, 18 M leaves)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="one projection, then per-kernel timings only (for counter collection)")
+    ap.add_argument("--advect-only", action="store_true",
+                    help="one projection, then the post-projection chain and advect, timed --kernel-reps "
+                         "times, and the copy kernel (for counter collection)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: runs on one GPU only (the projection's multi-GPU sharding is not built); "
@@ -466,11 +469,23 @@ def main():
     log("setup %.1f s, leaves %d" % (setup_s, info["leaves"]))
 
     if a.kernels_only:
-        ctx.snapshot(restore=True)
-        ctx.project(info["dt"], info["rho"], prm64)
-        for name, k in KERNELS:
-            ms, by = ctx.kernel_time(k, a.kernel_reps)
-            log("%-9s %.4f ms  %.1f GB/s" % (name, ms, by / ms / 1e6))
+        # FP64 kernels on the reference-numbered system, FP32 kernels and the
+        # mixed form's factorisation on its block-colour system
+        for prm_k, names in ((prm64, [(nm, k) for nm, k in KERNELS if not nm.endswith("32")]),
+                             (prmmx, [(nm, k) for nm, k in KERNELS if nm.endswith("32")] +
+                              [("mic_mixed", dict(KERNELS)["mic"])])):
+            ctx.snapshot(restore=True)
+            ctx.project(info["dt"], info["rho"], prm_k)
+            for name, k in names:
+                ms, by = ctx.kernel_time(k, a.kernel_reps)
+                log("%-9s %.4f ms  %.1f GB/s" % (name, ms, by / ms / 1e6))
+        return
+
+    if a.advect_only:
+        adv = advect_timing(ctx, info, prm64, a.kernel_reps)
+        log("advect: %s" % json.dumps(adv))
+        ms, by = ctx.kernel_time(dict(KERNELS)["copy"], a.kernel_reps)
+        log("copy %.4f ms %.0f bytes" % (ms, by))
         return
 
     ms64, r64 = time_steps(ctx, info, prm64, a.steps, a.warmup)
@@ -483,23 +498,31 @@ def main():
     del f64
     log("mixed vs fp64: %s" % errs)
 
-    # per-kernel device times (HIP events on the context stream)
-    ctx.snapshot(restore=True)
-    ctx.project(info["dt"], info["rho"], prm64)
+    # per-kernel device times (HIP events on the context stream), on the solver
+    # each form uses: the FP64 kernels on the reference-numbered system after
+    # an FP64 projection, the FP32 ones (and the factorisation, as mic_mixed)
+    # on the mixed form's block-colour system after a mixed projection
     kern = {}
-    for name, k in KERNELS:
-        ms, by = ctx.kernel_time(k, a.kernel_reps)
-        e = {"ms": round(ms, 4), "kernel_bytes": by, "kernel_GBs": round(by / ms / 1e6, 1)}
-        alg = algorithmic_bytes(name, last64.n, last64.nnz)
-        if alg is not None:
-            e["alg_bytes"] = int(alg)
-            e["GBs"] = round(alg / ms / 1e6, 1)
-        else:
-            e["GBs"] = e["kernel_GBs"]
-        kern[name] = e
+
+    def time_kernels(prm_k, names):
+        ctx.snapshot(restore=True)
+        ctx.project(info["dt"], info["rho"], prm_k)
+        for name, k in names:
+            ms, by = ctx.kernel_time(k, a.kernel_reps)
+            e = {"ms": round(ms, 4), "kernel_bytes": by, "kernel_GBs": round(by / ms / 1e6, 1)}
+            alg = algorithmic_bytes("mic" if name == "mic_mixed" else name, last64.n, last64.nnz)
+            if alg is not None:
+                e["alg_bytes"] = int(alg)
+                e["GBs"] = round(alg / ms / 1e6, 1)
+            else:
+                e["GBs"] = e["kernel_GBs"]
+            kern[name] = e
+
+    time_kernels(prm64, [(nm, k) for nm, k in KERNELS if not nm.endswith("32")])
+    time_kernels(prmmx, [(nm, k) for nm, k in KERNELS if nm.endswith("32")] + [("mic_mixed", dict(KERNELS)["mic"])])
     # dominant kernel of the timed (mixed) form: largest share of a step
     share = {k: kern[k]["ms"] * lastmx.iterations for k in ("spmv32", "fwd32", "bwd32")}
-    share["mic"] = kern["mic"]["ms"]  # one FP64 factorisation per step
+    share["mic_mixed"] = kern["mic_mixed"]["ms"]  # one FP64 factorisation per step
     dom = max(share, key=share.get)
     ach = kern[dom]["GBs"]
     roofline = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": MI355X_HBM_GBS, "unit": "GB/s",
@@ -576,7 +599,8 @@ def main():
                   "ms_solve": round(lastmx.ms_solve, 3), "rel_residual": lastmx.rel_residual},
         "speedup_mixed_vs_fp64": round(ms64 / msmx, 3),
         "mixed_vs_fp64": errs,
-        "sweep_depth": last64.depth,
+        "sweep_depth": {"fp64": last64.depth, "mixed": lastmx.depth},
+        "ordering": {"fp64": "reference numbering", "mixed": "block colours (8 colours of 8^3-cell cubes)"},
         "setup_s": round(setup_s, 2),
         "roofline": roofline,
         "advect": adv,

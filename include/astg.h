@@ -53,6 +53,20 @@ extern "C" {
 #define ASTG_PREC_FP64 0  /* FP64 MIC(0)-PCG, the reference's arithmetic */
 #define ASTG_PREC_MIXED 1 /* FP32 inner PCG, FP64 MIC pivots and FP64 outer residual b-Ax */
 
+/* Ordering of the unknowns in the projection's solve (astg_project,
+ * astg_project_fields and the differentiable step; the CSR seam and the
+ * batched solves keep the caller's order). The reference numbering gives the
+ * reference's MIC(0) factor and iterates; the block-colour ordering (8
+ * colours of 8^3-cell cubes, the reference index inside a cube) factors the
+ * same matrix in another order: a different preconditioner with far fewer
+ * dependency levels in its triangular solves (at 256^3: 67 against 262) and
+ * more iterations (353 against 267 to 1e-6 in FP64). The solution, the system
+ * astg_get_system returns and the velocity update stay in the reference
+ * numbering either way. */
+#define ASTG_ORDER_AUTO 0         /* reference numbering for FP64, block colours for the mixed form */
+#define ASTG_ORDER_REFERENCE 1
+#define ASTG_ORDER_BLOCK_COLOUR 2
+
 /* Norm of the convergence test. */
 #define ASTG_NORM_INF 0 /* reference: ||r||_inf <= tol * ||b||_inf (pcg_solver.h:376-408) */
 #define ASTG_NORM_L2 1
@@ -67,6 +81,7 @@ typedef struct astg_pcg_params {
     double min_diag_ratio; /* small-pivot fallback ratio, reference 0.25 */
     int precision;         /* ASTG_PREC_* */
     int max_outer;         /* mixed form: max refinement sweeps (0 = default 20) */
+    int ordering;          /* ASTG_ORDER_* (default ASTG_ORDER_AUTO) */
 } astg_pcg_params;
 
 typedef struct astg_pcg_result {
@@ -93,7 +108,14 @@ const char* astg_last_error(const astg_ctx* ctx);
  * colindex/value nnz), symmetric, with rows sorted by column as the
  * reference's FixedSparseMatrix (include/pcgsolver/sparse_matrix.h:276-325).
  * Host pointers; x is fully overwritten. Returns ASTG_ERR_NOT_CONVERGED when
- * the tolerance is not met (x still holds the last iterate). */
+ * the tolerance is not met (x still holds the last iterate).
+ * Limit: at most 16 entries on each side of the diagonal per row (a sweep row
+ * keeps its dependencies in registers); a wider row is rejected with
+ * ASTG_ERR_ARG and the message "row i has more than 16 entries on one side of
+ * the diagonal (unsupported)". The octree's matrices have at most 15 entries
+ * per row, diagonal included, so no matrix project() assembles reaches it.
+ * Unsorted rows, out-of-range columns and an unsymmetric pattern are rejected
+ * with ASTG_ERR_ARG too. */
 int astg_pcg_solve_csr(astg_ctx* ctx, int n, const int* rowstart, const int* colindex, const double* value,
                        const double* b, double* x, const astg_pcg_params* params, astg_pcg_result* result);
 

@@ -13,9 +13,12 @@ Two solves that differ only in rounding (dot and row-sum orders) follow
 trajectories that drift apart by up to a fraction of one iteration's
 progress over hundreds of iterations, so the distance between the two 1e-6
 solutions is compared with the size of that tolerance, not with zero. The
-mixed form within 1e-4 relative L2 of the reference (its stated bound against
-FP64). Skipped where the reference build is absent. The reference solve takes
-about 15 s on 16 cores.
+mixed form in the reference numbering within 1e-4 relative L2 of the
+reference (its stated bound against FP64); the mixed form in its default
+block-colour ordering (another preconditioner, so another trajectory) to the
+FP64 form's bar: host-computed relative residual within 1e-6 and a distance
+to the exact solution at most 1.5x the reference's. Skipped where the
+reference build is absent. The reference solve takes about 15 s on 16 cores.
 """
 import numpy as np
 import pytest
@@ -32,9 +35,11 @@ def test_full_size_pressure_matches_reference_solver():
     ctx, info = scenes.projection_256(device=0, seed=0)
     ctx.snapshot(restore=False)
     sols = {}
-    for p, name in ((astg.PREC_FP64, "fp64"), (astg.PREC_MIXED, "mixed")):
+    for p, order, name in ((astg.PREC_FP64, astg.ORDER_AUTO, "fp64"), (astg.PREC_MIXED, astg.ORDER_AUTO, "mixed"),
+                           (astg.PREC_MIXED, astg.ORDER_REFERENCE, "mixed_ref")):
         ctx.snapshot(restore=True)
-        r = ctx.project(info["dt"], info["rho"], astg.default_params(tol=1e-6, max_iterations=3000, precision=p))
+        r = ctx.project(info["dt"], info["rho"], astg.default_params(tol=1e-6, max_iterations=3000, precision=p,
+                                                                     ordering=order))
         rs, col, val, b, x = ctx.system(r.n, r.nnz)
         sols[name] = (x.copy(), r.iterations)
         print("%s: n=%d iterations %d" % (name, r.n, r.iterations), flush=True)
@@ -45,14 +50,19 @@ def test_full_size_pressure_matches_reference_solver():
     ctx.project(info["dt"], info["rho"], astg.default_params(tol=1e-12, max_iterations=3000))
     xt = ctx.system(r.n, r.nnz)[4]
     rows = np.repeat(np.arange(len(rs) - 1), np.diff(rs))
-    res = np.max(np.abs(b - np.bincount(rows, weights=val * sols["fp64"][0][col], minlength=len(b)))) / np.max(np.abs(b))
-    e_ref, e_fp64 = rel(xr, xt), rel(sols["fp64"][0], xt)
+    resid = lambda x: np.max(np.abs(b - np.bincount(rows, weights=val * x[col], minlength=len(b)))) / np.max(np.abs(b))
+    res, res_mx = resid(sols["fp64"][0]), resid(sols["mixed"][0])
+    e_ref, e_fp64, e_mx = rel(xr, xt), rel(sols["fp64"][0], xt), rel(sols["mixed"][0], xt)
     print("reference: %d iterations; fp64 %d iterations, relative residual %.3e; distance to the 1e-12 solution: "
-          "reference %.3e, fp64 %.3e; fp64 to reference %.3e, mixed to reference %.3e" % (
-              it, sols["fp64"][1], res, e_ref, e_fp64, rel(sols["fp64"][0], xr), rel(sols["mixed"][0], xr)),
+          "reference %.3e, fp64 %.3e; fp64 to reference %.3e, mixed (reference numbering) to reference %.3e" % (
+              it, sols["fp64"][1], res, e_ref, e_fp64, rel(sols["fp64"][0], xr), rel(sols["mixed_ref"][0], xr)),
           flush=True)
+    print("mixed (block colours): %d iterations, relative residual %.3e, distance to the 1e-12 solution %.3e" % (
+        sols["mixed"][1], res_mx, e_mx), flush=True)
     assert abs(sols["fp64"][1] - it) <= max(2, it // 100)
     assert res <= 1e-6
     assert e_fp64 <= 1.5 * e_ref
-    assert rel(sols["mixed"][0], xr) < 1e-4
+    assert rel(sols["mixed_ref"][0], xr) < 1e-4
+    assert res_mx <= 1e-6
+    assert e_mx <= 1.5 * e_ref
     ctx.close()
