@@ -210,7 +210,7 @@ __device__ inline auto trilerp(const V& f000, const V& f100, const V& f010, cons
 }
 
 // levelCoordinate (octree_grid_layout3.cpp:16-41): level of the leaf holding p
-__device__ int locate(const Topo& T, double px, double py, double pz) {
+__device__ __forceinline__ int locate(const Topo& T, double px, double py, double pz) {
     int xi, yi, zi;
     double f;
     bary(px / c_h0, c_R, xi, f);
@@ -228,7 +228,7 @@ __device__ int locate(const Topo& T, double px, double py, double pz) {
 
 // OctreeFaceCenteredGrid3::sampleX/Y/Z (octree_face_centered_grid3.cpp:47-113)
 template <class P, class LD>
-__device__ P macAxisR(const Topo& T, LD& ld, const double* mac, int a, int l, P rx, P ry, P rz) {
+__device__ __forceinline__ P macAxisR(const Topo& T, LD& ld, const double* mac, int a, int l, P rx, P ry, P rz) {
     const int r = c_res[l];
     const P rr[3] = {rx, ry, rz};
     int i0[3], i1[3];
@@ -254,7 +254,7 @@ __device__ P macAxisR(const Topo& T, LD& ld, const double* mac, int a, int l, P 
 
 // OctreeFaceCenteredGrid3::sample (octree_face_centered_grid3.cpp:36-45)
 template <class P, class LD>
-__device__ bool macSampleR(const Topo& T, LD& ld, const double* mac, P px, P py, P pz, P* out) {
+__device__ __forceinline__ bool macSampleR(const Topo& T, LD& ld, const double* mac, P px, P py, P pz, P* out) {
     const int l = locate(T, val(px), val(py), val(pz));
     if (l < 0) {
         out[0] = out[1] = out[2] = P(NAN);
@@ -275,7 +275,7 @@ __device__ inline bool macSample(const Topo& T, const double* mac, double px, do
 // cell-centred trilinear with clamped indices (getBaryCentric, vector form).
 // No collider: the reference's constant positive SDF.
 template <class P>
-__device__ P sdfSampleT(const Topo& T, P px, P py, P pz) {
+__device__ __forceinline__ P sdfSampleT(const Topo& T, P px, P py, P pz) {
     if (!T.sdf) return P(1e4);
     int i0, i1, i2;
     P f0, f1, f2;
@@ -295,7 +295,7 @@ __device__ inline double sdfSample(const Topo& T, double px, double py, double p
 // cell-centred trilinear at a given level (sampleGivenRelativePosition,
 // octree_cell_centered_grid3.h:156-177); NC values per node
 template <int NC, class P, class LD>
-__device__ void cellRelT(const Topo& T, LD& ld, const double* cell, int l, P rx, P ry, P rz, P* out) {
+__device__ __forceinline__ void cellRelT(const Topo& T, LD& ld, const double* cell, int l, P rx, P ry, P rz, P* out) {
     const int r = c_res[l];
     int x0, y0, z0;
     P fx, fy, fz;
@@ -316,7 +316,7 @@ __device__ void cellRelT(const Topo& T, LD& ld, const double* cell, int l, P rx,
 
 // OctreeCellCenteredGrid3::sample (octree_cell_centered_grid3.h:97-123)
 template <int NC, class P, class LD>
-__device__ void cellSampleT(const Topo& T, LD& ld, const double* cell, P px, P py, P pz, P* out) {
+__device__ __forceinline__ void cellSampleT(const Topo& T, LD& ld, const double* cell, P px, P py, P pz, P* out) {
     const int l = locate(T, val(px), val(py), val(pz));
     if (l < 0) {
         for (int c = 0; c < NC; ++c) out[c] = P(NAN);
@@ -341,66 +341,82 @@ __device__ inline double cellv(const Topo& T, const double* cell, int NC, int l,
     return cellvT(T, ld, cell, NC, l, x, y, z, c);
 }
 
-// getNeighborCubeValue (octree_util.h:124-185)
+// getNeighborCubeValue (octree_util.h:124-185), one corner at a time: corner i
+// of the cube around the vertex of region dir holds the vertex value (i ==
+// dir), the cell centre (the opposite corner, where the reference's marker is
+// 0x1000 too), a half-tilt or closed-by-T-joint vertex value at a T-joint
+// (the last such match of the axis loop wins, as in the reference's
+// overwrites), or the mean of the centre and its neighbours along the
+// marker's axes. Returned by value, so the eight corners live in registers.
+template <int NC>
+struct CubeVal {
+    double c[NC];
+};
 template <int NC, class LD>
-__device__ void neighborCube(const Topo& T, LD& ld, const double* cell, const double* vert, int l, int xi, int yi,
-                             int zi, int dir, double (*v)[NC], bool tjoint) {
-    const int reDir = ((dir & 1) ? 8 : 6) - dir;
-    double centerV[NC];
-    const int gc = nd(T, l, xi, yi, zi);
-    for (int c = 0; c < NC; ++c) centerV[c] = gc >= 0 ? ld(cell, (size_t)NC * gc + c) : 0.0;
-    for (int c = 0; c < NC; ++c) v[reDir][c] = centerV[c];
-    const int r = c_res[l];
-    for (int i = 0; i < 8; ++i) {
-        const int marker = c_marker[dir][i];
-        if (marker == 0x1000) continue;
-        const int m[3] = {(marker >> 8) & 0xf, (marker >> 4) & 0xf, marker & 0xf};
-        if (tjoint && l > 0) {
-            bool flag = false;
-            for (int k = 0; k < 3; ++k) {
-                const int k1 = (k + 1) % 3, k2 = (k + 2) % 3;
-                if (m[k] != 0 && m[k1] == 0 && m[k2] == 0) {  // face node
-                    int id[3] = {xi, yi, zi};
-                    id[k] += (m[k] == 0x1);
-                    id[0] *= 2;
-                    id[1] *= 2;
-                    id[2] *= 2;
-                    id[k1] += 1;
-                    id[k2] += 1;
-                    const int g = nd(T, l - 1, id[0], id[1], id[2]);
-                    if (g >= 0 && (T.tb[g] & TB_HALF)) {
-                        for (int c = 0; c < NC; ++c) v[i][c] = ld(vert, (size_t)NC * g + c);
-                        flag = true;
-                    }
-                }
-                if (m[k] == 0 && m[k1] != 0 && m[k2] != 0) {  // edge node
-                    const int id0 = (xi << 1) + (dir == 0 || dir == 3 || dir == 4 || dir == 7);
-                    const int id1 = (yi << 1) + ((dir % 4) < 2);
-                    const int id2 = (zi << 1) + (dir < 4);
-                    const int g = nd(T, l - 1, id0 + c_nbt[i][0], id1 + c_nbt[i][1], id2 + c_nbt[i][2]);
-                    if (g >= 0 && !ghost(T, g) && (T.tb[g] & TB_CLOSED_BY_TJ)) {
-                        for (int c = 0; c < NC; ++c) v[i][c] = ld(vert, (size_t)NC * g + c);
-                        flag = true;
-                    }
+__device__ __forceinline__ CubeVal<NC> cubeCorner(const Topo& T, LD& ld, const double* cell, const double* vert,
+                                                  int l, int xi, int yi, int zi, int dir, int i, int gt,
+                                                  const CubeVal<NC>& centerV, bool tjoint) {
+    CubeVal<NC> v;
+    if (i == dir) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) v.c[c] = gt >= 0 ? ld(vert, (size_t)NC * gt + c) : 0.0;
+        return v;
+    }
+    const int marker = c_marker[dir][i];
+    if (marker == 0x1000) return centerV;  // the opposite corner: the cell centre
+    const int m0 = (marker >> 8) & 0xf, m1 = (marker >> 4) & 0xf, m2 = marker & 0xf;
+    if (tjoint && l > 0) {
+        bool flag = false;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int mk = k == 0 ? m0 : k == 1 ? m1 : m2;
+            const int mk1 = k == 0 ? m1 : k == 1 ? m2 : m0;
+            const int mk2 = k == 0 ? m2 : k == 1 ? m0 : m1;
+            if (mk != 0 && mk1 == 0 && mk2 == 0) {  // face node
+                // the finer level's node at the face centre on axis k
+                const int fx = 2 * (xi + (k == 0 && mk == 0x1)) + (k != 0);
+                const int fy = 2 * (yi + (k == 1 && mk == 0x1)) + (k != 1);
+                const int fz = 2 * (zi + (k == 2 && mk == 0x1)) + (k != 2);
+                const int g = nd(T, l - 1, fx, fy, fz);
+                if (g >= 0 && (T.tb[g] & TB_HALF)) {
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) v.c[c] = ld(vert, (size_t)NC * g + c);
+                    flag = true;
                 }
             }
-            if (flag) continue;
-        }
-        double tv[NC];
-        for (int c = 0; c < NC; ++c) tv[c] = centerV[c];
-        double count = 1;
-        for (int x = -(m[0] == 0x2); x <= +(m[0] == 0x1); ++x)
-            for (int y = -(m[1] == 0x2); y <= +(m[1] == 0x1); ++y)
-                for (int z = -(m[2] == 0x2); z <= +(m[2] == 0x1); ++z) {
-                    if (x == 0 && y == 0 && z == 0) continue;
-                    // OctreeCellCenteredGrid3::clamp: indices into [0, res-2]
-                    const int cx = min(max(0, xi + x), r - 2), cy = min(max(0, yi + y), r - 2),
-                              cz = min(max(0, zi + z), r - 2);
-                    for (int c = 0; c < NC; ++c) tv[c] += cellvT(T, ld, cell, NC, l, cx, cy, cz, c);
-                    count += 1;
+            if (mk == 0 && mk1 != 0 && mk2 != 0) {  // edge node
+                const int id0 = (xi << 1) + (dir == 0 || dir == 3 || dir == 4 || dir == 7);
+                const int id1 = (yi << 1) + ((dir % 4) < 2);
+                const int id2 = (zi << 1) + (dir < 4);
+                const int g = nd(T, l - 1, id0 + c_nbt[i][0], id1 + c_nbt[i][1], id2 + c_nbt[i][2]);
+                if (g >= 0 && !ghost(T, g) && (T.tb[g] & TB_CLOSED_BY_TJ)) {
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) v.c[c] = ld(vert, (size_t)NC * g + c);
+                    flag = true;
                 }
-        for (int c = 0; c < NC; ++c) v[i][c] = tv[c] / count;
+            }
+        }
+        if (flag) return v;
     }
+    const int r = c_res[l];
+    double tv[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) tv[c] = centerV.c[c];
+    double count = 1;
+    for (int x = -(m0 == 0x2); x <= +(m0 == 0x1); ++x)
+        for (int y = -(m1 == 0x2); y <= +(m1 == 0x1); ++y)
+            for (int z = -(m2 == 0x2); z <= +(m2 == 0x1); ++z) {
+                if (x == 0 && y == 0 && z == 0) continue;
+                // OctreeCellCenteredGrid3::clamp: indices into [0, res-2]
+                const int cx = min(max(0, xi + x), r - 2), cy = min(max(0, yi + y), r - 2),
+                          cz = min(max(0, zi + z), r - 2);
+#pragma unroll
+                for (int c = 0; c < NC; ++c) tv[c] += cellvT(T, ld, cell, NC, l, cx, cy, cz, c);
+                count += 1;
+            }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) v.c[c] = tv[c] / count;
+    return v;
 }
 
 template <class P>
@@ -412,7 +428,7 @@ __device__ inline int region3D(const P& x, const P& y, const P& z) {
 
 // OctreeOmniGrid3<T>::sampleSub (octree_omni_grid3.h:242-286)
 template <int NC, class P, class LD>
-__device__ void sampleSubT(const Topo& T, LD& ld, const double* cell, const double* vert, P px, P py, P pz, P* out) {
+__device__ __forceinline__ void sampleSubT(const Topo& T, LD& ld, const double* cell, const double* vert, P px, P py, P pz, P* out) {
     const int l = locate(T, val(px), val(py), val(pz));
     if (l < 0) {
         for (int c = 0; c < NC; ++c) out[c] = P(NAN);
@@ -435,9 +451,19 @@ __device__ void sampleSubT(const Topo& T, LD& ld, const double* cell, const doub
             return;
         }
     }
-    double v[8][NC];
-    for (int c = 0; c < NC; ++c) v[reg][c] = gt >= 0 ? ld(vert, (size_t)NC * gt + c) : 0.0;
-    neighborCube<NC>(T, ld, cell, vert, l, xi, yi, zi, reg, v, (b & TB_TJOINT) != 0);
+    const bool tj = (b & TB_TJOINT) != 0;
+    CubeVal<NC> cv;
+    const int gc = nd(T, l, xi, yi, zi);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) cv.c[c] = gc >= 0 ? ld(cell, (size_t)NC * gc + c) : 0.0;
+    const CubeVal<NC> v0 = cubeCorner<NC>(T, ld, cell, vert, l, xi, yi, zi, reg, 0, gt, cv, tj),
+                      v1 = cubeCorner<NC>(T, ld, cell, vert, l, xi, yi, zi, reg, 1, gt, cv, tj),
+                      v2 = cubeCorner<NC>(T, ld, cell, vert, l, xi, yi, zi, reg, 2, gt, cv, tj),
+                      v3 = cubeCorner<NC>(T, ld, cell, vert, l, xi, yi, zi, reg, 3, gt, cv, tj),
+                      v4 = cubeCorner<NC>(T, ld, cell, vert, l, xi, yi, zi, reg, 4, gt, cv, tj),
+                      v5 = cubeCorner<NC>(T, ld, cell, vert, l, xi, yi, zi, reg, 5, gt, cv, tj),
+                      v6 = cubeCorner<NC>(T, ld, cell, vert, l, xi, yi, zi, reg, 6, gt, cv, tj),
+                      v7 = cubeCorner<NC>(T, ld, cell, vert, l, xi, yi, zi, reg, 7, gt, cv, tj);
     switch (reg) {
         case 0: fx -= 0.5, fy -= 0.5, fz -= 0.5; break;
         case 1: fy -= 0.5, fz -= 0.5; break;
@@ -451,8 +477,9 @@ __device__ void sampleSubT(const Topo& T, LD& ld, const double* cell, const doub
     fx *= 2;
     fy *= 2;
     fz *= 2;
+#pragma unroll
     for (int c = 0; c < NC; ++c)
-        out[c] = trilerp(v[6][c], v[7][c], v[5][c], v[4][c], v[2][c], v[3][c], v[1][c], v[0][c], fx, fy, fz);
+        out[c] = trilerp(v6.c[c], v7.c[c], v5.c[c], v4.c[c], v2.c[c], v3.c[c], v1.c[c], v0.c[c], fx, fy, fz);
 }
 template <int NC>
 __device__ inline void sampleSub(const Topo& T, const double* cell, const double* vert, double px, double py, double pz,
@@ -472,7 +499,7 @@ __device__ inline bool nearTransition(const Topo& T, int l, int xi, int yi, int 
 // OctreeOmniFaceCenteredGrid3::sample, correction-based branch
 // (octree_omni_face_centered_grid3.h:37-80)
 template <class P, class LD>
-__device__ void velSampleT(const Topo& T, const Vel& V, LD& ld, P px, P py, P pz, P* out) {
+__device__ __forceinline__ void velSampleT(const Topo& T, const Vel& V, LD& ld, P px, P py, P pz, P* out) {
     const int l = locate(T, val(px), val(py), val(pz));
     if (l < 0) {
         out[0] = out[1] = out[2] = P(NAN);
@@ -513,7 +540,7 @@ __device__ inline void velSample(const Topo& T, const Vel& V, double px, double 
 
 // OctreeOmniGrid3<REAL>::sample, correction-based branch (octree_omni_grid3.h:146-191)
 template <class P, class LD>
-__device__ P scalarSampleT(const Topo& T, LD& ld, const double* cell, const double* vert, P px, P py, P pz) {
+__device__ __forceinline__ P scalarSampleT(const Topo& T, LD& ld, const double* cell, const double* vert, P px, P py, P pz) {
     const int l = locate(T, val(px), val(py), val(pz));
     if (l < 0) return P(NAN);
     const double dh = c_hl[l];
@@ -551,23 +578,45 @@ __device__ inline double scalarSample(const Topo& T, const double* cell, const d
 // BackTrace, midpoint branch (advection.h:45-62): where the boundary SDF
 // changes sign between p and the traced point, the point is pulled back to
 // the blend w p + (1 - w) ret, w = |phi1| / (|phi0| + |phi1|).
-__device__ void backTrace(const Topo& T, const Vel& V, const double* p, double dt, double* out) {
-    double v[3], mid[3];
-    velSample(T, V, p[0], p[1], p[2], v);
+// The back-trace and, with `sampleEnd`, the velocity at its end point: the
+// velocity samples run in one loop that is not unrolled, so a kernel holds one
+// inlined copy of the sampler (a call would pass its arrays through scratch
+// memory). Step 0 samples at p, step 1 at the midpoint, step 2 (sampleEnd) at
+// the end point; `mid` receives the midpoint when not null.
+__device__ __forceinline__ void traceSteps(const Topo& T, const Vel& V, const double* p, double dt, bool sampleEnd,
+                                           double* out, double* vEnd, double* mid) {
+    double pos[3] = {p[0], p[1], p[2]}, v[3];
+    const int steps = sampleEnd ? 3 : 2;
+#pragma unroll 1
+    for (int s = 0; s < steps; ++s) {
+        velSample(T, V, pos[0], pos[1], pos[2], v);
+        if (s == 0) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) mid[c] = p[c] - v[c] * dt * 0.5;
-    velSample(T, V, mid[0], mid[1], mid[2], v);
+            for (int c = 0; c < 3; ++c) pos[c] = p[c] - v[c] * dt * 0.5;
+            if (mid)
+                for (int c = 0; c < 3; ++c) mid[c] = pos[c];
+        } else if (s == 1) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) out[c] = p[c] - v[c] * dt;
-    if (T.sdf) {
-        const double phi0 = sdfSample(T, p[0], p[1], p[2]);
-        const double phi1 = sdfSample(T, out[0], out[1], out[2]);
-        if (phi0 * phi1 < 0.0) {
-            const double w = fabs(phi1) / (fabs(phi0) + fabs(phi1));
+            for (int c = 0; c < 3; ++c) pos[c] = p[c] - v[c] * dt;
+            if (T.sdf) {
+                const double phi0 = sdfSample(T, p[0], p[1], p[2]);
+                const double phi1 = sdfSample(T, pos[0], pos[1], pos[2]);
+                if (phi0 * phi1 < 0.0) {
+                    const double w = fabs(phi1) / (fabs(phi0) + fabs(phi1));
 #pragma unroll
-            for (int c = 0; c < 3; ++c) out[c] = w * p[c] + (1.0 - w) * out[c];
+                    for (int c = 0; c < 3; ++c) pos[c] = w * p[c] + (1.0 - w) * pos[c];
+                }
+            }
         }
     }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) out[c] = pos[c];
+    if (sampleEnd)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) vEnd[c] = v[c];
+}
+__device__ __forceinline__ void backTrace(const Topo& T, const Vel& V, const double* p, double dt, double* out) {
+    traceSteps(T, V, p, dt, false, out, nullptr, nullptr);
 }
 
 __device__ inline void nodeXYZ(const Topo& T, int g, int& x, int& y, int& z) {
@@ -1199,8 +1248,11 @@ __global__ void k_adv_scalar(Topo T, const uint8_t* level, Vel V, const int* lis
     }
     double q[3];
     backTrace(T, V, p, dt, q);
-    d1[g] = scalarSample(T, d0c, d0v, q[0], q[1], q[2]);
-    h1[g] = scalarSample(T, h0c, h0v, q[0], q[1], q[2]);
+#pragma unroll 1
+    for (int f = 0; f < 2; ++f) {  // density, then heat: one inlined sampler
+        const double r = scalarSample(T, f ? h0c : d0c, f ? h0v : d0v, q[0], q[1], q[2]);
+        (f ? h1 : d1)[g] = r;
+    }
 }
 
 __global__ void k_adv_mac(Topo T, const uint8_t* level, Vel V, const int* leafList, int n, double dt, double* mac1) {
@@ -1214,8 +1266,7 @@ __global__ void k_adv_mac(Topo T, const uint8_t* level, Vel V, const int* leafLi
     const double p[3] = {x * h + (a == 0 ? 0.0 : 0.5) * h, y * h + (a == 1 ? 0.0 : 0.5) * h,
                          z * h + (a == 2 ? 0.0 : 0.5) * h};
     double q[3], v[3];
-    backTrace(T, V, p, dt, q);
-    velSample(T, V, q[0], q[1], q[2], v);
+    traceSteps(T, V, p, dt, true, q, v, nullptr);
     mac1[(size_t)a * c_G + g] = v[a];
 }
 
@@ -1226,12 +1277,14 @@ __device__ inline void tiltPos(const double* c, int k, double e, double* p) {
     p[1] = c[1] + (k4 < 2 ? 1 : -1) * e / 3.0;
     p[2] = c[2] + (k < 4 ? 1 : -1) * e / 3.0;
 }
-// TILT3_UNIT_DIR_UP[k] components as signs of 1/sqrt(3)
+// TILT3_UNIT_DIR_UP[k]: the tilt's offset direction from its centre (the
+// signs of positionTilt), turned to point up (+z), times 1/sqrt(3)
 __device__ inline void tiltDir(int k, double* d) {
     const double s = 1.0 / kSqrt3;
-    const int sx[8] = {1, -1, -1, 1, -1, 1, 1, -1}, sy[8] = {1, 1, -1, -1, -1, -1, 1, 1};
-    d[0] = sx[k] * s;
-    d[1] = sy[k] * s;
+    const int k4 = k & 0x3;
+    const int up = k < 4 ? 1 : -1;
+    d[0] = ((k4 == 0 || k4 == 3) ? up : -up) * s;
+    d[1] = (k4 < 2 ? up : -up) * s;
     d[2] = s;
 }
 
@@ -1249,8 +1302,7 @@ __global__ void k_adv_tilt(Topo T, const double* teDst, const uint8_t* level, Ve
     const double c[3] = {x * h, y * h, z * h};
     double p[3], q[3], v[3], d[3];
     tiltPos(c, k, teDst[g], p);
-    backTrace(T, V, p, dt, q);
-    velSample(T, V, q[0], q[1], q[2], v);
+    traceSteps(T, V, p, dt, true, q, v, nullptr);
     tiltDir(k, d);
     tilt1[(size_t)k * c_G + g] = v[0] * d[0] + v[1] * d[1] + v[2] * d[2];
 }
@@ -1280,8 +1332,7 @@ __global__ void k_adv_half(Topo T, const double* teDst, const uint8_t* level, Ve
         tiltPos(c, k, teDst[g], p);
         tiltDir(k, d);
     }
-    backTrace(T, V, p, dt, q);
-    velSample(T, V, q[0], q[1], q[2], v);
+    traceSteps(T, V, p, dt, true, q, v, nullptr);
     tilt1[(size_t)k * c_G + g] = v[0] * d[0] + v[1] * d[1] + v[2] * d[2];
     if (i == 0) {
         unsigned used = 0;
@@ -1529,7 +1580,7 @@ __device__ inline D3 seed(double v, int k) {
 }
 
 // Scatters sum_c ubar[c] * d out[c] / d field into the adjoints, out = U(p).
-__device__ void velSampleTranspose(const Topo& T, const Vel& V, const AdjMap& A, const double* p, const double* ubar) {
+__device__ __noinline__ void velSampleTranspose(const Topo& T, const Vel& V, const AdjMap& A, const double* p, const double* ubar) {
     if (ubar[0] == 0.0 && ubar[1] == 0.0 && ubar[2] == 0.0) return;
     double o[3];
     ProbeLd c0(-1);
@@ -1546,7 +1597,7 @@ __device__ void velSampleTranspose(const Topo& T, const Vel& V, const AdjMap& A,
 }
 
 // J_U(p)^T ubar: the position gradient of ubar . U(p).
-__device__ void velSampleGradT(const Topo& T, const Vel& V, const double* p, const double* ubar, double* pbar) {
+__device__ __noinline__ void velSampleGradT(const Topo& T, const Vel& V, const double* p, const double* ubar, double* pbar) {
     PlainLd ld;
     D3 o[3];
     velSampleT(T, V, ld, seed(p[0], 0), seed(p[1], 1), seed(p[2], 2), o);
@@ -1560,7 +1611,7 @@ __device__ void velSampleGradT(const Topo& T, const Vel& V, const double* p, con
 // Back-trace of the forward (advection.h:45-62, no collider) and its adjoint:
 // given qbar = d loss / d q, scatters the velocity-field adjoints of both
 // velocity samples (at p and at the midpoint).
-__device__ void backTraceAdjoint(const Topo& T, const Vel& V, const AdjMap& A, const double* p, double dt,
+__device__ __noinline__ void backTraceAdjoint(const Topo& T, const Vel& V, const AdjMap& A, const double* p, double dt,
                                  const double* mid, const double* qbar) {
     double v2bar[3], mbar[3], v1bar[3];
     for (int c = 0; c < 3; ++c) v2bar[c] = -dt * qbar[c];
@@ -1570,12 +1621,9 @@ __device__ void backTraceAdjoint(const Topo& T, const Vel& V, const AdjMap& A, c
     velSampleTranspose(T, V, A, p, v1bar);
 }
 
-__device__ inline void traceFwd(const Topo& T, const Vel& V, const double* p, double dt, double* mid, double* q) {
-    double v[3];
-    velSample(T, V, p[0], p[1], p[2], v);
-    for (int c = 0; c < 3; ++c) mid[c] = p[c] - v[c] * dt * 0.5;
-    velSample(T, V, mid[0], mid[1], mid[2], v);
-    for (int c = 0; c < 3; ++c) q[c] = p[c] - v[c] * dt;
+// The forward back-trace with its midpoint (no collider on this path)
+__device__ __noinline__ void traceFwd(const Topo& T, const Vel& V, const double* p, double dt, double* mid, double* q) {
+    traceSteps(T, V, p, dt, false, q, nullptr, mid);
 }
 
 // Adjoint of the density advection (k_adv_scalar): DOF g received
