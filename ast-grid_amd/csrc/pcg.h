@@ -289,6 +289,7 @@ private:
     bool hostPlan_ = false;                 // ASTG_HOST_PLAN: symbolic plans on the host
     bool perm_ = false;                     // PCG vectors in sweep order (single-system solves)
     int spmvLpr_ = 4;                       // ASTG_SPMV_LPR: lanes per row of the single-system SpMV
+    int spmvRows_ = 2;                      // ASTG_SPMV_ROWS: rows in flight per lane group (1, 2 or 4; 4 lanes per row)
     bool rowOrderForced_ = false;           // setRowOrder (batched form)
     DevBuf pos_;                            // row -> sweep position
     DevBuf prs_, pcol_, pmap_;              // CSR in sweep order: rowstart, columns as positions, source entry

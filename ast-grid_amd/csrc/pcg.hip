@@ -581,6 +581,34 @@ __global__ void __launch_bounds__(BS) k_spmv_dot(int n, const int* rs, const int
     if (threadIdx.x == 0) Pt.sq[blockIdx.x] = t;
 }
 
+template <typename T, int LPR, int U>
+__global__ void __launch_bounds__(BS) k_spmv_u_dot(int n, const int* rs, const int* col, const T* A, const T* s,
+                                                   T* q, Parts Pt, const double* scal) {
+    if (scal[S_DONE] != 0.0) return;
+    const int sub = threadIdx.x & (LPR - 1);
+    double v = 0.0;
+    const int rowsPerIter = gridDim.x * (BS / LPR);
+    for (int base = blockIdx.x * (BS / LPR); base < n; base += U * rowsPerIter) {
+        int i[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int ii = base + threadIdx.x / LPR + u * rowsPerIter;
+            i[u] = ii < n ? ii : -1;
+        }
+        T acc[U];
+        row_dot_u<T, LPR, U>(rs, col, A, s, i, sub, acc);
+        if (sub == 0)
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (i[u] >= 0) {
+                    q[i[u]] = acc[u];
+                    v += (double)s[i[u]] * (double)acc[u];
+                }
+    }
+    const double t = block_reduce<false>(v);
+    if (threadIdx.x == 0) Pt.sq[blockIdx.x] = t;
+}
+
 // x accumulates in X (FP64 in the mixed form: the FP32 iteration's steps are
 // summed without FP32 rounding).
 template <typename T, typename X>
@@ -795,6 +823,8 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
     // 4 lanes per row: 16 rows per wavefront, 2 entries per lane on average
     spmvLpr_ = 4;
     if (const char* g = getenv("ASTG_SPMV_LPR")) spmvLpr_ = (atoi(g) == 8 || atoi(g) == 16) ? atoi(g) : 4;
+    spmvRows_ = 2;
+    if (const char* g = getenv("ASTG_SPMV_ROWS")) spmvRows_ = atoi(g) == 4 ? 4 : atoi(g) == 1 ? 1 : 2;
     // mixed form: residual replacement period and inner tolerance factor
     if (const char* g = getenv("ASTG_MIXED_REPLACE")) mixedReplace_ = std::max(0, atoi(g));
     if (const char* g = getenv("ASTG_MIXED_ETA")) mixedEta_ = std::max(0.01, atof(g));
@@ -1000,7 +1030,11 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
 template <typename T>
 void PcgDevice::launchSpmv(int n, const int* rs, const int* col, const T* A, const T* s, T* q, const Parts& Pt,
                            const double* scal) {
-    if (spmvLpr_ == 4)
+    if (spmvLpr_ == 4 && spmvRows_ == 4)
+        k_spmv_u_dot<T, 4, 4><<<gs_, BS, 0, stream_>>>(n, rs, col, A, s, q, Pt, scal);
+    else if (spmvLpr_ == 4 && spmvRows_ == 2)
+        k_spmv_u_dot<T, 4, 2><<<gs_, BS, 0, stream_>>>(n, rs, col, A, s, q, Pt, scal);
+    else if (spmvLpr_ == 4)
         k_spmv_dot<T, 4><<<gs_, BS, 0, stream_>>>(n, rs, col, A, s, q, Pt, scal);
     else if (spmvLpr_ == 16)
         k_spmv_dot<T, 16><<<gs_, BS, 0, stream_>>>(n, rs, col, A, s, q, Pt, scal);

@@ -145,13 +145,17 @@ __global__ void __launch_bounds__(BS) kb_spmv_dot(BatchView V, const int* rs, co
     for (int c = blockIdx.x; c < V.nchunks; c += gridDim.x) {
         if (!chunk_live(V, c)) continue;
         double v = 0.0;
-        for (int j = 0; j < LPR; ++j) {
-            const int i = c * BS + j * (BS / LPR) + threadIdx.x / LPR;
-            const T acc = row_dot<T, LPR>(rs, col, A, s, i, sub);
-            if (sub == 0) {
-                q[i] = acc;
-                v += (double)s[i] * (double)acc;
-            }
+        // two of the lane group's rows at a time (row_dot_u), in row order
+        for (int j = 0; j < LPR; j += 2) {
+            const int lane = (int)threadIdx.x / LPR;
+            const int i[2] = {c * BS + j * (BS / LPR) + lane, c * BS + (j + 1) * (BS / LPR) + lane};
+            T acc[2];
+            row_dot_u<T, LPR, 2>(rs, col, A, s, i, sub, acc);
+            if (sub == 0)
+                for (int u = 0; u < 2; ++u) {
+                    q[i[u]] = acc[u];
+                    v += (double)s[i[u]] * (double)acc[u];
+                }
         }
         const double t = block_reduce<false>(v);
         if (threadIdx.x == 0) V.part[c] = t;

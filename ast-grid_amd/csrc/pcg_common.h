@@ -60,4 +60,46 @@ __device__ inline T row_dot(const int* rs, const int* col, const T* A, const T* 
     return acc;
 }
 
+// The same product with more loads in flight: a lane group holds U rows at
+// once and each lane's first four entries (rows of up to 4 * LPR entries; the
+// octree's hold at most 15) are loaded without a loop, so the index, value and
+// gather loads of all U rows issue before any is used. Each lane sums its
+// entries in the same order as row_dot, so the results are bit-identical.
+template <typename T, int LPR, int U>
+__device__ inline void row_dot_u(const int* rs, const int* col, const T* A, const T* s, const int* i, int sub,
+                                 T* out) {
+    int p[U], e[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        p[u] = i[u] >= 0 ? rs[i[u]] + sub : 0;
+        e[u] = i[u] >= 0 ? rs[i[u] + 1] : 0;
+    }
+    T a[U][4];
+    int c[U][4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int q = p[u] + k * LPR;
+            c[u][k] = q < e[u] ? col[q] : -1;
+            a[u][k] = q < e[u] ? A[q] : (T)0;
+        }
+    T x[U][4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int u = 0; u < U; ++u) x[u][k] = c[u][k] >= 0 ? s[c[u][k]] : (T)0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        T acc = (T)0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (c[u][k] >= 0) acc += a[u][k] * x[u][k];
+        for (int q = p[u] + 4 * LPR; q < e[u]; q += LPR) acc += A[q] * s[col[q]];
+#pragma unroll
+        for (int o = LPR / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, LPR);
+        out[u] = acc;
+    }
+}
+
 }  // namespace astg

@@ -148,15 +148,20 @@ def batched_timing(seed=0, count=24, tune=False):
     """Batched solves: `count` independent octree systems (leaf counts
     log-uniform in [5e4, 2e6]) solved by one astg_pcg_solve_batch call, FP64
     and mixed, against the same systems solved one after another (each its own
-    factor + solve through the single-system path). Device times (factor +
-    solve), second call of each (the first builds the stacked pattern)."""
+    factor + solve through the single-system path, in the reference numbering
+    the batch keeps; the mixed form's block-colour single solves are timed
+    too). Device times (factor + solve), second call of each (the first builds
+    the stacked pattern)."""
     import numpy as np
     import astg
     from astg import scenes
-    prm = {p: astg.default_params(tol=1e-6, max_iterations=3000, precision=p)
+    # single solves in the reference numbering, the ordering the batch keeps
+    prm = {p: astg.default_params(tol=1e-6, max_iterations=3000, precision=p, ordering=astg.ORDER_REFERENCE)
            for p in (astg.PREC_FP64, astg.PREC_MIXED)}
+    prm_blk = astg.default_params(tol=1e-6, max_iterations=3000, precision=astg.PREC_MIXED)
     t0 = time.perf_counter()
     systems, leaves, seq, single_its = [], [], {p: 0.0 for p in prm}, {p: [] for p in prm}
+    seq_blk = 0.0
     for k, target in enumerate(scenes.batch_targets(seed, count)):
         ctx, info = scenes.batch_scene(seed, k, target)
         ctx.snapshot(restore=False)
@@ -167,6 +172,12 @@ def batched_timing(seed=0, count=24, tune=False):
             r = ctx.project(info["dt"], info["rho"], prm[p])
             seq[p] += r.ms_factor + r.ms_solve
             single_its[p].append(r.iterations)
+        for _ in range(2):  # the mixed form's default ordering (block colours), second call timed
+            ctx.snapshot(restore=True)
+            rb = ctx.project(info["dt"], info["rho"], prm_blk)
+        seq_blk += rb.ms_factor + rb.ms_solve
+        ctx.snapshot(restore=True)
+        r = ctx.project(info["dt"], info["rho"], prm[astg.PREC_FP64])  # the system handed to the batch
         rs, col, val, b, _ = ctx.system(r.n, r.nnz)
         systems.append((rs, col, val, b))
         leaves.append(info["leaves"])
@@ -197,6 +208,7 @@ def batched_timing(seed=0, count=24, tune=False):
                      "max_iteration_diff": int(max(abs(a - b) for a, b in zip(its, single_its[p]))),
                      "all_converged": bool(all(r.converged for r in res)), "depth": res[0].depth,
                      "first_call_s": round(first_s, 2)}
+    out["mixed"]["ms_sequential_block_colour"] = round(seq_blk, 3)
     c.close()
     return out
 

@@ -66,3 +66,46 @@ def test_full_size_pressure_matches_reference_solver():
     assert res_mx <= 1e-6
     assert e_mx <= 1.5 * e_ref
     ctx.close()
+
+
+def test_full_size_assembly_matches_reference():
+    """The reference's own numbering and assembly at the full size: the same
+    256^3 scene (split map, opened level-0 tilts, FP32-drawn face velocities)
+    built in the reference on the host and projected with no PCG iteration
+    (updateIndexParallel, A setup, the factorisation; octree_omnigrid_solver3.cpp
+    292-347, 350-652). Numbering and CSR pattern bit-exact, A and b within
+    1e-12 of their largest magnitude."""
+    from oracle import ref
+    if not ref.available():
+        pytest.skip("reference build (oracle/_ref) not present")
+    import astg
+    from astg import scenes
+    ctx, info = scenes.projection_256(device=0, seed=0)
+    R, L = info["R"], info["levels"]
+    r = ctx.project(info["dt"], info["rho"], astg.default_params(tol=1e-6, max_iterations=3000))
+    rs, col, val, b, _ = ctx.system(r.n, r.nnz)
+    nums = [ctx.numbering(l)[:2] for l in range(L)]
+    sizes = [ctx.level_info(l)[0] for l in range(L)]
+    ctx.close()
+    split, spheres = scenes.projection_256_split(info["seed"], R, L)
+    rf = ref.RefSolver(split, R, L, scenes.CLOSED_XY)
+    del split
+    assert [rf.level_size(l, ghost=True) for l in range(L)] == sizes
+    xyz, _ = rf.nodes(0)
+    rf.set_tilt_open(0, scenes.inside_mask(xyz[:rf.level_size(0, ghost=False)], R, spheres))
+    for l, (mac, til) in enumerate(scenes.velocity_frames(sizes, info["seed"])):
+        rf.set_velocity(l, mac, til)
+    print("reference scene built (n = %d device unknowns)" % r.n, flush=True)
+    rf.project(info["dt"], tol=1e-6, maxit=0)
+    for l in range(L):
+        o, t = rf.pindex(l)
+        assert np.array_equal(o, nums[l][0]), "octagon numbering, level %d" % l
+        assert np.array_equal(t, nums[l][1]), "tilt numbering, level %d" % l
+    rrs, rcol, rval, rb, _ = rf.system()
+    rf.close()
+    assert np.array_equal(rs, rrs) and np.array_equal(col, rcol), "CSR pattern"
+    ea = np.max(np.abs(val - rval)) / np.max(np.abs(rval))
+    eb = np.max(np.abs(b - rb)) / np.max(np.abs(rb))
+    print("n = %d, nnz = %d: A within %.2e, b within %.2e of their largest magnitude" % (len(b), len(col), ea, eb),
+          flush=True)
+    assert ea <= 1e-12 and eb <= 1e-12
