@@ -39,7 +39,8 @@ EXPORTS = [
     "astg_set_scalar", "astg_get_scalar", "astg_post_project", "astg_scalar_update_ghost", "astg_get_dual",
     "astg_sample_points", "astg_sample_points_noalloc", "astg_apply_sphere_source", "astg_add_force", "astg_adjust_e",
     "astg_set_tilt_adaptivity", "astg_advect", "astg_step",
-    "astg_set_boundary_sdf", "astg_sphere_sdf", "astg_wipe_boundary",
+    "astg_set_boundary_sdf", "astg_sphere_sdf", "astg_wipe_boundary", "astg_split_distance",
+    "astg_apply_sphere_source_velocity",
     "astg_field_layout", "astg_fields_upload", "astg_fields_download", "astg_project_fields",
     "astg_diff_state_size", "astg_diff_get_state", "astg_diff_set_state", "astg_diff_step",
     "astg_diff_step_adjoint", "astg_diff_apply",
@@ -152,6 +153,8 @@ def lib():
         "astg_set_boundary_sdf": (I, [P, P, P]),
         "astg_sphere_sdf": (I, [I, D, D, D, D, D, P]),
         "astg_wipe_boundary": (I, [P]),
+        "astg_split_distance": (I, [I, I, I, P, P, P]),
+        "astg_apply_sphere_source_velocity": (I, [P, D, D, D, D, D, D, P, I]),
         "astg_field_layout": (I, [P, P, P]),
         "astg_fields_upload": (I, [P, ctypes.POINTER(Fields)]),
         "astg_fields_download": (I, [P, ctypes.POINTER(Fields)]),
@@ -192,6 +195,22 @@ def split_spheres(R, levels, spheres, band):
     rc = lib().astg_split_spheres(R, levels, len(sph), _ptr(sph), _ptr(band), _ptr(out))
     if rc:
         raise AstgError(rc, "split_spheres")
+    return out
+
+
+def split_distance(R, levels, dist, band, shift=0):
+    """Split map (R^3 int32, x fastest) refined where the distance field `dist`
+    lies within band[l-1] of a level-l leaf, then balanced. dist holds
+    (R >> shift)^3 values at the centres of cells 2^shift finest cells wide."""
+    d = np.ascontiguousarray(np.asarray(dist, np.float32).reshape(-1))
+    dr = R >> shift
+    if d.size != dr ** 3:
+        raise ValueError("dist must hold (R >> shift)^3 values")
+    band = np.ascontiguousarray(np.asarray(band, np.float64))
+    out = np.empty(R * R * R, np.int32)
+    rc = lib().astg_split_distance(R, levels, shift, _ptr(d), _ptr(band), _ptr(out))
+    if rc:
+        raise AstgError(rc, "split_distance")
     return out
 
 
@@ -476,11 +495,19 @@ class Context:
         self._chk(f(self.h, what, len(xyz), _ptr(xyz), dt, _ptr(out)), "sample_points")
         return out
 
-    def apply_sphere_source(self, center, radius, density=1.0, heat=1.0, speed=1.0, update_ghost=False):
-        """The octree smoke example's sphere source (density, heat, +z inflow)."""
+    def apply_sphere_source(self, center, radius, density=1.0, heat=1.0, speed=1.0, update_ghost=False,
+                            velocity=None):
+        """The octree smoke example's sphere source (density, heat, +z inflow at
+        `speed`); with `velocity` (3 values) that inflow vector instead."""
         cx, cy, cz = center
-        self._chk(self._L.astg_apply_sphere_source(self.h, cx, cy, cz, radius, density, heat, speed,
-                                                   1 if update_ghost else 0), "apply_sphere_source")
+        if velocity is None:
+            rc = self._L.astg_apply_sphere_source(self.h, cx, cy, cz, radius, density, heat, speed,
+                                                  1 if update_ghost else 0)
+        else:
+            v = np.ascontiguousarray(np.asarray(velocity, np.float64).reshape(3))
+            rc = self._L.astg_apply_sphere_source_velocity(self.h, cx, cy, cz, radius, density, heat, _ptr(v),
+                                                           1 if update_ghost else 0)
+        self._chk(rc, "apply_sphere_source")
 
     def add_force(self, dt, smoke_density_factor=-0.000625, temperature_factor=5.0):
         """addForce: buoyancy along +z (the reference's default factors)."""

@@ -19,6 +19,10 @@ a scene is a pure function of (name, seed).
   projection_256 spheres with radii x 1.2 and bands 1.2 * 24 * 2^l / 512
   (seed 0: 18.0 M leaves), level-0 tilts inside the spheres opened, N(0,1)
   FP32 face velocities; closed x and y walls, open z walls; dt = 0.5/512.
+* ``full_scale_run``: the adaptive full-scale configuration at a chosen size:
+  two sources r=0.05 at (0.3,0.5,0.1) and (0.7,0.5,0.1) with lateral inflow
+  +-0.5 in x, buoyancy along +z, tilt adaptivity, the layout rebuilt from the
+  density field every few steps (fields carried over by the device samplers).
 * ``smoke_64``: the plain smoke configuration: 64^3, 3 levels, one source
   sphere r=0.08 rising along +z, the reference's buoyancy axis
   (addForce, octree_omnigrid_solver3.cpp:917-972): centre (0.5,0.5,0.12);
@@ -29,7 +33,7 @@ a scene is a pure function of (name, seed).
 """
 import numpy as np
 
-from . import Context, split_spheres, WALL_XNEG, WALL_XPOS, WALL_YNEG, WALL_YPOS, WALL_ZNEG, WALL_ZPOS
+from . import Context, split_distance, split_spheres, WALL_XNEG, WALL_XPOS, WALL_YNEG, WALL_YPOS, WALL_ZNEG, WALL_ZPOS
 
 CLOSED_XY = WALL_XNEG | WALL_XPOS | WALL_YNEG | WALL_YPOS
 
@@ -234,3 +238,209 @@ def diff_smoke(device=0, R=128, levels=3, seed=0):
     info = {"R": R, "levels": levels, "dt": 0.5 / R, "rho": 2.0, "G": G, "target": target, "leaf_cells": leaf,
             "u0": u0, "centres": centres, "leaves": int(sum(ctx.level_info(l)[1] for l in range(levels)))}
     return ctx, info
+
+
+# ---- the full-scale adaptive configuration ---------------------------------
+# Two sources of radius 0.05 near the floor with lateral inflow toward the
+# middle (+-0.5 in x), buoyancy along +z (the reference's axis; addForce
+# 917-972), closed x and y walls, open z walls, tilt adaptivity on. The layout
+# is rebuilt from the density field every `rebuild` steps: a build-only
+# extension (the reference builds its layout once, initData 68).
+FULL_SOURCES = ((0.3, 0.5, 0.1, 0.05, (0.5, 0.0, 0.0)), (0.7, 0.5, 0.1, 0.05, (-0.5, 0.0, 0.0)))
+
+# HALF_TILT_SLOT_INDEX rows (constants.h): the 4 tilt slots and the face slot
+# of a half-tilt for each half direction
+_HALF_SLOT = ((0, 3, 4, 7, 1), (0, 1, 4, 5, 2), (0, 1, 2, 3, 5), (1, 2, 5, 6, 0), (2, 3, 6, 7, 4), (4, 5, 6, 7, 3))
+_TB_HALF = 32
+
+
+# Refinement bands of the full-scale layouts, in domain units: the projection
+# scenes' widths (BAND_256) and 0.6 for a fifth level. The same layout at every
+# R: a reduced run is the full-scale configuration at a coarser resolution.
+# At 512^3 and 5 levels the initial layout (the two sources) has 4.5 M leaves.
+FULL_BANDS = BAND_256 + (0.6,)
+
+
+def _bands(R, levels):
+    return list(FULL_BANDS[:levels - 1])
+
+
+def full_scale_initial_split(R, levels):
+    sph = np.array([[cx, cy, cz, r] for cx, cy, cz, r, _ in FULL_SOURCES])
+    return split_spheres(R, levels, sph.ravel(), _bands(R, levels))
+
+
+def _tilt_dirs():
+    s = 1.0 / np.sqrt(3.0)
+    k = np.arange(8)
+    k4 = k & 3
+    up = np.where(k < 4, 1.0, -1.0)
+    return np.stack([np.where((k4 == 0) | (k4 == 3), up, -up) * s, np.where(k4 < 2, up, -up) * s,
+                     np.full(8, s)], axis=1)
+
+
+def _tilt_offsets():
+    """positionTilt's offset signs (octree_omni_tilt_grid3.h:33-40), per slot."""
+    k = np.arange(8)
+    k4 = k & 3
+    return np.stack([np.where((k4 == 0) | (k4 == 3), 1.0, -1.0), np.where(k4 < 2, 1.0, -1.0),
+                     np.where(k < 4, 1.0, -1.0)], axis=1)
+
+
+def _sample(ctx, what, pts):
+    v = ctx.sample_points(what, pts)
+    if not np.isfinite(v).all():
+        raise RuntimeError("sampled field holds %d non-finite values" % int((~np.isfinite(v)).sum()))
+    return v
+
+
+def transfer_fields(old, new, R, levels):
+    """Carries velocity, density and heat from `old` to `new` (another layout of
+    the same domain) by sampling the old context's samplers at the new
+    context's degrees of freedom: MAC faces (the omni velocity's axis
+    component), tilt slots (its projection on the slot direction, at the slot
+    position), half-tilt face slots, cell centres and vertices."""
+    import astg
+    dirs, offs = _tilt_dirs(), _tilt_offsets()
+    for l in range(levels):
+        xyz, _ = new.nodes(l)
+        nn = len(xyz)
+        h = 1.0 / (R >> l)
+        base = xyz.astype(np.float64) * h
+        mac = np.empty((nn, 3))
+        for a in range(3):
+            p = base + 0.5 * h
+            p[:, a] = base[:, a]
+            mac[:, a] = _sample(old, astg.SAMPLE_VELOCITY, p)[:, a]
+        e, bits, hd = new.tilt(l)
+        til = np.empty((nn, 8))
+        for k in range(8):
+            p = base + offs[k][None, :] * (e[:, None] / 3.0)
+            til[:, k] = _sample(old, astg.SAMPLE_VELOCITY, p) @ dirs[k]
+        half = np.flatnonzero((bits & _TB_HALF) != 0)
+        if len(half):
+            # a half-tilt keeps its 4 tilt slots and its face slot (the velocity's
+            # component along the face's axis); its other slots are zero
+            vv = _sample(old, astg.SAMPLE_VELOCITY, base[half])
+            d = hd[half].astype(np.int64)
+            rows = np.flatnonzero(d >= 0)
+            slots = np.asarray(_HALF_SLOT)[d[rows]]
+            kf = slots[:, 4]
+            ax = np.where(kf <= 1, 0, np.where((kf == 2) | (kf == 4), 1, 2))
+            sub = til[half]
+            keep = np.zeros((len(half), 8), bool)
+            keep[rows[:, None], slots[:, :4]] = True
+            keep[rows, kf] = True
+            sub[rows, kf] = vv[rows, ax]
+            sub[~keep] = 0.0
+            til[half] = sub
+        new.set_velocity(l, mac, til)
+        for f, what in ((astg.FIELD_DENSITY, astg.SAMPLE_DENSITY), (astg.FIELD_HEAT, astg.SAMPLE_HEAT)):
+            new.set_scalar(f, l, _sample(old, what, base + 0.5 * h), _sample(old, what, base))
+    new.scalar_update_ghost(astg.FIELD_DENSITY)
+    new.scalar_update_ghost(astg.FIELD_HEAT)
+
+
+def density_split(ctx, R, levels, threshold=0.01, lattice=256):
+    """A new split map from the density field: finest where the density at a
+    cell centre of a lattice at most `lattice` cells wide (the finest cells,
+    or blocks of 2^s of them above that) exceeds `threshold` or a source sits,
+    graded outward with the smoke layouts' bands (astg_split_distance)."""
+    import astg
+    from scipy import ndimage
+    shift = 0
+    while (R >> shift) > lattice and shift + 1 < levels:
+        shift += 1
+    n = R >> shift
+    c = (np.arange(n) + 0.5) / n
+    Z, Y, X = np.meshgrid(c, c, c, indexing="ij")
+    pts = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    del X, Y, Z
+    d = _sample(ctx, astg.SAMPLE_DENSITY, pts)
+    region = d > threshold
+    for cx, cy, cz, r, _ in FULL_SOURCES:
+        region |= ((pts[:, 0] - cx) ** 2 + (pts[:, 1] - cy) ** 2 + (pts[:, 2] - cz) ** 2) < r * r
+    del pts
+    region = region.reshape(n, n, n)
+    dist = ndimage.distance_transform_edt(~region) / n  # z, y, x order: x fastest in ravel
+    return split_distance(R, levels, dist.astype(np.float32).ravel(), _bands(R, levels), shift)
+
+
+def full_scale_run(R=128, levels=4, steps=100, rebuild=10, precision=0, tol=1e-6, lattice=32, device=0,
+                   splits=None, log=None):
+    """The adaptive full-scale configuration: `steps` steps from rest, each the
+    two sources, addForce and onUpdate's chain (projection in `precision` to
+    `tol`, tilt adaptivity), the layout rebuilt from the density every
+    `rebuild` steps (density_split), or from `splits[i]` at the i-th rebuild
+    when given, so that a second run can replay the first one's layouts.
+    Returns per-step wall and projection times, rebuild times, PCG
+    iterations, unknowns, the split maps used and, every `rebuild` steps
+    (before the rebuild), the density, heat and velocity sampled at the
+    centres of a lattice^3 grid."""
+    import time
+    import astg
+    import torch
+    dt, rho = 0.5 / R, 2.0
+    prm = astg.default_params(tol=tol, max_iterations=3000, precision=precision)
+    split = full_scale_initial_split(R, levels)
+    ctx = Context(device)
+    ctx.set_topology(split, R, levels, CLOSED_XY)
+    c = (np.arange(lattice) + 0.5) / lattice
+    Z, Y, X = np.meshgrid(c, c, c, indexing="ij")
+    probe = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    out = {"step_ms": [], "project_ms": [], "rebuild_ms": [], "iterations": [], "snapshots": [], "unknowns": [],
+           "splits": []}
+    first = True
+    for k in range(1, steps + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for cx, cy, cz, r, v in FULL_SOURCES:
+            ctx.apply_sphere_source((cx, cy, cz), r, 1.0, 1.0, update_ghost=first, velocity=v)
+        first = False
+        ctx.add_force(dt, 1.0, 0.0)
+        ctx.set_tilt_adaptivity(True)
+        res = ctx.step(dt, rho, prm)
+        torch.cuda.synchronize()
+        out["step_ms"].append((time.perf_counter() - t0) * 1e3)
+        out["project_ms"].append(res.ms_total)
+        out["iterations"].append(res.iterations)
+        out["unknowns"].append(res.n)
+        if k % rebuild == 0:
+            out["snapshots"].append({"step": k, "density": _sample(ctx, astg.SAMPLE_DENSITY, probe),
+                                     "heat": _sample(ctx, astg.SAMPLE_HEAT, probe),
+                                     "velocity": _sample(ctx, astg.SAMPLE_VELOCITY, probe)})
+            if k < steps:
+                t0 = time.perf_counter()
+                i = len(out["splits"])
+                nsplit = splits[i] if splits is not None else density_split(ctx, R, levels)
+                out["splits"].append(np.asarray(nsplit, np.uint8))  # levels < 16: a byte per cell
+                new = Context(device)
+                new.set_topology(nsplit, R, levels, CLOSED_XY)
+                transfer_fields(ctx, new, R, levels)
+                ctx.close()
+                ctx = new
+                torch.cuda.synchronize()
+                out["rebuild_ms"].append((time.perf_counter() - t0) * 1e3)
+                if log:
+                    log("step %d: layout rebuilt (%.0f ms), %d unknowns before" % (k, out["rebuild_ms"][-1], res.n))
+    ctx.close()
+    return out
+
+
+def full_scale_compare(R=128, levels=4, steps=100, rebuild=10, tol=1e-6, lattice=32, device=0, log=None):
+    """The full-scale run in FP64, then in mixed precision on the FP64 run's
+    layouts. Per snapshot and field: the mixed run's relative L2 error and
+    largest absolute difference against the FP64 run at the lattice points."""
+    import astg
+    ref = full_scale_run(R, levels, steps, rebuild, astg.PREC_FP64, tol, lattice, device, log=log)
+    mix = full_scale_run(R, levels, steps, rebuild, astg.PREC_MIXED, tol, lattice, device, splits=ref["splits"],
+                         log=log)
+    errors = []
+    for a, b in zip(mix["snapshots"], ref["snapshots"]):
+        e = {"step": b["step"]}
+        for f in ("density", "heat", "velocity"):
+            d = a[f] - b[f]
+            e[f] = {"rel_l2": float(np.linalg.norm(d) / max(np.linalg.norm(b[f]), 1e-300)),
+                    "max_abs": float(np.abs(d).max())}
+        errors.append(e)
+    return {"fp64": ref, "mixed": mix, "errors": errors}

@@ -111,6 +111,10 @@ public:
     // the scalar ghosts afterwards (the example does on its first call).
     int applySource(ProjectDevice& pd, double cx, double cy, double cz, double r, double dval, double hval,
                     double speed, bool ghost, std::string& err);
+    // The same source with an inflow vector v (3 doubles) on every MAC face and
+    // tilt slot inside the sphere; v = null is applySource with `speed`.
+    int applySourceV(ProjectDevice& pd, double cx, double cy, double cz, double r, double dval, double hval,
+                     const double* v, double speed, bool ghost, std::string& err);
     // addForce (octree_omnigrid_solver3.cpp:917-972): buoyancy along +z.
     int addForce(ProjectDevice& pd, double dt, double fd, double ft, std::string& err);
     // wipeBoundary (octree_omnigrid_solver3.cpp:1240-1292): density and heat
@@ -182,7 +186,6 @@ private:
     DevBuf scal_[2][2][2];           // [field][buffer][0 octagon, 1 tilt]
     int cur_[2] = {0, 0};            // current buffer of each scalar field
     DevBuf dualOct_, dualTilt_;      // 3 per node
-    DevBuf tMac_, tTilt_;            // advected velocity (swapped with the live one)
     DevBuf pts_, out_;               // point queries
     DevBuf sampTe_, sampTb_, sampThd_, open_;  // tilt state before an update, decisions
     bool sampValid_ = false;

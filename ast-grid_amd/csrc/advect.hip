@@ -1388,9 +1388,13 @@ __device__ inline bool inSphere(double px, double py, double pz, double cx, doub
 // parameters: inside the sphere, density and heat on cell and vertex DOFs, +z
 // velocity on MAC z-faces, and speed/sqrt(3) on all 8 slots of every leaf tilt
 // whose centre is inside.
+// VEC = false: the example's source, rising at `speed` (v = (0, 0, speed):
+// z-faces and tilt slots only). VEC = true: an inflow vector v on every MAC
+// face and tilt slot inside the sphere.
+template <bool VEC>
 __global__ void k_source(Topo T, const uint8_t* level, const int* leaf, int n, double cx, double cy, double cz,
-                         double r, double dval, double hval, double speed, double* dC, double* dV, double* hC,
-                         double* hV, double* mac, double* tilt) {
+                         double r, double dval, double hval, double vx, double vy, double vz, double* dC, double* dV,
+                         double* hC, double* hV, double* mac, double* tilt) {
     const int t = blockIdx.x * BS + threadIdx.x;
     if (t >= n) return;
     const int g = leaf[t];
@@ -1405,15 +1409,30 @@ __global__ void k_source(Topo T, const uint8_t* level, const int* leaf, int n, d
     if (inSphere(X, Y, Z, cx, cy, cz, r)) {
         dV[g] = dval;
         hV[g] = hval;
-        for (int k = 0; k < 8; ++k) tilt[(size_t)k * c_G + g] = speed / kSqrt3;
+        for (int k = 0; k < 8; ++k) {
+            if (VEC) {
+                double d[3];
+                tiltDir(k, d);
+                tilt[(size_t)k * c_G + g] = vx * d[0] + vy * d[1] + vz * d[2];
+            } else {
+                tilt[(size_t)k * c_G + g] = vz / kSqrt3;
+            }
+        }
     }
-    if (inSphere(X + 0.5 * h, Y + 0.5 * h, Z, cx, cy, cz, r)) mac[2 * (size_t)c_G + g] = speed;
+    if (VEC) {
+        if (inSphere(X, Y + 0.5 * h, Z + 0.5 * h, cx, cy, cz, r)) mac[g] = vx;
+        if (inSphere(X + 0.5 * h, Y, Z + 0.5 * h, cx, cy, cz, r)) mac[(size_t)c_G + g] = vy;
+    }
+    if (inSphere(X + 0.5 * h, Y + 0.5 * h, Z, cx, cy, cz, r)) mac[2 * (size_t)c_G + g] = vz;
 }
 
 // The same source on half-tilts: density and heat at the vertex, and the four
-// tilt slots of the half-tilt (plus its face slot when the face is a z-face).
+// tilt slots of the half-tilt (plus its face slot: for the rising source only
+// when the face is a z-face).
+template <bool VEC>
 __global__ void k_source_half(Topo T, const uint8_t* level, const int* half, int n, double cx, double cy, double cz,
-                              double r, double dval, double hval, double speed, double* dV, double* hV, double* tilt) {
+                              double r, double dval, double hval, double vx, double vy, double vz, double* dV,
+                              double* hV, double* tilt) {
     const int t = blockIdx.x * BS + threadIdx.x;
     if (t >= n) return;
     const int g = half[t];
@@ -1425,8 +1444,22 @@ __global__ void k_source_half(Topo T, const uint8_t* level, const int* half, int
     hV[g] = hval;
     const int dir = T.thd[g];
     double nv[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int i = 0; i < 4; ++i) nv[c_halfSlot[dir][i]] = speed / kSqrt3;
-    if (dir % 3 == 2) nv[c_halfSlot[dir][4]] = speed;
+    for (int i = 0; i < 4; ++i) {
+        const int k = c_halfSlot[dir][i];
+        if (VEC) {
+            double d[3];
+            tiltDir(k, d);
+            nv[k] = vx * d[0] + vy * d[1] + vz * d[2];
+        } else {
+            nv[k] = vz / kSqrt3;
+        }
+    }
+    if (VEC) {
+        const int ax = dir % 3;
+        nv[c_halfSlot[dir][4]] = ax == 0 ? vx : ax == 1 ? vy : vz;
+    } else if (dir % 3 == 2) {
+        nv[c_halfSlot[dir][4]] = vz;
+    }
     for (int k = 0; k < 8; ++k) tilt[(size_t)k * c_G + g] = nv[k];
 }
 
@@ -2052,10 +2085,6 @@ int AdvectDevice::upload(const Octree& o, const ProjectTables& t, bool keepField
     HIPCHK(dualTilt_.alloc((size_t)G_ * 24));
     HIPCHK(hipMemsetAsync(dualOct_.p, 0, (size_t)G_ * 24, stream_));
     HIPCHK(hipMemsetAsync(dualTilt_.p, 0, (size_t)G_ * 24, stream_));
-    HIPCHK(tMac_.alloc((size_t)G_ * 24));
-    HIPCHK(tTilt_.alloc((size_t)G_ * 64));
-    HIPCHK(hipMemsetAsync(tMac_.p, 0, (size_t)G_ * 24, stream_));
-    HIPCHK(hipMemsetAsync(tTilt_.p, 0, (size_t)G_ * 64, stream_));
     HIPCHK(bad_.alloc(4));
     HIPCHK(hipStreamSynchronize(stream_));
     return ASTG_OK;
@@ -2424,15 +2453,15 @@ int AdvectDevice::advect(ProjectDevice& pd, double dt, std::string& err) {
     if (rc) return rc;
     // velocity (advection.h:195-239) into the temporaries, then swap
     if (nLeaf_)
-        k_adv_mac<<<nblocks(3L * nLeaf_), BS, 0, stream_>>>(T, lev, V, leaf_.as<int>(), nLeaf_, dt, tMac_.as<double>());
+        k_adv_mac<<<nblocks(3L * nLeaf_), BS, 0, stream_>>>(T, lev, V, leaf_.as<int>(), nLeaf_, dt, pd.tMac());
     if (nVelTilt_)
         k_adv_tilt<<<nblocks(8L * nVelTilt_), BS, 0, stream_>>>(T, te_.as<double>(), lev, V, velTilt_.as<int>(),
-                                                                 nVelTilt_, dt, tTilt_.as<double>());
+                                                                 nVelTilt_, dt, pd.tTilt());
     if (nHalf_)
         k_adv_half<<<nblocks(5L * nHalf_), BS, 0, stream_>>>(T, te_.as<double>(), lev, V, half_.as<int>(), nHalf_, dt,
-                                                         tTilt_.as<double>());
+                                                         pd.tTilt());
     HIPCHK(hipGetLastError());
-    pd.swapVelocity(tMac_, tTilt_);
+    pd.swapVelocity();
     // decay (density->scale, heat->scale)
     const long G = G_;
     k_scale2<<<nblocks(G), BS, 0, stream_>>>(G, 0.999, scal_[0][cur_[0]][0].as<double>(),
@@ -2594,8 +2623,8 @@ int AdvectDevice::advectCarry(ProjectDevice& pd, double dt, std::string& err) {
         for (int k = 0; k < 2; ++k)
             HIPCHK(hipMemcpyAsync(scal_[f][1 - cur_[f]][k].p, scal_[f][cur_[f]][k].p, b, hipMemcpyDeviceToDevice,
                                   stream_));
-    HIPCHK(hipMemcpyAsync(tMac_.p, pd.uMac(), 3 * b, hipMemcpyDeviceToDevice, stream_));
-    HIPCHK(hipMemcpyAsync(tTilt_.p, pd.uTilt(), 8 * b, hipMemcpyDeviceToDevice, stream_));
+    HIPCHK(hipMemcpyAsync(pd.tMac(), pd.uMac(), 3 * b, hipMemcpyDeviceToDevice, stream_));
+    HIPCHK(hipMemcpyAsync(pd.tTilt(), pd.uTilt(), 8 * b, hipMemcpyDeviceToDevice, stream_));
     return advect(pd, dt, err);
 }
 
@@ -2684,8 +2713,8 @@ int AdvectDevice::mirrorTemporaries(ProjectDevice& pd, const bool scalars[2], bo
             HIPCHK(hipMemcpyAsync(scal_[f][1 - c][k].p, scal_[f][c][k].p, b, hipMemcpyDeviceToDevice, stream_));
     }
     if (velocity) {
-        HIPCHK(hipMemcpyAsync(tMac_.p, pd.uMac(), 3 * b, hipMemcpyDeviceToDevice, stream_));
-        HIPCHK(hipMemcpyAsync(tTilt_.p, pd.uTilt(), 8 * b, hipMemcpyDeviceToDevice, stream_));
+        HIPCHK(hipMemcpyAsync(pd.tMac(), pd.uMac(), 3 * b, hipMemcpyDeviceToDevice, stream_));
+        HIPCHK(hipMemcpyAsync(pd.tTilt(), pd.uTilt(), 8 * b, hipMemcpyDeviceToDevice, stream_));
     }
     HIPCHK(hipStreamSynchronize(stream_));
     return ASTG_OK;
@@ -2693,6 +2722,11 @@ int AdvectDevice::mirrorTemporaries(ProjectDevice& pd, const bool scalars[2], bo
 
 int AdvectDevice::applySource(ProjectDevice& pd, double cx, double cy, double cz, double r, double dval, double hval,
                               double speed, bool ghost, std::string& err) {
+    return applySourceV(pd, cx, cy, cz, r, dval, hval, nullptr, speed, ghost, err);
+}
+
+int AdvectDevice::applySourceV(ProjectDevice& pd, double cx, double cy, double cz, double r, double dval,
+                               double hval, const double* v, double speed, bool ghost, std::string& err) {
     std::lock_guard<std::recursive_mutex> lk_(constMutex());
     if (int rc_ = bindConstants(false, err)) return rc_;
     const Topo T = topo();
@@ -2701,12 +2735,25 @@ int AdvectDevice::applySource(ProjectDevice& pd, double cx, double cy, double cz
     double* dV = scal_[0][cur_[0]][1].as<double>();
     double* hC = scal_[1][cur_[1]][0].as<double>();
     double* hV = scal_[1][cur_[1]][1].as<double>();
-    if (nLeaf_)
-        k_source<<<nblocks(nLeaf_), BS, 0, stream_>>>(T, lev, leaf_.as<int>(), nLeaf_, cx, cy, cz, r, dval, hval, speed,
-                                                      dC, dV, hC, hV, pd.uMac(), pd.uTilt());
-    if (nHalf_)
-        k_source_half<<<nblocks(nHalf_), BS, 0, stream_>>>(T, lev, half_.as<int>(), nHalf_, cx, cy, cz, r, dval, hval,
-                                                           speed, dV, hV, pd.uTilt());
+    const double vx = v ? v[0] : 0.0, vy = v ? v[1] : 0.0, vz = v ? v[2] : speed;
+    if (nLeaf_) {
+        if (v)
+            k_source<true><<<nblocks(nLeaf_), BS, 0, stream_>>>(T, lev, leaf_.as<int>(), nLeaf_, cx, cy, cz, r, dval,
+                                                                hval, vx, vy, vz, dC, dV, hC, hV, pd.uMac(),
+                                                                pd.uTilt());
+        else
+            k_source<false><<<nblocks(nLeaf_), BS, 0, stream_>>>(T, lev, leaf_.as<int>(), nLeaf_, cx, cy, cz, r, dval,
+                                                                 hval, vx, vy, vz, dC, dV, hC, hV, pd.uMac(),
+                                                                 pd.uTilt());
+    }
+    if (nHalf_) {
+        if (v)
+            k_source_half<true><<<nblocks(nHalf_), BS, 0, stream_>>>(T, lev, half_.as<int>(), nHalf_, cx, cy, cz, r,
+                                                                     dval, hval, vx, vy, vz, dV, hV, pd.uTilt());
+        else
+            k_source_half<false><<<nblocks(nHalf_), BS, 0, stream_>>>(T, lev, half_.as<int>(), nHalf_, cx, cy, cz, r,
+                                                                      dval, hval, vx, vy, vz, dV, hV, pd.uTilt());
+    }
     HIPCHK(hipGetLastError());
     if (ghost) {
         int rc = scalarGhost(0, err);

@@ -37,6 +37,8 @@ struct astg_ctx {
     astg::BlockOrder blk;
     astg::DevBuf blkPerm, blkSrc;
     bool blkReady = false;
+    int patternSolves = 0;     // projection solves on the current pattern (ASTG_ORDER_AUTO's choice)
+    bool factoredBlk = false;  // the last factorisation was of the block-colour copy
     std::unique_ptr<astg::ProjectDevice> proj;
     std::unique_ptr<astg::AdvectDevice> adv;              // post-projection step and advection
     std::unique_ptr<astg::PcgDevice> csr;                 // CSR seam
@@ -86,6 +88,7 @@ int ensureTables(astg_ctx* c) {
         c->tablesReady = true;
         c->deviceReady = false;
         c->blkReady = false;
+        c->patternSolves = 0;
         if (c->device < 0) {
             c->hostMac.assign(c->tree->L, {});
             c->hostTilt.assign(c->tree->L, {});
@@ -107,11 +110,13 @@ int ensureTables(astg_ctx* c) {
         if (rc) return rc;
         rc = c->pcg->setPattern(c->tables.n, c->tables.rowstart.data(), c->tables.col.data(), c->err);
         if (rc) return rc;
-        // fields start at zero
+        // fields, and the advection's output buffers, start at zero
         const size_t G = c->tables.G;
         std::vector<double> z((size_t)8 * G, 0.0);
         if (hipMemcpyAsync(c->proj->uMac(), z.data(), 3 * G * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
             hipMemcpyAsync(c->proj->uTilt(), z.data(), 8 * G * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+            hipMemcpyAsync(c->proj->tMac(), z.data(), 3 * G * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+            hipMemcpyAsync(c->proj->tTilt(), z.data(), 8 * G * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
             hipStreamSynchronize(c->stream) != hipSuccess)
             return fail(c, ASTG_ERR_HIP, "field upload failed");
         c->deviceReady = true;
@@ -144,6 +149,7 @@ int retopologize(astg_ctx* c) {
 int retopologizeSteps(astg_ctx* c) {
     astg::PhaseTimer tm("retopologize");
     c->blkReady = false;  // the block-colour system follows the new numbering on its next use
+    c->patternSolves = 0;
     astg::numberUnknowns(*c->tree, c->num);
     tm.mark("numbering");
     astg::computeWeights(*c->tree, c->w);
@@ -502,10 +508,15 @@ int blockMerge() {
     return v > 0 ? v : 1;
 }
 
-bool blockOrdered(const astg_pcg_params& prm) {
+// ASTG_ORDER_AUTO: the block-colour form for the mixed precision once its
+// set-up is paid for, i.e. from the second solve on one pattern (or when the
+// copy exists already). Its set-up (permutation, a second solver pattern and
+// its plans) costs about one solve, so a pattern solved once, as in a run
+// whose tilts change every step, stays in the reference numbering.
+bool blockOrdered(const astg_ctx* c, const astg_pcg_params& prm) {
     if (prm.ordering == ASTG_ORDER_BLOCK_COLOUR) return true;
     if (prm.ordering == ASTG_ORDER_REFERENCE) return false;
-    return prm.precision == ASTG_PREC_MIXED;
+    return prm.precision == ASTG_PREC_MIXED && (c->blkReady || c->patternSolves > 0);
 }
 
 // The block-colour copy of the projection system: permutation, pattern and
@@ -543,11 +554,18 @@ int solveAssembled(astg_ctx* c, const astg_pcg_params& prm, astg_pcg_result& res
     if (prm.ordering < ASTG_ORDER_AUTO || prm.ordering > ASTG_ORDER_BLOCK_COLOUR)
         return fail(c, ASTG_ERR_ARG, "unknown ordering");
     int r;
-    if (!blockOrdered(prm)) {
-        if (refactor && (r = c->pcg->factor(prm.mic_tau, prm.min_diag_ratio, c->err))) return r;
+    // a solve on the last factor keeps that factor's ordering
+    const bool blk = refactor ? blockOrdered(c, prm) : c->factoredBlk;
+    if (!blk) {
+        if (refactor) {
+            if ((r = c->pcg->factor(prm.mic_tau, prm.min_diag_ratio, c->err))) return r;
+            c->factoredBlk = false;
+        }
         if (evFactored) (void)hipEventRecord(evFactored, c->stream);
         c->lastPcg = c->pcg.get();
-        return c->pcg->solve(prm, res, c->err);
+        if ((r = c->pcg->solve(prm, res, c->err))) return r;
+        ++c->patternSolves;
+        return ASTG_OK;
     }
     if ((r = ensureBlockOrder(c))) return r;
     astg::PcgDevice& B = *c->pcgBlk;
@@ -555,10 +573,14 @@ int solveAssembled(astg_ctx* c, const astg_pcg_params& prm, astg_pcg_result& res
     if ((r = astg::gatherPermuted(c->stream, c->pcg->nnz(), c->blkSrc.as<int>(), refactor ? c->pcg->values() : nullptr,
                                   B.values(), n, c->blkPerm.as<int>(), c->pcg->rhs(), B.rhs(), c->err)))
         return r;
-    if (refactor && (r = B.factor(prm.mic_tau, prm.min_diag_ratio, c->err))) return r;
+    if (refactor) {
+        if ((r = B.factor(prm.mic_tau, prm.min_diag_ratio, c->err))) return r;
+        c->factoredBlk = true;
+    }
     if (evFactored) (void)hipEventRecord(evFactored, c->stream);
     c->lastPcg = &B;
     if ((r = B.solve(prm, res, c->err))) return r;
+    ++c->patternSolves;
     return astg::scatterPermuted(c->stream, n, c->blkPerm.as<int>(), B.solution(), c->pcg->solution(), c->err);
 }
 
@@ -1393,6 +1415,31 @@ int astg_apply_sphere_source(astg_ctx* c, double cx, double cy, double cz, doubl
     int rc = needAdvect(c);
     if (rc) return rc;
     return c->adv->applySource(*c->proj, cx, cy, cz, radius, density, heat, speed, update_ghost != 0, c->err);
+}
+
+int astg_apply_sphere_source_velocity(astg_ctx* c, double cx, double cy, double cz, double radius, double density,
+                                      double heat, const double* velocity, int update_ghost) {
+    if (!(radius > 0)) return fail(c, ASTG_ERR_ARG, "radius must be positive");
+    if (!velocity) return fail(c, ASTG_ERR_ARG, "null velocity");
+    int rc = needAdvect(c);
+    if (rc) return rc;
+    return c->adv->applySourceV(*c->proj, cx, cy, cz, radius, density, heat, velocity, 0.0, update_ghost != 0,
+                                c->err);
+}
+
+int astg_split_distance(int R, int levels, int dist_shift, const float* dist, const double* band, int* split_out) {
+    if (R <= 0 || levels <= 0 || levels > 15 || (R >> (levels - 1)) < 2 || (R % (1 << (levels - 1))) != 0 || !dist ||
+        (levels > 1 && !band) || !split_out || dist_shift < 0 || dist_shift >= levels)
+        return ASTG_ERR_ARG;
+    try {
+        std::vector<int> split;
+        astg::splitDistance(R, levels, dist_shift, dist, band, split);
+        astg::balanceSplit(R, levels, split);
+        std::memcpy(split_out, split.data(), split.size() * sizeof(int));
+    } catch (...) {
+        return ASTG_ERR_ARG;
+    }
+    return ASTG_OK;
 }
 
 int astg_wipe_boundary(astg_ctx* c) {

@@ -26,7 +26,7 @@ enum ScalarSlot {
     S_SQ,           // s.q
     S_RES,          // max |r| (or sum r^2 for L2)
     S_TOL,          // absolute tolerance
-    S_DONE,         // 1.0 once converged or failed
+    S_DONE,         // 0 running; 1 converged, 2 breakdown, 3 iteration limit, 4 NaN or infinity
     S_ITER,         // iterations performed
     S_BNORM,        // ||b||
     S_ERR,          // nonzero: sweep spin limit hit / breakdown
@@ -85,8 +85,11 @@ __device__ inline double wave_sum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
+// max that keeps a NaN from either side (fmax drops it), so that a norm over
+// vectors holding a NaN is NaN
+__device__ inline double nan_max(double a, double b) { return (b > a || b != b) ? b : a; }
 __device__ inline double wave_max(double v) {
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    for (int o = 32; o > 0; o >>= 1) v = nan_max(v, __shfl_xor(v, o, 64));
     return v;
 }
 

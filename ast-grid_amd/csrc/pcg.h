@@ -101,7 +101,8 @@ enum BatchSlot {
     B_BNORM = 0,  // ||b|| of the system (inner right-hand side)
     B_TOL,        // absolute tolerance of the inner solve
     B_RES,        // last residual norm
-    B_DONE,       // 0 running; 1 converged, 2 breakdown, 3 iteration limit, 4 not solved this time
+    B_DONE,       // 0 running; 1 converged, 2 breakdown, 3 iteration limit, 4 not solved this time,
+                  // 5 NaN or infinity in b or the residual
     B_ITER,       // iterations performed
     B_RHO,        // z.r
     B_ALPHA,      // rho / s.q

@@ -25,6 +25,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -293,7 +294,7 @@ __global__ void __launch_bounds__(BS) k_init(int n, const T* b, X* x, T* r, Part
         x[i] = (X)0;
         r[i] = bi;
         const double a = (double)bi;
-        v = norm == ASTG_NORM_INF ? fmax(v, fabs(a)) : v + a * a;
+        v = norm == ASTG_NORM_INF ? nan_max(v, fabs(a)) : v + a * a;
     }
     const double t = norm == ASTG_NORM_INF ? block_reduce<true>(v) : block_reduce<false>(v);
     if (threadIdx.x == 0) Pt.res[blockIdx.x] = t;
@@ -308,6 +309,14 @@ __device__ inline bool fwd_prologue(const Parts& Pt, double* scal, int mode, int
                                     double tolRel) {
     double red = norm == ASTG_NORM_INF ? reduce_prev<true>(Pt.res, Pt.gs) : reduce_prev<false>(Pt.res, Pt.gs);
     if (norm != ASTG_NORM_INF) red = sqrt(red);
+    if (!(red <= DBL_MAX)) {  // a NaN or infinity in b or in the residual: the solve stops (state 4)
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            scal[S_RES] = red;
+            scal[S_ITER] = mode == 0 ? 0.0 : (double)(it + 1);
+            scal[S_DONE] = 4.0;
+        }
+        return true;
+    }
     if (mode == 0) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             scal[S_BNORM] = red;
@@ -626,7 +635,7 @@ __global__ void __launch_bounds__(BS) k_axpy_res(int n, const T* s, const T* q, 
         const T ri = r[i] - alpha * q[i];
         r[i] = ri;
         const double a = (double)ri;
-        v = norm == ASTG_NORM_INF ? fmax(v, fabs(a)) : v + a * a;
+        v = norm == ASTG_NORM_INF ? nan_max(v, fabs(a)) : v + a * a;
     }
     const double t = norm == ASTG_NORM_INF ? block_reduce<true>(v) : block_reduce<false>(v);
     if (threadIdx.x == 0) Pt.res[blockIdx.x] = t;
@@ -645,7 +654,7 @@ __global__ void __launch_bounds__(BS) k_residual64(int n, const int* rs, const i
         if (i >= 0 && sub == 0) {
             const double ri = b[i] - acc;
             r[i] = ri;
-            v = norm == ASTG_NORM_INF ? fmax(v, fabs(ri)) : v + ri * ri;
+            v = norm == ASTG_NORM_INF ? nan_max(v, fabs(ri)) : v + ri * ri;
         }
     }
     const double t = norm == ASTG_NORM_INF ? block_reduce<true>(v) : block_reduce<false>(v);
@@ -683,7 +692,7 @@ __global__ void __launch_bounds__(BS) k_replace_res(int n, const int* rs, const 
         if (i >= 0 && sub == 0) {
             const float ri = (float)(b[i] - acc);
             r[i] = ri;
-            v = norm == ASTG_NORM_INF ? fmax(v, fabs((double)ri)) : v + (double)ri * (double)ri;
+            v = norm == ASTG_NORM_INF ? nan_max(v, fabs((double)ri)) : v + (double)ri * (double)ri;
         }
     }
     const double t = norm == ASTG_NORM_INF ? block_reduce<true>(v) : block_reduce<false>(v);
@@ -1282,6 +1291,12 @@ int PcgDevice::pcgInner(const T* A, const T* invd, const T* b, double* x, double
         if (hscal_[S_DONE] != 0.0 || it >= maxIt) break;
         batch = std::min(batch * 2, 32);
     }
+    if (hscal_[S_DONE] == 4.0) {
+        const int at = (int)hscal_[S_ITER];
+        err = at == 0 ? "PCG: the right-hand side holds a NaN or infinity"
+                      : "PCG: the residual holds a NaN or infinity at iteration " + std::to_string(at);
+        return ASTG_ERR_NOT_CONVERGED;
+    }
     iters = (int)hscal_[S_ITER];
     resOut = hscal_[S_RES];
     conv = hscal_[S_DONE] == 1.0;
@@ -1351,6 +1366,12 @@ int PcgDevice::solveOnce(const astg_pcg_params& prm, astg_pcg_result& res, std::
         int rc = syncScalars(err);
         if (rc) return rc;
         rnorm = hscal_[S_OUTER_RES];
+        if (!(rnorm <= DBL_MAX)) {
+            err = outer == 0 ? "PCG: the right-hand side holds a NaN or infinity"
+                             : "PCG: the FP64 residual holds a NaN or infinity after " + std::to_string(outer) +
+                                   " inner solves";
+            return ASTG_ERR_NOT_CONVERGED;
+        }
         if (bnorm < 0) {
             bnorm = rnorm;
             target = prm.tol * bnorm;

@@ -19,6 +19,7 @@
 // kernels. The sweeps run over all rows: a finished system's residual no
 // longer changes, so its rows recompute values nobody reads.
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -64,11 +65,18 @@ __global__ void __launch_bounds__(BS) kb_reduce(BatchView V, int phase, int norm
     const bool mx = isNorm && norm == ASTG_NORM_INF;
     const int c0 = V.segChunk[k], c1 = V.segChunk[k + 1];
     double acc = 0.0;
-    for (int c = c0 + threadIdx.x; c < c1; c += BS) acc = mx ? fmax(acc, V.part[c]) : acc + V.part[c];
+    for (int c = c0 + threadIdx.x; c < c1; c += BS) acc = mx ? nan_max(acc, V.part[c]) : acc + V.part[c];
     double red = mx ? block_reduce<true>(acc) : block_reduce<false>(acc);
     if (threadIdx.x != 0) return;
     if (isNorm && !mx) red = sqrt(red);
     bool fin = false;
+    if ((phase == P_INIT || phase == P_RES) && s[B_SKIP] == 0.0 && !(red <= DBL_MAX)) {
+        s[B_RES] = red;  // a NaN or infinity in this system's b or residual: it stops (state 5)
+        if (phase == P_RES) s[B_ITER] += 1.0;
+        s[B_DONE] = 5.0;
+        if (atomicAdd(V.ndone, 1) + 1 == V.K) scal[S_DONE] = 1.0;
+        return;
+    }
     switch (phase) {
         case P_INIT:
             s[B_ITER] = 0.0;
@@ -207,7 +215,7 @@ __global__ void __launch_bounds__(BS) kb_replace_res(BatchView V, const int* rs,
             if (sub == 0) {
                 const float ri = (float)(b[i] - acc);
                 r[i] = ri;
-                v = norm == ASTG_NORM_INF ? fmax(v, fabs((double)ri)) : v + (double)ri * (double)ri;
+                v = norm == ASTG_NORM_INF ? nan_max(v, fabs((double)ri)) : v + (double)ri * (double)ri;
             }
         }
         const double t = norm_reduce(v, norm);
@@ -228,7 +236,7 @@ __global__ void __launch_bounds__(BS) kb_residual64(BatchView V, const int* rs, 
             if (sub == 0) {
                 const double ri = b[i] - acc;
                 r[i] = ri;
-                v = norm == ASTG_NORM_INF ? fmax(v, fabs(ri)) : v + ri * ri;
+                v = norm == ASTG_NORM_INF ? nan_max(v, fabs(ri)) : v + ri * ri;
             }
         }
         const double t = norm_reduce(v, norm);
@@ -320,6 +328,19 @@ BatchView PcgDevice::bview() const {
     v.K = K_;
     v.nchunks = n_ / BS;
     return v;
+}
+
+// A system stopped on a NaN or infinity (state 5) fails the whole call.
+static int nonFinite(const std::vector<double>& st, std::string& err) {
+    for (size_t k = 0; k * B_COUNT < st.size(); ++k)
+        if (st[k * B_COUNT + B_DONE] == 5.0) {
+            const int at = (int)st[k * B_COUNT + B_ITER];
+            err = "PCG (batched): system " + std::to_string(k) +
+                  (at == 0 ? " has a NaN or infinity in its right-hand side"
+                           : " has a NaN or infinity in its residual at iteration " + std::to_string(at));
+            return ASTG_ERR_NOT_CONVERGED;
+        }
+    return ASTG_OK;
 }
 
 int PcgDevice::batchState(std::vector<double>& st, bool toDevice, std::string& err) {
@@ -510,6 +531,7 @@ int PcgDevice::solveBatchOnce(const astg_pcg_params& prm, std::vector<astg_pcg_r
                                    prm.norm, err, nullptr, 0);
         if (rc) return rc;
         if ((rc = batchState(st, false, err))) return rc;
+        if ((rc = nonFinite(st, err))) return rc;
         for (int k = 0; k < K; ++k) {
             astg_pcg_result& R = res[k];
             const double done = S(k, B_DONE), bn = S(k, B_BNORM);
@@ -558,6 +580,11 @@ int PcgDevice::solveBatchOnce(const astg_pcg_params& prm, std::vector<astg_pcg_r
         int active = 0, innerMax = 0;
         for (int k = 0; k < K; ++k) {
             const double rn = S(k, B_OUTER);
+            if (!stopped[k] && !(rn <= DBL_MAX)) {
+                err = "PCG (batched): the FP64 residual of system " + std::to_string(k) +
+                      " holds a NaN or infinity";
+                return ASTG_ERR_NOT_CONVERGED;
+            }
             if (!stopped[k]) rnorm[k] = rn;
             if (bnorm[k] < 0) {
                 bnorm[k] = rn;
@@ -592,6 +619,7 @@ int PcgDevice::solveBatchOnce(const astg_pcg_params& prm, std::vector<astg_pcg_r
         kb_add_corr<<<g, BS, 0, stream_>>>(V, x_.as<double>(), d64);
         HIPCHK(hipGetLastError());
         if ((rc = batchState(st, false, err))) return rc;
+        if ((rc = nonFinite(st, err))) return rc;
         for (int k = 0; k < K; ++k) {
             if (S(k, B_SKIP) != 0.0) continue;
             const int it = S(k, B_DONE) == 2.0 ? 0 : (int)S(k, B_ITER);

@@ -34,7 +34,7 @@ __device__ inline double block_reduce(double v) {
     if (lane == 0) sm[wid] = v;
     __syncthreads();
     double t = sm[0];
-    for (int w = 1; w < BS / 64; ++w) t = MAX ? fmax(t, sm[w]) : t + sm[w];
+    for (int w = 1; w < BS / 64; ++w) t = MAX ? nan_max(t, sm[w]) : t + sm[w];
     return t;
 }
 // Sum (or max) of np partials written by the previous kernel, same order in
@@ -42,7 +42,7 @@ __device__ inline double block_reduce(double v) {
 template <bool MAX>
 __device__ inline double reduce_prev(const double* part, int np) {
     double acc = 0.0;
-    for (int s = threadIdx.x; s < np; s += BS) acc = MAX ? fmax(acc, part[s]) : acc + part[s];
+    for (int s = threadIdx.x; s < np; s += BS) acc = MAX ? nan_max(acc, part[s]) : acc + part[s];
     return block_reduce<MAX>(acc);
 }
 

@@ -29,16 +29,16 @@ public:
     // Device-side copy of the velocity fields (restore = false saves).
     int snapshot(bool restore, std::string& err);
     double assembleBytes() const;
-    // Exchanges the live velocity with another pair of buffers of the same
-    // sizes (the advected velocity becomes live, the old one the temporary).
-    void swapVelocity(DevBuf& mac, DevBuf& tilt) {
-        std::swap(u_mac_.p, mac.p);
-        std::swap(u_mac_.bytes, mac.bytes);
-        std::swap(u_tilt_.p, tilt.p);
-        std::swap(u_tilt_.bytes, tilt.bytes);
+    // Exchanges the live velocity with the advection's output pair (the
+    // advected velocity becomes live, the old one the next output buffer).
+    void swapVelocity() {
+        std::swap(u_mac_, t_mac_);
+        std::swap(u_tilt_, t_tilt_);
     }
     double* uMac() { return u_mac_.as<double>(); }    // 3*G, axis-major
     double* uTilt() { return u_tilt_.as<double>(); }  // 8*G, slot-major
+    double* tMac() { return t_mac_.as<double>(); }    // advection output, as uMac
+    double* tTilt() { return t_tilt_.as<double>(); }  // advection output, as uTilt
     // The assembly's b part and the velocity correction as linear passes
     // (project.hip): their bodies with this context's arrays.
     void asmPasses(double* A, double* b, const int* rs, double rho, double dt, AsmPasses& ps);
@@ -64,7 +64,8 @@ private:
     DevBuf hc_row_, hc_start_, hc_q_, hc_pos_diag_, hc_pos_off_;
     DevBuf hr_row_, hr_start_, hr_q_, hr_pos_;
     DevBuf vm_node_, vm_c_, vm_nb_, vt_node_, vt_c_, vt_nb_, vt_lev_;
-    DevBuf u_mac_, u_tilt_, snap_mac_, snap_tilt_;
+    DevBuf u_mac_, u_tilt_, t_mac_, t_tilt_, snap_mac_, snap_tilt_;
+    long long velG_ = -1;  // node count the four velocity buffers are sized for
     bool haveSnap_ = false;
 };
 

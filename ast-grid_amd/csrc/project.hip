@@ -447,8 +447,22 @@ int ProjectDevice::upload(const ProjectTables& t, const Octree& o, std::string& 
     UP(vt_nb_, t.vt_nb);
     UP(vt_lev_, t.vt_lev);
 #undef UP
-    HIPCHK(u_mac_.alloc((size_t)3 * t.G * 8 + 16));
-    HIPCHK(u_tilt_.alloc((size_t)8 * t.G * 8 + 16));
+    // The live velocity and the advection's output pair it is swapped with,
+    // all four at exactly 3G and 8G doubles, allocated when the node count
+    // changes (a new topology: the caller clears them). A tilt update keeps
+    // the node sets, so the live fields must stay where they are: a buffer
+    // reallocated here would drop the velocity.
+    if (velG_ != (long long)t.G) {
+        for (DevBuf* d : {&u_mac_, &t_mac_}) {
+            d->release();
+            HIPCHK(d->alloc((size_t)3 * t.G * 8));
+        }
+        for (DevBuf* d : {&u_tilt_, &t_tilt_}) {
+            d->release();
+            HIPCHK(d->alloc((size_t)8 * t.G * 8));
+        }
+        velG_ = (long long)t.G;
+    }
     double h[16] = {0};
     for (int l = 0; l < o.L && l < 16; ++l) h[l] = o.lv[l].h;
     int faceCorner[6][4];

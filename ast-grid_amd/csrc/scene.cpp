@@ -51,6 +51,46 @@ void splitSpheres(int R, int levels, const double* sph, int n, const double* ban
     }
 }
 
+void splitDistance(int R, int levels, int shift, const float* dist, const double* band, std::vector<int>& split) {
+    split.assign((size_t)R * R * R, levels - 1);
+    // smallest distance over every aligned level-l block from the lattice's
+    // level up; a block finer than the lattice takes its lattice cell's value
+    std::vector<std::vector<float>> mn(levels);
+    const int dr = R >> shift;
+    mn[shift].assign(dist, dist + (size_t)dr * dr * dr);
+    for (int l = shift + 1; l < levels; ++l) {
+        const int res = R >> l, pr = R >> (l - 1);
+        mn[l].assign((size_t)res * res * res, 0.f);
+        for (int z = 0; z < res; ++z)
+            for (int y = 0; y < res; ++y)
+                for (int x = 0; x < res; ++x) {
+                    float m = 1e30f;
+                    for (int k = 0; k < 8; ++k) {
+                        const int cx = 2 * x + (k & 1), cy = 2 * y + ((k >> 1) & 1), cz = 2 * z + (k >> 2);
+                        m = std::min(m, mn[l - 1][((size_t)cz * pr + cy) * pr + cx]);
+                    }
+                    mn[l][((size_t)z * res + y) * res + x] = m;
+                }
+    }
+    for (int l = levels - 1; l >= 1; --l) {
+        const int b = 1 << l, res = R >> l;
+        for (int z = 0; z < res; ++z)
+            for (int y = 0; y < res; ++y)
+                for (int x = 0; x < res; ++x) {
+                    size_t c0 = ((size_t)(z * b) * R + y * b) * R + x * b;
+                    if (split[c0] != l) continue;
+                    const float m = l >= shift ? mn[l][((size_t)z * res + y) * res + x]
+                                               : mn[shift][((size_t)(z >> (shift - l)) * dr + (y >> (shift - l))) * dr +
+                                                           (x >> (shift - l))];
+                    if (m > band[l - 1]) continue;
+                    for (int k = 0; k < b; ++k)
+                        for (int j = 0; j < b; ++j)
+                            for (int i = 0; i < b; ++i)
+                                split[((size_t)(z * b + k) * R + y * b + j) * R + x * b + i] = l - 1;
+                }
+    }
+}
+
 int balanceSplit(int R, int levels, std::vector<int>& split) {
     int passes = 0;
     for (;;) {

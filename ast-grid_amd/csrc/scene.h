@@ -15,6 +15,13 @@ namespace astg {
 // leaves. spheres: n x (cx, cy, cz, r) in the unit cube.
 void splitSpheres(int R, int levels, const double* spheres, int n, const double* band, std::vector<int>& split);
 
+// Refines top-down: a level-l leaf becomes 2x2x2 level-(l-1) leaves when the
+// smallest of dist over its cells is within band[l-1]. dist: (R >> shift)^3
+// values at the centres of cells 2^shift finest cells wide (x fastest; shift
+// below levels), e.g. the distance to a region that must be finest (<= 0
+// inside it); a block finer than those cells takes its cell's value.
+void splitDistance(int R, int levels, int shift, const float* dist, const double* band, std::vector<int>& split);
+
 // Enforces the neighbourhood condition the reference's layer construction
 // relies on: for every leaf of level l, each of its 26 level-l neighbour blocks
 // is covered only by leaves of level l-1, l or l+1 (so that every such block

@@ -282,6 +282,35 @@ def full_scale_timing(reps=2):
     return out
 
 
+def c5_timing(R=512, levels=5, steps=20, rebuild=10, lattice=32):
+    """The full-scale adaptive run (scenes.full_scale_compare): `steps` steps
+    from rest in FP64, then in mixed precision on the FP64 run's layouts, the
+    layout rebuilt from the density every `rebuild` steps; per-step times and
+    the mixed run's per-field error against FP64 at a lattice^3 grid every
+    `rebuild` steps."""
+    import numpy as np
+    from astg import scenes
+    t0 = time.perf_counter()
+    cmp = scenes.full_scale_compare(R=R, levels=levels, steps=steps, rebuild=rebuild, lattice=lattice,
+                                    log=lambda m: log("c5 " + m))
+    wall = time.perf_counter() - t0
+    out = {"workload": "full_scale: %d^3-effective AST octree, %d levels, two r=0.05 sources with +-0.5 lateral "
+                       "inflow, tilt adaptivity on, layout rebuilt from the density every %d steps" % (R, levels, rebuild),
+           "steps": steps, "rebuild_every": rebuild, "wall_s": round(wall, 1),
+           "leaves_initial": int(scenes.leaf_count(scenes.full_scale_initial_split(R, levels))),
+           "leaves_rebuilt": [int(scenes.leaf_count(s.astype(np.int32))) for s in cmp["fp64"]["splits"]]}
+    for name in ("fp64", "mixed"):
+        o = cmp[name]
+        out[name] = {"ms_per_step": round(float(np.mean(o["step_ms"])), 2),
+                     "ms_projection": round(float(np.mean(o["project_ms"])), 2),
+                     "ms_rebuild": round(float(np.mean(o["rebuild_ms"])), 1) if o["rebuild_ms"] else None,
+                     "iterations": [int(min(o["iterations"])), int(max(o["iterations"]))],
+                     "unknowns": [int(min(o["unknowns"])), int(max(o["unknowns"]))]}
+    out["mixed_vs_fp64"] = [{"step": e["step"], **{f: {k: float("%.3e" % v) for k, v in e[f].items()}
+                                                   for f in ("density", "heat", "velocity")}} for e in cmp["errors"]]
+    return out
+
+
 def cpu_smoke_baseline(steps=50):
     """The reference's own onUpdate (oracle/_ref) on the smoke configuration,
     the same `steps` steps as smoke_timing, on the host's cores."""
@@ -442,6 +471,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=5)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-solve workload")
+    ap.add_argument("--c5-steps", type=int, default=20,
+                    help="steps of the full-scale adaptive run (512^3, 5 levels; 0 skips it)")
+    ap.add_argument("--c5-only", action="store_true", help="run only the full-scale adaptive run")
     ap.add_argument("--batch-only", action="store_true", help="run only the batched-solve workload")
     ap.add_argument("--tune", action="store_true", help="with --batch-only: run the batch autotuner first")
     ap.add_argument("--full-scale", action="store_true",
@@ -469,6 +501,10 @@ def main():
     if a.batch_only:
         bt = batched_timing(tune=a.tune)
         log("batched: %s" % json.dumps(bt))
+        return
+    if a.c5_only:
+        c5 = c5_timing(steps=a.c5_steps)
+        print(json.dumps({"c5": c5}))
         return
     t0 = time.perf_counter()
     ctx, info = scenes.projection_256(device=0, seed=a.seed)
@@ -570,6 +606,13 @@ def main():
         c3["mixed"]["ms_fwd_bwd_per_step"], c3["fp64"]["ms_fwd_bwd_per_step"], c3["grad_mixed_vs_fp64_rel_l2"],
         c3["fd_check"]))
 
+    c5 = None
+    if a.c5_steps > 0:
+        c5 = c5_timing(steps=a.c5_steps)
+        log("c5 (%d steps): fp64 %.0f ms/step, mixed %.0f ms/step; last velocity error %.2e" % (
+            c5["steps"], c5["fp64"]["ms_per_step"], c5["mixed"]["ms_per_step"],
+            c5["mixed_vs_fp64"][-1]["velocity"]["rel_l2"]))
+
     cpu = None
     if not a.no_cpu:
         try:
@@ -619,6 +662,7 @@ def main():
         "smoke_64": smoke,
         "batched": batched,
         "c3": c3,
+        "c5": c5,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

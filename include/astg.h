@@ -63,7 +63,9 @@ extern "C" {
  * more iterations (353 against 267 to 1e-6 in FP64). The solution, the system
  * astg_get_system returns and the velocity update stay in the reference
  * numbering either way. */
-#define ASTG_ORDER_AUTO 0         /* reference numbering for FP64, block colours for the mixed form */
+#define ASTG_ORDER_AUTO 0         /* reference numbering for FP64; for the mixed form block colours from the \
+                                     second solve on one pattern on (their set-up, a permutation and a second \
+                                     solver pattern, costs about one solve), the reference numbering before */
 #define ASTG_ORDER_REFERENCE 1
 #define ASTG_ORDER_BLOCK_COLOUR 2
 
@@ -158,6 +160,15 @@ int astg_pcg_tune_batch(astg_ctx* ctx, int count, const int* n, const int* const
  * of spheres (n x {cx,cy,cz,r}); band[l] = refinement distance for level l+1 -> l;
  * then balanced so every leaf's 26 same-level neighbour blocks exist. */
 int astg_split_spheres(int R, int levels, int nspheres, const double* spheres, const double* band, int* split_out);
+
+/* A split map from a distance field instead of spheres: dist holds
+ * (R >> dist_shift)^3 values at the centres of cells 2^dist_shift finest cells
+ * wide (x fastest; 0 <= dist_shift < levels; e.g. the distance to the region
+ * that must be finest, <= 0 inside it); a level-l leaf is split while the
+ * smallest dist over its cells is within band[l-1], then the map is balanced
+ * as above. Used to rebuild the layout from a density field (no reference
+ * counterpart: the reference builds its layout once, initData 68). */
+int astg_split_distance(int R, int levels, int dist_shift, const float* dist, const double* band, int* split_out);
 
 /* ---- topology (reference node order) ---------------------------------- */
 /* Builds layers, ghosts, T-joint / half-tilt state (constrainOctreeTiltE3,
@@ -352,6 +363,16 @@ int astg_apply_sphere_source(astg_ctx* ctx, double cx, double cy, double cz, dou
  * the boundary SDF is negative become 0 (onUpdate runs it after sceneSetup,
  * before addForce). A no-op without a collider. */
 int astg_wipe_boundary(astg_ctx* ctx);
+
+/* The example's sphere source with an inflow vector instead of the rising
+ * speed: inside the sphere density and heat as astg_apply_sphere_source, every
+ * MAC face gets its component of velocity[3], every tilt slot of a leaf tilt
+ * inside and the tilt and face slots of a half-tilt the projection of
+ * velocity on the slot's direction. No counterpart in the reference, whose
+ * source rises along +z only (example/octree_smoke_3d/main.cpp:23-81): it
+ * drives the two lateral sources of the full-scale configuration. */
+int astg_apply_sphere_source_velocity(astg_ctx* ctx, double cx, double cy, double cz, double radius, double density,
+                                      double heat, const double* velocity, int update_ghost);
 
 /* OctreeOmniGridSolver3::addForce (src/octree_omnigrid_solver3.cpp:917-972):
  * buoyancy f = smoke_density_factor * density + temperature_factor * (heat -

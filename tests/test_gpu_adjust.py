@@ -10,7 +10,10 @@ Through the C ABI on device 0, from the inputs of the advection fixtures:
     -ffast-math) and the tilt state, bit-exact;
   * the next projection on the updated tilts: PCG iterations within 1 of the
     reference's, velocity per node to 1e-6 relative after the reference's
-    number of iterations.
+    number of iterations;
+  * a tilt update keeps every field as it was, bit-exact, also after an odd
+    number of advection steps (when the live velocity sits in the buffer the
+    advection wrote and the update must not reallocate it).
 """
 import glob
 import os
@@ -109,4 +112,25 @@ def test_projection_after_tilt_update_matches_reference(adj):
         m, t = c.velocity(l)
         assert rel2(m, j["vel_proj_mac_%d" % l]) < 1e-6, l
         assert rel2(t, j["vel_proj_tilt_%d" % l]) < 1e-6, l
+    c.close()
+
+
+def test_tilt_update_keeps_fields(adj):
+    g, a, j = adj
+    c, L = prepared(g, a)
+    c.advect(float(j["dt"]))  # tilts fixed: one exchange of the velocity buffers
+    c.post_project()
+    before = [(c.velocity(l), c.scalar(0, l), c.scalar(1, l), c.tilt(l)[1]) for l in range(L)]
+    c.adjust_e(float(j["d_threshold"]), float(j["dt"]), float(j["v_threshold"]), float(j["max_threshold"]))
+    changed = 0
+    for l in range(L):
+        (m0, t0), s0, h0, bits0 = before[l]
+        m1, t1 = c.velocity(l)
+        assert np.array_equal(m1, m0) and np.array_equal(t1, t0), l
+        for f, old in ((0, s0), (1, h0)):
+            o1, v1 = c.scalar(f, l)
+            assert np.array_equal(o1, old[0]) and np.array_equal(v1, old[1]), (f, l)
+        changed += int(np.count_nonzero(c.tilt(l)[1] != bits0))
+    print("tilt nodes changed by the update: %d" % changed)
+    assert changed > 0  # the update went through the tilt-update path
     c.close()

@@ -35,7 +35,7 @@ def test_full_size_pressure_matches_reference_solver():
     ctx, info = scenes.projection_256(device=0, seed=0)
     ctx.snapshot(restore=False)
     sols = {}
-    for p, order, name in ((astg.PREC_FP64, astg.ORDER_AUTO, "fp64"), (astg.PREC_MIXED, astg.ORDER_AUTO, "mixed"),
+    for p, order, name in ((astg.PREC_FP64, astg.ORDER_AUTO, "fp64"), (astg.PREC_MIXED, astg.ORDER_BLOCK_COLOUR, "mixed"),
                            (astg.PREC_MIXED, astg.ORDER_REFERENCE, "mixed_ref")):
         ctx.snapshot(restore=True)
         r = ctx.project(info["dt"], info["rho"], astg.default_params(tol=1e-6, max_iterations=3000, precision=p,
