@@ -254,11 +254,12 @@ _HALF_SLOT = ((0, 3, 4, 7, 1), (0, 1, 4, 5, 2), (0, 1, 2, 3, 5), (1, 2, 5, 6, 0)
 _TB_HALF = 32
 
 
-# Refinement bands of the full-scale layouts, in domain units: the projection
-# scenes' widths (BAND_256) and 0.6 for a fifth level. The same layout at every
-# R: a reduced run is the full-scale configuration at a coarser resolution.
-# At 512^3 and 5 levels the initial layout (the two sources) has 4.5 M leaves.
-FULL_BANDS = BAND_256 + (0.6,)
+# Refinement bands of the full-scale layouts, in domain units: finest within
+# 0.2 of the smoke, one level coarser at 0.3, 0.45 and 0.7. The same layout at
+# every R: a reduced run is the full-scale configuration at a coarser
+# resolution. At 512^3 and 5 levels the initial layout (the two sources) has
+# 15.9 M leaves.
+FULL_BANDS = (0.2, 0.3, 0.45, 0.7)
 
 
 def _bands(R, levels):
