@@ -28,8 +28,8 @@ KERNELS = tuple((nm, i) for i, nm in enumerate(KERNEL_NAMES))
 # workload, as summarised in profiles/traffic_projection_256.md. Constants from
 # that file, not measured by this run.
 TRAFFIC_SOURCE = "profiles/traffic_projection_256.md (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated)"
-TRAFFIC = {"spmv64": 688377572, "fwd64": 629399413, "bwd64": 685888379, "spmv32": 416803946,
-           "fwd32": 434900715, "bwd32": 499123025, "copy": 56200384, "mic": 7374325768}
+TRAFFIC = {"spmv64": 760900000, "fwd64": 629600000, "bwd64": 684900000, "spmv32": 554000000,
+           "fwd32": 344800000, "bwd32": 349800000, "copy": 56200000, "mic": 7371200000, "mic_mixed": 6413100000}
 
 
 def algorithmic_bytes(name, n, nnz_a):
