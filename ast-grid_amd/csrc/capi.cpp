@@ -148,6 +148,9 @@ int retopologize(astg_ctx* c) {
 }
 int retopologizeSteps(astg_ctx* c) {
     astg::PhaseTimer tm("retopologize");
+    long long na0;
+    double ma0;
+    astg::allocStats(na0, ma0);
     c->blkReady = false;  // the block-colour system follows the new numbering on its next use
     c->patternSolves = 0;
     astg::numberUnknowns(*c->tree, c->num);
@@ -164,6 +167,10 @@ int retopologizeSteps(astg_ctx* c) {
     tm.mark("advect upload");
     rc = c->pcg->setPattern(c->tables.n, c->tables.rowstart.data(), c->tables.col.data(), c->err);
     tm.mark("solver pattern");
+    long long na1;
+    double ma1;
+    astg::allocStats(na1, ma1);
+    tm.note("device allocations", (double)(na1 - na0), ma1 - ma0);
     return rc;
 }
 

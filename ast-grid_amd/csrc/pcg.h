@@ -26,9 +26,13 @@
 
 namespace astg {
 
+// Device allocations made through DevBuf so far in this process, and the
+// time spent in them (hipFree of the old buffer included).
+void allocStats(long long& count, double& ms);
+
 struct DevBuf {
     void* p = nullptr;
-    size_t bytes = 0;
+    size_t bytes = 0;  // capacity: at least what the last alloc asked for
     // One owner per allocation: copies are not allowed, moves hand it over.
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;

@@ -25,6 +25,8 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -41,6 +43,14 @@
 
 namespace astg {
 
+namespace {
+std::atomic<long long> g_allocCount{0}, g_allocNs{0};
+}
+void allocStats(long long& count, double& ms) {
+    count = g_allocCount.load();
+    ms = g_allocNs.load() * 1e-6;
+}
+
 DevBuf::~DevBuf() { release(); }
 void DevBuf::release() {
     if (p) (void)hipFree(p);
@@ -49,11 +59,19 @@ void DevBuf::release() {
 }
 hipError_t DevBuf::alloc(size_t b) {
     if (b <= bytes && p) return hipSuccess;
+    const auto t0 = std::chrono::steady_clock::now();
+    // A buffer that has to grow gets 1/8 more than asked: tilt updates move
+    // the unknown count up and down by a little every step, and each
+    // reallocation costs a device synchronisation (hipFree) and a hipMalloc.
+    const size_t cap = p ? b + b / 8 : b;
     if (p) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
-    hipError_t e = hipMalloc(&p, b ? b : 16);
-    if (e == hipSuccess) bytes = b;
+    hipError_t e = hipMalloc(&p, cap ? cap : 16);
+    if (e == hipSuccess) bytes = cap;
+    g_allocCount.fetch_add(1);
+    g_allocNs.fetch_add(
+        std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - t0).count());
     return e;
 }
 

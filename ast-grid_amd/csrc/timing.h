@@ -13,6 +13,10 @@ public:
     explicit PhaseTimer(const char* scope) : scope_(scope), on_(std::getenv("ASTG_TIMING") != nullptr) {
         t_ = std::chrono::steady_clock::now();
     }
+    // a count and the milliseconds it took, outside the phase clock
+    void note(const char* what, double count, double ms) const {
+        if (on_) std::fprintf(stderr, "[astg timing] %s/%s %.0f in %.2f ms\n", scope_, what, count, ms);
+    }
     void mark(const char* phase) {
         if (!on_) return;
         const auto now = std::chrono::steady_clock::now();
