@@ -26,6 +26,7 @@ ASTG_ERR_IO = -6
 WALL_XNEG, WALL_XPOS, WALL_ZNEG, WALL_ZPOS, WALL_YNEG, WALL_YPOS = 1, 2, 4, 8, 16, 32
 PREC_FP64, PREC_MIXED = 0, 1
 ORDER_AUTO, ORDER_REFERENCE, ORDER_BLOCK_COLOUR = 0, 1, 2
+TAU_AUTO = -1.0  # mic_tau: 0.97 (the reference's MIC(0)) in FP64, 0 (IC(0)) in the mixed form
 NORM_INF, NORM_L2 = 0, 1
 
 # Header symbols, in the order include/astg.h declares them.

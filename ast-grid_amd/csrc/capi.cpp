@@ -201,7 +201,7 @@ void astg_pcg_params_default(astg_pcg_params* p) {
     p->tol = 1e-5;
     p->max_iterations = 3000;
     p->norm = ASTG_NORM_INF;
-    p->mic_tau = 0.97;
+    p->mic_tau = ASTG_TAU_AUTO;
     p->min_diag_ratio = 0.25;
     p->precision = ASTG_PREC_FP64;
     p->max_outer = 0;
@@ -520,6 +520,15 @@ int blockMerge() {
 // copy exists already). Its set-up (permutation, a second solver pattern and
 // its plans) costs about one solve, so a pattern solved once, as in a run
 // whose tilts change every step, stays in the reference numbering.
+// The MIC(0) modification parameter a solve uses: the caller's, or with
+// ASTG_TAU_AUTO the reference's 0.97 in FP64 (its factor and iterates) and
+// plain IC(0) in the mixed form, which needs about half the iterations of
+// MIC(0) at 0.97 on these systems (C2: 176 against 363 in block colours).
+double micTau(const astg_pcg_params& prm) {
+    if (prm.mic_tau >= 0) return prm.mic_tau;
+    return prm.precision == ASTG_PREC_MIXED ? 0.0 : 0.97;
+}
+
 bool blockOrdered(const astg_ctx* c, const astg_pcg_params& prm) {
     if (prm.ordering == ASTG_ORDER_BLOCK_COLOUR) return true;
     if (prm.ordering == ASTG_ORDER_REFERENCE) return false;
@@ -565,7 +574,7 @@ int solveAssembled(astg_ctx* c, const astg_pcg_params& prm, astg_pcg_result& res
     const bool blk = refactor ? blockOrdered(c, prm) : c->factoredBlk;
     if (!blk) {
         if (refactor) {
-            if ((r = c->pcg->factor(prm.mic_tau, prm.min_diag_ratio, c->err))) return r;
+            if ((r = c->pcg->factor(micTau(prm), prm.min_diag_ratio, c->err))) return r;
             c->factoredBlk = false;
         }
         if (evFactored) (void)hipEventRecord(evFactored, c->stream);
@@ -581,7 +590,7 @@ int solveAssembled(astg_ctx* c, const astg_pcg_params& prm, astg_pcg_result& res
                                   B.values(), n, c->blkPerm.as<int>(), c->pcg->rhs(), B.rhs(), c->err)))
         return r;
     if (refactor) {
-        if ((r = B.factor(prm.mic_tau, prm.min_diag_ratio, c->err))) return r;
+        if ((r = B.factor(micTau(prm), prm.min_diag_ratio, c->err))) return r;
         c->factoredBlk = true;
     }
     if (evFactored) (void)hipEventRecord(evFactored, c->stream);
@@ -1036,7 +1045,7 @@ int astg_pcg_solve_csr(astg_ctx* c, int n, const int* rowstart, const int* colin
             hipMemcpyAsync(c->csr->rhs(), b, (size_t)n * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess)
             return fail(c, ASTG_ERR_HIP, "upload failed");
         (void)hipEventRecord(ev[0], c->stream);
-        int r = c->csr->factor(prm.mic_tau, prm.min_diag_ratio, c->err);
+        int r = c->csr->factor(micTau(prm), prm.min_diag_ratio, c->err);
         if (r) return r;
         (void)hipEventRecord(ev[1], c->stream);
         r = c->csr->solve(prm, res, c->err);
@@ -1157,7 +1166,7 @@ int batchRun(astg_ctx* c, const StackedBatch& S, const astg_pcg_params& prm, std
         hipMemcpyAsync(c->batch->rhs(), S.b.data(), (size_t)S.N * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess)
         return fail(c, ASTG_ERR_HIP, "upload failed");
     (void)hipEventRecord(ev[0], c->stream);
-    int r = c->batch->factor(prm.mic_tau, prm.min_diag_ratio, c->err);
+    int r = c->batch->factor(micTau(prm), prm.min_diag_ratio, c->err);
     if (r) return r;
     (void)hipEventRecord(ev[1], c->stream);
     r = c->batch->solveBatch(prm, res, c->err);

@@ -405,7 +405,9 @@ def c3_timing(fd_coords=8, steps=20, every=5):
     `steps` steps, gradient with respect to the initial MAC velocity through
     the adjoint kernels, checkpoints every `every` steps): wall time per step
     of the forward and of forward + backward, in the mixed form (FP32 inner
-    PCG, FP64 refinement, 1e-6) and in FP64 (1e-12); the gradients' relative
+    PCG, FP64 refinement, IC(0), 1e-8: the tolerance at which its gradient
+    meets the 1e-4 finite-difference bar; at 1e-6 it misses by 1.5e-3) and in
+    FP64 (1e-12); the gradients' relative
     L2 difference; central differences (eps 1e-5, FP64 forward) on
     `fd_coords` seeded face coordinates against both gradients."""
     import numpy as np
@@ -424,7 +426,7 @@ def c3_timing(fd_coords=8, steps=20, every=5):
     out = {"workload": "diff_smoke: 128^3-effective, 3 levels, %d leaves, frozen topology, %d steps, "
                        "checkpoints every %d" % (info["leaves"], steps, every)}
     grads, sims = {}, {}
-    for name, prec, tol in (("mixed", astg.PREC_MIXED, 1e-6), ("fp64", astg.PREC_FP64, 1e-12)):
+    for name, prec, tol in (("mixed", astg.PREC_MIXED, 1e-8), ("fp64", astg.PREC_FP64, 1e-12)):
         sim = DiffSim(ctx, info["dt"], info["rho"], astg.default_params(tol=tol, max_iterations=5000,
                                                                           precision=prec))
         u = torch.tensor(info["u0"], device="cuda", requires_grad=True)
@@ -545,6 +547,12 @@ def main():
     errs = field_errors(projected_fields(ctx, lastmx, info["levels"]), f64)
     del f64
     log("mixed vs fp64: %s" % errs)
+    # FP64 with the mixed form's preconditioner (block colours, IC(0)): what the
+    # preconditioner choice gives without the precision change
+    prm64ic = astg.default_params(tol=1e-6, max_iterations=3000, precision=astg.PREC_FP64,
+                                  ordering=astg.ORDER_BLOCK_COLOUR, mic_tau=0.0)
+    ms64ic, r64ic = time_steps(ctx, info, prm64ic, a.steps, a.warmup)
+    log("fp64 in block colours, IC(0): %.2f ms/step iters %d" % (ms64ic, r64ic[-1].iterations))
 
     # per-kernel device times (HIP events on the context stream), on the solver
     # each form uses: the FP64 kernels on the reference-numbered system after
@@ -656,6 +664,10 @@ def main():
         "mixed_vs_fp64": errs,
         "sweep_depth": {"fp64": last64.depth, "mixed": lastmx.depth},
         "ordering": {"fp64": "reference numbering", "mixed": "block colours (8 colours of 8^3-cell cubes)"},
+        "preconditioner": {"fp64": "MIC(0), tau 0.97 (the reference's)", "mixed": "IC(0) (MIC(0) with tau 0)"},
+        "fp64_mixed_preconditioner": {"ms_per_step": round(ms64ic, 3), "iterations": r64ic[-1].iterations,
+                                      "note": "FP64 in block colours with IC(0): the mixed form's preconditioner "
+                                              "without the precision change"},
         "setup_s": round(setup_s, 2),
         "roofline": roofline,
         "advect": adv,

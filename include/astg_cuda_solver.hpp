@@ -30,7 +30,7 @@ class HipSolver {
 public:
     double solver_error = 0.0;  // final residual, ||r||_inf by default (absolute)
     int solver_iters = 0;       // PCG iterations of the last solve
-    astg_pcg_params params;     // tol 1e-5, 3000 iterations, tau 0.97, ratio 0.25, FP64
+    astg_pcg_params params;     // tol 1e-5, 3000 iterations, tau ASTG_TAU_AUTO (0.97 in FP64), ratio 0.25, FP64
     astg_pcg_result last{};     // full result of the last solve
 
     explicit HipSolver(int device = 0) {

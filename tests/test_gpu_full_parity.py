@@ -13,10 +13,10 @@ Two solves that differ only in rounding (dot and row-sum orders) follow
 trajectories that drift apart by up to a fraction of one iteration's
 progress over hundreds of iterations, so the distance between the two 1e-6
 solutions is compared with the size of that tolerance, not with zero. The
-mixed form in the reference numbering within 1e-4 relative L2 of the
-reference (its stated bound against FP64); the mixed form in its default
-block-colour ordering (another preconditioner, so another trajectory) to the
-FP64 form's bar: host-computed relative residual within 1e-6 and a distance
+mixed form in the reference numbering with the reference's MIC(0) (tau 0.97)
+within 1e-4 relative L2 of the reference (its stated bound against FP64);
+the mixed form in its default configuration (block colours, IC(0): another
+preconditioner, so another trajectory) to the FP64 form's bar: host-computed relative residual within 1e-6 and a distance
 to the exact solution at most 1.5x the reference's. Skipped where the
 reference build is absent. The reference solve takes about 15 s on 16 cores.
 """
@@ -35,11 +35,12 @@ def test_full_size_pressure_matches_reference_solver():
     ctx, info = scenes.projection_256(device=0, seed=0)
     ctx.snapshot(restore=False)
     sols = {}
-    for p, order, name in ((astg.PREC_FP64, astg.ORDER_AUTO, "fp64"), (astg.PREC_MIXED, astg.ORDER_BLOCK_COLOUR, "mixed"),
-                           (astg.PREC_MIXED, astg.ORDER_REFERENCE, "mixed_ref")):
+    for p, order, tau, name in ((astg.PREC_FP64, astg.ORDER_AUTO, astg.TAU_AUTO, "fp64"),
+                                (astg.PREC_MIXED, astg.ORDER_BLOCK_COLOUR, astg.TAU_AUTO, "mixed"),
+                                (astg.PREC_MIXED, astg.ORDER_REFERENCE, 0.97, "mixed_ref")):
         ctx.snapshot(restore=True)
         r = ctx.project(info["dt"], info["rho"], astg.default_params(tol=1e-6, max_iterations=3000, precision=p,
-                                                                     ordering=order))
+                                                                     ordering=order, mic_tau=tau))
         rs, col, val, b, x = ctx.system(r.n, r.nnz)
         sols[name] = (x.copy(), r.iterations)
         print("%s: n=%d iterations %d" % (name, r.n, r.iterations), flush=True)
