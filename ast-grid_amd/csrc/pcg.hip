@@ -833,8 +833,15 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
     PhaseTimer tm("setPattern");
     n_ = n;
     nnz_ = rowstart[n];
-    hrs_.assign(rowstart, rowstart + n + 1);
-    hcol_.assign(col, col + nnz_);
+    // host copies of the pattern (host plans, factor download, checks), by
+    // the worker pool: at 16 M rows the column array alone is 0.5 GB
+    hrs_.resize((size_t)n + 1);
+    hcol_.resize((size_t)nnz_);
+    parallelFor((long)n + 1, [&](long a, long b) { std::memcpy(hrs_.data() + a, rowstart + a, (b - a) * sizeof(int)); },
+                1 << 20);
+    parallelFor((long)nnz_, [&](long a, long b) { std::memcpy(hcol_.data() + a, col + a, (b - a) * sizeof(int)); },
+                1 << 20);
+    tm.mark("host copy of the pattern");
     if (const char* g = getenv("ASTG_GATE_DIST")) gateDist_ = std::max(1, atoi(g));
     if (const char* g = getenv("ASTG_POLL_CAP")) pollCap_ = std::max(1, atoi(g));
     if (const char* g = getenv("ASTG_GATE_CAP")) gateCap_ = std::max(1, atoi(g));
