@@ -117,9 +117,17 @@ __global__ void __launch_bounds__(BS) k_copy64(int n, const double* __restrict__
 
 // ---------------------------------------------------------------------------
 // dependency levels of tril(A): level(i) = 1 + max level(k), k in lower(i).
-// Setup only. One row per wavefront (its lanes poll the row's dependencies in
-// parallel), rows handed out in index order through a ticket: every row a wave
-// waits on belongs to an earlier, already running wave.
+// Setup only. One lane per row, blocks of BS consecutive rows handed out in
+// index order through a ticket: every row a lane waits on has a smaller index,
+// so it belongs to this wave, an earlier wave of this block or an earlier,
+// already running block. Each lane keeps its row's lower columns in registers
+// (at most kMaxDeps, checked by k_split_rows) and polls the ones not seen yet
+// in one round (all loads issued before any is examined); the loop is
+// wave-uniform and a lane publishes in the round its last dependency appears,
+// so lanes of one wave that depend on each other finish in turn. A row that
+// waits past the spin limit raises E_LEVELS and publishes what it has.
+// (One wavefront per row kept only ~8 k rows in flight; rows depend on their
+// predecessors along a grid line, so that form crawled: 276 ms at 16 M rows.)
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(BS) k_levels(int n, const int* rs, const int* lend, const int* col, int* lev,
                                                int* cnt) {
@@ -127,29 +135,43 @@ __global__ void __launch_bounds__(BS) k_levels(int n, const int* rs, const int* 
     __shared__ int chunk;
     if (threadIdx.x == 0) chunk = atomicAdd(&cnt[C_TICKET_L], 1);
     __syncthreads();
-    const int lane = threadIdx.x & 63;
-    const int i = chunk * (BS / 64) + (threadIdx.x >> 6);
-    if (i >= n) return;
-    const int p0 = rs[i], pe = lend[i];
-    int L = 0;
-    for (int base = p0; base < pe; base += 64) {
-        const int p = base + lane;
-        if (p < pe) {
-            const int k = col[p];
-            int lk, spins = 0;
-            while ((lk = ld_agent_i32(&lev[k])) < 0) {
-                if (++spins > kSpinLimit) {
-                    atomicOr(&cnt[C_ERR], E_LEVELS);
-                    lk = 0;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-            }
-            L = max(L, lk + 1);
-        }
+    const int i = chunk * BS + (int)threadIdx.x;
+    bool pending = i < n;
+    int nd = 0;
+    int c[kMaxDeps];
+    if (pending) {
+        const int p0 = rs[i];
+        nd = lend[i] - p0;
+#pragma unroll
+        for (int j = 0; j < kMaxDeps; ++j) c[j] = j < nd ? col[p0 + j] : 0;
     }
-    for (int o = 32; o > 0; o >>= 1) L = max(L, __shfl_xor(L, o, 64));
-    if (lane == 0) st_agent_i32(&lev[i], L);
+    unsigned seen = 0;
+    const unsigned full = nd >= 32 ? ~0u : (1u << nd) - 1u;
+    int L = 0, spins = 0;
+    for (;;) {
+        if (pending) {
+            int v[kMaxDeps];
+#pragma unroll
+            for (int j = 0; j < kMaxDeps; ++j)
+                v[j] = (j < nd && !((seen >> j) & 1u)) ? ld_agent_i32(&lev[c[j]]) : kSentLevel;
+#pragma unroll
+            for (int j = 0; j < kMaxDeps; ++j)
+                if (v[j] >= 0) {
+                    L = max(L, v[j] + 1);
+                    seen |= 1u << j;
+                }
+            if (seen == full) {
+                st_agent_i32(&lev[i], L);
+                pending = false;
+            } else if (++spins > kSpinLimit) {
+                atomicOr(&cnt[C_ERR], E_LEVELS);
+                st_agent_i32(&lev[i], L);
+                pending = false;
+            }
+        }
+        if (!__any(pending)) break;
+        __builtin_amdgcn_s_sleep(1);
+    }
 }
 
 // Split of row i at the diagonal (lend, ubeg), transpose map (tpos: position
@@ -895,8 +917,8 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
                                                 tpos_.as<int>(), cnt_.as<int>());
     // dependency levels on the device (skipped if the check failed)
     k_fill_i32<<<blocks(n), BS, 0, stream_>>>(lev_.as<int>(), kSentLevel, n);
-    k_levels<<<(n + BS / 64 - 1) / (BS / 64), BS, 0, stream_>>>(n, rs_.as<int>(), lend_.as<int>(), col_.as<int>(),
-                                                                lev_.as<int>(), cnt_.as<int>());
+    k_levels<<<blocks(n), BS, 0, stream_>>>(n, rs_.as<int>(), lend_.as<int>(), col_.as<int>(), lev_.as<int>(),
+                                            cnt_.as<int>());
     HIPCHK(hipGetLastError());
     // stable sort of (level, row): rows in level order, ties by index
     HIPCHK(sortKeys_.alloc(n4));

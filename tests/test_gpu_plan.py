@@ -8,6 +8,8 @@ them must be bit-identical: the factor, the PCG solution, iteration count and
 residual (FP64 and mixed forms), the corrected velocities, a batched solve,
 and a run of smoke steps whose tilt adaptivity rebuilds the plans every step.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -145,4 +147,43 @@ def test_invalid_patterns_are_rejected(case, message):
     val = np.where(np.array(col) == np.repeat(np.arange(4), np.diff(rs)), 4.0, -1.0)
     x, res = c.pcg_solve_csr(np.array(rs), np.array(col), val, np.ones(4))
     assert res.converged == 1
+    c.close()
+
+
+def _longest_path_depth(rs, col):
+    """Dependency depth of tril(A) on the host: level(i) = 1 + max level(k)
+    over the row's entries left of the diagonal (0 without any), rows in
+    index order (every such k < i)."""
+    rs, col = np.asarray(rs), np.asarray(col)
+    n = len(rs) - 1
+    lev = np.zeros(n, np.int64)
+    for i in range(n):
+        c = col[rs[i]:rs[i + 1]]
+        c = c[c < i]
+        if len(c):
+            lev[i] = lev[c].max() + 1
+    return int(lev.max()) + 1
+
+
+def test_device_levels_give_the_longest_path_depth():
+    """The device level computation (k_levels, one lane per row, sync-free)
+    against a plain host longest-path count: the depth the solve reports is
+    the DAG's, on every golden system and on one batched scene system."""
+    import astg
+    from astg import scenes
+    c = astg.Context(0)
+    systems = []
+    for p in GOLDEN:
+        g = load_golden(p)
+        systems.append((os.path.basename(p), g["A_rowstart"], g["A_col"], g["A_val"], g["b"]))
+    sc, info = scenes.batch_scene(0, 0, scenes.batch_targets(0)[0])
+    r0 = sc.project(info["dt"], info["rho"], astg.default_params(tol=1e-6, max_iterations=3000))
+    rs, col, val, b, _ = sc.system(r0.n, r0.nnz)
+    sc.close()
+    systems.append(("batch scene 0", rs, col, val, b))
+    for name, rs, col, val, b in systems:
+        _, r = c.pcg_solve_csr(rs, col, val, b, astg.default_params(tol=1e-6, max_iterations=3000))
+        want = _longest_path_depth(rs, col)
+        print("%s: n %d depth %d (host %d)" % (name, len(rs) - 1, r.depth, want), flush=True)
+        assert r.depth == want, name
     c.close()
