@@ -273,7 +273,8 @@ private:
     DevBuf scal_, cnt_, part_;              // device scalars, counters, partials
     double* hscal_ = nullptr;               // pinned mirror of the scalars
     long long* hplan_ = nullptr;            // pinned: set-up counters and plan sizes
-    DevBuf sortKeys_, sortIdx_;             // set-up: level sort keys and row indices
+    DevBuf sortKeys_, sortIdx_, sortRows_;  // set-up: level sort keys, row indices, rows in level order
+    DevBuf lvlStart_, grpStart_, grpFirst_; // set-up: first sorted row of each level; group layout of an order
     DevBuf planLen_, planPos_, planCnt_, planTmp_;  // set-up: plan counts, sweep positions, scan scratch
     int P_ = 0;                             // padded length of the level order
     int PS_ = 0;                            // padded length of the sweep order

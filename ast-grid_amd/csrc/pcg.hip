@@ -846,6 +846,64 @@ static int splitPatternHost(int n, const int* rowstart, const int* col, std::vec
     return ASTG_OK;
 }
 
+// Host construction of a padded level order (the device builds the same with
+// k_level_starts / k_pad_order; the host planners check one against the
+// other): groups of q consecutive levels, each padded to a multiple of 64
+// rows, the whole to a multiple of BS; gate rows per 64-row segment from
+// groups G-dist and G+dist.
+static void hostOrder(int q, int dist, int n, int depth, const std::vector<int>& sortedLev,
+                      const std::vector<int>& sortedRow, std::vector<int>& padded, std::vector<int>& gF,
+                      std::vector<int>& gB) {
+    const int ng = (depth + q - 1) / q;
+    std::vector<int> gStart(ng, 0), gRows(ng, 0);
+    padded.clear();
+    padded.reserve((size_t)n + (size_t)ng * 64 + BS);
+    for (int i = 0; i < n;) {
+        int j = i;
+        const int G = sortedLev[i] / q;
+        gStart[G] = (int)padded.size();
+        while (j < n && sortedLev[j] / q == G) padded.push_back(sortedRow[j++]);
+        gRows[G] = j - i;
+        while (padded.size() % 64) padded.push_back(-1);
+        i = j;
+    }
+    while (padded.size() % BS) padded.push_back(-1);
+    const int nseg = (int)padded.size() / 64;
+    gF.assign(nseg, -1);
+    gB.assign(nseg, -1);
+    std::vector<int> segG(nseg, -1);
+    for (int G = 0; G < ng; ++G) {
+        const int s0 = gStart[G] / 64, s1 = (gStart[G] + gRows[G] + 63) / 64;
+        for (int sg = s0; sg < s1; ++sg) segG[sg] = G;
+    }
+    for (int sg = 0; sg < nseg; ++sg) {
+        const int G = segG[sg];
+        if (G < 0) continue;
+        const unsigned h = (unsigned)sg * 2654435761u;
+        const int gf = G - dist, gb = G + dist;
+        if (gf >= 0) gF[sg] = gStart[gf] + (int)(h % (unsigned)gRows[gf]);
+        if (gb < ng) gB[sg] = gStart[gb] + (int)(h % (unsigned)gRows[gb]);
+    }
+}
+
+// First sorted position of every level (rows sorted by level; every level
+// from 0 to the last holds a row), and n after the last.
+__global__ void k_level_starts(int n, const int* __restrict__ key, int* __restrict__ lvlStart) {
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= n) return;
+    if (t == 0 || key[t] != key[t - 1]) lvlStart[key[t]] = t;
+    if (t == n - 1) lvlStart[key[t] + 1] = n;
+}
+// Rows in level order into their group's padded slots: the row at sorted
+// position t goes to gStart[G] + (t - gFirst[G]), G = level / q.
+__global__ void k_pad_order(int n, const int* __restrict__ key, const int* __restrict__ row, int q,
+                            const int* __restrict__ gStart, const int* __restrict__ gFirst, int* __restrict__ out) {
+    const int t = blockIdx.x * BS + threadIdx.x;
+    if (t >= n) return;
+    const int G = key[t] / q;
+    out[gStart[G] + (t - gFirst[G])] = row[t];
+}
+
 int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::string& err) {
     f32Valid_ = false;
     if (n <= 0) {
@@ -923,18 +981,17 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
     // stable sort of (level, row): rows in level order, ties by index
     HIPCHK(sortKeys_.alloc(n4));
     HIPCHK(sortIdx_.alloc(n4));
+    HIPCHK(sortRows_.alloc(n4));
     k_iota<<<blocks(n), BS, 0, stream_>>>(sortIdx_.as<int>(), n);
     size_t tmpBytes = 0;
     HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmpBytes, lev_.as<int>(), sortKeys_.as<int>(),
-                                              sortIdx_.as<int>(), order_.as<int>(), n, 0, 31, stream_));
+                                              sortIdx_.as<int>(), sortRows_.as<int>(), n, 0, 31, stream_));
     HIPCHK(planTmp_.alloc(tmpBytes));
     HIPCHK(hipcub::DeviceRadixSort::SortPairs(planTmp_.p, tmpBytes, lev_.as<int>(), sortKeys_.as<int>(),
-                                              sortIdx_.as<int>(), order_.as<int>(), n, 0, 31, stream_));
-    std::vector<int> sortedLev(n), sortedRow(n);
-    HIPCHK(hipMemcpyAsync(sortedLev.data(), sortKeys_.p, n4, hipMemcpyDeviceToHost, stream_));
-    HIPCHK(hipMemcpyAsync(sortedRow.data(), order_.p, n4, hipMemcpyDeviceToHost, stream_));
+                                              sortIdx_.as<int>(), sortRows_.as<int>(), n, 0, 31, stream_));
     int* hcnt = reinterpret_cast<int*>(hplan_);
     HIPCHK(hipMemcpyAsync(hcnt, cnt_.p, C_COUNT * 4, hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipMemcpyAsync(hcnt + C_COUNT, sortKeys_.as<int>() + (n - 1), 4, hipMemcpyDeviceToHost, stream_));
     HIPCHK(hipStreamSynchronize(stream_));
     if (hcnt[C_PATTERN]) {
         std::vector<int> lend, ubeg;
@@ -946,34 +1003,44 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
         err = "level computation did not complete";
         return ASTG_ERR_HIP;
     }
-    depth_ = sortedLev[n - 1] + 1;
+    depth_ = hcnt[C_COUNT] + 1;
     tm.mark("split, transpose map, levels, sort");
 
-    // Rows in order of groups of q consecutive levels, every group padded to a
-    // multiple of 64 rows so that a wavefront never holds rows of two groups;
-    // gate rows: for each 64-row segment of group G, one row of group G-dist
-    // (forward) and one of group G+dist (backward), spread over that group.
-    auto buildOrder = [&](int q, int dist, std::vector<int>& padded, std::vector<int>& gF, std::vector<int>& gB) {
-        const int ng = (depth_ + q - 1) / q;
-        std::vector<int> gStart(ng, 0), gRows(ng, 0);
-        padded.clear();
-        padded.reserve((size_t)n + (size_t)ng * 64 + BS);
-        for (int i = 0; i < n;) {
-            int j = i;
-            const int G = sortedLev[i] / q;
-            gStart[G] = (int)padded.size();
-            while (j < n && sortedLev[j] / q == G) padded.push_back(sortedRow[j++]);
-            gRows[G] = j - i;
-            while (padded.size() % 64) padded.push_back(-1);
-            i = j;
+    // Orders: rows in order of groups of q consecutive levels, every group
+    // padded to a multiple of 64 rows so that a wavefront never holds rows of
+    // two groups, the whole padded to a multiple of BS; gate rows: for each
+    // 64-row segment of group G, one row of group G-dist (forward) and one of
+    // group G+dist (backward), spread over that group. Every level 0..depth-1
+    // holds at least one row, so every group is non-empty. The group layout
+    // follows from the first sorted row of each level (lvlStart, depth+1
+    // entries): computed on the host from that, rows scattered on the device.
+    std::vector<int> lvlStart(depth_ + 1);
+    struct Layout {
+        int ng = 0, P = 0;
+        std::vector<int> gStart, gFirst, gRows, gF, gB;
+    };
+    auto layout = [&](int q, int dist, bool gates, Layout& o) {
+        o.ng = (depth_ + q - 1) / q;
+        o.gStart.assign(o.ng, 0);
+        o.gFirst.assign(o.ng, 0);
+        o.gRows.assign(o.ng, 0);
+        long long pos = 0;
+        for (int G = 0; G < o.ng; ++G) {
+            const int first = lvlStart[G * q], end = lvlStart[std::min((G + 1) * q, depth_)];
+            o.gFirst[G] = first;
+            o.gRows[G] = end - first;
+            o.gStart[G] = (int)pos;
+            pos = (pos + o.gRows[G] + 63) / 64 * 64;
         }
-        while (padded.size() % BS) padded.push_back(-1);
-        const int nseg = (int)padded.size() / 64;
-        gF.assign(nseg, -1);
-        gB.assign(nseg, -1);
+        pos = (pos + BS - 1) / BS * BS;
+        o.P = (int)pos;
+        if (!gates) return;
+        const int nseg = o.P / 64;
+        o.gF.assign(nseg, -1);
+        o.gB.assign(nseg, -1);
         std::vector<int> segG(nseg, -1);
-        for (int G = 0; G < ng; ++G) {
-            const int s0 = gStart[G] / 64, s1 = (gStart[G] + gRows[G] + 63) / 64;
+        for (int G = 0; G < o.ng; ++G) {
+            const int s0 = o.gStart[G] / 64, s1 = (o.gStart[G] + o.gRows[G] + 63) / 64;
             for (int sg = s0; sg < s1; ++sg) segG[sg] = G;
         }
         for (int sg = 0; sg < nseg; ++sg) {
@@ -981,8 +1048,8 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
             if (G < 0) continue;
             const unsigned h = (unsigned)sg * 2654435761u;
             const int gf = G - dist, gb = G + dist;
-            if (gf >= 0) gF[sg] = gStart[gf] + (int)(h % (unsigned)gRows[gf]);
-            if (gb < ng) gB[sg] = gStart[gb] + (int)(h % (unsigned)gRows[gb]);
+            if (gf >= 0) o.gF[sg] = o.gStart[gf] + (int)(h % (unsigned)o.gRows[gf]);
+            if (gb < o.ng) o.gB[sg] = o.gStart[gb] + (int)(h % (unsigned)o.gRows[gb]);
         }
     };
     auto upload = [&](DevBuf& d, const void* src, size_t bytes) -> hipError_t {
@@ -990,35 +1057,82 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
         if (e != hipSuccess) return e;
         return hipMemcpyAsync(d.p, src, bytes, hipMemcpyHostToDevice, stream_);
     };
-    // factorisation: rows in level order, one launch per level. The host
-    // vectors uploaded below live until the synchronisation at the end.
-    std::vector<int> padded, gF, gB;
-    buildOrder(1, 1, padded, gF, gB);
-    P_ = (int)padded.size();
-    HIPCHK(upload(order_, padded.data(), (size_t)P_ * 4));
-    {
-        std::vector<int> levOfRow(n);
-        for (int q = 0; q < n; ++q) levOfRow[sortedRow[q]] = sortedLev[q];
-        std::vector<int> first(depth_, -1), last(depth_, -1);
-        for (int q = 0; q < P_; ++q)
-            if (padded[q] >= 0) {
-                const int L = levOfRow[padded[q]];
-                if (first[L] < 0) first[L] = q;
-                last[L] = q;
-            }
-        levRange_.assign(2 * (size_t)depth_, 0);
-        for (int L = 0; L < depth_; ++L) {
-            levRange_[2 * L] = first[L];
-            levRange_[2 * L + 1] = last[L] + 1;
-        }
+    HIPCHK(lvlStart_.alloc(((size_t)depth_ + 1) * 4));
+    k_fill_i32<<<blocks(depth_ + 1), BS, 0, stream_>>>(lvlStart_.as<int>(), -1, depth_ + 1);
+    k_level_starts<<<blocks(n), BS, 0, stream_>>>(n, sortKeys_.as<int>(), lvlStart_.as<int>());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(lvlStart.data(), lvlStart_.p, ((size_t)depth_ + 1) * 4, hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    // the layout below indexes by these: every level must start after the last
+    bool starts = lvlStart[0] == 0 && lvlStart[depth_] == n;
+    for (int L = 0; starts && L < depth_; ++L) starts = lvlStart[L] < lvlStart[L + 1];
+    if (!starts) {
+        err = "dependency levels are not contiguous";
+        return ASTG_ERR_HIP;
+    }
+    // the host-planner path (ASTG_HOST_PLAN) also needs the sorted rows and
+    // both orders on the host
+    std::vector<int> sortedLev, sortedRow, spadded;
+    if (hostPlan_) {
+        sortedLev.resize(n);
+        sortedRow.resize(n);
+        HIPCHK(hipMemcpyAsync(sortedLev.data(), sortKeys_.p, n4, hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipMemcpyAsync(sortedRow.data(), sortRows_.p, n4, hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+    }
+    // The host vectors uploaded below live until the synchronisation at the
+    // end of this function.
+    auto scatter = [&](const Layout& o, int q, DevBuf& ord) -> int {
+        HIPCHK(upload(grpStart_, o.gStart.data(), (size_t)o.ng * 4));
+        HIPCHK(upload(grpFirst_, o.gFirst.data(), (size_t)o.ng * 4));
+        HIPCHK(ord.alloc((size_t)o.P * 4));
+        k_fill_i32<<<blocks(o.P), BS, 0, stream_>>>(ord.as<int>(), -1, o.P);
+        k_pad_order<<<blocks(n), BS, 0, stream_>>>(n, sortKeys_.as<int>(), sortRows_.as<int>(), q,
+                                                  grpStart_.as<int>(), grpFirst_.as<int>(), ord.as<int>());
+        HIPCHK(hipGetLastError());
+        // the small group arrays are reused by the next call: finish this one
+        HIPCHK(hipStreamSynchronize(stream_));
+        return ASTG_OK;
+    };
+    // factorisation: rows in level order, one launch per level
+    Layout lf, ls;
+    layout(1, 1, false, lf);
+    P_ = lf.P;
+    if (int rc = scatter(lf, 1, order_)) return rc;
+    levRange_.assign(2 * (size_t)depth_, 0);
+    for (int L = 0; L < depth_; ++L) {
+        levRange_[2 * L] = lf.gStart[L];
+        levRange_[2 * L + 1] = lf.gStart[L] + lf.gRows[L];
     }
     // sweeps: q levels per group
-    std::vector<int> spadded, sgF, sgB;
-    buildOrder(merge_, gateDist_, spadded, sgF, sgB);
-    PS_ = (int)spadded.size();
-    HIPCHK(upload(sorder_, spadded.data(), (size_t)PS_ * 4));
-    HIPCHK(upload(sgateF_, sgF.data(), sgF.size() * 4));
-    HIPCHK(upload(sgateB_, sgB.data(), sgB.size() * 4));
+    layout(merge_, gateDist_, true, ls);
+    PS_ = ls.P;
+    if (int rc = scatter(ls, merge_, sorder_)) return rc;
+    HIPCHK(upload(sgateF_, ls.gF.data(), ls.gF.size() * 4));
+    HIPCHK(upload(sgateB_, ls.gB.data(), ls.gB.size() * 4));
+    if (hostPlan_) {
+        // the host construction of both orders and the gates, from the sorted
+        // rows, must give the device's entry for entry
+        std::vector<int> padded, gF, gB, devOrd, devF, devB;
+        hostOrder(1, 1, n, depth_, sortedLev, sortedRow, padded, gF, gB);
+        devOrd.resize(P_);
+        HIPCHK(hipMemcpyAsync(devOrd.data(), order_.p, (size_t)P_ * 4, hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+        if ((int)padded.size() != P_ || padded != devOrd) {
+            err = "device-built factorisation order differs from the host construction";
+            return ASTG_ERR_HIP;
+        }
+        hostOrder(merge_, gateDist_, n, depth_, sortedLev, sortedRow, spadded, gF, gB);
+        devOrd.resize(PS_);
+        devF.resize(ls.gF.size());
+        devB.resize(ls.gB.size());
+        HIPCHK(hipMemcpyAsync(devOrd.data(), sorder_.p, (size_t)PS_ * 4, hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+        if ((int)spadded.size() != PS_ || spadded != devOrd || gF != ls.gF || gB != ls.gB) {
+            err = "device-built sweep order or gates differ from the host construction";
+            return ASTG_ERR_HIP;
+        }
+    }
     tm.mark("level and sweep orders");
     // symbolic plans: factorisation programs and merged stencils
     const int maxY = kMaxDeps;
