@@ -38,6 +38,32 @@ lcoor_t tiltNeighbor(const Octree& o, int l, int x, int y, int z, int k) {
     return c;
 }
 
+// Appends off + i for every i in [0, n) with keep(i), in increasing order, on
+// the worker pool: fixed chunks counted, offsets by a prefix sum, then filled.
+template <class P>
+void compactAppend(int n, int off, P keep, std::vector<int>& out) {
+    constexpr int S = 256;
+    auto lo = [&](long c) { return (int)((long long)n * c / S); };
+    std::vector<long> cnt(S + 1, 0);
+    parallelFor(S, [&](long c0, long c1) {
+        for (long c = c0; c < c1; ++c) {
+            long k = 0;
+            for (int i = lo(c); i < lo(c + 1); ++i) k += keep(i) ? 1 : 0;
+            cnt[c + 1] = k;
+        }
+    }, 1);
+    for (int c = 0; c < S; ++c) cnt[c + 1] += cnt[c];
+    const size_t base = out.size();
+    out.resize(base + (size_t)cnt[S]);
+    parallelFor(S, [&](long c0, long c1) {
+        for (long c = c0; c < c1; ++c) {
+            size_t w = base + (size_t)cnt[c];
+            for (int i = lo(c); i < lo(c + 1); ++i)
+                if (keep(i)) out[w++] = off + i;
+        }
+    }, 1);
+}
+
 // Pattern rows under construction. Octagon and tilt rows (each its own
 // unknown, at most 15 entries: 6 faces, 8 tilt corners, the diagonal) are
 // sorted lists in a flat array with stride kRowCap, each written by one
@@ -112,12 +138,14 @@ void buildProjectTables(const Octree& o, const Numbering& num, const Weights& w,
     // ---- octagon and tilt rows (project 367-552) ----
     std::vector<int> octJ, tiltJ;
     for (int l = 0; l < L; ++l) {
-        const Level& Lv = o.lv[l];
-        for (int i = 0; i < Lv.nleaf; ++i) {
-            if (num.oct[l][i] >= 0) octJ.push_back(t.off[l] + i);
-            if (num.tilt[l][i] >= 0 && (o.tilt[l][i].bits & TB_LAYOUT_OPEN)) tiltJ.push_back(t.off[l] + i);
-        }
+        const int* no_ = num.oct[l].data();
+        const int* nt_ = num.tilt[l].data();
+        const auto* ts = o.tilt[l].data();
+        compactAppend(o.lv[l].nleaf, t.off[l], [&](int i) { return no_[i] >= 0; }, octJ);
+        compactAppend(o.lv[l].nleaf, t.off[l], [&](int i) { return nt_[i] >= 0 && (ts[i].bits & TB_LAYOUT_OPEN); },
+                      tiltJ);
     }
+    tm.mark("row lists");
     const int no = t.n_oct = (int)octJ.size();
     t.oct_row.resize(no);
     t.oct_corner.resize((size_t)8 * no);
@@ -266,6 +294,7 @@ void buildProjectTables(const Octree& o, const Numbering& num, const Weights& w,
     // ---- CSR: each row's own list merged with its shared couplings ----
     std::sort(rs.extra.begin(), rs.extra.end());
     rs.extra.erase(std::unique(rs.extra.begin(), rs.extra.end()), rs.extra.end());
+    tm.mark("shared entries sorted");
     const auto& ex = rs.extra;
     // merge of row r's list and its shared entries [e0, e1); emit(col) per entry
     auto mergeRow = [&](int r, size_t e0, size_t e1, auto&& emit) {
@@ -305,6 +334,7 @@ void buildProjectTables(const Octree& o, const Numbering& num, const Weights& w,
         int32_t* o = t.col.data() + t.rowstart[r];
         mergeRow(r, e0, e1, [&](int c) { *o++ = c; });
     });
+    tm.mark("CSR");
     auto posOf = [&](int r, int c) -> uint8_t {
         const int32_t* b = t.col.data() + t.rowstart[r];
         const int32_t* e = t.col.data() + t.rowstart[r + 1];
