@@ -34,10 +34,8 @@ struct astg_ctx {
     std::vector<std::vector<double>> hostMac, hostTilt;  // host-only contexts
     std::unique_ptr<astg::PcgDevice> pcg;                 // projection system
     std::unique_ptr<astg::PcgDevice> pcgBlk;              // the same system in block-colour order (reorder.h)
-    astg::BlockOrder blk;
-    astg::DevBuf blkPerm, blkSrc;
+    astg::BlockOrderDevice blk;  // permutation and permuted pattern, device-built
     bool blkReady = false;
-    int patternSolves = 0;     // projection solves on the current pattern (ASTG_ORDER_AUTO's choice)
     bool factoredBlk = false;  // the last factorisation was of the block-colour copy
     std::unique_ptr<astg::ProjectDevice> proj;
     std::unique_ptr<astg::AdvectDevice> adv;              // post-projection step and advection
@@ -88,7 +86,6 @@ int ensureTables(astg_ctx* c) {
         c->tablesReady = true;
         c->deviceReady = false;
         c->blkReady = false;
-        c->patternSolves = 0;
         if (c->device < 0) {
             c->hostMac.assign(c->tree->L, {});
             c->hostTilt.assign(c->tree->L, {});
@@ -152,7 +149,6 @@ int retopologizeSteps(astg_ctx* c) {
     double ma0;
     astg::allocStats(na0, ma0);
     c->blkReady = false;  // the block-colour system follows the new numbering on its next use
-    c->patternSolves = 0;
     astg::numberUnknowns(*c->tree, c->num);
     tm.mark("numbering");
     astg::computeWeights(*c->tree, c->w);
@@ -251,8 +247,7 @@ int astg_destroy(astg_ctx* c) {
         c->adv.reset();
         c->pcg.reset();
         c->pcgBlk.reset();
-        c->blkPerm.release();
-        c->blkSrc.release();
+        c->blk = astg::BlockOrderDevice();
         c->proj.reset();
         c->csr.reset();
         c->batch.reset();
@@ -515,11 +510,6 @@ int blockMerge() {
     return v > 0 ? v : 1;
 }
 
-// ASTG_ORDER_AUTO: the block-colour form for the mixed precision once its
-// set-up is paid for, i.e. from the second solve on one pattern (or when the
-// copy exists already). Its set-up (permutation, a second solver pattern and
-// its plans) costs about one solve, so a pattern solved once, as in a run
-// whose tilts change every step, stays in the reference numbering.
 // The MIC(0) modification parameter a solve uses: the caller's, or with
 // ASTG_TAU_AUTO the reference's 0.97 in FP64 (its factor and iterates) and
 // plain IC(0) in the mixed form, which needs about half the iterations of
@@ -529,31 +519,51 @@ double micTau(const astg_pcg_params& prm) {
     return prm.precision == ASTG_PREC_MIXED ? 0.0 : 0.97;
 }
 
-bool blockOrdered(const astg_ctx* c, const astg_pcg_params& prm) {
+// ASTG_ORDER_AUTO: the block-colour form for the mixed precision, the
+// reference numbering for FP64. The block-colour copy is built on the device
+// (C2: 26 ms against a 79 ms solve), so it pays from the first solve on a
+// pattern, also in a run whose tilts change every step.
+bool blockOrdered(const astg_pcg_params& prm) {
     if (prm.ordering == ASTG_ORDER_BLOCK_COLOUR) return true;
     if (prm.ordering == ASTG_ORDER_REFERENCE) return false;
-    return prm.precision == ASTG_PREC_MIXED && (c->blkReady || c->patternSolves > 0);
+    return prm.precision == ASTG_PREC_MIXED;
 }
 
 // The block-colour copy of the projection system: permutation, pattern and
-// solver plans, built on first use after a topology or tilt change.
+// solver plans, built on first use after a topology or tilt change. Keys on
+// host threads, the sort and the permuted pattern on the device from A's
+// device pattern; ASTG_CHECK_BLOCK_ORDER=1 also builds the host form
+// (buildBlockOrder) and fails unless both agree entry for entry.
 int ensureBlockOrder(astg_ctx* c) {
     if (c->blkReady) return ASTG_OK;
     astg::PhaseTimer tm("block order");
-    astg::buildBlockOrder(*c->tree, c->num, c->tables.rowstart, c->tables.col, blockCells(), c->blk);
+    const int n = c->pcg->n();
+    const long long nnz = c->pcg->nnz();
+    std::vector<int> keys;
+    astg::blockKeys(*c->tree, c->num, blockCells(), keys);
+    tm.mark("keys");
+    int rc = c->blk.build(c->stream, n, nnz, keys.data(), c->pcg->devRowstart(), c->pcg->devCol(), c->err);
+    if (rc) return rc;
     tm.mark("permutation and pattern");
+    if (const char* g = std::getenv("ASTG_CHECK_BLOCK_ORDER"); g && std::atoi(g) != 0) {
+        astg::BlockOrder h;
+        astg::buildBlockOrder(*c->tree, c->num, c->tables.rowstart, c->tables.col, blockCells(), h);
+        std::vector<int> perm(n), rs(n + 1), col(nnz), src(nnz);
+        if (hipMemcpyAsync(perm.data(), c->blk.perm.p, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipMemcpyAsync(rs.data(), c->blk.rowstart.p, ((size_t)n + 1) * 4, hipMemcpyDeviceToHost, c->stream) !=
+                hipSuccess ||
+            hipMemcpyAsync(col.data(), c->blk.col.p, (size_t)nnz * 4, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipMemcpyAsync(src.data(), c->blk.src.p, (size_t)nnz * 4, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+            return fail(c, ASTG_ERR_HIP, "block-order check: download failed");
+        if (perm != h.perm || rs != h.rowstart || col != h.col || src != h.src)
+            return fail(c, ASTG_ERR_HIP, "block order: the device build differs from the host construction");
+        tm.mark("host check");
+    }
     if (!c->pcgBlk) c->pcgBlk.reset(new astg::PcgDevice(c->stream));
     c->pcgBlk->setMergeLevels(blockMerge());
-    int rc = c->pcgBlk->setPattern(c->blk.n, c->blk.rowstart.data(), c->blk.col.data(), c->err);
+    rc = c->pcgBlk->setPatternDevice(n, nnz, c->blk.rowstart.as<int>(), c->blk.col.as<int>(), c->err);
     if (rc) return rc;
-    if (c->blkPerm.alloc(c->blk.perm.size() * 4) != hipSuccess || c->blkSrc.alloc(c->blk.src.size() * 4) != hipSuccess)
-        return fail(c, ASTG_ERR_HIP, "block-order buffers: device allocation failed");
-    if (hipMemcpyAsync(c->blkPerm.p, c->blk.perm.data(), c->blk.perm.size() * 4, hipMemcpyHostToDevice, c->stream) !=
-            hipSuccess ||
-        hipMemcpyAsync(c->blkSrc.p, c->blk.src.data(), c->blk.src.size() * 4, hipMemcpyHostToDevice, c->stream) !=
-            hipSuccess ||
-        hipStreamSynchronize(c->stream) != hipSuccess)
-        return fail(c, ASTG_ERR_HIP, "block-order upload failed");
     tm.mark("solver pattern");
     c->blkReady = true;
     return ASTG_OK;
@@ -571,7 +581,7 @@ int solveAssembled(astg_ctx* c, const astg_pcg_params& prm, astg_pcg_result& res
         return fail(c, ASTG_ERR_ARG, "unknown ordering");
     int r;
     // a solve on the last factor keeps that factor's ordering
-    const bool blk = refactor ? blockOrdered(c, prm) : c->factoredBlk;
+    const bool blk = refactor ? blockOrdered(prm) : c->factoredBlk;
     if (!blk) {
         if (refactor) {
             if ((r = c->pcg->factor(micTau(prm), prm.min_diag_ratio, c->err))) return r;
@@ -579,15 +589,13 @@ int solveAssembled(astg_ctx* c, const astg_pcg_params& prm, astg_pcg_result& res
         }
         if (evFactored) (void)hipEventRecord(evFactored, c->stream);
         c->lastPcg = c->pcg.get();
-        if ((r = c->pcg->solve(prm, res, c->err))) return r;
-        ++c->patternSolves;
-        return ASTG_OK;
+        return c->pcg->solve(prm, res, c->err);
     }
     if ((r = ensureBlockOrder(c))) return r;
     astg::PcgDevice& B = *c->pcgBlk;
     const int n = c->pcg->n();
-    if ((r = astg::gatherPermuted(c->stream, c->pcg->nnz(), c->blkSrc.as<int>(), refactor ? c->pcg->values() : nullptr,
-                                  B.values(), n, c->blkPerm.as<int>(), c->pcg->rhs(), B.rhs(), c->err)))
+    if ((r = astg::gatherPermuted(c->stream, c->pcg->nnz(), c->blk.src.as<int>(), refactor ? c->pcg->values() : nullptr,
+                                  B.values(), n, c->blk.perm.as<int>(), c->pcg->rhs(), B.rhs(), c->err)))
         return r;
     if (refactor) {
         if ((r = B.factor(micTau(prm), prm.min_diag_ratio, c->err))) return r;
@@ -596,8 +604,7 @@ int solveAssembled(astg_ctx* c, const astg_pcg_params& prm, astg_pcg_result& res
     if (evFactored) (void)hipEventRecord(evFactored, c->stream);
     c->lastPcg = &B;
     if ((r = B.solve(prm, res, c->err))) return r;
-    ++c->patternSolves;
-    return astg::scatterPermuted(c->stream, n, c->blkPerm.as<int>(), B.solution(), c->pcg->solution(), c->err);
+    return astg::scatterPermuted(c->stream, n, c->blk.perm.as<int>(), B.solution(), c->pcg->solution(), c->err);
 }
 
 int runProject(astg_ctx* c, double dt, double rho, const astg_pcg_params& prm, astg_pcg_result* result) {

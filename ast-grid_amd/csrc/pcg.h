@@ -25,6 +25,7 @@
 #include "device_common.h"
 
 namespace astg {
+class PhaseTimer;
 
 // Device allocations made through DevBuf so far in this process, and the
 // time spent in them (hipFree of the old buffer included).
@@ -138,6 +139,9 @@ public:
     // orders, the symbolic plans and, for single-system solves, A's pattern
     // permuted to sweep order.
     int setPattern(int n, const int* rowstart, const int* col, std::string& err);
+    // The same from a pattern already in device memory (rowstart n+1, col
+    // nnz, on this object's stream): no host copy is made until one is needed.
+    int setPatternDevice(int n, long long nnz, const int* dRowstart, const int* dCol, std::string& err);
     // Device pointer to the FP64 value array (nnz) for in-place assembly.
     double* values() { return val_.as<double>(); }
     double* rhs() { return b_.as<double>(); }
@@ -187,8 +191,6 @@ public:
     // test and iteration count (rhs(), solution(), values() hold the stack).
     int solveBatch(const astg_pcg_params& prm, std::vector<astg_pcg_result>& res, std::string& err);
     bool cooperativeSweeps() const { return coop_; }
-    const std::vector<int>& hostRowstart() const { return hrs_; }
-    const std::vector<int>& hostCol() const { return hcol_; }
 
 private:
     // x: FP64 solution (the FP32 iteration accumulates its steps in FP64).
@@ -257,6 +259,10 @@ private:
     long long nnz_ = 0;
     int depth_ = 0;
     std::vector<int> hrs_, hcol_;
+    bool hostPatternValid_ = true;  // hrs_ / hcol_ hold the current pattern
+    int ensureHostPattern(std::string& err);
+    int setPatternImpl(int n, long long nnz, const int* dRowstart, const int* dCol, PhaseTimer& tm,
+                       std::string& err);
     DevBuf rs_, col_, lend_, ubeg_, tpos_, order_, lev_;  // pattern split, level order (factorisation)
     std::vector<int> levRange_;             // per level: [first, last + 1) position in the level order
     DevBuf micStart_, micProg_, live_;      // factorisation programs (mic_plan.h), eliminated columns

@@ -905,23 +905,52 @@ __global__ void k_pad_order(int n, const int* __restrict__ key, const int* __res
 }
 
 int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::string& err) {
-    f32Valid_ = false;
     if (n <= 0) {
         err = "empty system";
         return ASTG_ERR_ARG;
     }
     PhaseTimer tm("setPattern");
-    n_ = n;
-    nnz_ = rowstart[n];
     // host copies of the pattern (host plans, factor download, checks), by
     // the worker pool: at 16 M rows the column array alone is 0.5 GB
+    const long long nnz = rowstart[n];
     hrs_.resize((size_t)n + 1);
-    hcol_.resize((size_t)nnz_);
+    hcol_.resize((size_t)nnz);
     parallelFor((long)n + 1, [&](long a, long b) { std::memcpy(hrs_.data() + a, rowstart + a, (b - a) * sizeof(int)); },
                 1 << 20);
-    parallelFor((long)nnz_, [&](long a, long b) { std::memcpy(hcol_.data() + a, col + a, (b - a) * sizeof(int)); },
+    parallelFor((long)nnz, [&](long a, long b) { std::memcpy(hcol_.data() + a, col + a, (b - a) * sizeof(int)); },
                 1 << 20);
+    hostPatternValid_ = true;
     tm.mark("host copy of the pattern");
+    return setPatternImpl(n, nnz, nullptr, nullptr, tm, err);
+}
+
+int PcgDevice::setPatternDevice(int n, long long nnz, const int* dRowstart, const int* dCol, std::string& err) {
+    if (n <= 0) {
+        err = "empty system";
+        return ASTG_ERR_ARG;
+    }
+    PhaseTimer tm("setPattern");
+    hostPatternValid_ = false;
+    return setPatternImpl(n, nnz, dRowstart, dCol, tm, err);
+}
+
+int PcgDevice::ensureHostPattern(std::string& err) {
+    if (hostPatternValid_) return ASTG_OK;
+    hrs_.resize((size_t)n_ + 1);
+    hcol_.resize((size_t)nnz_);
+    HIPCHK(hipMemcpyAsync(hrs_.data(), rs_.p, ((size_t)n_ + 1) * 4, hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipMemcpyAsync(hcol_.data(), col_.p, (size_t)nnz_ * 4, hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    hostPatternValid_ = true;
+    return ASTG_OK;
+}
+
+// The pattern is in hrs_ / hcol_ (dRowstart null) or in device memory.
+int PcgDevice::setPatternImpl(int n, long long nnz, const int* dRowstart, const int* dCol, PhaseTimer& tm,
+                              std::string& err) {
+    f32Valid_ = false;
+    n_ = n;
+    nnz_ = nnz;
     if (const char* g = getenv("ASTG_GATE_DIST")) gateDist_ = std::max(1, atoi(g));
     if (const char* g = getenv("ASTG_POLL_CAP")) pollCap_ = std::max(1, atoi(g));
     if (const char* g = getenv("ASTG_GATE_CAP")) gateCap_ = std::max(1, atoi(g));
@@ -966,8 +995,13 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
     HIPCHK(cnt_.alloc(C_COUNT * 4));
     if (!hscal_) HIPCHK(hipHostMalloc((void**)&hscal_, 32 * 8, hipHostMallocDefault));
     if (!hplan_) HIPCHK(hipHostMalloc((void**)&hplan_, 16 * 8, hipHostMallocDefault));
-    HIPCHK(hipMemcpyAsync(rs_.p, rowstart, ((size_t)n + 1) * 4, hipMemcpyHostToDevice, stream_));
-    HIPCHK(hipMemcpyAsync(col_.p, col, (size_t)nnz_ * 4, hipMemcpyHostToDevice, stream_));
+    if (dRowstart) {
+        HIPCHK(hipMemcpyAsync(rs_.p, dRowstart, ((size_t)n + 1) * 4, hipMemcpyDeviceToDevice, stream_));
+        HIPCHK(hipMemcpyAsync(col_.p, dCol, (size_t)nnz_ * 4, hipMemcpyDeviceToDevice, stream_));
+    } else {
+        HIPCHK(hipMemcpyAsync(rs_.p, hrs_.data(), ((size_t)n + 1) * 4, hipMemcpyHostToDevice, stream_));
+        HIPCHK(hipMemcpyAsync(col_.p, hcol_.data(), (size_t)nnz_ * 4, hipMemcpyHostToDevice, stream_));
+    }
     k_zero_counters<<<1, 64, 0, stream_>>>(cnt_.as<int>());
     k_zero64<<<1, BS, 0, stream_>>>(32, scal_.as<double>());
     // split at the diagonal, transpose map and structural checks
@@ -995,7 +1029,8 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
     HIPCHK(hipStreamSynchronize(stream_));
     if (hcnt[C_PATTERN]) {
         std::vector<int> lend, ubeg;
-        if (splitPatternHost(n, rowstart, col, lend, ubeg, err)) return ASTG_ERR_ARG;
+        if (int rc = ensureHostPattern(err)) return rc;
+        if (splitPatternHost(n, hrs_.data(), hcol_.data(), lend, ubeg, err)) return ASTG_ERR_ARG;
         err = "the device pattern check failed where the host check passes";
         return ASTG_ERR_HIP;
     }
@@ -1141,6 +1176,9 @@ int PcgDevice::setPattern(int n, const int* rowstart, const int* col, std::strin
         if (rc) return rc;
     } else {
         std::vector<int> lend, ubeg;
+        if (int rc = ensureHostPattern(err)) return rc;
+        const int* rowstart = hrs_.data();
+        const int* col = hcol_.data();
         if (splitPatternHost(n, rowstart, col, lend, ubeg, err)) return ASTG_ERR_ARG;
         std::vector<long long> ms;
         std::vector<int> mp;
@@ -1570,6 +1608,8 @@ int PcgDevice::solveOnce(const astg_pcg_params& prm, astg_pcg_result& res, std::
 int PcgDevice::downloadFactor(std::vector<double>& invdiag, std::vector<unsigned>& colstart,
                               std::vector<unsigned>& rowindex, std::vector<double>& value) {
     const int n = n_;
+    std::string perr;
+    if (ensureHostPattern(perr)) return ASTG_ERR_HIP;
     std::vector<double> F(nnz_);
     std::vector<int> ubeg(n);
     invdiag.resize(n);

@@ -2,11 +2,59 @@
 #include "reorder.h"
 
 #include <algorithm>
+#include <climits>
 #include <stdexcept>
 
 #include "parallel.h"
 
 namespace astg {
+
+void blockKeys(const Octree& o, const Numbering& num, int blockCells, std::vector<int>& keys) {
+    const int n = num.n;
+    if (blockCells < 1) throw std::runtime_error("block order: cube size must be positive");
+    // first node holding each row: 2 * (global node index) + 0 for an
+    // octagon row, + 1 for a tilt row (a row is of one kind only)
+    std::vector<int> off(o.L + 1, 0);
+    for (int l = 0; l < o.L; ++l) off[l + 1] = off[l] + o.lv[l].nnode;
+    if ((long long)off[o.L] * 2 + 1 >= INT_MAX) throw std::runtime_error("block order: too many nodes");
+    std::vector<int> first(n, INT_MAX);
+    auto amin = [](int* p, int v) {
+        int cur = __atomic_load_n(p, __ATOMIC_RELAXED);
+        while (v < cur && !__atomic_compare_exchange_n(p, &cur, v, true, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+        }
+    };
+    for (int l = 0; l < o.L; ++l)
+        parallelFor(o.lv[l].nnode, [&](long i0, long i1) {
+            for (long i = i0; i < i1; ++i) {
+                const int g = off[l] + (int)i, a = num.oct[l][i], t = num.tilt[l][i];
+                if (a >= 0) amin(&first[a], 2 * g);
+                if (t >= 0) amin(&first[t], 2 * g + 1);
+            }
+        });
+    const int w = 2 * blockCells;
+    const int nb = (o.R + blockCells) / blockCells + 1;
+    const long long cubes = (long long)nb * nb * nb;
+    if (8 * cubes + 1 >= INT_MAX) throw std::runtime_error("block order: too many cubes");
+    keys.resize(n);
+    parallelFor(n, [&](long r0, long r1) {
+        for (long r = r0; r < r1; ++r) {
+            const int e = first[r];
+            if (e == INT_MAX) {
+                keys[r] = (int)(8 * cubes);
+                continue;
+            }
+            const int g = e >> 1;
+            const int l = (int)(std::upper_bound(off.begin(), off.end(), g) - off.begin()) - 1;
+            const int i = g - off[l];
+            const Level& Lv = o.lv[l];
+            const int s = 2 << l, h = (e & 1) ? 0 : (s >> 1);
+            const int px = Lv.x[i] * s + h, py = Lv.y[i] * s + h, pz = Lv.z[i] * s + h;
+            const int bx = px / w, by = py / w, bz = pz / w;
+            const int colour = (bx & 1) | ((by & 1) << 1) | ((bz & 1) << 2);
+            keys[r] = (int)(colour * cubes + ((long long)bz * nb + by) * nb + bx);
+        }
+    });
+}
 
 void buildBlockOrder(const Octree& o, const Numbering& num, const std::vector<int>& rowstart,
                      const std::vector<int>& col, int blockCells, BlockOrder& out) {

@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "octree.h"
+#include "pcg.h"
 
 namespace astg {
 
@@ -37,6 +38,24 @@ struct BlockOrder {
 // sorted by column). Throws std::runtime_error on inconsistent input.
 void buildBlockOrder(const Octree& o, const Numbering& num, const std::vector<int>& rowstart,
                      const std::vector<int>& col, int blockCells, BlockOrder& out);
+
+// The sort key of every row of the reference numbering: colour * cubes + cube
+// of the row's position (the first node, in level and node order, that holds
+// the row), 8 * cubes for a row without one. Host worker pool. buildBlockOrder
+// sorts the rows by these keys, stably.
+void blockKeys(const Octree& o, const Numbering& num, int blockCells, std::vector<int>& keys);
+
+// buildBlockOrder on the device from the keys (uploaded here) and A's device
+// pattern (reference numbering, rows sorted by column): the same perm,
+// rowstart, col and src, in device buffers. Rows of more than 64 entries are
+// an error. On `stream`; returns after it has finished.
+struct BlockOrderDevice {
+    int n = 0;
+    long long nnz = 0;
+    DevBuf keys, sortedKeys, iota, perm, inv, rowstart, col, src, tmp, flag;
+    int build(hipStream_t stream, int n, long long nnz, const int* hostKeys, const int* rs, const int* colA,
+              std::string& err);
+};
 
 // Device side (reorder.hip), on `stream`:
 //   valOut[e] = valIn[src[e]] for e < nnz (skipped when valIn is null),
