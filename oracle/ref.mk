@@ -1,5 +1,7 @@
 # Builds the reference solver from its own sources, where they lie, plus the
-# checker harness (oracle/ref_harness.cpp) into oracle/_ref/libastgref.so.
+# checker harnesses (oracle/ref_harness.cpp: the octree solver;
+# oracle/ast_harness.cpp: the uniform-layout AST solver) into
+# oracle/_ref/libastgref.so.
 # TEST INFRASTRUCTURE ONLY: nothing in the product links or loads this library.
 #
 #   make -f oracle/ref.mk -j8            (from the repository root)
@@ -23,7 +25,7 @@ REF_SRCS := $(wildcard $(REF)/src/*.cpp) $(wildcard $(REF)/src/viewer/*.cpp) \
 # uses them and one of them references a symbol defined nowhere in the tree.
 REF_SRCS := $(filter-out %2.cpp %2set.cpp,$(REF_SRCS))
 REF_OBJS := $(patsubst $(REF)/src/%.cpp,$(OBJ)/%.o,$(REF_SRCS))
-HARNESS  := $(OBJ)/ref_harness.o
+HARNESS  := $(OBJ)/ref_harness.o $(OBJ)/ast_harness.o
 
 all: $(OUT)/libastgref.so
 
@@ -31,7 +33,11 @@ $(OBJ)/%.o: $(REF)/src/%.cpp
 	@mkdir -p $(dir $@)
 	$(CXX) $(CXXFLAGS) -c $< -o $@
 
-$(HARNESS): oracle/ref_harness.cpp
+$(OBJ)/ref_harness.o: oracle/ref_harness.cpp
+	@mkdir -p $(dir $@)
+	$(CXX) $(CXXFLAGS) -c $< -o $@
+
+$(OBJ)/ast_harness.o: oracle/ast_harness.cpp
 	@mkdir -p $(dir $@)
 	$(CXX) $(CXXFLAGS) -c $< -o $@
 

@@ -81,6 +81,16 @@ def lib():
         "ref_wipe_boundary": (I, [P]),
         "ref_benchmark_reset": (None, []),
         "ref_benchmark_report": (I, [ctypes.c_char_p, I]),
+        "ast_ref_last_error": (ctypes.c_char_p, []),
+        "ast_ref_create": (P, [I, P, D]),
+        "ast_ref_destroy": (None, [P]),
+        "ast_ref_set_layout": (I, [P, P]),
+        "ast_ref_set_velocity": (I, [P, P, P, P, P]),
+        "ast_ref_velocity": (I, [P, P, P, P, P]),
+        "ast_ref_project": (I, [P, D, D, P, P]),
+        "ast_ref_system": (I, [P, P, P, P, P, P]),
+        "ast_ref_numbering": (I, [P, P, P]),
+        "ast_ref_solve_info": (I, [P, P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -371,3 +381,88 @@ def pcg_csr(rs, col, val, b, tol=1e-6, maxit=3000, tau=0.97, ratio=0.25):
     if ok < 0:
         raise RuntimeError(L.ref_last_error())
     return x, r.value, it.value, bool(ok)
+
+
+class RefAST:
+    """The reference's uniform-layout AST solver (ast::ASTSolver<d>,
+    include/ast_solver.h) through oracle/ast_harness.cpp. Shapes follow the
+    reference's grids, x fastest: tilt nodes prod(res + 1), MAC faces of axis a
+    prod(res + e_a), tilt velocities (prod(res + 1), 4 or 8) node-major."""
+
+    def __init__(self, dim, res, dx):
+        self.dim, self.res, self.dx = dim, tuple(int(r) for r in res), float(dx)
+        r = np.array(self.res, dtype=np.int32)
+        self.h = lib().ast_ref_create(dim, _p(r), self.dx)
+        if not self.h:
+            raise RuntimeError(lib().ast_ref_last_error().decode())
+        self.kt = 4 if dim == 2 else 8
+
+    def close(self):
+        if self.h:
+            lib().ast_ref_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc:
+            raise RuntimeError(lib().ast_ref_last_error().decode())
+
+    def tilt_count(self):
+        return int(np.prod([r + 1 for r in self.res]))
+
+    def mac_count(self, a):
+        return int(np.prod([r + (1 if i == a else 0) for i, r in enumerate(self.res)]))
+
+    def set_layout(self, open_):
+        o = np.ascontiguousarray(open_, dtype=np.uint8).reshape(-1)
+        assert o.size == self.tilt_count()
+        self._chk(lib().ast_ref_set_layout(self.h, _p(o)))
+
+    def set_velocity(self, mac, tilt):
+        m = [np.ascontiguousarray(mac[a], dtype=np.float64).reshape(-1) for a in range(self.dim)]
+        t = np.ascontiguousarray(tilt, dtype=np.float64).reshape(-1)
+        assert t.size == self.tilt_count() * self.kt
+        for a in range(self.dim):
+            assert m[a].size == self.mac_count(a)
+        self._chk(lib().ast_ref_set_velocity(self.h, _p(m[0]), _p(m[1]), _p(m[2]) if self.dim == 3 else None, _p(t)))
+
+    def velocity(self):
+        m = [np.zeros(self.mac_count(a)) for a in range(self.dim)]
+        t = np.zeros(self.tilt_count() * self.kt)
+        self._chk(lib().ast_ref_velocity(self.h, _p(m[0]), _p(m[1]), _p(m[2]) if self.dim == 3 else None, _p(t)))
+        return m, t.reshape(-1, self.kt)
+
+    def project(self, dt, rho=2.0):
+        n = np.zeros(1, np.int32)
+        nnz = np.zeros(1, np.int64)
+        self._chk(lib().ast_ref_project(self.h, float(dt), float(rho), _p(n), _p(nnz)))
+        self.n, self.nnz = int(n[0]), int(nnz[0])
+        return self.n, self.nnz
+
+    def system(self):
+        rs = np.zeros(self.n + 1, np.int32)
+        col = np.zeros(self.nnz, np.int32)
+        val = np.zeros(self.nnz)
+        b = np.zeros(self.n)
+        x = np.zeros(self.n)
+        self._chk(lib().ast_ref_system(self.h, _p(rs), _p(col), _p(val), _p(b), _p(x)))
+        return rs, col, val, b, x
+
+    def numbering(self):
+        cell = np.zeros(int(np.prod(self.res)), np.int32)
+        tilt = np.zeros(self.tilt_count(), np.int32)
+        self._chk(lib().ast_ref_numbering(self.h, _p(cell), _p(tilt)))
+        return cell, tilt
+
+    def solve_info(self):
+        """(iterations, final infinity-norm residual) of the last projection's
+        solve; iterations is -1 if the re-solve did not reproduce it."""
+        it = np.zeros(1, np.int32)
+        err = np.zeros(1)
+        self._chk(lib().ast_ref_solve_info(self.h, _p(it), _p(err)))
+        return int(it[0]), float(err[0])

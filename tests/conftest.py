@@ -12,9 +12,11 @@ sys.path.insert(0, os.path.join(ROOT, "ast-grid_amd"))
 
 # projection fixtures (the adv_*.npz advection and adj_*.npz tilt-adaptivity
 # fixtures build on them; smoke64.npz is the multi-step smoke configuration,
-# frame32.npz the on-disk frame formats, col_*.npz the collider scenes)
+# frame32.npz the on-disk frame formats, col_*.npz the collider scenes,
+# ast*.npz the uniform-layout AST projection)
 GOLDEN = sorted(p for p in glob.glob(os.path.join(ROOT, "tests", "golden", "*.npz"))
-                if not os.path.basename(p).startswith(("adv_", "adj_", "smoke", "frame", "col_")))
+                if not os.path.basename(p).startswith(("adv_", "adj_", "smoke", "frame", "col_", "ast")))
+AST_GOLDEN = sorted(glob.glob(os.path.join(ROOT, "tests", "golden", "ast*.npz")))
 
 
 def pytest_configure(config):
