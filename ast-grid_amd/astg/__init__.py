@@ -45,6 +45,8 @@ EXPORTS = [
     "astg_field_layout", "astg_fields_upload", "astg_fields_download", "astg_project_fields",
     "astg_diff_state_size", "astg_diff_get_state", "astg_diff_set_state", "astg_diff_step",
     "astg_diff_step_adjoint", "astg_diff_apply",
+    "astg_ast_params_default", "astg_ast_create", "astg_ast_destroy", "astg_ast_last_error", "astg_ast_set_layout",
+    "astg_ast_set_velocity", "astg_ast_get_velocity", "astg_ast_project", "astg_ast_system", "astg_ast_tilt_index",
 ]
 FIELD_DENSITY, FIELD_HEAT = 0, 1
 SAMPLE_VELOCITY, SAMPLE_MAC, SAMPLE_BACKTRACE, SAMPLE_DENSITY, SAMPLE_HEAT, SAMPLE_SDF = range(6)
@@ -161,6 +163,16 @@ def lib():
         "astg_fields_download": (I, [P, ctypes.POINTER(Fields)]),
         "astg_project_fields": (I, [P, D, D, ctypes.POINTER(PcgParams), ctypes.POINTER(Fields),
                                     ctypes.POINTER(PcgResult)]),
+        "astg_ast_params_default": (None, [ctypes.POINTER(PcgParams)]),
+        "astg_ast_create": (I, [ctypes.POINTER(P), I, I, P, D]),
+        "astg_ast_destroy": (I, [P]),
+        "astg_ast_last_error": (ctypes.c_char_p, [P]),
+        "astg_ast_set_layout": (I, [P, P]),
+        "astg_ast_set_velocity": (I, [P, P, P, P, P]),
+        "astg_ast_get_velocity": (I, [P, P, P, P, P]),
+        "astg_ast_project": (I, [P, D, D, ctypes.POINTER(PcgParams), ctypes.POINTER(PcgResult)]),
+        "astg_ast_system": (I, [P, I, ctypes.c_longlong, P, P, P, P, P]),
+        "astg_ast_tilt_index": (I, [P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

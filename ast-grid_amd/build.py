@@ -20,7 +20,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["octree.cpp", "scene.cpp", "weights.cpp", "project_host.cpp", "capi.cpp", "sweep_merge.cpp", "mic_plan.cpp", "frame_io.cpp",
            "consts.cpp", "reorder.cpp", "pcg.hip", "pcg_batch.hip", "plan_device.hip", "project.hip", "advect.hip",
-           "linrec.hip", "reorder.hip"]
+           "linrec.hip", "reorder.hip", "ast_uniform.hip", "capi_ast.cpp"]
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -464,6 +464,72 @@ def c3_timing(fd_coords=8, steps=20, every=5):
     return out
 
 
+AST_CONFIGS = [(2, (192, 384)), (2, (384, 768)), (3, (64, 64, 64)), (3, (128, 128, 128))]
+
+
+def ast_uniform_timing(cpu=True, seed=0):
+    """The uniform-layout AST projection (astg.uniform: the reference's
+    ASTSolver<d>::project) on seeded inputs: 30 % of the tilt nodes open,
+    N(0,1) face and tilt velocities, dt 0.025 (the reference's 2D example),
+    rho 2. Per configuration: the device projection in FP64 (the reference's
+    MIC(0)-PCG, absolute 1e-5 residual) and in the mixed form, the first call
+    on the layout (pattern set-up included) and a repeated one, and the
+    reference's own project() on the host cores for the first three."""
+    import numpy as np
+    import astg
+    from astg import uniform
+    rows = []
+    for ci, (dim, res) in enumerate(AST_CONFIGS):
+        rng = np.random.default_rng(seed + ci)
+        dx = 1.0 / res[0]
+        tn = int(np.prod([r + 1 for r in res]))
+        kt = 4 if dim == 2 else 8
+        open_ = (rng.random(tn) < 0.3).astype(np.uint8)
+        mac = [rng.standard_normal(int(np.prod([r + (a == i) for i, r in enumerate(res)]))) for a in range(dim)]
+        tilt = rng.standard_normal((tn, kt))
+        s = uniform.ASTSolver(dim, res, dx, device=0)
+        s.set_layout(open_)
+        row = {"dim": dim, "res": list(res)}
+        for name, prm in (("fp64", uniform.default_params()),
+                          ("mixed", uniform.default_params(precision=astg.PREC_MIXED))):
+            s.set_layout(open_)  # the first call after a layout change sets the pattern up
+            s.set_velocity(mac, tilt)
+            r0 = s.project(0.025, 2.0, prm)
+            s.set_velocity(mac, tilt)
+            r = s.project(0.025, 2.0, prm)
+            row[name] = {"ms": round(r.ms_total, 3), "ms_first": round(r0.ms_total, 3), "iterations": r.iterations,
+                         "ms_solve": round(r.ms_solve, 3), "ms_factor": round(r.ms_factor, 3),
+                         "ms_assemble": round(r.ms_assemble, 3), "depth": r.depth}
+            row["n"], row["nnz"] = r.n, r.nnz
+            if name == "fp64":
+                p64 = s.system()[4]
+        s.close()
+        if cpu and ci < 3:
+            try:
+                from oracle import ref
+                if ref.available():
+                    rf = ref.RefAST(dim, res, dx)
+                    rf.set_layout(open_)
+                    rf.set_velocity(mac, tilt)
+                    t0 = time.perf_counter()
+                    rf.project(0.025, 2.0, report=False)
+                    ms = (time.perf_counter() - t0) * 1e3
+                    pr = rf.system()[4]
+                    rf.close()
+                    row["reference_cpu"] = {"ms": round(ms, 1), "cores": ref.lib().ref_num_workers(),
+                                            "kind": "reference",
+                                            "fp64_pressure_rel_l2": float(np.linalg.norm(p64 - pr) /
+                                                                          np.linalg.norm(pr))}
+                    row["speedup_fp64_vs_cpu"] = round(ms / row["fp64"]["ms"], 2)
+            except Exception as e:  # the baseline is reported, never required
+                log("ast cpu baseline failed: %s" % e)
+        log("ast %dD %s: n %d, fp64 %.1f ms (%d it, depth %d), mixed %.1f ms (%d it), cpu %s" % (
+            dim, res, row["n"], row["fp64"]["ms"], row["fp64"]["iterations"], row["fp64"]["depth"],
+            row["mixed"]["ms"], row["mixed"]["iterations"], row.get("reference_cpu", {}).get("ms")))
+        rows.append(row)
+    return rows
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
@@ -482,6 +548,7 @@ def main():
                     help="run only the full-scale size (512^3-effective, 5 levels, 18 M leaves)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="one projection, then per-kernel timings only (for counter collection)")
+    ap.add_argument("--ast-only", action="store_true", help="run only the uniform-layout AST projection")
     ap.add_argument("--advect-only", action="store_true",
                     help="one projection, then the post-projection chain and advect, timed --kernel-reps "
                          "times, and the copy kernel (for counter collection)")
@@ -503,6 +570,9 @@ def main():
     if a.batch_only:
         bt = batched_timing(tune=a.tune)
         log("batched: %s" % json.dumps(bt))
+        return
+    if a.ast_only:
+        print(json.dumps({"ast_uniform": ast_uniform_timing(cpu=not a.no_cpu)}))
         return
     if a.c5_only:
         c5 = c5_timing(steps=a.c5_steps)
@@ -621,6 +691,8 @@ def main():
             c5["steps"], c5["fp64"]["ms_per_step"], c5["mixed"]["ms_per_step"],
             c5["mixed_vs_fp64"][-1]["velocity"]["rel_l2"]))
 
+    ast_rows = ast_uniform_timing(cpu=not a.no_cpu)
+
     cpu = None
     if not a.no_cpu:
         try:
@@ -675,6 +747,7 @@ def main():
         "batched": batched,
         "c3": c3,
         "c5": c5,
+        "ast_uniform": ast_rows,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

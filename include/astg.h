@@ -474,6 +474,52 @@ int astg_write_frame(astg_ctx* ctx, const char* layout, const char* tilt, const 
 int astg_read_frame(astg_ctx* ctx, const char* layout, const char* tilt, const char* density, const char* heat,
                     const char* velocity);
 
+/* ---- uniform-layout AST projection ------------------------------------ */
+/* The reference's uniform-grid AST solver, ast::ASTSolver<d> (d = 2 or 3,
+ * include/ast_solver.h:20-673; the pybind module ast_solver exposes it as
+ * ASTSolver2 / ASTSolver3, python_bindings/ast_solver.cc:43-54), projection
+ * path: ASTSolver::project (ast_solver.h:322-498) with the reference's build
+ * configuration (CORRECTION_MAC, no edge-node unknowns, ast_general.h:11-15).
+ * Unknowns: every cell in box order (x fastest), then every open tilt node in
+ * box order (updateIndex, 599-621). Grids in the reference's layouts, x
+ * fastest: tilt nodes prod(res + 1), MAC faces of axis a prod(res + e_a)
+ * (FaceCenteredGrid::vecIndex, face_centered_grid.h:72-80), tilt velocities
+ * node-major with 4 (2D) or 8 (3D) slots per node. Host pointers. */
+typedef struct astg_ast astg_ast;
+
+/* ASTSolver::initialize's solver settings (ast_solver.h:40): tol 1e-5 as an
+ * ABSOLUTE bound on ||r||_inf (the reference's std::vector PCGSolver::solve,
+ * pcg_solver.h:316, does not scale it by ||b||), 5000 iterations, MIC(0) 0.97,
+ * min-diagonal ratio 0.95, FP64, reference numbering. */
+void astg_ast_params_default(astg_pcg_params* p);
+
+/* ASTSolver(res, dx) + initialize() (ast_solver.h:27-51): fields zero, every
+ * tilt closed. res[d] >= 2 each; in 3D res[1] <= res[0] (constrainVelocity
+ * runs x over [0, res_y) on the y walls, ast_solver.h:634, and writes past its
+ * face grid otherwise). On failure *out is still set: read astg_ast_last_error,
+ * then destroy it. */
+int astg_ast_create(astg_ast** out, int device, int dim, const int* res, double dx);
+int astg_ast_destroy(astg_ast* ast);
+const char* astg_ast_last_error(const astg_ast* ast);
+/* The tilt layout (TiltNode::open per node, ast_grid_layout.h:13-60): one
+ * flag per tilt node; boundary nodes are held closed (initData 79-87,
+ * constrainFixedTiltNode). */
+int astg_ast_set_layout(astg_ast* ast, const uint8_t* tilt_open);
+/* ASTFaceGrid's MAC and tilt data (ast_face_grid.h:16-58); mac_z unused in 2D. */
+int astg_ast_set_velocity(astg_ast* ast, const double* mac_x, const double* mac_y, const double* mac_z,
+                          const double* tilt);
+int astg_ast_get_velocity(astg_ast* ast, double* mac_x, double* mac_y, double* mac_z, double* tilt);
+/* project(dt) (ast_solver.h:322-498): numbering, assembly, MIC(0)-PCG,
+ * correctVelocityByPressure, constrainVelocity; rho is the solver's rho
+ * (2.0 in the reference). params may be null (astg_ast_params_default);
+ * ASTG_ORDER_BLOCK_COLOUR is rejected. */
+int astg_ast_project(astg_ast* ast, double dt, double rho, const astg_pcg_params* params, astg_pcg_result* result);
+/* The last projection's system (CSR, rows sorted by column), right-hand side
+ * and pressure; n and nnz as in its result. Any pointer may be null. */
+int astg_ast_system(astg_ast* ast, int n, long long nnz, int* rowstart, int* col, double* val, double* b, double* p);
+/* The unknown of every tilt node (pIndex, -1 where closed). */
+int astg_ast_tilt_index(astg_ast* ast, int* tilt_index);
+
 #ifdef __cplusplus
 }
 #endif

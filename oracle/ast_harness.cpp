@@ -49,7 +49,7 @@ struct AstBase {
     virtual void setLayout(const uint8_t* open) = 0;
     virtual void setVelocity(const double* const* mac, const double* tilt) = 0;
     virtual void getVelocity(double* const* mac, double* tilt) = 0;
-    virtual void project(double dt, double rho, int* n, long long* nnz) = 0;
+    virtual void project(double dt, double rho, int* n, long long* nnz, bool report) = 0;
     virtual void system(int* rs, int* col, double* val, double* b, double* x) = 0;
     virtual void numbering(int* cell, int* tilt) = 0;
     virtual void solveInfo(int* iters, double* err) = 0;
@@ -117,13 +117,15 @@ struct AstImpl : AstBase {
         if constexpr (d == 3) macCopy<2>(nullptr, mac[2]);
         tiltCopy(nullptr, tilt);
     }
-    void project(double dt, double rho, int* n, long long* nnz) override {
+    void project(double dt, double rho, int* n, long long* nnz, bool report) override {
         s.rho = rho;
         s.project(dt);
         *n = s.total_index_size;
         long long z = 0;
         for (int i = 0; i < (int)s.A.n; ++i) z += (long long)s.A.index[i].size();
         *nnz = z;
+        same = false;
+        if (!report) return;
         // project() keeps no record of its solve: the same solver settings
         // (ASTSolver::initialize, ast_solver.h:40) on the same A and b report it
         PCGSolver<double> again;
@@ -206,8 +208,10 @@ int ast_ref_velocity(void* h, double* mac0, double* mac1, double* mac2, double* 
     return guard([&] { static_cast<AstBase*>(h)->getVelocity(m, tilt); });
 }
 
-int ast_ref_project(void* h, double dt, double rho, int* n, long long* nnz) {
-    return guard([&] { static_cast<AstBase*>(h)->project(dt, rho, n, nnz); });
+// report != 0 also re-solves the system to report the solve's iterations
+// (ast_ref_solve_info); leave it 0 to time project() alone.
+int ast_ref_project(void* h, double dt, double rho, int* n, long long* nnz, int report) {
+    return guard([&] { static_cast<AstBase*>(h)->project(dt, rho, n, nnz, report != 0); });
 }
 
 int ast_ref_system(void* h, int* rs, int* col, double* val, double* b, double* x) {

@@ -87,7 +87,7 @@ def lib():
         "ast_ref_set_layout": (I, [P, P]),
         "ast_ref_set_velocity": (I, [P, P, P, P, P]),
         "ast_ref_velocity": (I, [P, P, P, P, P]),
-        "ast_ref_project": (I, [P, D, D, P, P]),
+        "ast_ref_project": (I, [P, D, D, P, P, I]),
         "ast_ref_system": (I, [P, P, P, P, P, P]),
         "ast_ref_numbering": (I, [P, P, P]),
         "ast_ref_solve_info": (I, [P, P, P]),
@@ -437,10 +437,12 @@ class RefAST:
         self._chk(lib().ast_ref_velocity(self.h, _p(m[0]), _p(m[1]), _p(m[2]) if self.dim == 3 else None, _p(t)))
         return m, t.reshape(-1, self.kt)
 
-    def project(self, dt, rho=2.0):
+    def project(self, dt, rho=2.0, report=True):
+        """project(dt); report also re-solves to record the iteration count
+        (solve_info), which doubles the solve time: pass False to time it."""
         n = np.zeros(1, np.int32)
         nnz = np.zeros(1, np.int64)
-        self._chk(lib().ast_ref_project(self.h, float(dt), float(rho), _p(n), _p(nnz)))
+        self._chk(lib().ast_ref_project(self.h, float(dt), float(rho), _p(n), _p(nnz), 1 if report else 0))
         self.n, self.nnz = int(n[0]), int(nnz[0])
         return self.n, self.nnz
 
