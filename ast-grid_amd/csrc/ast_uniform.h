@@ -13,6 +13,15 @@
 // scan, sorted columns, then values and right-hand side, one lane per row) and
 // solved by the same MIC(0)-PCG as the octree projection; the velocity
 // correction and the wall constraint run one lane per face or tilt.
+//
+// Solver orderings: the reference numbering (the reference's MIC(0) and
+// iterates), or block colours (an added form, as for the octree: squares or
+// cubes of B = 4 cells coloured by the parity of their coordinates, 4 colours in
+// 2D and 8 in 3D, rows ordered by colour, block and reference index; a tilt
+// row belongs to the block of its lower corner cell). Every coupling joins
+// unknowns at most one cell apart, so same-coloured blocks are uncoupled and
+// the sweeps' dependency chains are one block deep per colour instead of
+// nx + ny (+ nz) deep. ASTG_ORDER_AUTO picks block colours for the mixed form.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -21,6 +30,7 @@
 #include <string>
 
 #include "pcg.h"
+#include "reorder.h"
 
 namespace astg {
 
@@ -62,14 +72,18 @@ public:
 
 private:
     int ensurePattern(std::string& err);
+    int ensureBlockOrder(std::string& err);
 
     hipStream_t stream_;
     AstGeom g_;
     PcgDevice pcg_;
-    DevBuf open_, tid_, tnode_, flag_, scan_, mac_[3], tilt_, len_, rs_, col_, tmp_, bmax_;
+    DevBuf open_, tid_, tnode_, flag_, scan_, mac_[3], tilt_, len_, rs_, col_, tmp_, bmax_, keys_;
+    BlockOrderDevice blk_;
+    std::unique_ptr<PcgDevice> pcgBlk_;  // the block-colour copy of the system
     int n_ = 0;
     long long nnz_ = 0;
     bool patternReady_ = false;
+    bool blkReady_ = false;
     bool solved_ = false;
     hipEvent_t ev_[5] = {};
 };

@@ -48,13 +48,14 @@ void blockKeys(const Octree& o, const Numbering& num, int blockCells, std::vecto
 // buildBlockOrder on the device from the keys (uploaded here) and A's device
 // pattern (reference numbering, rows sorted by column): the same perm,
 // rowstart, col and src, in device buffers. Rows of more than 64 entries are
-// an error. On `stream`; returns after it has finished.
+// an error. On `stream`; returns after it has finished. keysOnDevice: `keys`
+// points to device memory (copied on the device) instead of host memory.
 struct BlockOrderDevice {
     int n = 0;
     long long nnz = 0;
     DevBuf keys, sortedKeys, iota, perm, inv, rowstart, col, src, tmp, flag;
     int build(hipStream_t stream, int n, long long nnz, const int* hostKeys, const int* rs, const int* colA,
-              std::string& err);
+              std::string& err, bool keysOnDevice = false);
 };
 
 // Device side (reorder.hip), on `stream`:

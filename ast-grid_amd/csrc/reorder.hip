@@ -70,7 +70,7 @@ __global__ void k_perm_rows(int n, const int* perm, const int* inv, const int* r
 }  // namespace
 
 int BlockOrderDevice::build(hipStream_t stream, int n_, long long nnz_, const int* hostKeys, const int* rs,
-                            const int* colA, std::string& err) {
+                            const int* colA, std::string& err, bool keysOnDevice) {
     n = n_;
     nnz = nnz_;
     const size_t n4 = (size_t)n * 4;
@@ -84,7 +84,8 @@ int BlockOrderDevice::build(hipStream_t stream, int n_, long long nnz_, const in
     HIPCHK(col.alloc((size_t)nnz * 4));
     HIPCHK(src.alloc((size_t)nnz * 4));
     HIPCHK(flag.alloc(4));
-    HIPCHK(hipMemcpyAsync(keys.p, hostKeys, n4, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(keys.p, hostKeys, n4, keysOnDevice ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                          stream));
     HIPCHK(hipMemsetAsync(flag.p, 0, 4, stream));
     k_iota_i<<<gb, kBS, 0, stream>>>(n, iota.as<int>());
     // stable sort by key: ties keep the reference index order

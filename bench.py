@@ -472,9 +472,11 @@ def ast_uniform_timing(cpu=True, seed=0):
     ASTSolver<d>::project) on seeded inputs: 30 % of the tilt nodes open,
     N(0,1) face and tilt velocities, dt 0.025 (the reference's 2D example),
     rho 2. Per configuration: the device projection in FP64 (the reference's
-    MIC(0)-PCG, absolute 1e-5 residual) and in the mixed form, the first call
-    on the layout (pattern set-up included) and a repeated one, and the
-    reference's own project() on the host cores for the first three."""
+    MIC(0)-PCG in its numbering, absolute 1e-5 residual), in FP64 with block
+    colours (same MIC(0), another ordering) and in the mixed form (block
+    colours, IC(0), FP32 inner solve), the first call on the layout
+    (pattern set-up included) and a repeated one, and the reference's own
+    project() on the host cores for the first three."""
     import numpy as np
     import astg
     from astg import uniform
@@ -491,7 +493,9 @@ def ast_uniform_timing(cpu=True, seed=0):
         s.set_layout(open_)
         row = {"dim": dim, "res": list(res)}
         for name, prm in (("fp64", uniform.default_params()),
-                          ("mixed", uniform.default_params(precision=astg.PREC_MIXED))):
+                          ("fp64_block_colours", uniform.default_params(ordering=astg.ORDER_BLOCK_COLOUR)),
+                          ("mixed", uniform.default_params(precision=astg.PREC_MIXED, ordering=astg.ORDER_AUTO,
+                                                           mic_tau=astg.TAU_AUTO))):
             s.set_layout(open_)  # the first call after a layout change sets the pattern up
             s.set_velocity(mac, tilt)
             r0 = s.project(0.025, 2.0, prm)
@@ -503,6 +507,8 @@ def ast_uniform_timing(cpu=True, seed=0):
             row["n"], row["nnz"] = r.n, r.nnz
             if name == "fp64":
                 p64 = s.system()[4]
+            elif name == "mixed":
+                pmx = s.system()[4]
         s.close()
         if cpu and ci < 3:
             try:
@@ -519,8 +525,11 @@ def ast_uniform_timing(cpu=True, seed=0):
                     row["reference_cpu"] = {"ms": round(ms, 1), "cores": ref.lib().ref_num_workers(),
                                             "kind": "reference",
                                             "fp64_pressure_rel_l2": float(np.linalg.norm(p64 - pr) /
-                                                                          np.linalg.norm(pr))}
+                                                                          np.linalg.norm(pr)),
+                                            "mixed_pressure_rel_l2": float(np.linalg.norm(pmx - pr) /
+                                                                           np.linalg.norm(pr))}
                     row["speedup_fp64_vs_cpu"] = round(ms / row["fp64"]["ms"], 2)
+                    row["speedup_mixed_vs_cpu"] = round(ms / row["mixed"]["ms"], 2)
             except Exception as e:  # the baseline is reported, never required
                 log("ast cpu baseline failed: %s" % e)
         log("ast %dD %s: n %d, fp64 %.1f ms (%d it, depth %d), mixed %.1f ms (%d it), cpu %s" % (

@@ -511,8 +511,11 @@ int astg_ast_set_velocity(astg_ast* ast, const double* mac_x, const double* mac_
 int astg_ast_get_velocity(astg_ast* ast, double* mac_x, double* mac_y, double* mac_z, double* tilt);
 /* project(dt) (ast_solver.h:322-498): numbering, assembly, MIC(0)-PCG,
  * correctVelocityByPressure, constrainVelocity; rho is the solver's rho
- * (2.0 in the reference). params may be null (astg_ast_params_default);
- * ASTG_ORDER_BLOCK_COLOUR is rejected. */
+ * (2.0 in the reference). params may be null (astg_ast_params_default).
+ * ASTG_ORDER_BLOCK_COLOUR (ASTG_ORDER_AUTO with the mixed precision) solves in
+ * block colours: squares / cubes of 4 cells in 4 / 8 colours, an added form
+ * with another preconditioner; ASTG_TAU_AUTO is 0.97 in FP64, 0 (IC(0)) in
+ * the mixed form. */
 int astg_ast_project(astg_ast* ast, double dt, double rho, const astg_pcg_params* params, astg_pcg_result* result);
 /* The last projection's system (CSR, rows sorted by column), right-hand side
  * and pressure; n and nnz as in its result. Any pointer may be null. */

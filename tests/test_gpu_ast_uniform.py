@@ -154,6 +154,29 @@ def test_mixed_precision_meets_the_bound():
     s.close()
 
 
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_block_colour_solve_meets_the_bound(precision):
+    """The block-colour ordering (an added form: another preconditioner, so
+    another trajectory): the reference's residual bound on the device's own
+    system, pressure within 1e-4 relative L2 of the reference's, on every
+    fixture."""
+    import astg
+    from astg import uniform
+    for path in AST_GOLDEN:
+        g = load_golden(path)
+        d, res, dx, open_, mac, tilt, dt, rho = _inputs(g)
+        s = _device(d, res, dx, open_, mac, tilt)
+        r = s.project(dt, rho, uniform.default_params(precision=precision, ordering=astg.ORDER_BLOCK_COLOUR,
+                                                      mic_tau=astg.TAU_AUTO))
+        rs, col, val, b, p = s.system()
+        resid = _residual(rs, col, val, b, p)
+        ep = np.linalg.norm(p - g["p"]) / np.linalg.norm(g["p"])
+        print("%s: %d iterations (reference numbering %d), depth %d, residual %.2e, pressure %.2e" % (
+            os.path.basename(path), r.iterations, int(g["iters"]), r.depth, resid, ep), flush=True)
+        assert r.converged == 1 and resid <= 1e-5 and ep <= 1e-4
+        s.close()
+
+
 def test_bad_arguments_are_rejected():
     import astg
     from astg import uniform
@@ -167,7 +190,7 @@ def test_bad_arguments_are_rejected():
         s.project(-1.0)
     assert e.value.code == astg.ASTG_ERR_ARG
     with pytest.raises(astg.AstgError) as e:
-        s.project(0.01, 2.0, uniform.default_params(ordering=astg.ORDER_BLOCK_COLOUR))
+        s.project(0.01, 2.0, uniform.default_params(ordering=7))
     assert e.value.code == astg.ASTG_ERR_ARG
     r = s.project(0.01)  # zero field: zero right-hand side, no iteration
     assert r.iterations == 0 and r.converged == 1
