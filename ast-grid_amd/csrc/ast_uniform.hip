@@ -11,6 +11,7 @@
 
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
 #include <cfloat>
 #include <cstdlib>
 #include <cstring>
@@ -255,12 +256,15 @@ __global__ void k_ast_fill(AstGeom g, int n, const int* tid, const int* tnode, M
     }
 }
 
-// Largest |b| as ordered bits (non-negative doubles order as integers).
+// Largest |b| as ordered bits (non-negative doubles order as integers): a
+// grid-stride pass, one block maximum and one atomic per block.
+constexpr int kMaxBlocks = 1024;
 __global__ void k_ast_absmax(int n, const double* b, unsigned long long* out) {
-    const int i = blockIdx.x * BS + threadIdx.x;
-    double v = i < n ? fabs(b[i]) : 0.0;
-    v = wave_max(v);
-    if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(v));
+    double v = 0.0;
+    for (long long i = (long long)blockIdx.x * BS + threadIdx.x; i < n; i += (long long)gridDim.x * BS)
+        v = nan_max(v, fabs(b[i]));
+    v = block_reduce<true>(v);
+    if (threadIdx.x == 0) atomicMax(out, (unsigned long long)__double_as_longlong(v));
 }
 __global__ void k_ast_zero_u64(unsigned long long* p) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0ull;
@@ -560,7 +564,7 @@ int AstUniform::project(double dt, double rho, const astg_pcg_params& prmIn, ast
                                                           tilt_.as<double>(), pcg_.devRowstart(), nullptr,
                                                           pcg_.values(), pcg_.rhs(), scale);
     k_ast_zero_u64<<<1, 64, 0, stream_>>>(bmax_.as<unsigned long long>());
-    k_ast_absmax<<<nblk(n), BS, 0, stream_>>>(n, pcg_.rhs(), bmax_.as<unsigned long long>());
+    k_ast_absmax<<<std::min(nblk(n), kMaxBlocks), BS, 0, stream_>>>(n, pcg_.rhs(), bmax_.as<unsigned long long>());
     unsigned long long bits = 0;
     HIPCHK(hipMemcpyAsync(&bits, bmax_.p, 8, hipMemcpyDeviceToHost, stream_));
     (void)hipEventRecord(ev_[1], stream_);
