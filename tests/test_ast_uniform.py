@@ -84,3 +84,16 @@ def test_oracle_matches_live_reference(dim, res):
     assert out["iters"] == it
     _check(g, out, dim)
     r.close()
+
+
+def test_device_entry_points_fail_loudly_without_a_device():
+    """The product path has no host fallback: creating a device solver where
+    no GPU is visible is an error, not a silent CPU path."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    import astg
+    from astg import uniform
+    with pytest.raises(astg.AstgError) as e:
+        uniform.ASTSolver(2, (8, 8), 0.125, device=0)
+    assert e.value.code == astg.ASTG_ERR_HIP
