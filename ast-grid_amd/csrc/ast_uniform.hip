@@ -452,6 +452,8 @@ int AstUniform::getVelocity(double* const* mac, double* tilt, std::string& err) 
     return ASTG_OK;
 }
 
+static int astRefMerge(int dim);
+
 // Numbering (updateIndex 599-621) and the pattern; the solver's pattern set-up.
 int AstUniform::ensurePattern(std::string& err) {
     if (patternReady_) return ASTG_OK;
@@ -492,6 +494,7 @@ int AstUniform::ensurePattern(std::string& err) {
                                                            tilt_.as<double>(), rs_.as<int>(), col_.as<int>(), nullptr,
                                                            nullptr, 0.0);
     HIPCHK(hipGetLastError());
+    pcg_.setMergeLevels(astRefMerge(g_.d));
     int rc = pcg_.setPatternDevice(n, nnz, rs_.as<int>(), col_.as<int>(), err);
     if (rc) return rc;
     n_ = n;
@@ -512,6 +515,17 @@ static int astBlock() {
     const char* e = std::getenv("ASTG_AST_BLOCK");
     const int v = e ? std::atoi(e) : 0;
     return v > 0 ? v : 4;
+}
+// Levels per sweep group of the reference-numbered solver: in 2D the levels
+// are a few hundred rows (the chain is nx + ny deep), so deep groups pay:
+// 192x384 FP64, ms per projection at 2 / 3 / 4 / 6 / 8 levels per group:
+// 342 / 235 / 180 / 136 / 133 (384x768: 1342 / - / 694 / 503 / 480), the same
+// iterations. In 3D the default pairs stay (128^3: 86 / 102 / 141 at 2 / 3 / 4).
+// ASTG_AST_REF_MERGE overrides it.
+static int astRefMerge(int dim) {
+    const char* e = std::getenv("ASTG_AST_REF_MERGE");
+    const int v = e ? std::atoi(e) : 0;
+    return v > 0 ? v : (dim == 2 ? 8 : 2);
 }
 static int astMerge(int n) {
     const char* e = std::getenv("ASTG_AST_MERGE");
