@@ -50,9 +50,13 @@ struct AstGeom {
 
 class AstUniform {
 public:
-    AstUniform(hipStream_t s) : stream_(s), pcg_(s) {}
+    explicit AstUniform(hipStream_t s) : stream_(s), pcg_(s) {}
+    ~AstUniform();
+    AstUniform(const AstUniform&) = delete;
+    AstUniform& operator=(const AstUniform&) = delete;
     // Sizes and spacing; fields zero, every tilt closed.
     int init(int dim, const int* res, double dx, std::string& err);
+    bool ready() const { return ready_; }  // init succeeded
     const AstGeom& geom() const { return g_; }
     // Tilt-node flags (host, prod(res + 1)); boundary nodes are held closed
     // (initData 79-87, constrainFixedTiltNode). Renumbers and rebuilds the
@@ -85,6 +89,7 @@ private:
     bool patternReady_ = false;
     bool blkReady_ = false;
     bool solved_ = false;
+    bool ready_ = false;
     hipEvent_t ev_[5] = {};
 };
 

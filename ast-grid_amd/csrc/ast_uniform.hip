@@ -357,7 +357,13 @@ __global__ void k_ast_constrain(AstGeom g, double* ux, double* uy) {
 
 }  // namespace
 
+AstUniform::~AstUniform() {
+    for (auto& e : ev_)
+        if (e) (void)hipEventDestroy(e);
+}
+
 int AstUniform::init(int dim, const int* res, double dx, std::string& err) {
+    ready_ = false;
     if (dim != 2 && dim != 3) {
         err = "dimension must be 2 or 3";
         return ASTG_ERR_ARG;
@@ -419,11 +425,14 @@ int AstUniform::init(int dim, const int* res, double dx, std::string& err) {
     k_ast_zero<<<nblk(nt * kT), BS, 0, stream_>>>(nt * kT, tilt_.as<double>());
     HIPCHK(bmax_.alloc(8));
     HIPCHK(hipMemsetAsync(open_.p, 0, (size_t)nt, stream_));
-    for (auto& e : ev_) HIPCHK(hipEventCreate(&e));
+    for (auto& e : ev_)
+        if (!e) HIPCHK(hipEventCreate(&e));
     patternReady_ = false;
+    blkReady_ = false;
     solved_ = false;
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(stream_));
+    ready_ = true;
     return ASTG_OK;
 }
 

@@ -28,7 +28,7 @@ int fail(astg_ast* a, int code, const std::string& m) {
 template <typename F>
 int guarded(astg_ast* a, F&& f) {
     if (!a) return ASTG_ERR_ARG;
-    if (!a->ast || a->ast->geom().d == 0) return fail(a, ASTG_ERR_STATE, "context was not created");
+    if (!a->ast || !a->ast->ready()) return fail(a, ASTG_ERR_STATE, "context was not created");
     if (hipSetDevice(a->device) != hipSuccess) return fail(a, ASTG_ERR_HIP, "hipSetDevice failed");
     try {
         return f();
