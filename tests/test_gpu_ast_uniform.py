@@ -19,7 +19,8 @@ Bars:
 At 512^2, 64^3 and 48x40x56 (sizes the fixtures do not reach) the pattern, A
 and b, and the velocity correction applied to the device's own pressure, are
 checked bit for bit against the oracle, and the residual bound on the device's
-own system.
+own system. At the reference's example size (192x384) and at 64^3 the live
+reference (where built) is held to the fixtures' bars.
 """
 import os
 
@@ -194,4 +195,49 @@ def test_bad_arguments_are_rejected():
     assert e.value.code == astg.ASTG_ERR_ARG
     r = s.project(0.01)  # zero field: zero right-hand side, no iteration
     assert r.iterations == 0 and r.converged == 1
+    s.close()
+
+
+@pytest.mark.parametrize("dim,res", [(2, (192, 384)), (3, (64, 64, 64))])
+def test_device_matches_live_reference_at_example_size(dim, res):
+    """The reference's own example size (example/ast_smoke_2d/main.cpp,
+    python/ast_smoke_2d.ipynb: 192x384, dx 1/192, dt 0.025) and a 3D grid,
+    against the reference built from its sources (skipped where absent), to
+    the fixtures' bars."""
+    from oracle import ref
+    if not ref.available():
+        pytest.skip("reference build (oracle/_ref) not present")
+    rng = np.random.default_rng(21)
+    dx = 1.0 / res[0]
+    rf = ref.RefAST(dim, res, dx)
+    open_ = (rng.random(rf.tilt_count()) < 0.3).astype(np.uint8)
+    mac = [rng.standard_normal(rf.mac_count(a)) for a in range(dim)]
+    tilt = rng.standard_normal((rf.tilt_count(), rf.kt))
+    rf.set_layout(open_)
+    rf.set_velocity(mac, tilt)
+    rf.project(0.025, 2.0)
+    grs, gcol, gval, gb, gp = rf.system()
+    git, _ = rf.solve_info()
+    gm, gt = rf.velocity()
+    rf.close()
+    print("size case: dim %d res %s" % (dim, res), flush=True)
+    s = _device(dim, res, dx, open_, mac, tilt)
+    r = s.project(0.025, 2.0)
+    rs, col, val, b, p = s.system()
+    assert np.array_equal(rs, grs) and np.array_equal(col, gcol)
+    assert np.max(np.abs(val - gval)) <= 1e-12 * np.max(np.abs(gval))
+    assert np.max(np.abs(b - gb)) <= 1e-12 * np.max(np.abs(gb))
+    ep = np.linalg.norm(p - gp) / np.linalg.norm(gp)
+    m2, t2 = s.velocity()
+    scale = max(max(np.max(np.abs(gm[a])) for a in range(dim)), np.max(np.abs(gt)))
+    em = max(np.max(np.abs(m2[a] - gm[a])) for a in range(dim)) / scale
+    et = np.max(np.abs(t2 - gt)) / scale
+    print("n %d: iterations %d (reference %d), pressure %.2e, MAC %.2e, tilt %.2e, %.1f ms" % (
+        r.n, r.iterations, git, ep, em, et, r.ms_total), flush=True)
+    assert git >= 0 and abs(r.iterations - git) <= 1
+    assert _residual(rs, col, val, b, p) <= 1e-5
+    if r.iterations == git:
+        assert ep <= 1e-7 and em <= 1e-7 and et <= 1e-7
+    else:
+        assert ep <= 1e-5 and em <= 1e-4 and et <= 1e-4
     s.close()
