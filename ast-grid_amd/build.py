@@ -28,12 +28,12 @@ FLAGS = [
 ]
 
 
-def compile_one(src):
+def compile_one(src, force=False):
     path = os.path.join(SRC, src)
     obj = os.path.join(OBJ, src + ".o")
     deps = [path] + [os.path.join(SRC, h) for h in os.listdir(SRC) if h.endswith(".h")]
     deps.append(os.path.join(HERE, "..", "include", "astg.h"))
-    if os.path.exists(obj) and os.path.getmtime(obj) >= max(os.path.getmtime(d) for d in deps):
+    if not force and os.path.exists(obj) and os.path.getmtime(obj) >= max(os.path.getmtime(d) for d in deps):
         return obj, ""
     lang = ["-x", "hip"]
     cmd = [HIPCC] + lang + FLAGS + ["-c", path, "-o", obj]
@@ -43,10 +43,11 @@ def compile_one(src):
     return obj, p.stderr
 
 
-def build(jobs=8, verbose=False):
+def build(jobs=8, verbose=False, force=False):
+    """force: compile every source even where its object is newer."""
     os.makedirs(OBJ, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        results = list(ex.map(compile_one, SOURCES))
+        results = list(ex.map(lambda src: compile_one(src, force), SOURCES))
     if verbose:
         for _, warn in results:
             if warn.strip():
