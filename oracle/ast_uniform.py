@@ -1,7 +1,7 @@
 """CPU restatement of the reference's uniform-layout AST projection.
 
-TEST INFRASTRUCTURE ONLY: the tests, smoke() and bench.py's CPU baseline use
-it as the checker; nothing in the product imports it.
+TEST INFRASTRUCTURE ONLY: the tests use it as the checker; nothing in the
+product imports it.
 
 The reference is ast::ASTSolver<d>::project (include/ast_solver.h:322-498) with
 the build's own configuration (include/ast_general.h:11-15: CORRECTION_MAC,
